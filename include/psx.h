@@ -1,0 +1,152 @@
+/*
+ * psx.h — C ABI of the MI355X-native voxel FEA linear-static solve.
+ *
+ * The boundary the reference's GPU solver class sits on: ps::GpuFeaSolver
+ * (native/src/gpu/gpu_fea.hpp:17-35) over ps::VoxelFEA (native/src/fea/voxel_fea.hpp:130-189).
+ * A model owns the multigrid hierarchy and every device buffer of one part; the C++ shim in
+ * parts-sim_amd/host/gpu_fea.hpp re-exposes the reference's class over these calls so the bend,
+ * break, nonlinear-ramp and topology loops call it unchanged.
+ *
+ * Conventions
+ *  - Units: the normalised system of the reference (E = 1, voxel size = 1; voxel_fea.hpp:17-18):
+ *    forces in N, u_physical = u / (E h).
+ *  - Layout: density[e], e = i + nx (j + ny k); DOF vectors [node][3], node n = i + NX (j + NY k),
+ *    NX = nx + 1 (structural.hpp:67, voxel_fea.hpp:132-135).
+ *  - Every call returns PSX_OK (0) or an error code; psx_last_error() gives the message of the
+ *    calling thread's last failure. No exception crosses this ABI.
+ *  - Host-pointer calls copy in and out; *_device calls take HBM pointers on the model's device
+ *    and run on the model's stream.
+ *  - A model is used by one thread at a time (the reference locks its GPU context per call,
+ *    gpu_fea.cpp:256, 433, 459); different models may run concurrently on their own streams.
+ */
+#ifndef PSX_H
+#define PSX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct psx_model_s* psx_model;
+
+enum psx_status {
+    PSX_OK = 0,
+    PSX_ERR_ARGUMENT = 1,   /* bad sizes / values: VoxelFEA ctor and solve() throw invalid_argument (voxel_fea.cpp:141-146, 866-868) */
+    PSX_ERR_NO_DEVICE = 2,  /* no usable GPU: GpuFeaSolver ctor throws runtime_error (gpu_fea.cpp:254) */
+    PSX_ERR_NO_COARSE = 3,  /* coarsest level without a direct solve, m = 0 or m > 3000 (gpu_fea.cpp:255) */
+    PSX_ERR_DEVICE = 4,     /* HIP runtime or kernel failure (the reference's error scopes, gpu.cpp:220-246) */
+    PSX_ERR_INTERNAL = 5
+};
+
+/* Arithmetic of a solve.
+ *  PSX_PREC_FP64   plain FP64 MG-PCG: FP64 CG, FP64 operator, FP64 V-cycle (the comparator)
+ *  PSX_PREC_REF    the reference CPU algorithm (voxel_fea.cpp:863-927): FP64 CG, FP32 V-cycle with r
+ *                  scaled to order one (voxel_fea.cpp:835-861)
+ *  PSX_PREC_MIXED  FP32 CG and operator with FP64 dots and alpha/beta, FP32 V-cycle, FP64 true
+ *                  residual and residual-corrected refinement (reliable updates), FP64 coarsest solve */
+enum psx_precision { PSX_PREC_FP64 = 0, PSX_PREC_REF = 1, PSX_PREC_MIXED = 2 };
+
+/* Replaces VoxelFEA(dims, density, nu, bc, coarsestMaxDof, diagAdd) (voxel_fea.hpp:136-137). */
+typedef struct {
+    int nx, ny, nz;
+    const float* density;       /* [nx*ny*nz], values in [0, 1], 0 = empty */
+    double nu;                  /* Poisson's ratio in (-1, 0.5) */
+    const uint8_t* bc;          /* [3*(nx+1)*(ny+1)*(nz+1)], 1 = held at zero */
+    const double* diag_add;     /* optional [3*nodes] shift added to K's diagonal, or NULL */
+    int64_t coarsest_max_dof;   /* coarsen until free DOFs <= this (default 1100, gpu_fea.hpp:15) */
+    int max_levels;             /* 0 = the reference's cap of 12 (voxel_fea.cpp:176) */
+} psx_model_desc;
+
+/* Replaces SolveOptions (voxel_fea.hpp:112-117). */
+typedef int (*psx_progress_fn)(int iteration, double rel_residual, void* user); /* nonzero cancels */
+typedef struct {
+    double tol;                 /* relative residual ||f - K u|| / ||f_free|| (default 1e-6) */
+    int max_iter;               /* default 500 */
+    int precision;              /* enum psx_precision */
+    int check_every;            /* iterations queued between host reads of the state (0 = 8) */
+    int fold_every;             /* PSX_PREC_MIXED: true residual at least every N iterations (0 = 20) */
+    psx_progress_fn on_progress;
+    void* user;
+} psx_solve_opts;
+
+/* Replaces SolveResult's scalars (voxel_fea.hpp:119-124). */
+typedef struct {
+    int iterations;
+    double residual;            /* final true relative residual, FP64 */
+    int converged;              /* residual <= 10 tol and not cancelled (voxel_fea.cpp:925) */
+    int cancelled;
+    int broken;                 /* p.q <= 0: loss of positive definiteness (voxel_fea.cpp:904) */
+    int folds;                  /* PSX_PREC_MIXED: FP64 true-residual refinements */
+    double device_ms;           /* wall time of the device solve */
+} psx_solve_info;
+
+typedef struct {
+    int nx, ny, nz;
+    int64_t n_nodes, n_dof, free_dof, n_elems, n_own;
+    double lmax;                /* largest eigenvalue estimate of D^-1 K (0 = not estimated yet) */
+} psx_level_info;
+
+const char* psx_last_error(void);
+/* 1 when a GPU is usable (the reference's gpuAvailable(), gpu.cpp:250-257). */
+int psx_available(void);
+
+/* Builds the hierarchy and uploads it (VoxelFEA ctor voxel_fea.cpp:138-183 + GpuFeaSolver ctor
+ * gpu_fea.cpp:252-423). */
+int psx_create(const psx_model_desc* desc, psx_model* out);
+void psx_destroy(psx_model m);
+
+int psx_num_levels(psx_model m);
+int psx_get_level_info(psx_model m, int level, psx_level_info* out);
+/* Smoother eigenvalue estimates per level (the reference stores them in Level::lmax,
+ * gpu_fea.cpp:412-417); set_lmax overrides one (e.g. the CPU model's value). */
+int psx_estimate_smoothers(psx_model m);
+int psx_set_lmax(psx_model m, int level, double lmax);
+
+/* y = K x on a level, held DOFs zeroed unless raw (VoxelFEA::apply, voxel_fea.cpp:388). */
+int psx_apply(psx_model m, int level, const double* x, double* y, int raw);
+/* The same in FP32 (VoxelFEA::applyT<float>, voxel_fea.cpp:393). */
+int psx_apply32(psx_model m, int level, const float* x, float* y, int raw);
+
+/* One V-cycle z ~ K^-1 r, FP32 with r scaled to order one (GpuFeaSolver::precondition,
+ * gpu_fea.cpp:431-450; VoxelFEA::precondition voxel_fea.cpp:830-861). */
+int psx_precondition(psx_model m, const double* r, double* z);
+int psx_precondition_device(psx_model m, const double* d_r, double* d_z);
+
+/* K u = f (GpuFeaSolver::solve gpu_fea.cpp:452-574, VoxelFEA::solve voxel_fea.cpp:863-927).
+ * x0 may be NULL (zero start). opts may be NULL (defaults). */
+int psx_solve(psx_model m, const double* f, const double* x0, const psx_solve_opts* opts, double* u, psx_solve_info* info);
+int psx_solve_device(psx_model m, const double* d_f, const double* d_x0, const psx_solve_opts* opts, double* d_u, psx_solve_info* info);
+
+/* Support reactions: sum over held active DOFs of (K u)_raw - f (VoxelFEA::reactions,
+ * voxel_fea.cpp:929-940). f may be NULL. */
+int psx_reactions(psx_model m, const double* u, const double* f, double out[3]);
+
+/* Per-level kernel timings, one line each (GpuFeaSolver::profile, gpu_fea.cpp:576-652). */
+int psx_profile(psx_model m, int reps, char* buf, size_t len);
+
+/* Device memory of the model's buffers in bytes (hierarchy + work vectors). */
+int64_t psx_device_bytes(psx_model m);
+
+/* ---- host-side inspection (no device needed): the hierarchy exactly as it is uploaded ---- */
+typedef struct psx_hierarchy_s* psx_hierarchy;
+/* Element matrix K (576) and corner stresses D*Bbar (8 x 6 x 24) of the unit cube for E = 1
+ * (hexElement, native/src/fea/hex8.cpp:104-155). */
+int psx_hex_element(double nu, double* K, double* corner_stress);
+/* Builds the hierarchy on the host only (VoxelFEA ctor, voxel_fea.cpp:138-183). */
+int psx_hierarchy_create(const psx_model_desc* desc, psx_hierarchy* out);
+void psx_hierarchy_destroy(psx_hierarchy h);
+int psx_hierarchy_num_levels(psx_hierarchy h);
+int psx_hierarchy_level_info(psx_hierarchy h, int level, psx_level_info* out);
+/* Any pointer may be NULL. fixed/inv_diag: [n_dof]; node_scale: [n_nodes]; words: [voxels];
+ * kb: [576]; own: [n_own * 576] (row-major per element, elements in voxel order). */
+int psx_hierarchy_level(psx_hierarchy h, int level, uint8_t* fixed, double* inv_diag, float* node_scale, int32_t* words, double* kb, float* own);
+/* Coarsest free DOFs: free_idx [m] (level DOF of each dense index) and the dense inverse [m * m]. */
+int psx_hierarchy_coarse(psx_hierarchy h, int64_t* m, int32_t* free_idx, double* ainv);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PSX_H */

@@ -1,0 +1,263 @@
+"""ORACLE — test infrastructure only, never the product.
+
+ctypes bindings for the CPU restatement (``oracle/liboracle.so``, prefix ``orc_``) and for the
+reference's own solver compiled from its sources (``oracle/_ref/libpsref.so``, prefix ``ref_``).
+Both expose the same entry points, so a test can run one case through either and compare.
+Only tests/, ``__graft_entry__.smoke()`` and bench.py's CPU-baseline leg import this module.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PATHS = {"orc": os.path.join(HERE, "liboracle.so"), "ref": os.path.join(HERE, "_ref", "libpsref.so")}
+
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+_f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+_u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+_libs = {}
+
+
+def available(kind):
+    return os.path.exists(PATHS[kind])
+
+
+def lib(kind):
+    """The loaded library for ``kind`` in {"orc", "ref"} with argtypes set."""
+    if kind in _libs:
+        return _libs[kind]
+    if not available(kind):
+        raise FileNotFoundError(PATHS[kind] + " is not built (run make -C oracle)")
+    L = C.CDLL(PATHS[kind])
+    p = kind + "_"
+    vp, i64, dbl = C.c_void_p, C.c_int64, C.c_double
+    sig = {
+        "last_error": (C.c_char_p, []),
+        "hex_element": (C.c_int, [dbl, _f64p, _f64p, _f64p]),
+        "create": (vp, [C.c_int, C.c_int, C.c_int, _f32p, dbl, _u8p, i64, vp]),
+        "destroy": (None, [vp]),
+        "num_levels": (C.c_int, [vp]),
+        "level_info": (None, [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
+        "level_arrays": (None, [vp, C.c_int, vp, vp, vp, vp]),
+        "level_kb": (None, [vp, C.c_int, vp, vp]),
+        "apply": (None, [vp, C.c_int, _f64p, _f64p, C.c_int]),
+        "apply32": (None, [vp, C.c_int, _f32p, _f32p, C.c_int]),
+        "restrict": (None, [vp, C.c_int, _f64p, _f64p, C.c_int]),
+        "prolong_add": (None, [vp, C.c_int, _f64p, _f64p]),
+        "prepare_smoothers": (None, [vp]),
+        "coarse_m": (i64, [vp]),
+        "coarse_solve64": (None, [vp, _f64p, _f64p]),
+        "precondition": (None, [vp, _f64p, _f64p]),
+        "solve": (C.c_int, [vp, _f64p, vp, dbl, C.c_int, _f64p, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
+        "reactions": (None, [vp, _f64p, vp, _f64p]),
+        "stresses": (None, [vp, _f64p, dbl, dbl, C.c_int, _f32p, vp, vp]),
+        "stress_tensors": (None, [vp, _f64p, dbl, dbl, _f32p]),
+        "principal": (None, [dbl] * 6 + [_f64p]),
+        "von_mises": (dbl, [_f64p]),
+        "prune_floating": (C.c_int, [C.c_int, C.c_int, C.c_int, _f32p, _u8p, _f32p]),
+        "tridiagonal_max": (dbl, [_f64p, C.c_int, _f64p]),
+        "cheb_coefficients": (None, [dbl, _f64p]),
+    }
+    if kind == "ref":
+        sig["timed_solve"] = (dbl, [C.c_int, C.c_int, C.c_int, _f32p, dbl, _u8p, i64, _f64p, dbl, C.c_int, _f64p,
+                                    C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_double)])
+        sig["threads"] = (C.c_uint, [])
+    else:
+        sig["set_threads"] = (None, [C.c_int])
+    for name, (res, args) in sig.items():
+        fn = getattr(L, p + name)
+        fn.restype = res
+        fn.argtypes = args
+    _libs[kind] = L
+    return L
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def hex_element(nu, kind="orc"):
+    L = lib(kind)
+    K = np.zeros(576)
+    cs = np.zeros(8 * 144)
+    gb = np.zeros(8 * 144)
+    if getattr(L, kind + "_hex_element")(nu, K, cs, gb) != 0:
+        raise ValueError(getattr(L, kind + "_last_error")().decode())
+    return K.reshape(24, 24), cs.reshape(8, 6, 24), gb.reshape(8, 6, 24)
+
+
+class Model:
+    """One VoxelFEA (hierarchy + coarse factor) built by the restatement or the reference."""
+
+    def __init__(self, dims, density, nu, bc, coarsest_max_dof=1100, diag_add=None, kind="orc"):
+        self.kind = kind
+        self.L = lib(kind)
+        self.dims = tuple(int(d) for d in dims)
+        nx, ny, nz = self.dims
+        self.density = np.ascontiguousarray(density, np.float32).reshape(-1)
+        self.bc = np.ascontiguousarray(bc, np.uint8).reshape(-1)
+        self._da = None if diag_add is None else np.ascontiguousarray(diag_add, np.float64)
+        self.h = self._f("create")(nx, ny, nz, self.density, float(nu), self.bc, int(coarsest_max_dof), _ptr(self._da))
+        if not self.h:
+            raise ValueError(self._f("last_error")().decode())
+        self.nu = nu
+
+    def _f(self, name):
+        return getattr(self.L, self.kind + "_" + name)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self._f("destroy")(self.h)
+            self.h = None
+
+    @property
+    def num_levels(self):
+        return self._f("num_levels")(self.h)
+
+    def level(self, l):
+        dims = (C.c_int * 6)()
+        info = (C.c_int64 * 4)()
+        lm = C.c_double()
+        self._f("level_info")(self.h, l, dims, info, C.byref(lm))
+        return {"dims": tuple(dims[:3]), "NXYZ": tuple(dims[3:]), "nNodes": info[0], "nDof": info[1],
+                "freeDof": info[2], "nOwn": info[3], "lmax": lm.value}
+
+    @property
+    def n_dof(self):
+        return self.level(0)["nDof"]
+
+    def level_arrays(self, l):
+        lv = self.level(l)
+        fixed = np.zeros(lv["nDof"], np.uint8)
+        inv = np.zeros(lv["nDof"])
+        ns = np.zeros(lv["nNodes"], np.float32)
+        vs = np.zeros(int(np.prod(lv["dims"])), np.float32)
+        self._f("level_arrays")(self.h, l, _ptr(fixed), _ptr(inv), _ptr(ns), _ptr(vs))
+        return fixed, inv, ns, vs
+
+    def level_kb(self, l):
+        kb = np.zeros(576)
+        sb = np.zeros(324)
+        self._f("level_kb")(self.h, l, _ptr(kb), _ptr(sb))
+        return kb.reshape(24, 24), sb
+
+    def apply(self, x, level=0, raw=False):
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.zeros_like(x)
+        self._f("apply")(self.h, level, x, y, int(raw))
+        return y
+
+    def apply32(self, x, level=0, raw=False):
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.zeros_like(x)
+        self._f("apply32")(self.h, level, x, y, int(raw))
+        return y
+
+    def restrict(self, rf, level, zero_fixed=True):
+        rf = np.ascontiguousarray(rf, np.float64)
+        rc = np.zeros(self.level(level + 1)["nDof"])
+        self._f("restrict")(self.h, level, rf, rc, int(zero_fixed))
+        return rc
+
+    def prolong_add(self, zc, zf, level):
+        zf = np.array(zf, np.float64, copy=True)
+        self._f("prolong_add")(self.h, level, np.ascontiguousarray(zc, np.float64), zf)
+        return zf
+
+    def prepare_smoothers(self):
+        self._f("prepare_smoothers")(self.h)
+        return [self.level(l)["lmax"] for l in range(self.num_levels)]
+
+    @property
+    def coarse_m(self):
+        return self._f("coarse_m")(self.h)
+
+    def coarse_solve64(self, r):
+        r = np.ascontiguousarray(r, np.float64)
+        z = np.zeros_like(r)
+        self._f("coarse_solve64")(self.h, r, z)
+        return z
+
+    def precondition(self, r):
+        r = np.ascontiguousarray(r, np.float64)
+        z = np.zeros_like(r)
+        self._f("precondition")(self.h, r, z)
+        return z
+
+    def solve(self, f, tol=1e-6, max_iter=500, x0=None):
+        f = np.ascontiguousarray(f, np.float64)
+        x0a = None if x0 is None else np.ascontiguousarray(x0, np.float64)
+        u = np.zeros_like(f)
+        its = C.c_int()
+        res = C.c_double()
+        ok = self._f("solve")(self.h, f, _ptr(x0a), float(tol), int(max_iter), u, C.byref(its), C.byref(res))
+        if ok < 0:
+            raise ValueError(self._f("last_error")().decode())
+        return {"u": u, "iterations": its.value, "residual": res.value, "converged": ok == 1}
+
+    def reactions(self, u, f=None):
+        R = np.zeros(3)
+        fa = None if f is None else np.ascontiguousarray(f, np.float64)
+        self._f("reactions")(self.h, np.ascontiguousarray(u, np.float64), _ptr(fa), R)
+        return R
+
+    def stresses(self, u, E, h, principals=True):
+        nN = self.level(0)["nNodes"]
+        vm = np.zeros(nN, np.float32)
+        p1 = np.zeros(nN, np.float32) if principals else None
+        p3 = np.zeros(nN, np.float32) if principals else None
+        self._f("stresses")(self.h, np.ascontiguousarray(u, np.float64), float(E), float(h), int(principals), vm, _ptr(p1), _ptr(p3))
+        return vm, p1, p3
+
+    def stress_tensors(self, u, E, h):
+        out = np.zeros(6 * self.level(0)["nNodes"], np.float32)
+        self._f("stress_tensors")(self.h, np.ascontiguousarray(u, np.float64), float(E), float(h), out)
+        return out.reshape(-1, 6)
+
+
+def prune_floating(dims, density, held, kind="orc"):
+    L = lib(kind)
+    density = np.ascontiguousarray(density, np.float32).reshape(-1)
+    out = np.zeros_like(density)
+    removed = getattr(L, kind + "_prune_floating")(*[int(d) for d in dims], density, np.ascontiguousarray(held, np.uint8), out)
+    return out, removed
+
+
+def principal(s, kind="orc"):
+    out = np.zeros(3)
+    getattr(lib(kind), kind + "_principal")(*[float(v) for v in s], out)
+    return out
+
+
+def von_mises(s, kind="orc"):
+    return getattr(lib(kind), kind + "_von_mises")(np.ascontiguousarray(s, np.float64))
+
+
+def tridiagonal_max(a, b, kind="orc"):
+    a = np.ascontiguousarray(a, np.float64)
+    b = np.ascontiguousarray(b if len(b) else [0.0], np.float64)
+    return getattr(lib(kind), kind + "_tridiagonal_max")(a, len(a), b)
+
+
+def cheb_coefficients(lmax, kind="orc"):
+    out = np.zeros(3)
+    getattr(lib(kind), kind + "_cheb_coefficients")(float(lmax), out)
+    return out
+
+
+def ref_timed_solve(dims, density, nu, bc, f, coarsest_max_dof, tol, max_iter):
+    """The reference's own multithreaded build + solve, timed inside the library (CPU baseline)."""
+    L = lib("ref")
+    u = np.zeros(len(f))
+    its, res, tb = C.c_int(), C.c_double(), C.c_double()
+    t = L.ref_timed_solve(*[int(d) for d in dims], np.ascontiguousarray(density, np.float32), float(nu),
+                          np.ascontiguousarray(bc, np.uint8), int(coarsest_max_dof), np.ascontiguousarray(f, np.float64),
+                          float(tol), int(max_iter), u, C.byref(its), C.byref(res), C.byref(tb))
+    if t < 0:
+        raise ValueError(L.ref_last_error().decode())
+    return {"solve_s": t, "build_s": tb.value, "u": u, "iterations": its.value, "residual": res.value}
+
+
+def ref_threads():
+    return int(lib("ref").ref_threads())

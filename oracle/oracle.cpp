@@ -1,0 +1,1422 @@
+// ORACLE — test infrastructure only, never the product.
+//
+// A CPU restatement of Parts Sim's native linear-static voxel FEA solve, used by tests/, by
+// __graft_entry__.smoke() and by bench.py's cpu_baseline leg as the checker. Nothing in the
+// product (parts-sim_amd/) links, loads or calls this file.
+//
+// Every function follows the reference file:line given beside it, keeping the floating-point
+// operation order of the reference where that decides bitwise results (no FMA contraction:
+// build with -ffp-contract=off, as the reference's x86-64 build has no FMA instructions), and the
+// reference's chunked reduction order (native/src/util/parallel.hpp:287-325) for a given thread
+// count, so results can be compared bit for bit with the compiled reference in oracle/_ref.
+//
+// Pins (tests/test_oracle.py):
+//   hex_element        K0 symmetry + rigid-body null space + Gauss-point reproduction (closed forms),
+//                      golden K0 vector, bitwise vs oracle/_ref
+//   apply (64/32)      brute-force assembled sparse K (scipy) on tiny grids; bitwise vs oracle/_ref
+//   coarsen/Galerkin   explicit P^T K P with P assembled from transfer maps; vs oracle/_ref
+//   coarse Cholesky    numpy dense solve of the assembled coarsest matrix
+//   lanczos lmax       numpy eigvalsh of D^-1/2 K D^-1/2 (upper bound); bitwise vs oracle/_ref
+//   precondition       bitwise vs oracle/_ref
+//   solve              scipy spsolve of the assembled K; Timoshenko beam (6 %); equilibrium;
+//                      bitwise u + iteration count vs oracle/_ref and the golden fixtures
+//   stresses           patch test (linear u -> constant sigma); M c / I; vs oracle/_ref
+//   prune_floating     known-answer 4x1x1 case (test_fea.cpp:115-122); brute force BFS; vs _ref
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <vector>
+
+namespace orc {
+
+using i64 = int64_t;
+
+// ---------------------------------------------------------------------------------------------
+// deterministic chunking (native/src/util/parallel.hpp:287-325, parallel.cpp:7-16)
+
+static int g_threads = 0;
+static int threads() {
+    if (g_threads > 0) return g_threads;
+    if (const char* env = std::getenv("PARTS_SIM_THREADS")) {
+        int n = std::atoi(env);
+        if (n > 0) return g_threads = n;
+    }
+    return g_threads = std::max(1u, std::thread::hardware_concurrency());
+}
+static int chunks(i64 n, i64 grain) {
+    const i64 byGrain = std::max<i64>(1, n / std::max<i64>(1, grain));
+    return int(std::min<i64>(byGrain, i64(threads()) * 4));
+}
+// sum of fn(lo, hi) partials over [0, n), each chunk's partial added in chunk order
+template <class F> static double chunkedSum(i64 n, F&& fn, i64 grain = 8192) {
+    if (n <= 0) return 0.0;
+    const int c = chunks(n, grain);
+    if (c <= 1) return fn(i64(0), n);
+    double s = 0;
+    for (int q = 0; q < c; q++) {
+        const i64 b = n * q / c, e = n * (q + 1) / c;
+        s += b < e ? fn(b, e) : 0.0;
+    }
+    return s;
+}
+static double dotAll(const double* a, const double* b, i64 n) {
+    return chunkedSum(n, [&](i64 lo, i64 hi) {
+        double s = 0;
+        for (i64 i = lo; i < hi; i++) s += a[i] * b[i];
+        return s;
+    });
+}
+
+// ---------------------------------------------------------------------------------------------
+// hex8 with Wilson incompatible modes (native/src/fea/hex8.cpp)
+
+static const int HX[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0}, {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+
+// hex8.cpp:8-19
+static void elasticity(double nu, double* D) {
+    if (!std::isfinite(nu) || nu <= -1 || nu >= 0.5) throw std::invalid_argument("Poisson's ratio must be greater than -1 and less than 0.5.");
+    const double lam = nu / ((1 + nu) * (1 - 2 * nu)), mu = 1 / (2 * (1 + nu));
+    std::fill(D, D + 36, 0.0);
+    for (int i = 0; i < 3; i++) {
+        for (int j = 0; j < 3; j++) D[i * 6 + j] = lam;
+        D[i * 6 + i] = lam + 2 * mu;
+    }
+    D[21] = D[28] = D[35] = mu;
+}
+
+// hex8.cpp:21-30: the six strain rows that the three displacement derivatives of one column triple feed
+static void strainColumns(double* B, int cols, int c, double gx, double gy, double gz) {
+    B[c] = gx;
+    B[cols + c + 1] = gy;
+    B[2 * cols + c + 2] = gz;
+    B[3 * cols + c] = gy;
+    B[3 * cols + c + 1] = gx;
+    B[4 * cols + c + 1] = gz;
+    B[4 * cols + c + 2] = gy;
+    B[5 * cols + c] = gz;
+    B[5 * cols + c + 2] = gx;
+}
+
+// hex8.cpp:32-43 (6 x 24, unit cube: d(xi)/dx = 2)
+static std::vector<double> bmat(double xi, double eta, double zeta) {
+    std::vector<double> B(144, 0.0);
+    for (int a = 0; a < 8; a++) {
+        const double sx = 2 * HX[a][0] - 1, sy = 2 * HX[a][1] - 1, sz = 2 * HX[a][2] - 1;
+        const double gx = 2 * sx * (1 + sy * eta) * (1 + sz * zeta) / 8;
+        const double gy = 2 * sy * (1 + sx * xi) * (1 + sz * zeta) / 8;
+        const double gz = 2 * sz * (1 + sx * xi) * (1 + sy * eta) / 8;
+        strainColumns(B.data(), 24, 3 * a, gx, gy, gz);
+    }
+    return B;
+}
+
+// hex8.cpp:46-51 (6 x 9 bubble-mode strains)
+static std::vector<double> gmat(double xi, double eta, double zeta) {
+    std::vector<double> G(54, 0.0);
+    strainColumns(G.data(), 9, 0, -4 * xi, 0, 0);
+    strainColumns(G.data(), 9, 3, 0, -4 * eta, 0);
+    strainColumns(G.data(), 9, 6, 0, 0, -4 * zeta);
+    return G;
+}
+
+// hex8.cpp:54-62: C (r x c) += s * A^T B with A k x r, B k x c
+static void accAtB(double* C, const double* A, const double* B, int k, int r, int c, double s) {
+    for (int i = 0; i < r; i++)
+        for (int j = 0; j < c; j++) {
+            double acc = 0;
+            for (int q = 0; q < k; q++) acc += A[q * r + i] * B[q * c + j];
+            C[i * c + j] += s * acc;
+        }
+}
+
+// hex8.cpp:64-73
+static std::vector<double> mul(const double* A, const double* B, int n, int k, int m) {
+    std::vector<double> C(size_t(n) * m, 0.0);
+    for (int i = 0; i < n; i++)
+        for (int q = 0; q < k; q++) {
+            const double a = A[i * k + q];
+            if (a == 0) continue;
+            for (int j = 0; j < m; j++) C[i * m + j] += a * B[q * m + j];
+        }
+    return C;
+}
+
+// hex8.cpp:75-103: Gauss-Jordan with partial pivoting
+static std::vector<double> inverse(std::vector<double> A, int n) {
+    std::vector<double> I(size_t(n) * n, 0.0);
+    for (int i = 0; i < n; i++) I[i * n + i] = 1;
+    for (int col = 0; col < n; col++) {
+        int piv = col;
+        for (int r = col + 1; r < n; r++)
+            if (std::abs(A[r * n + col]) > std::abs(A[piv * n + col])) piv = r;
+        if (piv != col)
+            for (int j = 0; j < n; j++) {
+                std::swap(A[col * n + j], A[piv * n + j]);
+                std::swap(I[col * n + j], I[piv * n + j]);
+            }
+        const double d = A[col * n + col];
+        for (int j = 0; j < n; j++) {
+            A[col * n + j] /= d;
+            I[col * n + j] /= d;
+        }
+        for (int r = 0; r < n; r++) {
+            if (r == col) continue;
+            const double f = A[r * n + col];
+            if (f == 0) continue;
+            for (int j = 0; j < n; j++) {
+                A[r * n + j] -= f * A[col * n + j];
+                I[r * n + j] -= f * I[col * n + j];
+            }
+        }
+    }
+    return I;
+}
+
+struct Hex {
+    double K[576];
+    double corner[8][144];
+    double gauss[8][144];
+};
+
+// hex8.cpp:104-155
+static Hex hexElement(double nu, bool modes = true) {
+    double D[36];
+    elasticity(nu, D);
+    const double g = 1 / std::sqrt(3.0), w = 1.0 / 8;
+    std::vector<double> Kuu(576, 0.0), Kua(216, 0.0), Kaa(81, 0.0);
+    const double pts[2] = {-g, g};
+    for (double xi : pts)
+        for (double eta : pts)
+            for (double zeta : pts) {
+                const auto B = bmat(xi, eta, zeta);
+                const auto DB = mul(D, B.data(), 6, 6, 24);
+                accAtB(Kuu.data(), B.data(), DB.data(), 6, 24, 24, w);
+                if (modes) {
+                    const auto G = gmat(xi, eta, zeta);
+                    const auto DG = mul(D, G.data(), 6, 6, 9);
+                    accAtB(Kua.data(), B.data(), DG.data(), 6, 24, 9, w);
+                    accAtB(Kaa.data(), G.data(), DG.data(), 6, 9, 9, w);
+                }
+            }
+    std::vector<double> A;  // internal mode amplitudes alpha = A u (9 x 24)
+    if (modes) {
+        const auto Ki = inverse(Kaa, 9);
+        std::vector<double> Kau(216);
+        for (int i = 0; i < 24; i++)
+            for (int q = 0; q < 9; q++) Kau[q * 24 + i] = Kua[i * 9 + q];
+        A = mul(Ki.data(), Kau.data(), 9, 9, 24);
+        for (double& v : A) v = -v;
+        const auto X = mul(Kua.data(), A.data(), 24, 9, 24);
+        for (int i = 0; i < 576; i++) Kuu[i] += X[i];
+        for (int i = 0; i < 24; i++)
+            for (int j = i + 1; j < 24; j++) {
+                const double m = 0.5 * (Kuu[i * 24 + j] + Kuu[j * 24 + i]);
+                Kuu[i * 24 + j] = Kuu[j * 24 + i] = m;
+            }
+    }
+    auto enhanced = [&](double xi, double eta, double zeta) {
+        auto Bt = bmat(xi, eta, zeta);
+        if (!A.empty()) {
+            const auto G = gmat(xi, eta, zeta);
+            const auto GA = mul(G.data(), A.data(), 6, 9, 24);
+            for (int i = 0; i < 144; i++) Bt[i] += GA[i];
+        }
+        return Bt;
+    };
+    Hex h;
+    std::copy(Kuu.begin(), Kuu.end(), h.K);
+    for (int a = 0; a < 8; a++) {
+        const double cx = 2 * HX[a][0] - 1, cy = 2 * HX[a][1] - 1, cz = 2 * HX[a][2] - 1;
+        const auto Bc = enhanced(cx, cy, cz);
+        const auto S = mul(D, Bc.data(), 6, 6, 24);
+        std::copy(S.begin(), S.end(), h.corner[a]);
+        const auto Bg = enhanced(cx * g, cy * g, cz * g);
+        std::copy(Bg.begin(), Bg.end(), h.gauss[a]);
+    }
+    return h;
+}
+
+// ---------------------------------------------------------------------------------------------
+// levels (native/src/fea/voxel_fea.hpp:33-110, voxel_fea.cpp:51-386)
+
+// voxel_fea.cpp:52: element corner of the node at unit offset (x, y, z)
+static inline int cornerOf(int x, int y, int z) { return (y ? (x ? 2 : 3) : (x ? 1 : 0)) + 4 * z; }
+
+struct Transfer {
+    std::vector<int> c0, c1;
+};
+
+// voxel_fea.cpp:124-134
+static Transfer transferMap(int nFine) {
+    Transfer m;
+    m.c0.resize(nFine + 1);
+    m.c1.resize(nFine + 1);
+    for (int i = 0; i <= nFine; i++) {
+        m.c0[i] = i / 2;
+        m.c1[i] = (i % 2) ? (i + 1) / 2 : -1;
+    }
+    return m;
+}
+
+struct Level {
+    int nx = 0, ny = 0, nz = 0, NX = 0, NY = 0, NZ = 0;
+    i64 nNodes = 0, nDof = 0;
+    i64 off[8]{};
+    std::vector<int> elems;
+    std::vector<i64> base;
+    std::vector<double> rho;
+    std::vector<float> K;
+    std::vector<int> kIdx;
+    double Kb[580]{};
+    float Kbf[580]{};
+    double Sb[324]{};
+    float Sbf[324]{};
+    std::vector<int> emap;
+    std::vector<float> voxelScale, nodeScale;
+    std::vector<int> rowFirst, rowLast;
+    std::vector<i64> activeRows;
+    std::vector<uint8_t> bc, fixed, activeNode;
+    std::vector<double> invDiag, diagAdd;
+    std::vector<float> invDiag32;
+    double omega = 0.5, lmax = 0;
+    i64 freeDof = 0;
+    std::vector<float> r, z, t, d;
+    Transfer mx, my, mz;
+    std::vector<int> childStart, children;
+
+    bool scaled(i64 e) const { return kIdx.empty() || kIdx[e] < 0; }
+    const float* matrix(i64 e) const { return K.data() + i64(kIdx[e]) * 576; }
+    double scale(i64 e) const { return scaled(e) ? rho[e] : 1.0; }
+    // fn(lo, hi) over the node ranges of the active row spans (voxel_fea.hpp:88-99)
+    template <class F> void forActiveNodes(F&& fn) const {
+        for (i64 row : activeRows) fn(row * NX + rowFirst[row], row * NX + rowLast[row] + 1);
+    }
+    template <class F> void forActive(F&& fn) const {
+        forActiveNodes([&](i64 lo, i64 hi) { fn(3 * lo, 3 * hi); });
+    }
+    // voxel_fea.hpp:101-110: chunked over active rows (grain 4096 / NX), sequential within a row
+    template <class F> double sumActive(F&& fn) const {
+        const i64 nr = i64(activeRows.size());
+        return chunkedSum(nr, [&](i64 a, i64 b) {
+            double s = 0;
+            for (i64 q = a; q < b; q++) {
+                const i64 row = activeRows[q];
+                s += fn(3 * (row * NX + rowFirst[row]), 3 * (row * NX + rowLast[row] + 1));
+            }
+            return s;
+        }, std::max<i64>(1, 4096 / NX));
+    }
+};
+
+// voxel_fea.cpp:113-122
+static Level makeLevel(int nx, int ny, int nz) {
+    Level L;
+    L.nx = nx; L.ny = ny; L.nz = nz;
+    L.NX = nx + 1; L.NY = ny + 1; L.NZ = nz + 1;
+    const i64 NXY = i64(L.NX) * L.NY;
+    L.nNodes = NXY * L.NZ;
+    L.nDof = 3 * L.nNodes;
+    const i64 o[8] = {0, 1, 1 + L.NX, L.NX, NXY, 1 + NXY, 1 + L.NX + NXY, L.NX + NXY};
+    std::copy(o, o + 8, L.off);
+    return L;
+}
+
+// voxel_fea.cpp:54-83: fine child-node a of child c interpolates coarse corner b with weight w
+struct ChildInterp {
+    int count[8][8];
+    int node[8][8][8];
+    double w[8][8][8];
+};
+static const ChildInterp& childInterp() {
+    static const ChildInterp P = [] {
+        ChildInterp p{};
+        for (int c = 0; c < 8; c++) {
+            const int ci = c & 1, cj = (c >> 1) & 1, ck = (c >> 2) & 1;
+            for (int a = 0; a < 8; a++) {
+                const double sx = (ci + HX[a][0]) / 2.0, sy = (cj + HX[a][1]) / 2.0, sz = (ck + HX[a][2]) / 2.0;
+                int n = 0;
+                for (int b = 0; b < 8; b++) {
+                    const double w = (HX[b][0] ? sx : 1 - sx) * (HX[b][1] ? sy : 1 - sy) * (HX[b][2] ? sz : 1 - sz);
+                    if (w > 1e-12) {
+                        p.node[c][a][n] = b;
+                        p.w[c][a][n] = w;
+                        n++;
+                    }
+                }
+                p.count[c][a] = n;
+            }
+        }
+        return p;
+    }();
+    return P;
+}
+
+// voxel_fea.cpp:85-111: out += scale * P_c^T K P_c (T: 576 scratch)
+static void galerkinAdd(double* out, const double* K, double scale, int c, double* T) {
+    const ChildInterp& P = childInterp();
+    std::fill(T, T + 576, 0.0);
+    for (int a = 0; a < 8; a++)
+        for (int t = 0; t < P.count[c][a]; t++) {
+            const int b3 = 3 * P.node[c][a][t], a3 = 3 * a;
+            const double w = P.w[c][a][t];
+            for (int r = 0; r < 24; r++) {
+                const double* kr = K + r * 24 + a3;
+                double* tr = T + r * 24 + b3;
+                tr[0] += kr[0] * w;
+                tr[1] += kr[1] * w;
+                tr[2] += kr[2] * w;
+            }
+        }
+    for (int a = 0; a < 8; a++)
+        for (int t = 0; t < P.count[c][a]; t++) {
+            const int b3 = 3 * P.node[c][a][t], a3 = 3 * a;
+            const double ws = P.w[c][a][t] * scale;
+            for (int d = 0; d < 3; d++) {
+                double* o = out + (b3 + d) * 24;
+                const double* tr = T + (a3 + d) * 24;
+                for (int q = 0; q < 24; q++) o[q] += ws * tr[q];
+            }
+        }
+}
+
+// voxel_fea.cpp:186-206: the level's uniform element, its float copy and 27-point stencil
+static void setBase(Level& L, const double* K) {
+    std::fill(L.Kb, L.Kb + 580, 0.0);
+    std::fill(L.Kbf, L.Kbf + 580, 0.f);
+    std::fill(L.Sb, L.Sb + 324, 0.0);
+    for (int i = 0; i < 576; i++) {
+        L.Kb[i] = K[i];
+        L.Kbf[i] = float(K[i]);
+    }
+    for (int v = 0; v < 8; v++) {
+        const int di = v & 1, dj = (v >> 1) & 1, dk = v >> 2, a = cornerOf(1 - di, 1 - dj, 1 - dk);
+        for (int w = 0; w < 8; w++) {
+            const int bx = w & 1, by = (w >> 1) & 1, bz = w >> 2, b = cornerOf(bx, by, bz);
+            const int row = (dj + by) + 3 * (dk + bz), col = 3 * (di + bx);
+            for (int d = 0; d < 3; d++)
+                for (int c = 0; c < 3; c++) L.Sb[(row * 9 + col + c) * 4 + d] += K[(3 * a + d) * 24 + 3 * b + c];
+        }
+    }
+    for (int i = 0; i < 324; i++) L.Sbf[i] = float(L.Sb[i]);
+}
+
+// voxel_fea.cpp:208-266
+static void finishLevel(Level& L) {
+    L.activeNode.assign(L.nNodes, 0);
+    std::vector<double> diag(L.nDof, 0.0);
+    const i64 nE = i64(L.elems.size());
+    for (i64 e = 0; e < nE; e++) {
+        const i64 n0 = L.base[e];
+        const bool sc = L.scaled(e);
+        const float* K = sc ? nullptr : L.matrix(e);
+        for (int a = 0; a < 8; a++) {
+            const i64 n = n0 + L.off[a];
+            L.activeNode[n] = 1;
+            for (int d = 0; d < 3; d++) diag[3 * n + d] += sc ? L.rho[e] * L.Kb[(3 * a + d) * 25] : double(K[(3 * a + d) * 25]);
+        }
+    }
+    if (!L.diagAdd.empty())
+        for (i64 i = 0; i < L.nDof; i++) diag[i] += L.diagAdd[i];
+    L.fixed.assign(L.nDof, 0);
+    L.invDiag.assign(L.nDof, 0.0);
+    i64 nfree = 0;
+    for (i64 n = 0; n < L.nNodes; n++)
+        for (int d = 0; d < 3; d++) {
+            const i64 i = 3 * n + d;
+            if (!L.activeNode[n] || L.bc[i] || !(diag[i] > 0)) L.fixed[i] = 1;
+            else { L.invDiag[i] = 1 / diag[i]; nfree++; }
+        }
+    L.freeDof = nfree;
+    L.invDiag32.assign(L.invDiag.begin(), L.invDiag.end());
+    const i64 rows = i64(L.NY) * L.NZ;
+    L.rowFirst.assign(rows, -1);
+    L.rowLast.assign(rows, -1);
+    L.activeRows.clear();
+    for (i64 row = 0; row < rows; row++) {
+        for (int i = 0; i < L.NX; i++)
+            if (L.activeNode[row * L.NX + i]) {
+                if (L.rowFirst[row] < 0) L.rowFirst[row] = i;
+                L.rowLast[row] = i;
+            }
+        if (L.rowFirst[row] >= 0) L.activeRows.push_back(row);
+    }
+    L.voxelScale.assign(L.emap.size(), 0.f);
+    for (size_t v = 0; v < L.emap.size(); v++)
+        if (L.emap[v] >= 0) L.voxelScale[v] = L.scaled(L.emap[v]) ? float(L.rho[L.emap[v]]) : -1.f;
+    L.nodeScale.assign(L.nNodes, 0.f);
+    auto vs = [&](int i, int j, int k) { return L.voxelScale[i + i64(L.nx) * (j + i64(L.ny) * k)]; };
+    for (int k = 1; k < L.nz; k++)
+        for (int j = 1; j < L.ny; j++)
+            for (int i = 1; i < L.nx; i++) {
+                const float s0 = vs(i - 1, j - 1, k - 1);
+                bool u = s0 > 0;
+                for (int v = 1; v < 8 && u; v++) u = vs(i - 1 + (v & 1), j - 1 + ((v >> 1) & 1), k - 1 + (v >> 2)) == s0;
+                if (u) L.nodeScale[i + i64(L.NX) * (j + i64(L.NY) * k)] = s0;
+            }
+    L.r.assign(L.nDof, 0.f);
+    L.z.assign(L.nDof, 0.f);
+    L.t.assign(L.nDof, 0.f);
+    L.d.assign(L.nDof, 0.f);
+}
+
+struct Coarse {
+    std::vector<i64> map;
+    i64 m = 0;
+    std::vector<double> A, y;
+};
+
+struct Model {
+    double nu = 0.3;
+    Hex el;
+    std::vector<Level> levels;
+    Coarse coarse;
+};
+
+template <class T> static void restrictT(const Level& F, const Level& C, const T* rf, T* rc, bool zeroFixed);
+
+// voxel_fea.cpp:268-386
+static Level coarsen(Level& F) {
+    Level C = makeLevel((F.nx + 1) >> 1, (F.ny + 1) >> 1, (F.nz + 1) >> 1);
+    const i64 nCv = i64(C.nx) * C.ny * C.nz;
+    C.emap.assign(nCv, -1);
+    const i64 nF = i64(F.elems.size());
+    std::vector<int> fineCoarse(nF);
+    std::vector<uint8_t> fineChild(nF);
+    for (i64 q = 0; q < nF; q++) {
+        const int e = F.elems[q];
+        const int i = e % F.nx, j = (e / F.nx) % F.ny, k = e / (F.nx * F.ny);
+        const int ce = (i >> 1) + C.nx * ((j >> 1) + C.ny * (k >> 1));
+        if (C.emap[ce] < 0) { C.emap[ce] = int(C.elems.size()); C.elems.push_back(ce); }
+        fineCoarse[q] = C.emap[ce];
+        fineChild[q] = uint8_t((i & 1) | ((j & 1) << 1) | ((k & 1) << 2));
+    }
+    {   // coarse elements in voxel order
+        std::vector<int> order(C.elems);
+        std::sort(order.begin(), order.end());
+        std::vector<int> remap(C.elems.size());
+        for (size_t q = 0; q < order.size(); q++) C.emap[order[q]] = int(q);
+        for (size_t q = 0; q < C.elems.size(); q++) remap[q] = C.emap[C.elems[q]];
+        for (auto& v : fineCoarse) v = remap[v];
+        C.elems = std::move(order);
+    }
+    const i64 nE = i64(C.elems.size());
+    C.base.resize(nE);
+    for (i64 q = 0; q < nE; q++) {
+        const int ce = C.elems[q];
+        const int I = ce % C.nx, J = (ce / C.nx) % C.ny, K = ce / (C.nx * C.ny);
+        C.base[q] = I + i64(C.NX) * (J + i64(C.NY) * K);
+    }
+    C.childStart.assign(nE + 1, 0);
+    for (i64 q = 0; q < nF; q++) C.childStart[fineCoarse[q] + 1]++;
+    for (i64 q = 0; q < nE; q++) C.childStart[q + 1] += C.childStart[q];
+    C.children.resize(nF);
+    {
+        std::vector<int> fill(C.childStart.begin(), C.childStart.end() - 1);
+        for (i64 q = 0; q < nF; q++) C.children[fill[fineCoarse[q]]++] = int(q);
+    }
+    std::vector<std::array<double, 576>> M(8);
+    {
+        std::vector<double> T(576), Kc(576, 0.0);
+        for (int c = 0; c < 8; c++) {
+            std::fill(M[c].begin(), M[c].end(), 0.0);
+            galerkinAdd(M[c].data(), F.Kb, 1.0, c, T.data());
+            for (int u = 0; u < 576; u++) Kc[u] += M[c][u];
+        }
+        setBase(C, Kc.data());
+    }
+    C.rho.assign(nE, 0.0);
+    C.kIdx.assign(nE, -1);
+    int nK = 0;
+    for (i64 q = 0; q < nE; q++) {
+        const int b = C.childStart[q], e = C.childStart[q + 1];
+        bool uniform = e - b == 8 && F.scaled(C.children[b]);
+        const double s = uniform ? F.rho[C.children[b]] : 0;
+        for (int t = b; t < e && uniform; t++) uniform = F.scaled(C.children[t]) && F.rho[C.children[t]] == s;
+        if (uniform) C.rho[q] = s;
+        else C.kIdx[q] = nK++;
+    }
+    C.K.assign(i64(nK) * 576 + 4, 0.f);
+    {
+        std::vector<double> T(576), out(576), Kf(576);
+        for (i64 q = 0; q < nE; q++) {
+            if (C.kIdx[q] < 0) continue;
+            std::fill(out.begin(), out.end(), 0.0);
+            for (int t = C.childStart[q]; t < C.childStart[q + 1]; t++) {
+                const int f = C.children[t];
+                const int c = fineChild[f];
+                if (F.scaled(f)) {
+                    const double s = F.rho[f];
+                    const double* Mc = M[c].data();
+                    for (int u = 0; u < 576; u++) out[u] += s * Mc[u];
+                } else {
+                    const float* Kc = F.matrix(f);
+                    for (int u = 0; u < 576; u++) Kf[u] = Kc[u];
+                    galerkinAdd(out.data(), Kf.data(), 1.0, c, T.data());
+                }
+            }
+            float* dst = C.K.data() + i64(C.kIdx[q]) * 576;
+            for (int u = 0; u < 576; u++) dst[u] = float(out[u]);
+        }
+    }
+    F.mx = transferMap(F.nx);
+    F.my = transferMap(F.ny);
+    F.mz = transferMap(F.nz);
+    C.bc.assign(C.nDof, 0);
+    for (int k = 0; k < F.NZ; k++)
+        for (int j = 0; j < F.NY; j++)
+            for (int i = 0; i < F.NX; i++) {
+                const i64 fn = 3 * (i + i64(F.NX) * (j + i64(F.NY) * k));
+                if (!(F.bc[fn] | F.bc[fn + 1] | F.bc[fn + 2])) continue;
+                const int Ks[2] = {F.mz.c0[k], F.mz.c1[k]}, Js[2] = {F.my.c0[j], F.my.c1[j]}, Is[2] = {F.mx.c0[i], F.mx.c1[i]};
+                for (int K : Ks) {
+                    if (K < 0) continue;
+                    for (int J : Js) {
+                        if (J < 0) continue;
+                        for (int I : Is) {
+                            if (I < 0) continue;
+                            const i64 cn = 3 * (I + i64(C.NX) * (J + i64(C.NY) * K));
+                            C.bc[cn] |= F.bc[fn];
+                            C.bc[cn + 1] |= F.bc[fn + 1];
+                            C.bc[cn + 2] |= F.bc[fn + 2];
+                        }
+                    }
+                }
+            }
+    if (!F.diagAdd.empty()) {
+        C.diagAdd.assign(C.nDof, 0.0);
+        restrictT<double>(F, C, F.diagAdd.data(), C.diagAdd.data(), false);
+    }
+    finishLevel(C);
+    return C;
+}
+
+// voxel_fea.cpp:390-504: y = K x on a level, one node at a time (gather, no write conflicts)
+template <class T> static void applyT(const Level& L, const T* x, T* y, bool raw) {
+    const int NX = L.NX, NY = L.NY, NZ = L.NZ, nx = L.nx, ny = L.ny, nz = L.nz;
+    const i64 NXY = i64(NX) * NY;
+    const T* Kt;
+    const T* St;
+    if constexpr (sizeof(T) == 4) { Kt = L.Kbf; St = L.Sbf; }
+    else { Kt = L.Kb; St = L.Sb; }
+    for (i64 row = 0; row < i64(NY) * NZ; row++) {
+        const int first = L.rowFirst[row], last = L.rowLast[row];
+        T* yrow = y + 3 * NX * row;
+        if (first < 0) {
+            std::fill(yrow, yrow + 3 * NX, T(0));
+            continue;
+        }
+        std::fill(yrow, yrow + 3 * first, T(0));
+        std::fill(yrow + 3 * (last + 1), yrow + 3 * NX, T(0));
+        const int j = int(row % NY), k = int(row / NY);
+        i64 vrow[4];
+        bool has[4];
+        for (int q = 0; q < 4; q++) {
+            const int ej = j - 1 + (q & 1), ek = k - 1 + (q >> 1);
+            has[q] = ej >= 0 && ek >= 0 && ej < ny && ek < nz;
+            vrow[q] = has[q] ? i64(nx) * (ej + i64(ny) * ek) : 0;
+        }
+        for (int i = first; i <= last; i++) {
+            const i64 n = i + NX * row;
+            T acc[3] = {0, 0, 0};
+            if (L.activeNode[n]) {
+                const float s0 = L.nodeScale[n];
+                if (s0 > 0) {
+                    // 27-point stencil: nine neighbour rows of 9 contiguous DOFs; lane d = output row d
+                    const T* x0 = x + 3 * (n - 1 - NX - NXY);
+                    T lane[4] = {0, 0, 0, 0};
+                    for (int r = 0; r < 9; r++) {
+                        const T* xr = x0 + 3 * ((r % 3) * NX + (r / 3) * NXY);
+                        const T* col = St + r * 36;
+                        for (int d = 0; d < 4; d++) {
+                            T b = col[d] * xr[0];
+                            for (int c = 1; c < 9; c++) b += col[4 * c + d] * xr[c];
+                            lane[d] += b;
+                        }
+                    }
+                    for (int d = 0; d < 3; d++) acc[d] = T(s0) * lane[d];
+                } else {
+                    for (int v = 0; v < 8; v++) {
+                        const int q = v >> 1, di = v & 1, ei = i - 1 + di;
+                        if (!has[q] || ei < 0 || ei >= nx) continue;
+                        const int e = L.emap[vrow[q] + ei];
+                        if (e < 0) continue;
+                        const int dj = q & 1, dk = q >> 1, a = cornerOf(1 - di, 1 - dj, 1 - dk);
+                        // two partial accumulators alternate over the 24 columns (lanes 0..2 used)
+                        T b0[4] = {0, 0, 0, 0}, b1[4] = {0, 0, 0, 0};
+                        auto columns = [&](const auto* colBase) {
+                            for (int f = 0; f < 4; f++) {
+                                const int by = f & 1, bz = f >> 1;
+                                const T* xs = x + 3 * (n - (1 - di) + NX * (by - (1 - dj)) + NXY * (bz - (1 - dk)));
+                                const auto* c0 = colBase + 72 * cornerOf(0, by, bz);
+                                const auto* c1 = colBase + 72 * cornerOf(1, by, bz);
+                                for (int d = 0; d < 4; d++) {
+                                    b0[d] += T(c0[d]) * xs[0];
+                                    b1[d] += T(c0[24 + d]) * xs[1];
+                                    b0[d] += T(c0[48 + d]) * xs[2];
+                                    b1[d] += T(c1[d]) * xs[3];
+                                    b0[d] += T(c1[24 + d]) * xs[4];
+                                    b1[d] += T(c1[48 + d]) * xs[5];
+                                }
+                            }
+                        };
+                        T s = 1;
+                        if (L.scaled(e)) {
+                            columns(Kt + 3 * a);
+                            s = T(L.rho[e]);
+                        } else columns(L.matrix(e) + 3 * a);
+                        for (int d = 0; d < 3; d++) acc[d] += s * (b0[d] + b1[d]);
+                    }
+                }
+                if (!L.diagAdd.empty())
+                    for (int d = 0; d < 3; d++) acc[d] += T(L.diagAdd[3 * n + d]) * x[3 * n + d];
+            }
+            T* yp = y + 3 * n;
+            for (int d = 0; d < 3; d++) yp[d] = (!raw && L.fixed[3 * n + d]) ? T(0) : acc[d];
+        }
+    }
+}
+
+// voxel_fea.cpp:506-530: largest eigenvalue of a symmetric tridiagonal (Sturm bisection)
+static double tridiagonalMax(const std::vector<double>& a, const std::vector<double>& b) {
+    const int k = int(a.size());
+    double lo = INFINITY, hi = -INFINITY;
+    for (int i = 0; i < k; i++) {
+        const double r = (i > 0 ? std::abs(b[i - 1]) : 0) + (i + 1 < k ? std::abs(b[i]) : 0);
+        lo = std::min(lo, a[i] - r);
+        hi = std::max(hi, a[i] + r);
+    }
+    auto above = [&](double x) {
+        int count = 0;
+        double q = 1;
+        for (int i = 0; i < k; i++) {
+            q = a[i] - x - (i > 0 ? b[i - 1] * b[i - 1] / q : 0);
+            if (q == 0) q = 1e-300;
+            if (q > 0) count++;
+        }
+        return count;
+    };
+    for (int it = 0; it < 100 && hi - lo > 1e-12 * std::abs(hi); it++) {
+        const double mid = 0.5 * (lo + hi);
+        if (above(mid) > 0) lo = mid;
+        else hi = mid;
+    }
+    return hi;
+}
+
+// voxel_fea.cpp:579-610: 10 Lanczos steps on D^-1/2 K D^-1/2 from an LCG start (seed 12345)
+static void estimateOmega(Level& L) {
+    const i64 n = L.nDof;
+    std::vector<double> sq(n), v(n), vPrev(n, 0.0), w(n), x(n);
+    for (i64 i = 0; i < n; i++) sq[i] = std::sqrt(L.invDiag[i]);
+    uint32_t seed = 12345;
+    for (i64 i = 0; i < n; i++) {
+        seed = (seed * 1103515245u + 12345u) & 0x7fffffffu;
+        v[i] = L.fixed[i] ? 0 : double(seed) / 0x7fffffff - 0.5;
+    }
+    const double nv = std::sqrt(dotAll(v.data(), v.data(), n));
+    for (auto& e : v) e /= nv > 0 ? nv : 1;
+    std::vector<double> alpha, beta;
+    double b = 0;
+    for (int j = 0; j < 10; j++) {
+        for (i64 i = 0; i < n; i++) x[i] = sq[i] * v[i];
+        applyT<double>(L, x.data(), w.data(), false);
+        for (i64 i = 0; i < n; i++) w[i] = sq[i] * w[i] - b * vPrev[i];
+        const double a = dotAll(w.data(), v.data(), n);
+        for (i64 i = 0; i < n; i++) w[i] -= a * v[i];
+        alpha.push_back(a);
+        b = std::sqrt(dotAll(w.data(), w.data(), n));
+        if (!(b > 1e-12 * std::abs(a))) break;
+        beta.push_back(b);
+        std::swap(vPrev, v);
+        for (i64 i = 0; i < n; i++) v[i] = w[i] / b;
+    }
+    beta.resize(alpha.size() - 1);
+    const double lambda = tridiagonalMax(alpha, beta);
+    L.omega = 1.2 / (lambda * 1.05);
+    L.lmax = lambda;
+}
+
+// native/src/util/dense.hpp:9-36
+static double dot4(const double* a, const double* b, i64 n) {
+    double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+    i64 k = 0;
+    for (; k + 4 <= n; k += 4) {
+        s0 += a[k] * b[k];
+        s1 += a[k + 1] * b[k + 1];
+        s2 += a[k + 2] * b[k + 2];
+        s3 += a[k + 3] * b[k + 3];
+    }
+    for (; k < n; k++) s0 += a[k] * b[k];
+    return (s0 + s1) + (s2 + s3);
+}
+static void choleskySolve(const double* L, double* y, i64 m) {
+    for (i64 i = 0; i < m; i++) y[i] = (y[i] - dot4(L + i * m, y, i)) / L[i * m + i];
+    for (i64 i = m - 1; i >= 0; i--) {
+        const double* row = L + i * m;
+        const double x = y[i] / row[i];
+        y[i] = x;
+        for (i64 k = 0; k < i; k++) y[k] -= row[k] * x;
+    }
+}
+
+// voxel_fea.cpp:612-661: dense free-DOF matrix of the coarsest level, right-looking Cholesky
+static void buildCoarseSolver(Coarse& co, const Level& L) {
+    co.map.assign(L.nDof, -1);
+    i64 m = 0;
+    for (i64 i = 0; i < L.nDof; i++)
+        if (!L.fixed[i]) co.map[i] = m++;
+    co.m = m;
+    co.A.clear();
+    if (m == 0 || m > 3000) return;
+    std::vector<double>& A = co.A;
+    A.assign(size_t(m * m), 0.0);
+    i64 dofs[24];
+    for (size_t e = 0; e < L.elems.size(); e++) {
+        const i64 n0 = L.base[e];
+        for (int a = 0; a < 8; a++)
+            for (int d = 0; d < 3; d++) dofs[3 * a + d] = co.map[3 * (n0 + L.off[a]) + d];
+        const bool sc = L.scaled(e);
+        const float* K = sc ? nullptr : L.matrix(e);
+        const double s = L.scale(e);
+        for (int r = 0; r < 24; r++) {
+            if (dofs[r] < 0) continue;
+            for (int c = 0; c < 24; c++)
+                if (dofs[c] >= 0) A[dofs[r] * m + dofs[c]] += sc ? s * L.Kb[r * 24 + c] : double(K[r * 24 + c]);
+        }
+    }
+    if (!L.diagAdd.empty())
+        for (i64 i = 0; i < L.nDof; i++)
+            if (co.map[i] >= 0) A[co.map[i] * m + co.map[i]] += L.diagAdd[i];
+    double maxDiag = 0;
+    for (i64 i = 0; i < m; i++) maxDiag = std::max(maxDiag, A[i * m + i]);
+    for (i64 j = 0; j < m; j++) {
+        double s = A[j * m + j];
+        if (!(s > 1e-10 * maxDiag)) s = maxDiag;
+        const double ljj = std::sqrt(s);
+        A[j * m + j] = ljj;
+        const double inv = 1 / ljj;
+        for (i64 i = j + 1; i < m; i++) A[i * m + j] *= inv;
+        for (i64 i = j + 1; i < m; i++) {
+            const double lij = A[i * m + j];
+            if (lij == 0) continue;
+            double* row = A.data() + i * m;
+            for (i64 k = j + 1; k <= i; k++) row[k] -= lij * A[k * m + j];
+        }
+    }
+    co.y.assign(m, 0.0);
+}
+
+// voxel_fea.cpp:692-701: degree-2 Chebyshev on [0.1, 1.15] lmax
+struct Cheb {
+    double first, c1, c2;
+};
+static Cheb chebCoefficients(double lmax) {
+    const double b = 1.15 * lmax, a = 0.1 * lmax;
+    const double theta = (b + a) / 2, delta = (b - a) / 2, sigma = theta / delta;
+    Cheb c;
+    c.first = 1 / theta;
+    const double rho0 = 1 / sigma, rho1 = 1 / (2 * sigma - rho0);
+    c.c1 = rho1 * rho0;
+    c.c2 = 2 * rho1 / delta;
+    return c;
+}
+
+// voxel_fea.cpp:739-779: rc = P^T rf, 27-point full weighting per coarse node
+template <class T> static void restrictT(const Level& F, const Level& C, const T* rf, T* rc, bool zeroFixed) {
+    const i64 CNXY = i64(C.NX) * C.NY;
+    auto body = [&](i64 lo, i64 hi) {
+        for (i64 cn = lo; cn < hi; cn++) {
+            const int I = int(cn % C.NX), J = int((cn / C.NX) % C.NY), K = int(cn / CNXY);
+            T s[3] = {0, 0, 0};
+            for (int dk = -1; dk <= 1; dk++) {
+                const int kf = 2 * K + dk;
+                if (kf < 0 || kf >= F.NZ) continue;
+                const T wk = dk == 0 ? 1 : 0.5;
+                for (int dj = -1; dj <= 1; dj++) {
+                    const int jf = 2 * J + dj;
+                    if (jf < 0 || jf >= F.NY) continue;
+                    const T wj = dj == 0 ? 1 : 0.5;
+                    for (int di = -1; di <= 1; di++) {
+                        const int fi = 2 * I + di;
+                        if (fi < 0 || fi >= F.NX) continue;
+                        const T w = T(di == 0 ? 1 : 0.5) * wj * wk;
+                        const T* p = rf + 3 * (fi + i64(F.NX) * (jf + i64(F.NY) * kf));
+                        for (int d = 0; d < 3; d++) s[d] += w * p[d];
+                    }
+                }
+            }
+            T* o = rc + 3 * cn;
+            for (int d = 0; d < 3; d++) o[d] = (zeroFixed && C.fixed[3 * cn + d]) ? T(0) : s[d];
+        }
+    };
+    if (zeroFixed) C.forActiveNodes(body);
+    else body(0, C.nNodes);
+}
+
+// voxel_fea.cpp:781-815: zf += P zc (trilinear) on the free fine DOFs
+template <class T> static void prolongAdd(const Level& F, const Level& C, const T* zc, T* zf) {
+    const i64 FNXY = i64(F.NX) * F.NY;
+    F.forActiveNodes([&](i64 lo, i64 hi) {
+        for (i64 fn = lo; fn < hi; fn++) {
+            const uint8_t* fx = F.fixed.data() + 3 * fn;
+            if (fx[0] && fx[1] && fx[2]) continue;
+            const int i = int(fn % F.NX), j = int((fn / F.NX) % F.NY), k = int(fn / FNXY);
+            const int ii[2] = {F.mx.c0[i], F.mx.c1[i]}, jj[2] = {F.my.c0[j], F.my.c1[j]}, kk[2] = {F.mz.c0[k], F.mz.c1[k]};
+            const T wi = ii[1] < 0 ? 1 : 0.5, wj = jj[1] < 0 ? 1 : 0.5, wk = kk[1] < 0 ? 1 : 0.5;
+            T s[3] = {0, 0, 0};
+            for (int c = 0; c < 2; c++) {
+                if (kk[c] < 0) continue;
+                for (int b = 0; b < 2; b++) {
+                    if (jj[b] < 0) continue;
+                    for (int a = 0; a < 2; a++) {
+                        if (ii[a] < 0) continue;
+                        const T w = wi * wj * wk;
+                        const T* p = zc + 3 * (ii[a] + i64(C.NX) * (jj[b] + i64(C.NY) * kk[c]));
+                        for (int d = 0; d < 3; d++) s[d] += w * p[d];
+                    }
+                }
+            }
+            T* o = zf + 3 * fn;
+            for (int d = 0; d < 3; d++)
+                if (!fx[d]) o[d] += s[d];
+        }
+    });
+}
+
+// ---------------------------------------------------------------------------------------------
+// the model (voxel_fea.cpp:138-183) and its multigrid cycle
+
+static void build(Model& M, int nx, int ny, int nz, const float* density, double nu, const uint8_t* bc, i64 coarsestMaxDof, const double* diagAdd) {
+    if (nx < 1 || ny < 1 || nz < 1) throw std::invalid_argument("The structural grid must have three positive dimensions.");
+    const i64 nE = i64(nx) * ny * nz;
+    for (i64 e = 0; e < nE; e++)
+        if (!std::isfinite(density[e]) || density[e] < 0 || density[e] > 1) throw std::invalid_argument("Voxel density must be between zero and one.");
+    M.nu = nu;
+    M.el = hexElement(nu);
+    Level L0 = makeLevel(nx, ny, nz);
+    setBase(L0, M.el.K);
+    L0.emap.assign(nE, -1);
+    for (int k = 0; k < nz; k++)
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++) {
+                const int e = i + nx * (j + ny * k);
+                if (density[e] > 0) {
+                    L0.emap[e] = int(L0.elems.size());
+                    L0.elems.push_back(e);
+                    L0.base.push_back(i + i64(L0.NX) * (j + i64(L0.NY) * k));
+                    L0.rho.push_back(density[e]);
+                }
+            }
+    L0.bc.assign(bc, bc + L0.nDof);
+    if (diagAdd) L0.diagAdd.assign(diagAdd, diagAdd + L0.nDof);
+    finishLevel(L0);
+    M.levels.push_back(std::move(L0));
+    for (;;) {
+        Level& L = M.levels.back();
+        if (L.freeDof <= coarsestMaxDof || M.levels.size() >= 12) break;
+        if (L.nx <= 1 && L.ny <= 1 && L.nz <= 1) break;
+        Level C = coarsen(L);
+        M.levels.push_back(std::move(C));
+    }
+    buildCoarseSolver(M.coarse, M.levels.back());
+}
+
+static void prepareSmoothers(Model& M) {
+    for (size_t l = 0; l + 1 < M.levels.size(); l++)
+        if (!(M.levels[l].lmax > 0)) estimateOmega(M.levels[l]);
+}
+
+// voxel_fea.cpp:727-736
+static void jacobi(Level& L, bool first) {
+    const float om = float(L.omega);
+    if (first) {
+        L.forActive([&](i64 lo, i64 hi) { for (i64 i = lo; i < hi; i++) L.z[i] = om * L.invDiag32[i] * L.r[i]; });
+        return;
+    }
+    applyT<float>(L, L.z.data(), L.t.data(), false);
+    L.forActive([&](i64 lo, i64 hi) { for (i64 i = lo; i < hi; i++) L.z[i] += om * L.invDiag32[i] * (L.r[i] - L.t[i]); });
+}
+
+// voxel_fea.cpp:674-688
+static void coarseSolve(Model& M, Level& L) {
+    std::fill(L.z.begin(), L.z.end(), 0.f);
+    Coarse& co = M.coarse;
+    if (co.A.empty()) {
+        for (int s = 0; s < 40; s++) jacobi(L, s == 0);
+        return;
+    }
+    for (i64 i = 0; i < L.nDof; i++)
+        if (co.map[i] >= 0) co.y[co.map[i]] = L.r[i];
+    choleskySolve(co.A.data(), co.y.data(), co.m);
+    for (i64 i = 0; i < L.nDof; i++)
+        if (co.map[i] >= 0) L.z[i] = float(co.y[co.map[i]]);
+}
+
+// voxel_fea.cpp:663-672
+static void coarseSolve64(Model& M, const double* r, double* z) {
+    const Level& L = M.levels.back();
+    const Coarse& co = M.coarse;
+    std::vector<double> y(co.m);
+    for (i64 i = 0; i < L.nDof; i++)
+        if (co.map[i] >= 0) y[co.map[i]] = r[i];
+    choleskySolve(co.A.data(), y.data(), co.m);
+    for (i64 i = 0; i < L.nDof; i++) z[i] = co.map[i] >= 0 ? y[co.map[i]] : 0;
+}
+
+// voxel_fea.cpp:703-725
+static void chebyshev(Level& L, bool first) {
+    const Cheb cc = chebCoefficients(L.lmax);
+    const float c0 = float(cc.first), c1 = float(cc.c1), c2 = float(cc.c2);
+    const float* D = L.invDiag32.data();
+    float *r = L.r.data(), *z = L.z.data(), *t = L.t.data(), *d = L.d.data();
+    if (first) {
+        L.forActive([&](i64 lo, i64 hi) {
+            for (i64 i = lo; i < hi; i++) { d[i] = c0 * D[i] * r[i]; z[i] = d[i]; }
+        });
+    } else {
+        applyT<float>(L, z, t, false);
+        L.forActive([&](i64 lo, i64 hi) {
+            for (i64 i = lo; i < hi; i++) { d[i] = c0 * D[i] * (r[i] - t[i]); z[i] += d[i]; }
+        });
+    }
+    applyT<float>(L, z, t, false);
+    L.forActive([&](i64 lo, i64 hi) {
+        for (i64 i = lo; i < hi; i++) { d[i] = c1 * d[i] + c2 * D[i] * (r[i] - t[i]); z[i] += d[i]; }
+    });
+}
+
+// voxel_fea.cpp:817-828
+static void vcycle(Model& M, size_t l) {
+    Level& L = M.levels[l];
+    if (l + 1 == M.levels.size()) { coarseSolve(M, L); return; }
+    Level& C = M.levels[l + 1];
+    chebyshev(L, true);
+    applyT<float>(L, L.z.data(), L.t.data(), false);
+    L.forActive([&](i64 lo, i64 hi) { for (i64 i = lo; i < hi; i++) L.t[i] = L.r[i] - L.t[i]; });
+    restrictT<float>(L, C, L.t.data(), C.r.data(), true);
+    vcycle(M, l + 1);
+    prolongAdd<float>(L, C, C.z.data(), L.z.data());
+    chebyshev(L, false);
+}
+
+// voxel_fea.cpp:835-861
+static void precondition32(Model& M, const double* r, double* z) {
+    Level& L = M.levels[0];
+    const i64 n = L.nDof;
+    if (M.levels.size() == 1 && !M.coarse.A.empty()) { coarseSolve64(M, r, z); return; }
+    double rmax = 0;
+    L.forActive([&](i64 lo, i64 hi) {
+        for (i64 i = lo; i < hi; i++)
+            if (!L.fixed[i]) rmax = std::max(rmax, std::abs(r[i]));
+    });
+    if (!(rmax > 0)) { std::fill(z, z + n, 0.0); return; }
+    const double s = 1 / rmax;
+    L.forActive([&](i64 lo, i64 hi) { for (i64 i = lo; i < hi; i++) L.r[i] = L.fixed[i] ? 0.f : float(r[i] * s); });
+    vcycle(M, 0);
+    std::fill(z, z + n, 0.0);
+    L.forActive([&](i64 lo, i64 hi) { for (i64 i = lo; i < hi; i++) z[i] = double(L.z[i]) * rmax; });
+}
+
+struct SolveOut {
+    int iterations = 0;
+    double residual = 0;
+    bool converged = false;
+};
+
+// voxel_fea.cpp:863-927: flexible (Polak-Ribiere) PCG in 64-bit, one V-cycle per iteration
+static SolveOut solve(Model& M, const double* f, const double* x0, double tol, int maxIter, double* x) {
+    Level& L = M.levels[0];
+    const i64 n = L.nDof;
+    for (i64 i = 0; i < n; i++)
+        if (!std::isfinite(f[i]) || (x0 && !std::isfinite(x0[i]))) throw std::invalid_argument("Forces and displacements must be finite numbers.");
+    SolveOut res;
+    for (i64 i = 0; i < n; i++) x[i] = x0 ? x0[i] : 0.0;
+    const auto& fixed = L.fixed;
+    for (i64 i = 0; i < n; i++)
+        if (fixed[i]) x[i] = 0;
+    std::vector<double> r(n), p(n), q(n), z(n);
+    double bnorm = 0;
+    for (i64 i = 0; i < n; i++)
+        if (!fixed[i]) bnorm += f[i] * f[i];
+    bnorm = std::sqrt(bnorm);
+    if (bnorm == 0) {
+        std::fill(x, x + n, 0.0);
+        res.converged = true;
+        return res;
+    }
+    applyT<double>(L, x, q.data(), false);
+    for (i64 i = 0; i < n; i++) r[i] = fixed[i] ? 0 : f[i] - q[i];
+    prepareSmoothers(M);
+    auto dotA = [&](const std::vector<double>& a, const std::vector<double>& b) {
+        return L.sumActive([&](i64 lo, i64 hi) {
+            double s = 0;
+            for (i64 i = lo; i < hi; i++) s += a[i] * b[i];
+            return s;
+        });
+    };
+    precondition32(M, r.data(), z.data());
+    p = z;
+    double rz = dotA(r, z);
+    double rel = std::sqrt(dotA(r, r)) / bnorm;
+    int it = 0;
+    for (; it < maxIter && rel > tol; it++) {
+        applyT<double>(L, p.data(), q.data(), false);
+        const double pq = dotA(p, q);
+        if (!(pq > 0)) break;
+        const double alpha = rz / pq;
+        L.forActive([&](i64 lo, i64 hi) {
+            for (i64 i = lo; i < hi; i++) { x[i] += alpha * p[i]; r[i] -= alpha * q[i]; }
+        });
+        rel = std::sqrt(dotA(r, r)) / bnorm;
+        if (rel <= tol) { it++; break; }
+        precondition32(M, r.data(), z.data());
+        const double rzNew = dotA(r, z);
+        const double beta = std::max(0.0, -alpha * dotA(z, q) / rz);
+        rz = rzNew;
+        L.forActive([&](i64 lo, i64 hi) { for (i64 i = lo; i < hi; i++) p[i] = z[i] + beta * p[i]; });
+    }
+    applyT<double>(L, x, q.data(), false);
+    for (i64 i = 0; i < n; i++) r[i] = fixed[i] ? 0 : f[i] - q[i];
+    res.residual = std::sqrt(dotAll(r.data(), r.data(), n)) / bnorm;
+    res.iterations = it;
+    res.converged = std::isfinite(res.residual) && res.residual <= tol * 10;
+    return res;
+}
+
+// voxel_fea.cpp:929-940
+static void reactions(const Model& M, const double* u, const double* f, double* R) {
+    const Level& L = M.levels[0];
+    std::vector<double> y(L.nDof);
+    applyT<double>(L, u, y.data(), true);
+    R[0] = R[1] = R[2] = 0;
+    for (i64 n = 0; n < L.nNodes; n++)
+        for (int d = 0; d < 3; d++) {
+            const i64 i = 3 * n + d;
+            if (L.bc[i] && L.activeNode[n]) R[d] += y[i] - (f ? f[i] : 0.0);
+        }
+}
+
+// voxel_fea.cpp:942-945
+static double vonMises(const double* s) {
+    return std::sqrt(0.5 * ((s[0] - s[1]) * (s[0] - s[1]) + (s[1] - s[2]) * (s[1] - s[2]) + (s[2] - s[0]) * (s[2] - s[0])) +
+                     3 * (s[3] * s[3] + s[4] * s[4] + s[5] * s[5]));
+}
+
+// voxel_fea.cpp:947-966: closed-form eigenvalues of a symmetric 3x3 (trigonometric form)
+static void principalStresses(double sx, double sy, double sz, double txy, double tyz, double tzx, double out[3]) {
+    const double p1 = txy * txy + tyz * tyz + tzx * tzx;
+    const double q = (sx + sy + sz) / 3;
+    const double scale = std::abs(sx) + std::abs(sy) + std::abs(sz) + std::sqrt(p1);
+    if (p1 <= 1e-24 * scale * scale) {
+        double v[3] = {sx, sy, sz};
+        std::sort(v, v + 3, [](double a, double b) { return a > b; });
+        out[0] = v[0]; out[1] = v[1]; out[2] = v[2];
+        return;
+    }
+    const double p2 = (sx - q) * (sx - q) + (sy - q) * (sy - q) + (sz - q) * (sz - q) + 2 * p1;
+    const double p = std::sqrt(p2 / 6);
+    const double b11 = (sx - q) / p, b22 = (sy - q) / p, b33 = (sz - q) / p, b12 = txy / p, b23 = tyz / p, b13 = tzx / p;
+    const double det = b11 * (b22 * b33 - b23 * b23) - b12 * (b12 * b33 - b23 * b13) + b13 * (b12 * b23 - b22 * b13);
+    const double rr = std::min(1.0, std::max(-1.0, det / 2));
+    const double phi = std::acos(rr) / 3;
+    out[0] = q + 2 * p * std::cos(phi);
+    out[2] = q + 2 * p * std::cos(phi + 2 * M_PI / 3);
+    out[1] = 3 * q - out[0] - out[2];
+}
+
+// voxel_fea.cpp:968-1037: nodal von Mises / P1 / P3, density-weighted mean of corner stresses
+static void stresses(const Model& M, const double* x, double E, double h, bool principals, float* vmOut, float* p1Out, float* p3Out) {
+    const Level& L = M.levels[0];
+    std::vector<double> ST(8 * 24 * 8, 0.0);
+    for (int a = 0; a < 8; a++)
+        for (int r = 0; r < 6; r++)
+            for (int q = 0; q < 24; q++) ST[(a * 24 + q) * 8 + r] = M.el.corner[a][r * 24 + q] * E / h;
+    std::fill(vmOut, vmOut + L.nNodes, 0.f);
+    if (principals) {
+        std::fill(p1Out, p1Out + L.nNodes, 0.f);
+        std::fill(p3Out, p3Out + L.nNodes, 0.f);
+    }
+    const int NX = L.NX, nx = L.nx, ny = L.ny;
+    const i64 NXY = i64(NX) * L.NY;
+    L.forActiveNodes([&](i64 lo, i64 hi) {
+        double sig[6], pr[3];
+        for (i64 n = lo; n < hi; n++) {
+            if (!L.activeNode[n]) continue;
+            const int i = int(n % NX), j = int((n / NX) % L.NY), k = int(n / NXY);
+            double vm = 0, p1 = 0, p3 = 0, w = 0;
+            for (int v = 0; v < 8; v++) {
+                const int di = v & 1, dj = (v >> 1) & 1, dk = v >> 2;
+                const int ei = i + di - 1, ej = j + dj - 1, ek = k + dk - 1;
+                if (ei < 0 || ej < 0 || ek < 0 || ei >= nx || ej >= ny || ek >= L.nz) continue;
+                const int e = L.emap[ei + nx * (ej + i64(ny) * ek)];
+                if (e < 0) continue;
+                const int a = cornerOf(1 - di, 1 - dj, 1 - dk);
+                const double* col = ST.data() + a * 24 * 8;
+                double s0[4] = {0, 0, 0, 0}, s1[4] = {0, 0, 0, 0};
+                for (int f = 0; f < 4; f++) {
+                    const int by = f & 1, bz = f >> 1;
+                    const double* xs = x + 3 * (n - (1 - di) + NX * (by - (1 - dj)) + NXY * (bz - (1 - dk)));
+                    const int q0 = 3 * cornerOf(0, by, bz), q1 = 3 * cornerOf(1, by, bz);
+                    for (int d = 0; d < 3; d++) {
+                        const double* c0 = col + (q0 + d) * 8;
+                        const double* c1 = col + (q1 + d) * 8;
+                        for (int l = 0; l < 4; l++) s0[l] += c0[l] * xs[d];
+                        for (int l = 0; l < 4; l++) s1[l] += c0[4 + l] * xs[d];
+                        for (int l = 0; l < 4; l++) s0[l] += c1[l] * xs[3 + d];
+                        for (int l = 0; l < 4; l++) s1[l] += c1[4 + l] * xs[3 + d];
+                    }
+                }
+                for (int r = 0; r < 4; r++) sig[r] = s0[r];
+                sig[4] = s1[0];
+                sig[5] = s1[1];
+                const double rho = L.rho[e];
+                vm += rho * vonMises(sig);
+                if (principals) {
+                    principalStresses(sig[0], sig[1], sig[2], sig[3], sig[4], sig[5], pr);
+                    p1 += rho * pr[0];
+                    p3 += rho * pr[2];
+                }
+                w += rho;
+            }
+            if (w > 0) {
+                vmOut[n] = float(vm / w);
+                if (principals) {
+                    p1Out[n] = float(p1 / w);
+                    p3Out[n] = float(p3 / w);
+                }
+            }
+        }
+    });
+}
+
+// voxel_fea.cpp:1039-1084
+static void stressTensors(const Model& M, const double* u, double E, double h, float* out) {
+    const Level& L = M.levels[0];
+    std::vector<double> S(8 * 144);
+    for (int a = 0; a < 8; a++)
+        for (int i = 0; i < 144; i++) S[a * 144 + i] = M.el.corner[a][i] * E / h;
+    std::fill(out, out + 6 * L.nNodes, 0.f);
+    const int NX = L.NX, NY = L.NY, nx = L.nx, ny = L.ny, nz = L.nz;
+    const i64 NXY = i64(NX) * NY;
+    double ue[24], acc[6];
+    for (i64 n = 0; n < L.nNodes; n++) {
+        if (!L.activeNode[n]) continue;
+        const int i = int(n % NX), j = int((n / NX) % NY), k = int(n / NXY);
+        double w = 0;
+        std::fill(acc, acc + 6, 0.0);
+        for (int dk = 0; dk < 2; dk++) {
+            const int ek = k + dk - 1;
+            if (ek < 0 || ek >= nz) continue;
+            for (int dj = 0; dj < 2; dj++) {
+                const int ej = j + dj - 1;
+                if (ej < 0 || ej >= ny) continue;
+                for (int di = 0; di < 2; di++) {
+                    const int ei = i + di - 1;
+                    if (ei < 0 || ei >= nx) continue;
+                    const int e = L.emap[ei + nx * (ej + ny * ek)];
+                    if (e < 0) continue;
+                    const int a = cornerOf(1 - di, 1 - dj, 1 - dk);
+                    const i64 nb = ei + NX * (ej + i64(NY) * ek);
+                    for (int c = 0; c < 8; c++)
+                        for (int d = 0; d < 3; d++) ue[3 * c + d] = u[3 * (nb + L.off[c]) + d];
+                    const double rho = L.rho[e];
+                    for (int r = 0; r < 6; r++) {
+                        double s = 0;
+                        for (int q = 0; q < 24; q++) s += S[a * 144 + r * 24 + q] * ue[q];
+                        acc[r] += rho * s;
+                    }
+                    w += rho;
+                }
+            }
+        }
+        if (w > 0)
+            for (int r = 0; r < 6; r++) out[6 * n + r] = float(acc[r] / w);
+    }
+}
+
+// voxel_fea.cpp:1086-1119: keep the voxels face-connected to one touching a held node
+static int pruneFloating(int nx, int ny, int nz, const float* density, const uint8_t* held, float* out) {
+    const i64 NX = nx + 1, NY = ny + 1;
+    const i64 nE = i64(nx) * ny * nz;
+    std::vector<uint8_t> seen(nE, 0);
+    std::vector<i64> queue;
+    queue.reserve(nE);
+    const i64 off[8] = {0, 1, 1 + NX, NX, NX * NY, 1 + NX * NY, 1 + NX + NX * NY, NX + NX * NY};
+    for (int k = 0; k < nz; k++)
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++) {
+                const i64 e = i + i64(nx) * (j + i64(ny) * k);
+                if (!(density[e] > 0)) continue;
+                const i64 n0 = i + NX * (j + NY * k);
+                for (int a = 0; a < 8; a++)
+                    if (held[n0 + off[a]]) { seen[e] = 1; queue.push_back(e); break; }
+            }
+    for (size_t h = 0; h < queue.size(); h++) {
+        const i64 e = queue[h];
+        const int i = int(e % nx), j = int((e / nx) % ny), k = int(e / (i64(nx) * ny));
+        const i64 nb[6] = {i > 0 ? e - 1 : -1, i < nx - 1 ? e + 1 : -1, j > 0 ? e - nx : -1, j < ny - 1 ? e + nx : -1,
+                           k > 0 ? e - i64(nx) * ny : -1, k < nz - 1 ? e + i64(nx) * ny : -1};
+        for (i64 m : nb)
+            if (m >= 0 && !seen[m] && density[m] > 0) { seen[m] = 1; queue.push_back(m); }
+    }
+    int removed = 0;
+    for (i64 e = 0; e < nE; e++) {
+        out[e] = 0.f;
+        if (density[e] > 0) {
+            if (seen[e]) out[e] = density[e];
+            else removed++;
+        }
+    }
+    return removed;
+}
+
+}  // namespace orc
+
+// ---------------------------------------------------------------------------------------------
+// C ABI for the tests (ctypes). Errors return -1 / nullptr and set orc_last_error().
+
+static std::string g_err;
+
+extern "C" {
+
+const char* orc_last_error() { return g_err.c_str(); }
+
+void orc_set_threads(int n) { orc::g_threads = n; }
+
+int orc_hex_element(double nu, double* K, double* corner, double* gauss) {
+    try {
+        const orc::Hex h = orc::hexElement(nu);
+        std::memcpy(K, h.K, sizeof h.K);
+        if (corner) std::memcpy(corner, h.corner, sizeof h.corner);
+        if (gauss) std::memcpy(gauss, h.gauss, sizeof h.gauss);
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
+void* orc_create(int nx, int ny, int nz, const float* density, double nu, const uint8_t* bc, int64_t coarsestMaxDof, const double* diagAdd) {
+    try {
+        auto M = std::make_unique<orc::Model>();
+        orc::build(*M, nx, ny, nz, density, nu, bc, coarsestMaxDof, diagAdd);
+        return M.release();
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return nullptr;
+    }
+}
+
+void orc_destroy(void* h) { delete static_cast<orc::Model*>(h); }
+
+int orc_num_levels(void* h) { return int(static_cast<orc::Model*>(h)->levels.size()); }
+
+// dims[6] = nx ny nz NX NY NZ; info[4] = nNodes nDof freeDof nOwn
+void orc_level_info(void* h, int l, int* dims, int64_t* info, double* lmax) {
+    const orc::Level& L = static_cast<orc::Model*>(h)->levels[l];
+    const int d[6] = {L.nx, L.ny, L.nz, L.NX, L.NY, L.NZ};
+    std::memcpy(dims, d, sizeof d);
+    info[0] = L.nNodes;
+    info[1] = L.nDof;
+    info[2] = L.freeDof;
+    info[3] = int64_t(L.K.size() / 576);
+    *lmax = L.lmax;
+}
+
+// fixed[nDof], invDiag[nDof], nodeScale[nNodes], voxelScale[nVox] (any may be null)
+void orc_level_arrays(void* h, int l, uint8_t* fixed, double* invDiag, float* nodeScale, float* voxelScale) {
+    const orc::Level& L = static_cast<orc::Model*>(h)->levels[l];
+    if (fixed) std::copy(L.fixed.begin(), L.fixed.end(), fixed);
+    if (invDiag) std::copy(L.invDiag.begin(), L.invDiag.end(), invDiag);
+    if (nodeScale) std::copy(L.nodeScale.begin(), L.nodeScale.end(), nodeScale);
+    if (voxelScale) std::copy(L.voxelScale.begin(), L.voxelScale.end(), voxelScale);
+}
+
+void orc_level_kb(void* h, int l, double* Kb576, double* Sb324) {
+    const orc::Level& L = static_cast<orc::Model*>(h)->levels[l];
+    if (Kb576) std::memcpy(Kb576, L.Kb, 576 * sizeof(double));
+    if (Sb324) std::memcpy(Sb324, L.Sb, 324 * sizeof(double));
+}
+
+void orc_apply(void* h, int l, const double* x, double* y, int raw) {
+    orc::applyT<double>(static_cast<orc::Model*>(h)->levels[l], x, y, raw != 0);
+}
+
+void orc_apply32(void* h, int l, const float* x, float* y, int raw) {
+    orc::applyT<float>(static_cast<orc::Model*>(h)->levels[l], x, y, raw != 0);
+}
+
+// P^T applied to a fine vector (zeroFixed as in the V-cycle) and P applied to a coarse vector
+void orc_restrict(void* h, int l, const double* rf, double* rc, int zeroFixed) {
+    auto* M = static_cast<orc::Model*>(h);
+    orc::restrictT<double>(M->levels[l], M->levels[l + 1], rf, rc, zeroFixed != 0);
+}
+void orc_prolong_add(void* h, int l, const double* zc, double* zf) {
+    auto* M = static_cast<orc::Model*>(h);
+    orc::prolongAdd<double>(M->levels[l], M->levels[l + 1], zc, zf);
+}
+
+void orc_prepare_smoothers(void* h) { orc::prepareSmoothers(*static_cast<orc::Model*>(h)); }
+
+int64_t orc_coarse_m(void* h) { return static_cast<orc::Model*>(h)->coarse.m; }
+
+void orc_coarse_solve64(void* h, const double* r, double* z) { orc::coarseSolve64(*static_cast<orc::Model*>(h), r, z); }
+
+void orc_precondition(void* h, const double* r, double* z) {
+    auto* M = static_cast<orc::Model*>(h);
+    orc::prepareSmoothers(*M);
+    orc::precondition32(*M, r, z);
+}
+
+// returns 1 converged, 0 not, -1 error; out: iterations, residual
+int orc_solve(void* h, const double* f, const double* x0, double tol, int maxIter, double* u, int* iterations, double* residual) {
+    try {
+        const auto r = orc::solve(*static_cast<orc::Model*>(h), f, x0, tol, maxIter, u);
+        *iterations = r.iterations;
+        *residual = r.residual;
+        return r.converged ? 1 : 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
+void orc_reactions(void* h, const double* u, const double* f, double* R) { orc::reactions(*static_cast<orc::Model*>(h), u, f, R); }
+
+void orc_stresses(void* h, const double* u, double E, double hvox, int principals, float* vm, float* p1, float* p3) {
+    orc::stresses(*static_cast<orc::Model*>(h), u, E, hvox, principals != 0, vm, p1, p3);
+}
+
+void orc_stress_tensors(void* h, const double* u, double E, double hvox, float* out) {
+    orc::stressTensors(*static_cast<orc::Model*>(h), u, E, hvox, out);
+}
+
+void orc_principal(double sx, double sy, double sz, double txy, double tyz, double tzx, double* out) {
+    orc::principalStresses(sx, sy, sz, txy, tyz, tzx, out);
+}
+
+double orc_von_mises(const double* s) { return orc::vonMises(s); }
+
+int orc_prune_floating(int nx, int ny, int nz, const float* density, const uint8_t* held, float* out) {
+    return orc::pruneFloating(nx, ny, nz, density, held, out);
+}
+
+double orc_tridiagonal_max(const double* a, int k, const double* b) {
+    return orc::tridiagonalMax(std::vector<double>(a, a + k), std::vector<double>(b, b + std::max(0, k - 1)));
+}
+
+void orc_cheb_coefficients(double lmax, double* out3) {
+    const auto c = orc::chebCoefficients(lmax);
+    out3[0] = c.first;
+    out3[1] = c.c1;
+    out3[2] = c.c2;
+}
+
+}  // extern "C"

@@ -1,0 +1,210 @@
+// ORACLE — test infrastructure only, never the product.
+//
+// C ABI over the reference's own VoxelFEA, compiled from the reference sources where they lie
+// (oracle/Makefile: native/src/fea/hex8.cpp, native/src/fea/voxel_fea.cpp, native/src/util/parallel.cpp)
+// into oracle/_ref/libpsref.so. The tests use it to pin the restatement in oracle/oracle.cpp bit for
+// bit, and bench.py may time it as the CPU baseline. The entry points have the same shapes as
+// oracle.cpp's orc_* functions.
+#include <algorithm>
+#include <array>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+// the member templates the tests call (applyT<float>, restrict, prolongAdd) are private: open them
+// for this test harness only, and instantiate them in this translation unit
+#define private public
+#include "fea/voxel_fea.hpp"
+#undef private
+#include "fea/voxel_fea.cpp"
+
+static std::string g_err;
+
+extern "C" {
+
+const char* ref_last_error() { return g_err.c_str(); }
+
+int ref_hex_element(double nu, double* K, double* corner, double* gauss) {
+    try {
+        const auto h = ps::hexElement(nu);
+        std::memcpy(K, h.K.data(), 576 * sizeof(double));
+        for (int a = 0; a < 8; a++) {
+            if (corner) std::memcpy(corner + 144 * a, h.cornerStress[a].data(), 144 * sizeof(double));
+            if (gauss) std::memcpy(gauss + 144 * a, h.gaussB[a].data(), 144 * sizeof(double));
+        }
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
+void* ref_create(int nx, int ny, int nz, const float* density, double nu, const uint8_t* bc, int64_t coarsestMaxDof, const double* diagAdd) {
+    try {
+        const int64_t nE = int64_t(nx) * ny * nz, nD = 3 * int64_t(nx + 1) * (ny + 1) * (nz + 1);
+        std::vector<float> dens(density, density + nE);
+        std::vector<uint8_t> b(bc, bc + nD);
+        std::vector<double> da;
+        if (diagAdd) da.assign(diagAdd, diagAdd + nD);
+        return new ps::VoxelFEA({nx, ny, nz}, dens, nu, b, coarsestMaxDof, diagAdd ? &da : nullptr);
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return nullptr;
+    }
+}
+
+void ref_destroy(void* h) { delete static_cast<ps::VoxelFEA*>(h); }
+
+int ref_num_levels(void* h) { return int(static_cast<ps::VoxelFEA*>(h)->levels.size()); }
+
+void ref_level_info(void* h, int l, int* dims, int64_t* info, double* lmax) {
+    const ps::Level& L = static_cast<ps::VoxelFEA*>(h)->levels[l];
+    const int d[6] = {L.nx, L.ny, L.nz, L.NX, L.NY, L.NZ};
+    std::memcpy(dims, d, sizeof d);
+    info[0] = L.nNodes;
+    info[1] = L.nDof;
+    info[2] = L.freeDof;
+    info[3] = int64_t(L.K.size() / 576);
+    *lmax = L.lmax;
+}
+
+void ref_level_arrays(void* h, int l, uint8_t* fixed, double* invDiag, float* nodeScale, float* voxelScale) {
+    const ps::Level& L = static_cast<ps::VoxelFEA*>(h)->levels[l];
+    if (fixed) std::copy(L.fixed.begin(), L.fixed.end(), fixed);
+    if (invDiag) std::copy(L.invDiag.begin(), L.invDiag.end(), invDiag);
+    if (nodeScale) std::copy(L.nodeScale.begin(), L.nodeScale.end(), nodeScale);
+    if (voxelScale) std::copy(L.voxelScale.begin(), L.voxelScale.end(), voxelScale);
+}
+
+void ref_level_kb(void* h, int l, double* Kb576, double* Sb324) {
+    const ps::Level& L = static_cast<ps::VoxelFEA*>(h)->levels[l];
+    if (Kb576) std::memcpy(Kb576, L.Kb.data(), 576 * sizeof(double));
+    if (Sb324) std::memcpy(Sb324, L.Sb.data(), 324 * sizeof(double));
+}
+
+void ref_apply(void* h, int l, const double* x, double* y, int raw) { static_cast<ps::VoxelFEA*>(h)->apply(l, x, y, raw != 0); }
+
+void ref_apply32(void* h, int l, const float* x, float* y, int raw) { static_cast<ps::VoxelFEA*>(h)->applyT<float>(l, x, y, raw != 0); }
+
+void ref_restrict(void* h, int l, const double* rf, double* rc, int zeroFixed) {
+    auto* M = static_cast<ps::VoxelFEA*>(h);
+    M->restrict<double>(M->levels[l], M->levels[l + 1], rf, rc, zeroFixed != 0);
+}
+
+void ref_prolong_add(void* h, int l, const double* zc, double* zf) {
+    auto* M = static_cast<ps::VoxelFEA*>(h);
+    M->prolongAdd<double>(M->levels[l], M->levels[l + 1], zc, zf);
+}
+
+void ref_prepare_smoothers(void* h) { static_cast<ps::VoxelFEA*>(h)->prepareSmoothers(); }
+
+int64_t ref_coarse_m(void* h) { return static_cast<ps::VoxelFEA*>(h)->coarse.m; }
+
+void ref_coarse_solve64(void* h, const double* r, double* z) { static_cast<ps::VoxelFEA*>(h)->coarseSolve64(r, z); }
+
+void ref_precondition(void* h, const double* r, double* z) { static_cast<ps::VoxelFEA*>(h)->precondition(r, z); }
+
+int ref_solve(void* h, const double* f, const double* x0, double tol, int maxIter, double* u, int* iterations, double* residual) {
+    try {
+        auto* M = static_cast<ps::VoxelFEA*>(h);
+        const int64_t n = M->nDof();
+        std::vector<double> fv(f, f + n), xv;
+        ps::SolveOptions o;
+        o.tol = tol;
+        o.maxIter = maxIter;
+        if (x0) {
+            xv.assign(x0, x0 + n);
+            o.x0 = &xv;
+        }
+        const auto r = M->solve(fv, o);
+        std::copy(r.u.begin(), r.u.end(), u);
+        *iterations = r.iterations;
+        *residual = r.residual;
+        return r.converged ? 1 : 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
+void ref_reactions(void* h, const double* u, const double* f, double* R) {
+    auto* M = static_cast<ps::VoxelFEA*>(h);
+    const int64_t n = M->nDof();
+    std::vector<double> uv(u, u + n), fv;
+    if (f) fv.assign(f, f + n);
+    const auto r = M->reactions(uv, f ? &fv : nullptr);
+    R[0] = r[0]; R[1] = r[1]; R[2] = r[2];
+}
+
+void ref_stresses(void* h, const double* u, double E, double hvox, int principals, float* vm, float* p1, float* p3) {
+    auto* M = static_cast<ps::VoxelFEA*>(h);
+    std::vector<double> uv(u, u + M->nDof());
+    const auto s = M->stresses(uv, E, hvox, principals != 0);
+    std::copy(s.nodeVM.begin(), s.nodeVM.end(), vm);
+    if (principals) {
+        std::copy(s.nodeP1.begin(), s.nodeP1.end(), p1);
+        std::copy(s.nodeP3.begin(), s.nodeP3.end(), p3);
+    }
+}
+
+void ref_stress_tensors(void* h, const double* u, double E, double hvox, float* out) {
+    auto* M = static_cast<ps::VoxelFEA*>(h);
+    std::vector<double> uv(u, u + M->nDof());
+    const auto s = M->stressTensors(uv, E, hvox);
+    std::copy(s.begin(), s.end(), out);
+}
+
+void ref_principal(double sx, double sy, double sz, double txy, double tyz, double tzx, double* out) { ps::principalStresses(sx, sy, sz, txy, tyz, tzx, out); }
+
+double ref_von_mises(const double* s) { return ps::vonMises(s); }
+
+int ref_prune_floating(int nx, int ny, int nz, const float* density, const uint8_t* held, float* out) {
+    const int64_t nE = int64_t(nx) * ny * nz, nN = int64_t(nx + 1) * (ny + 1) * (nz + 1);
+    const auto r = ps::pruneFloating({nx, ny, nz}, std::vector<float>(density, density + nE), std::vector<uint8_t>(held, held + nN));
+    std::copy(r.density.begin(), r.density.end(), out);
+    return r.removed;
+}
+
+double ref_tridiagonal_max(const double* a, int k, const double* b) {
+    return ps::tridiagonalMax(std::vector<double>(a, a + k), std::vector<double>(b, b + std::max(0, k - 1)));
+}
+
+void ref_cheb_coefficients(double lmax, double* out3) {
+    const auto c = ps::chebyshevCoefficients(lmax);
+    out3[0] = c.first;
+    out3[1] = c.c1;
+    out3[2] = c.c2;
+}
+
+// wall time of the reference's own multithreaded solve (bench.py's CPU baseline): build + solve
+double ref_timed_solve(int nx, int ny, int nz, const float* density, double nu, const uint8_t* bc, int64_t coarsestMaxDof, const double* f, double tol,
+                       int maxIter, double* u, int* iterations, double* residual, double* buildSeconds) {
+    try {
+        const auto t0 = std::chrono::steady_clock::now();
+        void* h = ref_create(nx, ny, nz, density, nu, bc, coarsestMaxDof, nullptr);
+        if (!h) return -1;
+        const auto t1 = std::chrono::steady_clock::now();
+        const int ok = ref_solve(h, f, nullptr, tol, maxIter, u, iterations, residual);
+        const auto t2 = std::chrono::steady_clock::now();
+        ref_destroy(h);
+        if (ok < 0) return -1;
+        *buildSeconds = std::chrono::duration<double>(t1 - t0).count();
+        return std::chrono::duration<double>(t2 - t1).count();
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
+unsigned ref_threads() { return ps::ThreadPool::instance().size(); }
+
+}  // extern "C"

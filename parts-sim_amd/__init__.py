@@ -1,0 +1,327 @@
+"""MI355X-native voxel FEA linear-static solve: Python binding of the C ABI in include/psx.h.
+
+The compute path is libpsx.so (HIP kernels for gfx950 + host orchestration). This module only
+marshals arrays across the ABI; there is no Python or CPU fallback: without the built library every
+call raises. Class and argument names follow the reference's solver boundary
+(native/src/gpu/gpu_fea.hpp:17-35, native/src/fea/voxel_fea.hpp:112-189).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpsx.so")
+
+PREC = {"fp64": 0, "ref": 1, "mixed": 2}
+OK, ERR_ARGUMENT, ERR_NO_DEVICE, ERR_NO_COARSE, ERR_DEVICE, ERR_INTERNAL = range(6)
+
+
+class PsxError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("psx error %d: %s" % (code, msg))
+        self.code = code
+
+
+class ModelDesc(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("density", C.c_void_p), ("nu", C.c_double),
+                ("bc", C.c_void_p), ("diag_add", C.c_void_p), ("coarsest_max_dof", C.c_int64), ("max_levels", C.c_int)]
+
+
+PROGRESS = C.CFUNCTYPE(C.c_int, C.c_int, C.c_double, C.c_void_p)
+
+
+class SolveOpts(C.Structure):
+    _fields_ = [("tol", C.c_double), ("max_iter", C.c_int), ("precision", C.c_int), ("check_every", C.c_int),
+                ("fold_every", C.c_int), ("on_progress", PROGRESS), ("user", C.c_void_p)]
+
+
+class SolveInfo(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("residual", C.c_double), ("converged", C.c_int), ("cancelled", C.c_int),
+                ("broken", C.c_int), ("folds", C.c_int), ("device_ms", C.c_double)]
+
+
+class LevelInfo(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("n_nodes", C.c_int64), ("n_dof", C.c_int64),
+                ("free_dof", C.c_int64), ("n_elems", C.c_int64), ("n_own", C.c_int64), ("lmax", C.c_double)]
+
+
+# every entry point declared in include/psx.h: name -> (restype, argtypes)
+_vp, _i, _d, _i64 = C.c_void_p, C.c_int, C.c_double, C.c_int64
+SIGNATURES = {
+    "psx_last_error": (C.c_char_p, []),
+    "psx_available": (_i, []),
+    "psx_create": (_i, [C.POINTER(ModelDesc), C.POINTER(_vp)]),
+    "psx_destroy": (None, [_vp]),
+    "psx_num_levels": (_i, [_vp]),
+    "psx_get_level_info": (_i, [_vp, _i, C.POINTER(LevelInfo)]),
+    "psx_estimate_smoothers": (_i, [_vp]),
+    "psx_set_lmax": (_i, [_vp, _i, _d]),
+    "psx_apply": (_i, [_vp, _i, _vp, _vp, _i]),
+    "psx_apply32": (_i, [_vp, _i, _vp, _vp, _i]),
+    "psx_precondition": (_i, [_vp, _vp, _vp]),
+    "psx_precondition_device": (_i, [_vp, _vp, _vp]),
+    "psx_solve": (_i, [_vp, _vp, _vp, C.POINTER(SolveOpts), _vp, C.POINTER(SolveInfo)]),
+    "psx_solve_device": (_i, [_vp, _vp, _vp, C.POINTER(SolveOpts), _vp, C.POINTER(SolveInfo)]),
+    "psx_reactions": (_i, [_vp, _vp, _vp, _vp]),
+    "psx_profile": (_i, [_vp, _i, C.c_char_p, C.c_size_t]),
+    "psx_device_bytes": (_i64, [_vp]),
+    "psx_hex_element": (_i, [_d, _vp, _vp]),
+    "psx_hierarchy_create": (_i, [C.POINTER(ModelDesc), C.POINTER(_vp)]),
+    "psx_hierarchy_destroy": (None, [_vp]),
+    "psx_hierarchy_num_levels": (_i, [_vp]),
+    "psx_hierarchy_level_info": (_i, [_vp, _i, C.POINTER(LevelInfo)]),
+    "psx_hierarchy_level": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "psx_hierarchy_coarse": (_i, [_vp, C.POINTER(_i64), _vp, _vp]),
+}
+
+_lib = None
+
+
+def lib():
+    """The loaded libpsx.so (raises when it is not built: there is no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(LIB_PATH + " is not built: run make -C parts-sim_amd (or __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(code):
+    if code != OK:
+        raise PsxError(code, lib().psx_last_error().decode())
+
+
+def _p(a):
+    return None if a is None else C.c_void_p(a.ctypes.data)
+
+
+def available():
+    return bool(lib().psx_available())
+
+
+def _desc(dims, density, nu, bc, coarsest_max_dof, diag_add, max_levels, keep):
+    density = np.ascontiguousarray(density, np.float32).reshape(-1)
+    bc = np.ascontiguousarray(bc, np.uint8).reshape(-1)
+    nx, ny, nz = (int(v) for v in dims)
+    if density.size != nx * ny * nz or bc.size != 3 * (nx + 1) * (ny + 1) * (nz + 1):
+        raise ValueError("Structural grid, density and fixture sizes do not match.")
+    da = None if diag_add is None else np.ascontiguousarray(diag_add, np.float64)
+    keep.extend([density, bc, da])
+    return ModelDesc(nx, ny, nz, density.ctypes.data, float(nu), bc.ctypes.data, None if da is None else da.ctypes.data,
+                     int(coarsest_max_dof), int(max_levels))
+
+
+def _level_info(fn, h, l):
+    li = LevelInfo()
+    _check(fn(h, l, C.byref(li)))
+    return {"dims": (li.nx, li.ny, li.nz), "nNodes": li.n_nodes, "nDof": li.n_dof, "freeDof": li.free_dof,
+            "nElems": li.n_elems, "nOwn": li.n_own, "lmax": li.lmax}
+
+
+def hex_element(nu):
+    K = np.zeros(576)
+    cs = np.zeros(8 * 144)
+    _check(lib().psx_hex_element(float(nu), _p(K), _p(cs)))
+    return K.reshape(24, 24), cs.reshape(8, 6, 24)
+
+
+class Hierarchy:
+    """The multigrid hierarchy built on the host, exactly as a model uploads it (no device)."""
+
+    def __init__(self, dims, density, nu, bc, coarsest_max_dof=1100, diag_add=None, max_levels=0):
+        keep = []
+        d = _desc(dims, density, nu, bc, coarsest_max_dof, diag_add, max_levels, keep)
+        h = C.c_void_p()
+        _check(lib().psx_hierarchy_create(C.byref(d), C.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().psx_hierarchy_destroy(self.h)
+            self.h = None
+
+    @property
+    def num_levels(self):
+        return lib().psx_hierarchy_num_levels(self.h)
+
+    def level(self, l):
+        return _level_info(lib().psx_hierarchy_level_info, self.h, l)
+
+    def level_arrays(self, l):
+        info = self.level(l)
+        nvox = int(np.prod(info["dims"]))
+        fixed = np.zeros(info["nDof"], np.uint8)
+        inv = np.zeros(info["nDof"])
+        ns = np.zeros(info["nNodes"], np.float32)
+        words = np.zeros(nvox, np.int32)
+        kb = np.zeros(576)
+        own = np.zeros(max(1, info["nOwn"]) * 576, np.float32)
+        _check(lib().psx_hierarchy_level(self.h, l, _p(fixed), _p(inv), _p(ns), _p(words), _p(kb), _p(own)))
+        return {"fixed": fixed, "invDiag": inv, "nodeScale": ns, "words": words, "Kb": kb.reshape(24, 24),
+                "own": own[:info["nOwn"] * 576].reshape(-1, 24, 24)}
+
+    def coarse(self):
+        m = C.c_int64()
+        _check(lib().psx_hierarchy_coarse(self.h, C.byref(m), None, None))
+        idx = np.zeros(max(1, m.value), np.int32)
+        ainv = np.zeros(max(1, m.value * m.value))
+        _check(lib().psx_hierarchy_coarse(self.h, C.byref(m), _p(idx), _p(ainv)))
+        return idx[:m.value], ainv[:m.value * m.value].reshape(m.value, m.value)
+
+
+class Model:
+    """One part on the GPU: its hierarchy in HBM and the solver around it
+    (VoxelFEA + GpuFeaSolver of the reference)."""
+
+    def __init__(self, dims, density, nu, bc, coarsest_max_dof=1100, diag_add=None, max_levels=0):
+        keep = []
+        d = _desc(dims, density, nu, bc, coarsest_max_dof, diag_add, max_levels, keep)
+        h = C.c_void_p()
+        _check(lib().psx_create(C.byref(d), C.byref(h)))
+        self.h = h
+        self.dims = tuple(int(v) for v in dims)
+        self.n_dof = self.level(0)["nDof"]
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().psx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    @property
+    def num_levels(self):
+        return lib().psx_num_levels(self.h)
+
+    def level(self, l):
+        return _level_info(lib().psx_get_level_info, self.h, l)
+
+    def estimate_smoothers(self):
+        _check(lib().psx_estimate_smoothers(self.h))
+        return [self.level(l)["lmax"] for l in range(self.num_levels)]
+
+    def set_lmax(self, level, lmax):
+        _check(lib().psx_set_lmax(self.h, int(level), float(lmax)))
+
+    def apply(self, x, level=0, raw=False):
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.zeros_like(x)
+        _check(lib().psx_apply(self.h, int(level), _p(x), _p(y), int(raw)))
+        return y
+
+    def apply32(self, x, level=0, raw=False):
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.zeros_like(x)
+        _check(lib().psx_apply32(self.h, int(level), _p(x), _p(y), int(raw)))
+        return y
+
+    def precondition(self, r):
+        r = np.ascontiguousarray(r, np.float64)
+        z = np.zeros_like(r)
+        _check(lib().psx_precondition(self.h, _p(r), _p(z)))
+        return z
+
+    @staticmethod
+    def _opts(tol, max_iter, precision, check_every, fold_every, on_progress):
+        cb = PROGRESS(0) if on_progress is None else PROGRESS(lambda it, rel, user: int(bool(on_progress(it, rel))))
+        return SolveOpts(float(tol), int(max_iter), PREC[precision], int(check_every), int(fold_every), cb, None), cb
+
+    @staticmethod
+    def _info(info):
+        return {"iterations": info.iterations, "residual": info.residual, "converged": bool(info.converged),
+                "cancelled": bool(info.cancelled), "broken": bool(info.broken), "folds": info.folds, "device_ms": info.device_ms}
+
+    def solve(self, f, tol=1e-6, max_iter=500, x0=None, precision="fp64", check_every=0, fold_every=0, on_progress=None):
+        f = np.ascontiguousarray(f, np.float64)
+        if f.size != self.n_dof or (x0 is not None and np.size(x0) != self.n_dof):
+            raise ValueError("Force or displacement vector has the wrong size.")
+        x0a = None if x0 is None else np.ascontiguousarray(x0, np.float64)
+        u = np.zeros_like(f)
+        o, cb = self._opts(tol, max_iter, precision, check_every, fold_every, on_progress)
+        info = SolveInfo()
+        _check(lib().psx_solve(self.h, _p(f), _p(x0a), C.byref(o), _p(u), C.byref(info)))
+        out = self._info(info)
+        out["u"] = u
+        return out
+
+    def solve_device(self, d_f, d_u, d_x0=None, tol=1e-6, max_iter=500, precision="fp64", check_every=0, fold_every=0):
+        """Solve with HBM-resident vectors (DeviceArray or raw device addresses)."""
+        o, cb = self._opts(tol, max_iter, precision, check_every, fold_every, None)
+        info = SolveInfo()
+        addr = lambda a: None if a is None else C.c_void_p(int(a))
+        _check(lib().psx_solve_device(self.h, addr(d_f), addr(d_x0), C.byref(o), addr(d_u), C.byref(info)))
+        return self._info(info)
+
+    def reactions(self, u, f=None):
+        R = np.zeros(3)
+        fa = None if f is None else np.ascontiguousarray(f, np.float64)
+        _check(lib().psx_reactions(self.h, _p(np.ascontiguousarray(u, np.float64)), _p(fa), _p(R)))
+        return R
+
+    def profile(self, reps=20):
+        buf = C.create_string_buffer(1 << 16)
+        _check(lib().psx_profile(self.h, int(reps), buf, len(buf)))
+        return buf.value.decode()
+
+    @property
+    def device_bytes(self):
+        return int(lib().psx_device_bytes(self.h))
+
+
+# ---- HBM buffers for the *_device calls (the HIP runtime directly; no torch needed) ----
+
+_hip = None
+
+
+def _hiprt():
+    global _hip
+    if _hip is None:
+        _hip = C.CDLL("libamdhip64.so")
+        _hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        _hip.hipFree.argtypes = [C.c_void_p]
+        _hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        _hip.hipDeviceSynchronize.argtypes = []
+    return _hip
+
+
+class DeviceArray:
+    """A flat float64 buffer in HBM."""
+
+    def __init__(self, n, host=None):
+        self.n = int(n)
+        self.ptr = C.c_void_p()
+        if _hiprt().hipMalloc(C.byref(self.ptr), max(8, 8 * self.n)) != 0:
+            raise PsxError(ERR_DEVICE, "hipMalloc failed")
+        if host is not None:
+            self.upload(host)
+
+    def __int__(self):
+        return int(self.ptr.value)
+
+    def upload(self, host):
+        h = np.ascontiguousarray(host, np.float64).reshape(-1)
+        assert h.size == self.n
+        if _hiprt().hipMemcpy(self.ptr, C.c_void_p(h.ctypes.data), 8 * self.n, 1) != 0:
+            raise PsxError(ERR_DEVICE, "hipMemcpy H2D failed")
+
+    def download(self):
+        h = np.zeros(self.n)
+        if _hiprt().hipMemcpy(C.c_void_p(h.ctypes.data), self.ptr, 8 * self.n, 2) != 0:
+            raise PsxError(ERR_DEVICE, "hipMemcpy D2H failed")
+        return h
+
+    def free(self):
+        if self.ptr:
+            _hiprt().hipFree(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        self.free()

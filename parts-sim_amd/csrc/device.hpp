@@ -1,0 +1,91 @@
+// Device-side view of one multigrid level and the launchers of the HIP kernels (kernels.hip).
+// Everything here is plain data and pointers into HBM; the host code in solver.cpp owns the buffers.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <cstdint>
+
+namespace psx {
+
+// One level as the kernels see it. Vectors are AoS [node][3] over the full (NX, NY, NZ) node box,
+// node n = i + NX (j + NY k); voxels v = i + nx (j + ny k). A DOF is held (or inactive) exactly
+// when its inverse diagonal is 0, so no separate mask array is read.
+struct DevLevel {
+    int nx, ny, nz, NX, NY, NZ;
+    int64_t nNodes, nDof;
+    const int32_t* words;      // per voxel: 0 empty, >0 float bits of s (s * Kb), <0 0x80000000 | own index
+    const float* Kown;         // own element matrices, 576 floats each
+    const double* Kb64;        // the level's uniform element matrix
+    const float* Kb32;
+    const double* St64;        // 27-point stencil of Kb: [27 neighbours][3 rows][3 cols]
+    const float* St32;
+    const float* nodeScale;    // s for nodes whose 8 voxels are s * Kb, else 0
+    const double* invD64;      // inverse diagonal, 0 on held / inactive DOFs
+    const float* invD32;
+    const double* diagAdd;     // optional diagonal shift (nullptr when absent)
+};
+
+// Scalars of one conjugate-gradient run, resident on the device. Every kernel of an iteration
+// returns at once when `stop` is set, so the host can queue iterations ahead and read the state
+// only now and then.
+struct CgState {
+    double rz, pq, alpha, beta, rr, rzNew, zq, bnorm, tol, rel;
+    double rmax, scale, peak, trueRel, spare[6];
+    int stop, iters, maxIter, broken, cancel, folds, sinceFold, foldEvery;
+    double lz[32];             // Lanczos alpha[0..9], beta at [16..25]
+};
+
+// partial sums of the reductions (one slot per block, up to 4 values per kernel)
+constexpr int RED_BLOCKS = 2048;
+struct Reduce {
+    double* partials;          // 4 * RED_BLOCKS
+    unsigned* counter;
+};
+
+enum ApplyMode { AM_APPLY = 0, AM_RAW = 1, AM_RESID = 2, AM_CHEB1 = 3, AM_CHEB2 = 4 };
+
+// y = K x on a level with the epilogue of `mode` (T = double or float):
+//   AM_APPLY  y = K x, held DOFs 0               AM_RAW   y = K x
+//   AM_RESID  y = r - K x, held DOFs 0
+//   AM_CHEB1  d = c1 D^-1 (r - K x);        y = x + d
+//   AM_CHEB2  d = c1 d + c2 D^-1 (r - K x); y = x + d
+template <class T> void launchApply(const DevLevel& L, int mode, const T* x, T* y, const T* r, T* d, double c1, double c2, const int* stop,
+                                    hipStream_t s);
+// z = d = c0 D^-1 r (first Chebyshev step from zero)
+template <class T> void launchChebFirst(const DevLevel& L, const T* r, T* z, T* d, double c0, const int* stop, hipStream_t s);
+// rc = P^T rf on the coarse level's free DOFs
+template <class T> void launchRestrict(const DevLevel& F, const DevLevel& C, const T* rf, T* rc, const int* stop, hipStream_t s);
+// zf += P zc on the fine level's free DOFs
+template <class T> void launchProlong(const DevLevel& F, const DevLevel& C, const T* zc, T* zf, const int* stop, hipStream_t s);
+// z[freeIdx] = Ainv r[freeIdx] (coarsest level; other entries of z untouched)
+template <class T> void launchCoarse(const int32_t* freeIdx, int64_t m, const double* Ainv, const T* r, T* z, const int* stop, hipStream_t s);
+
+// conjugate-gradient steps (all accumulate in FP64; T is the vector type)
+// pq = p.q; alpha = rz / pq (or broken + stop when pq <= 0)
+template <class T> void launchDotPQ(const T* p, const T* q, int64_t n, CgState* st, Reduce red, hipStream_t s);
+// x += alpha p; r -= alpha q; rr = r.r; iteration count, relative residual, stop test
+template <class T> void launchUpdateXR(T* x, const T* p, T* r, const T* q, int64_t n, CgState* st, Reduce red, hipStream_t s);
+// rzNew = r.z, zq = z.q; beta = max(0, -alpha zq / rz) (flexible); rz = rzNew
+template <class T> void launchDotsRZ(const T* r, const T* z, const T* q, int64_t n, CgState* st, Reduce red, hipStream_t s);
+// p = z + beta p
+template <class T> void launchUpdateP(const T* z, T* p, int64_t n, CgState* st, hipStream_t s);
+
+// r = f - Kx on free DOFs (Kx already masked), out[0] += |r|^2 over the grid (FP64)
+void launchResidual64(const double* f, const double* Kx, const double* invD, double* r, int64_t n, double* out, Reduce red, hipStream_t s);
+// out = sum over free DOFs of f^2
+void launchNormFree64(const double* f, const double* invD, int64_t n, double* out, Reduce red, hipStream_t s);
+// x[q] = held ? 0 : x0[q] (or 0 when x0 is null)
+void launchMaskCopy64(const double* x0, const double* invD, double* x, int64_t n, hipStream_t s);
+// out = max |v| over DOFs (FP64 vector)
+void launchMaxAbs64(const double* v, int64_t n, double* out, Reduce red, hipStream_t s);
+// y32 = float(x64 * (1 / *scaleFrom)) (the V-cycle's order-one input), z64 = double(z32) * *scaleFrom
+void launchToF32(const double* x, float* y, int64_t n, const double* scaleFrom, hipStream_t s);
+void launchFromF32(const float* z, double* y, int64_t n, const double* scaleFrom, hipStream_t s);
+// Lanczos steps for lmax(D^-1 K) (voxel_fea.cpp:579-610)
+void launchLzX(const double* v, const double* invD, double* x, int64_t n, hipStream_t s);
+void launchLzW(double* w, const double* invD, const double* vPrev, const double* v, int64_t n, CgState* st, int j, Reduce red, hipStream_t s);
+void launchLzAxpy(double* w, const double* v, int64_t n, CgState* st, int j, Reduce red, hipStream_t s);
+void launchLzNext(const double* w, double* v, double* vPrev, int64_t n, CgState* st, int j, hipStream_t s);
+
+}  // namespace psx

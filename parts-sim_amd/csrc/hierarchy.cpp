@@ -1,0 +1,589 @@
+// Host-side hierarchy build: element matrix, levels, exact Galerkin coarsening and the coarsest
+// level's dense inverse. Runs once per model on the CPU cores; everything per-iteration runs on the
+// GPU (solver.cpp, kernels.hip).
+#include "psx_core.hpp"
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <stdexcept>
+#include <thread>
+
+namespace psx {
+
+int hostThreads() {
+    static const int n = [] {
+        if (const char* e = std::getenv("PSX_HOST_THREADS")) {
+            const int v = std::atoi(e);
+            if (v > 0) return v;
+        }
+        return int(std::max(1u, std::min(64u, std::thread::hardware_concurrency())));
+    }();
+    return n;
+}
+
+// fn(lo, hi) over [0, n) in contiguous chunks on the host threads
+template <class F> static void parallelRange(i64 n, F&& fn, i64 minChunk = 256) {
+    const int T = int(std::min<i64>(hostThreads(), std::max<i64>(1, n / minChunk)));
+    if (T <= 1) {
+        fn(i64(0), n);
+        return;
+    }
+    std::vector<std::thread> th;
+    th.reserve(T - 1);
+    for (int t = 1; t < T; t++) th.emplace_back([&, t] { fn(n * t / T, n * (t + 1) / T); });
+    fn(0, n / T);
+    for (auto& x : th) x.join();
+}
+
+// ---------------------------------------------------------------------------------------------
+// hex8 element (native/src/fea/hex8.cpp:8-155)
+
+// isotropic D for E = 1 (Voigt xx yy zz xy yz zx, engineering shear)
+static void elasticityD(double nu, double D[6][6]) {
+    if (!std::isfinite(nu) || nu <= -1 || nu >= 0.5) throw std::invalid_argument("Poisson's ratio must be greater than -1 and less than 0.5.");
+    const double lam = nu / ((1 + nu) * (1 - 2 * nu)), mu = 1 / (2 * (1 + nu));
+    for (int i = 0; i < 6; i++)
+        for (int j = 0; j < 6; j++) D[i][j] = 0;
+    for (int i = 0; i < 3; i++) {
+        for (int j = 0; j < 3; j++) D[i][j] = lam;
+        D[i][i] = lam + 2 * mu;
+        D[i + 3][i + 3] = mu;
+    }
+}
+
+// strain rows of a displacement triple with gradient g: exx eyy ezz gxy gyz gzx
+static void strainOf(double (*B)[24], int col, const double g[3]) {
+    B[0][col] = g[0];
+    B[1][col + 1] = g[1];
+    B[2][col + 2] = g[2];
+    B[3][col] = g[1];
+    B[3][col + 1] = g[0];
+    B[4][col + 1] = g[2];
+    B[4][col + 2] = g[1];
+    B[5][col] = g[2];
+    B[5][col + 2] = g[0];
+}
+
+// B (6 x 24) of the trilinear shape functions and G (6 x 9) of the three bubble modes 1 - xi^2,
+// 1 - eta^2, 1 - zeta^2 at natural coordinates p in [-1, 1]^3 (unit cube: d xi / dx = 2)
+static void shapeGradients(const double p[3], double B[6][24], double G[6][24]) {
+    for (int r = 0; r < 6; r++)
+        for (int c = 0; c < 24; c++) B[r][c] = G[r][c] = 0;
+    for (int a = 0; a < 8; a++) {
+        const double s[3] = {2.0 * CX[a] - 1, 2.0 * CY[a] - 1, 2.0 * CZ[a] - 1};
+        double g[3];
+        for (int ax = 0; ax < 3; ax++) {
+            double v = 2 * s[ax] / 8;
+            for (int o = 0; o < 3; o++)
+                if (o != ax) v *= 1 + s[o] * p[o];
+            g[ax] = v;
+        }
+        strainOf(B, 3 * a, g);
+    }
+    for (int m = 0; m < 3; m++) {
+        double g[3] = {0, 0, 0};
+        g[m] = -4 * p[m];
+        strainOf(G, 3 * m, g);
+    }
+}
+
+Element hexElement(double nu) {
+    double D[6][6];
+    elasticityD(nu, D);
+    double Kuu[24][24] = {}, Kua[24][9] = {}, Kaa[9][9] = {};
+    const double gp = 1 / std::sqrt(3.0);
+    double B[6][24], G[6][24], DB[6][24], DG[6][24];
+    for (int q = 0; q < 8; q++) {
+        const double p[3] = {(2.0 * CX[q] - 1) * gp, (2.0 * CY[q] - 1) * gp, (2.0 * CZ[q] - 1) * gp};
+        shapeGradients(p, B, G);
+        for (int r = 0; r < 6; r++)
+            for (int c = 0; c < 24; c++) {
+                double b = 0, g = 0;
+                for (int t = 0; t < 6; t++) {
+                    b += D[r][t] * B[t][c];
+                    g += D[r][t] * G[t][c];
+                }
+                DB[r][c] = b;
+                DG[r][c] = g;
+            }
+        // Gauss weight 1, det J = 1/8
+        for (int i = 0; i < 24; i++) {
+            for (int j = 0; j < 24; j++) {
+                double s = 0;
+                for (int t = 0; t < 6; t++) s += B[t][i] * DB[t][j];
+                Kuu[i][j] += s / 8;
+            }
+            for (int j = 0; j < 9; j++) {
+                double s = 0;
+                for (int t = 0; t < 6; t++) s += B[t][i] * DG[t][j];
+                Kua[i][j] += s / 8;
+            }
+        }
+        for (int i = 0; i < 9; i++)
+            for (int j = 0; j < 9; j++) {
+                double s = 0;
+                for (int t = 0; t < 6; t++) s += G[t][i] * DG[t][j];
+                Kaa[i][j] += s / 8;
+            }
+    }
+    // static condensation of the bubble amplitudes: alpha = A u with A = -Kaa^-1 Kau (Cholesky of Kaa)
+    double Lc[9][9] = {};
+    for (int j = 0; j < 9; j++) {
+        double s = Kaa[j][j];
+        for (int k = 0; k < j; k++) s -= Lc[j][k] * Lc[j][k];
+        Lc[j][j] = std::sqrt(s);
+        for (int i = j + 1; i < 9; i++) {
+            double t = Kaa[i][j];
+            for (int k = 0; k < j; k++) t -= Lc[i][k] * Lc[j][k];
+            Lc[i][j] = t / Lc[j][j];
+        }
+    }
+    double A[9][24];
+    for (int c = 0; c < 24; c++) {
+        double y[9];
+        for (int i = 0; i < 9; i++) {
+            double t = -Kua[c][i];
+            for (int k = 0; k < i; k++) t -= Lc[i][k] * y[k];
+            y[i] = t / Lc[i][i];
+        }
+        for (int i = 8; i >= 0; i--) {
+            double t = y[i];
+            for (int k = i + 1; k < 9; k++) t -= Lc[k][i] * y[k];
+            y[i] = t / Lc[i][i];
+        }
+        for (int i = 0; i < 9; i++) A[i][c] = y[i];
+    }
+    Element el;
+    for (int i = 0; i < 24; i++)
+        for (int j = 0; j < 24; j++) {
+            double s = Kuu[i][j];
+            for (int t = 0; t < 9; t++) s += Kua[i][t] * A[t][j];
+            el.K[i * 24 + j] = s;
+        }
+    for (int i = 0; i < 24; i++)
+        for (int j = i + 1; j < 24; j++) el.K[i * 24 + j] = el.K[j * 24 + i] = 0.5 * (el.K[i * 24 + j] + el.K[j * 24 + i]);
+    // corner stresses D (B + G A) at the element corners
+    for (int a = 0; a < 8; a++) {
+        const double p[3] = {2.0 * CX[a] - 1, 2.0 * CY[a] - 1, 2.0 * CZ[a] - 1};
+        shapeGradients(p, B, G);
+        double Be[6][24];
+        for (int r = 0; r < 6; r++)
+            for (int c = 0; c < 24; c++) {
+                double s = B[r][c];
+                for (int t = 0; t < 9; t++) s += G[r][t] * A[t][c];
+                Be[r][c] = s;
+            }
+        for (int r = 0; r < 6; r++)
+            for (int c = 0; c < 24; c++) {
+                double s = 0;
+                for (int t = 0; t < 6; t++) s += D[r][t] * Be[t][c];
+                el.corner[a][r * 24 + c] = s;
+            }
+    }
+    return el;
+}
+
+void stencil(const double* K, double* S) {
+    std::fill(S, S + 243, 0.0);
+    for (int v = 0; v < 8; v++) {
+        const int di = v & 1, dj = (v >> 1) & 1, dk = v >> 2;
+        const int a = corner(1 - di, 1 - dj, 1 - dk);
+        for (int b = 0; b < 8; b++) {
+            const int m = (di + CX[b]) + 3 * (dj + CY[b]) + 9 * (dk + CZ[b]);
+            for (int d = 0; d < 3; d++)
+                for (int c = 0; c < 3; c++) S[m * 9 + d * 3 + c] += K[(3 * a + d) * 24 + 3 * b + c];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// levels
+
+static Level emptyLevel(int nx, int ny, int nz) {
+    Level L;
+    L.nx = nx; L.ny = ny; L.nz = nz;
+    L.NX = nx + 1; L.NY = ny + 1; L.NZ = nz + 1;
+    L.nNodes = i64(L.NX) * L.NY * L.NZ;
+    L.nDof = 3 * L.nNodes;
+    L.nVox = i64(nx) * ny * nz;
+    return L;
+}
+
+// diagonal, held DOFs, stencil scales and device words of a level whose elements are set
+// (voxel_fea.cpp:208-266)
+static void finishLevel(Level& L) {
+    const i64 nE = i64(L.elemVox.size());
+    L.active.assign(L.nNodes, 0);
+    std::vector<double> diag(L.nDof, 0.0);
+    for (i64 e = 0; e < nE; e++) {
+        const int v = L.elemVox[e];
+        const int i = v % L.nx, j = (v / L.nx) % L.ny, k = v / (L.nx * L.ny);
+        const float* Ko = L.own[e] >= 0 ? L.K.data() + i64(L.own[e]) * 576 : nullptr;
+        for (int a = 0; a < 8; a++) {
+            const i64 n = L.node(i + CX[a], j + CY[a], k + CZ[a]);
+            L.active[n] = 1;
+            for (int d = 0; d < 3; d++) {
+                const int r = 3 * a + d;
+                diag[3 * n + d] += Ko ? double(Ko[r * 25]) : L.rho[e] * L.Kb[r * 25];
+            }
+        }
+    }
+    if (!L.diagAdd.empty())
+        for (i64 q = 0; q < L.nDof; q++) diag[q] += L.diagAdd[q];
+    L.fixed.assign(L.nDof, 0);
+    L.invDiag.assign(L.nDof, 0.0);
+    i64 nfree = 0;
+    for (i64 q = 0; q < L.nDof; q++) {
+        if (!L.active[q / 3] || L.bc[q] || !(diag[q] > 0)) L.fixed[q] = 1;
+        else {
+            L.invDiag[q] = 1 / diag[q];
+            nfree++;
+        }
+    }
+    L.freeDof = nfree;
+    L.words.assign(L.nVox, 0);
+    std::vector<float> vs(L.nVox, 0.f);
+    for (i64 e = 0; e < nE; e++) {
+        const int v = L.elemVox[e];
+        if (L.own[e] >= 0) {
+            L.words[v] = int32_t(0x80000000u | uint32_t(L.own[e]));
+            vs[v] = -1.f;
+        } else {
+            const float s = float(L.rho[e]);
+            std::memcpy(&L.words[v], &s, 4);
+            vs[v] = s;
+        }
+    }
+    L.nodeScale.assign(L.nNodes, 0.f);
+    for (int k = 1; k < L.nz; k++)
+        for (int j = 1; j < L.ny; j++)
+            for (int i = 1; i < L.nx; i++) {
+                const i64 v0 = (i - 1) + i64(L.nx) * ((j - 1) + i64(L.ny) * (k - 1));
+                const float s0 = vs[v0];
+                if (!(s0 > 0)) continue;
+                bool same = true;
+                for (int v = 1; v < 8 && same; v++) same = vs[v0 + (v & 1) + i64(L.nx) * (((v >> 1) & 1) + i64(L.ny) * (v >> 2))] == s0;
+                if (same) L.nodeScale[L.node(i, j, k)] = s0;
+            }
+}
+
+// interpolation weight of coarse corner b at fine corner a of child c (trilinear)
+static double childWeight(int c, int a, int b) {
+    const int cb[3] = {c & 1, (c >> 1) & 1, c >> 2};
+    const int ab[3] = {CX[a], CY[a], CZ[a]}, bb[3] = {CX[b], CY[b], CZ[b]};
+    double w = 1;
+    for (int ax = 0; ax < 3; ax++) {
+        const double s = (cb[ax] + ab[ax]) / 2.0;
+        w *= bb[ax] ? s : 1 - s;
+    }
+    return w;
+}
+
+// out += P_c^T K P_c for child position c (P_c: 24 fine DOFs of the child <- 24 coarse DOFs)
+static void galerkinChild(double* out, const double* K, int c) {
+    double W[8][8];
+    for (int a = 0; a < 8; a++)
+        for (int b = 0; b < 8; b++) W[a][b] = childWeight(c, a, b);
+    double T[24][24];  // K P
+    for (int r = 0; r < 24; r++)
+        for (int b = 0; b < 8; b++)
+            for (int d = 0; d < 3; d++) {
+                double s = 0;
+                for (int a = 0; a < 8; a++)
+                    if (W[a][b] != 0) s += K[r * 24 + 3 * a + d] * W[a][b];
+                T[r][3 * b + d] = s;
+            }
+    for (int b = 0; b < 8; b++)
+        for (int d = 0; d < 3; d++)
+            for (int col = 0; col < 24; col++) {
+                double s = 0;
+                for (int a = 0; a < 8; a++)
+                    if (W[a][b] != 0) s += W[a][b] * T[3 * a + d][col];
+                out[(3 * b + d) * 24 + col] += s;
+            }
+}
+
+// fine level F -> coarse level (voxel_fea.cpp:268-386): children of each coarse voxel, exact
+// Galerkin element matrices, held coarse DOFs and the lumped diagonal shift
+static Level coarsen(const Level& F) {
+    Level C = emptyLevel((F.nx + 1) >> 1, (F.ny + 1) >> 1, (F.nz + 1) >> 1);
+    const i64 nF = i64(F.elemVox.size());
+    std::vector<int32_t> count(C.nVox, 0);
+    std::vector<int32_t> parent(nF);
+    std::vector<uint8_t> pos(nF);
+    for (i64 q = 0; q < nF; q++) {
+        const int v = F.elemVox[q];
+        const int i = v % F.nx, j = (v / F.nx) % F.ny, k = v / (F.nx * F.ny);
+        parent[q] = (i >> 1) + C.nx * ((j >> 1) + C.ny * (k >> 1));
+        pos[q] = uint8_t((i & 1) | ((j & 1) << 1) | ((k & 1) << 2));
+        count[parent[q]]++;
+    }
+    C.emap.assign(C.nVox, -1);
+    for (i64 v = 0; v < C.nVox; v++)
+        if (count[v]) {
+            C.emap[v] = int32_t(C.elemVox.size());
+            C.elemVox.push_back(int32_t(v));
+        }
+    const i64 nE = i64(C.elemVox.size());
+    std::vector<int32_t> start(nE + 1, 0), kids(nF);
+    for (i64 q = 0; q < nF; q++) start[C.emap[parent[q]] + 1]++;
+    for (i64 e = 0; e < nE; e++) start[e + 1] += start[e];
+    {
+        std::vector<int32_t> fill(start.begin(), start.end() - 1);
+        for (i64 q = 0; q < nF; q++) kids[fill[C.emap[parent[q]]]++] = int32_t(q);
+    }
+    // the fine uniform element's Galerkin product per child position; their sum is the coarse Kb
+    std::vector<double> M(8 * 576, 0.0);
+    for (int c = 0; c < 8; c++) galerkinChild(M.data() + c * 576, F.Kb, c);
+    for (int u = 0; u < 576; u++) {
+        double s = 0;
+        for (int c = 0; c < 8; c++) s += M[c * 576 + u];
+        C.Kb[u] = s;
+    }
+    C.rho.assign(nE, 0.0);
+    C.own.assign(nE, -1);
+    int nOwn = 0;
+    for (i64 e = 0; e < nE; e++) {
+        const int b = start[e], t = start[e + 1];
+        bool uniform = t - b == 8 && F.own[kids[b]] < 0;
+        const double s = uniform ? F.rho[kids[b]] : 0;
+        for (int q = b; q < t && uniform; q++) uniform = F.own[kids[q]] < 0 && F.rho[kids[q]] == s;
+        if (uniform) C.rho[e] = s;
+        else C.own[e] = nOwn++;
+    }
+    C.nOwn = nOwn;
+    C.K.assign(i64(nOwn) * 576, 0.f);
+    parallelRange(nE, [&](i64 lo, i64 hi) {
+        std::vector<double> out(576), Kf(576);
+        for (i64 e = lo; e < hi; e++) {
+            if (C.own[e] < 0) continue;
+            std::fill(out.begin(), out.end(), 0.0);
+            for (int q = start[e]; q < start[e + 1]; q++) {
+                const int f = kids[q];
+                const int c = pos[f];
+                if (F.own[f] < 0) {
+                    const double s = F.rho[f];
+                    const double* Mc = M.data() + c * 576;
+                    for (int u = 0; u < 576; u++) out[u] += s * Mc[u];
+                } else {
+                    const float* Ko = F.K.data() + i64(F.own[f]) * 576;
+                    for (int u = 0; u < 576; u++) Kf[u] = Ko[u];
+                    galerkinChild(out.data(), Kf.data(), c);
+                }
+            }
+            float* dst = C.K.data() + i64(C.own[e]) * 576;
+            for (int u = 0; u < 576; u++) dst[u] = float(out[u]);
+        }
+    }, 64);
+    // a coarse DOF is held when it interpolates onto a held fine DOF
+    C.bc.assign(C.nDof, 0);
+    for (int k = 0; k < F.NZ; k++)
+        for (int j = 0; j < F.NY; j++)
+            for (int i = 0; i < F.NX; i++) {
+                const i64 fn = 3 * F.node(i, j, k);
+                const uint8_t h0 = F.bc[fn], h1 = F.bc[fn + 1], h2 = F.bc[fn + 2];
+                if (!(h0 | h1 | h2)) continue;
+                const int Ks[2] = {k / 2, (k & 1) ? (k + 1) / 2 : -1}, Js[2] = {j / 2, (j & 1) ? (j + 1) / 2 : -1},
+                          Is[2] = {i / 2, (i & 1) ? (i + 1) / 2 : -1};
+                for (int K : Ks)
+                    for (int J : Js)
+                        for (int I : Is) {
+                            if (K < 0 || J < 0 || I < 0) continue;
+                            const i64 cn = 3 * C.node(I, J, K);
+                            C.bc[cn] |= h0;
+                            C.bc[cn + 1] |= h1;
+                            C.bc[cn + 2] |= h2;
+                        }
+            }
+    if (!F.diagAdd.empty()) {
+        // lumped Galerkin product of a diagonal: P^T (D 1), full weighting over all coarse nodes
+        C.diagAdd.assign(C.nDof, 0.0);
+        for (int K = 0; K < C.NZ; K++)
+            for (int J = 0; J < C.NY; J++)
+                for (int I = 0; I < C.NX; I++) {
+                    double s[3] = {0, 0, 0};
+                    for (int dk = -1; dk <= 1; dk++) {
+                        const int kf = 2 * K + dk;
+                        if (kf < 0 || kf >= F.NZ) continue;
+                        for (int dj = -1; dj <= 1; dj++) {
+                            const int jf = 2 * J + dj;
+                            if (jf < 0 || jf >= F.NY) continue;
+                            for (int di = -1; di <= 1; di++) {
+                                const int fi = 2 * I + di;
+                                if (fi < 0 || fi >= F.NX) continue;
+                                const double w = (di ? 0.5 : 1.0) * (dj ? 0.5 : 1.0) * (dk ? 0.5 : 1.0);
+                                const i64 fn = 3 * F.node(fi, jf, kf);
+                                for (int d = 0; d < 3; d++) s[d] += w * F.diagAdd[fn + d];
+                            }
+                        }
+                    }
+                    const i64 cn = 3 * C.node(I, J, K);
+                    for (int d = 0; d < 3; d++) C.diagAdd[cn + d] = s[d];
+                }
+    }
+    finishLevel(C);
+    return C;
+}
+
+// coarsest free-DOF matrix, its Cholesky factor (pivots regularised like voxel_fea.cpp:639-659)
+// and the explicit inverse L^-T L^-1 that the device applies as one matrix-vector product
+static void coarseInverse(Hierarchy& H) {
+    const Level& L = H.levels.back();
+    std::vector<i64> map(L.nDof, -1);
+    H.freeIdx.clear();
+    for (i64 q = 0; q < L.nDof; q++)
+        if (!L.fixed[q]) {
+            map[q] = i64(H.freeIdx.size());
+            H.freeIdx.push_back(int32_t(q));
+        }
+    const i64 m = i64(H.freeIdx.size());
+    H.m = m;
+    if (m == 0 || m > 3000) throw std::runtime_error("the coarsest level has no direct solver");
+    std::vector<double> A(size_t(m * m), 0.0);
+    i64 dofs[24];
+    for (size_t e = 0; e < L.elemVox.size(); e++) {
+        const int v = L.elemVox[e];
+        const int i = v % L.nx, j = (v / L.nx) % L.ny, k = v / (L.nx * L.ny);
+        for (int a = 0; a < 8; a++)
+            for (int d = 0; d < 3; d++) dofs[3 * a + d] = map[3 * L.node(i + CX[a], j + CY[a], k + CZ[a]) + d];
+        const float* Ko = L.own[e] >= 0 ? L.K.data() + i64(L.own[e]) * 576 : nullptr;
+        for (int r = 0; r < 24; r++) {
+            if (dofs[r] < 0) continue;
+            for (int c = 0; c < 24; c++)
+                if (dofs[c] >= 0) A[dofs[r] * m + dofs[c]] += Ko ? double(Ko[r * 24 + c]) : L.rho[e] * L.Kb[r * 24 + c];
+        }
+    }
+    if (!L.diagAdd.empty())
+        for (i64 q = 0; q < L.nDof; q++)
+            if (map[q] >= 0) A[map[q] * m + map[q]] += L.diagAdd[q];
+    double maxDiag = 0;
+    for (i64 q = 0; q < m; q++) maxDiag = std::max(maxDiag, A[q * m + q]);
+    // right-looking Cholesky, lower triangle, trailing rows in parallel
+    for (i64 j = 0; j < m; j++) {
+        double s = A[j * m + j];
+        if (!(s > 1e-10 * maxDiag)) s = maxDiag;
+        const double ljj = std::sqrt(s), inv = 1 / ljj;
+        A[j * m + j] = ljj;
+        for (i64 q = j + 1; q < m; q++) A[q * m + j] *= inv;
+        const i64 rest = m - j - 1;
+        auto upd = [&](i64 lo, i64 hi) {
+            for (i64 q = j + 1 + lo; q < j + 1 + hi; q++) {
+                const double lqj = A[q * m + j];
+                if (lqj == 0) continue;
+                double* row = A.data() + q * m;
+                for (i64 t = j + 1; t <= q; t++) row[t] -= lqj * A[t * m + j];
+            }
+        };
+        if (rest > 256) parallelRange(rest, upd, 64);
+        else upd(0, rest);
+    }
+    // W = L^-1 (lower), column by column
+    std::vector<double> W(size_t(m * m), 0.0);
+    parallelRange(m, [&](i64 lo, i64 hi) {
+        std::vector<double> y(m);
+        for (i64 c = lo; c < hi; c++) {
+            std::fill(y.begin(), y.end(), 0.0);
+            y[c] = 1 / A[c * m + c];
+            for (i64 q = c + 1; q < m; q++) {
+                double s = 0;
+                const double* row = A.data() + q * m;
+                for (i64 t = c; t < q; t++) s += row[t] * y[t];
+                y[q] = -s / row[q];
+            }
+            for (i64 q = c; q < m; q++) W[q * m + c] = y[q];
+        }
+    }, 8);
+    // A^-1 = W^T W (symmetric)
+    H.Ainv.assign(size_t(m * m), 0.0);
+    parallelRange(m, [&](i64 lo, i64 hi) {
+        for (i64 r = lo; r < hi; r++)
+            for (i64 c = 0; c <= r; c++) {
+                double s = 0;
+                for (i64 t = r; t < m; t++) s += W[t * m + r] * W[t * m + c];
+                H.Ainv[r * m + c] = s;
+                H.Ainv[c * m + r] = s;
+            }
+    }, 8);
+}
+
+void buildHierarchy(Hierarchy& H, int nx, int ny, int nz, const float* density, double nu, const uint8_t* bc, i64 coarsestMaxDof,
+                    const double* diagAdd, int maxLevels) {
+    if (nx < 1 || ny < 1 || nz < 1) throw std::invalid_argument("The structural grid must have three positive dimensions.");
+    if (!density || !bc) throw std::invalid_argument("density and bc are required");
+    const i64 nVox = i64(nx) * ny * nz;
+    for (i64 e = 0; e < nVox; e++)
+        if (!std::isfinite(density[e]) || density[e] < 0 || density[e] > 1) throw std::invalid_argument("Voxel density must be between zero and one.");
+    if (maxLevels <= 0 || maxLevels > 12) maxLevels = 12;
+    H.nu = nu;
+    H.el = hexElement(nu);
+    H.levels.clear();
+    Level L0 = emptyLevel(nx, ny, nz);
+    std::memcpy(L0.Kb, H.el.K, sizeof L0.Kb);
+    L0.emap.assign(nVox, -1);
+    for (i64 v = 0; v < nVox; v++)
+        if (density[v] > 0) {
+            L0.emap[v] = int32_t(L0.elemVox.size());
+            L0.elemVox.push_back(int32_t(v));
+            L0.rho.push_back(density[v]);
+        }
+    L0.own.assign(L0.elemVox.size(), -1);
+    L0.bc.assign(bc, bc + L0.nDof);
+    if (diagAdd) L0.diagAdd.assign(diagAdd, diagAdd + L0.nDof);
+    finishLevel(L0);
+    H.levels.push_back(std::move(L0));
+    for (;;) {
+        const Level& L = H.levels.back();
+        if (L.freeDof <= coarsestMaxDof || int(H.levels.size()) >= maxLevels) break;
+        if (L.nx <= 1 && L.ny <= 1 && L.nz <= 1) break;
+        Level C = coarsen(L);
+        H.levels.push_back(std::move(C));
+    }
+    coarseInverse(H);
+}
+
+std::vector<double> lanczosStart(const Level& L) {
+    std::vector<double> v(L.nDof);
+    uint32_t seed = 12345;
+    double nn = 0;
+    for (i64 q = 0; q < L.nDof; q++) {
+        seed = (seed * 1103515245u + 12345u) & 0x7fffffffu;
+        v[q] = L.fixed[q] ? 0 : double(seed) / 0x7fffffff - 0.5;
+        nn += v[q] * v[q];
+    }
+    const double s = nn > 0 ? 1 / std::sqrt(nn) : 1;
+    for (auto& x : v) x *= s;
+    return v;
+}
+
+double tridiagonalMax(const std::vector<double>& a, const std::vector<double>& b) {
+    const int k = int(a.size());
+    if (k == 0) return 0;
+    double lo = INFINITY, hi = -INFINITY;
+    for (int i = 0; i < k; i++) {
+        const double r = (i > 0 ? std::abs(b[i - 1]) : 0) + (i + 1 < k ? std::abs(b[i]) : 0);
+        lo = std::min(lo, a[i] - r);
+        hi = std::max(hi, a[i] + r);
+    }
+    // number of eigenvalues above x from the signs of the LDL^T pivots of T - x I
+    auto above = [&](double x) {
+        int n = 0;
+        double q = 1;
+        for (int i = 0; i < k; i++) {
+            q = a[i] - x - (i > 0 ? b[i - 1] * b[i - 1] / q : 0);
+            if (q == 0) q = 1e-300;
+            if (q > 0) n++;
+        }
+        return n;
+    };
+    for (int it = 0; it < 100 && hi - lo > 1e-12 * std::abs(hi); it++) {
+        const double mid = 0.5 * (lo + hi);
+        if (above(mid) > 0) lo = mid;
+        else hi = mid;
+    }
+    return hi;
+}
+
+}  // namespace psx

@@ -1,0 +1,556 @@
+// HIP kernels of the voxel FEA solve for gfx950 (MI355X): the matrix-free hex8 operator with its
+// fused Chebyshev / residual epilogues, grid transfers, the coarsest dense solve, and the
+// conjugate-gradient vector steps with deterministic FP64 reductions.
+//
+// Operator (native/src/fea/voxel_fea.cpp:390-504 is the reference form): one thread per node gathers
+// its 27 neighbours. A node whose 8 voxels are all s * Kb uses the assembled 27-point stencil
+// (243 multiply-adds); any other node sums the rows of its up to 8 voxels (72 multiply-adds each),
+// from s * Kb or from the voxel's own matrix. Nothing is scattered, so there are no write conflicts.
+//
+// Reductions: every block reduces with wavefront shuffles (64 lanes) and LDS, writes one partial per
+// block; the block that finishes last (atomic counter) sums the partials in block order. The grid
+// size depends only on the vector length, so every sum is bitwise reproducible and independent of
+// what else runs on the GPU.
+#include <hip/hip_runtime.h>
+
+#include "device.hpp"
+
+namespace psx {
+
+namespace {
+
+constexpr int BLOCK = 256;
+
+__device__ constexpr int d_corner(int x, int y, int z) { return (y ? (x ? 2 : 3) : (x ? 1 : 0)) + 4 * z; }
+// corner offsets as compile-time tables for the unrolled loops
+__device__ constexpr int kCX(int b) { return (b == 1 || b == 2 || b == 5 || b == 6) ? 1 : 0; }
+__device__ constexpr int kCY(int b) { return (b == 2 || b == 3 || b == 6 || b == 7) ? 1 : 0; }
+__device__ constexpr int kCZ(int b) { return b >= 4 ? 1 : 0; }
+
+template <class T> struct Prec;
+template <> struct Prec<double> {
+    __device__ static const double* Kb(const DevLevel& L) { return L.Kb64; }
+    __device__ static const double* St(const DevLevel& L) { return L.St64; }
+    __device__ static const double* invD(const DevLevel& L) { return L.invD64; }
+};
+template <> struct Prec<float> {
+    __device__ static const float* Kb(const DevLevel& L) { return L.Kb32; }
+    __device__ static const float* St(const DevLevel& L) { return L.St32; }
+    __device__ static const float* invD(const DevLevel& L) { return L.invD32; }
+};
+
+__device__ __forceinline__ bool stopped(const int* stop) { return stop && __builtin_nontemporal_load(stop) != 0; }
+
+// ------------------------------------------------------------------------------------------------
+// reductions
+
+__device__ __forceinline__ double waveSum(double v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// sums v[0..NV) over the block; the result is valid in thread 0
+template <int NV> __device__ __forceinline__ void blockSum(double (&v)[NV]) {
+    __shared__ double sh[NV][BLOCK / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < NV; k++) v[k] = waveSum(v[k]);
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < NV; k++) sh[k][wid] = v[k];
+    __syncthreads();
+    if (wid == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; k++) v[k] = waveSum(lane < BLOCK / 64 ? sh[k][lane] : 0.0);
+    }
+    __syncthreads();
+}
+
+// block partials -> the last block sums them in block order and calls fin(total) in thread 0
+template <int NV, class Fin> __device__ __forceinline__ void gridSum(double (&v)[NV], Reduce red, Fin fin) {
+    blockSum<NV>(v);
+    __shared__ int isLast;
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; k++) red.partials[k * RED_BLOCKS + blockIdx.x] = v[k];
+        __threadfence();
+        const unsigned t = atomicAdd(red.counter, 1u);
+        isLast = t == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!isLast) return;
+    __threadfence();
+    double s[NV];
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+        s[k] = 0;
+        for (unsigned b = threadIdx.x; b < gridDim.x; b += BLOCK)
+            s[k] += __hip_atomic_load(red.partials + k * RED_BLOCKS + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    blockSum<NV>(s);
+    if (threadIdx.x == 0) {
+        fin(s);
+        __hip_atomic_store(red.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+inline unsigned reduceGrid(int64_t n) {
+    const int64_t g = (n + 8 * BLOCK - 1) / (8 * BLOCK);
+    return unsigned(g < 1 ? 1 : (g > RED_BLOCKS ? RED_BLOCKS : g));
+}
+inline unsigned flatGrid(int64_t n) { return unsigned((n + BLOCK - 1) / BLOCK); }
+
+// ------------------------------------------------------------------------------------------------
+// operator
+
+// acc = (K x) at node n = (i, j, k)
+template <class T>
+__device__ __forceinline__ void applyNode(const DevLevel& L, const T* __restrict__ x, int i, int j, int k, int64_t n, T acc[3]) {
+    acc[0] = acc[1] = acc[2] = T(0);
+    const int64_t NX = L.NX, NXY = int64_t(L.NX) * L.NY;
+    const float s0 = L.nodeScale[n];
+    if (s0 > 0.f) {
+        const T* __restrict__ S = Prec<T>::St(L);
+#pragma unroll
+        for (int m = 0; m < 27; m++) {
+            const int ox = m % 3 - 1, oy = (m / 3) % 3 - 1, oz = m / 9 - 1;
+            const T* xm = x + 3 * (n + ox + oy * NX + oz * NXY);
+            const T x0 = xm[0], x1 = xm[1], x2 = xm[2];
+#pragma unroll
+            for (int d = 0; d < 3; d++) acc[d] += S[m * 9 + d * 3] * x0 + S[m * 9 + d * 3 + 1] * x1 + S[m * 9 + d * 3 + 2] * x2;
+        }
+        const T s = T(s0);
+        acc[0] *= s;
+        acc[1] *= s;
+        acc[2] *= s;
+    } else {
+#pragma unroll
+        for (int v = 0; v < 8; v++) {
+            const int di = v & 1, dj = (v >> 1) & 1, dk = v >> 2;
+            const int ei = i - 1 + di, ej = j - 1 + dj, ek = k - 1 + dk;
+            if (ei < 0 || ej < 0 || ek < 0 || ei >= L.nx || ej >= L.ny || ek >= L.nz) continue;
+            const int32_t w = L.words[ei + int64_t(L.nx) * (ej + int64_t(L.ny) * ek)];
+            if (w == 0) continue;
+            const int a = d_corner(1 - di, 1 - dj, 1 - dk);
+            const int64_t nb = n - (1 - di) - (1 - dj) * NX - (1 - dk) * NXY;  // the voxel's corner 0
+            T xe[24];
+#pragma unroll
+            for (int b = 0; b < 8; b++) {
+                const T* xm = x + 3 * (nb + kCX(b) + kCY(b) * NX + kCZ(b) * NXY);
+                xe[3 * b] = xm[0];
+                xe[3 * b + 1] = xm[1];
+                xe[3 * b + 2] = xm[2];
+            }
+            T b0 = 0, b1 = 0, b2 = 0;
+            if (w > 0) {
+                const T* __restrict__ K = Prec<T>::Kb(L) + 72 * a;
+#pragma unroll
+                for (int c = 0; c < 24; c++) {
+                    b0 += K[c] * xe[c];
+                    b1 += K[24 + c] * xe[c];
+                    b2 += K[48 + c] * xe[c];
+                }
+                const T s = T(__int_as_float(w));
+                acc[0] += s * b0;
+                acc[1] += s * b1;
+                acc[2] += s * b2;
+            } else {
+                const float4* __restrict__ K4 = reinterpret_cast<const float4*>(L.Kown + int64_t(w & 0x7fffffff) * 576 + 72 * a);
+#pragma unroll
+                for (int q = 0; q < 6; q++) {
+                    const float4 r0 = K4[q], r1 = K4[6 + q], r2 = K4[12 + q];
+                    b0 += T(r0.x) * xe[4 * q] + T(r0.y) * xe[4 * q + 1] + T(r0.z) * xe[4 * q + 2] + T(r0.w) * xe[4 * q + 3];
+                    b1 += T(r1.x) * xe[4 * q] + T(r1.y) * xe[4 * q + 1] + T(r1.z) * xe[4 * q + 2] + T(r1.w) * xe[4 * q + 3];
+                    b2 += T(r2.x) * xe[4 * q] + T(r2.y) * xe[4 * q + 1] + T(r2.z) * xe[4 * q + 2] + T(r2.w) * xe[4 * q + 3];
+                }
+                acc[0] += b0;
+                acc[1] += b1;
+                acc[2] += b2;
+            }
+        }
+    }
+    if (L.diagAdd) {
+#pragma unroll
+        for (int d = 0; d < 3; d++) acc[d] += T(L.diagAdd[3 * n + d]) * x[3 * n + d];
+    }
+}
+
+template <class T, int MODE>
+__global__ __launch_bounds__(BLOCK) void kApply(DevLevel L, const T* __restrict__ x, T* __restrict__ y, const T* __restrict__ r, T* __restrict__ dv,
+                                                 T c1, T c2, const int* stop) {
+    if (stopped(stop)) return;
+    const int64_t n = int64_t(blockIdx.x) * BLOCK + threadIdx.x;
+    if (n >= L.nNodes) return;
+    const unsigned nn = unsigned(n), NX = unsigned(L.NX), NY = unsigned(L.NY);
+    const int i = int(nn % NX), j = int((nn / NX) % NY), k = int(nn / (NX * NY));
+    T acc[3];
+    applyNode<T>(L, x, i, j, k, n, acc);
+    const T* __restrict__ invD = Prec<T>::invD(L);
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const int64_t q = 3 * n + d;
+        const T id = invD[q];
+        if constexpr (MODE == AM_APPLY) y[q] = id != T(0) ? acc[d] : T(0);
+        else if constexpr (MODE == AM_RAW) y[q] = acc[d];
+        else if constexpr (MODE == AM_RESID) y[q] = id != T(0) ? r[q] - acc[d] : T(0);
+        else if constexpr (MODE == AM_CHEB1) {
+            const T dn = c1 * id * (r[q] - acc[d]);
+            dv[q] = dn;
+            y[q] = x[q] + dn;
+        } else {
+            const T dn = c1 * dv[q] + c2 * id * (r[q] - acc[d]);
+            dv[q] = dn;
+            y[q] = x[q] + dn;
+        }
+    }
+}
+
+template <class T> __global__ __launch_bounds__(BLOCK) void kChebFirst(const T* __restrict__ invD, const T* __restrict__ r, T* __restrict__ z, T* __restrict__ d,
+                                                                       T c0, int64_t n, const int* stop) {
+    if (stopped(stop)) return;
+    const int64_t q = int64_t(blockIdx.x) * BLOCK + threadIdx.x;
+    if (q >= n) return;
+    const T v = c0 * invD[q] * r[q];
+    d[q] = v;
+    z[q] = v;
+}
+
+// ------------------------------------------------------------------------------------------------
+// grid transfers (voxel_fea.cpp:738-815): full weighting P^T and trilinear P, coarse node I <-> fine 2I
+
+template <class T>
+__global__ __launch_bounds__(BLOCK) void kRestrict(DevLevel F, DevLevel C, const T* __restrict__ rf, T* __restrict__ rc, const int* stop) {
+    if (stopped(stop)) return;
+    const int64_t cn = int64_t(blockIdx.x) * BLOCK + threadIdx.x;
+    if (cn >= C.nNodes) return;
+    const unsigned u = unsigned(cn), CNX = unsigned(C.NX), CNY = unsigned(C.NY);
+    const int I = int(u % CNX), J = int((u / CNX) % CNY), K = int(u / (CNX * CNY));
+    const T* __restrict__ cinv = Prec<T>::invD(C);
+    const T i0 = cinv[3 * cn], i1 = cinv[3 * cn + 1], i2 = cinv[3 * cn + 2];
+    T s0 = 0, s1 = 0, s2 = 0;
+    if (i0 != T(0) || i1 != T(0) || i2 != T(0)) {
+        for (int dk = -1; dk <= 1; dk++) {
+            const int kf = 2 * K + dk;
+            if (kf < 0 || kf >= F.NZ) continue;
+            const T wk = dk == 0 ? T(1) : T(0.5);
+            for (int dj = -1; dj <= 1; dj++) {
+                const int jf = 2 * J + dj;
+                if (jf < 0 || jf >= F.NY) continue;
+                const T wj = dj == 0 ? wk : T(0.5) * wk;
+#pragma unroll
+                for (int di = -1; di <= 1; di++) {
+                    const int fi = 2 * I + di;
+                    if (fi < 0 || fi >= F.NX) continue;
+                    const T w = di == 0 ? wj : T(0.5) * wj;
+                    const T* p = rf + 3 * (fi + int64_t(F.NX) * (jf + int64_t(F.NY) * kf));
+                    s0 += w * p[0];
+                    s1 += w * p[1];
+                    s2 += w * p[2];
+                }
+            }
+        }
+    }
+    rc[3 * cn] = i0 != T(0) ? s0 : T(0);
+    rc[3 * cn + 1] = i1 != T(0) ? s1 : T(0);
+    rc[3 * cn + 2] = i2 != T(0) ? s2 : T(0);
+}
+
+template <class T>
+__global__ __launch_bounds__(BLOCK) void kProlong(DevLevel F, DevLevel C, const T* __restrict__ zc, T* __restrict__ zf, const int* stop) {
+    if (stopped(stop)) return;
+    const int64_t fn = int64_t(blockIdx.x) * BLOCK + threadIdx.x;
+    if (fn >= F.nNodes) return;
+    const T* __restrict__ finv = Prec<T>::invD(F);
+    const bool f0 = finv[3 * fn] != T(0), f1 = finv[3 * fn + 1] != T(0), f2 = finv[3 * fn + 2] != T(0);
+    if (!(f0 || f1 || f2)) return;
+    const unsigned u = unsigned(fn), FNX = unsigned(F.NX), FNY = unsigned(F.NY);
+    const int i = int(u % FNX), j = int((u / FNX) % FNY), k = int(u / (FNX * FNY));
+    const int ni = (i & 1) ? 2 : 1, nj = (j & 1) ? 2 : 1, nk = (k & 1) ? 2 : 1;
+    const T w = T(1.0 / (ni * nj * nk));
+    T s0 = 0, s1 = 0, s2 = 0;
+    for (int c = 0; c < nk; c++) {
+        const int K = (k >> 1) + c;
+        for (int b = 0; b < nj; b++) {
+            const int J = (j >> 1) + b;
+            for (int a = 0; a < ni; a++) {
+                const int I = (i >> 1) + a;
+                const T* p = zc + 3 * (I + int64_t(C.NX) * (J + int64_t(C.NY) * K));
+                s0 += w * p[0];
+                s1 += w * p[1];
+                s2 += w * p[2];
+            }
+        }
+    }
+    if (f0) zf[3 * fn] += s0;
+    if (f1) zf[3 * fn + 1] += s1;
+    if (f2) zf[3 * fn + 2] += s2;
+}
+
+// z[freeIdx[i]] = sum_j Ainv[i][j] r[freeIdx[j]]: r gathered into LDS, one wavefront per row
+constexpr int COARSE_MAX = 3072;
+template <class T>
+__global__ __launch_bounds__(BLOCK) void kCoarse(const int32_t* __restrict__ freeIdx, int m, const double* __restrict__ Ainv, const T* __restrict__ r,
+                                                 T* __restrict__ z, const int* stop) {
+    if (stopped(stop)) return;
+    __shared__ double rl[COARSE_MAX];
+    for (int q = threadIdx.x; q < m; q += BLOCK) rl[q] = double(r[freeIdx[q]]);
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int rowsPerBlock = 16;
+    for (int row = blockIdx.x * rowsPerBlock + wid; row < min(m, int(blockIdx.x + 1) * rowsPerBlock); row += BLOCK / 64) {
+        const double* a = Ainv + int64_t(row) * m;
+        double s = 0;
+        for (int q = lane; q < m; q += 64) s += a[q] * rl[q];
+        s = waveSum(s);
+        if (lane == 0) z[freeIdx[row]] = T(s);
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// conjugate-gradient vector steps
+
+template <class T>
+__global__ __launch_bounds__(BLOCK) void kDotPQ(const T* __restrict__ p, const T* __restrict__ q, int64_t n, CgState* st, Reduce red) {
+    if (st->stop) return;
+    double v[1] = {0};
+    for (int64_t i = int64_t(blockIdx.x) * BLOCK + threadIdx.x; i < n; i += int64_t(gridDim.x) * BLOCK) v[0] += double(p[i]) * double(q[i]);
+    gridSum<1>(v, red, [&](const double* s) {
+        st->pq = s[0];
+        if (!(s[0] > 0)) {  // loss of positive-definiteness (a mechanism)
+            st->broken = 1;
+            st->stop = 1;
+        } else st->alpha = st->rz / s[0];
+    });
+}
+
+template <class T>
+__global__ __launch_bounds__(BLOCK) void kUpdateXR(T* __restrict__ x, const T* __restrict__ p, T* __restrict__ r, const T* __restrict__ q, int64_t n,
+                                                   CgState* st, Reduce red) {
+    if (st->stop) return;
+    const T a = T(st->alpha);
+    double v[1] = {0};
+    for (int64_t i = int64_t(blockIdx.x) * BLOCK + threadIdx.x; i < n; i += int64_t(gridDim.x) * BLOCK) {
+        x[i] += a * p[i];
+        const T ri = r[i] - a * q[i];
+        r[i] = ri;
+        v[0] += double(ri) * double(ri);
+    }
+    gridSum<1>(v, red, [&](const double* s) {
+        st->rr = s[0];
+        st->iters += 1;
+        st->sinceFold += 1;
+        st->rel = sqrt(s[0]) * st->scale / st->bnorm;
+        if (!(st->rel > st->tol) || st->iters >= st->maxIter || !isfinite(st->rel)) st->stop = 1;
+    });
+}
+
+template <class T>
+__global__ __launch_bounds__(BLOCK) void kDotsRZ(const T* __restrict__ r, const T* __restrict__ z, const T* __restrict__ q, int64_t n, CgState* st,
+                                                 Reduce red) {
+    if (st->stop) return;
+    double v[2] = {0, 0};
+    for (int64_t i = int64_t(blockIdx.x) * BLOCK + threadIdx.x; i < n; i += int64_t(gridDim.x) * BLOCK) {
+        const double zi = double(z[i]);
+        v[0] += double(r[i]) * zi;
+        v[1] += zi * double(q[i]);
+    }
+    gridSum<2>(v, red, [&](const double* s) {
+        st->rzNew = s[0];
+        st->zq = s[1];
+        const double b = st->rz != 0 ? -st->alpha * s[1] / st->rz : 0.0;
+        st->beta = b > 0 ? b : 0.0;
+        st->rz = s[0];
+    });
+}
+
+template <class T>
+__global__ __launch_bounds__(BLOCK) void kUpdateP(const T* __restrict__ z, T* __restrict__ p, int64_t n, const CgState* st) {
+    if (st->stop) return;
+    const int64_t i = int64_t(blockIdx.x) * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    p[i] = z[i] + T(st->beta) * p[i];
+}
+
+__global__ __launch_bounds__(BLOCK) void kResidual64(const double* __restrict__ f, const double* __restrict__ Kx, const double* __restrict__ invD,
+                                                     double* __restrict__ r, int64_t n, double* out, Reduce red) {
+    double v[1] = {0};
+    for (int64_t i = int64_t(blockIdx.x) * BLOCK + threadIdx.x; i < n; i += int64_t(gridDim.x) * BLOCK) {
+        const double ri = invD[i] != 0 ? f[i] - Kx[i] : 0.0;
+        r[i] = ri;
+        v[0] += ri * ri;
+    }
+    gridSum<1>(v, red, [&](const double* s) { *out = s[0]; });
+}
+
+__global__ __launch_bounds__(BLOCK) void kNormFree64(const double* __restrict__ f, const double* __restrict__ invD, int64_t n, double* out, Reduce red) {
+    double v[1] = {0};
+    for (int64_t i = int64_t(blockIdx.x) * BLOCK + threadIdx.x; i < n; i += int64_t(gridDim.x) * BLOCK)
+        if (invD[i] != 0) v[0] += f[i] * f[i];
+    gridSum<1>(v, red, [&](const double* s) { *out = s[0]; });
+}
+
+__global__ __launch_bounds__(BLOCK) void kMaskCopy64(const double* __restrict__ x0, const double* __restrict__ invD, double* __restrict__ x, int64_t n) {
+    const int64_t i = int64_t(blockIdx.x) * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    x[i] = (x0 && invD[i] != 0) ? x0[i] : 0.0;
+}
+
+__global__ __launch_bounds__(BLOCK) void kMaxAbs64(const double* __restrict__ v, int64_t n, double* out, Reduce red) {
+    double m = 0;
+    for (int64_t i = int64_t(blockIdx.x) * BLOCK + threadIdx.x; i < n; i += int64_t(gridDim.x) * BLOCK) m = fmax(m, fabs(v[i]));
+    // max through the sum machinery: the block maxima are combined in thread 0 of the last block
+    __shared__ double sh[BLOCK / 64];
+    for (int o = 32; o >= 1; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+    __syncthreads();
+    __shared__ int isLast;
+    if (threadIdx.x == 0) {
+        double b = sh[0];
+        for (int w = 1; w < BLOCK / 64; w++) b = fmax(b, sh[w]);
+        red.partials[blockIdx.x] = b;
+        __threadfence();
+        isLast = atomicAdd(red.counter, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (isLast && threadIdx.x == 0) {
+        __threadfence();
+        double b = 0;
+        for (unsigned q = 0; q < gridDim.x; q++) b = fmax(b, __hip_atomic_load(red.partials + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        *out = b;
+        __hip_atomic_store(red.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void kToF32(const double* __restrict__ x, float* __restrict__ y, int64_t n, const double* scaleFrom) {
+    const int64_t i = int64_t(blockIdx.x) * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const double s = *scaleFrom;
+    y[i] = s > 0 ? float(x[i] * (1.0 / s)) : 0.f;
+}
+
+__global__ __launch_bounds__(BLOCK) void kFromF32(const float* __restrict__ z, double* __restrict__ y, int64_t n, const double* scaleFrom) {
+    const int64_t i = int64_t(blockIdx.x) * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    y[i] = double(z[i]) * *scaleFrom;
+}
+
+// Lanczos on D^-1/2 K D^-1/2
+__global__ __launch_bounds__(BLOCK) void kLzX(const double* __restrict__ v, const double* __restrict__ invD, double* __restrict__ x, int64_t n) {
+    const int64_t i = int64_t(blockIdx.x) * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    x[i] = sqrt(invD[i]) * v[i];
+}
+
+__global__ __launch_bounds__(BLOCK) void kLzW(double* __restrict__ w, const double* __restrict__ invD, const double* __restrict__ vPrev,
+                                              const double* __restrict__ v, int64_t n, CgState* st, int j, Reduce red) {
+    const double b = j > 0 ? st->lz[16 + j - 1] : 0.0;
+    double s[1] = {0};
+    for (int64_t i = int64_t(blockIdx.x) * BLOCK + threadIdx.x; i < n; i += int64_t(gridDim.x) * BLOCK) {
+        const double wi = sqrt(invD[i]) * w[i] - b * vPrev[i];
+        w[i] = wi;
+        s[0] += wi * v[i];
+    }
+    gridSum<1>(s, red, [&](const double* t) { st->lz[j] = t[0]; });
+}
+
+__global__ __launch_bounds__(BLOCK) void kLzAxpy(double* __restrict__ w, const double* __restrict__ v, int64_t n, CgState* st, int j, Reduce red) {
+    const double a = st->lz[j];
+    double s[1] = {0};
+    for (int64_t i = int64_t(blockIdx.x) * BLOCK + threadIdx.x; i < n; i += int64_t(gridDim.x) * BLOCK) {
+        const double wi = w[i] - a * v[i];
+        w[i] = wi;
+        s[0] += wi * wi;
+    }
+    gridSum<1>(s, red, [&](const double* t) { st->lz[16 + j] = sqrt(t[0]); });
+}
+
+__global__ __launch_bounds__(BLOCK) void kLzNext(const double* __restrict__ w, double* __restrict__ v, double* __restrict__ vPrev, int64_t n,
+                                                 const CgState* st, int j) {
+    const int64_t i = int64_t(blockIdx.x) * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const double b = st->lz[16 + j];
+    const double vi = v[i];
+    vPrev[i] = vi;
+    v[i] = b > 0 ? w[i] / b : 0.0;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------------------
+// launchers
+
+template <class T> void launchApply(const DevLevel& L, int mode, const T* x, T* y, const T* r, T* d, double c1, double c2, const int* stop, hipStream_t s) {
+    const dim3 g(flatGrid(L.nNodes)), b(BLOCK);
+    switch (mode) {
+        case AM_APPLY: kApply<T, AM_APPLY><<<g, b, 0, s>>>(L, x, y, r, d, T(c1), T(c2), stop); break;
+        case AM_RAW: kApply<T, AM_RAW><<<g, b, 0, s>>>(L, x, y, r, d, T(c1), T(c2), stop); break;
+        case AM_RESID: kApply<T, AM_RESID><<<g, b, 0, s>>>(L, x, y, r, d, T(c1), T(c2), stop); break;
+        case AM_CHEB1: kApply<T, AM_CHEB1><<<g, b, 0, s>>>(L, x, y, r, d, T(c1), T(c2), stop); break;
+        default: kApply<T, AM_CHEB2><<<g, b, 0, s>>>(L, x, y, r, d, T(c1), T(c2), stop); break;
+    }
+}
+
+template <class T> void launchChebFirst(const DevLevel& L, const T* r, T* z, T* d, double c0, const int* stop, hipStream_t s) {
+    const T* invD = sizeof(T) == 8 ? reinterpret_cast<const T*>(L.invD64) : reinterpret_cast<const T*>(L.invD32);
+    kChebFirst<T><<<flatGrid(L.nDof), BLOCK, 0, s>>>(invD, r, z, d, T(c0), L.nDof, stop);
+}
+
+template <class T> void launchRestrict(const DevLevel& F, const DevLevel& C, const T* rf, T* rc, const int* stop, hipStream_t s) {
+    kRestrict<T><<<flatGrid(C.nNodes), BLOCK, 0, s>>>(F, C, rf, rc, stop);
+}
+
+template <class T> void launchProlong(const DevLevel& F, const DevLevel& C, const T* zc, T* zf, const int* stop, hipStream_t s) {
+    kProlong<T><<<flatGrid(F.nNodes), BLOCK, 0, s>>>(F, C, zc, zf, stop);
+}
+
+template <class T> void launchCoarse(const int32_t* freeIdx, int64_t m, const double* Ainv, const T* r, T* z, const int* stop, hipStream_t s) {
+    kCoarse<T><<<unsigned((m + 15) / 16), BLOCK, 0, s>>>(freeIdx, int(m), Ainv, r, z, stop);
+}
+
+template <class T> void launchDotPQ(const T* p, const T* q, int64_t n, CgState* st, Reduce red, hipStream_t s) {
+    kDotPQ<T><<<reduceGrid(n), BLOCK, 0, s>>>(p, q, n, st, red);
+}
+template <class T> void launchUpdateXR(T* x, const T* p, T* r, const T* q, int64_t n, CgState* st, Reduce red, hipStream_t s) {
+    kUpdateXR<T><<<reduceGrid(n), BLOCK, 0, s>>>(x, p, r, q, n, st, red);
+}
+template <class T> void launchDotsRZ(const T* r, const T* z, const T* q, int64_t n, CgState* st, Reduce red, hipStream_t s) {
+    kDotsRZ<T><<<reduceGrid(n), BLOCK, 0, s>>>(r, z, q, n, st, red);
+}
+template <class T> void launchUpdateP(const T* z, T* p, int64_t n, CgState* st, hipStream_t s) { kUpdateP<T><<<flatGrid(n), BLOCK, 0, s>>>(z, p, n, st); }
+
+void launchResidual64(const double* f, const double* Kx, const double* invD, double* r, int64_t n, double* out, Reduce red, hipStream_t s) {
+    kResidual64<<<reduceGrid(n), BLOCK, 0, s>>>(f, Kx, invD, r, n, out, red);
+}
+void launchNormFree64(const double* f, const double* invD, int64_t n, double* out, Reduce red, hipStream_t s) {
+    kNormFree64<<<reduceGrid(n), BLOCK, 0, s>>>(f, invD, n, out, red);
+}
+void launchMaskCopy64(const double* x0, const double* invD, double* x, int64_t n, hipStream_t s) { kMaskCopy64<<<flatGrid(n), BLOCK, 0, s>>>(x0, invD, x, n); }
+void launchMaxAbs64(const double* v, int64_t n, double* out, Reduce red, hipStream_t s) { kMaxAbs64<<<reduceGrid(n), BLOCK, 0, s>>>(v, n, out, red); }
+void launchToF32(const double* x, float* y, int64_t n, const double* scaleFrom, hipStream_t s) { kToF32<<<flatGrid(n), BLOCK, 0, s>>>(x, y, n, scaleFrom); }
+void launchFromF32(const float* z, double* y, int64_t n, const double* scaleFrom, hipStream_t s) { kFromF32<<<flatGrid(n), BLOCK, 0, s>>>(z, y, n, scaleFrom); }
+void launchLzX(const double* v, const double* invD, double* x, int64_t n, hipStream_t s) { kLzX<<<flatGrid(n), BLOCK, 0, s>>>(v, invD, x, n); }
+void launchLzW(double* w, const double* invD, const double* vPrev, const double* v, int64_t n, CgState* st, int j, Reduce red, hipStream_t s) {
+    kLzW<<<reduceGrid(n), BLOCK, 0, s>>>(w, invD, vPrev, v, n, st, j, red);
+}
+void launchLzAxpy(double* w, const double* v, int64_t n, CgState* st, int j, Reduce red, hipStream_t s) {
+    kLzAxpy<<<reduceGrid(n), BLOCK, 0, s>>>(w, v, n, st, j, red);
+}
+void launchLzNext(const double* w, double* v, double* vPrev, int64_t n, CgState* st, int j, hipStream_t s) {
+    kLzNext<<<flatGrid(n), BLOCK, 0, s>>>(w, v, vPrev, n, st, j);
+}
+
+#define PSX_INSTANTIATE(T)                                                                                                              \
+    template void launchApply<T>(const DevLevel&, int, const T*, T*, const T*, T*, double, double, const int*, hipStream_t);            \
+    template void launchChebFirst<T>(const DevLevel&, const T*, T*, T*, double, const int*, hipStream_t);                               \
+    template void launchRestrict<T>(const DevLevel&, const DevLevel&, const T*, T*, const int*, hipStream_t);                          \
+    template void launchProlong<T>(const DevLevel&, const DevLevel&, const T*, T*, const int*, hipStream_t);                           \
+    template void launchCoarse<T>(const int32_t*, int64_t, const double*, const T*, T*, const int*, hipStream_t);                      \
+    template void launchDotPQ<T>(const T*, const T*, int64_t, CgState*, Reduce, hipStream_t);                                         \
+    template void launchUpdateXR<T>(T*, const T*, T*, const T*, int64_t, CgState*, Reduce, hipStream_t);                              \
+    template void launchDotsRZ<T>(const T*, const T*, const T*, int64_t, CgState*, Reduce, hipStream_t);                              \
+    template void launchUpdateP<T>(const T*, T*, int64_t, CgState*, hipStream_t);
+PSX_INSTANTIATE(double)
+PSX_INSTANTIATE(float)
+
+}  // namespace psx

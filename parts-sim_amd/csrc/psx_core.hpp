@@ -1,0 +1,78 @@
+// Host-side model of the voxel hierarchy: the level data the device kernels read, built on the CPU
+// from the solver's inputs (dims, density, nu, held DOFs, optional diagonal shift).
+//
+// Level contents follow the reference's Level (native/src/fea/voxel_fea.hpp:37-78): element list,
+// uniform element matrix Kb (K0 on the finest level, then the exact Galerkin product of 8 uniform
+// children), own 24x24 matrices for the coarse elements that are not s * Kb, held DOFs, inverse
+// diagonal, and the per-node stencil scale. The device form packs an element's kind into one
+// 32-bit word per voxel: 0 empty, the float bits of s for s * Kb, or 0x80000000 | own index.
+#pragma once
+
+#include <array>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace psx {
+
+using i64 = int64_t;
+
+// element corner at unit offset (x, y, z): bottom face counter-clockwise, then the top face
+// (native/src/fea/hex8.hpp:14, voxel_fea.cpp:52)
+constexpr int corner(int x, int y, int z) { return (y ? (x ? 2 : 3) : (x ? 1 : 0)) + 4 * z; }
+constexpr int CX[8] = {0, 1, 1, 0, 0, 1, 1, 0}, CY[8] = {0, 0, 1, 1, 0, 0, 1, 1}, CZ[8] = {0, 0, 0, 0, 1, 1, 1, 1};
+
+struct Element {
+    double K[576];          // condensed 24x24 stiffness of the unit cube, E = 1
+    double corner[8][144];  // D * Bbar at each corner (6 x 24), for stresses
+};
+// Wilson incompatible-mode hex8 on the unit cube (native/src/fea/hex8.cpp:104-155); throws for nu
+// outside (-1, 0.5)
+Element hexElement(double nu);
+
+struct Level {
+    int nx = 0, ny = 0, nz = 0, NX = 0, NY = 0, NZ = 0;
+    i64 nNodes = 0, nDof = 0, nVox = 0, freeDof = 0, nOwn = 0;
+    std::vector<int32_t> emap;      // voxel -> element, -1 empty
+    std::vector<int32_t> elemVox;   // element -> voxel (increasing)
+    std::vector<double> rho;        // element -> s for s * Kb (0 for own-matrix elements)
+    std::vector<int32_t> own;       // element -> own matrix index, -1 for s * Kb
+    std::vector<float> K;           // own matrices, 576 each (row-major)
+    double Kb[576]{};
+    std::vector<uint8_t> bc, fixed, active;
+    std::vector<double> invDiag, diagAdd;
+    std::vector<float> nodeScale;   // s when the node's 8 voxels are all s * Kb, else 0
+    std::vector<int32_t> words;     // device voxel words
+    double lmax = 0;
+    i64 node(int i, int j, int k) const { return i + i64(NX) * (j + i64(NY) * k); }
+};
+
+struct Hierarchy {
+    double nu = 0.3;
+    Element el;
+    std::vector<Level> levels;
+    i64 m = 0;                       // free DOFs of the coarsest level
+    std::vector<int32_t> freeIdx;    // dense index -> coarsest DOF
+    std::vector<double> Ainv;        // m x m inverse of the coarsest free-DOF matrix (row-major)
+};
+
+// Builds the hierarchy (voxel_fea.cpp:138-183): coarsens until the free DOFs are at most
+// coarsestMaxDof, 12 levels, or a single voxel; then factors the coarsest level. Throws
+// std::invalid_argument on bad input and std::runtime_error when the coarsest level has no direct
+// solve (0 or more than 3000 free DOFs).
+void buildHierarchy(Hierarchy& H, int nx, int ny, int nz, const float* density, double nu, const uint8_t* bc, i64 coarsestMaxDof,
+                    const double* diagAdd, int maxLevels);
+
+// The 27-point stencil of a uniform node for element matrix K: S[m][d][c] couples DOF d of the node
+// with DOF c of neighbour m = ox + 3 oy + 9 oz (offsets -1..1 shifted to 0..2).
+void stencil(const double* K, double* S243);
+
+// Lanczos start vector of a level (an LCG with seed 12345, fixed DOFs 0; voxel_fea.cpp:584-588).
+std::vector<double> lanczosStart(const Level& L);
+// largest eigenvalue of a symmetric tridiagonal (Sturm-count bisection; voxel_fea.cpp:506-530)
+double tridiagonalMax(const std::vector<double>& a, const std::vector<double>& b);
+
+// number of worker threads for host-side builds
+int hostThreads();
+
+}  // namespace psx

@@ -1,0 +1,39 @@
+"""The C-ABI library loads and exports every entry point include/psx.h declares; without a GPU the
+device calls fail loudly (no CPU fallback)."""
+import os
+import re
+
+import pytest
+
+from conftest import ROOT
+
+
+def declared():
+    src = open(os.path.join(ROOT, "include", "psx.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(psx_[a-z0-9_]+)\s*\(", src)) - {"psx_progress_fn"})
+
+
+def test_every_declared_symbol_is_exported(P):
+    L = P.lib()
+    names = declared()
+    assert len(names) >= 20
+    for name in names:
+        assert hasattr(L, name), name
+    assert set(names) == set(P.SIGNATURES), set(names) ^ set(P.SIGNATURES)
+
+
+def test_no_device_fails_loudly(P, W):
+    if P.available():
+        pytest.skip("a GPU is present")
+    w = W.cantilever(2)
+    with pytest.raises(P.PsxError) as e:
+        P.Model(w["dims"], w["density"], w["nu"], w["bc"])
+    assert e.value.code == P.ERR_NO_DEVICE
+
+
+def test_missing_library_raises(P, monkeypatch):
+    monkeypatch.setattr(P, "_lib", None)
+    monkeypatch.setattr(P, "LIB_PATH", "/nonexistent/libpsx.so")
+    with pytest.raises(RuntimeError):
+        P.lib()
