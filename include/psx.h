@@ -91,6 +91,8 @@ typedef struct {
 const char* psx_last_error(void);
 /* 1 when a GPU is usable (the reference's gpuAvailable(), gpu.cpp:250-257). */
 int psx_available(void);
+/* Name of the current HIP device (the reference's gpuName(), gpu.hpp:26). */
+int psx_device_name(char* buf, size_t len);
 
 /* Builds the hierarchy and uploads it (VoxelFEA ctor voxel_fea.cpp:138-183 + GpuFeaSolver ctor
  * gpu_fea.cpp:252-423). */
@@ -108,6 +110,11 @@ int psx_set_lmax(psx_model m, int level, double lmax);
 int psx_apply(psx_model m, int level, const double* x, double* y, int raw);
 /* The same in FP32 (VoxelFEA::applyT<float>, voxel_fea.cpp:393). */
 int psx_apply32(psx_model m, int level, const float* x, float* y, int raw);
+
+/* y = K x with the BF16 matrix-core operator the mixed-precision V-cycle smooths with (levels
+ * without own element matrices; FP32 vectors, BF16 products, FP32 accumulation). No reference
+ * counterpart: the reference's smoother operator is FP32 (voxel_fea.cpp:393, fea.wgsl:173-206). */
+int psx_apply_bf16(psx_model m, int level, const float* x, float* y);
 
 /* One V-cycle z ~ K^-1 r, FP32 with r scaled to order one (GpuFeaSolver::precondition,
  * gpu_fea.cpp:431-450; VoxelFEA::precondition voxel_fea.cpp:830-861). */

@@ -11,7 +11,7 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libpsx.so")
+LIB_PATH = os.environ.get("PSX_LIB") or os.path.join(HERE, "libpsx.so")
 
 PREC = {"fp64": 0, "ref": 1, "mixed": 2}
 OK, ERR_ARGUMENT, ERR_NO_DEVICE, ERR_NO_COARSE, ERR_DEVICE, ERR_INTERNAL = range(6)
@@ -51,6 +51,7 @@ _vp, _i, _d, _i64 = C.c_void_p, C.c_int, C.c_double, C.c_int64
 SIGNATURES = {
     "psx_last_error": (C.c_char_p, []),
     "psx_available": (_i, []),
+    "psx_device_name": (_i, [C.c_char_p, C.c_size_t]),
     "psx_create": (_i, [C.POINTER(ModelDesc), C.POINTER(_vp)]),
     "psx_destroy": (None, [_vp]),
     "psx_num_levels": (_i, [_vp]),
@@ -59,6 +60,7 @@ SIGNATURES = {
     "psx_set_lmax": (_i, [_vp, _i, _d]),
     "psx_apply": (_i, [_vp, _i, _vp, _vp, _i]),
     "psx_apply32": (_i, [_vp, _i, _vp, _vp, _i]),
+    "psx_apply_bf16": (_i, [_vp, _i, _vp, _vp]),
     "psx_precondition": (_i, [_vp, _vp, _vp]),
     "psx_precondition_device": (_i, [_vp, _vp, _vp]),
     "psx_solve": (_i, [_vp, _vp, _vp, C.POINTER(SolveOpts), _vp, C.POINTER(SolveInfo)]),
@@ -86,6 +88,8 @@ def lib():
             raise RuntimeError(LIB_PATH + " is not built: run make -C parts-sim_amd (or __graft_entry__.build())")
         L = C.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
+            if not hasattr(L, name):  # an older build: the entry point raises when called
+                continue
             fn = getattr(L, name)
             fn.restype = res
             fn.argtypes = args
@@ -104,6 +108,12 @@ def _p(a):
 
 def available():
     return bool(lib().psx_available())
+
+
+def device_name():
+    buf = C.create_string_buffer(256)
+    _check(lib().psx_device_name(buf, len(buf)))
+    return buf.value.decode()
 
 
 def _desc(dims, density, nu, bc, coarsest_max_dof, diag_add, max_levels, keep):
@@ -221,6 +231,12 @@ class Model:
         x = np.ascontiguousarray(x, np.float32)
         y = np.zeros_like(x)
         _check(lib().psx_apply32(self.h, int(level), _p(x), _p(y), int(raw)))
+        return y
+
+    def apply_bf16(self, x, level=0):
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.zeros_like(x)
+        _check(lib().psx_apply_bf16(self.h, int(level), _p(x), _p(y)))
         return y
 
     def precondition(self, r):

@@ -33,6 +33,7 @@ struct CgState {
     double rz, pq, alpha, beta, rr, rzNew, zq, bnorm, tol, rel;
     double rmax, scale, peak, trueRel, spare[6];
     int stop, iters, maxIter, broken, cancel, folds, sinceFold, foldEvery;
+    int noFold, pad[3];
     double lz[32];             // Lanczos alpha[0..9], beta at [16..25]
 };
 
@@ -52,14 +53,24 @@ enum ApplyMode { AM_APPLY = 0, AM_RAW = 1, AM_RESID = 2, AM_CHEB1 = 3, AM_CHEB2 
 //   AM_CHEB2  d = c1 d + c2 D^-1 (r - K x); y = x + d
 template <class T> void launchApply(const DevLevel& L, int mode, const T* x, T* y, const T* r, T* d, double c1, double c2, const int* stop,
                                     hipStream_t s);
+// the same for levels without own matrices: z-marching LDS planes, stencil + packed per-voxel
+// sums (fine.hip)
+template <class T> void launchFine(const DevLevel& L, int mode, const T* x, T* y, const T* r, T* d, double c1, double c2, const int* stop,
+                                   hipStream_t s);
+// the same in BF16 on the matrix cores, FP32 vectors (mfma.hip); Aop from mfmaOperands()
+void launchFineMfma(const DevLevel& L, const void* Aop, int mode, const float* x, float* y, const float* r, float* d, double c1, double c2, const int* stop,
+                    hipStream_t s);
+size_t mfmaOperandBytes();
+void mfmaOperands(const double* Kb, uint16_t* out);
 // z = d = c0 D^-1 r (first Chebyshev step from zero)
 template <class T> void launchChebFirst(const DevLevel& L, const T* r, T* z, T* d, double c0, const int* stop, hipStream_t s);
 // rc = P^T rf on the coarse level's free DOFs
 template <class T> void launchRestrict(const DevLevel& F, const DevLevel& C, const T* rf, T* rc, const int* stop, hipStream_t s);
 // zf += P zc on the fine level's free DOFs
 template <class T> void launchProlong(const DevLevel& F, const DevLevel& C, const T* zc, T* zf, const int* stop, hipStream_t s);
-// z[freeIdx] = Ainv r[freeIdx] (coarsest level; other entries of z untouched)
-template <class T> void launchCoarse(const int32_t* freeIdx, int64_t m, const double* Ainv, const T* r, T* z, const int* stop, hipStream_t s);
+// coarsest level: z[freeIdx] = Ainv r[freeIdx], z = 0 on held DOFs (dense[dof] = dense index or -1)
+template <class T>
+void launchCoarse(const int32_t* freeIdx, const int32_t* dense, int64_t m, int64_t nDof, const double* Ainv, const T* r, T* z, const int* stop, hipStream_t s);
 
 // conjugate-gradient steps (all accumulate in FP64; T is the vector type)
 // pq = p.q; alpha = rz / pq (or broken + stop when pq <= 0)
@@ -70,6 +81,15 @@ template <class T> void launchUpdateXR(T* x, const T* p, T* r, const T* q, int64
 template <class T> void launchDotsRZ(const T* r, const T* z, const T* q, int64_t n, CgState* st, Reduce red, hipStream_t s);
 // p = z + beta p
 template <class T> void launchUpdateP(const T* z, T* p, int64_t n, CgState* st, hipStream_t s);
+
+// mixed precision: x += alpha p; r -= alpha q; rr; then decides whether to fold (reliable update):
+// fold when the estimate fell to a tenth of its peak since the last fold, every foldEvery
+// iterations, at the tolerance, or at the iteration limit (gpu_fea.cpp:496-557 for the rule)
+template <class T> void launchUpdateXRFold(T* x, const T* p, T* r, const T* q, int64_t n, CgState* st, Reduce red, hipStream_t s);
+// fold, part 1: u += x * rmax, x = 0 (only when a fold is pending)
+void launchFoldX(double* u, float* x, int64_t n, CgState* st, hipStream_t s);
+// fold, part 2 (after Kx = K u in FP64): r64 = f - Kx, r32 = r64 / rmax, true residual, stop test
+void launchFoldR(const double* f, const double* Kx, const double* invD, double* r64, float* r32, int64_t n, CgState* st, Reduce red, hipStream_t s);
 
 // r = f - Kx on free DOFs (Kx already masked), out[0] += |r|^2 over the grid (FP64)
 void launchResidual64(const double* f, const double* Kx, const double* invD, double* r, int64_t n, double* out, Reduce red, hipStream_t s);

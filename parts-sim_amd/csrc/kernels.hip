@@ -287,23 +287,29 @@ __global__ __launch_bounds__(BLOCK) void kProlong(DevLevel F, DevLevel C, const 
     if (f2) zf[3 * fn + 2] += s2;
 }
 
-// z[freeIdx[i]] = sum_j Ainv[i][j] r[freeIdx[j]]: r gathered into LDS, one wavefront per row
+// z = Ainv r on the coarsest level's free DOFs and 0 on every held one (the whole vector is
+// written, so nothing left in the buffer by another use leaks into the cycle): r gathered into LDS,
+// one wavefront per DOF
 constexpr int COARSE_MAX = 3072;
 template <class T>
-__global__ __launch_bounds__(BLOCK) void kCoarse(const int32_t* __restrict__ freeIdx, int m, const double* __restrict__ Ainv, const T* __restrict__ r,
-                                                 T* __restrict__ z, const int* stop) {
+__global__ __launch_bounds__(BLOCK) void kCoarse(const int32_t* __restrict__ freeIdx, const int32_t* __restrict__ dense, int m, int64_t nDof,
+                                                 const double* __restrict__ Ainv, const T* __restrict__ r, T* __restrict__ z, const int* stop) {
     if (stopped(stop)) return;
     __shared__ double rl[COARSE_MAX];
     for (int q = threadIdx.x; q < m; q += BLOCK) rl[q] = double(r[freeIdx[q]]);
     __syncthreads();
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    const int rowsPerBlock = 16;
-    for (int row = blockIdx.x * rowsPerBlock + wid; row < min(m, int(blockIdx.x + 1) * rowsPerBlock); row += BLOCK / 64) {
-        const double* a = Ainv + int64_t(row) * m;
+    constexpr int PER_BLOCK = 16;
+    const int64_t end = min(nDof, int64_t(blockIdx.x + 1) * PER_BLOCK);
+    for (int64_t q = int64_t(blockIdx.x) * PER_BLOCK + wid; q < end; q += BLOCK / 64) {
+        const int row = dense[q];
         double s = 0;
-        for (int q = lane; q < m; q += 64) s += a[q] * rl[q];
-        s = waveSum(s);
-        if (lane == 0) z[freeIdx[row]] = T(s);
+        if (row >= 0) {
+            const double* a = Ainv + int64_t(row) * m;
+            for (int c = lane; c < m; c += 64) s += a[c] * rl[c];
+            s = waveSum(s);
+        }
+        if (lane == 0) z[q] = T(s);
     }
 }
 
@@ -370,6 +376,61 @@ __global__ __launch_bounds__(BLOCK) void kUpdateP(const T* __restrict__ z, T* __
     const int64_t i = int64_t(blockIdx.x) * BLOCK + threadIdx.x;
     if (i >= n) return;
     p[i] = z[i] + T(st->beta) * p[i];
+}
+
+template <class T>
+__global__ __launch_bounds__(BLOCK) void kUpdateXRFold(T* __restrict__ x, const T* __restrict__ p, T* __restrict__ r, const T* __restrict__ q, int64_t n,
+                                                       CgState* st, Reduce red) {
+    if (st->stop) return;
+    const T a = T(st->alpha);
+    double v[1] = {0};
+    for (int64_t i = int64_t(blockIdx.x) * BLOCK + threadIdx.x; i < n; i += int64_t(gridDim.x) * BLOCK) {
+        x[i] += a * p[i];
+        const T ri = r[i] - a * q[i];
+        r[i] = ri;
+        v[0] += double(ri) * double(ri);
+    }
+    gridSum<1>(v, red, [&](const double* s) {
+        st->rr = s[0];
+        st->iters += 1;
+        st->sinceFold += 1;
+        const double est = sqrt(s[0]) * st->scale / st->bnorm;
+        st->rel = est;
+        st->peak = fmax(st->peak, est);
+        const bool fold = !(est > st->tol) || !(est > 0.1 * st->peak) || st->sinceFold >= st->foldEvery || st->iters >= st->maxIter || !isfinite(est);
+        st->noFold = fold ? 0 : 1;
+    });
+}
+
+__global__ __launch_bounds__(BLOCK) void kFoldX(double* __restrict__ u, float* __restrict__ x, int64_t n, const CgState* st) {
+    if (st->noFold) return;
+    const int64_t i = int64_t(blockIdx.x) * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    u[i] += double(x[i]) * st->rmax;
+    x[i] = 0.f;
+}
+
+__global__ __launch_bounds__(BLOCK) void kFoldR(const double* __restrict__ f, const double* __restrict__ Kx, const double* __restrict__ invD,
+                                                double* __restrict__ r64, float* __restrict__ r32, int64_t n, CgState* st, Reduce red) {
+    if (st->noFold) return;
+    const double inv = 1.0 / st->rmax;
+    double v[1] = {0};
+    for (int64_t i = int64_t(blockIdx.x) * BLOCK + threadIdx.x; i < n; i += int64_t(gridDim.x) * BLOCK) {
+        const double ri = invD[i] != 0 ? f[i] - Kx[i] : 0.0;
+        r64[i] = ri;
+        r32[i] = float(ri * inv);
+        v[0] += ri * ri;
+    }
+    gridSum<1>(v, red, [&](const double* s) {
+        const double rel = sqrt(s[0]) / st->bnorm;
+        st->trueRel = rel;
+        st->rel = rel;
+        st->peak = rel;
+        st->sinceFold = 0;
+        st->folds += 1;
+        st->noFold = 1;
+        if (!(rel > st->tol) || st->iters >= st->maxIter || !isfinite(rel)) st->stop = 1;
+    });
 }
 
 __global__ __launch_bounds__(BLOCK) void kResidual64(const double* __restrict__ f, const double* __restrict__ Kx, const double* __restrict__ invD,
@@ -504,8 +565,9 @@ template <class T> void launchProlong(const DevLevel& F, const DevLevel& C, cons
     kProlong<T><<<flatGrid(F.nNodes), BLOCK, 0, s>>>(F, C, zc, zf, stop);
 }
 
-template <class T> void launchCoarse(const int32_t* freeIdx, int64_t m, const double* Ainv, const T* r, T* z, const int* stop, hipStream_t s) {
-    kCoarse<T><<<unsigned((m + 15) / 16), BLOCK, 0, s>>>(freeIdx, int(m), Ainv, r, z, stop);
+template <class T>
+void launchCoarse(const int32_t* freeIdx, const int32_t* dense, int64_t m, int64_t nDof, const double* Ainv, const T* r, T* z, const int* stop, hipStream_t s) {
+    kCoarse<T><<<unsigned((nDof + 15) / 16), BLOCK, 0, s>>>(freeIdx, dense, int(m), nDof, Ainv, r, z, stop);
 }
 
 template <class T> void launchDotPQ(const T* p, const T* q, int64_t n, CgState* st, Reduce red, hipStream_t s) {
@@ -518,6 +580,14 @@ template <class T> void launchDotsRZ(const T* r, const T* z, const T* q, int64_t
     kDotsRZ<T><<<reduceGrid(n), BLOCK, 0, s>>>(r, z, q, n, st, red);
 }
 template <class T> void launchUpdateP(const T* z, T* p, int64_t n, CgState* st, hipStream_t s) { kUpdateP<T><<<flatGrid(n), BLOCK, 0, s>>>(z, p, n, st); }
+
+template <class T> void launchUpdateXRFold(T* x, const T* p, T* r, const T* q, int64_t n, CgState* st, Reduce red, hipStream_t s) {
+    kUpdateXRFold<T><<<reduceGrid(n), BLOCK, 0, s>>>(x, p, r, q, n, st, red);
+}
+void launchFoldX(double* u, float* x, int64_t n, CgState* st, hipStream_t s) { kFoldX<<<flatGrid(n), BLOCK, 0, s>>>(u, x, n, st); }
+void launchFoldR(const double* f, const double* Kx, const double* invD, double* r64, float* r32, int64_t n, CgState* st, Reduce red, hipStream_t s) {
+    kFoldR<<<reduceGrid(n), BLOCK, 0, s>>>(f, Kx, invD, r64, r32, n, st, red);
+}
 
 void launchResidual64(const double* f, const double* Kx, const double* invD, double* r, int64_t n, double* out, Reduce red, hipStream_t s) {
     kResidual64<<<reduceGrid(n), BLOCK, 0, s>>>(f, Kx, invD, r, n, out, red);
@@ -545,11 +615,12 @@ void launchLzNext(const double* w, double* v, double* vPrev, int64_t n, CgState*
     template void launchChebFirst<T>(const DevLevel&, const T*, T*, T*, double, const int*, hipStream_t);                               \
     template void launchRestrict<T>(const DevLevel&, const DevLevel&, const T*, T*, const int*, hipStream_t);                          \
     template void launchProlong<T>(const DevLevel&, const DevLevel&, const T*, T*, const int*, hipStream_t);                           \
-    template void launchCoarse<T>(const int32_t*, int64_t, const double*, const T*, T*, const int*, hipStream_t);                      \
+    template void launchCoarse<T>(const int32_t*, const int32_t*, int64_t, int64_t, const double*, const T*, T*, const int*, hipStream_t); \
     template void launchDotPQ<T>(const T*, const T*, int64_t, CgState*, Reduce, hipStream_t);                                         \
     template void launchUpdateXR<T>(T*, const T*, T*, const T*, int64_t, CgState*, Reduce, hipStream_t);                              \
     template void launchDotsRZ<T>(const T*, const T*, const T*, int64_t, CgState*, Reduce, hipStream_t);                              \
-    template void launchUpdateP<T>(const T*, T*, int64_t, CgState*, hipStream_t);
+    template void launchUpdateP<T>(const T*, T*, int64_t, CgState*, hipStream_t);                                                     \
+    template void launchUpdateXRFold<T>(T*, const T*, T*, const T*, int64_t, CgState*, Reduce, hipStream_t);
 PSX_INSTANTIATE(double)
 PSX_INSTANTIATE(float)
 

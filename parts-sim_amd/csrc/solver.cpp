@@ -10,6 +10,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <stdexcept>
@@ -56,11 +57,13 @@ public:
 
     struct Lvl {
         DevLevel dl{};
+        void* aop = nullptr;  // BF16 matrix-core operands of Kb (levels without own matrices)
         double *r64 = nullptr, *a64 = nullptr, *b64 = nullptr, *d64 = nullptr;
         float *r32 = nullptr, *a32 = nullptr, *b32 = nullptr, *d32 = nullptr;
     };
     std::vector<Lvl> lv;
     int32_t* freeIdx = nullptr;
+    int32_t* denseIdx = nullptr;  // coarsest DOF -> dense index, -1 when held
     double* Ainv = nullptr;
     // level-0 conjugate-gradient vectors
     double *x = nullptr, *r = nullptr, *p = nullptr, *q = nullptr, *f = nullptr, *u = nullptr;
@@ -127,6 +130,11 @@ public:
             D.invD64 = upload(L.invDiag.data(), L.nDof);
             D.invD32 = upload(inv32.data(), L.nDof);
             D.diagAdd = L.diagAdd.empty() ? nullptr : upload(L.diagAdd.data(), L.nDof);
+            if (L.nOwn == 0) {
+                std::vector<uint16_t> a(mfmaOperandBytes() / 2);
+                mfmaOperands(L.Kb, a.data());
+                lv[l].aop = upload(a.data(), int64_t(a.size()));
+            }
             lv[l].r64 = alloc<double>(L.nDof);
             lv[l].a64 = alloc<double>(L.nDof);
             lv[l].b64 = alloc<double>(L.nDof);
@@ -137,6 +145,11 @@ public:
             lv[l].d32 = alloc<float>(L.nDof);
         }
         freeIdx = upload(H.freeIdx.data(), H.m);
+        {
+            std::vector<int32_t> dense(H.levels.back().nDof, -1);
+            for (int64_t q = 0; q < H.m; q++) dense[H.freeIdx[q]] = int32_t(q);
+            denseIdx = upload(dense.data(), int64_t(dense.size()));
+        }
         Ainv = upload(H.Ainv.data(), H.m * H.m);
         const int64_t n = nDof();
         x = alloc<double>(n);
@@ -156,6 +169,28 @@ public:
         sync();
     }
 
+    // ----- operator dispatch: the z-marching kernel on levels without own matrices -----
+    bool useFine = [] {
+        const char* e = std::getenv("PSX_FINE");
+        return !(e && e[0] == '0');
+    }();
+    bool useMfma = [] {
+        const char* e = std::getenv("PSX_MFMA");
+        return !(e && e[0] == '0');
+    }();
+    // mfma: the BF16 matrix-core operator (FP32 vectors) where the level allows it
+    template <class T>
+    void op(size_t l, int mode, const T* xx, T* yy, const T* rr, T* dd, double c1, double c2, const int* stop, bool mfma = false) {
+        if constexpr (sizeof(T) == 4) {
+            if (mfma && useMfma && lv[l].aop) {
+                launchFineMfma(lv[l].dl, lv[l].aop, mode, xx, yy, rr, dd, c1, c2, stop, stream);
+                return;
+            }
+        }
+        if (useFine && H.levels[l].nOwn == 0) launchFine<T>(lv[l].dl, mode, xx, yy, rr, dd, c1, c2, stop, stream);
+        else launchApply<T>(lv[l].dl, mode, xx, yy, rr, dd, c1, c2, stop, stream);
+    }
+
     // ----- multigrid -----
 
     // largest eigenvalue of D^-1 K per level: 10 Lanczos steps on the device from the reference's
@@ -172,7 +207,7 @@ public:
             HIPCHK(hipMemsetAsync(vPrev, 0, size_t(n) * 8, stream));
             for (int j = 0; j < 10; j++) {
                 launchLzX(v, D.invD64, xx, n, stream);
-                launchApply<double>(D, AM_APPLY, xx, w, nullptr, nullptr, 0, 0, nullptr, stream);
+                op<double>(l, AM_APPLY, xx, w, nullptr, nullptr, 0, 0, nullptr);
                 launchLzW(w, D.invD64, vPrev, v, n, st, j, red, stream);
                 launchLzAxpy(w, v, n, st, j, red, stream);
                 launchLzNext(w, v, vPrev, n, st, j, stream);
@@ -205,24 +240,24 @@ public:
     }
 
     // one V-cycle from rIn on level l (voxel_fea.cpp:817-828); returns the buffer holding z
-    template <class T> T* vcycle(size_t l, const T* rIn, const int* stop) {
+    template <class T> T* vcycle(size_t l, const T* rIn, const int* stop, bool mfma = false) {
         Vecs<T> V = vecs<T>(l);
         const DevLevel& D = lv[l].dl;
         if (l + 1 == lv.size()) {
-            launchCoarse<T>(freeIdx, H.m, Ainv, rIn, V.b, stop, stream);
+            launchCoarse<T>(freeIdx, denseIdx, H.m, D.nDof, Ainv, rIn, V.b, stop, stream);
             return V.b;
         }
         const Cheb c = chebyshev(H.levels[l].lmax);
         const DevLevel& C = lv[l + 1].dl;
         launchChebFirst<T>(D, rIn, V.a, V.d, c.c0, stop, stream);
-        launchApply<T>(D, AM_CHEB2, V.a, V.b, rIn, V.d, c.c1, c.c2, stop, stream);
-        launchApply<T>(D, AM_RESID, V.b, V.a, rIn, nullptr, 0, 0, stop, stream);
+        op<T>(l, AM_CHEB2, V.a, V.b, rIn, V.d, c.c1, c.c2, stop, mfma);
+        op<T>(l, AM_RESID, V.b, V.a, rIn, nullptr, 0, 0, stop, mfma);
         Vecs<T> W = vecs<T>(l + 1);
         launchRestrict<T>(D, C, V.a, W.r, stop, stream);
-        const T* zc = vcycle<T>(l + 1, W.r, stop);
+        const T* zc = vcycle<T>(l + 1, W.r, stop, mfma);
         launchProlong<T>(D, C, zc, V.b, stop, stream);
-        launchApply<T>(D, AM_CHEB1, V.b, V.a, rIn, V.d, c.c0, 0, stop, stream);
-        launchApply<T>(D, AM_CHEB2, V.a, V.b, rIn, V.d, c.c1, c.c2, stop, stream);
+        op<T>(l, AM_CHEB1, V.b, V.a, rIn, V.d, c.c0, 0, stop, mfma);
+        op<T>(l, AM_CHEB2, V.a, V.b, rIn, V.d, c.c1, c.c2, stop, mfma);
         return V.b;
     }
 
@@ -251,7 +286,7 @@ public:
         // x = x0 on free DOFs; bnorm = |f_free|; r = f - K x (FP64)
         launchMaskCopy64(dx0, D0.invD64, du, n, stream);
         launchNormFree64(df, D0.invD64, n, &st->bnorm, red, stream);
-        launchApply<double>(D0, AM_APPLY, du, q, nullptr, nullptr, 0, 0, nullptr, stream);
+        op<double>(0, AM_APPLY, du, q, nullptr, nullptr, 0, 0, nullptr);
         launchResidual64(df, q, D0.invD64, r, n, &st->rr, red, stream);
         readState();
         const double bnorm = std::sqrt(stHost->bnorm);
@@ -265,24 +300,53 @@ public:
         double rel = std::sqrt(stHost->rr) / bnorm;
         const int prec = o.precision;
         const int batch = o.check_every > 0 ? o.check_every : 8;
+        double rmax = 1;
+        if (prec == PSX_PREC_MIXED) {
+            // FP32 units for the whole solve: the first residual scaled to order one
+            launchMaxAbs64(r, n, &st->rmax, red, stream);
+            readState();
+            rmax = stHost->rmax > 0 ? stHost->rmax : 1;
+        }
         CgState s0{};
         s0.rz = 1;
         s0.bnorm = bnorm;
         s0.tol = o.tol;
         s0.maxIter = o.max_iter;
-        s0.scale = 1;
+        s0.scale = prec == PSX_PREC_MIXED ? rmax : 1;
+        s0.rmax = rmax;
         s0.rel = rel;
         s0.trueRel = rel;
         s0.peak = rel;
+        s0.noFold = 1;
         s0.foldEvery = o.fold_every > 0 ? o.fold_every : 20;
         s0.stop = (rel <= o.tol || o.max_iter <= 0) ? 1 : 0;
         HIPCHK(hipMemcpyAsync(st, &s0, sizeof s0, hipMemcpyHostToDevice, stream));
-        HIPCHK(hipMemsetAsync(p, 0, size_t(n) * 8, stream));
-        HIPCHK(hipMemsetAsync(q, 0, size_t(n) * 8, stream));
         int* stop = &st->stop;
-        if (prec == PSX_PREC_MIXED) throw std::invalid_argument("mixed precision is not available yet");
+        float* r32 = lv[0].r32;
+        if (prec == PSX_PREC_MIXED) {
+            HIPCHK(hipMemsetAsync(p32, 0, size_t(n) * 4, stream));
+            HIPCHK(hipMemsetAsync(q32, 0, size_t(n) * 4, stream));
+            HIPCHK(hipMemsetAsync(x32, 0, size_t(n) * 4, stream));
+            launchToF32(r, r32, n, &st->rmax, stream);
+        } else {
+            HIPCHK(hipMemsetAsync(p, 0, size_t(n) * 8, stream));
+            HIPCHK(hipMemsetAsync(q, 0, size_t(n) * 8, stream));
+        }
         while (!s0.stop) {
             for (int k = 0; k < batch; k++) {
+                if (prec == PSX_PREC_MIXED) {
+                    const float* z = vcycle<float>(0, r32, stop, true);
+                    launchDotsRZ<float>(r32, z, q32, n, st, red, stream);
+                    launchUpdateP<float>(z, p32, n, st, stream);
+                    op<float>(0, AM_APPLY, p32, q32, nullptr, nullptr, 0, 0, stop);
+                    launchDotPQ<float>(p32, q32, n, st, red, stream);
+                    launchUpdateXRFold<float>(x32, p32, r32, q32, n, st, red, stream);
+                    // reliable update: u += x, FP64 true residual replaces the recurrence's
+                    launchFoldX(du, x32, n, st, stream);
+                    op<double>(0, AM_APPLY, du, q, nullptr, nullptr, 0, 0, &st->noFold);
+                    launchFoldR(df, q, D0.invD64, r, r32, n, st, red, stream);
+                    continue;
+                }
                 const double* z;
                 if (prec == PSX_PREC_FP64) z = vcycle<double>(0, r, stop);
                 else {
@@ -291,7 +355,7 @@ public:
                 }
                 launchDotsRZ<double>(r, z, q, n, st, red, stream);
                 launchUpdateP<double>(z, p, n, st, stream);
-                launchApply<double>(D0, AM_APPLY, p, q, nullptr, nullptr, 0, 0, stop, stream);
+                op<double>(0, AM_APPLY, p, q, nullptr, nullptr, 0, 0, stop);
                 launchDotPQ<double>(p, q, n, st, red, stream);
                 launchUpdateXR<double>(du, p, r, q, n, st, red, stream);
             }
@@ -302,11 +366,18 @@ public:
                 break;
             }
         }
+        if (prec == PSX_PREC_MIXED) {
+            // a cancelled run still folds in its last correction
+            const int zero = 0;
+            HIPCHK(hipMemcpyAsync(&st->noFold, &zero, sizeof zero, hipMemcpyHostToDevice, stream));
+            launchFoldX(du, x32, n, st, stream);
+        }
         readState();
         info.iterations = stHost->iters;
         info.broken = stHost->broken;
+        info.folds = stHost->folds;
         // the final true residual against the equilibrium equations (voxel_fea.cpp:920-925)
-        launchApply<double>(D0, AM_APPLY, du, q, nullptr, nullptr, 0, 0, nullptr, stream);
+        op<double>(0, AM_APPLY, du, q, nullptr, nullptr, 0, 0, nullptr);
         launchResidual64(df, q, D0.invD64, r, n, &st->rr, red, stream);
         readState();
         info.residual = std::sqrt(stHost->rr) / bnorm;
@@ -386,6 +457,18 @@ int psx_available(void) {
     return hipGetDeviceCount(&n) == hipSuccess && n > 0 ? 1 : 0;
 }
 
+int psx_device_name(char* buf, size_t len) {
+    return guarded([&] {
+        if (!buf || len == 0) throw std::invalid_argument("bad buffer");
+        int dev = 0;
+        HIPCHK(hipGetDevice(&dev));
+        hipDeviceProp_t prop;
+        HIPCHK(hipGetDeviceProperties(&prop, dev));
+        std::snprintf(buf, len, "%s (%s, %d CUs)", prop.name, prop.gcnArchName, prop.multiProcessorCount);
+        return PSX_OK;
+    });
+}
+
 int psx_create(const psx_model_desc* desc, psx_model* out) {
     return guarded([&] {
         if (!desc || !out) throw std::invalid_argument("desc and out are required");
@@ -439,7 +522,7 @@ int psx_apply(psx_model m, int level, const double* x, double* y, int raw) {
         auto& V = M.lv[level];
         const int64_t n = V.dl.nDof;
         HIPCHK(hipMemcpyAsync(V.a64, x, size_t(n) * 8, hipMemcpyHostToDevice, M.stream));
-        psx::launchApply<double>(V.dl, raw ? psx::AM_RAW : psx::AM_APPLY, V.a64, V.b64, nullptr, nullptr, 0, 0, nullptr, M.stream);
+        M.op<double>(size_t(level), raw ? psx::AM_RAW : psx::AM_APPLY, V.a64, V.b64, nullptr, nullptr, 0, 0, nullptr);
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(y, V.b64, size_t(n) * 8, hipMemcpyDeviceToHost, M.stream));
         M.sync();
@@ -454,7 +537,23 @@ int psx_apply32(psx_model m, int level, const float* x, float* y, int raw) {
         auto& V = M.lv[level];
         const int64_t n = V.dl.nDof;
         HIPCHK(hipMemcpyAsync(V.a32, x, size_t(n) * 4, hipMemcpyHostToDevice, M.stream));
-        psx::launchApply<float>(V.dl, raw ? psx::AM_RAW : psx::AM_APPLY, V.a32, V.b32, nullptr, nullptr, 0, 0, nullptr, M.stream);
+        M.op<float>(size_t(level), raw ? psx::AM_RAW : psx::AM_APPLY, V.a32, V.b32, nullptr, nullptr, 0, 0, nullptr);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(y, V.b32, size_t(n) * 4, hipMemcpyDeviceToHost, M.stream));
+        M.sync();
+        return PSX_OK;
+    });
+}
+
+int psx_apply_bf16(psx_model m, int level, const float* x, float* y) {
+    return guarded([&] {
+        if (!m || !x || !y || level < 0 || level >= int(m->m->H.levels.size())) throw std::invalid_argument("bad arguments");
+        psx::Model& M = *m->m;
+        auto& V = M.lv[level];
+        if (!V.aop) throw std::invalid_argument("the level has own element matrices: no matrix-core operator");
+        const int64_t n = V.dl.nDof;
+        HIPCHK(hipMemcpyAsync(V.a32, x, size_t(n) * 4, hipMemcpyHostToDevice, M.stream));
+        psx::launchFineMfma(V.dl, V.aop, psx::AM_APPLY, V.a32, V.b32, nullptr, nullptr, 0, 0, nullptr, M.stream);
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(y, V.b32, size_t(n) * 4, hipMemcpyDeviceToHost, M.stream));
         M.sync();
@@ -526,7 +625,7 @@ int psx_reactions(psx_model m, const double* u, const double* f, double out[3]) 
         const int64_t n = L.nDof;
         std::vector<double> y(n);
         HIPCHK(hipMemcpyAsync(M.x, u, size_t(n) * 8, hipMemcpyHostToDevice, M.stream));
-        psx::launchApply<double>(M.lv[0].dl, psx::AM_RAW, M.x, M.q, nullptr, nullptr, 0, 0, nullptr, M.stream);
+        M.op<double>(0, psx::AM_RAW, M.x, M.q, nullptr, nullptr, 0, 0, nullptr);
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(y.data(), M.q, size_t(n) * 8, hipMemcpyDeviceToHost, M.stream));
         M.sync();
@@ -564,15 +663,18 @@ int psx_profile(psx_model m, int reps, char* buf, size_t len) {
         for (size_t l = 0; l < M.lv.size(); l++) {
             auto& V = M.lv[l];
             const psx::Level& L = M.H.levels[l];
-            const double t64 = timeIt([&] { psx::launchApply<double>(V.dl, psx::AM_APPLY, V.a64, V.b64, nullptr, nullptr, 0, 0, nullptr, M.stream); });
-            const double t32 = timeIt([&] { psx::launchApply<float>(V.dl, psx::AM_APPLY, V.a32, V.b32, nullptr, nullptr, 0, 0, nullptr, M.stream); });
-            std::snprintf(line, sizeof line, "L%zu nodes=%lld elems=%zu own matrices=%lld: matvec64 %.4f ms  matvec32 %.4f ms\n", l, (long long)L.nNodes,
-                          L.elemVox.size(), (long long)L.nOwn, t64, t32);
+            const double t64 = timeIt([&] { M.op<double>(l, psx::AM_APPLY, V.a64, V.b64, nullptr, nullptr, 0, 0, nullptr); });
+            const double t32 = timeIt([&] { M.op<float>(l, psx::AM_APPLY, V.a32, V.b32, nullptr, nullptr, 0, 0, nullptr); });
+            double tbf = 0;
+            if (V.aop) tbf = timeIt([&] { psx::launchFineMfma(V.dl, V.aop, psx::AM_APPLY, V.a32, V.b32, nullptr, nullptr, 0, 0, nullptr, M.stream); });
+            std::snprintf(line, sizeof line, "L%zu nodes=%lld elems=%zu own matrices=%lld: matvec64 %.4f ms  matvec32 %.4f ms  matvec-bf16 %.4f ms\n", l,
+                          (long long)L.nNodes, L.elemVox.size(), (long long)L.nOwn, t64, t32, tbf);
             out += line;
         }
         const double tv64 = timeIt([&] { M.vcycle<double>(0, M.lv[0].r64, nullptr); });
         const double tv32 = timeIt([&] { M.vcycle<float>(0, M.lv[0].r32, nullptr); });
-        std::snprintf(line, sizeof line, "V-cycle FP64 %.4f ms  FP32 %.4f ms\n", tv64, tv32);
+        const double tvbf = timeIt([&] { M.vcycle<float>(0, M.lv[0].r32, nullptr, true); });
+        std::snprintf(line, sizeof line, "V-cycle FP64 %.4f ms  FP32 %.4f ms  FP32+BF16 matrix cores %.4f ms\n", tv64, tv32, tvbf);
         out += line;
         hipEventDestroy(e0);
         hipEventDestroy(e1);
