@@ -24,6 +24,12 @@ struct DevLevel {
     const double* invD64;      // inverse diagonal, 0 on held / inactive DOFs
     const float* invD32;
     const double* diagAdd;     // optional diagonal shift (nullptr when absent)
+    // The cube's symmetry leaves 22 distinct values in Kb and its stencil (kFine reads them from
+    // here, as wave-uniform kernel arguments): [0, 14) the element entries, [14, 20) the stencil
+    // diagonal classes, [20, 22) its off-diagonal classes. Valid when symOk.
+    double sym64[22];
+    float sym32[22];
+    int symOk;
 };
 
 // Scalars of one conjugate-gradient run, resident on the device. Every kernel of an iteration
@@ -37,10 +43,11 @@ struct CgState {
     double lz[32];             // Lanczos alpha[0..9], beta at [16..25]
 };
 
-// partial sums of the reductions (one slot per block, up to 4 values per kernel)
-constexpr int RED_BLOCKS = 2048;
+// partial sums of the reductions: one slot per block (blocks own RED_TILE elements), up to 4
+// values per kernel
+constexpr int RED_TILE = 2048;
 struct Reduce {
-    double* partials;          // 4 * RED_BLOCKS
+    double* partials;          // 4 * (nDof / RED_TILE + 1)
     unsigned* counter;
 };
 
@@ -53,6 +60,9 @@ enum ApplyMode { AM_APPLY = 0, AM_RAW = 1, AM_RESID = 2, AM_CHEB1 = 3, AM_CHEB2 
 //   AM_CHEB2  d = c1 d + c2 D^-1 (r - K x); y = x + d
 template <class T> void launchApply(const DevLevel& L, int mode, const T* x, T* y, const T* r, T* d, double c1, double c2, const int* stop,
                                     hipStream_t s);
+// the same with 8 lanes per node (one per voxel): levels with own matrices
+template <class T> void launchApplyOwn(const DevLevel& L, int mode, const T* x, T* y, const T* r, T* d, double c1, double c2, const int* stop,
+                                       hipStream_t s);
 // the same for levels without own matrices: z-marching LDS planes, stencil + packed per-voxel
 // sums (fine.hip)
 template <class T> void launchFine(const DevLevel& L, int mode, const T* x, T* y, const T* r, T* d, double c1, double c2, const int* stop,

@@ -200,6 +200,77 @@ void stencil(const double* K, double* S) {
 }
 
 // ---------------------------------------------------------------------------------------------
+// cube symmetry of the element and its stencil
+
+int elementClass(int a, int d, int b, int c) {
+    // reflect corner a onto corner 0 (flips the sign of each DOF component along a reflected axis)
+    const int ca[3] = {CX[a], CY[a], CZ[a]};
+    int r[3] = {CX[a] ^ CX[b], CY[a] ^ CY[b], CZ[a] ^ CZ[b]};
+    const bool neg = ((ca[d] + ca[c]) & 1) != 0;
+    // permute axis d onto axis 0 (row d of corner 0 becomes row 0)
+    std::swap(r[0], r[d]);
+    int c2 = c == 0 ? d : (c == d ? 0 : c);
+    // y <-> z mirror to a canonical form
+    if (c2 == 2) {
+        std::swap(r[1], r[2]);
+        c2 = 1;
+    } else if (c2 == 0 && r[1] < r[2]) std::swap(r[1], r[2]);
+    const int idx = c2 == 0 ? r[0] * 3 + r[1] + r[2] : 6 + r[0] + 2 * r[1] + 4 * r[2];
+    return idx | (neg ? 0x100 : 0);
+}
+
+int stencilClass(int m, int d, int c) {
+    const int o[3] = {m % 3 - 1, (m / 3) % 3 - 1, m / 9 - 1};
+    if (d == c) {
+        int other = 0;
+        for (int e = 0; e < 3; e++)
+            if (e != d) other += o[e] != 0;
+        return 14 + (o[d] != 0) * 3 + other;
+    }
+    if (o[d] == 0 || o[c] == 0) return -1;
+    const int e = 3 - d - c;
+    return (20 + (o[e] != 0)) | (o[d] * o[c] < 0 ? 0x100 : 0);
+}
+
+double symmetryClasses(const double* K, double* out) {
+    double sum[22] = {}, lo[22], hi[22];
+    int cnt[22] = {};
+    for (int q = 0; q < 22; q++) lo[q] = INFINITY, hi[q] = -INFINITY;
+    auto add = [&](int cls, double v) {
+        const int q = cls & 0xff;
+        const double s = (cls & 0x100) ? -v : v;
+        sum[q] += s;
+        cnt[q]++;
+        lo[q] = std::min(lo[q], s);
+        hi[q] = std::max(hi[q], s);
+    };
+    double kmax = 0;
+    for (int a = 0; a < 8; a++)
+        for (int d = 0; d < 3; d++)
+            for (int b = 0; b < 8; b++)
+                for (int c = 0; c < 3; c++) {
+                    const double v = K[(3 * a + d) * 24 + 3 * b + c];
+                    kmax = std::max(kmax, std::abs(v));
+                    add(elementClass(a, d, b, c), v);
+                }
+    double S[243];
+    stencil(K, S);
+    double dev = 0;
+    for (int m = 0; m < 27; m++)
+        for (int d = 0; d < 3; d++)
+            for (int c = 0; c < 3; c++) {
+                const int cls = stencilClass(m, d, c);
+                if (cls < 0) dev = std::max(dev, std::abs(S[m * 9 + d * 3 + c]));
+                else add(cls, S[m * 9 + d * 3 + c]);
+            }
+    for (int q = 0; q < 22; q++) {
+        out[q] = cnt[q] ? sum[q] / cnt[q] : 0;
+        if (cnt[q]) dev = std::max(dev, hi[q] - lo[q]);
+    }
+    return kmax > 0 ? dev / kmax : 0;
+}
+
+// ---------------------------------------------------------------------------------------------
 // levels
 
 static Level emptyLevel(int nx, int ny, int nz) {

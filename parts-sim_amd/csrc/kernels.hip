@@ -67,38 +67,41 @@ template <int NV> __device__ __forceinline__ void blockSum(double (&v)[NV]) {
     __syncthreads();
 }
 
-// block partials -> the last block sums them in block order and calls fin(total) in thread 0
-template <int NV, class Fin> __device__ __forceinline__ void gridSum(double (&v)[NV], Reduce red, Fin fin) {
+// Grid reductions in two kernels: every block writes its partial sums (plain stores, no fence or
+// atomic per block), then a one-block kernel adds the partials in block order and applies fin().
+// The kernel boundary makes the partials visible; the order is fixed, so results are bitwise
+// reproducible.
+template <int NV> __device__ __forceinline__ void blockPartials(double (&v)[NV], Reduce red) {
     blockSum<NV>(v);
-    __shared__ int isLast;
-    if (threadIdx.x == 0) {
+    if (threadIdx.x == 0)
 #pragma unroll
-        for (int k = 0; k < NV; k++) red.partials[k * RED_BLOCKS + blockIdx.x] = v[k];
-        __threadfence();
-        const unsigned t = atomicAdd(red.counter, 1u);
-        isLast = t == gridDim.x - 1;
-    }
-    __syncthreads();
-    if (!isLast) return;
-    __threadfence();
+        for (int k = 0; k < NV; k++) red.partials[k * gridDim.x + blockIdx.x] = v[k];
+}
+
+// guard: skip when *guard != 0 (the producer kernel returned early for the same reason)
+template <int NV, class Fin> __global__ __launch_bounds__(BLOCK) void kFinish(Reduce red, unsigned nb, const int* guard, Fin fin) {
+    if (guard && *guard) return;
     double s[NV];
 #pragma unroll
     for (int k = 0; k < NV; k++) {
         s[k] = 0;
-        for (unsigned b = threadIdx.x; b < gridDim.x; b += BLOCK)
-            s[k] += __hip_atomic_load(red.partials + k * RED_BLOCKS + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (unsigned b = threadIdx.x; b < nb; b += BLOCK) s[k] += red.partials[k * nb + b];
     }
     blockSum<NV>(s);
-    if (threadIdx.x == 0) {
-        fin(s);
-        __hip_atomic_store(red.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
+    if (threadIdx.x == 0) fin(s);
 }
 
+// reductions: each block owns a contiguous tile of RED_TILE elements, 8 per thread, loads issued
+// together (unrolled) so every thread keeps 8 independent requests in flight
 inline unsigned reduceGrid(int64_t n) {
-    const int64_t g = (n + 8 * BLOCK - 1) / (8 * BLOCK);
-    return unsigned(g < 1 ? 1 : (g > RED_BLOCKS ? RED_BLOCKS : g));
+    const int64_t g = (n + RED_TILE - 1) / RED_TILE;
+    return unsigned(g < 1 ? 1 : g);
 }
+#define TILE_LOOP(body)                                                                  \
+    _Pragma("unroll") for (int k_ = 0; k_ < RED_TILE / BLOCK; k_++) {                    \
+        const int64_t i = int64_t(blockIdx.x) * RED_TILE + threadIdx.x + k_ * BLOCK;    \
+        if (i < n) { body }                                                              \
+    }
 inline unsigned flatGrid(int64_t n) { return unsigned((n + BLOCK - 1) / BLOCK); }
 
 // ------------------------------------------------------------------------------------------------
@@ -186,6 +189,96 @@ __global__ __launch_bounds__(BLOCK) void kApply(DevLevel L, const T* __restrict_
     const int i = int(nn % NX), j = int((nn / NX) % NY), k = int(nn / (NX * NY));
     T acc[3];
     applyNode<T>(L, x, i, j, k, n, acc);
+    const T* __restrict__ invD = Prec<T>::invD(L);
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const int64_t q = 3 * n + d;
+        const T id = invD[q];
+        if constexpr (MODE == AM_APPLY) y[q] = id != T(0) ? acc[d] : T(0);
+        else if constexpr (MODE == AM_RAW) y[q] = acc[d];
+        else if constexpr (MODE == AM_RESID) y[q] = id != T(0) ? r[q] - acc[d] : T(0);
+        else if constexpr (MODE == AM_CHEB1) {
+            const T dn = c1 * id * (r[q] - acc[d]);
+            dv[q] = dn;
+            y[q] = x[q] + dn;
+        } else {
+            const T dn = c1 * dv[q] + c2 * id * (r[q] - acc[d]);
+            dv[q] = dn;
+            y[q] = x[q] + dn;
+        }
+    }
+}
+
+// The same product for levels with own element matrices, 8 lanes per node: lane v takes voxel v
+// (3 rows x 24 columns from s * Kb in LDS or from the voxel's own matrix), the 8 partial rows are
+// summed with shuffles inside the aligned 8-lane group, lane 0 writes. Coarse levels are small:
+// this spreads one node's dependent loads over 8 lanes instead of one thread walking 8 voxels.
+template <class T, int MODE>
+__global__ __launch_bounds__(BLOCK) void kApplyOwn(DevLevel L, const T* __restrict__ x, T* __restrict__ y, const T* __restrict__ r, T* __restrict__ dv,
+                                                   T c1, T c2, const int* stop) {
+    if (stopped(stop)) return;
+    __shared__ T Kb[576];
+    for (int q = threadIdx.x; q < 576; q += BLOCK) Kb[q] = Prec<T>::Kb(L)[q];
+    __syncthreads();
+    const int64_t n = int64_t(blockIdx.x) * (BLOCK / 8) + threadIdx.x / 8;
+    const int v = threadIdx.x & 7;
+    T b0 = 0, b1 = 0, b2 = 0;
+    if (n < L.nNodes) {
+        const unsigned nn = unsigned(n), NXu = unsigned(L.NX), NYu = unsigned(L.NY);
+        const int i = int(nn % NXu), j = int((nn / NXu) % NYu), k = int(nn / (NXu * NYu));
+        const int di = v & 1, dj = (v >> 1) & 1, dk = v >> 2;
+        const int ei = i - 1 + di, ej = j - 1 + dj, ek = k - 1 + dk;
+        if (ei >= 0 && ej >= 0 && ek >= 0 && ei < L.nx && ej < L.ny && ek < L.nz) {
+            const int32_t w = L.words[ei + int64_t(L.nx) * (ej + int64_t(L.ny) * ek)];
+            if (w != 0) {
+                const int64_t NX = L.NX, NXY = int64_t(L.NX) * L.NY;
+                const int a = d_corner(1 - di, 1 - dj, 1 - dk);
+                const int64_t nb = ei + NX * ej + NXY * ek;
+                T xe[24];
+#pragma unroll
+                for (int b = 0; b < 8; b++) {
+                    const T* xm = x + 3 * (nb + kCX(b) + kCY(b) * NX + kCZ(b) * NXY);
+                    xe[3 * b] = xm[0];
+                    xe[3 * b + 1] = xm[1];
+                    xe[3 * b + 2] = xm[2];
+                }
+                if (w > 0) {
+                    const T* K = Kb + 72 * a;
+#pragma unroll
+                    for (int c = 0; c < 24; c++) {
+                        b0 += K[c] * xe[c];
+                        b1 += K[24 + c] * xe[c];
+                        b2 += K[48 + c] * xe[c];
+                    }
+                    const T s = T(__int_as_float(w));
+                    b0 *= s;
+                    b1 *= s;
+                    b2 *= s;
+                } else {
+                    const float4* __restrict__ K4 = reinterpret_cast<const float4*>(L.Kown + int64_t(w & 0x7fffffff) * 576 + 72 * a);
+#pragma unroll
+                    for (int q = 0; q < 6; q++) {
+                        const float4 r0 = K4[q], r1 = K4[6 + q], r2 = K4[12 + q];
+                        b0 += T(r0.x) * xe[4 * q] + T(r0.y) * xe[4 * q + 1] + T(r0.z) * xe[4 * q + 2] + T(r0.w) * xe[4 * q + 3];
+                        b1 += T(r1.x) * xe[4 * q] + T(r1.y) * xe[4 * q + 1] + T(r1.z) * xe[4 * q + 2] + T(r1.w) * xe[4 * q + 3];
+                        b2 += T(r2.x) * xe[4 * q] + T(r2.y) * xe[4 * q + 1] + T(r2.z) * xe[4 * q + 2] + T(r2.w) * xe[4 * q + 3];
+                    }
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int o = 4; o >= 1; o >>= 1) {
+        b0 += __shfl_xor(b0, o, 64);
+        b1 += __shfl_xor(b1, o, 64);
+        b2 += __shfl_xor(b2, o, 64);
+    }
+    if (v != 0 || n >= L.nNodes) return;
+    T acc[3] = {b0, b1, b2};
+    if (L.diagAdd) {
+#pragma unroll
+        for (int d = 0; d < 3; d++) acc[d] += T(L.diagAdd[3 * n + d]) * x[3 * n + d];
+    }
     const T* __restrict__ invD = Prec<T>::invD(L);
 #pragma unroll
     for (int d = 0; d < 3; d++) {
@@ -299,18 +392,24 @@ __global__ __launch_bounds__(BLOCK) void kCoarse(const int32_t* __restrict__ fre
     for (int q = threadIdx.x; q < m; q += BLOCK) rl[q] = double(r[freeIdx[q]]);
     __syncthreads();
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    constexpr int PER_BLOCK = 16;
-    const int64_t end = min(nDof, int64_t(blockIdx.x + 1) * PER_BLOCK);
-    for (int64_t q = int64_t(blockIdx.x) * PER_BLOCK + wid; q < end; q += BLOCK / 64) {
-        const int row = dense[q];
-        double s = 0;
-        if (row >= 0) {
-            const double* a = Ainv + int64_t(row) * m;
-            for (int c = lane; c < m; c += 64) s += a[c] * rl[c];
-            s = waveSum(s);
+    const int64_t q = int64_t(blockIdx.x) * (BLOCK / 64) + wid;  // one wavefront per DOF
+    if (q >= nDof) return;
+    const int row = dense[q];
+    double s = 0;
+    if (row >= 0) {
+        const double* a = Ainv + int64_t(row) * m;
+        double s1 = 0, s2 = 0, s3 = 0;
+        int c = lane;
+        for (; c + 192 < m; c += 256) {
+            s += a[c] * rl[c];
+            s1 += a[c + 64] * rl[c + 64];
+            s2 += a[c + 128] * rl[c + 128];
+            s3 += a[c + 192] * rl[c + 192];
         }
-        if (lane == 0) z[q] = T(s);
+        for (; c < m; c += 64) s += a[c] * rl[c];
+        s = waveSum((s + s1) + (s2 + s3));
     }
+    if (lane == 0) z[q] = T(s);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -320,14 +419,8 @@ template <class T>
 __global__ __launch_bounds__(BLOCK) void kDotPQ(const T* __restrict__ p, const T* __restrict__ q, int64_t n, CgState* st, Reduce red) {
     if (st->stop) return;
     double v[1] = {0};
-    for (int64_t i = int64_t(blockIdx.x) * BLOCK + threadIdx.x; i < n; i += int64_t(gridDim.x) * BLOCK) v[0] += double(p[i]) * double(q[i]);
-    gridSum<1>(v, red, [&](const double* s) {
-        st->pq = s[0];
-        if (!(s[0] > 0)) {  // loss of positive-definiteness (a mechanism)
-            st->broken = 1;
-            st->stop = 1;
-        } else st->alpha = st->rz / s[0];
-    });
+    TILE_LOOP(v[0] += double(p[i]) * double(q[i]);)
+    blockPartials<1>(v, red);
 }
 
 template <class T>
@@ -336,19 +429,12 @@ __global__ __launch_bounds__(BLOCK) void kUpdateXR(T* __restrict__ x, const T* _
     if (st->stop) return;
     const T a = T(st->alpha);
     double v[1] = {0};
-    for (int64_t i = int64_t(blockIdx.x) * BLOCK + threadIdx.x; i < n; i += int64_t(gridDim.x) * BLOCK) {
+    TILE_LOOP(
         x[i] += a * p[i];
         const T ri = r[i] - a * q[i];
         r[i] = ri;
-        v[0] += double(ri) * double(ri);
-    }
-    gridSum<1>(v, red, [&](const double* s) {
-        st->rr = s[0];
-        st->iters += 1;
-        st->sinceFold += 1;
-        st->rel = sqrt(s[0]) * st->scale / st->bnorm;
-        if (!(st->rel > st->tol) || st->iters >= st->maxIter || !isfinite(st->rel)) st->stop = 1;
-    });
+        v[0] += double(ri) * double(ri);)
+    blockPartials<1>(v, red);
 }
 
 template <class T>
@@ -356,18 +442,11 @@ __global__ __launch_bounds__(BLOCK) void kDotsRZ(const T* __restrict__ r, const 
                                                  Reduce red) {
     if (st->stop) return;
     double v[2] = {0, 0};
-    for (int64_t i = int64_t(blockIdx.x) * BLOCK + threadIdx.x; i < n; i += int64_t(gridDim.x) * BLOCK) {
+    TILE_LOOP(
         const double zi = double(z[i]);
         v[0] += double(r[i]) * zi;
-        v[1] += zi * double(q[i]);
-    }
-    gridSum<2>(v, red, [&](const double* s) {
-        st->rzNew = s[0];
-        st->zq = s[1];
-        const double b = st->rz != 0 ? -st->alpha * s[1] / st->rz : 0.0;
-        st->beta = b > 0 ? b : 0.0;
-        st->rz = s[0];
-    });
+        v[1] += zi * double(q[i]);)
+    blockPartials<2>(v, red);
 }
 
 template <class T>
@@ -384,22 +463,12 @@ __global__ __launch_bounds__(BLOCK) void kUpdateXRFold(T* __restrict__ x, const 
     if (st->stop) return;
     const T a = T(st->alpha);
     double v[1] = {0};
-    for (int64_t i = int64_t(blockIdx.x) * BLOCK + threadIdx.x; i < n; i += int64_t(gridDim.x) * BLOCK) {
+    TILE_LOOP(
         x[i] += a * p[i];
         const T ri = r[i] - a * q[i];
         r[i] = ri;
-        v[0] += double(ri) * double(ri);
-    }
-    gridSum<1>(v, red, [&](const double* s) {
-        st->rr = s[0];
-        st->iters += 1;
-        st->sinceFold += 1;
-        const double est = sqrt(s[0]) * st->scale / st->bnorm;
-        st->rel = est;
-        st->peak = fmax(st->peak, est);
-        const bool fold = !(est > st->tol) || !(est > 0.1 * st->peak) || st->sinceFold >= st->foldEvery || st->iters >= st->maxIter || !isfinite(est);
-        st->noFold = fold ? 0 : 1;
-    });
+        v[0] += double(ri) * double(ri);)
+    blockPartials<1>(v, red);
 }
 
 __global__ __launch_bounds__(BLOCK) void kFoldX(double* __restrict__ u, float* __restrict__ x, int64_t n, const CgState* st) {
@@ -415,40 +484,28 @@ __global__ __launch_bounds__(BLOCK) void kFoldR(const double* __restrict__ f, co
     if (st->noFold) return;
     const double inv = 1.0 / st->rmax;
     double v[1] = {0};
-    for (int64_t i = int64_t(blockIdx.x) * BLOCK + threadIdx.x; i < n; i += int64_t(gridDim.x) * BLOCK) {
+    TILE_LOOP(
         const double ri = invD[i] != 0 ? f[i] - Kx[i] : 0.0;
         r64[i] = ri;
         r32[i] = float(ri * inv);
-        v[0] += ri * ri;
-    }
-    gridSum<1>(v, red, [&](const double* s) {
-        const double rel = sqrt(s[0]) / st->bnorm;
-        st->trueRel = rel;
-        st->rel = rel;
-        st->peak = rel;
-        st->sinceFold = 0;
-        st->folds += 1;
-        st->noFold = 1;
-        if (!(rel > st->tol) || st->iters >= st->maxIter || !isfinite(rel)) st->stop = 1;
-    });
+        v[0] += ri * ri;)
+    blockPartials<1>(v, red);
 }
 
 __global__ __launch_bounds__(BLOCK) void kResidual64(const double* __restrict__ f, const double* __restrict__ Kx, const double* __restrict__ invD,
-                                                     double* __restrict__ r, int64_t n, double* out, Reduce red) {
+                                                     double* __restrict__ r, int64_t n, Reduce red) {
     double v[1] = {0};
-    for (int64_t i = int64_t(blockIdx.x) * BLOCK + threadIdx.x; i < n; i += int64_t(gridDim.x) * BLOCK) {
+    TILE_LOOP(
         const double ri = invD[i] != 0 ? f[i] - Kx[i] : 0.0;
         r[i] = ri;
-        v[0] += ri * ri;
-    }
-    gridSum<1>(v, red, [&](const double* s) { *out = s[0]; });
+        v[0] += ri * ri;)
+    blockPartials<1>(v, red);
 }
 
-__global__ __launch_bounds__(BLOCK) void kNormFree64(const double* __restrict__ f, const double* __restrict__ invD, int64_t n, double* out, Reduce red) {
+__global__ __launch_bounds__(BLOCK) void kNormFree64(const double* __restrict__ f, const double* __restrict__ invD, int64_t n, Reduce red) {
     double v[1] = {0};
-    for (int64_t i = int64_t(blockIdx.x) * BLOCK + threadIdx.x; i < n; i += int64_t(gridDim.x) * BLOCK)
-        if (invD[i] != 0) v[0] += f[i] * f[i];
-    gridSum<1>(v, red, [&](const double* s) { *out = s[0]; });
+    TILE_LOOP(if (invD[i] != 0) v[0] += f[i] * f[i];)
+    blockPartials<1>(v, red);
 }
 
 __global__ __launch_bounds__(BLOCK) void kMaskCopy64(const double* __restrict__ x0, const double* __restrict__ invD, double* __restrict__ x, int64_t n) {
@@ -457,29 +514,31 @@ __global__ __launch_bounds__(BLOCK) void kMaskCopy64(const double* __restrict__ 
     x[i] = (x0 && invD[i] != 0) ? x0[i] : 0.0;
 }
 
-__global__ __launch_bounds__(BLOCK) void kMaxAbs64(const double* __restrict__ v, int64_t n, double* out, Reduce red) {
+__global__ __launch_bounds__(BLOCK) void kMaxAbs64(const double* __restrict__ v, int64_t n, Reduce red) {
     double m = 0;
-    for (int64_t i = int64_t(blockIdx.x) * BLOCK + threadIdx.x; i < n; i += int64_t(gridDim.x) * BLOCK) m = fmax(m, fabs(v[i]));
-    // max through the sum machinery: the block maxima are combined in thread 0 of the last block
+    TILE_LOOP(m = fmax(m, fabs(v[i]));)
     __shared__ double sh[BLOCK / 64];
     for (int o = 32; o >= 1; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
     __syncthreads();
-    __shared__ int isLast;
     if (threadIdx.x == 0) {
         double b = sh[0];
         for (int w = 1; w < BLOCK / 64; w++) b = fmax(b, sh[w]);
         red.partials[blockIdx.x] = b;
-        __threadfence();
-        isLast = atomicAdd(red.counter, 1u) == gridDim.x - 1;
     }
+}
+
+__global__ __launch_bounds__(BLOCK) void kMaxFinish(Reduce red, unsigned nb, double* out) {
+    double m = 0;
+    for (unsigned b = threadIdx.x; b < nb; b += BLOCK) m = fmax(m, red.partials[b]);
+    __shared__ double sh[BLOCK / 64];
+    for (int o = 32; o >= 1; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
     __syncthreads();
-    if (isLast && threadIdx.x == 0) {
-        __threadfence();
-        double b = 0;
-        for (unsigned q = 0; q < gridDim.x; q++) b = fmax(b, __hip_atomic_load(red.partials + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    if (threadIdx.x == 0) {
+        double b = sh[0];
+        for (int w = 1; w < BLOCK / 64; w++) b = fmax(b, sh[w]);
         *out = b;
-        __hip_atomic_store(red.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
@@ -507,23 +566,21 @@ __global__ __launch_bounds__(BLOCK) void kLzW(double* __restrict__ w, const doub
                                               const double* __restrict__ v, int64_t n, CgState* st, int j, Reduce red) {
     const double b = j > 0 ? st->lz[16 + j - 1] : 0.0;
     double s[1] = {0};
-    for (int64_t i = int64_t(blockIdx.x) * BLOCK + threadIdx.x; i < n; i += int64_t(gridDim.x) * BLOCK) {
+    TILE_LOOP(
         const double wi = sqrt(invD[i]) * w[i] - b * vPrev[i];
         w[i] = wi;
-        s[0] += wi * v[i];
-    }
-    gridSum<1>(s, red, [&](const double* t) { st->lz[j] = t[0]; });
+        s[0] += wi * v[i];)
+    blockPartials<1>(s, red);
 }
 
 __global__ __launch_bounds__(BLOCK) void kLzAxpy(double* __restrict__ w, const double* __restrict__ v, int64_t n, CgState* st, int j, Reduce red) {
     const double a = st->lz[j];
     double s[1] = {0};
-    for (int64_t i = int64_t(blockIdx.x) * BLOCK + threadIdx.x; i < n; i += int64_t(gridDim.x) * BLOCK) {
+    TILE_LOOP(
         const double wi = w[i] - a * v[i];
         w[i] = wi;
-        s[0] += wi * wi;
-    }
-    gridSum<1>(s, red, [&](const double* t) { st->lz[16 + j] = sqrt(t[0]); });
+        s[0] += wi * wi;)
+    blockPartials<1>(s, red);
 }
 
 __global__ __launch_bounds__(BLOCK) void kLzNext(const double* __restrict__ w, double* __restrict__ v, double* __restrict__ vPrev, int64_t n,
@@ -535,6 +592,78 @@ __global__ __launch_bounds__(BLOCK) void kLzNext(const double* __restrict__ w, d
     vPrev[i] = vi;
     v[i] = b > 0 ? w[i] / b : 0.0;
 }
+
+// finalizers of the reductions (thread 0 of the one-block kernel)
+struct FinPQ {
+    CgState* st;
+    __device__ void operator()(const double* s) const {
+        st->pq = s[0];
+        if (!(s[0] > 0)) {  // loss of positive-definiteness (a mechanism)
+            st->broken = 1;
+            st->stop = 1;
+        } else st->alpha = st->rz / s[0];
+    }
+};
+struct FinXR {
+    CgState* st;
+    __device__ void operator()(const double* s) const {
+        st->rr = s[0];
+        st->iters += 1;
+        st->sinceFold += 1;
+        st->rel = sqrt(s[0]) * st->scale / st->bnorm;
+        if (!(st->rel > st->tol) || st->iters >= st->maxIter || !isfinite(st->rel)) st->stop = 1;
+    }
+};
+struct FinRZ {
+    CgState* st;
+    __device__ void operator()(const double* s) const {
+        st->rzNew = s[0];
+        st->zq = s[1];
+        const double b = st->rz != 0 ? -st->alpha * s[1] / st->rz : 0.0;
+        st->beta = b > 0 ? b : 0.0;
+        st->rz = s[0];
+    }
+};
+struct FinXRFold {
+    CgState* st;
+    __device__ void operator()(const double* s) const {
+        st->rr = s[0];
+        st->iters += 1;
+        st->sinceFold += 1;
+        const double est = sqrt(s[0]) * st->scale / st->bnorm;
+        st->rel = est;
+        st->peak = fmax(st->peak, est);
+        const bool fold = !(est > st->tol) || !(est > 0.1 * st->peak) || st->sinceFold >= st->foldEvery || st->iters >= st->maxIter || !isfinite(est);
+        st->noFold = fold ? 0 : 1;
+    }
+};
+struct FinFoldR {
+    CgState* st;
+    __device__ void operator()(const double* s) const {
+        const double rel = sqrt(s[0]) / st->bnorm;
+        st->trueRel = rel;
+        st->rel = rel;
+        st->peak = rel;
+        st->sinceFold = 0;
+        st->folds += 1;
+        st->noFold = 1;
+        if (!(rel > st->tol) || st->iters >= st->maxIter || !isfinite(rel)) st->stop = 1;
+    }
+};
+struct FinStore {
+    double* out;
+    __device__ void operator()(const double* s) const { *out = s[0]; }
+};
+struct FinLzA {
+    CgState* st;
+    int j;
+    __device__ void operator()(const double* s) const { st->lz[j] = s[0]; }
+};
+struct FinLzB {
+    CgState* st;
+    int j;
+    __device__ void operator()(const double* s) const { st->lz[16 + j] = sqrt(s[0]); }
+};
 
 }  // namespace
 
@@ -549,6 +678,17 @@ template <class T> void launchApply(const DevLevel& L, int mode, const T* x, T* 
         case AM_RESID: kApply<T, AM_RESID><<<g, b, 0, s>>>(L, x, y, r, d, T(c1), T(c2), stop); break;
         case AM_CHEB1: kApply<T, AM_CHEB1><<<g, b, 0, s>>>(L, x, y, r, d, T(c1), T(c2), stop); break;
         default: kApply<T, AM_CHEB2><<<g, b, 0, s>>>(L, x, y, r, d, T(c1), T(c2), stop); break;
+    }
+}
+
+template <class T> void launchApplyOwn(const DevLevel& L, int mode, const T* x, T* y, const T* r, T* d, double c1, double c2, const int* stop, hipStream_t s) {
+    const dim3 g(unsigned((L.nNodes + BLOCK / 8 - 1) / (BLOCK / 8))), b(BLOCK);
+    switch (mode) {
+        case AM_APPLY: kApplyOwn<T, AM_APPLY><<<g, b, 0, s>>>(L, x, y, r, d, T(c1), T(c2), stop); break;
+        case AM_RAW: kApplyOwn<T, AM_RAW><<<g, b, 0, s>>>(L, x, y, r, d, T(c1), T(c2), stop); break;
+        case AM_RESID: kApplyOwn<T, AM_RESID><<<g, b, 0, s>>>(L, x, y, r, d, T(c1), T(c2), stop); break;
+        case AM_CHEB1: kApplyOwn<T, AM_CHEB1><<<g, b, 0, s>>>(L, x, y, r, d, T(c1), T(c2), stop); break;
+        default: kApplyOwn<T, AM_CHEB2><<<g, b, 0, s>>>(L, x, y, r, d, T(c1), T(c2), stop); break;
     }
 }
 
@@ -567,44 +707,66 @@ template <class T> void launchProlong(const DevLevel& F, const DevLevel& C, cons
 
 template <class T>
 void launchCoarse(const int32_t* freeIdx, const int32_t* dense, int64_t m, int64_t nDof, const double* Ainv, const T* r, T* z, const int* stop, hipStream_t s) {
-    kCoarse<T><<<unsigned((nDof + 15) / 16), BLOCK, 0, s>>>(freeIdx, dense, int(m), nDof, Ainv, r, z, stop);
+    kCoarse<T><<<unsigned((nDof + BLOCK / 64 - 1) / (BLOCK / 64)), BLOCK, 0, s>>>(freeIdx, dense, int(m), nDof, Ainv, r, z, stop);
 }
 
 template <class T> void launchDotPQ(const T* p, const T* q, int64_t n, CgState* st, Reduce red, hipStream_t s) {
-    kDotPQ<T><<<reduceGrid(n), BLOCK, 0, s>>>(p, q, n, st, red);
+    const unsigned g = reduceGrid(n);
+    kDotPQ<T><<<g, BLOCK, 0, s>>>(p, q, n, st, red);
+    kFinish<1, FinPQ><<<1, BLOCK, 0, s>>>(red, g, &st->stop, FinPQ{st});
 }
 template <class T> void launchUpdateXR(T* x, const T* p, T* r, const T* q, int64_t n, CgState* st, Reduce red, hipStream_t s) {
-    kUpdateXR<T><<<reduceGrid(n), BLOCK, 0, s>>>(x, p, r, q, n, st, red);
+    const unsigned g = reduceGrid(n);
+    kUpdateXR<T><<<g, BLOCK, 0, s>>>(x, p, r, q, n, st, red);
+    kFinish<1, FinXR><<<1, BLOCK, 0, s>>>(red, g, &st->stop, FinXR{st});
 }
 template <class T> void launchDotsRZ(const T* r, const T* z, const T* q, int64_t n, CgState* st, Reduce red, hipStream_t s) {
-    kDotsRZ<T><<<reduceGrid(n), BLOCK, 0, s>>>(r, z, q, n, st, red);
+    const unsigned g = reduceGrid(n);
+    kDotsRZ<T><<<g, BLOCK, 0, s>>>(r, z, q, n, st, red);
+    kFinish<2, FinRZ><<<1, BLOCK, 0, s>>>(red, g, &st->stop, FinRZ{st});
 }
 template <class T> void launchUpdateP(const T* z, T* p, int64_t n, CgState* st, hipStream_t s) { kUpdateP<T><<<flatGrid(n), BLOCK, 0, s>>>(z, p, n, st); }
 
 template <class T> void launchUpdateXRFold(T* x, const T* p, T* r, const T* q, int64_t n, CgState* st, Reduce red, hipStream_t s) {
-    kUpdateXRFold<T><<<reduceGrid(n), BLOCK, 0, s>>>(x, p, r, q, n, st, red);
+    const unsigned g = reduceGrid(n);
+    kUpdateXRFold<T><<<g, BLOCK, 0, s>>>(x, p, r, q, n, st, red);
+    kFinish<1, FinXRFold><<<1, BLOCK, 0, s>>>(red, g, &st->stop, FinXRFold{st});
 }
 void launchFoldX(double* u, float* x, int64_t n, CgState* st, hipStream_t s) { kFoldX<<<flatGrid(n), BLOCK, 0, s>>>(u, x, n, st); }
 void launchFoldR(const double* f, const double* Kx, const double* invD, double* r64, float* r32, int64_t n, CgState* st, Reduce red, hipStream_t s) {
-    kFoldR<<<reduceGrid(n), BLOCK, 0, s>>>(f, Kx, invD, r64, r32, n, st, red);
+    const unsigned g = reduceGrid(n);
+    kFoldR<<<g, BLOCK, 0, s>>>(f, Kx, invD, r64, r32, n, st, red);
+    kFinish<1, FinFoldR><<<1, BLOCK, 0, s>>>(red, g, &st->noFold, FinFoldR{st});
 }
 
 void launchResidual64(const double* f, const double* Kx, const double* invD, double* r, int64_t n, double* out, Reduce red, hipStream_t s) {
-    kResidual64<<<reduceGrid(n), BLOCK, 0, s>>>(f, Kx, invD, r, n, out, red);
+    const unsigned g = reduceGrid(n);
+    kResidual64<<<g, BLOCK, 0, s>>>(f, Kx, invD, r, n, red);
+    kFinish<1, FinStore><<<1, BLOCK, 0, s>>>(red, g, nullptr, FinStore{out});
 }
 void launchNormFree64(const double* f, const double* invD, int64_t n, double* out, Reduce red, hipStream_t s) {
-    kNormFree64<<<reduceGrid(n), BLOCK, 0, s>>>(f, invD, n, out, red);
+    const unsigned g = reduceGrid(n);
+    kNormFree64<<<g, BLOCK, 0, s>>>(f, invD, n, red);
+    kFinish<1, FinStore><<<1, BLOCK, 0, s>>>(red, g, nullptr, FinStore{out});
 }
 void launchMaskCopy64(const double* x0, const double* invD, double* x, int64_t n, hipStream_t s) { kMaskCopy64<<<flatGrid(n), BLOCK, 0, s>>>(x0, invD, x, n); }
-void launchMaxAbs64(const double* v, int64_t n, double* out, Reduce red, hipStream_t s) { kMaxAbs64<<<reduceGrid(n), BLOCK, 0, s>>>(v, n, out, red); }
+void launchMaxAbs64(const double* v, int64_t n, double* out, Reduce red, hipStream_t s) {
+    const unsigned g = reduceGrid(n);
+    kMaxAbs64<<<g, BLOCK, 0, s>>>(v, n, red);
+    kMaxFinish<<<1, BLOCK, 0, s>>>(red, g, out);
+}
 void launchToF32(const double* x, float* y, int64_t n, const double* scaleFrom, hipStream_t s) { kToF32<<<flatGrid(n), BLOCK, 0, s>>>(x, y, n, scaleFrom); }
 void launchFromF32(const float* z, double* y, int64_t n, const double* scaleFrom, hipStream_t s) { kFromF32<<<flatGrid(n), BLOCK, 0, s>>>(z, y, n, scaleFrom); }
 void launchLzX(const double* v, const double* invD, double* x, int64_t n, hipStream_t s) { kLzX<<<flatGrid(n), BLOCK, 0, s>>>(v, invD, x, n); }
 void launchLzW(double* w, const double* invD, const double* vPrev, const double* v, int64_t n, CgState* st, int j, Reduce red, hipStream_t s) {
-    kLzW<<<reduceGrid(n), BLOCK, 0, s>>>(w, invD, vPrev, v, n, st, j, red);
+    const unsigned g = reduceGrid(n);
+    kLzW<<<g, BLOCK, 0, s>>>(w, invD, vPrev, v, n, st, j, red);
+    kFinish<1, FinLzA><<<1, BLOCK, 0, s>>>(red, g, nullptr, FinLzA{st, j});
 }
 void launchLzAxpy(double* w, const double* v, int64_t n, CgState* st, int j, Reduce red, hipStream_t s) {
-    kLzAxpy<<<reduceGrid(n), BLOCK, 0, s>>>(w, v, n, st, j, red);
+    const unsigned g = reduceGrid(n);
+    kLzAxpy<<<g, BLOCK, 0, s>>>(w, v, n, st, j, red);
+    kFinish<1, FinLzB><<<1, BLOCK, 0, s>>>(red, g, nullptr, FinLzB{st, j});
 }
 void launchLzNext(const double* w, double* v, double* vPrev, int64_t n, CgState* st, int j, hipStream_t s) {
     kLzNext<<<flatGrid(n), BLOCK, 0, s>>>(w, v, vPrev, n, st, j);
@@ -613,6 +775,7 @@ void launchLzNext(const double* w, double* v, double* vPrev, int64_t n, CgState*
 #define PSX_INSTANTIATE(T)                                                                                                              \
     template void launchApply<T>(const DevLevel&, int, const T*, T*, const T*, T*, double, double, const int*, hipStream_t);            \
     template void launchChebFirst<T>(const DevLevel&, const T*, T*, T*, double, const int*, hipStream_t);                               \
+    template void launchApplyOwn<T>(const DevLevel&, int, const T*, T*, const T*, T*, double, double, const int*, hipStream_t);         \
     template void launchRestrict<T>(const DevLevel&, const DevLevel&, const T*, T*, const int*, hipStream_t);                          \
     template void launchProlong<T>(const DevLevel&, const DevLevel&, const T*, T*, const int*, hipStream_t);                           \
     template void launchCoarse<T>(const int32_t*, const int32_t*, int64_t, int64_t, const double*, const T*, T*, const int*, hipStream_t); \

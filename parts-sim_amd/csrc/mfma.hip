@@ -56,6 +56,23 @@ __global__ __launch_bounds__(NT) void kFineMfma(DevLevel L, const bf16x8* __rest
     const bf16x8 A0 = Aop[lane], A1 = Aop[64 + lane];
 
     constexpr int NPL = (PX * PY + NT - 1) / NT, NVL = (NV + NT - 1) / NT;
+    constexpr bool NEEDS_R = MODE == AM_RESID || MODE == AM_CHEB1 || MODE == AM_CHEB2;
+    constexpr bool NEEDS_D = MODE == AM_CHEB2;
+    const float* __restrict__ invD = L.invD32;
+    // the epilogue's own operands of node (i, j, z), fetched one step ahead
+    struct Own {
+        float inv[3], r[3], d[3];
+    };
+    auto fetchOwn = [&](int z, Own& o) {
+        const bool in = mine && z < z1;
+        const int64_t q0 = 3 * (in ? i + NX * j + NXY * z : 0);
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            o.inv[d] = in ? invD[q0 + d] : 0.f;
+            if constexpr (NEEDS_R) o.r[d] = in ? r[q0 + d] : 0.f;
+            if constexpr (NEEDS_D) o.d[d] = in ? dv[q0 + d] : 0.f;
+        }
+    };
     float px[NPL][3], pv[NVL];
     auto fetchPlane = [&](int z) {
 #pragma unroll
@@ -63,7 +80,7 @@ __global__ __launch_bounds__(NT) void kFineMfma(DevLevel L, const bf16x8* __rest
             const int e = tid + q * NT;
             const int gi = i0 - 1 + e % PX, gj = j0 - 1 + e / PX;
             const bool in = e < PX * PY && gi >= 0 && gj >= 0 && gi < L.NX && gj < L.NY && z >= 0 && z < L.NZ;
-            const float* p = x + 3 * (gi + NX * gj + NXY * z);
+            const float* p = x + 3 * (in ? gi + NX * gj + NXY * z : 0);
             px[q][0] = in ? p[0] : 0.f;
             px[q][1] = in ? p[1] : 0.f;
             px[q][2] = in ? p[2] : 0.f;
@@ -86,8 +103,8 @@ __global__ __launch_bounds__(NT) void kFineMfma(DevLevel L, const bf16x8* __rest
             const int e = tid + q * NT;
             const int vi = i0 - 1 + e % VX, vj = j0 - 1 + e / VX;
             const bool in = e < NV && vi >= 0 && vj >= 0 && vi < L.nx && vj < L.ny && k >= 0 && k < L.nz;
-            const int32_t w = in ? L.words[vi + int64_t(L.nx) * (vj + int64_t(L.ny) * k)] : 0;
-            pv[q] = w > 0 ? __int_as_float(w) : 0.f;
+            const int32_t w = L.words[in ? vi + int64_t(L.nx) * (vj + int64_t(L.ny) * k) : 0];
+            pv[q] = (in && w > 0) ? __int_as_float(w) : 0.f;
         }
     };
     auto storeVox = [&](int slot) {
@@ -133,21 +150,47 @@ __global__ __launch_bounds__(NT) void kFineMfma(DevLevel L, const bf16x8* __rest
         }
     };
 
-    for (int z = z0 - 1; z <= z0 + 1; z++) {
-        fetchPlane(z);
-        storePlane(z & 3);
+    // prologue: planes z0-1 .. z0+1 and voxel planes z0-1, z0; loads of the next fetch are issued
+    // before the previous one is stored so two round trips overlap
+    fetchPlane(z0 - 1);
+    fetchVox(z0 - 1);
+    {
+        float qx[NPL][3], qv[NVL];
+#pragma unroll
+        for (int q = 0; q < NPL; q++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) qx[q][c] = px[q][c];
+#pragma unroll
+        for (int q = 0; q < NVL; q++) qv[q] = pv[q];
+        fetchPlane(z0);
+        fetchVox(z0);
+#pragma unroll
+        for (int q = 0; q < NPL; q++) {
+            const int e = tid + q * NT;
+            if (e < PX * PY)
+#pragma unroll
+                for (int c = 0; c < 3; c++) sh.x[(z0 + 3) & 3][c][e / PX][e % PX] = qx[q][c];
+        }
+#pragma unroll
+        for (int q = 0; q < NVL; q++) {
+            const int e = tid + q * NT;
+            if (e < NV) sh.s[(z0 + 2) % 3][e / VX][e % VX] = qv[q];
+        }
     }
-    for (int k = z0 - 1; k <= z0; k++) {
-        fetchVox(k);
-        storeVox((k + 3) % 3);
-    }
+    storePlane(z0 & 3);
+    storeVox(z0 % 3);
+    fetchPlane(z0 + 1);
+    storePlane((z0 + 1) & 3);
+    Own cur;
+    fetchOwn(z0, cur);
     __syncthreads();
     products(z0 - 1);
 
-    const float* __restrict__ invD = L.invD32;
     for (int z = z0; z < z1; z++) {
         fetchPlane(z + 2);
         fetchVox(z + 1);
+        Own nxt;
+        fetchOwn(z + 1, nxt);
         products(z);
         __syncthreads();
         if (mine) {
@@ -170,16 +213,16 @@ __global__ __launch_bounds__(NT) void kFineMfma(DevLevel L, const bf16x8* __rest
 #pragma unroll
             for (int d = 0; d < 3; d++) {
                 const int64_t q = 3 * n + d;
-                const float id = invD[q];
+                const float id = cur.inv[d];
                 if constexpr (MODE == AM_APPLY) y[q] = id != 0.f ? acc[d] : 0.f;
                 else if constexpr (MODE == AM_RAW) y[q] = acc[d];
-                else if constexpr (MODE == AM_RESID) y[q] = id != 0.f ? r[q] - acc[d] : 0.f;
+                else if constexpr (MODE == AM_RESID) y[q] = id != 0.f ? cur.r[d] - acc[d] : 0.f;
                 else if constexpr (MODE == AM_CHEB1) {
-                    const float dn = c1 * id * (r[q] - acc[d]);
+                    const float dn = c1 * id * (cur.r[d] - acc[d]);
                     dv[q] = dn;
                     y[q] = sh.x[s0][d][ty + 1][tx + 1] + dn;
                 } else {
-                    const float dn = c1 * dv[q] + c2 * id * (r[q] - acc[d]);
+                    const float dn = c1 * cur.d[d] + c2 * id * (cur.r[d] - acc[d]);
                     dv[q] = dn;
                     y[q] = sh.x[s0][d][ty + 1][tx + 1] + dn;
                 }
@@ -187,6 +230,7 @@ __global__ __launch_bounds__(NT) void kFineMfma(DevLevel L, const bf16x8* __rest
         }
         storePlane((z + 2) & 3);
         storeVox((z + 1) % 3);
+        cur = nxt;
         __syncthreads();
     }
 }

@@ -67,6 +67,16 @@ void buildHierarchy(Hierarchy& H, int nx, int ny, int nz, const float* density, 
 // with DOF c of neighbour m = ox + 3 oy + 9 oz (offsets -1..1 shifted to 0..2).
 void stencil(const double* K, double* S243);
 
+// Symmetry-reduced coefficients of a level's uniform element: out[0..14) element classes,
+// [14..20) stencil diagonal classes, [20..22) stencil off-diagonal classes (class means). Returns the
+// largest deviation of any entry from its class value relative to max |K| (0 for an exactly
+// cube-symmetric element).
+double symmetryClasses(const double* K, double* out22);
+// class of element entry K[3a+d][3b+c] (index | 0x100 when negated) and of stencil entry S[m][d][c]
+// (-1: structural zero)
+int elementClass(int a, int d, int b, int c);
+int stencilClass(int m, int d, int c);
+
 // Lanczos start vector of a level (an LCG with seed 12345, fixed DOFs 0; voxel_fea.cpp:584-588).
 std::vector<double> lanczosStart(const Level& L);
 // largest eigenvalue of a symmetric tridiagonal (Sturm-count bisection; voxel_fea.cpp:506-530)

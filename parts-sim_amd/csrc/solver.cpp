@@ -130,6 +130,11 @@ public:
             D.invD64 = upload(L.invDiag.data(), L.nDof);
             D.invD32 = upload(inv32.data(), L.nDof);
             D.diagAdd = L.diagAdd.empty() ? nullptr : upload(L.diagAdd.data(), L.nDof);
+            {
+                const double dev = symmetryClasses(L.Kb, D.sym64);
+                for (int q = 0; q < 22; q++) D.sym32[q] = float(D.sym64[q]);
+                D.symOk = dev < 1e-12;
+            }
             if (L.nOwn == 0) {
                 std::vector<uint16_t> a(mfmaOperandBytes() / 2);
                 mfmaOperands(L.Kb, a.data());
@@ -162,7 +167,7 @@ public:
         p32 = alloc<float>(n);
         q32 = alloc<float>(n);
         st = alloc<CgState>(1);
-        red.partials = alloc<double>(4 * RED_BLOCKS);
+        red.partials = alloc<double>(4 * (n / RED_TILE + 2));
         red.counter = alloc<unsigned>(1);
         HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&stHost), sizeof(CgState), hipHostMallocDefault));
         std::memset(stHost, 0, sizeof(CgState));
@@ -187,8 +192,9 @@ public:
                 return;
             }
         }
-        if (useFine && H.levels[l].nOwn == 0) launchFine<T>(lv[l].dl, mode, xx, yy, rr, dd, c1, c2, stop, stream);
-        else launchApply<T>(lv[l].dl, mode, xx, yy, rr, dd, c1, c2, stop, stream);
+        if (!useFine) launchApply<T>(lv[l].dl, mode, xx, yy, rr, dd, c1, c2, stop, stream);
+        else if (H.levels[l].nOwn == 0 && lv[l].dl.symOk) launchFine<T>(lv[l].dl, mode, xx, yy, rr, dd, c1, c2, stop, stream);
+        else launchApplyOwn<T>(lv[l].dl, mode, xx, yy, rr, dd, c1, c2, stop, stream);
     }
 
     // ----- multigrid -----

@@ -126,9 +126,11 @@ def test_fine_kernel_matches_generic_kernel(P, W):
             b = P.Model(w["dims"], w["density"], w["nu"], w["bc"], 300)
         finally:
             del os.environ["PSX_FINE"]
+        for l in range(a.num_levels):
+            x = np.random.default_rng(l).standard_normal(a.level(l)["nDof"])
+            for raw in (False, True):
+                assert _rel(a.apply(x, l, raw), b.apply(x, l, raw)) < 1e-14
         x = np.random.default_rng(1).standard_normal(a.n_dof)
-        for raw in (False, True):
-            assert _rel(a.apply(x, 0, raw), b.apply(x, 0, raw)) < 1e-14
         x32 = x.astype(np.float32)
         assert _rel(a.apply32(x32).astype(float), b.apply32(x32).astype(float)) < 1e-6
         r = np.random.default_rng(2).standard_normal(a.n_dof)
