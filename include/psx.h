@@ -69,6 +69,8 @@ typedef struct {
     int fold_every;             /* PSX_PREC_MIXED: true residual at least every N iterations (0 = 20) */
     psx_progress_fn on_progress;
     void* user;
+    int cheb_degree;            /* Chebyshev smoother degree 1..6 (0 = 2, the reference's, voxel_fea.cpp:692-725) */
+    int mfma_smoother;          /* PSX_PREC_MIXED: smooth with the BF16 matrix-core operator (0 = FP32) */
 } psx_solve_opts;
 
 /* Replaces SolveResult's scalars (voxel_fea.hpp:119-124). */
@@ -110,6 +112,12 @@ int psx_set_lmax(psx_model m, int level, double lmax);
 int psx_apply(psx_model m, int level, const double* x, double* y, int raw);
 /* The same in FP32 (VoxelFEA::applyT<float>, voxel_fea.cpp:393). */
 int psx_apply32(psx_model m, int level, const float* x, float* y, int raw);
+
+/* y = K x as the FP64 and mixed solves apply it: own element matrices of a level whose finer level
+ * has none in child-scale form, sum_c s_c P_c^T Kb P_c over its 8 children evaluated in the
+ * working precision, instead of the reference's FP32-rounded stored entries (voxel_fea.cpp:362-375).
+ * fp32 selects float x and y. No reference counterpart (the reference always streams its matrices). */
+int psx_apply_child(psx_model m, int level, int fp32, const void* x, void* y, int raw);
 
 /* y = K x with the BF16 matrix-core operator the mixed-precision V-cycle smooths with (levels
  * without own element matrices; FP32 vectors, BF16 products, FP32 accumulation). No reference

@@ -63,8 +63,12 @@ def lib(kind):
         sig["timed_solve"] = (dbl, [C.c_int, C.c_int, C.c_int, _f32p, dbl, _u8p, i64, _f64p, dbl, C.c_int, _f64p,
                                     C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_double)])
         sig["threads"] = (C.c_uint, [])
+        sig["timed_phases"] = (C.c_int, [C.c_int, C.c_int, C.c_int, _f32p, dbl, _u8p, i64, _f64p, dbl, C.c_int, _f64p,
+                                         C.POINTER(C.c_int), C.POINTER(C.c_double), _f64p])
     else:
         sig["set_threads"] = (None, [C.c_int])
+        sig["set_cheb_degree"] = (None, [vp, C.c_int])
+        sig["smooth"] = (None, [vp, C.c_int, _f32p, _f32p])
     for name, (res, args) in sig.items():
         fn = getattr(L, p + name)
         fn.restype = res
@@ -165,6 +169,16 @@ class Model:
         self._f("prolong_add")(self.h, level, np.ascontiguousarray(zc, np.float64), zf)
         return zf
 
+    def set_cheb_degree(self, degree):
+        self._f("set_cheb_degree")(self.h, int(degree))
+
+    def smooth(self, r, level=0):
+        """The Chebyshev smoother's pre-smoothing pass from z = 0 (FP32)."""
+        r = np.ascontiguousarray(r, np.float32)
+        z = np.zeros_like(r)
+        self._f("smooth")(self.h, int(level), r, z)
+        return z
+
     def prepare_smoothers(self):
         self._f("prepare_smoothers")(self.h)
         return [self.level(l)["lmax"] for l in range(self.num_levels)]
@@ -257,6 +271,18 @@ def ref_timed_solve(dims, density, nu, bc, f, coarsest_max_dof, tol, max_iter):
     if t < 0:
         raise ValueError(L.ref_last_error().decode())
     return {"solve_s": t, "build_s": tb.value, "u": u, "iterations": its.value, "residual": res.value}
+
+
+def ref_timed_phases(dims, density, nu, bc, f, coarsest_max_dof, tol, max_iter):
+    """The reference's build, smoother estimates and max_iter PCG iterations, each timed (seconds)."""
+    L = lib("ref")
+    u = np.zeros(len(f))
+    its, res = C.c_int(), C.c_double()
+    out = np.zeros(3)
+    if L.ref_timed_phases(*[int(d) for d in dims], np.ascontiguousarray(density, np.float32), float(nu), np.ascontiguousarray(bc, np.uint8),
+                          int(coarsest_max_dof), np.ascontiguousarray(f, np.float64), float(tol), int(max_iter), u, C.byref(its), C.byref(res), out) < 0:
+        raise ValueError(L.ref_last_error().decode())
+    return {"build_s": out[0], "smoothers_s": out[1], "solve_s": out[2], "iterations": its.value, "residual": res.value, "u": u}
 
 
 def ref_threads():

@@ -477,6 +477,7 @@ struct Model {
     Hex el;
     std::vector<Level> levels;
     Coarse coarse;
+    int chebDegree = 2;  // the reference's smoother is degree 2 (voxel_fea.cpp:703-725)
 };
 
 template <class T> static void restrictT(const Level& F, const Level& C, const T* rf, T* rc, bool zeroFixed);
@@ -970,8 +971,23 @@ static void coarseSolve64(Model& M, const double* r, double* z) {
     for (i64 i = 0; i < L.nDof; i++) z[i] = co.map[i] >= 0 ? y[co.map[i]] : 0;
 }
 
-// voxel_fea.cpp:703-725
-static void chebyshev(Level& L, bool first) {
+// degree > 2: the same three-term recurrence continued, rho_j = 1 / (2 sigma - rho_(j-1)),
+// d = rho_j rho_(j-1) d + (2 rho_j / delta) D^-1 (r - K z) (Saad, Iterative Methods for Sparse
+// Linear Systems, 2nd ed., Algorithm 12.1); step 1 is the reference's c1, c2 exactly
+static void chebStep(double lmax, int j, double& c1, double& c2) {
+    const double b = 1.15 * lmax, a = 0.1 * lmax;
+    const double theta = (b + a) / 2, delta = (b - a) / 2, sigma = theta / delta;
+    double rho = 1 / sigma, next = rho;
+    for (int k = 1; k <= j; k++) {
+        next = 1 / (2 * sigma - rho);
+        if (k < j) rho = next;
+    }
+    c1 = next * rho;
+    c2 = 2 * next / delta;
+}
+
+// voxel_fea.cpp:703-725 (degree 2), further steps as chebStep
+static void chebyshev(Level& L, bool first, int degree = 2) {
     const Cheb cc = chebCoefficients(L.lmax);
     const float c0 = float(cc.first), c1 = float(cc.c1), c2 = float(cc.c2);
     const float* D = L.invDiag32.data();
@@ -986,10 +1002,20 @@ static void chebyshev(Level& L, bool first) {
             for (i64 i = lo; i < hi; i++) { d[i] = c0 * D[i] * (r[i] - t[i]); z[i] += d[i]; }
         });
     }
+    if (degree < 2) return;
     applyT<float>(L, z, t, false);
     L.forActive([&](i64 lo, i64 hi) {
         for (i64 i = lo; i < hi; i++) { d[i] = c1 * d[i] + c2 * D[i] * (r[i] - t[i]); z[i] += d[i]; }
     });
+    for (int j = 2; j < degree; j++) {
+        double e1, e2;
+        chebStep(L.lmax, j, e1, e2);
+        const float f1 = float(e1), f2 = float(e2);
+        applyT<float>(L, z, t, false);
+        L.forActive([&](i64 lo, i64 hi) {
+            for (i64 i = lo; i < hi; i++) { d[i] = f1 * d[i] + f2 * D[i] * (r[i] - t[i]); z[i] += d[i]; }
+        });
+    }
 }
 
 // voxel_fea.cpp:817-828
@@ -997,13 +1023,13 @@ static void vcycle(Model& M, size_t l) {
     Level& L = M.levels[l];
     if (l + 1 == M.levels.size()) { coarseSolve(M, L); return; }
     Level& C = M.levels[l + 1];
-    chebyshev(L, true);
+    chebyshev(L, true, M.chebDegree);
     applyT<float>(L, L.z.data(), L.t.data(), false);
     L.forActive([&](i64 lo, i64 hi) { for (i64 i = lo; i < hi; i++) L.t[i] = L.r[i] - L.t[i]; });
     restrictT<float>(L, C, L.t.data(), C.r.data(), true);
     vcycle(M, l + 1);
     prolongAdd<float>(L, C, C.z.data(), L.z.data());
-    chebyshev(L, false);
+    chebyshev(L, false, M.chebDegree);
 }
 
 // voxel_fea.cpp:835-861
@@ -1410,6 +1436,18 @@ int orc_prune_floating(int nx, int ny, int nz, const float* density, const uint8
 
 double orc_tridiagonal_max(const double* a, int k, const double* b) {
     return orc::tridiagonalMax(std::vector<double>(a, a + k), std::vector<double>(b, b + std::max(0, k - 1)));
+}
+
+void orc_set_cheb_degree(void* h, int degree) { static_cast<orc::Model*>(h)->chebDegree = degree; }
+
+// z = the smoother's first pass from z = 0 on level l (FP32, as inside the V-cycle)
+void orc_smooth(void* h, int l, const float* r, float* z) {
+    auto* M = static_cast<orc::Model*>(h);
+    orc::prepareSmoothers(*M);
+    orc::Level& L = M->levels[l];
+    std::copy(r, r + L.nDof, L.r.begin());
+    orc::chebyshev(L, true, M->chebDegree);
+    std::copy(L.z.begin(), L.z.end(), z);
 }
 
 void orc_cheb_coefficients(double lmax, double* out3) {

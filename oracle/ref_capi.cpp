@@ -205,6 +205,31 @@ double ref_timed_solve(int nx, int ny, int nz, const float* density, double nu, 
     }
 }
 
+// the same in phases: build, smoother eigenvalue estimates, then maxIter PCG iterations (a bounded
+// sample of a long solve); out[0..2] = seconds of each phase
+int ref_timed_phases(int nx, int ny, int nz, const float* density, double nu, const uint8_t* bc, int64_t coarsestMaxDof, const double* f,
+                     double tol, int maxIter, double* u, int* iterations, double* residual, double* out) {
+    try {
+        const auto t0 = std::chrono::steady_clock::now();
+        void* h = ref_create(nx, ny, nz, density, nu, bc, coarsestMaxDof, nullptr);
+        if (!h) return -1;
+        const auto t1 = std::chrono::steady_clock::now();
+        ref_prepare_smoothers(h);
+        const auto t2 = std::chrono::steady_clock::now();
+        const int ok = ref_solve(h, f, nullptr, tol, maxIter, u, iterations, residual);
+        const auto t3 = std::chrono::steady_clock::now();
+        ref_destroy(h);
+        if (ok < 0) return -1;
+        out[0] = std::chrono::duration<double>(t1 - t0).count();
+        out[1] = std::chrono::duration<double>(t2 - t1).count();
+        out[2] = std::chrono::duration<double>(t3 - t2).count();
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
 unsigned ref_threads() { return ps::ThreadPool::instance().size(); }
 
 }  // extern "C"

@@ -33,7 +33,8 @@ PROGRESS = C.CFUNCTYPE(C.c_int, C.c_int, C.c_double, C.c_void_p)
 
 class SolveOpts(C.Structure):
     _fields_ = [("tol", C.c_double), ("max_iter", C.c_int), ("precision", C.c_int), ("check_every", C.c_int),
-                ("fold_every", C.c_int), ("on_progress", PROGRESS), ("user", C.c_void_p)]
+                ("fold_every", C.c_int), ("on_progress", PROGRESS), ("user", C.c_void_p), ("cheb_degree", C.c_int),
+                ("mfma_smoother", C.c_int)]
 
 
 class SolveInfo(C.Structure):
@@ -61,6 +62,7 @@ SIGNATURES = {
     "psx_apply": (_i, [_vp, _i, _vp, _vp, _i]),
     "psx_apply32": (_i, [_vp, _i, _vp, _vp, _i]),
     "psx_apply_bf16": (_i, [_vp, _i, _vp, _vp]),
+    "psx_apply_child": (_i, [_vp, _i, _i, _vp, _vp, _i]),
     "psx_precondition": (_i, [_vp, _vp, _vp]),
     "psx_precondition_device": (_i, [_vp, _vp, _vp]),
     "psx_solve": (_i, [_vp, _vp, _vp, C.POINTER(SolveOpts), _vp, C.POINTER(SolveInfo)]),
@@ -233,6 +235,15 @@ class Model:
         _check(lib().psx_apply32(self.h, int(level), _p(x), _p(y), int(raw)))
         return y
 
+    def apply_child(self, x, level=0, raw=False):
+        """y = K x as the FP64 and mixed solves apply it (own matrices in child-scale form); float32 x
+        gives the FP32 operator."""
+        fp32 = np.asarray(x).dtype == np.float32
+        x = np.ascontiguousarray(x, np.float32 if fp32 else np.float64)
+        y = np.zeros_like(x)
+        _check(lib().psx_apply_child(self.h, int(level), int(fp32), _p(x), _p(y), int(raw)))
+        return y
+
     def apply_bf16(self, x, level=0):
         x = np.ascontiguousarray(x, np.float32)
         y = np.zeros_like(x)
@@ -246,31 +257,34 @@ class Model:
         return z
 
     @staticmethod
-    def _opts(tol, max_iter, precision, check_every, fold_every, on_progress):
+    def _opts(tol, max_iter, precision, check_every, fold_every, on_progress, cheb_degree=0, mfma_smoother=False):
         cb = PROGRESS(0) if on_progress is None else PROGRESS(lambda it, rel, user: int(bool(on_progress(it, rel))))
-        return SolveOpts(float(tol), int(max_iter), PREC[precision], int(check_every), int(fold_every), cb, None), cb
+        return SolveOpts(float(tol), int(max_iter), PREC[precision], int(check_every), int(fold_every), cb, None, int(cheb_degree),
+                         int(bool(mfma_smoother))), cb
 
     @staticmethod
     def _info(info):
         return {"iterations": info.iterations, "residual": info.residual, "converged": bool(info.converged),
                 "cancelled": bool(info.cancelled), "broken": bool(info.broken), "folds": info.folds, "device_ms": info.device_ms}
 
-    def solve(self, f, tol=1e-6, max_iter=500, x0=None, precision="fp64", check_every=0, fold_every=0, on_progress=None):
+    def solve(self, f, tol=1e-6, max_iter=500, x0=None, precision="fp64", check_every=0, fold_every=0, on_progress=None, cheb_degree=0,
+              mfma_smoother=False):
         f = np.ascontiguousarray(f, np.float64)
         if f.size != self.n_dof or (x0 is not None and np.size(x0) != self.n_dof):
             raise ValueError("Force or displacement vector has the wrong size.")
         x0a = None if x0 is None else np.ascontiguousarray(x0, np.float64)
         u = np.zeros_like(f)
-        o, cb = self._opts(tol, max_iter, precision, check_every, fold_every, on_progress)
+        o, cb = self._opts(tol, max_iter, precision, check_every, fold_every, on_progress, cheb_degree, mfma_smoother)
         info = SolveInfo()
         _check(lib().psx_solve(self.h, _p(f), _p(x0a), C.byref(o), _p(u), C.byref(info)))
         out = self._info(info)
         out["u"] = u
         return out
 
-    def solve_device(self, d_f, d_u, d_x0=None, tol=1e-6, max_iter=500, precision="fp64", check_every=0, fold_every=0):
+    def solve_device(self, d_f, d_u, d_x0=None, tol=1e-6, max_iter=500, precision="fp64", check_every=0, fold_every=0, cheb_degree=0,
+                     mfma_smoother=False):
         """Solve with HBM-resident vectors (DeviceArray or raw device addresses)."""
-        o, cb = self._opts(tol, max_iter, precision, check_every, fold_every, None)
+        o, cb = self._opts(tol, max_iter, precision, check_every, fold_every, None, cheb_degree, mfma_smoother)
         info = SolveInfo()
         addr = lambda a: None if a is None else C.c_void_p(int(a))
         _check(lib().psx_solve_device(self.h, addr(d_f), addr(d_x0), C.byref(o), addr(d_u), C.byref(info)))

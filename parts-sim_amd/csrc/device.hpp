@@ -21,6 +21,16 @@ struct DevLevel {
     const double* St64;        // 27-point stencil of Kb: [27 neighbours][3 rows][3 cols]
     const float* St32;
     const float* nodeScale;    // s for nodes whose 8 voxels are s * Kb, else 0
+    const uint16_t* nodeInfo;  // per node: bit v (0..7) set when voxel v's scale differs from the
+                               // reference voxel's (the lowest clear bit); bit 8 + d when DOF d is
+                               // held or inactive (inverse diagonal 0)
+    // child-scale form of the own matrices (levels whose finer level is all s * Kb): childS[8 e + c]
+    // scales the finer Kb's Galerkin product at child position c; crow = the corner-0 rows of those
+    // 8 products, [c][row][24]. kFine uses it when childForm is set.
+    const float* childS;
+    const double* crow64;
+    const float* crow32;
+    int childForm;
     const double* invD64;      // inverse diagonal, 0 on held / inactive DOFs
     const float* invD32;
     const double* diagAdd;     // optional diagonal shift (nullptr when absent)
@@ -52,6 +62,8 @@ struct Reduce {
 };
 
 enum ApplyMode { AM_APPLY = 0, AM_RAW = 1, AM_RESID = 2, AM_CHEB1 = 3, AM_CHEB2 = 4 };
+// operator forms (bit flags): own elements in child-scale form; BF16 matrix-core operator
+enum OpForm { OF_CHILD = 1, OF_MFMA = 2 };
 
 // y = K x on a level with the epilogue of `mode` (T = double or float):
 //   AM_APPLY  y = K x, held DOFs 0               AM_RAW   y = K x
@@ -92,12 +104,9 @@ template <class T> void launchDotsRZ(const T* r, const T* z, const T* q, int64_t
 // p = z + beta p
 template <class T> void launchUpdateP(const T* z, T* p, int64_t n, CgState* st, hipStream_t s);
 
-// mixed precision: x += alpha p; r -= alpha q; rr; then decides whether to fold (reliable update):
-// fold when the estimate fell to a tenth of its peak since the last fold, every foldEvery
-// iterations, at the tolerance, or at the iteration limit (gpu_fea.cpp:496-557 for the rule)
-template <class T> void launchUpdateXRFold(T* x, const T* p, T* r, const T* q, int64_t n, CgState* st, Reduce red, hipStream_t s);
 // fold, part 1: u += x * rmax, x = 0 (only when a fold is pending)
-void launchFoldX(double* u, float* x, int64_t n, CgState* st, hipStream_t s);
+// mixed: x += alpha rmax p, r -= alpha rmax q (x, r FP64; p, q FP32 scaled), r32 = r / rmax
+void launchUpdateXRMixed(double* x, const float* p, double* r, float* r32, const float* q, int64_t n, CgState* st, Reduce red, hipStream_t s);
 // fold, part 2 (after Kx = K u in FP64): r64 = f - Kx, r32 = r64 / rmax, true residual, stop test
 void launchFoldR(const double* f, const double* Kx, const double* invD, double* r64, float* r32, int64_t n, CgState* st, Reduce red, hipStream_t s);
 

@@ -1,19 +1,23 @@
 // Operator kernel for levels whose elements are all s * Kb (every finest level, and the coarse
 // levels of uniform parts): y = K x with the fused epilogues of kApply (kernels.hip).
 //
-// Layout of the work: a block of 256 threads owns a 32 x 8 column of nodes in x-y and marches along z.
-// Node planes of x (with a one-node halo) and voxel-scale planes are staged in LDS rings; the next
-// plane is fetched from HBM while the current one is computed.
+// Layout of the work: a block of 256 threads owns a 32 x 8 column of nodes in x-y and marches along
+// z. Node planes of x (with a one-node halo) and voxel-scale planes are staged in LDS rings, two
+// planes ahead of use. Each plane is read from LDS once per thread: its 9 x-y neighbours feed three
+// rolling accumulators, the stencil rows of the nodes above, at and below the plane, so a node's
+// 27-point sum is complete one step after its upper neighbour plane arrives.
 //
-// Divergence: 94 % of 64-node wavefronts of a void part contain a node whose 8 voxels differ (the
-// surface of a void). Such a node needs the per-voxel sum (8 x 72 multiply-adds) instead of the
-// assembled stencil (153). Rather than letting those lanes diverge inside every wavefront, each
-// z-step runs in two phases: (A) every uniform node applies the stencil; irregular nodes append
-// themselves to a block list (wavefront ballots, deterministic order); (B) the first threads of
-// the block take the listed nodes and do the per-voxel sums on packed lanes, with the element
-// matrix rows as wave-uniform (scalar) operands. Results go back to the owning thread through LDS.
+// Nodes whose 8 voxels do not all carry the same scale (the surface of a void, the domain boundary)
+// use the difference form K_node x = s_ref S x + sum over differing voxels v of (s_v - s_ref) E_v x,
+// where S is the assembled stencil, s_ref the scale of the node's most common voxel value and E_v
+// the 3 element rows of the node's corner in voxel v. The differing (node, voxel) pairs of a plane
+// (one bit each in a per-node byte precomputed on the host) are packed into a block list and
+// computed one pair per lane, each mirrored onto element corner 0 so that every lane uses the same
+// compile-time coefficients; the owner then adds its pairs in voxel order (deterministic sums).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cstdlib>
 #include <utility>
 
 #include "device.hpp"
@@ -26,11 +30,10 @@ constexpr int TX = 32, TY = 8, NT = TX * TY;
 constexpr int PX = TX + 2, PY = TY + 2;  // node plane with halo
 constexpr int VX = TX + 1, VY = TY + 1;  // voxel plane
 
-__device__ constexpr int corner(int x, int y, int z) { return (y ? (x ? 2 : 3) : (x ? 1 : 0)) + 4 * z; }
 __device__ constexpr int cx(int b) { return (b == 1 || b == 2 || b == 5 || b == 6) ? 1 : 0; }
 __device__ constexpr int cy(int b) { return (b == 2 || b == 3 || b == 6 || b == 7) ? 1 : 0; }
 __device__ constexpr int cz(int b) { return b >= 4 ? 1 : 0; }
-// Symmetry classes (the host's elementClass / stencilClass, hierarchy.cpp): every coefficient is
+// Symmetry classes (the host's elementClass, hierarchy.cpp): every element coefficient is
 // +-one of 22 values, at an index known at compile time once the loops are unrolled, so the values
 // stay in scalar registers for the whole kernel.
 __device__ constexpr int elementClass(int a, int d, int b, int c) {
@@ -54,18 +57,6 @@ __device__ constexpr int elementClass(int a, int d, int b, int c) {
     const int idx = c2 == 0 ? r[0] * 3 + r[1] + r[2] : 6 + r[0] + 2 * r[1] + 4 * r[2];
     return idx | (neg ? 0x100 : 0);
 }
-__device__ constexpr int stencilClass(int m, int d, int c) {
-    const int o[3] = {m % 3 - 1, (m / 3) % 3 - 1, m / 9 - 1};
-    if (d == c) {
-        int other = 0;
-        for (int e = 0; e < 3; e++)
-            if (e != d) other += o[e] != 0;
-        return 14 + (o[d] != 0) * 3 + other;
-    }
-    if (o[d] == 0 || o[c] == 0) return -1;
-    const int e = 3 - d - c;
-    return (20 + (o[e] != 0)) | (o[d] * o[c] < 0 ? 0x100 : 0);
-}
 template <class T> struct Sym {
     T v[22];
 };
@@ -88,19 +79,25 @@ template <> struct FineP<float> {
     __device__ static const float* invD(const DevLevel& L) { return L.invD32; }
 };
 
-template <class T> struct Planes {
-    T x[4][3][PY][PX];      // ring of node planes, one array per component
-    float s[3][VY][VX];     // ring of voxel-scale planes (0 = empty or outside)
-    T res[3][NT];           // phase-B results by local node
-    int list[NT];
-    int waveCount[NT / 64];
+constexpr int NW = NT / 64;    // wavefronts per block
+constexpr int WAVE_PAIRS = 512;  // 64 nodes x 8 voxels
+
+template <class T, bool OWNC> struct Ring {
+    T x[4][3][PY][PX];                 // node planes k-2 .. k+1, one array per component
+    float s[3][VY][VX];                // voxel-scale planes (0 = empty, outside or own matrix)
+    int own[OWNC ? 3 : 1][VY][VX];     // own-matrix index of the voxel, -1 for s * Kb
+    T pres[3][NT];                     // results of one pass of pairs
+    uint16_t list[NW][WAVE_PAIRS];     // pairs: local node << 6 | reference voxel << 3 | voxel
+    int waveTot[2][NW];
 };
 
-template <class T, int MODE>
-__global__ __launch_bounds__(NT) void kFine(DevLevel L, Sym<T> C, const T* __restrict__ x, T* __restrict__ y, const T* __restrict__ r, T* __restrict__ dv,
-                                            T c1, T c2, const int* stop, int zChunk) {
+// OWNC: the level has own elements in child-scale form (sum_c s_c M_c, crow = the corner-0 rows
+// of the 8 M_c, [c][row d][24]); an own voxel is always a differing voxel of its nodes
+template <class T, int MODE, bool OWNC>
+__global__ __launch_bounds__(NT) void kFine(DevLevel L, Sym<T> C, const T* __restrict__ crow, const T* __restrict__ x, T* __restrict__ y,
+                                            const T* __restrict__ r, T* __restrict__ dv, T c1, T c2, const int* stop, int zChunk) {
     if (stop && __builtin_nontemporal_load(stop) != 0) return;
-    __shared__ Planes<T> sh;
+    __shared__ Ring<T, OWNC> sh;
     const int tid = threadIdx.x, tx = tid % TX, ty = tid / TX, lane = tid & 63, wid = tid >> 6;
     const int i0 = blockIdx.x * TX, j0 = blockIdx.y * TY;
     const int z0 = blockIdx.z * zChunk, z1 = min(L.NZ, z0 + zChunk);
@@ -112,14 +109,31 @@ __global__ __launch_bounds__(NT) void kFine(DevLevel L, Sym<T> C, const T* __res
     constexpr bool NEEDS_D = MODE == AM_CHEB2;
 
     constexpr int NPL = (PX * PY + NT - 1) / NT, NVL = (VX * VY + NT - 1) / NT;
-    // global -> register fetches of a node plane / voxel plane (this thread's share)
+    // this thread's share of a node plane / voxel plane: x-y bounds and offsets fixed for the march
+    bool pin[NPL], vin[NVL];
+    int64_t poff[NPL], voff[NVL];
+#pragma unroll
+    for (int q = 0; q < NPL; q++) {
+        const int e = tid + q * NT;
+        const int gi = i0 - 1 + e % PX, gj = j0 - 1 + e / PX;
+        pin[q] = e < PX * PY && gi >= 0 && gj >= 0 && gi < L.NX && gj < L.NY;
+        poff[q] = pin[q] ? 3 * (gi + NX * gj) : 0;
+    }
+#pragma unroll
+    for (int q = 0; q < NVL; q++) {
+        const int e = tid + q * NT;
+        const int vi = i0 - 1 + e % VX, vj = j0 - 1 + e / VX;
+        vin[q] = e < VX * VY && vi >= 0 && vj >= 0 && vi < L.nx && vj < L.ny;
+        voff[q] = vin[q] ? vi + int64_t(L.nx) * vj : 0;
+    }
+    const int64_t nxy = int64_t(L.nx) * L.ny;
     auto fetchPlane = [&](int z, T (&px)[NPL][3]) {
+        const bool zin = z >= 0 && z < L.NZ;
+        const T* base = x + 3 * NXY * (zin ? z : 0);
 #pragma unroll
         for (int q = 0; q < NPL; q++) {
-            const int e = tid + q * NT;
-            const int gi = i0 - 1 + e % PX, gj = j0 - 1 + e / PX;
-            const bool in = e < PX * PY && gi >= 0 && gj >= 0 && gi < L.NX && gj < L.NY && z >= 0 && z < L.NZ;
-            const T* p = x + 3 * (in ? gi + NX * gj + NXY * z : 0);
+            const bool in = pin[q] && zin;
+            const T* p = base + poff[q];
             px[q][0] = in ? p[0] : T(0);
             px[q][1] = in ? p[1] : T(0);
             px[q][2] = in ? p[2] : T(0);
@@ -136,206 +150,339 @@ __global__ __launch_bounds__(NT) void kFine(DevLevel L, Sym<T> C, const T* __res
             }
         }
     };
-    auto fetchVox = [&](int k, float (&pv)[NVL]) {
+    // voxel words in flight: the word itself (scale bits, or the own index with the sign bit)
+    auto fetchVox = [&](int k, int32_t (&pv)[NVL]) {
+        const bool kin = k >= 0 && k < L.nz;
+        const int32_t* base = L.words + nxy * (kin ? k : 0);
+#pragma unroll
+        for (int q = 0; q < NVL; q++) {
+            const bool in = vin[q] && kin;
+            const int32_t w = base[voff[q]];
+            pv[q] = in ? w : 0;
+        }
+    };
+    auto storeVox = [&](int slot, const int32_t (&pv)[NVL]) {
 #pragma unroll
         for (int q = 0; q < NVL; q++) {
             const int e = tid + q * NT;
-            const int vi = i0 - 1 + e % VX, vj = j0 - 1 + e / VX;
-            const bool in = e < VX * VY && vi >= 0 && vj >= 0 && vi < L.nx && vj < L.ny && k >= 0 && k < L.nz;
-            const int32_t w = L.words[in ? vi + int64_t(L.nx) * (vj + int64_t(L.ny) * k) : 0];
-            pv[q] = (in && w > 0) ? __int_as_float(w) : 0.f;
+            if (e < VX * VY) {
+                sh.s[slot][e / VX][e % VX] = pv[q] > 0 ? __int_as_float(pv[q]) : 0.f;
+                if constexpr (OWNC) sh.own[slot][e / VX][e % VX] = pv[q] < 0 ? (pv[q] & 0x7fffffff) : -1;
+            }
         }
     };
-    auto storeVox = [&](int slot, const float (&pv)[NVL]) {
-#pragma unroll
-        for (int q = 0; q < NVL; q++) {
-            const int e = tid + q * NT;
-            if (e < VX * VY) sh.s[slot][e / VX][e % VX] = pv[q];
-        }
-    };
-    // the epilogue's own operands of node (i, j, z): fetched one step ahead
+    // the epilogue's own operands of node (i, j, z), fetched one step before use; info = the node's
+    // differing-voxel bits (0..7) and held-DOF bits (8..10)
+    constexpr bool NEEDS_INV = MODE == AM_CHEB1 || MODE == AM_CHEB2;
     struct Own {
         T inv[3], r[3], d[3];
+        uint32_t info;
     };
+    const int64_t col = mine ? i + NX * j : 0;
     auto fetchOwn = [&](int z, Own& o) {
-        const bool in = mine && z < z1;
-        const int64_t q0 = 3 * (in ? i + NX * j + NXY * z : 0);
+        const bool in = mine && z >= z0 && z < z1;
+        const int64_t nn = col + NXY * (in ? z : 0);
 #pragma unroll
         for (int d = 0; d < 3; d++) {
-            o.inv[d] = in ? invD[q0 + d] : T(0);
-            if constexpr (NEEDS_R) o.r[d] = in ? r[q0 + d] : T(0);
-            if constexpr (NEEDS_D) o.d[d] = in ? dv[q0 + d] : T(0);
+            if constexpr (NEEDS_INV) o.inv[d] = in ? invD[3 * nn + d] : T(0);
+            if constexpr (NEEDS_R) o.r[d] = in ? r[3 * nn + d] : T(0);
+            if constexpr (NEEDS_D) o.d[d] = in ? dv[3 * nn + d] : T(0);
         }
+        o.info = in ? uint32_t(L.nodeInfo[nn]) : 0u;
     };
 
-    // prologue: all loads first, then the LDS stores
+    // prologue: node plane z0 - 1 in LDS; node plane z0 and voxel plane z0 - 2 in flight
+    T pn[NPL][3];
+    int32_t vn[NVL];
     {
-        T pa[NPL][3], pb[NPL][3], pc[NPL][3];
-        float va[NVL], vb[NVL];
+        T pa[NPL][3];
         fetchPlane(z0 - 1, pa);
-        fetchPlane(z0, pb);
-        fetchPlane(z0 + 1, pc);
-        fetchVox(z0 - 1, va);
-        fetchVox(z0, vb);
-        storePlane((z0 + 3) & 3, pa);
-        storePlane(z0 & 3, pb);
-        storePlane((z0 + 1) & 3, pc);
-        storeVox((z0 + 2) % 3, va);
-        storeVox(z0 % 3, vb);
+        fetchPlane(z0, pn);
+        fetchVox(z0 - 2, vn);
+        storePlane((z0 - 1) & 3, pa);
     }
     Own cur;
-    fetchOwn(z0, cur);
+    cur.info = 0;
+    T aN[3] = {0, 0, 0}, aC[3] = {0, 0, 0}, aP[3] = {0, 0, 0}, xcPrev[3] = {0, 0, 0}, xcPrev2[3] = {0, 0, 0};
     __syncthreads();
 
-    for (int z = z0; z < z1; z++) {
-        // prefetch what the next step needs: node plane z+2, voxel plane z+1, own operands of z+1
-        T pn[NPL][3];
-        float vn[NVL];
+    for (int k = z0 - 1; k <= z1; k++) {
+        // stores of the previous step's fetches, then the next fetches
+        storePlane((k + 1) & 3, pn);
+        storeVox((k + 2) % 3, vn);  // voxel plane k - 1
+        fetchPlane(k + 2, pn);
+        fetchVox(k, vn);
         Own nxt;
-        fetchPlane(z + 2, pn);
-        fetchVox(z + 1, vn);
-        fetchOwn(z + 1, nxt);
-        const int sm = (z + 3) & 3, s0 = z & 3, sp = (z + 1) & 3;  // node planes z-1, z, z+1
-        const int vm = (z + 2) % 3, v0 = z % 3;                   // voxel planes z-1, z
-        float sv[8];
-#pragma unroll
-        for (int v = 0; v < 8; v++) sv[v] = sh.s[(v >> 2) ? v0 : vm][ty + ((v >> 1) & 1)][tx + (v & 1)];
-        bool uniform = sv[0] > 0.f;
-#pragma unroll
-        for (int v = 1; v < 8; v++) uniform = uniform && sv[v] == sv[0];
-        uniform = uniform && i > 0 && j > 0 && z > 0 && i < L.nx && j < L.ny && z < L.nz;
-        const bool irregular = mine && !uniform;
-        T acc[3] = {0, 0, 0};
-        // ---- phase A: uniform nodes, assembled stencil, one neighbour plane at a time
-        if (mine && uniform) {
-            staticFor<3>([&](auto OZ) {
-                const int sl = OZ == 0 ? sm : (OZ == 1 ? s0 : sp);
-                T xm[9][3];
-#pragma unroll
-                for (int k = 0; k < 9; k++)
-#pragma unroll
-                    for (int c = 0; c < 3; c++) xm[k][c] = sh.x[sl][c][ty + k / 3][tx + k % 3];
-                staticFor<9>([&](auto K) {
-                    constexpr int m = K + 9 * OZ;
-                    staticFor<3>([&](auto Dd) {
-                        staticFor<3>([&](auto Cc) {
-                            constexpr int cls = stencilClass(m, Dd, Cc);
-                            if constexpr (cls >= 0) acc[Dd] += coefMul<cls, T>(C, xm[K][Cc]);
-                        });
-                    });
-                });
-            });
-            const T s = T(sv[0]);
-            acc[0] *= s;
-            acc[1] *= s;
-            acc[2] *= s;
-        }
-        // irregular nodes -> block list (wavefront ballots, waves in order)
-        const uint64_t mask = __ballot(irregular);
-        if (lane == 0) sh.waveCount[wid] = __popcll(mask);
-        __syncthreads();
-        int base = 0, nIrr = 0;
-#pragma unroll
-        for (int w = 0; w < NT / 64; w++) {
-            base += w < wid ? sh.waveCount[w] : 0;
-            nIrr += sh.waveCount[w];
-        }
-        if (irregular) sh.list[base + __popcll(mask & ((uint64_t(1) << lane) - 1))] = tid;
-        __syncthreads();
-        // ---- phase B: 8 lanes per listed node, one voxel per lane. Voxel v is mirrored onto
-        // corner 0 (neighbour offsets (2 di - 1, 2 dj - 1, 2 dk - 1) * corner bits, components
-        // and rows sign-flipped along mirrored axes), so every lane uses the same element rows.
+        fetchOwn(k, nxt);
+
+        // stencil rows fed by plane k: node k + 1, node k and node k - 1
+        T xm[9][3];
         {
-            const int v = lane & 7, di = v & 1, dj = (v >> 1) & 1, dk = v >> 2;
-            const T sgx = di ? T(1) : T(-1), sgy = dj ? T(1) : T(-1), sgz = dk ? T(1) : T(-1);
-            for (int pass = 0; pass * (NT / 8) < nIrr; pass++) {
-                const int slot = pass * (NT / 8) + wid * 8 + (lane >> 3);
-                const bool act = slot < nIrr;
-                const int o = act ? sh.list[slot] : 0;
-                const int ox0 = o % TX, oy0 = o / TX;
-                const T s = act ? T(sh.s[dk ? v0 : vm][oy0 + dj][ox0 + di]) : T(0);
-                T xq[8][3];
+            const int sl = k & 3;
 #pragma unroll
-                for (int q = 0; q < 8; q++) {
-                    const int qx = (q == 1 || q == 2 || q == 5 || q == 6), qy = (q == 2 || q == 3 || q == 6 || q == 7), qz = q >= 4;
-                    const int lx = ox0 + 1 + (di ? qx : -qx), ly = oy0 + 1 + (dj ? qy : -qy);
-                    const int sl = qz ? (dk ? sp : sm) : s0;
-                    xq[q][0] = sgx * sh.x[sl][0][ly][lx];
-                    xq[q][1] = sgy * sh.x[sl][1][ly][lx];
-                    xq[q][2] = sgz * sh.x[sl][2][ly][lx];
-                }
-                T e[3] = {0, 0, 0};
-                staticFor<3>([&](auto Dd) {
-                    staticFor<8>([&](auto Q) {
-                        staticFor<3>([&](auto Cc) { e[Dd] += coefMul<elementClass(0, Dd, Q, Cc), T>(C, xq[Q][Cc]); });
-                    });
-                });
-                T b0 = s * sgx * e[0], b1 = s * sgy * e[1], b2 = s * sgz * e[2];
+            for (int q = 0; q < 9; q++)
 #pragma unroll
-                for (int off = 4; off >= 1; off >>= 1) {
-                    b0 += __shfl_xor(b0, off, 64);
-                    b1 += __shfl_xor(b1, off, 64);
-                    b2 += __shfl_xor(b2, off, 64);
+                for (int c = 0; c < 3; c++) xm[q][c] = sh.x[sl][c][ty + q / 3][tx + q % 3];
+        }
+        // The stencil in class-sum form: with X_c(ox, oy) the plane's 3 x 3 values of component c and
+        // oz = plane - node, every entry is D[|o_d|][nnz of the other two offsets] on the diagonal and
+        // E[|o_e|] sgn(o_d o_c) off it (zero unless o_d, o_c != 0). Grouping X by those classes gives
+        // the node at the plane (oz = 0) H, the nodes above and below T -+ U; 81 operations per plane
+        // instead of 153 multiply-adds.
+        // Difference form: the stencil sums to zero, so it is applied to the plane's differences from
+        // its centre value, X(ox, oy) - X(0, 0), and the centre column enters once per node as
+        // P (x(z - 1) - 2 x(z) + x(z + 1)), P = the diagonal sum of a neighbour plane's blocks. A
+        // rigid translation then cancels exactly and a rotation to element scale, which keeps the
+        // FP32 operator accurate on the smooth, low-energy modes the conjugate gradients resolve last.
+        {
+            const T D00 = C.v[14], D01 = C.v[15], D02 = C.v[16], D10 = C.v[17], D11 = C.v[18], D12 = C.v[19], E0 = C.v[20], E1 = C.v[21];
+            (void)D00;
+            // q = (ox + 1) + 3 (oy + 1)
+            T xs[3], xd[3], ys[3], yd[3], csum[3], cxy[3], cxs[3], cys[3];
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                const T p = xm[8][c], q = xm[0][c], u = xm[2][c], v = xm[6][c], ctr = xm[4][c];
+                xs[c] = (xm[5][c] + xm[3][c]) - T(2) * ctr;
+                xd[c] = xm[5][c] - xm[3][c];
+                ys[c] = (xm[7][c] + xm[1][c]) - T(2) * ctr;
+                yd[c] = xm[7][c] - xm[1][c];
+                const T s1 = p + q, s2 = u + v;
+                csum[c] = (s1 + s2) - T(4) * ctr;
+                cxy[c] = s1 - s2;
+                cxs[c] = (p + u) - (q + v);  // sum of sgn(ox) X over the corners
+                cys[c] = (p + v) - (q + u);  // sum of sgn(oy) X over the corners
+            }
+            const T G1 = xs[2] + ys[2];
+            if (k >= z0 && k < z1) {
+                aC[0] += D10 * xs[0] + D01 * ys[0] + D11 * csum[0] + E0 * cxy[1];
+                aC[1] += D10 * ys[1] + D01 * xs[1] + D11 * csum[1] + E0 * cxy[0];
+                aC[2] += D01 * G1 + D02 * csum[2];
+            }
+            const bool fN = k + 1 < z1, fP = k - 1 >= z0;
+            if (fN || fP) {
+                const T Tx = D11 * xs[0] + D02 * ys[0] + D12 * csum[0] + E1 * cxy[1];
+                const T Ty = D11 * ys[1] + D02 * xs[1] + D12 * csum[1] + E1 * cxy[0];
+                const T Tz = D11 * G1 + D12 * csum[2];
+                const T Ux = E0 * xd[2] + E1 * cxs[2];
+                const T Uy = E0 * yd[2] + E1 * cys[2];
+                const T Uz = E0 * (xd[0] + yd[1]) + E1 * (cxs[0] + cys[1]);
+                if (fN) {  // oz = -1
+                    aN[0] += Tx - Ux;
+                    aN[1] += Ty - Uy;
+                    aN[2] += Tz - Uz;
                 }
-                if (act && v == 0) {
-                    sh.res[0][o] = b0;
-                    sh.res[1][o] = b1;
-                    sh.res[2][o] = b2;
+                if (fP) {  // oz = +1
+                    aP[0] += Tx + Ux;
+                    aP[1] += Ty + Uy;
+                    aP[2] += Tz + Uz;
                 }
             }
         }
+
+        // node k - 1 is complete: list its differing pairs (per-wave regions, voxel order)
+        const bool done = k - 1 >= z0;
+        const uint32_t mask = done ? (cur.info & 0xffu) : 0u;
+        const int cnt = __popc(mask);
+        const uint64_t lt = (uint64_t(1) << lane) - 1;
+        const uint64_t b0 = __ballot(cnt & 1), b1 = __ballot(cnt & 2), b2 = __ballot(cnt & 4), b3 = __ballot(cnt & 8);
+        const int waveOff = __popcll(b0 & lt) + 2 * __popcll(b1 & lt) + 4 * __popcll(b2 & lt) + 8 * __popcll(b3 & lt);
+        // reference voxel: the lowest clear bit (all 8 differ only when all are own; voxel 0 then
+        // reads scale 0)
+        const int ref = mask == 0xffu ? 0 : __builtin_ctz(~mask);
+        {
+            uint32_t m = mask;
+            int at = waveOff;
+            while (m) {
+                const int v = __builtin_ctz(m);
+                m &= m - 1;
+                sh.list[wid][at++] = uint16_t((tid << 6) | (ref << 3) | v);
+            }
+        }
+        if (lane == 0) sh.waveTot[k & 1][wid] = __popcll(b0) + 2 * __popcll(b1) + 4 * __popcll(b2) + 8 * __popcll(b3);
         __syncthreads();
-        // ---- epilogue by the owning thread
-        if (mine) {
-            if (irregular) {
-                acc[0] = sh.res[0][tid];
-                acc[1] = sh.res[1][tid];
-                acc[2] = sh.res[2][tid];
-            }
-            const int64_t n = i + NX * j + NXY * z;
-            if (L.diagAdd) {
+
+        if (done) {
+            int tot[NW], total = 0, mineBase = 0;
 #pragma unroll
-                for (int d = 0; d < 3; d++) acc[d] += T(L.diagAdd[3 * n + d]) * sh.x[s0][d][ty + 1][tx + 1];
+            for (int w = 0; w < NW; w++) {
+                tot[w] = sh.waveTot[k & 1][w];
+                mineBase += w < wid ? tot[w] : 0;
+                total += tot[w];
             }
+            const int sm = (k + 2) & 3, s0 = (k + 3) & 3, sp = k & 3;  // node planes k - 2, k - 1, k
+            const int vm = (k + 1) % 3, v0 = (k + 2) % 3;              // voxel planes k - 2, k - 1
+            T out[3];
+            {
+                // the centre column's second difference through the neighbour planes' diagonal sums
+                const T Pxy = C.v[15] + T(2) * (C.v[18] + C.v[16]) + T(4) * C.v[19], Pz = C.v[17] + T(4) * (C.v[18] + C.v[19]);
+                const T sref = T(sh.s[(ref >> 2) ? v0 : vm][ty + ((ref >> 1) & 1)][tx + (ref & 1)]);
+                out[0] = sref * (aP[0] + Pxy * ((xcPrev2[0] + xm[4][0]) - T(2) * xcPrev[0]));
+                out[1] = sref * (aP[1] + Pxy * ((xcPrev2[1] + xm[4][1]) - T(2) * xcPrev[1]));
+                out[2] = sref * (aP[2] + Pz * ((xcPrev2[2] + xm[4][2]) - T(2) * xcPrev[2]));
+            }
+            const int mine0 = mineBase + waveOff;
+            for (int pass = 0; pass * NT < total; pass++) {
+                if (pass) __syncthreads();
+                if (pass * NT + wid * 64 < total) {
+                    const int p = pass * NT + wid * 64 + lane;
+                    if (p < total) {
+                        int w = 0, bw = 0, acc = tot[0];
 #pragma unroll
-            for (int d = 0; d < 3; d++) {
-                const int64_t q = 3 * n + d;
-                const T id = cur.inv[d];
-                if constexpr (MODE == AM_APPLY) y[q] = id != T(0) ? acc[d] : T(0);
-                else if constexpr (MODE == AM_RAW) y[q] = acc[d];
-                else if constexpr (MODE == AM_RESID) y[q] = id != T(0) ? cur.r[d] - acc[d] : T(0);
-                else if constexpr (MODE == AM_CHEB1) {
-                    const T dn = c1 * id * (cur.r[d] - acc[d]);
-                    dv[q] = dn;
-                    y[q] = sh.x[s0][d][ty + 1][tx + 1] + dn;
-                } else {
-                    const T dn = c1 * cur.d[d] + c2 * id * (cur.r[d] - acc[d]);
-                    dv[q] = dn;
-                    y[q] = sh.x[s0][d][ty + 1][tx + 1] + dn;
+                        for (int u = 1; u < NW; u++) {
+                            if (p >= acc) {
+                                w = u;
+                                bw = acc;
+                            }
+                            acc += tot[u];
+                        }
+                        const int ent = sh.list[w][p - bw];
+                        const int o = ent >> 6, rv = (ent >> 3) & 7, v = ent & 7;
+                        const int ox0 = o % TX, oy0 = o / TX;
+                        const int di = v & 1, dj = (v >> 1) & 1, dk = v >> 2;
+                        // voxel v mirrored onto corner 0: corner q sits at node offset
+                        // ((2 di - 1) qx, (2 dj - 1) qy, (2 dk - 1) qz); components and rows flip
+                        // sign along mirrored axes
+                        T xr[8][3];
+#pragma unroll
+                        for (int q = 0; q < 8; q++) {
+                            const int qx = cx(q), qy = cy(q), qz = cz(q);
+                            const int lx = ox0 + 1 + (di ? qx : -qx), ly = oy0 + 1 + (dj ? qy : -qy);
+                            const int sl = qz ? (dk ? sp : sm) : s0;
+                            xr[q][0] = sh.x[sl][0][ly][lx];
+                            xr[q][1] = sh.x[sl][1][ly][lx];
+                            xr[q][2] = sh.x[sl][2][ly][lx];
+                        }
+                        // element rows sum to zero: apply them to differences from corner 0 (the node)
+#pragma unroll
+                        for (int q = 1; q < 8; q++)
+#pragma unroll
+                            for (int c = 0; c < 3; c++) xr[q][c] -= xr[0][c];
+                        const T sg[3] = {di ? T(1) : T(-1), dj ? T(1) : T(-1), dk ? T(1) : T(-1)};
+                        const int oi = OWNC ? sh.own[dk ? v0 : vm][oy0 + dj][ox0 + di] : -1;
+                        T res[3];
+                        if (OWNC && oi >= 0) {
+                            // own element sum_c s_c M_c, mirrored: child c' of the mirrored element
+                            // is child c' ^ (7 ^ v) of the original
+                            const float* cs = L.childS + 8 * int64_t(oi);
+                            const int mb = 7 ^ v;
+                            const T sref = T(sh.s[(rv >> 2) ? v0 : vm][oy0 + ((rv >> 1) & 1)][ox0 + (rv & 1)]);
+                            T xs[8][3];
+#pragma unroll
+                            for (int q = 1; q < 8; q++)
+#pragma unroll
+                                for (int c = 0; c < 3; c++) xs[q][c] = sg[c] * xr[q][c];
+                            T o3[3] = {0, 0, 0};
+#pragma unroll
+                            for (int cc = 0; cc < 8; cc++) {
+                                const T wc = T(cs[cc ^ mb]) - sref;
+#pragma unroll
+                                for (int d = 0; d < 3; d++) {
+                                    T t = 0;
+#pragma unroll
+                                    for (int u = 3; u < 24; u++) t += crow[(cc * 3 + d) * 24 + u] * xs[u / 3][u % 3];
+                                    o3[d] += wc * t;
+                                }
+                            }
+#pragma unroll
+                            for (int d = 0; d < 3; d++) res[d] = sg[d] * o3[d];
+                        } else {
+                            const T ds = T(sh.s[dk ? v0 : vm][oy0 + dj][ox0 + di]) - T(sh.s[(rv >> 2) ? v0 : vm][oy0 + ((rv >> 1) & 1)][ox0 + (rv & 1)]);
+                            T Pm[3][3];
+                            staticFor<3>([&](auto Dd) {
+                                staticFor<3>([&](auto Cc) {
+                                    T a = 0;
+                                    staticFor<7>([&](auto Q1) { a += coefMul<elementClass(0, Dd, Q1 + 1, Cc), T>(C, xr[Q1 + 1][Cc]); });
+                                    Pm[Dd][Cc] = a;
+                                });
+                            });
+#pragma unroll
+                            for (int d = 0; d < 3; d++) res[d] = ds * sg[d] * (sg[0] * Pm[d][0] + sg[1] * Pm[d][1] + sg[2] * Pm[d][2]);
+                        }
+#pragma unroll
+                        for (int d = 0; d < 3; d++) sh.pres[d][p - pass * NT] = res[d];
+                    }
+                }
+                __syncthreads();
+                if (cnt) {
+                    const int lo = max(mine0, pass * NT), hi = min(mine0 + cnt, pass * NT + NT);
+                    for (int q = lo; q < hi; q++) {
+                        out[0] += sh.pres[0][q - pass * NT];
+                        out[1] += sh.pres[1][q - pass * NT];
+                        out[2] += sh.pres[2][q - pass * NT];
+                    }
+                }
+            }
+            // epilogue for node k - 1 by the owning thread
+            if (mine) {
+                const int64_t n = i + NX * j + NXY * (k - 1);
+                if (L.diagAdd) {
+#pragma unroll
+                    for (int d = 0; d < 3; d++) out[d] += T(L.diagAdd[3 * n + d]) * xcPrev[d];
+                }
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    const int64_t q = 3 * n + d;
+                    const bool held = (cur.info >> (8 + d)) & 1u;
+                    if constexpr (MODE == AM_APPLY) y[q] = held ? T(0) : out[d];
+                    else if constexpr (MODE == AM_RAW) y[q] = out[d];
+                    else if constexpr (MODE == AM_RESID) y[q] = held ? T(0) : cur.r[d] - out[d];
+                    else if constexpr (MODE == AM_CHEB1) {
+                        const T id = cur.inv[d];
+                        const T dn = c1 * id * (cur.r[d] - out[d]);
+                        dv[q] = dn;
+                        y[q] = xcPrev[d] + dn;
+                    } else {
+                        const T id = cur.inv[d];
+                        const T dn = c1 * cur.d[d] + c2 * id * (cur.r[d] - out[d]);
+                        dv[q] = dn;
+                        y[q] = xcPrev[d] + dn;
+                    }
                 }
             }
         }
-        // rotate: plane z+2 replaces z-2, voxel plane z+1 replaces z-2
-        storePlane((z + 2) & 3, pn);
-        storeVox((z + 1) % 3, vn);
+        // rotate: accumulators move up one node, the centre of plane k becomes node k's x
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            aP[d] = aC[d];
+            aC[d] = aN[d];
+            aN[d] = T(0);
+            xcPrev2[d] = xcPrev[d];
+            xcPrev[d] = xm[4][d];
+        }
         cur = nxt;
-        __syncthreads();
     }
 }
 
 }  // namespace
 
+template <class T, bool OWNC>
+static void launchFineT(const DevLevel& L, int mode, const Sym<T>& C, const T* crow, const T* x, T* y, const T* r, T* d, double c1, double c2,
+                        const int* stop, dim3 g, int zChunk, hipStream_t s) {
+    const dim3 b(NT);
+    switch (mode) {
+        case AM_APPLY: kFine<T, AM_APPLY, OWNC><<<g, b, 0, s>>>(L, C, crow, x, y, r, d, T(c1), T(c2), stop, zChunk); break;
+        case AM_RAW: kFine<T, AM_RAW, OWNC><<<g, b, 0, s>>>(L, C, crow, x, y, r, d, T(c1), T(c2), stop, zChunk); break;
+        case AM_RESID: kFine<T, AM_RESID, OWNC><<<g, b, 0, s>>>(L, C, crow, x, y, r, d, T(c1), T(c2), stop, zChunk); break;
+        case AM_CHEB1: kFine<T, AM_CHEB1, OWNC><<<g, b, 0, s>>>(L, C, crow, x, y, r, d, T(c1), T(c2), stop, zChunk); break;
+        default: kFine<T, AM_CHEB2, OWNC><<<g, b, 0, s>>>(L, C, crow, x, y, r, d, T(c1), T(c2), stop, zChunk); break;
+    }
+}
+
 template <class T> void launchFine(const DevLevel& L, int mode, const T* x, T* y, const T* r, T* d, double c1, double c2, const int* stop, hipStream_t s) {
-    const int zChunk = 8;
     Sym<T> C;
     for (int q = 0; q < 22; q++) C.v[q] = sizeof(T) == 8 ? T(L.sym64[q]) : T(L.sym32[q]);
-    const dim3 g((L.NX + TX - 1) / TX, (L.NY + TY - 1) / TY, (L.NZ + zChunk - 1) / zChunk), b(NT);
-    switch (mode) {
-        case AM_APPLY: kFine<T, AM_APPLY><<<g, b, 0, s>>>(L, C, x, y, r, d, T(c1), T(c2), stop, zChunk); break;
-        case AM_RAW: kFine<T, AM_RAW><<<g, b, 0, s>>>(L, C, x, y, r, d, T(c1), T(c2), stop, zChunk); break;
-        case AM_RESID: kFine<T, AM_RESID><<<g, b, 0, s>>>(L, C, x, y, r, d, T(c1), T(c2), stop, zChunk); break;
-        case AM_CHEB1: kFine<T, AM_CHEB1><<<g, b, 0, s>>>(L, C, x, y, r, d, T(c1), T(c2), stop, zChunk); break;
-        default: kFine<T, AM_CHEB2><<<g, b, 0, s>>>(L, C, x, y, r, d, T(c1), T(c2), stop, zChunk); break;
+    // chunks of 16 planes when that still leaves several blocks per CU, else 8
+    const int gx = (L.NX + TX - 1) / TX, gy = (L.NY + TY - 1) / TY;
+    int zChunk = int64_t(gx) * gy * ((L.NZ + 15) / 16) >= 2048 ? 16 : 8;
+    if (const char* e = std::getenv("PSX_ZCHUNK")) zChunk = std::max(2, std::atoi(e));
+    const dim3 g(gx, gy, (L.NZ + zChunk - 1) / zChunk);
+    if (L.childForm) {
+        const T* crow = nullptr;
+        if constexpr (sizeof(T) == 8) crow = L.crow64;
+        else crow = L.crow32;
+        launchFineT<T, true>(L, mode, C, crow, x, y, r, d, c1, c2, stop, g, zChunk, s);
+    } else {
+        launchFineT<T, false>(L, mode, C, nullptr, x, y, r, d, c1, c2, stop, g, zChunk, s);
     }
 }
 

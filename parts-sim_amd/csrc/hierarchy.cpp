@@ -339,6 +339,33 @@ static void finishLevel(Level& L) {
                 for (int v = 1; v < 8 && same; v++) same = vs[v0 + (v & 1) + i64(L.nx) * (((v >> 1) & 1) + i64(L.ny) * (v >> 2))] == s0;
                 if (same) L.nodeScale[L.node(i, j, k)] = s0;
             }
+    // per node: the voxels (v = di + 2 dj + 4 dk at (i - 1 + di, j - 1 + dj, k - 1 + dk)) whose
+    // scale differs from the node's reference voxel, the lowest-index voxel of the most common
+    // scale (outside voxels count as 0, own-matrix voxels as distinct)
+    L.diffMask.assign(L.nNodes, 0);
+    parallelRange(i64(L.NZ) * L.NY, [&](i64 lo, i64 hi) {
+        for (i64 row = lo; row < hi; row++) {
+            const int j = int(row % L.NY), k = int(row / L.NY);
+            for (int i = 0; i < L.NX; i++) {
+                float sv[8];
+                for (int v = 0; v < 8; v++) {
+                    const int vi = i - 1 + (v & 1), vj = j - 1 + ((v >> 1) & 1), vk = k - 1 + (v >> 2);
+                    const bool in = vi >= 0 && vj >= 0 && vk >= 0 && vi < L.nx && vj < L.ny && vk < L.nz;
+                    sv[v] = in ? vs[vi + i64(L.nx) * (vj + i64(L.ny) * vk)] : 0.f;
+                }
+                int ref = 0, best = 0;
+                for (int v = 0; v < 8; v++) {
+                    int c = 0;
+                    for (int u = 0; u < 8; u++) c += sv[u] == sv[v] && sv[v] >= 0;
+                    if (c > best) best = c, ref = v;
+                }
+                uint8_t m = 0;
+                for (int v = 0; v < 8; v++)
+                    if (!(sv[v] == sv[ref] && sv[v] >= 0)) m |= uint8_t(1u << v);
+                L.diffMask[L.node(i, j, k)] = m;
+            }
+        }
+    }, 16);
 }
 
 // interpolation weight of coarse corner b at fine corner a of child c (trilinear)
@@ -449,6 +476,18 @@ static Level coarsen(const Level& F) {
             for (int u = 0; u < 576; u++) dst[u] = float(out[u]);
         }
     }, 64);
+    // child-scale form: when every fine element is s * Kb, an own coarse element is exactly
+    // sum_c s_c M_c (M_c the Galerkin product of the fine Kb at child position c, s_c = 0 for an
+    // absent child), so 8 scales replace its 576 stored entries
+    if (F.nOwn == 0) {
+        C.childForm = true;
+        C.childS.assign(i64(nOwn) * 8, 0.f);
+        for (i64 e = 0; e < nE; e++) {
+            if (C.own[e] < 0) continue;
+            for (int q = start[e]; q < start[e + 1]; q++) C.childS[i64(C.own[e]) * 8 + pos[kids[q]]] = float(F.rho[kids[q]]);
+        }
+        C.childM.assign(M.begin(), M.end());
+    }
     // a coarse DOF is held when it interpolates onto a held fine DOF
     C.bc.assign(C.nDof, 0);
     for (int k = 0; k < F.NZ; k++)

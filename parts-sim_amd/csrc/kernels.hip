@@ -457,26 +457,21 @@ __global__ __launch_bounds__(BLOCK) void kUpdateP(const T* __restrict__ z, T* __
     p[i] = z[i] + T(st->beta) * p[i];
 }
 
-template <class T>
-__global__ __launch_bounds__(BLOCK) void kUpdateXRFold(T* __restrict__ x, const T* __restrict__ p, T* __restrict__ r, const T* __restrict__ q, int64_t n,
-                                                       CgState* st, Reduce red) {
+// mixed precision: x and r in FP64, p and q in FP32 in units of 1 / rmax; r32 = r / rmax (the
+// preconditioner's input). The FP64 residual keeps the recurrence free of FP32 storage rounding.
+__global__ __launch_bounds__(BLOCK) void kUpdateXRMixed(double* __restrict__ x, const float* __restrict__ p, double* __restrict__ r,
+                                                        float* __restrict__ r32, const float* __restrict__ q, int64_t n, CgState* st, Reduce red) {
     if (st->stop) return;
-    const T a = T(st->alpha);
+    const double s = st->rmax, as = st->alpha * s, inv = 1.0 / s;
     double v[1] = {0};
     TILE_LOOP(
-        x[i] += a * p[i];
-        const T ri = r[i] - a * q[i];
+        x[i] += as * double(p[i]);
+        const double ri = r[i] - as * double(q[i]);
         r[i] = ri;
-        v[0] += double(ri) * double(ri);)
+        const double rs = ri * inv;
+        r32[i] = float(rs);
+        v[0] += rs * rs;)
     blockPartials<1>(v, red);
-}
-
-__global__ __launch_bounds__(BLOCK) void kFoldX(double* __restrict__ u, float* __restrict__ x, int64_t n, const CgState* st) {
-    if (st->noFold) return;
-    const int64_t i = int64_t(blockIdx.x) * BLOCK + threadIdx.x;
-    if (i >= n) return;
-    u[i] += double(x[i]) * st->rmax;
-    x[i] = 0.f;
 }
 
 __global__ __launch_bounds__(BLOCK) void kFoldR(const double* __restrict__ f, const double* __restrict__ Kx, const double* __restrict__ invD,
@@ -727,12 +722,11 @@ template <class T> void launchDotsRZ(const T* r, const T* z, const T* q, int64_t
 }
 template <class T> void launchUpdateP(const T* z, T* p, int64_t n, CgState* st, hipStream_t s) { kUpdateP<T><<<flatGrid(n), BLOCK, 0, s>>>(z, p, n, st); }
 
-template <class T> void launchUpdateXRFold(T* x, const T* p, T* r, const T* q, int64_t n, CgState* st, Reduce red, hipStream_t s) {
+void launchUpdateXRMixed(double* x, const float* p, double* r, float* r32, const float* q, int64_t n, CgState* st, Reduce red, hipStream_t s) {
     const unsigned g = reduceGrid(n);
-    kUpdateXRFold<T><<<g, BLOCK, 0, s>>>(x, p, r, q, n, st, red);
+    kUpdateXRMixed<<<g, BLOCK, 0, s>>>(x, p, r, r32, q, n, st, red);
     kFinish<1, FinXRFold><<<1, BLOCK, 0, s>>>(red, g, &st->stop, FinXRFold{st});
 }
-void launchFoldX(double* u, float* x, int64_t n, CgState* st, hipStream_t s) { kFoldX<<<flatGrid(n), BLOCK, 0, s>>>(u, x, n, st); }
 void launchFoldR(const double* f, const double* Kx, const double* invD, double* r64, float* r32, int64_t n, CgState* st, Reduce red, hipStream_t s) {
     const unsigned g = reduceGrid(n);
     kFoldR<<<g, BLOCK, 0, s>>>(f, Kx, invD, r64, r32, n, st, red);
@@ -782,8 +776,8 @@ void launchLzNext(const double* w, double* v, double* vPrev, int64_t n, CgState*
     template void launchDotPQ<T>(const T*, const T*, int64_t, CgState*, Reduce, hipStream_t);                                         \
     template void launchUpdateXR<T>(T*, const T*, T*, const T*, int64_t, CgState*, Reduce, hipStream_t);                              \
     template void launchDotsRZ<T>(const T*, const T*, const T*, int64_t, CgState*, Reduce, hipStream_t);                              \
-    template void launchUpdateP<T>(const T*, T*, int64_t, CgState*, hipStream_t);                                                     \
-    template void launchUpdateXRFold<T>(T*, const T*, T*, const T*, int64_t, CgState*, Reduce, hipStream_t);
+    template void launchUpdateP<T>(const T*, T*, int64_t, CgState*, hipStream_t);
+
 PSX_INSTANTIATE(double)
 PSX_INSTANTIATE(float)
 

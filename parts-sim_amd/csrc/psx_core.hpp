@@ -42,6 +42,13 @@ struct Level {
     std::vector<uint8_t> bc, fixed, active;
     std::vector<double> invDiag, diagAdd;
     std::vector<float> nodeScale;   // s when the node's 8 voxels are all s * Kb, else 0
+    std::vector<uint8_t> diffMask;  // per node: voxels whose scale differs from the reference voxel
+    // child-scale form of the own matrices (levels whose finer level has no own matrices): own
+    // element e = sum_c childS[8 e + c] * childM[c] (childM: 8 x 576, the Galerkin products of the
+    // finer Kb per child position c = cx + 2 cy + 4 cz)
+    bool childForm = false;
+    std::vector<float> childS;
+    std::vector<double> childM;
     std::vector<int32_t> words;     // device voxel words
     double lmax = 0;
     i64 node(int i, int j, int k) const { return i + i64(NX) * (j + i64(NY) * k); }

@@ -36,16 +36,27 @@ struct NoDevice : std::runtime_error {
         if (e_ != hipSuccess) throw ::psx::DeviceError(std::string(#expr) + " failed: " + hipGetErrorString(e_));     \
     } while (0)
 
-// degree-2 Chebyshev on [0.1, 1.15] lmax: first step 1/theta, then d = c1 d + c2 D^-1 (r - K z)
-// (voxel_fea.cpp:692-701)
+// Chebyshev smoother on [0.1, 1.15] lmax (voxel_fea.cpp:692-725): first step d = D^-1 r / theta,
+// then d = c1[j] d + c2[j] D^-1 (r - K z) for j = 1 .. degree - 1 with the three-term recurrence
+// rho_0 = 1 / sigma, rho_j = 1 / (2 sigma - rho_(j-1)), c1 = rho_j rho_(j-1), c2 = 2 rho_j / delta
+// (degree 2 is the reference's)
+constexpr int CHEB_MAX = 6;
 struct Cheb {
-    double c0, c1, c2;
+    double c0, c1[CHEB_MAX], c2[CHEB_MAX];
 };
-static Cheb chebyshev(double lmax) {
+static Cheb chebyshev(double lmax, int degree) {
     const double b = 1.15 * lmax, a = 0.1 * lmax;
     const double theta = (b + a) / 2, delta = (b - a) / 2, sigma = theta / delta;
-    const double rho0 = 1 / sigma, rho1 = 1 / (2 * sigma - rho0);
-    return {1 / theta, rho1 * rho0, 2 * rho1 / delta};
+    Cheb c{};
+    c.c0 = 1 / theta;
+    double rho = 1 / sigma;
+    for (int j = 1; j < degree; j++) {
+        const double next = 1 / (2 * sigma - rho);
+        c.c1[j] = next * rho;
+        c.c2[j] = 2 * next / delta;
+        rho = next;
+    }
+    return c;
 }
 
 class Model {
@@ -67,7 +78,7 @@ public:
     double* Ainv = nullptr;
     // level-0 conjugate-gradient vectors
     double *x = nullptr, *r = nullptr, *p = nullptr, *q = nullptr, *f = nullptr, *u = nullptr;
-    float *x32 = nullptr, *p32 = nullptr, *q32 = nullptr;
+    float *p32 = nullptr, *q32 = nullptr;
     CgState* st = nullptr;
     CgState* stHost = nullptr;  // pinned mirror
     Reduce red{};
@@ -126,6 +137,27 @@ public:
             D.St64 = upload(S, 243);
             D.St32 = upload(S32, 243);
             D.nodeScale = upload(L.nodeScale.data(), L.nNodes);
+            {
+                std::vector<uint16_t> info(L.nNodes);
+                for (int64_t n = 0; n < L.nNodes; n++) {
+                    uint16_t w = L.diffMask[n];
+                    for (int d = 0; d < 3; d++)
+                        if (L.invDiag[3 * n + d] == 0) w |= uint16_t(1u << (8 + d));
+                    info[n] = w;
+                }
+                D.nodeInfo = upload(info.data(), L.nNodes);
+            }
+            D.childForm = L.childForm && L.nOwn > 0;
+            if (D.childForm) {
+                D.childS = upload(L.childS.data(), int64_t(L.childS.size()));
+                double rows[8 * 72];
+                float rows32[8 * 72];
+                for (int c = 0; c < 8; c++)
+                    for (int u = 0; u < 72; u++) rows[c * 72 + u] = L.childM[size_t(c) * 576 + u];  // rows 0..2 of M_c
+                for (int u = 0; u < 8 * 72; u++) rows32[u] = float(rows[u]);
+                D.crow64 = upload(rows, 8 * 72);
+                D.crow32 = upload(rows32, 8 * 72);
+            }
             std::vector<float> inv32(L.invDiag.begin(), L.invDiag.end());
             D.invD64 = upload(L.invDiag.data(), L.nDof);
             D.invD32 = upload(inv32.data(), L.nDof);
@@ -163,7 +195,6 @@ public:
         q = alloc<double>(n);
         f = alloc<double>(n);
         u = alloc<double>(n);
-        x32 = alloc<float>(n);
         p32 = alloc<float>(n);
         q32 = alloc<float>(n);
         st = alloc<CgState>(1);
@@ -179,22 +210,21 @@ public:
         const char* e = std::getenv("PSX_FINE");
         return !(e && e[0] == '0');
     }();
-    bool useMfma = [] {
-        const char* e = std::getenv("PSX_MFMA");
-        return !(e && e[0] == '0');
-    }();
-    // mfma: the BF16 matrix-core operator (FP32 vectors) where the level allows it
+    // form: 0 = the reference's operator (own matrices as stored FP32 entries); OF_CHILD = own
+    // elements in child-scale form where the level has it (exact Galerkin products; the FP64 and
+    // mixed solves); OF_MFMA = the BF16 matrix-core operator (FP32 vectors) where the level allows it
     template <class T>
-    void op(size_t l, int mode, const T* xx, T* yy, const T* rr, T* dd, double c1, double c2, const int* stop, bool mfma = false) {
+    void op(size_t l, int mode, const T* xx, T* yy, const T* rr, T* dd, double c1, double c2, const int* stop, int form = 0) {
+        const DevLevel& D = lv[l].dl;
         if constexpr (sizeof(T) == 4) {
-            if (mfma && useMfma && lv[l].aop) {
-                launchFineMfma(lv[l].dl, lv[l].aop, mode, xx, yy, rr, dd, c1, c2, stop, stream);
+            if ((form & OF_MFMA) && lv[l].aop) {
+                launchFineMfma(D, lv[l].aop, mode, xx, yy, rr, dd, c1, c2, stop, stream);
                 return;
             }
         }
-        if (!useFine) launchApply<T>(lv[l].dl, mode, xx, yy, rr, dd, c1, c2, stop, stream);
-        else if (H.levels[l].nOwn == 0 && lv[l].dl.symOk) launchFine<T>(lv[l].dl, mode, xx, yy, rr, dd, c1, c2, stop, stream);
-        else launchApplyOwn<T>(lv[l].dl, mode, xx, yy, rr, dd, c1, c2, stop, stream);
+        if (!useFine) launchApply<T>(D, mode, xx, yy, rr, dd, c1, c2, stop, stream);
+        else if (D.symOk && (H.levels[l].nOwn == 0 || ((form & OF_CHILD) && D.childForm))) launchFine<T>(D, mode, xx, yy, rr, dd, c1, c2, stop, stream);
+        else launchApplyOwn<T>(D, mode, xx, yy, rr, dd, c1, c2, stop, stream);
     }
 
     // ----- multigrid -----
@@ -246,33 +276,41 @@ public:
     }
 
     // one V-cycle from rIn on level l (voxel_fea.cpp:817-828); returns the buffer holding z
-    template <class T> T* vcycle(size_t l, const T* rIn, const int* stop, bool mfma = false) {
+    template <class T> T* vcycle(size_t l, const T* rIn, const int* stop, int form = 0, int degree = 2) {
         Vecs<T> V = vecs<T>(l);
         const DevLevel& D = lv[l].dl;
         if (l + 1 == lv.size()) {
             launchCoarse<T>(freeIdx, denseIdx, H.m, D.nDof, Ainv, rIn, V.b, stop, stream);
             return V.b;
         }
-        const Cheb c = chebyshev(H.levels[l].lmax);
+        const Cheb c = chebyshev(H.levels[l].lmax, degree);
         const DevLevel& C = lv[l + 1].dl;
-        launchChebFirst<T>(D, rIn, V.a, V.d, c.c0, stop, stream);
-        op<T>(l, AM_CHEB2, V.a, V.b, rIn, V.d, c.c1, c.c2, stop, mfma);
-        op<T>(l, AM_RESID, V.b, V.a, rIn, nullptr, 0, 0, stop, mfma);
+        T *z = V.a, *t = V.b;
+        launchChebFirst<T>(D, rIn, z, V.d, c.c0, stop, stream);
+        for (int j = 1; j < degree; j++) {
+            op<T>(l, AM_CHEB2, z, t, rIn, V.d, c.c1[j], c.c2[j], stop, form);
+            std::swap(z, t);
+        }
+        op<T>(l, AM_RESID, z, t, rIn, nullptr, 0, 0, stop, form);
         Vecs<T> W = vecs<T>(l + 1);
-        launchRestrict<T>(D, C, V.a, W.r, stop, stream);
-        const T* zc = vcycle<T>(l + 1, W.r, stop, mfma);
-        launchProlong<T>(D, C, zc, V.b, stop, stream);
-        op<T>(l, AM_CHEB1, V.b, V.a, rIn, V.d, c.c0, 0, stop, mfma);
-        op<T>(l, AM_CHEB2, V.a, V.b, rIn, V.d, c.c1, c.c2, stop, mfma);
-        return V.b;
+        launchRestrict<T>(D, C, t, W.r, stop, stream);
+        const T* zc = vcycle<T>(l + 1, W.r, stop, form, degree);
+        launchProlong<T>(D, C, zc, z, stop, stream);
+        op<T>(l, AM_CHEB1, z, t, rIn, V.d, c.c0, 0, stop, form);
+        std::swap(z, t);
+        for (int j = 1; j < degree; j++) {
+            op<T>(l, AM_CHEB2, z, t, rIn, V.d, c.c1[j], c.c2[j], stop, form);
+            std::swap(z, t);
+        }
+        return z;
     }
 
     // z = M r on level 0 with the FP32 cycle and r scaled to order one (voxel_fea.cpp:835-861)
-    void precondition32(const double* dr, double* dz, const int* stop) {
+    void precondition32(const double* dr, double* dz, const int* stop, int degree = 2) {
         const int64_t n = nDof();
         launchMaxAbs64(dr, n, &st->rmax, red, stream);
         launchToF32(dr, lv[0].r32, n, &st->rmax, stream);
-        const float* z32 = vcycle<float>(0, lv[0].r32, stop);
+        const float* z32 = vcycle<float>(0, lv[0].r32, stop, 0, degree);
         launchFromF32(z32, dz, n, &st->rmax, stream);
     }
 
@@ -306,6 +344,11 @@ public:
         double rel = std::sqrt(stHost->rr) / bnorm;
         const int prec = o.precision;
         const int batch = o.check_every > 0 ? o.check_every : 8;
+        // the reference's algorithm keeps its operator; the FP64 and mixed solves use the exact
+        // child-scale form, and the mixed smoother the matrix cores when asked
+        const int form = prec == PSX_PREC_REF ? 0 : (OF_CHILD | (prec == PSX_PREC_MIXED && o.mfma_smoother ? OF_MFMA : 0));
+        const int degree = o.cheb_degree > 0 ? o.cheb_degree : 2;
+        if (degree > CHEB_MAX) throw std::invalid_argument("Chebyshev degree above 6");
         double rmax = 1;
         if (prec == PSX_PREC_MIXED) {
             // FP32 units for the whole solve: the first residual scaled to order one
@@ -332,7 +375,6 @@ public:
         if (prec == PSX_PREC_MIXED) {
             HIPCHK(hipMemsetAsync(p32, 0, size_t(n) * 4, stream));
             HIPCHK(hipMemsetAsync(q32, 0, size_t(n) * 4, stream));
-            HIPCHK(hipMemsetAsync(x32, 0, size_t(n) * 4, stream));
             launchToF32(r, r32, n, &st->rmax, stream);
         } else {
             HIPCHK(hipMemsetAsync(p, 0, size_t(n) * 8, stream));
@@ -341,22 +383,21 @@ public:
         while (!s0.stop) {
             for (int k = 0; k < batch; k++) {
                 if (prec == PSX_PREC_MIXED) {
-                    const float* z = vcycle<float>(0, r32, stop, true);
+                    const float* z = vcycle<float>(0, r32, stop, form, degree);
                     launchDotsRZ<float>(r32, z, q32, n, st, red, stream);
                     launchUpdateP<float>(z, p32, n, st, stream);
                     op<float>(0, AM_APPLY, p32, q32, nullptr, nullptr, 0, 0, stop);
                     launchDotPQ<float>(p32, q32, n, st, red, stream);
-                    launchUpdateXRFold<float>(x32, p32, r32, q32, n, st, red, stream);
-                    // reliable update: u += x, FP64 true residual replaces the recurrence's
-                    launchFoldX(du, x32, n, st, stream);
+                    launchUpdateXRMixed(du, p32, r, r32, q32, n, st, red, stream);
+                    // reliable update: the FP64 true residual replaces the recurrence's
                     op<double>(0, AM_APPLY, du, q, nullptr, nullptr, 0, 0, &st->noFold);
                     launchFoldR(df, q, D0.invD64, r, r32, n, st, red, stream);
                     continue;
                 }
                 const double* z;
-                if (prec == PSX_PREC_FP64) z = vcycle<double>(0, r, stop);
+                if (prec == PSX_PREC_FP64) z = vcycle<double>(0, r, stop, form, degree);
                 else {
-                    precondition32(r, lv[0].b64, stop);
+                    precondition32(r, lv[0].b64, stop, degree);
                     z = lv[0].b64;
                 }
                 launchDotsRZ<double>(r, z, q, n, st, red, stream);
@@ -371,12 +412,6 @@ public:
                 info.cancelled = 1;
                 break;
             }
-        }
-        if (prec == PSX_PREC_MIXED) {
-            // a cancelled run still folds in its last correction
-            const int zero = 0;
-            HIPCHK(hipMemcpyAsync(&st->noFold, &zero, sizeof zero, hipMemcpyHostToDevice, stream));
-            launchFoldX(du, x32, n, st, stream);
         }
         readState();
         info.iterations = stHost->iters;
@@ -551,6 +586,29 @@ int psx_apply32(psx_model m, int level, const float* x, float* y, int raw) {
     });
 }
 
+int psx_apply_child(psx_model m, int level, int fp32, const void* x, void* y, int raw) {
+    return guarded([&] {
+        if (!m || !x || !y || level < 0 || level >= int(m->m->H.levels.size())) throw std::invalid_argument("bad arguments");
+        psx::Model& M = *m->m;
+        auto& V = M.lv[level];
+        const int64_t n = V.dl.nDof;
+        const int mode = raw ? psx::AM_RAW : psx::AM_APPLY;
+        if (fp32) {
+            HIPCHK(hipMemcpyAsync(V.a32, x, size_t(n) * 4, hipMemcpyHostToDevice, M.stream));
+            M.op<float>(size_t(level), mode, V.a32, V.b32, nullptr, nullptr, 0, 0, nullptr, psx::OF_CHILD);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipMemcpyAsync(y, V.b32, size_t(n) * 4, hipMemcpyDeviceToHost, M.stream));
+        } else {
+            HIPCHK(hipMemcpyAsync(V.a64, x, size_t(n) * 8, hipMemcpyHostToDevice, M.stream));
+            M.op<double>(size_t(level), mode, V.a64, V.b64, nullptr, nullptr, 0, 0, nullptr, psx::OF_CHILD);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipMemcpyAsync(y, V.b64, size_t(n) * 8, hipMemcpyDeviceToHost, M.stream));
+        }
+        M.sync();
+        return PSX_OK;
+    });
+}
+
 int psx_apply_bf16(psx_model m, int level, const float* x, float* y) {
     return guarded([&] {
         if (!m || !x || !y || level < 0 || level >= int(m->m->H.levels.size())) throw std::invalid_argument("bad arguments");
@@ -665,22 +723,31 @@ int psx_profile(psx_model m, int reps, char* buf, size_t len) {
             HIPCHK(hipEventElapsedTime(&t, e0, e1));
             return double(t) / reps;
         };
-        char line[256];
+        char line[320];
         for (size_t l = 0; l < M.lv.size(); l++) {
             auto& V = M.lv[l];
             const psx::Level& L = M.H.levels[l];
             const double t64 = timeIt([&] { M.op<double>(l, psx::AM_APPLY, V.a64, V.b64, nullptr, nullptr, 0, 0, nullptr); });
             const double t32 = timeIt([&] { M.op<float>(l, psx::AM_APPLY, V.a32, V.b32, nullptr, nullptr, 0, 0, nullptr); });
-            double tbf = 0;
+            double tbf = 0, c64 = 0, c32 = 0;
             if (V.aop) tbf = timeIt([&] { psx::launchFineMfma(V.dl, V.aop, psx::AM_APPLY, V.a32, V.b32, nullptr, nullptr, 0, 0, nullptr, M.stream); });
-            std::snprintf(line, sizeof line, "L%zu nodes=%lld elems=%zu own matrices=%lld: matvec64 %.4f ms  matvec32 %.4f ms  matvec-bf16 %.4f ms\n", l,
-                          (long long)L.nNodes, L.elemVox.size(), (long long)L.nOwn, t64, t32, tbf);
+            if (V.dl.childForm) {
+                c64 = timeIt([&] { M.op<double>(l, psx::AM_APPLY, V.a64, V.b64, nullptr, nullptr, 0, 0, nullptr, psx::OF_CHILD); });
+                c32 = timeIt([&] { M.op<float>(l, psx::AM_APPLY, V.a32, V.b32, nullptr, nullptr, 0, 0, nullptr, psx::OF_CHILD); });
+            }
+            std::snprintf(line, sizeof line,
+                          "L%zu nodes=%lld elems=%zu own matrices=%lld: matvec64 %.4f ms  matvec32 %.4f ms  matvec-bf16 %.4f ms  child-form64 %.4f ms  child-form32 %.4f ms\n",
+                          l, (long long)L.nNodes, L.elemVox.size(), (long long)L.nOwn, t64, t32, tbf, c64, c32);
             out += line;
         }
-        const double tv64 = timeIt([&] { M.vcycle<double>(0, M.lv[0].r64, nullptr); });
-        const double tv32 = timeIt([&] { M.vcycle<float>(0, M.lv[0].r32, nullptr); });
-        const double tvbf = timeIt([&] { M.vcycle<float>(0, M.lv[0].r32, nullptr, true); });
-        std::snprintf(line, sizeof line, "V-cycle FP64 %.4f ms  FP32 %.4f ms  FP32+BF16 matrix cores %.4f ms\n", tv64, tv32, tvbf);
+        const double tv64 = timeIt([&] { M.vcycle<double>(0, M.lv[0].r64, nullptr, psx::OF_CHILD); });
+        const double tv32 = timeIt([&] { M.vcycle<float>(0, M.lv[0].r32, nullptr, psx::OF_CHILD); });
+        const double tv3 = timeIt([&] { M.vcycle<float>(0, M.lv[0].r32, nullptr, psx::OF_CHILD, 3); });
+        const double tvbf = timeIt([&] { M.vcycle<float>(0, M.lv[0].r32, nullptr, psx::OF_CHILD | psx::OF_MFMA); });
+        const double tvref = timeIt([&] { M.vcycle<float>(0, M.lv[0].r32, nullptr, 0); });
+        std::snprintf(line, sizeof line,
+                      "V-cycle FP64 %.4f ms  FP32 %.4f ms  FP32 degree-3 %.4f ms  FP32+BF16 matrix cores %.4f ms  FP32 stored own matrices %.4f ms\n",
+                      tv64, tv32, tv3, tvbf, tvref);
         out += line;
         hipEventDestroy(e0);
         hipEventDestroy(e1);

@@ -177,3 +177,62 @@ def test_mixed_iterations_close_to_fp64(models):
         b = g.solve(f, tol=1e-8, precision="mixed")
         print(w["name"], "fp64 its", a["iterations"], "mixed its", b["iterations"], "folds", b["folds"], flush=True)
         assert b["converged"] and b["iterations"] <= 1.5 * a["iterations"] + 5
+
+
+def test_child_form_is_the_galerkin_product(P, cpu):
+    """Own coarse elements in child-scale form (sum_c s_c P_c^T K0 P_c, as the FP64 and mixed solves
+    apply them) equal P^T K P of the assembled finest operator to FP64 rounding; the stored FP32
+    entries (the reference's form) differ by their rounding only."""
+    from test_oracle import assemble, prolongation, random_part
+    for dims, soft in (((16, 10, 8), True), ((14, 9, 7), False)):
+        dens = random_part(dims, 7, p_void=0.3, soft=soft)
+        bc = np.zeros(3 * int(np.prod([n + 1 for n in dims])), np.uint8)
+        g = P.Model(dims, dens, 0.3, bc, 60)
+        assert g.level(1)["nOwn"] > 0
+        K1 = (lambda Pm: (Pm.T @ assemble(dims, dens, cpu.hex_element(0.3)[0]) @ Pm).tocsr())(prolongation(dims))
+        x = np.random.default_rng(3).standard_normal(K1.shape[0])
+        y = K1 @ x
+        yc = g.apply_child(x, 1, raw=True)
+        print(dims, "child", np.abs(yc - y).max() / np.abs(y).max(), flush=True)
+        assert np.abs(yc - y).max() < 1e-12 * np.abs(y).max()
+        assert np.abs(g.apply(x, 1, raw=True) - y).max() < 2e-6 * np.abs(y).max()
+        y32 = g.apply_child(x.astype(np.float32), 1, raw=True).astype(np.float64)
+        assert np.abs(y32 - y).max() < 2e-5 * np.abs(y).max()
+
+
+def test_child_form_held_dofs_and_epilogues(P, W):
+    """The child-scale operator with held DOFs against the stored operator (FP32 rounding apart), and
+    a V-cycle built on it against the reference-form V-cycle."""
+    for w in (W.void_block((40, 18, 14), seed=5), W.void_block((21, 13, 11), seed=2, void_density=0.0, void_fraction=0.3)):
+        g = P.Model(w["dims"], w["density"], w["nu"], w["bc"], 300)
+        for l in range(1, g.num_levels):
+            x = np.random.default_rng(l).standard_normal(g.level(l)["nDof"])
+            a, b = g.apply_child(x, l), g.apply(x, l)
+            assert _rel(a, b) < 3e-6
+        s = g.solve(w["f"], tol=1e-10, precision="fp64")
+        r = g.solve(w["f"], tol=1e-10, precision="ref")
+        assert s["converged"] and r["converged"] and _rel(s["u"], r["u"]) < 1e-8
+
+
+@pytest.mark.parametrize("precision", ["ref", "fp64", "mixed"])
+def test_degree3_smoother_matches_oracle(P, W, cpu, precision):
+    """Degree-3 Chebyshev smoothing (the mixed-precision configuration) against the oracle's."""
+    w = W.void_block((48, 16, 16), seed=3)
+    o = cpu.Model(w["dims"], w["density"], w["nu"], w["bc"], 1200)
+    o.set_cheb_degree(3)
+    so = o.solve(w["f"], tol=1e-10, max_iter=500)
+    g = P.Model(w["dims"], w["density"], w["nu"], w["bc"], 1200)
+    s = g.solve(w["f"], tol=1e-10, max_iter=1000, precision=precision, cheb_degree=3)
+    print(precision, "its", s["iterations"], so["iterations"], _rel(s["u"], so["u"]), flush=True)
+    assert s["converged"] and so["converged"] and _rel(s["u"], so["u"]) < 1e-7
+    if precision == "ref":
+        assert abs(s["iterations"] - so["iterations"]) <= 2
+
+
+def test_mixed_with_matrix_core_smoother(P, W):
+    w = W.void_block((48, 16, 16), seed=3)
+    g = P.Model(w["dims"], w["density"], w["nu"], w["bc"], 1200)
+    a = g.solve(w["f"], tol=1e-10, precision="mixed", mfma_smoother=True)
+    b = g.solve(w["f"], tol=1e-10, precision="fp64")
+    print("mfma smoother its", a["iterations"], "fp64", b["iterations"], flush=True)
+    assert a["converged"] and _rel(a["u"], b["u"]) < 1e-7

@@ -210,6 +210,37 @@ def test_lanczos_lmax_bounds(cpu):
     assert abs(c[1] - rho1 / sigma) < 1e-14 and abs(c[2] - 2 * rho1 / delta) < 1e-14 * abs(c[2])
 
 
+@pytest.mark.parametrize("degree", [1, 2, 3, 4])
+def test_chebyshev_smoother_is_the_chebyshev_polynomial(cpu, degree):
+    """From z = 0 with r = K e, the smoother leaves the error p(A) e with A = D^-1 K and
+    p(l) = T_k((theta - l) / delta) / T_k(theta / delta) on [0.1, 1.15] lmax: the closed form of the
+    degree-k Chebyshev iteration (degree 2 is the reference's smoother, voxel_fea.cpp:692-725)."""
+    dims = (5, 4, 3)
+    dens = random_part(dims, 11, p_void=0.2, soft=True)
+    bc = np.zeros(3 * 6 * 5 * 4, np.uint8)
+    bc[: 3 * 6 * 5] = 1  # bottom face held
+    m = cpu.Model(dims, dens, 0.3, bc, coarsest_max_dof=50)
+    m.set_cheb_degree(degree)
+    lm = m.prepare_smoothers()[0]
+    fixed, inv, _, _ = m.level_arrays(0)
+    free = np.flatnonzero(fixed == 0)
+    K = assemble(dims, dens, cpu.hex_element(0.3)[0]).toarray()[np.ix_(free, free)]
+    e = np.zeros(m.n_dof)
+    e[free] = np.random.default_rng(degree).standard_normal(free.size)
+    r = np.zeros(m.n_dof)
+    r[free] = K @ e[free]
+    z = m.smooth(r.astype(np.float32)).astype(np.float64)
+    a, b = 0.1 * lm, 1.15 * lm
+    theta, delta = (a + b) / 2, (b - a) / 2
+    Dh = np.sqrt(inv[free])
+    lam, V = np.linalg.eigh(Dh[:, None] * K * Dh[None, :])
+    T = np.polynomial.chebyshev.Chebyshev.basis(degree)
+    p = T((theta - lam) / delta) / T(theta / delta)
+    want = (V @ (p * (V.T @ (e[free] / Dh)))) * Dh
+    assert np.linalg.norm((e - z)[free] - want) < 2e-5 * np.linalg.norm(e)
+    assert np.abs(z[fixed == 1]).max() == 0
+
+
 def test_tridiagonal_max(cpu):
     a = np.random.default_rng(4).standard_normal(8)
     b = np.random.default_rng(5).standard_normal(7)
