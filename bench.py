@@ -65,6 +65,7 @@ def cpu_baseline(w, coarsest, tol, full_iterations, sample_iterations):
     estimates and a bounded number of PCG iterations, scaled to one full solve of the GPU's
     iteration count (the same algorithm takes the same iterations)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    os.environ.setdefault("PARTS_SIM_THREADS", "16")  # the host share of one GPU
     import cpu
     if not cpu.available("ref"):
         return None
@@ -87,6 +88,8 @@ def main():
     ap.add_argument("--degree", type=int, default=2, help="Chebyshev smoother degree")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-iterations", type=int, default=12)
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="counter runs: three 1 GiB device copies (the calibration kernel), then the per-level operator and V-cycle timings")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (a part or request runs wholly on one GPU)", file=sys.stderr)
@@ -104,6 +107,12 @@ def main():
     m.estimate_smoothers()
     setup_s = time.perf_counter() - t0
     log("setup %.2f s, levels %d, device MB %.0f" % (setup_s, m.num_levels, m.device_bytes / 2 ** 20))
+    if args.kernels_only:
+        a, b = P.DeviceArray(1 << 27), P.DeviceArray(1 << 27)
+        for _ in range(3):
+            b.copy_from(a)
+        print(m.profile(reps=5), flush=True)
+        return
     d_f = P.DeviceArray(n_dof, w["f"])
     d_u = P.DeviceArray(n_dof)
 

@@ -348,6 +348,11 @@ class DeviceArray:
             raise PsxError(ERR_DEVICE, "hipMemcpy D2H failed")
         return h
 
+    def copy_from(self, other):
+        """Device-to-device copy (the runtime's copy kernel), n elements."""
+        if _hiprt().hipMemcpy(self.ptr, other.ptr, 8 * min(self.n, other.n), 3) != 0:
+            raise PsxError(ERR_DEVICE, "hipMemcpy D2D failed")
+
     def free(self):
         if self.ptr:
             _hiprt().hipFree(self.ptr)
