@@ -78,6 +78,68 @@ def cpu_baseline(w, coarsest, tol, full_iterations, sample_iterations):
                       % (r["iterations"], r["solve_s"], per_it, full_iterations, r["build_s"], r["smoothers_s"])}
 
 
+def serve(args, P, W):
+    """Serving workload: a seeded Poisson stream of void-block parts (edges log-uniform 32-256 voxels,
+    aspect 1-4, nu U[0.28, 0.36], 1-8 load cases, 30 % warm re-solves of an earlier part with 1 % of
+    its solid voxels removed and floating voxels pruned, warm-started from the earlier solution)
+    through the device-resident request loop; latency is arrival -> result."""
+    import threading
+    reqs = W.serving_stream(args.serve, seed=5, rate=args.rate)
+    srv = P.Server(max_inflight=args.inflight)
+    parts, results, lock = {}, {}, threading.Lock()
+    done_evt = {r["id"]: threading.Event() for r in reqs}
+
+    def loads(w, nrhs, seed):
+        rng = np.random.default_rng(seed)
+        f = w["f"]
+        return np.stack([f * (1 + 0.5 * k) + 0.05 * np.abs(f).max() * rng.standard_normal(f.size) * (f != 0) for k in range(nrhs)])
+
+    def waiter(r, ticket):
+        u, res = srv.wait(ticket)
+        t = time.perf_counter()
+        with lock:
+            results[r["id"]] = (r, res, t, u)
+        done_evt[r["id"]].set()
+
+    threads = []
+    t0 = time.perf_counter()
+    for r in reqs:
+        delay = t0 + r["t"] - time.perf_counter()
+        if delay > 0:
+            time.sleep(delay)
+        if r["warm"]:
+            done_evt[r["parent"]].wait()
+            base = parts[r["parent"]]
+            dens = W.remove_voxels(base["density"], r["remove_fraction"], r["remove_seed"])
+            dens, _ = P.prune_floating(base["dims"], dens, W.held_nodes(base["bc"]))
+            x0 = results[r["parent"]][3]
+            F = loads(base, r["nrhs"], r["remove_seed"])
+            tk = srv.submit(base["dims"], dens, base["nu"], base["bc"], F, x0=x0, coarsest_max_dof=1200, tol=args.tol, precision="mixed")
+        else:
+            w = W.void_block(r["dims"], seed=r["part_seed"], nu=r["nu"], E=r["E"])
+            parts[r["id"]] = w
+            tk = srv.submit(w["dims"], w["density"], w["nu"], w["bc"], loads(w, r["nrhs"], r["part_seed"]), coarsest_max_dof=1200, tol=args.tol,
+                            precision="mixed")
+        th = threading.Thread(target=waiter, args=(dict(r, arrival=t0 + r["t"]), tk))
+        th.start()
+        threads.append(th)
+    for th in threads:
+        th.join()
+    srv.close()
+    lat = np.array([(t - r["arrival"]) * 1e3 for r, res, t, u in results.values()])
+    span = max(t for r, res, t, u in results.values()) - t0
+    cases = sum(r["nrhs"] for r in reqs)
+    bad = [res for r, res, t, u in results.values() if not res["converged"]]
+    out = {"metric": "serving: solves/s + p50/p99 latency (arrival to result), mixed precision", "requests": len(reqs), "load_cases": cases,
+           "solves_per_s": round(cases / span, 2), "requests_per_s": round(len(reqs) / span, 2), "p50_ms": round(float(np.percentile(lat, 50)), 1),
+           "p99_ms": round(float(np.percentile(lat, 99)), 1), "max_inflight": args.inflight, "arrival_rate": args.rate, "unconverged": len(bad),
+           "mean_build_ms": round(float(np.mean([res["build_ms"] for r, res, t, u in results.values()])), 1),
+           "mean_solve_ms": round(float(np.mean([res["solve_ms"] for r, res, t, u in results.values()])), 1),
+           "mean_queue_ms": round(float(np.mean([res["queue_ms"] for r, res, t, u in results.values()])), 1),
+           "dof_total": int(sum(3 * W.nodes(r["dims"]) * r["nrhs"] for r in reqs)), "tol": args.tol}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -88,6 +150,9 @@ def main():
     ap.add_argument("--degree", type=int, default=2, help="Chebyshev smoother degree")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-iterations", type=int, default=12)
+    ap.add_argument("--serve", type=int, default=0, help="serving workload: this many requests of the Poisson stream")
+    ap.add_argument("--rate", type=float, default=200.0, help="serving arrival rate (requests/s)")
+    ap.add_argument("--inflight", type=int, default=4, help="serving: requests on the device at once")
     ap.add_argument("--kernels-only", action="store_true",
                     help="counter runs: three 1 GiB device copies (the calibration kernel), then the per-level operator and V-cycle timings")
     args = ap.parse_args()
@@ -96,6 +161,9 @@ def main():
         sys.exit(2)
     P = _package()
     import parts_sim_amd.workloads as W
+    if args.serve:
+        serve(args, P, W)
+        return
 
     dims = tuple(int(v) for v in args.dims.split(","))
     coarsest = 1200

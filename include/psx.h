@@ -138,17 +138,79 @@ int psx_solve_device(psx_model m, const double* d_f, const double* d_x0, const p
  * voxel_fea.cpp:929-940). f may be NULL. */
 int psx_reactions(psx_model m, const double* u, const double* f, double out[3]);
 
+/* Nodal stresses of a solution u (normalised units, nDof values): the density-weighted average over
+ * each active node's voxels of the corner stress (E / h) D Bbar u_e; von Mises always, the first and
+ * third principal stresses when principals != 0 (p1, p3 may be NULL otherwise); nNodes floats each,
+ * 0 at inactive nodes (VoxelFEA::stresses, voxel_fea.cpp:968-1037). */
+int psx_stresses(psx_model m, const double* u, double E, double h, int principals, float* vm, float* p1, float* p3);
+/* The same with HBM-resident u and outputs. */
+int psx_stresses_device(psx_model m, const double* d_u, double E, double h, int principals, float* d_vm, float* d_p1, float* d_p3);
+/* The averaged 6-component stress tensor per node, [node][xx yy zz xy yz zx] (VoxelFEA::stressTensors,
+ * voxel_fea.cpp:1039-1084). */
+int psx_stress_tensors(psx_model m, const double* u, double E, double h, float* out);
+
 /* Per-level kernel timings, one line each (GpuFeaSolver::profile, gpu_fea.cpp:576-652). */
 int psx_profile(psx_model m, int reps, char* buf, size_t len);
 
 /* Device memory of the model's buffers in bytes (hierarchy + work vectors). */
 int64_t psx_device_bytes(psx_model m);
 
+/* ---- serving: a device-resident request loop with continuous batching ----
+ * No reference counterpart: the reference solves one part at a time behind a global GPU lock
+ * (gpu.hpp:43, gpu_fea.cpp:256-578). A server admits queued solve requests in arrival order as
+ * slots (max_inflight) and device memory (hbm_budget) free up; each admitted request builds its
+ * part's hierarchy on the host, uploads it and solves its load cases on its own HIP stream, so
+ * requests that arrive while others run overlap on the device. Every result is the same as the
+ * request's solve alone (per-request deterministic reductions, no shared device state). */
+typedef struct psx_server_s* psx_server;
+typedef struct {
+    int max_inflight;           /* requests on the device at once (0 = 4) */
+    int64_t hbm_budget;         /* device bytes requests may hold at once (0 = 80 % of the free memory) */
+} psx_server_opts;
+typedef struct {
+    psx_model_desc part;        /* copied at submit */
+    int nrhs;                   /* load cases (>= 1) */
+    const double* f;            /* nrhs x n_dof, copied at submit */
+    const double* x0;           /* NULL, n_dof (every load case) or nrhs x n_dof (x0_per_rhs) */
+    int x0_per_rhs;
+    int64_t part_id;            /* >= 0: keep the part's model resident for later requests with the same
+                                   id, which must then carry the same part (ids < 0: build afresh) */
+    psx_solve_opts opts;        /* on_progress is not called */
+} psx_request;
+typedef struct {
+    int status;                 /* PSX_OK or the error of the request */
+    int iterations;             /* summed over the load cases */
+    int max_iterations;
+    int converged;              /* every load case */
+    double residual;            /* largest final true relative residual */
+    double queue_ms;            /* submit -> admitted */
+    double build_ms;            /* hierarchy build + upload + smoother estimates (0 when resident) */
+    double solve_ms;            /* all load cases incl. transfers */
+    double total_ms;            /* submit -> done */
+    int64_t device_bytes;       /* the model's device memory */
+    int reused_part;            /* the model was resident */
+} psx_result;
+int psx_server_create(const psx_server_opts* opts, psx_server* out);
+/* Queues a request (non-blocking); ticket identifies it for psx_server_wait. */
+int psx_server_submit(psx_server s, const psx_request* req, int64_t* ticket);
+/* Blocks until the request is done; u (nrhs x n_dof, may be NULL) receives the solutions. The
+ * return value is the request's status; the ticket is released. */
+int psx_server_wait(psx_server s, int64_t ticket, double* u, psx_result* out);
+/* Requests queued or running. */
+int psx_server_pending(psx_server s);
+/* Finishes the running requests, fails the queued ones, frees everything. */
+void psx_server_destroy(psx_server s);
+
 /* ---- host-side inspection (no device needed): the hierarchy exactly as it is uploaded ---- */
 typedef struct psx_hierarchy_s* psx_hierarchy;
 /* Element matrix K (576) and corner stresses D*Bbar (8 x 6 x 24) of the unit cube for E = 1
  * (hexElement, native/src/fea/hex8.cpp:104-155). */
 int psx_hex_element(double nu, double* K, double* corner_stress);
+/* Input rule before a re-solve of a damaged part: solid voxels (density > 0) not face-connected to a
+ * voxel touching a held node are set to 0 in out (may alias density). held: per node (any DOF held,
+ * static_study.cpp:17-21). Returns the number removed, or a negative status (pruneFloating,
+ * voxel_fea.cpp:1086-1119). */
+int psx_prune_floating(int nx, int ny, int nz, const float* density, const uint8_t* held, float* out);
 /* Builds the hierarchy on the host only (VoxelFEA ctor, voxel_fea.cpp:138-183). */
 int psx_hierarchy_create(const psx_model_desc* desc, psx_hierarchy* out);
 void psx_hierarchy_destroy(psx_hierarchy h);

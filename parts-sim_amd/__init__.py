@@ -37,6 +37,21 @@ class SolveOpts(C.Structure):
                 ("mfma_smoother", C.c_int)]
 
 
+class ServerOpts(C.Structure):
+    _fields_ = [("max_inflight", C.c_int), ("hbm_budget", C.c_int64)]
+
+
+class Request(C.Structure):
+    _fields_ = [("part", ModelDesc), ("nrhs", C.c_int), ("f", C.c_void_p), ("x0", C.c_void_p), ("x0_per_rhs", C.c_int),
+                ("part_id", C.c_int64), ("opts", SolveOpts)]
+
+
+class Result(C.Structure):
+    _fields_ = [("status", C.c_int), ("iterations", C.c_int), ("max_iterations", C.c_int), ("converged", C.c_int), ("residual", C.c_double),
+                ("queue_ms", C.c_double), ("build_ms", C.c_double), ("solve_ms", C.c_double), ("total_ms", C.c_double),
+                ("device_bytes", C.c_int64), ("reused_part", C.c_int)]
+
+
 class SolveInfo(C.Structure):
     _fields_ = [("iterations", C.c_int), ("residual", C.c_double), ("converged", C.c_int), ("cancelled", C.c_int),
                 ("broken", C.c_int), ("folds", C.c_int), ("device_ms", C.c_double)]
@@ -68,8 +83,17 @@ SIGNATURES = {
     "psx_solve": (_i, [_vp, _vp, _vp, C.POINTER(SolveOpts), _vp, C.POINTER(SolveInfo)]),
     "psx_solve_device": (_i, [_vp, _vp, _vp, C.POINTER(SolveOpts), _vp, C.POINTER(SolveInfo)]),
     "psx_reactions": (_i, [_vp, _vp, _vp, _vp]),
+    "psx_stresses": (_i, [_vp, _vp, _d, _d, _i, _vp, _vp, _vp]),
+    "psx_stresses_device": (_i, [_vp, _vp, _d, _d, _i, _vp, _vp, _vp]),
+    "psx_stress_tensors": (_i, [_vp, _vp, _d, _d, _vp]),
     "psx_profile": (_i, [_vp, _i, C.c_char_p, C.c_size_t]),
     "psx_device_bytes": (_i64, [_vp]),
+    "psx_server_create": (_i, [C.POINTER(ServerOpts), C.POINTER(_vp)]),
+    "psx_server_submit": (_i, [_vp, C.POINTER(Request), C.POINTER(_i64)]),
+    "psx_server_wait": (_i, [_vp, _i64, _vp, C.POINTER(Result)]),
+    "psx_server_pending": (_i, [_vp]),
+    "psx_server_destroy": (None, [_vp]),
+    "psx_prune_floating": (_i, [_i, _i, _i, _vp, _vp, _vp]),
     "psx_hex_element": (_i, [_d, _vp, _vp]),
     "psx_hierarchy_create": (_i, [C.POINTER(ModelDesc), C.POINTER(_vp)]),
     "psx_hierarchy_destroy": (None, [_vp]),
@@ -135,6 +159,20 @@ def _level_info(fn, h, l):
     _check(fn(h, l, C.byref(li)))
     return {"dims": (li.nx, li.ny, li.nz), "nNodes": li.n_nodes, "nDof": li.n_dof, "freeDof": li.free_dof,
             "nElems": li.n_elems, "nOwn": li.n_own, "lmax": li.lmax}
+
+
+def prune_floating(dims, density, held):
+    """Density with the voxels not face-connected to a held node removed; (density, removed)."""
+    nx, ny, nz = (int(v) for v in dims)
+    d = np.ascontiguousarray(density, np.float32).reshape(-1)
+    h = np.ascontiguousarray(held, np.uint8).reshape(-1)
+    if d.size != nx * ny * nz or h.size != (nx + 1) * (ny + 1) * (nz + 1):
+        raise ValueError("Structural grid, density and fixture sizes do not match.")
+    out = np.zeros_like(d)
+    r = lib().psx_prune_floating(nx, ny, nz, _p(d), _p(h), _p(out))
+    if r < 0:
+        raise PsxError(-r, lib().psx_last_error().decode())
+    return out, r
 
 
 def hex_element(nu):
@@ -296,6 +334,20 @@ class Model:
         _check(lib().psx_reactions(self.h, _p(np.ascontiguousarray(u, np.float64)), _p(fa), _p(R)))
         return R
 
+    def stresses(self, u, E, h, principals=True):
+        """Nodal von Mises (and P1, P3) of a solution u, float32 per node."""
+        nN = self.level(0)["nNodes"]
+        vm = np.zeros(nN, np.float32)
+        p1 = np.zeros(nN, np.float32) if principals else None
+        p3 = np.zeros(nN, np.float32) if principals else None
+        _check(lib().psx_stresses(self.h, _p(np.ascontiguousarray(u, np.float64)), float(E), float(h), int(principals), _p(vm), _p(p1), _p(p3)))
+        return vm, p1, p3
+
+    def stress_tensors(self, u, E, h):
+        out = np.zeros(6 * self.level(0)["nNodes"], np.float32)
+        _check(lib().psx_stress_tensors(self.h, _p(np.ascontiguousarray(u, np.float64)), float(E), float(h), _p(out)))
+        return out.reshape(-1, 6)
+
     def profile(self, reps=20):
         buf = C.create_string_buffer(1 << 16)
         _check(lib().psx_profile(self.h, int(reps), buf, len(buf)))
@@ -360,3 +412,57 @@ class DeviceArray:
 
     def __del__(self):
         self.free()
+
+
+class Server:
+    """A device-resident request loop (include/psx.h psx_server_*): requests are admitted as slots and
+    device memory free up and run concurrently, each on its own stream."""
+
+    def __init__(self, max_inflight=4, hbm_budget=0):
+        h = C.c_void_p()
+        _check(lib().psx_server_create(C.byref(ServerOpts(int(max_inflight), int(hbm_budget))), C.byref(h)))
+        self.h = h
+        self._keep = {}
+
+    def submit(self, dims, density, nu, bc, f, x0=None, part_id=-1, coarsest_max_dof=1100, tol=1e-6, max_iter=500, precision="fp64",
+               cheb_degree=0, diag_add=None):
+        """Queues one part with its load cases f (nrhs x nDof or nDof); returns a ticket."""
+        keep = []
+        d = _desc(dims, density, nu, bc, coarsest_max_dof, diag_add, 0, keep)
+        f = np.ascontiguousarray(f, np.float64)
+        n = 3 * int(np.prod([int(v) + 1 for v in dims]))
+        nrhs = f.size // n
+        if f.size != nrhs * n or nrhs < 1:
+            raise ValueError("Force or displacement vector has the wrong size.")
+        x0a = None if x0 is None else np.ascontiguousarray(x0, np.float64)
+        if x0a is not None and x0a.size not in (n, nrhs * n):
+            raise ValueError("Force or displacement vector has the wrong size.")
+        o, _ = Model._opts(tol, max_iter, precision, 0, 0, None, cheb_degree)
+        req = Request(d, nrhs, f.ctypes.data, None if x0a is None else x0a.ctypes.data, int(x0a is not None and x0a.size == nrhs * n),
+                      int(part_id), o)
+        t = C.c_int64()
+        _check(lib().psx_server_submit(self.h, C.byref(req), C.byref(t)))
+        self._keep[t.value] = (nrhs, n)
+        return t.value
+
+    def wait(self, ticket):
+        """Blocks until the request is done: (u as nrhs x nDof, result dict)."""
+        nrhs, n = self._keep.pop(ticket)
+        u = np.zeros((nrhs, n))
+        r = Result()
+        code = lib().psx_server_wait(self.h, int(ticket), _p(u), C.byref(r))
+        out = {k: getattr(r, k) for k, _ in Result._fields_}
+        if code != OK:
+            raise PsxError(code, lib().psx_last_error().decode())
+        return u, out
+
+    def pending(self):
+        return lib().psx_server_pending(self.h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().psx_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()

@@ -30,6 +30,11 @@ struct DevLevel {
     const float* childS;
     const double* crow64;
     const float* crow32;
+    // two-scale own elements: [a, b, mask of the b children or -1, 0] per own element and the
+    // subset table [256][64] (corner-0 rows of sum_{c in B} M_c over corners 1..7)
+    const int32_t* childTwo;  // 4 per own element, 16-byte aligned
+    const double* ctab64;
+    const float* ctab32;
     int childForm;
     const double* invD64;      // inverse diagonal, 0 on held / inactive DOFs
     const float* invD32;
@@ -68,24 +73,26 @@ enum OpForm { OF_CHILD = 1, OF_MFMA = 2 };
 // y = K x on a level with the epilogue of `mode` (T = double or float):
 //   AM_APPLY  y = K x, held DOFs 0               AM_RAW   y = K x
 //   AM_RESID  y = r - K x, held DOFs 0
-//   AM_CHEB1  d = c1 D^-1 (r - K x);        y = x + d
-//   AM_CHEB2  d = c1 d + c2 D^-1 (r - K x); y = x + d
-template <class T> void launchApply(const DevLevel& L, int mode, const T* x, T* y, const T* r, T* d, double c1, double c2, const int* stop,
+//   AM_CHEB1  y = x + c1 D^-1 (r - K x)                        (first step with a nonzero start)
+//   AM_CHEB2  y = x + c1 (x - zp) + c2 D^-1 (r - K x)          (later steps; zp = the previous
+//             iterate, nullptr for a zero one: the step d = y - x needs no vector of its own)
+// y must not alias x or zp.
+template <class T> void launchApply(const DevLevel& L, int mode, const T* x, T* y, const T* r, const T* zp, double c1, double c2, const int* stop,
                                     hipStream_t s);
 // the same with 8 lanes per node (one per voxel): levels with own matrices
-template <class T> void launchApplyOwn(const DevLevel& L, int mode, const T* x, T* y, const T* r, T* d, double c1, double c2, const int* stop,
+template <class T> void launchApplyOwn(const DevLevel& L, int mode, const T* x, T* y, const T* r, const T* zp, double c1, double c2, const int* stop,
                                        hipStream_t s);
-// the same for levels without own matrices: z-marching LDS planes, stencil + packed per-voxel
-// sums (fine.hip)
-template <class T> void launchFine(const DevLevel& L, int mode, const T* x, T* y, const T* r, T* d, double c1, double c2, const int* stop,
+// the same for levels whose elements are s * Kb, or own elements in child-scale form: z-marching
+// LDS planes, class-sum stencil in difference form + one lane per differing voxel (fine.hip)
+template <class T> void launchFine(const DevLevel& L, int mode, const T* x, T* y, const T* r, const T* zp, double c1, double c2, const int* stop,
                                    hipStream_t s);
 // the same in BF16 on the matrix cores, FP32 vectors (mfma.hip); Aop from mfmaOperands()
-void launchFineMfma(const DevLevel& L, const void* Aop, int mode, const float* x, float* y, const float* r, float* d, double c1, double c2, const int* stop,
-                    hipStream_t s);
+void launchFineMfma(const DevLevel& L, const void* Aop, int mode, const float* x, float* y, const float* r, const float* zp, double c1, double c2,
+                    const int* stop, hipStream_t s);
 size_t mfmaOperandBytes();
 void mfmaOperands(const double* Kb, uint16_t* out);
-// z = d = c0 D^-1 r (first Chebyshev step from zero)
-template <class T> void launchChebFirst(const DevLevel& L, const T* r, T* z, T* d, double c0, const int* stop, hipStream_t s);
+// z = c0 D^-1 r (first Chebyshev step from zero)
+template <class T> void launchChebFirst(const DevLevel& L, const T* r, T* z, double c0, const int* stop, hipStream_t s);
 // rc = P^T rf on the coarse level's free DOFs
 template <class T> void launchRestrict(const DevLevel& F, const DevLevel& C, const T* rf, T* rc, const int* stop, hipStream_t s);
 // zf += P zc on the fine level's free DOFs
@@ -105,6 +112,9 @@ template <class T> void launchDotsRZ(const T* r, const T* z, const T* q, int64_t
 template <class T> void launchUpdateP(const T* z, T* p, int64_t n, CgState* st, hipStream_t s);
 
 // fold, part 1: u += x * rmax, x = 0 (only when a fold is pending)
+// nodal stresses (stress.hip): S = the 8 corner stress matrices [a][6][24] scaled by E / h
+void launchStresses(const DevLevel& L, const double* S, const double* u, int principals, float* vm, float* p1, float* p3, hipStream_t s);
+void launchStressTensors(const DevLevel& L, const double* S, const double* u, float* out, hipStream_t s);
 // mixed: x += alpha rmax p, r -= alpha rmax q (x, r FP64; p, q FP32 scaled), r32 = r / rmax
 void launchUpdateXRMixed(double* x, const float* p, double* r, float* r32, const float* q, int64_t n, CgState* st, Reduce red, hipStream_t s);
 // fold, part 2 (after Kx = K u in FP64): r64 = f - Kx, r32 = r64 / rmax, true residual, stop test

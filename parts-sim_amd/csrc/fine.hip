@@ -94,13 +94,19 @@ template <class T, bool OWNC> struct Ring {
 // OWNC: the level has own elements in child-scale form (sum_c s_c M_c, crow = the corner-0 rows
 // of the 8 M_c, [c][row d][24]); an own voxel is always a differing voxel of its nodes
 template <class T, int MODE, bool OWNC>
-__global__ __launch_bounds__(NT) void kFine(DevLevel L, Sym<T> C, const T* __restrict__ crow, const T* __restrict__ x, T* __restrict__ y,
-                                            const T* __restrict__ r, T* __restrict__ dv, T c1, T c2, const int* stop, int zChunk) {
+__global__ __launch_bounds__(NT) void kFine(DevLevel L, Sym<T> C, const T* __restrict__ crow, const T* __restrict__ ctab, const T* __restrict__ x, T* __restrict__ y,
+                                            const T* __restrict__ r, const T* __restrict__ zp, T c1, T c2, const int* stop, int zChunk, int gx, int gy,
+                                            int nTiles, int xcdOrder) {
     if (stop && __builtin_nontemporal_load(stop) != 0) return;
+    // XCD-aware tile order: the hardware deals consecutive blocks round-robin to the 8 XCDs, so
+    // block b takes tile (b % 8) * per + b / 8 and each XCD walks one contiguous run of tiles (x
+    // fastest), whose halos then meet in that XCD's L2
+    const int per = gridDim.x / 8, t = xcdOrder ? int(blockIdx.x % 8) * per + int(blockIdx.x / 8) : int(blockIdx.x);
+    if (t >= nTiles) return;
     __shared__ Ring<T, OWNC> sh;
     const int tid = threadIdx.x, tx = tid % TX, ty = tid / TX, lane = tid & 63, wid = tid >> 6;
-    const int i0 = blockIdx.x * TX, j0 = blockIdx.y * TY;
-    const int z0 = blockIdx.z * zChunk, z1 = min(L.NZ, z0 + zChunk);
+    const int i0 = (t % gx) * TX, j0 = ((t / gx) % gy) * TY;
+    const int z0 = (t / (gx * gy)) * zChunk, z1 = min(L.NZ, z0 + zChunk);
     const int i = i0 + tx, j = j0 + ty;
     const bool mine = i < L.NX && j < L.NY;
     const int64_t NX = L.NX, NXY = int64_t(L.NX) * L.NY;
@@ -184,9 +190,9 @@ __global__ __launch_bounds__(NT) void kFine(DevLevel L, Sym<T> C, const T* __res
         const int64_t nn = col + NXY * (in ? z : 0);
 #pragma unroll
         for (int d = 0; d < 3; d++) {
-            if constexpr (NEEDS_INV) o.inv[d] = in ? invD[3 * nn + d] : T(0);
+            if constexpr (NEEDS_INV && OWNC) o.inv[d] = in ? invD[3 * nn + d] : T(0);
             if constexpr (NEEDS_R) o.r[d] = in ? r[3 * nn + d] : T(0);
-            if constexpr (NEEDS_D) o.d[d] = in ? dv[3 * nn + d] : T(0);
+            if constexpr (NEEDS_D) o.d[d] = in && zp ? zp[3 * nn + d] : T(0);
         }
         o.info = in ? uint32_t(L.nodeInfo[nn]) : 0u;
     };
@@ -359,34 +365,9 @@ __global__ __launch_bounds__(NT) void kFine(DevLevel L, Sym<T> C, const T* __res
                             for (int c = 0; c < 3; c++) xr[q][c] -= xr[0][c];
                         const T sg[3] = {di ? T(1) : T(-1), dj ? T(1) : T(-1), dk ? T(1) : T(-1)};
                         const int oi = OWNC ? sh.own[dk ? v0 : vm][oy0 + dj][ox0 + di] : -1;
-                        T res[3];
-                        if (OWNC && oi >= 0) {
-                            // own element sum_c s_c M_c, mirrored: child c' of the mirrored element
-                            // is child c' ^ (7 ^ v) of the original
-                            const float* cs = L.childS + 8 * int64_t(oi);
-                            const int mb = 7 ^ v;
-                            const T sref = T(sh.s[(rv >> 2) ? v0 : vm][oy0 + ((rv >> 1) & 1)][ox0 + (rv & 1)]);
-                            T xs[8][3];
-#pragma unroll
-                            for (int q = 1; q < 8; q++)
-#pragma unroll
-                                for (int c = 0; c < 3; c++) xs[q][c] = sg[c] * xr[q][c];
-                            T o3[3] = {0, 0, 0};
-#pragma unroll
-                            for (int cc = 0; cc < 8; cc++) {
-                                const T wc = T(cs[cc ^ mb]) - sref;
-#pragma unroll
-                                for (int d = 0; d < 3; d++) {
-                                    T t = 0;
-#pragma unroll
-                                    for (int u = 3; u < 24; u++) t += crow[(cc * 3 + d) * 24 + u] * xs[u / 3][u % 3];
-                                    o3[d] += wc * t;
-                                }
-                            }
-#pragma unroll
-                            for (int d = 0; d < 3; d++) res[d] = sg[d] * o3[d];
-                        } else {
-                            const T ds = T(sh.s[dk ? v0 : vm][oy0 + dj][ox0 + di]) - T(sh.s[(rv >> 2) ? v0 : vm][oy0 + ((rv >> 1) & 1)][ox0 + (rv & 1)]);
+                        // rows of the level's uniform element (compile-time class coefficients)
+                        // applied to the mirrored differences, signs restored
+                        auto uniformRows = [&](T (&u)[3]) {
                             T Pm[3][3];
                             staticFor<3>([&](auto Dd) {
                                 staticFor<3>([&](auto Cc) {
@@ -396,7 +377,59 @@ __global__ __launch_bounds__(NT) void kFine(DevLevel L, Sym<T> C, const T* __res
                                 });
                             });
 #pragma unroll
-                            for (int d = 0; d < 3; d++) res[d] = ds * sg[d] * (sg[0] * Pm[d][0] + sg[1] * Pm[d][1] + sg[2] * Pm[d][2]);
+                            for (int d = 0; d < 3; d++) u[d] = sg[d] * (sg[0] * Pm[d][0] + sg[1] * Pm[d][1] + sg[2] * Pm[d][2]);
+                        };
+                        const T sref = T(sh.s[(rv >> 2) ? v0 : vm][oy0 + ((rv >> 1) & 1)][ox0 + (rv & 1)]);
+                        T res[3];
+                        if (OWNC && oi >= 0) {
+                            // own element sum_c s_c M_c, mirrored: child c' of the mirrored element
+                            // is child c' ^ (7 ^ v) of the original
+                            const int mb = 7 ^ v;
+                            const int4 tw = *reinterpret_cast<const int4*>(L.childTwo + 4 * int64_t(oi));
+                            T xs[8][3];
+#pragma unroll
+                            for (int q = 1; q < 8; q++)
+#pragma unroll
+                                for (int c = 0; c < 3; c++) xs[q][c] = sg[c] * xr[q][c];
+                            if (tw.z >= 0) {
+                                // two scales: (a - s_ref) Kb + (b - a) M_B, B the children of scale b
+                                const T sa = T(__int_as_float(tw.x)), sb = T(__int_as_float(tw.y));
+                                uint32_t Bp = 0;
+#pragma unroll
+                                for (int c = 0; c < 8; c++) Bp |= ((uint32_t(tw.z) >> (c ^ mb)) & 1u) << c;
+                                const T* __restrict__ tab = ctab + 64 * Bp;
+                                T u[3];
+                                uniformRows(u);
+#pragma unroll
+                                for (int d = 0; d < 3; d++) {
+                                    T t = 0;
+#pragma unroll
+                                    for (int k2 = 0; k2 < 21; k2++) t += tab[d * 21 + k2] * xs[1 + k2 / 3][k2 % 3];
+                                    res[d] = (sa - sref) * u[d] + (sb - sa) * sg[d] * t;
+                                }
+                            } else {
+                                const float* cs = L.childS + 8 * int64_t(oi);
+                                T o3[3] = {0, 0, 0};
+#pragma unroll
+                                for (int cc = 0; cc < 8; cc++) {
+                                    const T wc = T(cs[cc ^ mb]) - sref;
+#pragma unroll
+                                    for (int d = 0; d < 3; d++) {
+                                        T t = 0;
+#pragma unroll
+                                        for (int k2 = 3; k2 < 24; k2++) t += crow[(cc * 3 + d) * 24 + k2] * xs[k2 / 3][k2 % 3];
+                                        o3[d] += wc * t;
+                                    }
+                                }
+#pragma unroll
+                                for (int d = 0; d < 3; d++) res[d] = sg[d] * o3[d];
+                            }
+                        } else {
+                            const T ds = T(sh.s[dk ? v0 : vm][oy0 + dj][ox0 + di]) - sref;
+                            T u[3];
+                            uniformRows(u);
+#pragma unroll
+                            for (int d = 0; d < 3; d++) res[d] = ds * u[d];
                         }
 #pragma unroll
                         for (int d = 0; d < 3; d++) sh.pres[d][p - pass * NT] = res[d];
@@ -419,6 +452,18 @@ __global__ __launch_bounds__(NT) void kFine(DevLevel L, Sym<T> C, const T* __res
 #pragma unroll
                     for (int d = 0; d < 3; d++) out[d] += T(L.diagAdd[3 * n + d]) * xcPrev[d];
                 }
+                if constexpr (NEEDS_INV && !OWNC) {
+                    // inverse diagonal from the voxel scales: every diagonal entry of Kb is the
+                    // same class value, so diag = Kb_00 sum_v s_v (+ the diagonal shift)
+                    T ssum = 0;
+#pragma unroll
+                    for (int v = 0; v < 8; v++) ssum += T(sh.s[(v >> 2) ? v0 : vm][ty + ((v >> 1) & 1)][tx + (v & 1)]);
+#pragma unroll
+                    for (int d = 0; d < 3; d++) {
+                        const T dg = C.v[0] * ssum + (L.diagAdd ? T(L.diagAdd[3 * n + d]) : T(0));
+                        cur.inv[d] = ((cur.info >> (8 + d)) & 1u) ? T(0) : T(1) / dg;
+                    }
+                }
 #pragma unroll
                 for (int d = 0; d < 3; d++) {
                     const int64_t q = 3 * n + d;
@@ -426,17 +471,8 @@ __global__ __launch_bounds__(NT) void kFine(DevLevel L, Sym<T> C, const T* __res
                     if constexpr (MODE == AM_APPLY) y[q] = held ? T(0) : out[d];
                     else if constexpr (MODE == AM_RAW) y[q] = out[d];
                     else if constexpr (MODE == AM_RESID) y[q] = held ? T(0) : cur.r[d] - out[d];
-                    else if constexpr (MODE == AM_CHEB1) {
-                        const T id = cur.inv[d];
-                        const T dn = c1 * id * (cur.r[d] - out[d]);
-                        dv[q] = dn;
-                        y[q] = xcPrev[d] + dn;
-                    } else {
-                        const T id = cur.inv[d];
-                        const T dn = c1 * cur.d[d] + c2 * id * (cur.r[d] - out[d]);
-                        dv[q] = dn;
-                        y[q] = xcPrev[d] + dn;
-                    }
+                    else if constexpr (MODE == AM_CHEB1) y[q] = xcPrev[d] + c1 * cur.inv[d] * (cur.r[d] - out[d]);
+                    else y[q] = xcPrev[d] + c1 * (xcPrev[d] - cur.d[d]) + c2 * cur.inv[d] * (cur.r[d] - out[d]);
                 }
             }
         }
@@ -456,37 +492,51 @@ __global__ __launch_bounds__(NT) void kFine(DevLevel L, Sym<T> C, const T* __res
 }  // namespace
 
 template <class T, bool OWNC>
-static void launchFineT(const DevLevel& L, int mode, const Sym<T>& C, const T* crow, const T* x, T* y, const T* r, T* d, double c1, double c2,
-                        const int* stop, dim3 g, int zChunk, hipStream_t s) {
-    const dim3 b(NT);
+static void launchFineT(const DevLevel& L, int mode, const Sym<T>& C, const T* crow, const T* ctab, const T* x, T* y, const T* r, const T* d, double c1, double c2,
+                        const int* stop, int zChunk, hipStream_t s) {
+    const int gx = (L.NX + TX - 1) / TX, gy = (L.NY + TY - 1) / TY, gz = (L.NZ + zChunk - 1) / zChunk;
+    const int nTiles = gx * gy * gz;
+    const dim3 g(unsigned(8 * ((nTiles + 7) / 8))), b(NT);  // whole rounds over the 8 XCDs
+    // measured on the 512x128x128 void block: the XCD-contiguous order is 8 % slower than the
+    // launch order for this kernel (its halos are served from the shared last-level cache), so it
+    // is off unless PSX_XCD_ORDER=1
+    static const int xcdOrder = [] {
+        const char* e = std::getenv("PSX_XCD_ORDER");
+        return e && e[0] == '1' ? 1 : 0;
+    }();
     switch (mode) {
-        case AM_APPLY: kFine<T, AM_APPLY, OWNC><<<g, b, 0, s>>>(L, C, crow, x, y, r, d, T(c1), T(c2), stop, zChunk); break;
-        case AM_RAW: kFine<T, AM_RAW, OWNC><<<g, b, 0, s>>>(L, C, crow, x, y, r, d, T(c1), T(c2), stop, zChunk); break;
-        case AM_RESID: kFine<T, AM_RESID, OWNC><<<g, b, 0, s>>>(L, C, crow, x, y, r, d, T(c1), T(c2), stop, zChunk); break;
-        case AM_CHEB1: kFine<T, AM_CHEB1, OWNC><<<g, b, 0, s>>>(L, C, crow, x, y, r, d, T(c1), T(c2), stop, zChunk); break;
-        default: kFine<T, AM_CHEB2, OWNC><<<g, b, 0, s>>>(L, C, crow, x, y, r, d, T(c1), T(c2), stop, zChunk); break;
+        case AM_APPLY: kFine<T, AM_APPLY, OWNC><<<g, b, 0, s>>>(L, C, crow, ctab, x, y, r, d, T(c1), T(c2), stop, zChunk, gx, gy, nTiles, xcdOrder); break;
+        case AM_RAW: kFine<T, AM_RAW, OWNC><<<g, b, 0, s>>>(L, C, crow, ctab, x, y, r, d, T(c1), T(c2), stop, zChunk, gx, gy, nTiles, xcdOrder); break;
+        case AM_RESID: kFine<T, AM_RESID, OWNC><<<g, b, 0, s>>>(L, C, crow, ctab, x, y, r, d, T(c1), T(c2), stop, zChunk, gx, gy, nTiles, xcdOrder); break;
+        case AM_CHEB1: kFine<T, AM_CHEB1, OWNC><<<g, b, 0, s>>>(L, C, crow, ctab, x, y, r, d, T(c1), T(c2), stop, zChunk, gx, gy, nTiles, xcdOrder); break;
+        default: kFine<T, AM_CHEB2, OWNC><<<g, b, 0, s>>>(L, C, crow, ctab, x, y, r, d, T(c1), T(c2), stop, zChunk, gx, gy, nTiles, xcdOrder); break;
     }
 }
 
-template <class T> void launchFine(const DevLevel& L, int mode, const T* x, T* y, const T* r, T* d, double c1, double c2, const int* stop, hipStream_t s) {
+template <class T> void launchFine(const DevLevel& L, int mode, const T* x, T* y, const T* r, const T* d, double c1, double c2, const int* stop, hipStream_t s) {
     Sym<T> C;
     for (int q = 0; q < 22; q++) C.v[q] = sizeof(T) == 8 ? T(L.sym64[q]) : T(L.sym32[q]);
-    // chunks of 16 planes when that still leaves several blocks per CU, else 8
+    // the longest z-chunk (16, 8 or 4 planes) that still gives >= 4096 blocks (16 per CU); shorter
+    // chunks repeat two halo planes per chunk but keep small levels' CUs busy
     const int gx = (L.NX + TX - 1) / TX, gy = (L.NY + TY - 1) / TY;
-    int zChunk = int64_t(gx) * gy * ((L.NZ + 15) / 16) >= 2048 ? 16 : 8;
+    int zChunk = 4;
+    for (int c : {16, 8})
+        if (int64_t(gx) * gy * ((L.NZ + c - 1) / c) >= 4096) {
+            zChunk = c;
+            break;
+        }
     if (const char* e = std::getenv("PSX_ZCHUNK")) zChunk = std::max(2, std::atoi(e));
-    const dim3 g(gx, gy, (L.NZ + zChunk - 1) / zChunk);
     if (L.childForm) {
-        const T* crow = nullptr;
-        if constexpr (sizeof(T) == 8) crow = L.crow64;
-        else crow = L.crow32;
-        launchFineT<T, true>(L, mode, C, crow, x, y, r, d, c1, c2, stop, g, zChunk, s);
+        const T *crow = nullptr, *ctab = nullptr;
+        if constexpr (sizeof(T) == 8) crow = L.crow64, ctab = L.ctab64;
+        else crow = L.crow32, ctab = L.ctab32;
+        launchFineT<T, true>(L, mode, C, crow, ctab, x, y, r, d, c1, c2, stop, zChunk, s);
     } else {
-        launchFineT<T, false>(L, mode, C, nullptr, x, y, r, d, c1, c2, stop, g, zChunk, s);
+        launchFineT<T, false>(L, mode, C, nullptr, nullptr, x, y, r, d, c1, c2, stop, zChunk, s);
     }
 }
 
-template void launchFine<double>(const DevLevel&, int, const double*, double*, const double*, double*, double, double, const int*, hipStream_t);
-template void launchFine<float>(const DevLevel&, int, const float*, float*, const float*, float*, double, double, const int*, hipStream_t);
+template void launchFine<double>(const DevLevel&, int, const double*, double*, const double*, const double*, double, double, const int*, hipStream_t);
+template void launchFine<float>(const DevLevel&, int, const float*, float*, const float*, const float*, double, double, const int*, hipStream_t);
 
 }  // namespace psx

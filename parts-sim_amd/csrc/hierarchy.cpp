@@ -487,6 +487,36 @@ static Level coarsen(const Level& F) {
             for (int q = start[e]; q < start[e + 1]; q++) C.childS[i64(C.own[e]) * 8 + pos[kids[q]]] = float(F.rho[kids[q]]);
         }
         C.childM.assign(M.begin(), M.end());
+        // two-valued children (void parts: solid and void scales, or a scale and absent children):
+        // sum_c s_c M_c = a Kb + (b - a) M_B with M_B the sum over the children of scale b
+        C.childTwo.assign(i64(nOwn) * 4, 0);
+        for (i64 e = 0; e < nE; e++) {
+            if (C.own[e] < 0) continue;
+            const float* cs = C.childS.data() + i64(C.own[e]) * 8;
+            const float a = cs[0];
+            float b = a;
+            int mask = 0;
+            bool two = true;
+            for (int c = 1; c < 8; c++) {
+                if (cs[c] == a) continue;
+                if (mask == 0) b = cs[c];
+                if (cs[c] == b) mask |= 1 << c;
+                else two = false;
+            }
+            int32_t* t = C.childTwo.data() + i64(C.own[e]) * 4;
+            std::memcpy(&t[0], &a, 4);
+            std::memcpy(&t[1], &b, 4);
+            t[2] = two && mask ? mask : -1;
+        }
+        // corner-0 rows of M_B for every subset B, columns of corners 1..7 (the rows sum to zero
+        // over the corners, so corner 0's columns are not needed): [256][3][21], padded to 64
+        C.childTab.assign(256 * 64, 0.0);
+        for (int B = 1; B < 256; B++)
+            for (int c = 0; c < 8; c++) {
+                if (!((B >> c) & 1)) continue;
+                for (int d = 0; d < 3; d++)
+                    for (int u = 0; u < 21; u++) C.childTab[B * 64 + d * 21 + u] += M[size_t(c) * 576 + size_t(d) * 24 + 3 + u];
+            }
     }
     // a coarse DOF is held when it interpolates onto a held fine DOF
     C.bc.assign(C.nDof, 0);
@@ -694,6 +724,52 @@ double tridiagonalMax(const std::vector<double>& a, const std::vector<double>& b
         else hi = mid;
     }
     return hi;
+}
+
+// voxels not face-connected to a held node, removed (density 0): the input rule applied before
+// every break-step re-solve (voxel_fea.cpp:1086-1119). Seeds are the solid voxels with a held corner;
+// the fill walks the six face neighbours. Returns the number of voxels removed.
+int pruneFloating(int nx, int ny, int nz, const float* density, const uint8_t* heldNode, float* out) {
+    const i64 NX = nx + 1, NY = ny + 1, nE = i64(nx) * ny * nz;
+    std::vector<uint8_t> reached(nE, 0);
+    std::vector<i64> frontier;
+    auto solid = [&](i64 e) { return density[e] > 0; };
+    for (int k = 0; k < nz; k++)
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++) {
+                const i64 e = i + i64(nx) * (j + i64(ny) * k);
+                if (!solid(e)) continue;
+                bool held = false;
+                for (int c = 0; c < 8 && !held; c++) held = heldNode[(i + CX[c]) + NX * ((j + CY[c]) + NY * (k + CZ[c]))] != 0;
+                if (held) {
+                    reached[e] = 1;
+                    frontier.push_back(e);
+                }
+            }
+    while (!frontier.empty()) {
+        const i64 e = frontier.back();
+        frontier.pop_back();
+        const int i = int(e % nx), j = int((e / nx) % ny), k = int(e / (i64(nx) * ny));
+        const i64 step[3] = {1, nx, i64(nx) * ny};
+        const int at[3] = {i, j, k}, len[3] = {nx, ny, nz};
+        for (int ax = 0; ax < 3; ax++)
+            for (int sgn = -1; sgn <= 1; sgn += 2) {
+                const int c = at[ax] + sgn;
+                if (c < 0 || c >= len[ax]) continue;
+                const i64 m = e + sgn * step[ax];
+                if (!reached[m] && solid(m)) {
+                    reached[m] = 1;
+                    frontier.push_back(m);
+                }
+            }
+    }
+    int removed = 0;
+    for (i64 e = 0; e < nE; e++) {
+        const bool keep = solid(e) && reached[e];
+        removed += solid(e) && !reached[e];
+        out[e] = keep ? density[e] : 0.f;
+    }
+    return removed;
 }
 
 }  // namespace psx

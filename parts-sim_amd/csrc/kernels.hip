@@ -180,7 +180,7 @@ __device__ __forceinline__ void applyNode(const DevLevel& L, const T* __restrict
 }
 
 template <class T, int MODE>
-__global__ __launch_bounds__(BLOCK) void kApply(DevLevel L, const T* __restrict__ x, T* __restrict__ y, const T* __restrict__ r, T* __restrict__ dv,
+__global__ __launch_bounds__(BLOCK) void kApply(DevLevel L, const T* __restrict__ x, T* __restrict__ y, const T* __restrict__ r, const T* __restrict__ zp,
                                                  T c1, T c2, const int* stop) {
     if (stopped(stop)) return;
     const int64_t n = int64_t(blockIdx.x) * BLOCK + threadIdx.x;
@@ -197,15 +197,8 @@ __global__ __launch_bounds__(BLOCK) void kApply(DevLevel L, const T* __restrict_
         if constexpr (MODE == AM_APPLY) y[q] = id != T(0) ? acc[d] : T(0);
         else if constexpr (MODE == AM_RAW) y[q] = acc[d];
         else if constexpr (MODE == AM_RESID) y[q] = id != T(0) ? r[q] - acc[d] : T(0);
-        else if constexpr (MODE == AM_CHEB1) {
-            const T dn = c1 * id * (r[q] - acc[d]);
-            dv[q] = dn;
-            y[q] = x[q] + dn;
-        } else {
-            const T dn = c1 * dv[q] + c2 * id * (r[q] - acc[d]);
-            dv[q] = dn;
-            y[q] = x[q] + dn;
-        }
+        else if constexpr (MODE == AM_CHEB1) y[q] = x[q] + c1 * id * (r[q] - acc[d]);
+        else y[q] = x[q] + c1 * (zp ? x[q] - zp[q] : x[q]) + c2 * id * (r[q] - acc[d]);
     }
 }
 
@@ -214,7 +207,7 @@ __global__ __launch_bounds__(BLOCK) void kApply(DevLevel L, const T* __restrict_
 // summed with shuffles inside the aligned 8-lane group, lane 0 writes. Coarse levels are small:
 // this spreads one node's dependent loads over 8 lanes instead of one thread walking 8 voxels.
 template <class T, int MODE>
-__global__ __launch_bounds__(BLOCK) void kApplyOwn(DevLevel L, const T* __restrict__ x, T* __restrict__ y, const T* __restrict__ r, T* __restrict__ dv,
+__global__ __launch_bounds__(BLOCK) void kApplyOwn(DevLevel L, const T* __restrict__ x, T* __restrict__ y, const T* __restrict__ r, const T* __restrict__ zp,
                                                    T c1, T c2, const int* stop) {
     if (stopped(stop)) return;
     __shared__ T Kb[576];
@@ -287,26 +280,17 @@ __global__ __launch_bounds__(BLOCK) void kApplyOwn(DevLevel L, const T* __restri
         if constexpr (MODE == AM_APPLY) y[q] = id != T(0) ? acc[d] : T(0);
         else if constexpr (MODE == AM_RAW) y[q] = acc[d];
         else if constexpr (MODE == AM_RESID) y[q] = id != T(0) ? r[q] - acc[d] : T(0);
-        else if constexpr (MODE == AM_CHEB1) {
-            const T dn = c1 * id * (r[q] - acc[d]);
-            dv[q] = dn;
-            y[q] = x[q] + dn;
-        } else {
-            const T dn = c1 * dv[q] + c2 * id * (r[q] - acc[d]);
-            dv[q] = dn;
-            y[q] = x[q] + dn;
-        }
+        else if constexpr (MODE == AM_CHEB1) y[q] = x[q] + c1 * id * (r[q] - acc[d]);
+        else y[q] = x[q] + c1 * (zp ? x[q] - zp[q] : x[q]) + c2 * id * (r[q] - acc[d]);
     }
 }
 
-template <class T> __global__ __launch_bounds__(BLOCK) void kChebFirst(const T* __restrict__ invD, const T* __restrict__ r, T* __restrict__ z, T* __restrict__ d,
-                                                                       T c0, int64_t n, const int* stop) {
+template <class T> __global__ __launch_bounds__(BLOCK) void kChebFirst(const T* __restrict__ invD, const T* __restrict__ r, T* __restrict__ z, T c0, int64_t n,
+                                                                       const int* stop) {
     if (stopped(stop)) return;
     const int64_t q = int64_t(blockIdx.x) * BLOCK + threadIdx.x;
     if (q >= n) return;
-    const T v = c0 * invD[q] * r[q];
-    d[q] = v;
-    z[q] = v;
+    z[q] = c0 * invD[q] * r[q];
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -665,7 +649,7 @@ struct FinLzB {
 // ------------------------------------------------------------------------------------------------
 // launchers
 
-template <class T> void launchApply(const DevLevel& L, int mode, const T* x, T* y, const T* r, T* d, double c1, double c2, const int* stop, hipStream_t s) {
+template <class T> void launchApply(const DevLevel& L, int mode, const T* x, T* y, const T* r, const T* d, double c1, double c2, const int* stop, hipStream_t s) {
     const dim3 g(flatGrid(L.nNodes)), b(BLOCK);
     switch (mode) {
         case AM_APPLY: kApply<T, AM_APPLY><<<g, b, 0, s>>>(L, x, y, r, d, T(c1), T(c2), stop); break;
@@ -676,7 +660,7 @@ template <class T> void launchApply(const DevLevel& L, int mode, const T* x, T* 
     }
 }
 
-template <class T> void launchApplyOwn(const DevLevel& L, int mode, const T* x, T* y, const T* r, T* d, double c1, double c2, const int* stop, hipStream_t s) {
+template <class T> void launchApplyOwn(const DevLevel& L, int mode, const T* x, T* y, const T* r, const T* d, double c1, double c2, const int* stop, hipStream_t s) {
     const dim3 g(unsigned((L.nNodes + BLOCK / 8 - 1) / (BLOCK / 8))), b(BLOCK);
     switch (mode) {
         case AM_APPLY: kApplyOwn<T, AM_APPLY><<<g, b, 0, s>>>(L, x, y, r, d, T(c1), T(c2), stop); break;
@@ -687,9 +671,9 @@ template <class T> void launchApplyOwn(const DevLevel& L, int mode, const T* x, 
     }
 }
 
-template <class T> void launchChebFirst(const DevLevel& L, const T* r, T* z, T* d, double c0, const int* stop, hipStream_t s) {
+template <class T> void launchChebFirst(const DevLevel& L, const T* r, T* z, double c0, const int* stop, hipStream_t s) {
     const T* invD = sizeof(T) == 8 ? reinterpret_cast<const T*>(L.invD64) : reinterpret_cast<const T*>(L.invD32);
-    kChebFirst<T><<<flatGrid(L.nDof), BLOCK, 0, s>>>(invD, r, z, d, T(c0), L.nDof, stop);
+    kChebFirst<T><<<flatGrid(L.nDof), BLOCK, 0, s>>>(invD, r, z, T(c0), L.nDof, stop);
 }
 
 template <class T> void launchRestrict(const DevLevel& F, const DevLevel& C, const T* rf, T* rc, const int* stop, hipStream_t s) {
@@ -767,9 +751,9 @@ void launchLzNext(const double* w, double* v, double* vPrev, int64_t n, CgState*
 }
 
 #define PSX_INSTANTIATE(T)                                                                                                              \
-    template void launchApply<T>(const DevLevel&, int, const T*, T*, const T*, T*, double, double, const int*, hipStream_t);            \
-    template void launchChebFirst<T>(const DevLevel&, const T*, T*, T*, double, const int*, hipStream_t);                               \
-    template void launchApplyOwn<T>(const DevLevel&, int, const T*, T*, const T*, T*, double, double, const int*, hipStream_t);         \
+    template void launchApply<T>(const DevLevel&, int, const T*, T*, const T*, const T*, double, double, const int*, hipStream_t);      \
+    template void launchChebFirst<T>(const DevLevel&, const T*, T*, double, const int*, hipStream_t);                                   \
+    template void launchApplyOwn<T>(const DevLevel&, int, const T*, T*, const T*, const T*, double, double, const int*, hipStream_t);   \
     template void launchRestrict<T>(const DevLevel&, const DevLevel&, const T*, T*, const int*, hipStream_t);                          \
     template void launchProlong<T>(const DevLevel&, const DevLevel&, const T*, T*, const int*, hipStream_t);                           \
     template void launchCoarse<T>(const int32_t*, const int32_t*, int64_t, int64_t, const double*, const T*, T*, const int*, hipStream_t); \

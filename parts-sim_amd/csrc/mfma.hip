@@ -42,7 +42,7 @@ __device__ constexpr int corner(int x, int y, int z) { return (y ? (x ? 2 : 3) :
 
 template <int MODE>
 __global__ __launch_bounds__(NT) void kFineMfma(DevLevel L, const bf16x8* __restrict__ Aop, const float* __restrict__ x, float* __restrict__ y,
-                                                const float* __restrict__ r, float* __restrict__ dv, float c1, float c2, const int* stop, int zChunk) {
+                                                const float* __restrict__ r, const float* __restrict__ zp, float c1, float c2, const int* stop, int zChunk) {
     if (stop && __builtin_nontemporal_load(stop) != 0) return;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     Smem& sh = *reinterpret_cast<Smem*>(smem_raw);
@@ -70,7 +70,7 @@ __global__ __launch_bounds__(NT) void kFineMfma(DevLevel L, const bf16x8* __rest
         for (int d = 0; d < 3; d++) {
             o.inv[d] = in ? invD[q0 + d] : 0.f;
             if constexpr (NEEDS_R) o.r[d] = in ? r[q0 + d] : 0.f;
-            if constexpr (NEEDS_D) o.d[d] = in ? dv[q0 + d] : 0.f;
+            if constexpr (NEEDS_D) o.d[d] = in && zp ? zp[q0 + d] : 0.f;
         }
     };
     float px[NPL][3], pv[NVL];
@@ -217,14 +217,10 @@ __global__ __launch_bounds__(NT) void kFineMfma(DevLevel L, const bf16x8* __rest
                 if constexpr (MODE == AM_APPLY) y[q] = id != 0.f ? acc[d] : 0.f;
                 else if constexpr (MODE == AM_RAW) y[q] = acc[d];
                 else if constexpr (MODE == AM_RESID) y[q] = id != 0.f ? cur.r[d] - acc[d] : 0.f;
-                else if constexpr (MODE == AM_CHEB1) {
-                    const float dn = c1 * id * (cur.r[d] - acc[d]);
-                    dv[q] = dn;
-                    y[q] = sh.x[s0][d][ty + 1][tx + 1] + dn;
-                } else {
-                    const float dn = c1 * cur.d[d] + c2 * id * (cur.r[d] - acc[d]);
-                    dv[q] = dn;
-                    y[q] = sh.x[s0][d][ty + 1][tx + 1] + dn;
+                else if constexpr (MODE == AM_CHEB1) y[q] = sh.x[s0][d][ty + 1][tx + 1] + c1 * id * (cur.r[d] - acc[d]);
+                else {
+                    const float xc = sh.x[s0][d][ty + 1][tx + 1];
+                    y[q] = xc + c1 * (xc - cur.d[d]) + c2 * id * (cur.r[d] - acc[d]);
                 }
             }
         }
@@ -235,7 +231,7 @@ __global__ __launch_bounds__(NT) void kFineMfma(DevLevel L, const bf16x8* __rest
     }
 }
 
-template <int MODE> void launchMode(const DevLevel& L, const void* Aop, const float* x, float* y, const float* r, float* d, float c1, float c2, const int* stop,
+template <int MODE> void launchMode(const DevLevel& L, const void* Aop, const float* x, float* y, const float* r, const float* d, float c1, float c2, const int* stop,
                                     hipStream_t s) {
     static bool attr = [] {
         return hipFuncSetAttribute(reinterpret_cast<const void*>(&kFineMfma<MODE>), hipFuncAttributeMaxDynamicSharedMemorySize, int(sizeof(Smem))) ==
@@ -267,7 +263,7 @@ void mfmaOperands(const double* Kb, uint16_t* out) {
             }
 }
 
-void launchFineMfma(const DevLevel& L, const void* Aop, int mode, const float* x, float* y, const float* r, float* d, double c1, double c2, const int* stop,
+void launchFineMfma(const DevLevel& L, const void* Aop, int mode, const float* x, float* y, const float* r, const float* d, double c1, double c2, const int* stop,
                     hipStream_t s) {
     switch (mode) {
         case AM_APPLY: launchMode<AM_APPLY>(L, Aop, x, y, r, d, float(c1), float(c2), stop, s); break;

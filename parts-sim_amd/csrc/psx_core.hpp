@@ -49,6 +49,11 @@ struct Level {
     bool childForm = false;
     std::vector<float> childS;
     std::vector<double> childM;
+    // own elements whose children carry at most two scales a (child 0's) and b: [a bits, b bits,
+    // mask of the b children or -1 when more than two scales, 0]; childTab = corner-0 rows of
+    // sum_{c in B} M_c for every subset B (columns of corners 1..7), [256][64]
+    std::vector<int32_t> childTwo;
+    std::vector<double> childTab;
     std::vector<int32_t> words;     // device voxel words
     double lmax = 0;
     i64 node(int i, int j, int k) const { return i + i64(NX) * (j + i64(NY) * k); }
@@ -88,6 +93,10 @@ int stencilClass(int m, int d, int c);
 std::vector<double> lanczosStart(const Level& L);
 // largest eigenvalue of a symmetric tridiagonal (Sturm-count bisection; voxel_fea.cpp:506-530)
 double tridiagonalMax(const std::vector<double>& a, const std::vector<double>& b);
+
+// solid voxels not face-connected to a held node set to 0 (voxel_fea.cpp:1086-1119); heldNode per
+// node; returns the count removed
+int pruneFloating(int nx, int ny, int nz, const float* density, const uint8_t* heldNode, float* out);
 
 // number of worker threads for host-side builds
 int hostThreads();

@@ -1,0 +1,328 @@
+// The serving loop of include/psx.h: continuous batching of independent solve requests on one GPU.
+//
+// Requests queue in arrival order. A pool of max_inflight workers admits the queue head as soon as a
+// worker is free and the head's estimated device footprint fits the budget beside what running
+// requests (and resident parts) hold; resident parts are evicted, least recently used first, to make
+// room. An admitted request builds its hierarchy on the host (or takes its resident part), uploads
+// it and solves its load cases on the model's own HIP stream, so the kernels of concurrent requests
+// overlap on the device while each request's arithmetic stays exactly that of a lone solve.
+#include <hip/hip_runtime_api.h>
+
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <list>
+#include <map>
+#include <mutex>
+#include <thread>
+
+#include "abi.hpp"
+#include "model.hpp"
+
+using namespace psx::abi;
+using Clock = std::chrono::steady_clock;
+
+namespace {
+
+double msBetween(Clock::time_point a, Clock::time_point b) { return std::chrono::duration<double, std::milli>(b - a).count(); }
+
+struct Request {
+    int64_t ticket = 0;
+    // the part, copied
+    int nx = 0, ny = 0, nz = 0, maxLevels = 0;
+    double nu = 0.3;
+    int64_t coarsest = 0;
+    std::vector<float> density;
+    std::vector<uint8_t> bc;
+    std::vector<double> diagAdd;
+    // the load cases
+    int nrhs = 1;
+    int64_t nDof = 0;
+    std::vector<double> f, x0, u;
+    bool x0PerRhs = false;
+    int64_t partId = -1;
+    psx_solve_opts opts{};
+    int64_t estimate = 0;
+    // outcome
+    bool done = false;
+    psx_result res{};
+    std::string error;
+    Clock::time_point submitted, admitted, built, finished;
+};
+
+// device bytes of a part before it is built: level-0 vectors and masks, the coarser levels' share,
+// the request's own f/u/x0 buffers, and room for own element matrices
+int64_t footprint(const Request& r) {
+    const int64_t n = r.nDof;
+    return n * (160 + 24) + 64 * (int64_t(1) << 20);
+}
+
+struct Resident {
+    std::unique_ptr<psx::Model> model;
+    bool busy = false;
+    Clock::time_point used;
+};
+
+}  // namespace
+
+struct psx_server_s {
+    std::mutex mu;
+    std::condition_variable wake, finished;
+    std::deque<Request*> queue;
+    std::map<int64_t, std::unique_ptr<Request>> requests;
+    std::map<int64_t, Resident> parts;
+    std::vector<std::thread> workers;
+    int64_t budget = 0, held = 0, nextTicket = 1;
+    int running = 0, device = 0;
+    bool stopping = false;
+
+    // bytes held by resident parts that are not in use (evictable)
+    int64_t evictable() const {
+        int64_t b = 0;
+        for (const auto& [id, p] : parts)
+            if (!p.busy) b += p.model->bytes;
+        return b;
+    }
+    // evict idle resident parts, least recently used first, until need fits (lock held)
+    void makeRoom(int64_t need) {
+        while (held + need > budget) {
+            auto victim = parts.end();
+            for (auto it = parts.begin(); it != parts.end(); ++it)
+                if (!it->second.busy && (victim == parts.end() || it->second.used < victim->second.used)) victim = it;
+            if (victim == parts.end()) return;
+            held -= victim->second.model->bytes;
+            parts.erase(victim);
+        }
+    }
+
+    void work() {
+        if (hipSetDevice(device) != hipSuccess) return;
+        std::unique_lock<std::mutex> lk(mu);
+        for (;;) {
+            // admission: the queue head, once its footprint fits beside what is held (a request
+            // that could never fit runs alone)
+            wake.wait(lk, [&] {
+                if (stopping) return true;
+                if (queue.empty()) return false;
+                const Request* r = queue.front();
+                const int64_t need = r->estimate;
+                return held + need <= budget || held + need - evictable() <= budget || running == 0;
+            });
+            if (stopping) return;
+            Request* r = queue.front();
+            queue.pop_front();
+            // a resident part that no other request is using: taken before any eviction, and its
+            // model's bytes are already held
+            bool reused = false;
+            if (r->partId >= 0) {
+                auto it = parts.find(r->partId);
+                if (it != parts.end() && !it->second.busy) {
+                    it->second.busy = true;
+                    reused = true;
+                    r->estimate = std::max<int64_t>(0, r->estimate - it->second.model->bytes);
+                }
+            }
+            makeRoom(r->estimate);
+            held += r->estimate;
+            running++;
+            r->admitted = Clock::now();
+            lk.unlock();
+            run(*r, reused);
+            lk.lock();
+            held -= r->estimate;
+            running--;
+            r->finished = Clock::now();
+            r->res.queue_ms = msBetween(r->submitted, r->admitted);
+            r->res.total_ms = msBetween(r->submitted, r->finished);
+            r->done = true;
+            finished.notify_all();
+            wake.notify_all();
+        }
+    }
+
+    // build (or take) the model and solve every load case; the lock is not held
+    void run(Request& r, bool reused) {
+        psx::Model* M = nullptr;
+        std::unique_ptr<psx::Model> own;
+        r.res.status = guarded([&] {
+            if (reused) {
+                std::lock_guard<std::mutex> g(mu);
+                M = parts.at(r.partId).model.get();
+            } else {
+                psx_model_desc d{};
+                d.nx = r.nx;
+                d.ny = r.ny;
+                d.nz = r.nz;
+                d.density = r.density.data();
+                d.nu = r.nu;
+                d.bc = r.bc.data();
+                d.diag_add = r.diagAdd.empty() ? nullptr : r.diagAdd.data();
+                d.coarsest_max_dof = r.coarsest;
+                d.max_levels = r.maxLevels;
+                own = std::make_unique<psx::Model>(d);
+                own->estimateSmoothers();
+                own->sync();
+                M = own.get();
+            }
+            r.built = Clock::now();
+            r.res.build_ms = reused ? 0 : msBetween(r.admitted, r.built);
+            r.res.reused_part = reused ? 1 : 0;
+            r.res.device_bytes = M->bytes;
+            const int64_t n = M->nDof();
+            if (n != r.nDof) throw std::invalid_argument("part does not match its load vectors");
+            // the request's vectors: f, u and x0 in HBM for the whole request
+            struct Buffers {
+                hipStream_t s;
+                double* p[3] = {nullptr, nullptr, nullptr};
+                ~Buffers() {
+                    for (double* q : p)
+                        if (q) (void)hipFreeAsync(q, s);
+                    (void)hipStreamSynchronize(s);
+                }
+            } buf{M->stream};
+            HIPCHK(hipMallocAsync(reinterpret_cast<void**>(&buf.p[0]), size_t(n) * 8, M->stream));
+            HIPCHK(hipMallocAsync(reinterpret_cast<void**>(&buf.p[1]), size_t(n) * 8, M->stream));
+            if (!r.x0.empty()) HIPCHK(hipMallocAsync(reinterpret_cast<void**>(&buf.p[2]), size_t(n) * 8, M->stream));
+            double *df = buf.p[0], *du = buf.p[1], *dx = buf.p[2];
+            r.u.assign(size_t(r.nrhs) * n, 0.0);
+            r.res.converged = 1;
+            for (int k = 0; k < r.nrhs; k++) {
+                HIPCHK(hipMemcpyAsync(df, r.f.data() + size_t(k) * n, size_t(n) * 8, hipMemcpyHostToDevice, M->stream));
+                if (dx && (k == 0 || r.x0PerRhs))
+                    HIPCHK(hipMemcpyAsync(dx, r.x0.data() + (r.x0PerRhs ? size_t(k) * n : 0), size_t(n) * 8, hipMemcpyHostToDevice, M->stream));
+                psx_solve_info info{};
+                M->solveDevice(df, dx, r.opts, du, info);
+                HIPCHK(hipMemcpyAsync(r.u.data() + size_t(k) * n, du, size_t(n) * 8, hipMemcpyDeviceToHost, M->stream));
+                M->sync();
+                r.res.iterations += info.iterations;
+                r.res.max_iterations = std::max(r.res.max_iterations, info.iterations);
+                r.res.residual = std::max(r.res.residual, info.residual);
+                r.res.converged = r.res.converged && info.converged;
+            }
+            r.res.solve_ms = msBetween(r.built, Clock::now());
+            return PSX_OK;
+        });
+        if (r.res.status != PSX_OK) r.error = g_error;
+        std::lock_guard<std::mutex> g(mu);
+        if (reused) {
+            auto& p = parts.at(r.partId);
+            p.busy = false;
+            p.used = Clock::now();
+        } else if (own && r.partId >= 0 && r.res.status == PSX_OK && parts.find(r.partId) == parts.end()) {
+            // keep the part resident (its bytes count against the budget until evicted)
+            held += own->bytes;
+            parts[r.partId] = Resident{std::move(own), false, Clock::now()};
+        }
+    }
+};
+
+extern "C" {
+
+int psx_server_create(const psx_server_opts* opts, psx_server* out) {
+    return guarded([&] {
+        if (!out) throw std::invalid_argument("null output");
+        *out = nullptr;
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess || n == 0) throw psx::NoDevice("no usable GPU was found");
+        auto s = std::make_unique<psx_server_s>();
+        HIPCHK(hipGetDevice(&s->device));
+        const int slots = opts && opts->max_inflight > 0 ? opts->max_inflight : 4;
+        if (opts && opts->hbm_budget > 0) s->budget = opts->hbm_budget;
+        else {
+            size_t freeB = 0, total = 0;
+            HIPCHK(hipMemGetInfo(&freeB, &total));
+            s->budget = int64_t(0.8 * double(freeB));
+        }
+        for (int k = 0; k < slots; k++) s->workers.emplace_back([p = s.get()] { p->work(); });
+        *out = s.release();
+        return PSX_OK;
+    });
+}
+
+int psx_server_submit(psx_server s, const psx_request* req, int64_t* ticket) {
+    return guarded([&] {
+        if (!s || !req || !ticket || !req->f || req->nrhs < 1) throw std::invalid_argument("bad arguments");
+        const psx_model_desc& d = req->part;
+        if (d.nx <= 0 || d.ny <= 0 || d.nz <= 0 || !d.density || !d.bc) throw std::invalid_argument("Structural grid, density and fixture sizes do not match.");
+        auto r = std::make_unique<Request>();
+        r->nx = d.nx;
+        r->ny = d.ny;
+        r->nz = d.nz;
+        r->nu = d.nu;
+        r->coarsest = d.coarsest_max_dof > 0 ? d.coarsest_max_dof : 1100;
+        r->maxLevels = d.max_levels;
+        const int64_t nVox = int64_t(d.nx) * d.ny * d.nz, nNodes = int64_t(d.nx + 1) * (d.ny + 1) * (d.nz + 1);
+        r->nDof = 3 * nNodes;
+        r->density.assign(d.density, d.density + nVox);
+        r->bc.assign(d.bc, d.bc + r->nDof);
+        if (d.diag_add) r->diagAdd.assign(d.diag_add, d.diag_add + r->nDof);
+        r->nrhs = req->nrhs;
+        r->f.assign(req->f, req->f + size_t(req->nrhs) * r->nDof);
+        checkFinite(r->f.data(), int64_t(r->f.size()), "Forces");
+        if (req->x0) {
+            r->x0PerRhs = req->x0_per_rhs != 0;
+            r->x0.assign(req->x0, req->x0 + (r->x0PerRhs ? size_t(req->nrhs) : 1) * r->nDof);
+            checkFinite(r->x0.data(), int64_t(r->x0.size()), "Displacements");
+        }
+        r->partId = req->part_id;
+        r->opts = defaults(&req->opts);
+        r->opts.on_progress = nullptr;
+        r->opts.user = nullptr;
+        r->estimate = footprint(*r);
+        r->submitted = Clock::now();
+        std::lock_guard<std::mutex> g(s->mu);
+        if (s->stopping) throw std::runtime_error("server is shutting down");
+        r->ticket = s->nextTicket++;
+        *ticket = r->ticket;
+        s->queue.push_back(r.get());
+        s->requests[r->ticket] = std::move(r);
+        s->wake.notify_one();
+        return PSX_OK;
+    });
+}
+
+int psx_server_wait(psx_server s, int64_t ticket, double* u, psx_result* out) {
+    std::unique_ptr<Request> r;
+    const int code = guarded([&] {
+        if (!s) throw std::invalid_argument("null server");
+        std::unique_lock<std::mutex> lk(s->mu);
+        auto it = s->requests.find(ticket);
+        if (it == s->requests.end()) throw std::invalid_argument("unknown ticket");
+        Request* p = it->second.get();
+        s->finished.wait(lk, [&] { return p->done; });
+        r = std::move(it->second);
+        s->requests.erase(it);
+        return PSX_OK;
+    });
+    if (code != PSX_OK) return code;
+    if (out) *out = r->res;
+    if (r->res.status != PSX_OK) return fail(r->res.status, r->error);
+    if (u) std::copy(r->u.begin(), r->u.end(), u);
+    return PSX_OK;
+}
+
+int psx_server_pending(psx_server s) {
+    if (!s) return 0;
+    std::lock_guard<std::mutex> g(s->mu);
+    return int(s->queue.size()) + s->running;
+}
+
+void psx_server_destroy(psx_server s) {
+    if (!s) return;
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        s->stopping = true;
+        for (Request* r : s->queue) {
+            r->res.status = PSX_ERR_ARGUMENT;
+            r->error = "server destroyed before the request ran";
+            r->done = true;
+        }
+        s->queue.clear();
+        s->wake.notify_all();
+        s->finished.notify_all();
+    }
+    for (auto& t : s->workers) t.join();
+    delete s;
+}
+
+}  // extern "C"

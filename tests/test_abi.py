@@ -37,3 +37,11 @@ def test_missing_library_raises(P, monkeypatch):
     monkeypatch.setattr(P, "LIB_PATH", "/nonexistent/libpsx.so")
     with pytest.raises(RuntimeError):
         P.lib()
+
+
+def test_server_without_device_raises(P):
+    if P.available():
+        pytest.skip("a GPU is present")
+    with pytest.raises(P.PsxError) as e:
+        P.Server(max_inflight=2)
+    assert e.value.code == P.ERR_NO_DEVICE
