@@ -101,12 +101,9 @@ def serve(args, P, W):
             results[r["id"]] = (r, res, t, u)
         done_evt[r["id"]].set()
 
-    threads = []
-    t0 = time.perf_counter()
-    for r in reqs:
-        delay = t0 + r["t"] - time.perf_counter()
-        if delay > 0:
-            time.sleep(delay)
+    def client(r):
+        # one thread per request: a warm re-solve waits for its parent's result without holding up
+        # later arrivals
         if r["warm"]:
             done_evt[r["parent"]].wait()
             base = parts[r["parent"]]
@@ -116,11 +113,26 @@ def serve(args, P, W):
             F = loads(base, r["nrhs"], r["remove_seed"])
             tk = srv.submit(base["dims"], dens, base["nu"], base["bc"], F, x0=x0, coarsest_max_dof=1200, tol=args.tol, precision="mixed")
         else:
-            w = W.void_block(r["dims"], seed=r["part_seed"], nu=r["nu"], E=r["E"])
-            parts[r["id"]] = w
+            w = parts[r["id"]]
             tk = srv.submit(w["dims"], w["density"], w["nu"], w["bc"], loads(w, r["nrhs"], r["part_seed"]), coarsest_max_dof=1200, tol=args.tol,
                             precision="mixed")
-        th = threading.Thread(target=waiter, args=(dict(r, arrival=t0 + r["t"]), tk))
+        waiter(r, tk)
+        with lock:
+            n = len(results)
+        if n % 10 == 0:
+            log("serving: %d of %d requests done" % (n, len(reqs)))
+
+    for r in reqs:
+        if not r["warm"]:
+            parts[r["id"]] = W.void_block(r["dims"], seed=r["part_seed"], nu=r["nu"], E=r["E"])
+    threads = []
+    t0 = time.perf_counter()
+    for r in reqs:
+        delay = t0 + r["t"] - time.perf_counter()
+        if delay > 0:
+            time.sleep(delay)
+        r["arrival"] = t0 + r["t"]
+        th = threading.Thread(target=client, args=(r,))
         th.start()
         threads.append(th)
     for th in threads:
