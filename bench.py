@@ -14,6 +14,8 @@ The CPU baseline times the reference's own multithreaded solver (oracle/_ref, co
 sources) on the host cores over a bounded sample of the same solve.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--dims 512,128,128] [--no-cpu]
+    python bench.py --batched [--nrhs 16]      (16 load cases of the 128x64x64 SIMP block, one solve)
+    python bench.py --serve N [--rate 200]     (the serving stream through the request loop)
 """
 import argparse
 import json
@@ -152,6 +154,77 @@ def serve(args, P, W):
     print(json.dumps(out), flush=True)
 
 
+def batched(args, P, W):
+    """Batched load cases: the 128x64x64 SIMP block (x ~ U[0.001, 1] per voxel, stiffness x^3,
+    E0 = 200 GPa, nu = 0.3, x = 0 clamped) with 16 load cases (1 kN point loads at random nodes of
+    the x = 128 face, directions uniform on the sphere), all 16 in one multi-instance solve (one
+    launch per V-cycle level and CG step), mixed precision, tol 1e-6; against the same 16 solved one
+    after another in mixed precision and in plain FP64. solves/s counts load cases."""
+    dims = tuple(int(v) for v in args.dims.split(",")) if args.dims != "512,128,128" else (128, 64, 64)
+    w = W.simp_block(dims, seed=4, nrhs=args.nrhs)
+    F = w["f"]
+    k, n = F.shape
+    m = P.Model(w["dims"], w["density"], w["nu"], w["bc"], 1200)
+    m.estimate_smoothers()
+    log("workload", w["name"], "DOF per case", n, "levels", m.num_levels)
+    d_f = P.DeviceArray(F.size, F.reshape(-1))
+    d_u = P.DeviceArray(F.size)
+    m.solve_multi_device(k, d_f, d_u, tol=1e-11, max_iter=3000, precision="fp64")
+    U_ref = d_u.download().reshape(k, n)
+    tol = args.tol if args.tol != 1e-8 else 1e-6
+
+    def errs(U):
+        return [float(np.linalg.norm(U[c] - U_ref[c]) / np.linalg.norm(U_ref[c])) for c in range(k)]
+
+    def timed(fn, steps, warmup):
+        lat = []
+        for s in range(warmup + steps):
+            t = time.perf_counter()
+            out = fn()
+            if s >= warmup:
+                lat.append(time.perf_counter() - t)
+        return np.array(lat), out
+
+    def multi(precision):
+        return m.solve_multi_device(k, d_f, d_u, tol=tol, max_iter=3000, precision=precision, check_every=8)
+
+    lat_b, infos = timed(lambda: multi("mixed"), args.steps, args.warmup)
+    e_b = errs(d_u.download().reshape(k, n))
+    if not all(i["converged"] for i in infos) or max(e_b) > 1e-6:
+        raise RuntimeError("batched mixed solve missed the accuracy bar")
+    lat_b64, infos64 = timed(lambda: multi("fp64"), max(2, args.steps // 2), 1)
+    e_b64 = errs(d_u.download().reshape(k, n))
+    # one load case after another (the reference's way), same device vectors
+    d_f1, d_u1 = P.DeviceArray(n), P.DeviceArray(n)
+
+    def sequential(precision):
+        its = []
+        for c in range(k):
+            d_f1.upload(F[c])
+            its.append(m.solve_device(d_f1, d_u1, tol=tol, max_iter=3000, precision=precision, check_every=8)["iterations"])
+        return its
+
+    lat_s, its_s = timed(lambda: sequential("mixed"), max(2, args.steps // 2), 1)
+    lat_s64, its_s64 = timed(lambda: sequential("fp64"), max(2, args.steps // 2), 1)
+    log("batched mixed %.1f ms (%d cases), batched FP64 %.1f ms, sequential mixed %.1f ms, sequential FP64 %.1f ms"
+        % (1e3 * lat_b.mean(), k, 1e3 * lat_b64.mean(), 1e3 * lat_s.mean(), 1e3 * lat_s64.mean()))
+    its = [i["iterations"] for i in infos]
+    out = {"metric": "batched load cases: solves/s at 1e-6 rel. error, one multi-instance launch per MG level", "value": round(k / lat_b.mean(), 3),
+           "unit": "solves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * lat_b.mean(), 3),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+           "config": {"workload": w["name"] + ", SIMP x^3 densities U[0.001,1], E0 200 GPa, nu 0.3, x=0 clamped, 1 kN point loads on x=%d" % dims[0],
+                      "load_cases": k, "dof_per_case": n, "levels": m.num_levels, "tol": tol, "parallelism": "single GPU"},
+           "p50_ms": round(1e3 * float(np.percentile(lat_b, 50)), 3), "p99_ms": round(1e3 * float(np.percentile(lat_b, 99)), 3),
+           "iterations_max": int(max(its)), "iterations_mean": float(np.mean(its)), "rel_error_vs_fp64_max": max(e_b),
+           "batched_fp64": {"solves_per_s": round(k / lat_b64.mean(), 3), "ms": round(1e3 * lat_b64.mean(), 3), "rel_error_max": max(e_b64),
+                            "iterations_max": int(max(i["iterations"] for i in infos64))},
+           "sequential_mixed": {"solves_per_s": round(k / lat_s.mean(), 3), "ms": round(1e3 * lat_s.mean(), 3), "iterations_max": int(max(its_s))},
+           "sequential_fp64": {"solves_per_s": round(k / lat_s64.mean(), 3), "ms": round(1e3 * lat_s64.mean(), 3), "iterations_max": int(max(its_s64))},
+           "speedup_vs_sequential_fp64": round(lat_s64.mean() / lat_b.mean(), 3),
+           "speedup_vs_sequential_mixed": round(lat_s.mean() / lat_b.mean(), 3)}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -165,6 +238,8 @@ def main():
     ap.add_argument("--serve", type=int, default=0, help="serving workload: this many requests of the Poisson stream")
     ap.add_argument("--rate", type=float, default=200.0, help="serving arrival rate (requests/s)")
     ap.add_argument("--inflight", type=int, default=4, help="serving: requests on the device at once")
+    ap.add_argument("--batched", action="store_true", help="batched load cases: the 128x64x64 SIMP block, --nrhs cases in one multi-instance solve")
+    ap.add_argument("--nrhs", type=int, default=16)
     ap.add_argument("--kernels-only", action="store_true",
                     help="counter runs: three 1 GiB device copies (the calibration kernel), then the per-level operator and V-cycle timings")
     args = ap.parse_args()
@@ -175,6 +250,9 @@ def main():
     import parts_sim_amd.workloads as W
     if args.serve:
         serve(args, P, W)
+        return
+    if args.batched:
+        batched(args, P, W)
         return
 
     dims = tuple(int(v) for v in args.dims.split(","))

@@ -134,6 +134,17 @@ int psx_precondition_device(psx_model m, const double* d_r, double* d_z);
 int psx_solve(psx_model m, const double* f, const double* x0, const psx_solve_opts* opts, double* u, psx_solve_info* info);
 int psx_solve_device(psx_model m, const double* d_f, const double* d_x0, const psx_solve_opts* opts, double* d_u, psx_solve_info* info);
 
+/* nrhs independent load cases on one model in one set of launches: every kernel of the V-cycle and
+ * the CG runs all load cases (one launch per level and step), and each case converges and stops on
+ * its own state, so each solution equals the one psx_solve gives for it alone. f, u (and x0 when not
+ * NULL) are nrhs x n_dof, load-case major; info receives nrhs entries. The reference solves its load
+ * cases one after another (GpuFeaSolver::solve per case, gpu_fea.cpp:452-574; the topology and
+ * eigen loops call it per vector, eigen.cpp:508-553). The BF16 matrix-core smoother is for
+ * nrhs = 1 only (opts->mfma_smoother is ignored above). */
+int psx_solve_multi(psx_model m, int nrhs, const double* f, const double* x0, const psx_solve_opts* opts, double* u, psx_solve_info* info);
+int psx_solve_multi_device(psx_model m, int nrhs, const double* d_f, const double* d_x0, const psx_solve_opts* opts, double* d_u,
+                           psx_solve_info* info);
+
 /* Support reactions: sum over held active DOFs of (K u)_raw - f (VoxelFEA::reactions,
  * voxel_fea.cpp:929-940). f may be NULL. */
 int psx_reactions(psx_model m, const double* u, const double* f, double out[3]);

@@ -82,6 +82,8 @@ SIGNATURES = {
     "psx_precondition_device": (_i, [_vp, _vp, _vp]),
     "psx_solve": (_i, [_vp, _vp, _vp, C.POINTER(SolveOpts), _vp, C.POINTER(SolveInfo)]),
     "psx_solve_device": (_i, [_vp, _vp, _vp, C.POINTER(SolveOpts), _vp, C.POINTER(SolveInfo)]),
+    "psx_solve_multi": (_i, [_vp, _i, _vp, _vp, C.POINTER(SolveOpts), _vp, C.POINTER(SolveInfo)]),
+    "psx_solve_multi_device": (_i, [_vp, _i, _vp, _vp, C.POINTER(SolveOpts), _vp, C.POINTER(SolveInfo)]),
     "psx_reactions": (_i, [_vp, _vp, _vp, _vp]),
     "psx_stresses": (_i, [_vp, _vp, _d, _d, _i, _vp, _vp, _vp]),
     "psx_stresses_device": (_i, [_vp, _vp, _d, _d, _i, _vp, _vp, _vp]),
@@ -327,6 +329,30 @@ class Model:
         addr = lambda a: None if a is None else C.c_void_p(int(a))
         _check(lib().psx_solve_device(self.h, addr(d_f), addr(d_x0), C.byref(o), addr(d_u), C.byref(info)))
         return self._info(info)
+
+    def solve_multi(self, F, tol=1e-6, max_iter=500, X0=None, precision="mixed", check_every=0, fold_every=0, on_progress=None, cheb_degree=0):
+        """Solve K U[k] = F[k] for nrhs load cases (F: nrhs x n_dof) in one set of launches per
+        V-cycle level; returns U and one info dict per load case."""
+        F = np.ascontiguousarray(F, np.float64)
+        if F.ndim != 2 or F.shape[1] != self.n_dof or F.shape[0] < 1:
+            raise ValueError("Force matrix must be nrhs x n_dof.")
+        k = F.shape[0]
+        X0a = None if X0 is None else np.ascontiguousarray(X0, np.float64)
+        if X0a is not None and X0a.size != F.size:
+            raise ValueError("Start displacements must be nrhs x n_dof.")
+        U = np.zeros_like(F)
+        o, cb = self._opts(tol, max_iter, precision, check_every, fold_every, on_progress, cheb_degree, False)
+        infos = (SolveInfo * k)()
+        _check(lib().psx_solve_multi(self.h, k, _p(F), _p(X0a), C.byref(o), _p(U), infos))
+        return U, [self._info(infos[i]) for i in range(k)]
+
+    def solve_multi_device(self, nrhs, d_f, d_u, d_x0=None, tol=1e-6, max_iter=500, precision="mixed", check_every=0, fold_every=0, cheb_degree=0):
+        """solve_multi with HBM-resident nrhs x n_dof vectors (DeviceArray or raw device addresses)."""
+        o, cb = self._opts(tol, max_iter, precision, check_every, fold_every, None, cheb_degree, False)
+        infos = (SolveInfo * int(nrhs))()
+        addr = lambda a: None if a is None else C.c_void_p(int(a))
+        _check(lib().psx_solve_multi_device(self.h, int(nrhs), addr(d_f), addr(d_x0), C.byref(o), addr(d_u), infos))
+        return [self._info(infos[i]) for i in range(int(nrhs))]
 
     def reactions(self, u, f=None):
         R = np.zeros(3)

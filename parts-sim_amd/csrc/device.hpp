@@ -62,9 +62,17 @@ struct CgState {
 // values per kernel
 constexpr int RED_TILE = 2048;
 struct Reduce {
-    double* partials;          // 4 * (nDof / RED_TILE + 1)
+    double* partials;          // per instance: 4 * (nDof / RED_TILE + 2) or more
     unsigned* counter;
+    int64_t stride;            // doubles between consecutive instances' partials
 };
+
+// Multi-instance launches (several right-hand sides of one level, one launch): blockIdx.y is the
+// instance; vectors are instance-major with the level's nDof per instance, and the conjugate-
+// gradient state is CgState[instance]. Flags and scalars that point into CgState step by these.
+constexpr int CG_INTS = int(sizeof(CgState) / sizeof(int));
+constexpr int CG_DBLS = int(sizeof(CgState) / sizeof(double));
+static_assert(sizeof(CgState) % sizeof(double) == 0, "CgState must tile in doubles");
 
 enum ApplyMode { AM_APPLY = 0, AM_RAW = 1, AM_RESID = 2, AM_CHEB1 = 3, AM_CHEB2 = 4 };
 // operator forms (bit flags): own elements in child-scale form; BF16 matrix-core operator
@@ -78,59 +86,74 @@ enum OpForm { OF_CHILD = 1, OF_MFMA = 2 };
 //             iterate, nullptr for a zero one: the step d = y - x needs no vector of its own)
 // y must not alias x or zp.
 template <class T> void launchApply(const DevLevel& L, int mode, const T* x, T* y, const T* r, const T* zp, double c1, double c2, const int* stop,
-                                    hipStream_t s);
+                                    hipStream_t s, int ninst = 1);
 // the same with 8 lanes per node (one per voxel): levels with own matrices
 template <class T> void launchApplyOwn(const DevLevel& L, int mode, const T* x, T* y, const T* r, const T* zp, double c1, double c2, const int* stop,
-                                       hipStream_t s);
+                                       hipStream_t s, int ninst = 1);
 // the same for levels whose elements are s * Kb, or own elements in child-scale form: z-marching
 // LDS planes, class-sum stencil in difference form + one lane per differing voxel (fine.hip)
 template <class T> void launchFine(const DevLevel& L, int mode, const T* x, T* y, const T* r, const T* zp, double c1, double c2, const int* stop,
-                                   hipStream_t s);
+                                   hipStream_t s, int ninst = 1);
+// Dot products fused into an operator launch (conjugate gradients), block partials into red:
+//   FD_PQ  (AM_APPLY)  p.q with p = x, q = y                         -> launchFinishPQ
+//   FD_RZ  (AM_CHEB1/2) r.z and z.q with z = y, q = qdot             -> launchFinishRZ
+enum FuseDot { FD_NONE = 0, FD_PQ = 1, FD_RZ = 2 };
+struct DotFusion {
+    int kind = FD_NONE;
+    const void* q = nullptr;  // FD_RZ: the previous search direction's product K p (type T)
+    Reduce red{};
+};
+// launchFine with a fused dot; returns the number of block partials written (0: not fused)
+template <class T> unsigned launchFineDot(const DevLevel& L, int mode, const T* x, T* y, const T* r, const T* zp, double c1, double c2, const int* stop,
+                                          const DotFusion& fd, hipStream_t s, int ninst = 1);
+void launchFinishPQ(Reduce red, unsigned nb, CgState* st, hipStream_t s, int ninst = 1);
+void launchFinishRZ(Reduce red, unsigned nb, CgState* st, hipStream_t s, int ninst = 1);
 // the same in BF16 on the matrix cores, FP32 vectors (mfma.hip); Aop from mfmaOperands()
 void launchFineMfma(const DevLevel& L, const void* Aop, int mode, const float* x, float* y, const float* r, const float* zp, double c1, double c2,
                     const int* stop, hipStream_t s);
 size_t mfmaOperandBytes();
 void mfmaOperands(const double* Kb, uint16_t* out);
 // z = c0 D^-1 r (first Chebyshev step from zero)
-template <class T> void launchChebFirst(const DevLevel& L, const T* r, T* z, double c0, const int* stop, hipStream_t s);
+template <class T> void launchChebFirst(const DevLevel& L, const T* r, T* z, double c0, const int* stop, hipStream_t s, int ninst = 1);
 // rc = P^T rf on the coarse level's free DOFs
-template <class T> void launchRestrict(const DevLevel& F, const DevLevel& C, const T* rf, T* rc, const int* stop, hipStream_t s);
+template <class T> void launchRestrict(const DevLevel& F, const DevLevel& C, const T* rf, T* rc, const int* stop, hipStream_t s, int ninst = 1);
 // zf += P zc on the fine level's free DOFs
-template <class T> void launchProlong(const DevLevel& F, const DevLevel& C, const T* zc, T* zf, const int* stop, hipStream_t s);
+template <class T> void launchProlong(const DevLevel& F, const DevLevel& C, const T* zc, T* zf, const int* stop, hipStream_t s, int ninst = 1);
 // coarsest level: z[freeIdx] = Ainv r[freeIdx], z = 0 on held DOFs (dense[dof] = dense index or -1)
 template <class T>
-void launchCoarse(const int32_t* freeIdx, const int32_t* dense, int64_t m, int64_t nDof, const double* Ainv, const T* r, T* z, const int* stop, hipStream_t s);
+void launchCoarse(const int32_t* freeIdx, const int32_t* dense, int64_t m, int64_t nDof, const double* Ainv, const T* r, T* z, const int* stop, hipStream_t s, int ninst = 1);
 
 // conjugate-gradient steps (all accumulate in FP64; T is the vector type)
 // pq = p.q; alpha = rz / pq (or broken + stop when pq <= 0)
-template <class T> void launchDotPQ(const T* p, const T* q, int64_t n, CgState* st, Reduce red, hipStream_t s);
+template <class T> void launchDotPQ(const T* p, const T* q, int64_t n, CgState* st, Reduce red, hipStream_t s, int ninst = 1);
 // x += alpha p; r -= alpha q; rr = r.r; iteration count, relative residual, stop test
-template <class T> void launchUpdateXR(T* x, const T* p, T* r, const T* q, int64_t n, CgState* st, Reduce red, hipStream_t s);
+template <class T> void launchUpdateXR(T* x, const T* p, T* r, const T* q, int64_t n, CgState* st, Reduce red, hipStream_t s, int ninst = 1);
 // rzNew = r.z, zq = z.q; beta = max(0, -alpha zq / rz) (flexible); rz = rzNew
-template <class T> void launchDotsRZ(const T* r, const T* z, const T* q, int64_t n, CgState* st, Reduce red, hipStream_t s);
+template <class T> void launchDotsRZ(const T* r, const T* z, const T* q, int64_t n, CgState* st, Reduce red, hipStream_t s, int ninst = 1);
 // p = z + beta p
-template <class T> void launchUpdateP(const T* z, T* p, int64_t n, CgState* st, hipStream_t s);
+template <class T> void launchUpdateP(const T* z, T* p, int64_t n, CgState* st, hipStream_t s, int ninst = 1);
 
 // fold, part 1: u += x * rmax, x = 0 (only when a fold is pending)
 // nodal stresses (stress.hip): S = the 8 corner stress matrices [a][6][24] scaled by E / h
 void launchStresses(const DevLevel& L, const double* S, const double* u, int principals, float* vm, float* p1, float* p3, hipStream_t s);
 void launchStressTensors(const DevLevel& L, const double* S, const double* u, float* out, hipStream_t s);
-// mixed: x += alpha rmax p, r -= alpha rmax q (x, r FP64; p, q FP32 scaled), r32 = r / rmax
-void launchUpdateXRMixed(double* x, const float* p, double* r, float* r32, const float* q, int64_t n, CgState* st, Reduce red, hipStream_t s);
+// mixed with FP32 recurrences: x32 += alpha p, r32 -= alpha q (fold decision); at a fold u += rmax x32
+void launchUpdateXR32Fold(float* x, const float* p, float* r, const float* q, int64_t n, CgState* st, Reduce red, hipStream_t s, int ninst = 1);
+void launchFoldX(double* u, float* x, int64_t n, CgState* st, hipStream_t s, int ninst = 1);
 // fold, part 2 (after Kx = K u in FP64): r64 = f - Kx, r32 = r64 / rmax, true residual, stop test
-void launchFoldR(const double* f, const double* Kx, const double* invD, double* r64, float* r32, int64_t n, CgState* st, Reduce red, hipStream_t s);
+void launchFoldR(const double* f, const double* Kx, const double* invD, double* r64, float* r32, int64_t n, CgState* st, Reduce red, hipStream_t s, int ninst = 1);
 
 // r = f - Kx on free DOFs (Kx already masked), out[0] += |r|^2 over the grid (FP64)
-void launchResidual64(const double* f, const double* Kx, const double* invD, double* r, int64_t n, double* out, Reduce red, hipStream_t s);
+void launchResidual64(const double* f, const double* Kx, const double* invD, double* r, int64_t n, double* out, Reduce red, hipStream_t s, int ninst = 1);
 // out = sum over free DOFs of f^2
-void launchNormFree64(const double* f, const double* invD, int64_t n, double* out, Reduce red, hipStream_t s);
+void launchNormFree64(const double* f, const double* invD, int64_t n, double* out, Reduce red, hipStream_t s, int ninst = 1);
 // x[q] = held ? 0 : x0[q] (or 0 when x0 is null)
-void launchMaskCopy64(const double* x0, const double* invD, double* x, int64_t n, hipStream_t s);
+void launchMaskCopy64(const double* x0, const double* invD, double* x, int64_t n, hipStream_t s, int ninst = 1);
 // out = max |v| over DOFs (FP64 vector)
-void launchMaxAbs64(const double* v, int64_t n, double* out, Reduce red, hipStream_t s);
+void launchMaxAbs64(const double* v, int64_t n, double* out, Reduce red, hipStream_t s, int ninst = 1);
 // y32 = float(x64 * (1 / *scaleFrom)) (the V-cycle's order-one input), z64 = double(z32) * *scaleFrom
-void launchToF32(const double* x, float* y, int64_t n, const double* scaleFrom, hipStream_t s);
-void launchFromF32(const float* z, double* y, int64_t n, const double* scaleFrom, hipStream_t s);
+void launchToF32(const double* x, float* y, int64_t n, const double* scaleFrom, hipStream_t s, int ninst = 1);
+void launchFromF32(const float* z, double* y, int64_t n, const double* scaleFrom, hipStream_t s, int ninst = 1);
 // Lanczos steps for lmax(D^-1 K) (voxel_fea.cpp:579-610)
 void launchLzX(const double* v, const double* invD, double* x, int64_t n, hipStream_t s);
 void launchLzW(double* w, const double* invD, const double* vPrev, const double* v, int64_t n, CgState* st, int j, Reduce red, hipStream_t s);

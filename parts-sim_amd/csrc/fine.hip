@@ -96,8 +96,19 @@ template <class T, bool OWNC> struct Ring {
 template <class T, int MODE, bool OWNC>
 __global__ __launch_bounds__(NT) void kFine(DevLevel L, Sym<T> C, const T* __restrict__ crow, const T* __restrict__ ctab, const T* __restrict__ x, T* __restrict__ y,
                                             const T* __restrict__ r, const T* __restrict__ zp, T c1, T c2, const int* stop, int zChunk, int gx, int gy,
-                                            int nTiles, int xcdOrder) {
+                                            int nTiles, int xcdOrder, double* __restrict__ dots, int64_t dotStride, const T* __restrict__ qdot) {
+    // multi-instance launches: blockIdx.y is the right-hand side (its vectors, stop flag, partials)
+    if (stop) stop += int64_t(blockIdx.y) * CG_INTS;
     if (stop && __builtin_nontemporal_load(stop) != 0) return;
+    {
+        const int64_t io = int64_t(blockIdx.y) * L.nDof;
+        x += io;
+        y += io;
+        if (r) r += io;
+        if (zp) zp += io;
+        if (qdot) qdot += io;
+        if (dots) dots += int64_t(blockIdx.y) * dotStride;
+    }
     // XCD-aware tile order: the hardware deals consecutive blocks round-robin to the 8 XCDs, so
     // block b takes tile (b % 8) * per + b / 8 and each XCD walks one contiguous run of tiles (x
     // fastest), whose halos then meet in that XCD's L2
@@ -181,7 +192,7 @@ __global__ __launch_bounds__(NT) void kFine(DevLevel L, Sym<T> C, const T* __res
     // differing-voxel bits (0..7) and held-DOF bits (8..10)
     constexpr bool NEEDS_INV = MODE == AM_CHEB1 || MODE == AM_CHEB2;
     struct Own {
-        T inv[3], r[3], d[3];
+        T inv[3], r[3], d[3], qd[3];
         uint32_t info;
     };
     const int64_t col = mine ? i + NX * j : 0;
@@ -193,6 +204,7 @@ __global__ __launch_bounds__(NT) void kFine(DevLevel L, Sym<T> C, const T* __res
             if constexpr (NEEDS_INV && OWNC) o.inv[d] = in ? invD[3 * nn + d] : T(0);
             if constexpr (NEEDS_R) o.r[d] = in ? r[3 * nn + d] : T(0);
             if constexpr (NEEDS_D) o.d[d] = in && zp ? zp[3 * nn + d] : T(0);
+            if constexpr (NEEDS_INV) o.qd[d] = in && qdot ? qdot[3 * nn + d] : T(0);
         }
         o.info = in ? uint32_t(L.nodeInfo[nn]) : 0u;
     };
@@ -210,6 +222,7 @@ __global__ __launch_bounds__(NT) void kFine(DevLevel L, Sym<T> C, const T* __res
     Own cur;
     cur.info = 0;
     T aN[3] = {0, 0, 0}, aC[3] = {0, 0, 0}, aP[3] = {0, 0, 0}, xcPrev[3] = {0, 0, 0}, xcPrev2[3] = {0, 0, 0};
+    double dot0 = 0, dot1 = 0;  // fused dot products (dots != nullptr)
     __syncthreads();
 
     for (int k = z0 - 1; k <= z1; k++) {
@@ -468,11 +481,20 @@ __global__ __launch_bounds__(NT) void kFine(DevLevel L, Sym<T> C, const T* __res
                 for (int d = 0; d < 3; d++) {
                     const int64_t q = 3 * n + d;
                     const bool held = (cur.info >> (8 + d)) & 1u;
-                    if constexpr (MODE == AM_APPLY) y[q] = held ? T(0) : out[d];
-                    else if constexpr (MODE == AM_RAW) y[q] = out[d];
-                    else if constexpr (MODE == AM_RESID) y[q] = held ? T(0) : cur.r[d] - out[d];
-                    else if constexpr (MODE == AM_CHEB1) y[q] = xcPrev[d] + c1 * cur.inv[d] * (cur.r[d] - out[d]);
-                    else y[q] = xcPrev[d] + c1 * (xcPrev[d] - cur.d[d]) + c2 * cur.inv[d] * (cur.r[d] - out[d]);
+                    T yv;
+                    if constexpr (MODE == AM_APPLY) yv = held ? T(0) : out[d];
+                    else if constexpr (MODE == AM_RAW) yv = out[d];
+                    else if constexpr (MODE == AM_RESID) yv = held ? T(0) : cur.r[d] - out[d];
+                    else if constexpr (MODE == AM_CHEB1) yv = xcPrev[d] + c1 * cur.inv[d] * (cur.r[d] - out[d]);
+                    else yv = xcPrev[d] + c1 * (xcPrev[d] - cur.d[d]) + c2 * cur.inv[d] * (cur.r[d] - out[d]);
+                    y[q] = yv;
+                    if (dots) {
+                        if constexpr (MODE == AM_APPLY) dot0 += double(xcPrev[d]) * double(yv);
+                        if constexpr (NEEDS_INV) {
+                            dot0 += double(cur.r[d]) * double(yv);
+                            dot1 += double(yv) * double(cur.qd[d]);
+                        }
+                    }
                 }
             }
         }
@@ -487,16 +509,40 @@ __global__ __launch_bounds__(NT) void kFine(DevLevel L, Sym<T> C, const T* __res
         }
         cur = nxt;
     }
+    if (dots) {
+        // block partials in a fixed order: wavefront butterflies, then the waves in order
+        __shared__ double wsum[2][NW];
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            dot0 += __shfl_xor(dot0, off, 64);
+            dot1 += __shfl_xor(dot1, off, 64);
+        }
+        if (lane == 0) {
+            wsum[0][wid] = dot0;
+            wsum[1][wid] = dot1;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            double a = 0, b = 0;
+#pragma unroll
+            for (int w = 0; w < NW; w++) {
+                a += wsum[0][w];
+                b += wsum[1][w];
+            }
+            dots[t] = a;
+            if constexpr (NEEDS_INV) dots[nTiles + t] = b;
+        }
+    }
 }
 
 }  // namespace
 
 template <class T, bool OWNC>
 static void launchFineT(const DevLevel& L, int mode, const Sym<T>& C, const T* crow, const T* ctab, const T* x, T* y, const T* r, const T* d, double c1, double c2,
-                        const int* stop, int zChunk, hipStream_t s) {
+                        const int* stop, int zChunk, double* dots, int64_t dotStride, const T* qdot, hipStream_t s, int ninst) {
     const int gx = (L.NX + TX - 1) / TX, gy = (L.NY + TY - 1) / TY, gz = (L.NZ + zChunk - 1) / zChunk;
     const int nTiles = gx * gy * gz;
-    const dim3 g(unsigned(8 * ((nTiles + 7) / 8))), b(NT);  // whole rounds over the 8 XCDs
+    const dim3 g(unsigned(8 * ((nTiles + 7) / 8)), unsigned(ninst)), b(NT);  // whole rounds over the 8 XCDs
     // measured on the 512x128x128 void block: the XCD-contiguous order is 8 % slower than the
     // launch order for this kernel (its halos are served from the shared last-level cache), so it
     // is off unless PSX_XCD_ORDER=1
@@ -504,20 +550,26 @@ static void launchFineT(const DevLevel& L, int mode, const Sym<T>& C, const T* c
         const char* e = std::getenv("PSX_XCD_ORDER");
         return e && e[0] == '1' ? 1 : 0;
     }();
+#define PSX_FINE_ARGS L, C, crow, ctab, x, y, r, d, T(c1), T(c2), stop, zChunk, gx, gy, nTiles, xcdOrder, dots, dotStride, qdot
     switch (mode) {
-        case AM_APPLY: kFine<T, AM_APPLY, OWNC><<<g, b, 0, s>>>(L, C, crow, ctab, x, y, r, d, T(c1), T(c2), stop, zChunk, gx, gy, nTiles, xcdOrder); break;
-        case AM_RAW: kFine<T, AM_RAW, OWNC><<<g, b, 0, s>>>(L, C, crow, ctab, x, y, r, d, T(c1), T(c2), stop, zChunk, gx, gy, nTiles, xcdOrder); break;
-        case AM_RESID: kFine<T, AM_RESID, OWNC><<<g, b, 0, s>>>(L, C, crow, ctab, x, y, r, d, T(c1), T(c2), stop, zChunk, gx, gy, nTiles, xcdOrder); break;
-        case AM_CHEB1: kFine<T, AM_CHEB1, OWNC><<<g, b, 0, s>>>(L, C, crow, ctab, x, y, r, d, T(c1), T(c2), stop, zChunk, gx, gy, nTiles, xcdOrder); break;
-        default: kFine<T, AM_CHEB2, OWNC><<<g, b, 0, s>>>(L, C, crow, ctab, x, y, r, d, T(c1), T(c2), stop, zChunk, gx, gy, nTiles, xcdOrder); break;
+        case AM_APPLY: kFine<T, AM_APPLY, OWNC><<<g, b, 0, s>>>(PSX_FINE_ARGS); break;
+        case AM_RAW: kFine<T, AM_RAW, OWNC><<<g, b, 0, s>>>(PSX_FINE_ARGS); break;
+        case AM_RESID: kFine<T, AM_RESID, OWNC><<<g, b, 0, s>>>(PSX_FINE_ARGS); break;
+        case AM_CHEB1: kFine<T, AM_CHEB1, OWNC><<<g, b, 0, s>>>(PSX_FINE_ARGS); break;
+        default: kFine<T, AM_CHEB2, OWNC><<<g, b, 0, s>>>(PSX_FINE_ARGS); break;
     }
+#undef PSX_FINE_ARGS
 }
 
-template <class T> void launchFine(const DevLevel& L, int mode, const T* x, T* y, const T* r, const T* d, double c1, double c2, const int* stop, hipStream_t s) {
+template <class T>
+unsigned launchFineDot(const DevLevel& L, int mode, const T* x, T* y, const T* r, const T* d, double c1, double c2, const int* stop, const DotFusion& fd,
+                       hipStream_t s, int ninst) {
     Sym<T> C;
     for (int q = 0; q < 22; q++) C.v[q] = sizeof(T) == 8 ? T(L.sym64[q]) : T(L.sym32[q]);
     // the longest z-chunk (16, 8 or 4 planes) that still gives >= 4096 blocks (16 per CU); shorter
-    // chunks repeat two halo planes per chunk but keep small levels' CUs busy
+    // chunks repeat two halo planes per chunk but keep small levels' CUs busy. The rule ignores the
+    // instance count so that an instance's tiles, and so its fused dot partials, are those of a lone
+    // launch (bitwise the same results)
     const int gx = (L.NX + TX - 1) / TX, gy = (L.NY + TY - 1) / TY;
     int zChunk = 4;
     for (int c : {16, 8})
@@ -526,17 +578,30 @@ template <class T> void launchFine(const DevLevel& L, int mode, const T* x, T* y
             break;
         }
     if (const char* e = std::getenv("PSX_ZCHUNK")) zChunk = std::max(2, std::atoi(e));
+    const bool fused = (fd.kind == FD_PQ && mode == AM_APPLY) || (fd.kind == FD_RZ && (mode == AM_CHEB1 || mode == AM_CHEB2));
+    double* dots = fused ? fd.red.partials : nullptr;
+    const T* qdot = fused && fd.kind == FD_RZ ? static_cast<const T*>(fd.q) : nullptr;
     if (L.childForm) {
         const T *crow = nullptr, *ctab = nullptr;
         if constexpr (sizeof(T) == 8) crow = L.crow64, ctab = L.ctab64;
         else crow = L.crow32, ctab = L.ctab32;
-        launchFineT<T, true>(L, mode, C, crow, ctab, x, y, r, d, c1, c2, stop, zChunk, s);
+        launchFineT<T, true>(L, mode, C, crow, ctab, x, y, r, d, c1, c2, stop, zChunk, dots, fd.red.stride, qdot, s, ninst);
     } else {
-        launchFineT<T, false>(L, mode, C, nullptr, nullptr, x, y, r, d, c1, c2, stop, zChunk, s);
+        launchFineT<T, false>(L, mode, C, nullptr, nullptr, x, y, r, d, c1, c2, stop, zChunk, dots, fd.red.stride, qdot, s, ninst);
     }
+    return fused ? unsigned(gx * gy * ((L.NZ + zChunk - 1) / zChunk)) : 0u;
 }
 
-template void launchFine<double>(const DevLevel&, int, const double*, double*, const double*, const double*, double, double, const int*, hipStream_t);
-template void launchFine<float>(const DevLevel&, int, const float*, float*, const float*, const float*, double, double, const int*, hipStream_t);
+template <class T>
+void launchFine(const DevLevel& L, int mode, const T* x, T* y, const T* r, const T* d, double c1, double c2, const int* stop, hipStream_t s, int ninst) {
+    launchFineDot<T>(L, mode, x, y, r, d, c1, c2, stop, DotFusion{}, s, ninst);
+}
+
+template unsigned launchFineDot<double>(const DevLevel&, int, const double*, double*, const double*, const double*, double, double, const int*,
+                                        const DotFusion&, hipStream_t, int);
+template unsigned launchFineDot<float>(const DevLevel&, int, const float*, float*, const float*, const float*, double, double, const int*,
+                                       const DotFusion&, hipStream_t, int);
+template void launchFine<double>(const DevLevel&, int, const double*, double*, const double*, const double*, double, double, const int*, hipStream_t, int);
+template void launchFine<float>(const DevLevel&, int, const float*, float*, const float*, const float*, double, double, const int*, hipStream_t, int);
 
 }  // namespace psx

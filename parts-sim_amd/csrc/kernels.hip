@@ -7,10 +7,12 @@
 // (243 multiply-adds); any other node sums the rows of its up to 8 voxels (72 multiply-adds each),
 // from s * Kb or from the voxel's own matrix. Nothing is scattered, so there are no write conflicts.
 //
-// Reductions: every block reduces with wavefront shuffles (64 lanes) and LDS, writes one partial per
-// block; the block that finishes last (atomic counter) sums the partials in block order. The grid
-// size depends only on the vector length, so every sum is bitwise reproducible and independent of
-// what else runs on the GPU.
+// Reductions: every block reduces with wavefront shuffles (64 lanes) and LDS and writes one partial;
+// a one-block finish kernel sums the partials in block order. The grid size depends only on the
+// vector length, so every sum is bitwise reproducible and independent of what else runs on the GPU.
+//
+// Every solve kernel also serves multi-instance launches (gridDim.y right-hand sides of one level):
+// blockIdx.y selects the instance's slice of each vector and its CgState.
 #include <hip/hip_runtime.h>
 
 #include "device.hpp"
@@ -40,6 +42,9 @@ template <> struct Prec<float> {
 };
 
 __device__ __forceinline__ bool stopped(const int* stop) { return stop && __builtin_nontemporal_load(stop) != 0; }
+// multi-instance launches: the instance's stop flag and the start of its slice of a vector
+__device__ __forceinline__ const int* instFlag(const int* f) { return f ? f + int64_t(blockIdx.y) * CG_INTS : nullptr; }
+__device__ __forceinline__ int64_t instOffset(int64_t n) { return int64_t(blockIdx.y) * n; }
 
 // ------------------------------------------------------------------------------------------------
 // reductions
@@ -75,20 +80,35 @@ template <int NV> __device__ __forceinline__ void blockPartials(double (&v)[NV],
     blockSum<NV>(v);
     if (threadIdx.x == 0)
 #pragma unroll
-        for (int k = 0; k < NV; k++) red.partials[k * gridDim.x + blockIdx.x] = v[k];
+        for (int k = 0; k < NV; k++) red.partials[int64_t(blockIdx.y) * red.stride + k * gridDim.x + blockIdx.x] = v[k];
 }
 
 // guard: skip when *guard != 0 (the producer kernel returned early for the same reason)
+// one block per instance (gridDim.x = instances)
 template <int NV, class Fin> __global__ __launch_bounds__(BLOCK) void kFinish(Reduce red, unsigned nb, const int* guard, Fin fin) {
-    if (guard && *guard) return;
+    const int inst = blockIdx.x;
+    if (guard && guard[int64_t(inst) * CG_INTS]) return;
+    red.partials += int64_t(inst) * red.stride;
+    // 8 independent partial sums per thread keep 8 loads in flight; they are combined in a fixed
+    // order, so the result does not depend on timing
+    constexpr int U = 8;
     double s[NV];
 #pragma unroll
     for (int k = 0; k < NV; k++) {
-        s[k] = 0;
-        for (unsigned b = threadIdx.x; b < nb; b += BLOCK) s[k] += red.partials[k * nb + b];
+        double a[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) a[u] = 0;
+        for (unsigned b0 = 0; b0 < nb; b0 += BLOCK * U) {
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const unsigned b = b0 + u * BLOCK + threadIdx.x;
+                if (b < nb) a[u] += red.partials[k * nb + b];
+            }
+        }
+        s[k] = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
     }
     blockSum<NV>(s);
-    if (threadIdx.x == 0) fin(s);
+    if (threadIdx.x == 0) fin(s, inst);
 }
 
 // reductions: each block owns a contiguous tile of RED_TILE elements, 8 per thread, loads issued
@@ -182,7 +202,15 @@ __device__ __forceinline__ void applyNode(const DevLevel& L, const T* __restrict
 template <class T, int MODE>
 __global__ __launch_bounds__(BLOCK) void kApply(DevLevel L, const T* __restrict__ x, T* __restrict__ y, const T* __restrict__ r, const T* __restrict__ zp,
                                                  T c1, T c2, const int* stop) {
+    stop = instFlag(stop);
     if (stopped(stop)) return;
+    {
+        const int64_t io = instOffset(L.nDof);
+        x += io;
+        y += io;
+        if (r) r += io;
+        if (zp) zp += io;
+    }
     const int64_t n = int64_t(blockIdx.x) * BLOCK + threadIdx.x;
     if (n >= L.nNodes) return;
     const unsigned nn = unsigned(n), NX = unsigned(L.NX), NY = unsigned(L.NY);
@@ -209,7 +237,15 @@ __global__ __launch_bounds__(BLOCK) void kApply(DevLevel L, const T* __restrict_
 template <class T, int MODE>
 __global__ __launch_bounds__(BLOCK) void kApplyOwn(DevLevel L, const T* __restrict__ x, T* __restrict__ y, const T* __restrict__ r, const T* __restrict__ zp,
                                                    T c1, T c2, const int* stop) {
+    stop = instFlag(stop);
     if (stopped(stop)) return;
+    {
+        const int64_t io = instOffset(L.nDof);
+        x += io;
+        y += io;
+        if (r) r += io;
+        if (zp) zp += io;
+    }
     __shared__ T Kb[576];
     for (int q = threadIdx.x; q < 576; q += BLOCK) Kb[q] = Prec<T>::Kb(L)[q];
     __syncthreads();
@@ -287,10 +323,12 @@ __global__ __launch_bounds__(BLOCK) void kApplyOwn(DevLevel L, const T* __restri
 
 template <class T> __global__ __launch_bounds__(BLOCK) void kChebFirst(const T* __restrict__ invD, const T* __restrict__ r, T* __restrict__ z, T c0, int64_t n,
                                                                        const int* stop) {
+    stop = instFlag(stop);
     if (stopped(stop)) return;
     const int64_t q = int64_t(blockIdx.x) * BLOCK + threadIdx.x;
     if (q >= n) return;
-    z[q] = c0 * invD[q] * r[q];
+    const int64_t io = instOffset(n);
+    z[io + q] = c0 * invD[q] * r[io + q];
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -298,7 +336,10 @@ template <class T> __global__ __launch_bounds__(BLOCK) void kChebFirst(const T* 
 
 template <class T>
 __global__ __launch_bounds__(BLOCK) void kRestrict(DevLevel F, DevLevel C, const T* __restrict__ rf, T* __restrict__ rc, const int* stop) {
+    stop = instFlag(stop);
     if (stopped(stop)) return;
+    rf += instOffset(F.nDof);
+    rc += instOffset(C.nDof);
     const int64_t cn = int64_t(blockIdx.x) * BLOCK + threadIdx.x;
     if (cn >= C.nNodes) return;
     const unsigned u = unsigned(cn), CNX = unsigned(C.NX), CNY = unsigned(C.NY);
@@ -335,7 +376,10 @@ __global__ __launch_bounds__(BLOCK) void kRestrict(DevLevel F, DevLevel C, const
 
 template <class T>
 __global__ __launch_bounds__(BLOCK) void kProlong(DevLevel F, DevLevel C, const T* __restrict__ zc, T* __restrict__ zf, const int* stop) {
+    stop = instFlag(stop);
     if (stopped(stop)) return;
+    zc += instOffset(C.nDof);
+    zf += instOffset(F.nDof);
     const int64_t fn = int64_t(blockIdx.x) * BLOCK + threadIdx.x;
     if (fn >= F.nNodes) return;
     const T* __restrict__ finv = Prec<T>::invD(F);
@@ -371,7 +415,10 @@ constexpr int COARSE_MAX = 3072;
 template <class T>
 __global__ __launch_bounds__(BLOCK) void kCoarse(const int32_t* __restrict__ freeIdx, const int32_t* __restrict__ dense, int m, int64_t nDof,
                                                  const double* __restrict__ Ainv, const T* __restrict__ r, T* __restrict__ z, const int* stop) {
+    stop = instFlag(stop);
     if (stopped(stop)) return;
+    r += instOffset(nDof);
+    z += instOffset(nDof);
     __shared__ double rl[COARSE_MAX];
     for (int q = threadIdx.x; q < m; q += BLOCK) rl[q] = double(r[freeIdx[q]]);
     __syncthreads();
@@ -401,7 +448,10 @@ __global__ __launch_bounds__(BLOCK) void kCoarse(const int32_t* __restrict__ fre
 
 template <class T>
 __global__ __launch_bounds__(BLOCK) void kDotPQ(const T* __restrict__ p, const T* __restrict__ q, int64_t n, CgState* st, Reduce red) {
+    st += blockIdx.y;
     if (st->stop) return;
+    p += instOffset(n);
+    q += instOffset(n);
     double v[1] = {0};
     TILE_LOOP(v[0] += double(p[i]) * double(q[i]);)
     blockPartials<1>(v, red);
@@ -410,7 +460,15 @@ __global__ __launch_bounds__(BLOCK) void kDotPQ(const T* __restrict__ p, const T
 template <class T>
 __global__ __launch_bounds__(BLOCK) void kUpdateXR(T* __restrict__ x, const T* __restrict__ p, T* __restrict__ r, const T* __restrict__ q, int64_t n,
                                                    CgState* st, Reduce red) {
+    st += blockIdx.y;
     if (st->stop) return;
+    {
+        const int64_t io = instOffset(n);
+        x += io;
+        p += io;
+        r += io;
+        q += io;
+    }
     const T a = T(st->alpha);
     double v[1] = {0};
     TILE_LOOP(
@@ -424,7 +482,14 @@ __global__ __launch_bounds__(BLOCK) void kUpdateXR(T* __restrict__ x, const T* _
 template <class T>
 __global__ __launch_bounds__(BLOCK) void kDotsRZ(const T* __restrict__ r, const T* __restrict__ z, const T* __restrict__ q, int64_t n, CgState* st,
                                                  Reduce red) {
+    st += blockIdx.y;
     if (st->stop) return;
+    {
+        const int64_t io = instOffset(n);
+        r += io;
+        z += io;
+        q += io;
+    }
     double v[2] = {0, 0};
     TILE_LOOP(
         const double zi = double(z[i]);
@@ -435,32 +500,35 @@ __global__ __launch_bounds__(BLOCK) void kDotsRZ(const T* __restrict__ r, const 
 
 template <class T>
 __global__ __launch_bounds__(BLOCK) void kUpdateP(const T* __restrict__ z, T* __restrict__ p, int64_t n, const CgState* st) {
+    st += blockIdx.y;
     if (st->stop) return;
     const int64_t i = int64_t(blockIdx.x) * BLOCK + threadIdx.x;
     if (i >= n) return;
-    p[i] = z[i] + T(st->beta) * p[i];
+    const int64_t io = instOffset(n);
+    p[io + i] = z[io + i] + T(st->beta) * p[io + i];
 }
 
-// mixed precision: x and r in FP64, p and q in FP32 in units of 1 / rmax; r32 = r / rmax (the
-// preconditioner's input). The FP64 residual keeps the recurrence free of FP32 storage rounding.
-__global__ __launch_bounds__(BLOCK) void kUpdateXRMixed(double* __restrict__ x, const float* __restrict__ p, double* __restrict__ r,
-                                                        float* __restrict__ r32, const float* __restrict__ q, int64_t n, CgState* st, Reduce red) {
-    if (st->stop) return;
-    const double s = st->rmax, as = st->alpha * s, inv = 1.0 / s;
-    double v[1] = {0};
-    TILE_LOOP(
-        x[i] += as * double(p[i]);
-        const double ri = r[i] - as * double(q[i]);
-        r[i] = ri;
-        const double rs = ri * inv;
-        r32[i] = float(rs);
-        v[0] += rs * rs;)
-    blockPartials<1>(v, red);
+// mixed precision with FP32 recurrences: the correction x32 and the residual r32 in units of
+// 1 / rmax; the correction joins the FP64 solution at every fold
+__global__ __launch_bounds__(BLOCK) void kFoldX(double* __restrict__ u, float* __restrict__ x, int64_t n, const CgState* st) {
+    st += blockIdx.y;
+    if (st->noFold) return;
+    const int64_t i = int64_t(blockIdx.x) * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    u += instOffset(n);
+    x += instOffset(n);
+    u[i] += double(x[i]) * st->rmax;
+    x[i] = 0.f;
 }
 
 __global__ __launch_bounds__(BLOCK) void kFoldR(const double* __restrict__ f, const double* __restrict__ Kx, const double* __restrict__ invD,
                                                 double* __restrict__ r64, float* __restrict__ r32, int64_t n, CgState* st, Reduce red) {
+    st += blockIdx.y;
     if (st->noFold) return;
+    f += instOffset(n);
+    Kx += instOffset(n);
+    r64 += instOffset(n);
+    r32 += instOffset(n);
     const double inv = 1.0 / st->rmax;
     double v[1] = {0};
     TILE_LOOP(
@@ -473,6 +541,9 @@ __global__ __launch_bounds__(BLOCK) void kFoldR(const double* __restrict__ f, co
 
 __global__ __launch_bounds__(BLOCK) void kResidual64(const double* __restrict__ f, const double* __restrict__ Kx, const double* __restrict__ invD,
                                                      double* __restrict__ r, int64_t n, Reduce red) {
+    f += instOffset(n);
+    Kx += instOffset(n);
+    r += instOffset(n);
     double v[1] = {0};
     TILE_LOOP(
         const double ri = invD[i] != 0 ? f[i] - Kx[i] : 0.0;
@@ -482,6 +553,7 @@ __global__ __launch_bounds__(BLOCK) void kResidual64(const double* __restrict__ 
 }
 
 __global__ __launch_bounds__(BLOCK) void kNormFree64(const double* __restrict__ f, const double* __restrict__ invD, int64_t n, Reduce red) {
+    f += instOffset(n);
     double v[1] = {0};
     TILE_LOOP(if (invD[i] != 0) v[0] += f[i] * f[i];)
     blockPartials<1>(v, red);
@@ -490,10 +562,12 @@ __global__ __launch_bounds__(BLOCK) void kNormFree64(const double* __restrict__ 
 __global__ __launch_bounds__(BLOCK) void kMaskCopy64(const double* __restrict__ x0, const double* __restrict__ invD, double* __restrict__ x, int64_t n) {
     const int64_t i = int64_t(blockIdx.x) * BLOCK + threadIdx.x;
     if (i >= n) return;
-    x[i] = (x0 && invD[i] != 0) ? x0[i] : 0.0;
+    const int64_t io = instOffset(n);
+    x[io + i] = (x0 && invD[i] != 0) ? x0[io + i] : 0.0;
 }
 
 __global__ __launch_bounds__(BLOCK) void kMaxAbs64(const double* __restrict__ v, int64_t n, Reduce red) {
+    v += instOffset(n);
     double m = 0;
     TILE_LOOP(m = fmax(m, fabs(v[i]));)
     __shared__ double sh[BLOCK / 64];
@@ -503,11 +577,14 @@ __global__ __launch_bounds__(BLOCK) void kMaxAbs64(const double* __restrict__ v,
     if (threadIdx.x == 0) {
         double b = sh[0];
         for (int w = 1; w < BLOCK / 64; w++) b = fmax(b, sh[w]);
-        red.partials[blockIdx.x] = b;
+        red.partials[int64_t(blockIdx.y) * red.stride + blockIdx.x] = b;
     }
 }
 
+// one block per instance; out steps by CgState
 __global__ __launch_bounds__(BLOCK) void kMaxFinish(Reduce red, unsigned nb, double* out) {
+    red.partials += int64_t(blockIdx.x) * red.stride;
+    out += int64_t(blockIdx.x) * CG_DBLS;
     double m = 0;
     for (unsigned b = threadIdx.x; b < nb; b += BLOCK) m = fmax(m, red.partials[b]);
     __shared__ double sh[BLOCK / 64];
@@ -521,17 +598,20 @@ __global__ __launch_bounds__(BLOCK) void kMaxFinish(Reduce red, unsigned nb, dou
     }
 }
 
+// scaleFrom points into CgState (steps by instance)
 __global__ __launch_bounds__(BLOCK) void kToF32(const double* __restrict__ x, float* __restrict__ y, int64_t n, const double* scaleFrom) {
     const int64_t i = int64_t(blockIdx.x) * BLOCK + threadIdx.x;
     if (i >= n) return;
-    const double s = *scaleFrom;
-    y[i] = s > 0 ? float(x[i] * (1.0 / s)) : 0.f;
+    const int64_t io = instOffset(n);
+    const double s = scaleFrom[int64_t(blockIdx.y) * CG_DBLS];
+    y[io + i] = s > 0 ? float(x[io + i] * (1.0 / s)) : 0.f;
 }
 
 __global__ __launch_bounds__(BLOCK) void kFromF32(const float* __restrict__ z, double* __restrict__ y, int64_t n, const double* scaleFrom) {
     const int64_t i = int64_t(blockIdx.x) * BLOCK + threadIdx.x;
     if (i >= n) return;
-    y[i] = double(z[i]) * *scaleFrom;
+    const int64_t io = instOffset(n);
+    y[io + i] = double(z[io + i]) * scaleFrom[int64_t(blockIdx.y) * CG_DBLS];
 }
 
 // Lanczos on D^-1/2 K D^-1/2
@@ -574,8 +654,9 @@ __global__ __launch_bounds__(BLOCK) void kLzNext(const double* __restrict__ w, d
 
 // finalizers of the reductions (thread 0 of the one-block kernel)
 struct FinPQ {
-    CgState* st;
-    __device__ void operator()(const double* s) const {
+    CgState* st0;
+    __device__ void operator()(const double* s, int inst) const {
+        CgState* st = st0 + inst;
         st->pq = s[0];
         if (!(s[0] > 0)) {  // loss of positive-definiteness (a mechanism)
             st->broken = 1;
@@ -584,8 +665,9 @@ struct FinPQ {
     }
 };
 struct FinXR {
-    CgState* st;
-    __device__ void operator()(const double* s) const {
+    CgState* st0;
+    __device__ void operator()(const double* s, int inst) const {
+        CgState* st = st0 + inst;
         st->rr = s[0];
         st->iters += 1;
         st->sinceFold += 1;
@@ -594,8 +676,9 @@ struct FinXR {
     }
 };
 struct FinRZ {
-    CgState* st;
-    __device__ void operator()(const double* s) const {
+    CgState* st0;
+    __device__ void operator()(const double* s, int inst) const {
+        CgState* st = st0 + inst;
         st->rzNew = s[0];
         st->zq = s[1];
         const double b = st->rz != 0 ? -st->alpha * s[1] / st->rz : 0.0;
@@ -604,8 +687,9 @@ struct FinRZ {
     }
 };
 struct FinXRFold {
-    CgState* st;
-    __device__ void operator()(const double* s) const {
+    CgState* st0;
+    __device__ void operator()(const double* s, int inst) const {
+        CgState* st = st0 + inst;
         st->rr = s[0];
         st->iters += 1;
         st->sinceFold += 1;
@@ -617,8 +701,9 @@ struct FinXRFold {
     }
 };
 struct FinFoldR {
-    CgState* st;
-    __device__ void operator()(const double* s) const {
+    CgState* st0;
+    __device__ void operator()(const double* s, int inst) const {
+        CgState* st = st0 + inst;
         const double rel = sqrt(s[0]) / st->bnorm;
         st->trueRel = rel;
         st->rel = rel;
@@ -630,18 +715,18 @@ struct FinFoldR {
     }
 };
 struct FinStore {
-    double* out;
-    __device__ void operator()(const double* s) const { *out = s[0]; }
+    double* out;  // into CgState: steps by instance
+    __device__ void operator()(const double* s, int inst) const { out[int64_t(inst) * CG_DBLS] = s[0]; }
 };
 struct FinLzA {
     CgState* st;
     int j;
-    __device__ void operator()(const double* s) const { st->lz[j] = s[0]; }
+    __device__ void operator()(const double* s, int) const { st->lz[j] = s[0]; }
 };
 struct FinLzB {
     CgState* st;
     int j;
-    __device__ void operator()(const double* s) const { st->lz[16 + j] = sqrt(s[0]); }
+    __device__ void operator()(const double* s, int) const { st->lz[16 + j] = sqrt(s[0]); }
 };
 
 }  // namespace
@@ -649,8 +734,9 @@ struct FinLzB {
 // ------------------------------------------------------------------------------------------------
 // launchers
 
-template <class T> void launchApply(const DevLevel& L, int mode, const T* x, T* y, const T* r, const T* d, double c1, double c2, const int* stop, hipStream_t s) {
-    const dim3 g(flatGrid(L.nNodes)), b(BLOCK);
+template <class T>
+void launchApply(const DevLevel& L, int mode, const T* x, T* y, const T* r, const T* d, double c1, double c2, const int* stop, hipStream_t s, int ninst) {
+    const dim3 g(flatGrid(L.nNodes), ninst), b(BLOCK);
     switch (mode) {
         case AM_APPLY: kApply<T, AM_APPLY><<<g, b, 0, s>>>(L, x, y, r, d, T(c1), T(c2), stop); break;
         case AM_RAW: kApply<T, AM_RAW><<<g, b, 0, s>>>(L, x, y, r, d, T(c1), T(c2), stop); break;
@@ -660,8 +746,9 @@ template <class T> void launchApply(const DevLevel& L, int mode, const T* x, T* 
     }
 }
 
-template <class T> void launchApplyOwn(const DevLevel& L, int mode, const T* x, T* y, const T* r, const T* d, double c1, double c2, const int* stop, hipStream_t s) {
-    const dim3 g(unsigned((L.nNodes + BLOCK / 8 - 1) / (BLOCK / 8))), b(BLOCK);
+template <class T>
+void launchApplyOwn(const DevLevel& L, int mode, const T* x, T* y, const T* r, const T* d, double c1, double c2, const int* stop, hipStream_t s, int ninst) {
+    const dim3 g(unsigned((L.nNodes + BLOCK / 8 - 1) / (BLOCK / 8)), ninst), b(BLOCK);
     switch (mode) {
         case AM_APPLY: kApplyOwn<T, AM_APPLY><<<g, b, 0, s>>>(L, x, y, r, d, T(c1), T(c2), stop); break;
         case AM_RAW: kApplyOwn<T, AM_RAW><<<g, b, 0, s>>>(L, x, y, r, d, T(c1), T(c2), stop); break;
@@ -671,70 +758,86 @@ template <class T> void launchApplyOwn(const DevLevel& L, int mode, const T* x, 
     }
 }
 
-template <class T> void launchChebFirst(const DevLevel& L, const T* r, T* z, double c0, const int* stop, hipStream_t s) {
+template <class T> void launchChebFirst(const DevLevel& L, const T* r, T* z, double c0, const int* stop, hipStream_t s, int ninst) {
     const T* invD = sizeof(T) == 8 ? reinterpret_cast<const T*>(L.invD64) : reinterpret_cast<const T*>(L.invD32);
-    kChebFirst<T><<<flatGrid(L.nDof), BLOCK, 0, s>>>(invD, r, z, T(c0), L.nDof, stop);
+    kChebFirst<T><<<dim3(flatGrid(L.nDof), ninst), BLOCK, 0, s>>>(invD, r, z, T(c0), L.nDof, stop);
 }
 
-template <class T> void launchRestrict(const DevLevel& F, const DevLevel& C, const T* rf, T* rc, const int* stop, hipStream_t s) {
-    kRestrict<T><<<flatGrid(C.nNodes), BLOCK, 0, s>>>(F, C, rf, rc, stop);
+template <class T> void launchRestrict(const DevLevel& F, const DevLevel& C, const T* rf, T* rc, const int* stop, hipStream_t s, int ninst) {
+    kRestrict<T><<<dim3(flatGrid(C.nNodes), ninst), BLOCK, 0, s>>>(F, C, rf, rc, stop);
 }
 
-template <class T> void launchProlong(const DevLevel& F, const DevLevel& C, const T* zc, T* zf, const int* stop, hipStream_t s) {
-    kProlong<T><<<flatGrid(F.nNodes), BLOCK, 0, s>>>(F, C, zc, zf, stop);
+template <class T> void launchProlong(const DevLevel& F, const DevLevel& C, const T* zc, T* zf, const int* stop, hipStream_t s, int ninst) {
+    kProlong<T><<<dim3(flatGrid(F.nNodes), ninst), BLOCK, 0, s>>>(F, C, zc, zf, stop);
 }
 
 template <class T>
-void launchCoarse(const int32_t* freeIdx, const int32_t* dense, int64_t m, int64_t nDof, const double* Ainv, const T* r, T* z, const int* stop, hipStream_t s) {
-    kCoarse<T><<<unsigned((nDof + BLOCK / 64 - 1) / (BLOCK / 64)), BLOCK, 0, s>>>(freeIdx, dense, int(m), nDof, Ainv, r, z, stop);
+void launchCoarse(const int32_t* freeIdx, const int32_t* dense, int64_t m, int64_t nDof, const double* Ainv, const T* r, T* z, const int* stop, hipStream_t s,
+                  int ninst) {
+    kCoarse<T><<<dim3(unsigned((nDof + BLOCK / 64 - 1) / (BLOCK / 64)), ninst), BLOCK, 0, s>>>(freeIdx, dense, int(m), nDof, Ainv, r, z, stop);
 }
 
-template <class T> void launchDotPQ(const T* p, const T* q, int64_t n, CgState* st, Reduce red, hipStream_t s) {
+template <class T> void launchDotPQ(const T* p, const T* q, int64_t n, CgState* st, Reduce red, hipStream_t s, int ninst) {
     const unsigned g = reduceGrid(n);
-    kDotPQ<T><<<g, BLOCK, 0, s>>>(p, q, n, st, red);
-    kFinish<1, FinPQ><<<1, BLOCK, 0, s>>>(red, g, &st->stop, FinPQ{st});
+    kDotPQ<T><<<dim3(g, ninst), BLOCK, 0, s>>>(p, q, n, st, red);
+    kFinish<1, FinPQ><<<ninst, BLOCK, 0, s>>>(red, g, &st->stop, FinPQ{st});
 }
-template <class T> void launchUpdateXR(T* x, const T* p, T* r, const T* q, int64_t n, CgState* st, Reduce red, hipStream_t s) {
+template <class T> void launchUpdateXR(T* x, const T* p, T* r, const T* q, int64_t n, CgState* st, Reduce red, hipStream_t s, int ninst) {
     const unsigned g = reduceGrid(n);
-    kUpdateXR<T><<<g, BLOCK, 0, s>>>(x, p, r, q, n, st, red);
-    kFinish<1, FinXR><<<1, BLOCK, 0, s>>>(red, g, &st->stop, FinXR{st});
+    kUpdateXR<T><<<dim3(g, ninst), BLOCK, 0, s>>>(x, p, r, q, n, st, red);
+    kFinish<1, FinXR><<<ninst, BLOCK, 0, s>>>(red, g, &st->stop, FinXR{st});
 }
-template <class T> void launchDotsRZ(const T* r, const T* z, const T* q, int64_t n, CgState* st, Reduce red, hipStream_t s) {
+template <class T> void launchDotsRZ(const T* r, const T* z, const T* q, int64_t n, CgState* st, Reduce red, hipStream_t s, int ninst) {
     const unsigned g = reduceGrid(n);
-    kDotsRZ<T><<<g, BLOCK, 0, s>>>(r, z, q, n, st, red);
-    kFinish<2, FinRZ><<<1, BLOCK, 0, s>>>(red, g, &st->stop, FinRZ{st});
+    kDotsRZ<T><<<dim3(g, ninst), BLOCK, 0, s>>>(r, z, q, n, st, red);
+    kFinish<2, FinRZ><<<ninst, BLOCK, 0, s>>>(red, g, &st->stop, FinRZ{st});
 }
-template <class T> void launchUpdateP(const T* z, T* p, int64_t n, CgState* st, hipStream_t s) { kUpdateP<T><<<flatGrid(n), BLOCK, 0, s>>>(z, p, n, st); }
-
-void launchUpdateXRMixed(double* x, const float* p, double* r, float* r32, const float* q, int64_t n, CgState* st, Reduce red, hipStream_t s) {
-    const unsigned g = reduceGrid(n);
-    kUpdateXRMixed<<<g, BLOCK, 0, s>>>(x, p, r, r32, q, n, st, red);
-    kFinish<1, FinXRFold><<<1, BLOCK, 0, s>>>(red, g, &st->stop, FinXRFold{st});
-}
-void launchFoldR(const double* f, const double* Kx, const double* invD, double* r64, float* r32, int64_t n, CgState* st, Reduce red, hipStream_t s) {
-    const unsigned g = reduceGrid(n);
-    kFoldR<<<g, BLOCK, 0, s>>>(f, Kx, invD, r64, r32, n, st, red);
-    kFinish<1, FinFoldR><<<1, BLOCK, 0, s>>>(red, g, &st->noFold, FinFoldR{st});
+template <class T> void launchUpdateP(const T* z, T* p, int64_t n, CgState* st, hipStream_t s, int ninst) {
+    kUpdateP<T><<<dim3(flatGrid(n), ninst), BLOCK, 0, s>>>(z, p, n, st);
 }
 
-void launchResidual64(const double* f, const double* Kx, const double* invD, double* r, int64_t n, double* out, Reduce red, hipStream_t s) {
+// finishers for dot products whose block partials another kernel wrote ([k * nb + block])
+void launchFinishPQ(Reduce red, unsigned nb, CgState* st, hipStream_t s, int ninst) { kFinish<1, FinPQ><<<ninst, BLOCK, 0, s>>>(red, nb, &st->stop, FinPQ{st}); }
+void launchFinishRZ(Reduce red, unsigned nb, CgState* st, hipStream_t s, int ninst) { kFinish<2, FinRZ><<<ninst, BLOCK, 0, s>>>(red, nb, &st->stop, FinRZ{st}); }
+
+void launchUpdateXR32Fold(float* x, const float* p, float* r, const float* q, int64_t n, CgState* st, Reduce red, hipStream_t s, int ninst) {
     const unsigned g = reduceGrid(n);
-    kResidual64<<<g, BLOCK, 0, s>>>(f, Kx, invD, r, n, red);
-    kFinish<1, FinStore><<<1, BLOCK, 0, s>>>(red, g, nullptr, FinStore{out});
+    kUpdateXR<float><<<dim3(g, ninst), BLOCK, 0, s>>>(x, p, r, q, n, st, red);
+    kFinish<1, FinXRFold><<<ninst, BLOCK, 0, s>>>(red, g, &st->stop, FinXRFold{st});
 }
-void launchNormFree64(const double* f, const double* invD, int64_t n, double* out, Reduce red, hipStream_t s) {
+void launchFoldX(double* u, float* x, int64_t n, CgState* st, hipStream_t s, int ninst) { kFoldX<<<dim3(flatGrid(n), ninst), BLOCK, 0, s>>>(u, x, n, st); }
+
+void launchFoldR(const double* f, const double* Kx, const double* invD, double* r64, float* r32, int64_t n, CgState* st, Reduce red, hipStream_t s,
+                 int ninst) {
     const unsigned g = reduceGrid(n);
-    kNormFree64<<<g, BLOCK, 0, s>>>(f, invD, n, red);
-    kFinish<1, FinStore><<<1, BLOCK, 0, s>>>(red, g, nullptr, FinStore{out});
+    kFoldR<<<dim3(g, ninst), BLOCK, 0, s>>>(f, Kx, invD, r64, r32, n, st, red);
+    kFinish<1, FinFoldR><<<ninst, BLOCK, 0, s>>>(red, g, &st->noFold, FinFoldR{st});
 }
-void launchMaskCopy64(const double* x0, const double* invD, double* x, int64_t n, hipStream_t s) { kMaskCopy64<<<flatGrid(n), BLOCK, 0, s>>>(x0, invD, x, n); }
-void launchMaxAbs64(const double* v, int64_t n, double* out, Reduce red, hipStream_t s) {
+
+void launchResidual64(const double* f, const double* Kx, const double* invD, double* r, int64_t n, double* out, Reduce red, hipStream_t s, int ninst) {
     const unsigned g = reduceGrid(n);
-    kMaxAbs64<<<g, BLOCK, 0, s>>>(v, n, red);
-    kMaxFinish<<<1, BLOCK, 0, s>>>(red, g, out);
+    kResidual64<<<dim3(g, ninst), BLOCK, 0, s>>>(f, Kx, invD, r, n, red);
+    kFinish<1, FinStore><<<ninst, BLOCK, 0, s>>>(red, g, nullptr, FinStore{out});
 }
-void launchToF32(const double* x, float* y, int64_t n, const double* scaleFrom, hipStream_t s) { kToF32<<<flatGrid(n), BLOCK, 0, s>>>(x, y, n, scaleFrom); }
-void launchFromF32(const float* z, double* y, int64_t n, const double* scaleFrom, hipStream_t s) { kFromF32<<<flatGrid(n), BLOCK, 0, s>>>(z, y, n, scaleFrom); }
+void launchNormFree64(const double* f, const double* invD, int64_t n, double* out, Reduce red, hipStream_t s, int ninst) {
+    const unsigned g = reduceGrid(n);
+    kNormFree64<<<dim3(g, ninst), BLOCK, 0, s>>>(f, invD, n, red);
+    kFinish<1, FinStore><<<ninst, BLOCK, 0, s>>>(red, g, nullptr, FinStore{out});
+}
+void launchMaskCopy64(const double* x0, const double* invD, double* x, int64_t n, hipStream_t s, int ninst) {
+    kMaskCopy64<<<dim3(flatGrid(n), ninst), BLOCK, 0, s>>>(x0, invD, x, n);
+}
+void launchMaxAbs64(const double* v, int64_t n, double* out, Reduce red, hipStream_t s, int ninst) {
+    const unsigned g = reduceGrid(n);
+    kMaxAbs64<<<dim3(g, ninst), BLOCK, 0, s>>>(v, n, red);
+    kMaxFinish<<<ninst, BLOCK, 0, s>>>(red, g, out);
+}
+void launchToF32(const double* x, float* y, int64_t n, const double* scaleFrom, hipStream_t s, int ninst) {
+    kToF32<<<dim3(flatGrid(n), ninst), BLOCK, 0, s>>>(x, y, n, scaleFrom);
+}
+void launchFromF32(const float* z, double* y, int64_t n, const double* scaleFrom, hipStream_t s, int ninst) {
+    kFromF32<<<dim3(flatGrid(n), ninst), BLOCK, 0, s>>>(z, y, n, scaleFrom);
+}
 void launchLzX(const double* v, const double* invD, double* x, int64_t n, hipStream_t s) { kLzX<<<flatGrid(n), BLOCK, 0, s>>>(v, invD, x, n); }
 void launchLzW(double* w, const double* invD, const double* vPrev, const double* v, int64_t n, CgState* st, int j, Reduce red, hipStream_t s) {
     const unsigned g = reduceGrid(n);
@@ -750,17 +853,17 @@ void launchLzNext(const double* w, double* v, double* vPrev, int64_t n, CgState*
     kLzNext<<<flatGrid(n), BLOCK, 0, s>>>(w, v, vPrev, n, st, j);
 }
 
-#define PSX_INSTANTIATE(T)                                                                                                              \
-    template void launchApply<T>(const DevLevel&, int, const T*, T*, const T*, const T*, double, double, const int*, hipStream_t);      \
-    template void launchChebFirst<T>(const DevLevel&, const T*, T*, double, const int*, hipStream_t);                                   \
-    template void launchApplyOwn<T>(const DevLevel&, int, const T*, T*, const T*, const T*, double, double, const int*, hipStream_t);   \
-    template void launchRestrict<T>(const DevLevel&, const DevLevel&, const T*, T*, const int*, hipStream_t);                          \
-    template void launchProlong<T>(const DevLevel&, const DevLevel&, const T*, T*, const int*, hipStream_t);                           \
-    template void launchCoarse<T>(const int32_t*, const int32_t*, int64_t, int64_t, const double*, const T*, T*, const int*, hipStream_t); \
-    template void launchDotPQ<T>(const T*, const T*, int64_t, CgState*, Reduce, hipStream_t);                                         \
-    template void launchUpdateXR<T>(T*, const T*, T*, const T*, int64_t, CgState*, Reduce, hipStream_t);                              \
-    template void launchDotsRZ<T>(const T*, const T*, const T*, int64_t, CgState*, Reduce, hipStream_t);                              \
-    template void launchUpdateP<T>(const T*, T*, int64_t, CgState*, hipStream_t);
+#define PSX_INSTANTIATE(T)                                                                                                                   \
+    template void launchApply<T>(const DevLevel&, int, const T*, T*, const T*, const T*, double, double, const int*, hipStream_t, int);     \
+    template void launchChebFirst<T>(const DevLevel&, const T*, T*, double, const int*, hipStream_t, int);                                  \
+    template void launchApplyOwn<T>(const DevLevel&, int, const T*, T*, const T*, const T*, double, double, const int*, hipStream_t, int);  \
+    template void launchRestrict<T>(const DevLevel&, const DevLevel&, const T*, T*, const int*, hipStream_t, int);                         \
+    template void launchProlong<T>(const DevLevel&, const DevLevel&, const T*, T*, const int*, hipStream_t, int);                          \
+    template void launchCoarse<T>(const int32_t*, const int32_t*, int64_t, int64_t, const double*, const T*, T*, const int*, hipStream_t, int); \
+    template void launchDotPQ<T>(const T*, const T*, int64_t, CgState*, Reduce, hipStream_t, int);                                        \
+    template void launchUpdateXR<T>(T*, const T*, T*, const T*, int64_t, CgState*, Reduce, hipStream_t, int);                             \
+    template void launchDotsRZ<T>(const T*, const T*, const T*, int64_t, CgState*, Reduce, hipStream_t, int);                             \
+    template void launchUpdateP<T>(const T*, T*, int64_t, CgState*, hipStream_t, int);
 
 PSX_INSTANTIATE(double)
 PSX_INSTANTIATE(float)

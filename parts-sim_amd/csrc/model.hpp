@@ -79,12 +79,16 @@ public:
     double* Ainv = nullptr;
     // level-0 conjugate-gradient vectors
     double *x = nullptr, *r = nullptr, *p = nullptr, *q = nullptr, *f = nullptr, *u = nullptr;
-    float *p32 = nullptr, *q32 = nullptr;
+    float *p32 = nullptr, *q32 = nullptr, *x32 = nullptr;
     double* stressS = nullptr;  // corner stress matrices scaled by E / h (8 x 144)
-    CgState* st = nullptr;
+    CgState* st = nullptr;      // one per right-hand side
     CgState* stHost = nullptr;  // pinned mirror
     Reduce red{};
     bool smoothersReady = false;
+    // right-hand sides the work vectors hold (instance-major, each level's nDof per instance) and
+    // the number the current launches run
+    int cap = 0;
+    int ni = 1;
 
     explicit Model(const psx_model_desc& d) {
         int n = 0;
@@ -101,17 +105,21 @@ public:
             for (void* ptr : allocs) (void)hipFreeAsync(ptr, stream);
             (void)hipStreamSynchronize(stream);
         }
+        if (stream) {
+            for (void* ptr : work) (void)hipFreeAsync(ptr, stream);
+            (void)hipStreamSynchronize(stream);
+        }
         if (stHost) (void)hipHostFree(stHost);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
-    template <class T> T* alloc(int64_t count) {
+    template <class T> T* alloc(int64_t count, bool perRhs = false) {
         void* ptr = nullptr;
         const size_t b = size_t(std::max<int64_t>(count, 1)) * sizeof(T);
         HIPCHK(hipMallocAsync(&ptr, b, stream));
         HIPCHK(hipMemsetAsync(ptr, 0, b, stream));
-        allocs.push_back(ptr);
-        bytes += int64_t(b);
+        (perRhs ? work : allocs).push_back(ptr);
+        (perRhs ? workBytes : bytes) += int64_t(b);
         return static_cast<T*>(ptr);
     }
     template <class T> T* upload(const T* src, int64_t count) {
@@ -182,14 +190,6 @@ public:
                 mfmaOperands(L.Kb, a.data());
                 lv[l].aop = upload(a.data(), int64_t(a.size()));
             }
-            lv[l].r64 = alloc<double>(L.nDof);
-            lv[l].a64 = alloc<double>(L.nDof);
-            lv[l].b64 = alloc<double>(L.nDof);
-            lv[l].d64 = alloc<double>(L.nDof);
-            lv[l].r32 = alloc<float>(L.nDof);
-            lv[l].a32 = alloc<float>(L.nDof);
-            lv[l].b32 = alloc<float>(L.nDof);
-            lv[l].d32 = alloc<float>(L.nDof);
         }
         freeIdx = upload(H.freeIdx.data(), H.m);
         {
@@ -198,23 +198,56 @@ public:
             denseIdx = upload(dense.data(), int64_t(dense.size()));
         }
         Ainv = upload(H.Ainv.data(), H.m * H.m);
-        const int64_t n = nDof();
-        x = alloc<double>(n);
-        r = alloc<double>(n);
-        p = alloc<double>(n);
-        q = alloc<double>(n);
-        f = alloc<double>(n);
-        u = alloc<double>(n);
-        p32 = alloc<float>(n);
-        q32 = alloc<float>(n);
-        st = alloc<CgState>(1);
         stressS = alloc<double>(8 * 144);
-        red.partials = alloc<double>(4 * (n / RED_TILE + 2));
         red.counter = alloc<unsigned>(1);
-        HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&stHost), sizeof(CgState), hipHostMallocDefault));
-        std::memset(stHost, 0, sizeof(CgState));
+        reserve(1);
+    }
+
+    // work vectors for k right-hand sides (grown, never shrunk; the contents do not survive)
+    void reserve(int k) {
+        if (k <= cap) return;
+        sync();
+        for (void* ptr : work) HIPCHK(hipFreeAsync(ptr, stream));
+        work.clear();
+        workBytes = 0;
+        for (size_t l = 0; l < H.levels.size(); l++) {
+            const int64_t nl = H.levels[l].nDof * int64_t(k);
+            lv[l].r64 = alloc<double>(nl, true);
+            lv[l].a64 = alloc<double>(nl, true);
+            lv[l].b64 = alloc<double>(nl, true);
+            lv[l].d64 = alloc<double>(nl, true);
+            lv[l].r32 = alloc<float>(nl, true);
+            lv[l].a32 = alloc<float>(nl, true);
+            lv[l].b32 = alloc<float>(nl, true);
+            lv[l].d32 = alloc<float>(nl, true);
+        }
+        const int64_t n = nDof(), nk = n * k;
+        x = alloc<double>(nk, true);
+        r = alloc<double>(nk, true);
+        p = alloc<double>(nk, true);
+        q = alloc<double>(nk, true);
+        f = alloc<double>(nk, true);
+        u = alloc<double>(nk, true);
+        p32 = alloc<float>(nk, true);
+        x32 = alloc<float>(nk, true);
+        q32 = alloc<float>(nk, true);
+        st = alloc<CgState>(k, true);
+        {
+            // per right-hand side: 4 partials per reduction block and the operator's fused dots (2 per
+            // tile of 32 x 8 nodes x the shortest z-chunk, 2 planes)
+            const Level& L0 = H.levels[0];
+            const int64_t tiles = int64_t((L0.NX + 31) / 32) * ((L0.NY + 7) / 8) * ((L0.NZ + 1) / 2);
+            red.stride = (std::max<int64_t>(4 * (n / RED_TILE + 2), 2 * tiles + 16) + 31) / 32 * 32;
+            red.partials = alloc<double>(red.stride * k, true);
+        }
+        if (stHost) HIPCHK(hipHostFree(stHost));
+        stHost = nullptr;
+        HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&stHost), sizeof(CgState) * size_t(k), hipHostMallocDefault));
+        std::memset(stHost, 0, sizeof(CgState) * size_t(k));
+        cap = k;
         sync();
     }
+    int64_t totalBytes() const { return bytes + workBytes; }
 
     // ----- operator dispatch: the z-marching kernel on levels without own matrices -----
     bool useFine = [] {
@@ -225,17 +258,39 @@ public:
     // elements in child-scale form where the level has it (exact Galerkin products; the FP64 and
     // mixed solves); OF_MFMA = the BF16 matrix-core operator (FP32 vectors) where the level allows it
     template <class T>
-    void op(size_t l, int mode, const T* xx, T* yy, const T* rr, const T* dd, double c1, double c2, const int* stop, int form = 0) {
+    // fd: a conjugate-gradient dot product over the level's DOFs (FD_PQ: x.y, FD_RZ: r.y and y.q),
+    // fused into the operator kernel where it can be, else run after it; either way finished into
+    // the CG state
+    void op(size_t l, int mode, const T* xx, T* yy, const T* rr, const T* dd, double c1, double c2, const int* stop, int form = 0,
+            const DotFusion* fd = nullptr) {
         const DevLevel& D = lv[l].dl;
+        bool done = false;
         if constexpr (sizeof(T) == 4) {
-            if ((form & OF_MFMA) && lv[l].aop) {
+            if ((form & OF_MFMA) && lv[l].aop && ni == 1) {
                 launchFineMfma(D, lv[l].aop, mode, xx, yy, rr, dd, c1, c2, stop, stream);
-                return;
+                done = true;
             }
         }
-        if (!useFine) launchApply<T>(D, mode, xx, yy, rr, dd, c1, c2, stop, stream);
-        else if (D.symOk && (H.levels[l].nOwn == 0 || ((form & OF_CHILD) && D.childForm))) launchFine<T>(D, mode, xx, yy, rr, dd, c1, c2, stop, stream);
-        else launchApplyOwn<T>(D, mode, xx, yy, rr, dd, c1, c2, stop, stream);
+        if (!done) {
+            if (!useFine) launchApply<T>(D, mode, xx, yy, rr, dd, c1, c2, stop, stream, ni);
+            else if (D.symOk && (H.levels[l].nOwn == 0 || ((form & OF_CHILD) && D.childForm))) {
+                if (fd) {
+                    const unsigned nb = launchFineDot<T>(D, mode, xx, yy, rr, dd, c1, c2, stop, *fd, stream, ni);
+                    if (nb) {
+                        if (fd->kind == FD_PQ) launchFinishPQ(fd->red, nb, st, stream, ni);
+                        else launchFinishRZ(fd->red, nb, st, stream, ni);
+                        return;
+                    }
+                } else {
+                    launchFine<T>(D, mode, xx, yy, rr, dd, c1, c2, stop, stream, ni);
+                }
+            } else launchApplyOwn<T>(D, mode, xx, yy, rr, dd, c1, c2, stop, stream, ni);
+        }
+        if (fd) dotAfter<T>(*fd, D.nDof, xx, yy, rr);
+    }
+    template <class T> void dotAfter(const DotFusion& fd, int64_t n, const T* xx, const T* yy, const T* rr) {
+        if (fd.kind == FD_PQ) launchDotPQ<T>(xx, yy, n, st, red, stream, ni);
+        else launchDotsRZ<T>(rr, yy, static_cast<const T*>(fd.q), n, st, red, stream, ni);
     }
 
     // ----- multigrid -----
@@ -243,6 +298,8 @@ public:
     // largest eigenvalue of D^-1 K per level: 10 Lanczos steps on the device from the reference's
     // LCG start vector (voxel_fea.cpp:579-610)
     void estimateSmoothers() {
+        const int saved = ni;
+        ni = 1;
         for (size_t l = 0; l + 1 < lv.size(); l++) {
             Level& L = H.levels[l];
             if (L.lmax > 0) continue;
@@ -275,6 +332,7 @@ public:
             HIPCHK(hipMemsetAsync(xx, 0, size_t(n) * 8, stream));
             HIPCHK(hipMemsetAsync(vPrev, 0, size_t(n) * 8, stream));
         }
+        ni = saved;
         smoothersReady = true;
     }
 
@@ -287,11 +345,13 @@ public:
     }
 
     // one V-cycle from rIn on level l (voxel_fea.cpp:817-828); returns the buffer holding z
-    template <class T> T* vcycle(size_t l, const T* rIn, const int* stop, int form = 0, int degree = 2) {
+    // rz: the conjugate gradients' r.z and z.q (q = the previous K p), fused into the last step
+    template <class T> T* vcycle(size_t l, const T* rIn, const int* stop, int form = 0, int degree = 2, const DotFusion* rz = nullptr) {
         Vecs<T> V = vecs<T>(l);
         const DevLevel& D = lv[l].dl;
         if (l + 1 == lv.size()) {
-            launchCoarse<T>(freeIdx, denseIdx, H.m, D.nDof, Ainv, rIn, V.b, stop, stream);
+            launchCoarse<T>(freeIdx, denseIdx, H.m, D.nDof, Ainv, rIn, V.b, stop, stream, ni);
+            if (rz) dotAfter<T>(*rz, D.nDof, nullptr, V.b, rIn);
             return V.b;
         }
         const Cheb c = chebyshev(H.levels[l].lmax, degree);
@@ -305,145 +365,200 @@ public:
                 if (b != z && b != zp) return b;
             return buf[2];
         };
-        auto step = [&](int mode, double c1, double c2) {
+        auto step = [&](int mode, double c1, double c2, const DotFusion* fd = nullptr) {
             T* y = freeBuf();
-            op<T>(l, mode, z, y, rIn, zp, c1, c2, stop, form);
+            op<T>(l, mode, z, y, rIn, zp, c1, c2, stop, form, fd);
             zp = z;
             z = y;
         };
-        launchChebFirst<T>(D, rIn, z, c.c0, stop, stream);
+        launchChebFirst<T>(D, rIn, z, c.c0, stop, stream, ni);
         for (int j = 1; j < degree; j++) step(AM_CHEB2, c.c1[j], c.c2[j]);
         zp = nullptr;
         T* t = freeBuf();
         op<T>(l, AM_RESID, z, t, rIn, nullptr, 0, 0, stop, form);
         Vecs<T> W = vecs<T>(l + 1);
-        launchRestrict<T>(D, C, t, W.r, stop, stream);
+        launchRestrict<T>(D, C, t, W.r, stop, stream, ni);
         const T* zc = vcycle<T>(l + 1, W.r, stop, form, degree);
-        launchProlong<T>(D, C, zc, z, stop, stream);
-        step(AM_CHEB1, c.c0, 0);
-        for (int j = 1; j < degree; j++) step(AM_CHEB2, c.c1[j], c.c2[j]);
+        launchProlong<T>(D, C, zc, z, stop, stream, ni);
+        step(AM_CHEB1, c.c0, 0, degree == 1 ? rz : nullptr);
+        for (int j = 1; j < degree; j++) step(AM_CHEB2, c.c1[j], c.c2[j], j + 1 == degree ? rz : nullptr);
         return z;
     }
 
     // z = M r on level 0 with the FP32 cycle and r scaled to order one (voxel_fea.cpp:835-861)
     void precondition32(const double* dr, double* dz, const int* stop, int degree = 2) {
         const int64_t n = nDof();
-        launchMaxAbs64(dr, n, &st->rmax, red, stream);
-        launchToF32(dr, lv[0].r32, n, &st->rmax, stream);
+        launchMaxAbs64(dr, n, &st->rmax, red, stream, ni);
+        launchToF32(dr, lv[0].r32, n, &st->rmax, stream, ni);
         const float* z32 = vcycle<float>(0, lv[0].r32, stop, 0, degree);
-        launchFromF32(z32, dz, n, &st->rmax, stream);
+        launchFromF32(z32, dz, n, &st->rmax, stream, ni);
     }
 
     // ----- conjugate gradients -----
 
     void readState() {
-        HIPCHK(hipMemcpyAsync(stHost, st, sizeof(CgState), hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipMemcpyAsync(stHost, st, sizeof(CgState) * size_t(ni), hipMemcpyDeviceToHost, stream));
         sync();
     }
 
-    void solveDevice(const double* df, const double* dx0, const psx_solve_opts& o, double* du, psx_solve_info& info) {
+    void solveDevice(const double* df, const double* dx0, const psx_solve_opts& o, double* du, psx_solve_info& info) { solveMulti(1, df, dx0, o, du, &info); }
+
+    // k independent right-hand sides in one set of launches: df, du (and dx0 when not null) hold k
+    // vectors of nDof, instance-major; every kernel runs all k (blockIdx.y) and each instance stops
+    // on its own state, so the result of each right-hand side is the one a lone solve gives
+    void solveMulti(int k, const double* df, const double* dx0, const psx_solve_opts& o, double* du, psx_solve_info* info) {
         const auto t0 = std::chrono::steady_clock::now();
+        if (k < 1) throw std::invalid_argument("no right-hand sides");
         if (!smoothersReady) estimateSmoothers();
+        reserve(k);
+        ni = k;
+        struct Restore {
+            int& n;
+            ~Restore() { n = 1; }
+        } restore{ni};
         const int64_t n = nDof();
+        const size_t nk = size_t(n) * size_t(k);
         const DevLevel& D0 = lv[0].dl;
-        info = psx_solve_info{};
+        for (int i = 0; i < k; i++) info[i] = psx_solve_info{};
         // x = x0 on free DOFs; bnorm = |f_free|; r = f - K x (FP64)
-        launchMaskCopy64(dx0, D0.invD64, du, n, stream);
-        launchNormFree64(df, D0.invD64, n, &st->bnorm, red, stream);
+        launchMaskCopy64(dx0, D0.invD64, du, n, stream, k);
+        launchNormFree64(df, D0.invD64, n, &st->bnorm, red, stream, k);
         op<double>(0, AM_APPLY, du, q, nullptr, nullptr, 0, 0, nullptr);
-        launchResidual64(df, q, D0.invD64, r, n, &st->rr, red, stream);
+        launchResidual64(df, q, D0.invD64, r, n, &st->rr, red, stream, k);
         readState();
-        const double bnorm = std::sqrt(stHost->bnorm);
-        if (bnorm == 0) {
-            HIPCHK(hipMemsetAsync(du, 0, size_t(n) * 8, stream));
+        std::vector<double> bnorm(k), rel(k), rmax(k, 1.0);
+        bool anyLoad = false;
+        for (int i = 0; i < k; i++) {
+            bnorm[i] = std::sqrt(stHost[i].bnorm);
+            rel[i] = bnorm[i] > 0 ? std::sqrt(stHost[i].rr) / bnorm[i] : 0;
+            anyLoad = anyLoad || bnorm[i] > 0;
+            // no load on the free DOFs: u = 0 (voxel_fea.cpp:872-876)
+            if (bnorm[i] == 0) HIPCHK(hipMemsetAsync(du + size_t(i) * n, 0, size_t(n) * 8, stream));
+        }
+        if (!anyLoad) {
             sync();
-            info.converged = 1;
-            info.device_ms = ms(t0);
+            for (int i = 0; i < k; i++) {
+                info[i].converged = 1;
+                info[i].device_ms = ms(t0);
+            }
             return;
         }
-        double rel = std::sqrt(stHost->rr) / bnorm;
         const int prec = o.precision;
         const int batch = o.check_every > 0 ? o.check_every : 8;
         // the reference's algorithm keeps its operator; the FP64 and mixed solves use the exact
-        // child-scale form, and the mixed smoother the matrix cores when asked
+        // child-scale form, and the mixed smoother the matrix cores when asked (one right-hand side)
         const int form = prec == PSX_PREC_REF ? 0 : (OF_CHILD | (prec == PSX_PREC_MIXED && o.mfma_smoother ? OF_MFMA : 0));
         const int degree = o.cheb_degree > 0 ? o.cheb_degree : 2;
         if (degree > CHEB_MAX) throw std::invalid_argument("Chebyshev degree above 6");
-        double rmax = 1;
         if (prec == PSX_PREC_MIXED) {
-            // FP32 units for the whole solve: the first residual scaled to order one
-            launchMaxAbs64(r, n, &st->rmax, red, stream);
+            // FP32 units for the whole solve: each first residual scaled to order one
+            launchMaxAbs64(r, n, &st->rmax, red, stream, k);
             readState();
-            rmax = stHost->rmax > 0 ? stHost->rmax : 1;
+            for (int i = 0; i < k; i++) rmax[i] = stHost[i].rmax > 0 ? stHost[i].rmax : 1;
         }
-        CgState s0{};
-        s0.rz = 1;
-        s0.bnorm = bnorm;
-        s0.tol = o.tol;
-        s0.maxIter = o.max_iter;
-        s0.scale = prec == PSX_PREC_MIXED ? rmax : 1;
-        s0.rmax = rmax;
-        s0.rel = rel;
-        s0.trueRel = rel;
-        s0.peak = rel;
-        s0.noFold = 1;
-        s0.foldEvery = o.fold_every > 0 ? o.fold_every : 20;
-        s0.stop = (rel <= o.tol || o.max_iter <= 0) ? 1 : 0;
-        HIPCHK(hipMemcpyAsync(st, &s0, sizeof s0, hipMemcpyHostToDevice, stream));
+        std::vector<CgState> s0(k);
+        bool allStopped = true;
+        for (int i = 0; i < k; i++) {
+            CgState& c = s0[i];
+            c = CgState{};
+            c.rz = 1;
+            c.bnorm = bnorm[i];
+            c.tol = o.tol;
+            c.maxIter = o.max_iter;
+            c.scale = prec == PSX_PREC_MIXED ? rmax[i] : 1;
+            c.rmax = rmax[i];
+            c.rel = rel[i];
+            c.trueRel = rel[i];
+            c.peak = rel[i];
+            c.noFold = 1;
+            c.foldEvery = o.fold_every > 0 ? o.fold_every : 20;
+            c.stop = (bnorm[i] == 0 || rel[i] <= o.tol || o.max_iter <= 0) ? 1 : 0;
+            allStopped = allStopped && c.stop;
+        }
+        HIPCHK(hipMemcpyAsync(st, s0.data(), sizeof(CgState) * size_t(k), hipMemcpyHostToDevice, stream));
         int* stop = &st->stop;
         float* r32 = lv[0].r32;
         if (prec == PSX_PREC_MIXED) {
-            HIPCHK(hipMemsetAsync(p32, 0, size_t(n) * 4, stream));
-            HIPCHK(hipMemsetAsync(q32, 0, size_t(n) * 4, stream));
-            launchToF32(r, r32, n, &st->rmax, stream);
+            HIPCHK(hipMemsetAsync(p32, 0, nk * 4, stream));
+            HIPCHK(hipMemsetAsync(q32, 0, nk * 4, stream));
+            HIPCHK(hipMemsetAsync(x32, 0, nk * 4, stream));
+            launchToF32(r, r32, n, &st->rmax, stream, k);
         } else {
-            HIPCHK(hipMemsetAsync(p, 0, size_t(n) * 8, stream));
-            HIPCHK(hipMemsetAsync(q, 0, size_t(n) * 8, stream));
+            HIPCHK(hipMemsetAsync(p, 0, nk * 8, stream));
+            HIPCHK(hipMemsetAsync(q, 0, nk * 8, stream));
         }
-        while (!s0.stop) {
-            for (int k = 0; k < batch; k++) {
+        bool cancelled = false;
+        while (!allStopped) {
+            for (int b = 0; b < batch; b++) {
                 if (prec == PSX_PREC_MIXED) {
-                    const float* z = vcycle<float>(0, r32, stop, form, degree);
-                    launchDotsRZ<float>(r32, z, q32, n, st, red, stream);
-                    launchUpdateP<float>(z, p32, n, st, stream);
-                    op<float>(0, AM_APPLY, p32, q32, nullptr, nullptr, 0, 0, stop);
-                    launchDotPQ<float>(p32, q32, n, st, red, stream);
-                    launchUpdateXRMixed(du, p32, r, r32, q32, n, st, red, stream);
-                    // reliable update: the FP64 true residual replaces the recurrence's
+                    const DotFusion fz{FD_RZ, q32, red}, fp{FD_PQ, nullptr, red};
+                    const float* z = vcycle<float>(0, r32, stop, form, degree, &fz);
+                    launchUpdateP<float>(z, p32, n, st, stream, k);
+                    // the outer operator stays FP32 (the matrix-core form is for the smoother only)
+                    op<float>(0, AM_APPLY, p32, q32, nullptr, nullptr, 0, 0, stop, form & ~OF_MFMA, &fp);
+                    launchUpdateXR32Fold(x32, p32, r32, q32, n, st, red, stream, k);
+                    // reliable update: the correction joins the FP64 solution and the FP64 true
+                    // residual replaces the recurrence's
+                    launchFoldX(du, x32, n, st, stream, k);
                     op<double>(0, AM_APPLY, du, q, nullptr, nullptr, 0, 0, &st->noFold);
-                    launchFoldR(df, q, D0.invD64, r, r32, n, st, red, stream);
+                    launchFoldR(df, q, D0.invD64, r, r32, n, st, red, stream, k);
                     continue;
                 }
                 const double* z;
-                if (prec == PSX_PREC_FP64) z = vcycle<double>(0, r, stop, form, degree);
+                const DotFusion fz{FD_RZ, q, red}, fp{FD_PQ, nullptr, red};
+                if (prec == PSX_PREC_FP64) z = vcycle<double>(0, r, stop, form, degree, &fz);
                 else {
                     precondition32(r, lv[0].b64, stop, degree);
                     z = lv[0].b64;
+                    launchDotsRZ<double>(r, z, q, n, st, red, stream, k);
                 }
-                launchDotsRZ<double>(r, z, q, n, st, red, stream);
-                launchUpdateP<double>(z, p, n, st, stream);
-                op<double>(0, AM_APPLY, p, q, nullptr, nullptr, 0, 0, stop);
-                launchDotPQ<double>(p, q, n, st, red, stream);
-                launchUpdateXR<double>(du, p, r, q, n, st, red, stream);
+                launchUpdateP<double>(z, p, n, st, stream, k);
+                op<double>(0, AM_APPLY, p, q, nullptr, nullptr, 0, 0, stop, form, &fp);
+                launchUpdateXR<double>(du, p, r, q, n, st, red, stream, k);
             }
             readState();
-            if (stHost->stop) break;
-            if (o.on_progress && o.on_progress(stHost->iters, stHost->rel, o.user)) {
-                info.cancelled = 1;
+            allStopped = true;
+            int itMax = 0;
+            double relMax = 0;
+            for (int i = 0; i < k; i++) {
+                allStopped = allStopped && stHost[i].stop;
+                itMax = std::max(itMax, stHost[i].iters);
+                relMax = std::max(relMax, stHost[i].rel);
+            }
+            if (allStopped) break;
+            if (o.on_progress && o.on_progress(itMax, relMax, o.user)) {
+                cancelled = true;
                 break;
             }
         }
+        if (prec == PSX_PREC_MIXED) {
+            // a cancelled run still folds in its last correction
+            for (int i = 0; i < k; i++) s0[i].noFold = 0;
+            for (int i = 0; i < k; i++)
+                HIPCHK(hipMemcpyAsync(&st[i].noFold, &s0[i].noFold, sizeof(int), hipMemcpyHostToDevice, stream));
+            launchFoldX(du, x32, n, st, stream, k);
+        }
         readState();
-        info.iterations = stHost->iters;
-        info.broken = stHost->broken;
-        info.folds = stHost->folds;
+        for (int i = 0; i < k; i++) {
+            info[i].iterations = stHost[i].iters;
+            info[i].broken = stHost[i].broken;
+            info[i].folds = stHost[i].folds;
+        }
         // the final true residual against the equilibrium equations (voxel_fea.cpp:920-925)
         op<double>(0, AM_APPLY, du, q, nullptr, nullptr, 0, 0, nullptr);
-        launchResidual64(df, q, D0.invD64, r, n, &st->rr, red, stream);
+        launchResidual64(df, q, D0.invD64, r, n, &st->rr, red, stream, k);
         readState();
-        info.residual = std::sqrt(stHost->rr) / bnorm;
-        info.converged = !info.cancelled && std::isfinite(info.residual) && info.residual <= o.tol * 10;
-        info.device_ms = ms(t0);
+        for (int i = 0; i < k; i++) {
+            if (bnorm[i] == 0) {
+                info[i].converged = 1;
+                info[i].device_ms = ms(t0);
+                continue;
+            }
+            info[i].cancelled = cancelled;
+            info[i].residual = std::sqrt(stHost[i].rr) / bnorm[i];
+            info[i].converged = !cancelled && std::isfinite(info[i].residual) && info[i].residual <= o.tol * 10;
+            info[i].device_ms = ms(t0);
+        }
     }
 
     static double ms(std::chrono::steady_clock::time_point t0) {
@@ -451,7 +566,8 @@ public:
     }
 
 private:
-    std::vector<void*> allocs;
+    std::vector<void*> allocs, work;
+    int64_t workBytes = 0;
 };
 
 }  // namespace psx

@@ -50,17 +50,19 @@ struct Request {
     Clock::time_point submitted, admitted, built, finished;
 };
 
-// device bytes of a part before it is built: level-0 vectors and masks, the coarser levels' share,
-// the request's own f/u/x0 buffers, and room for own element matrices
+// device bytes of a part before it is built: masks and coefficients (about 40 B per DOF over the
+// hierarchy), per load case the work vectors of every level and the request's own f/u/x0 buffers
+// (about 144 B per DOF), and room for own element matrices
 int64_t footprint(const Request& r) {
     const int64_t n = r.nDof;
-    return n * (160 + 24) + 64 * (int64_t(1) << 20);
+    return n * (40 + 144 * int64_t(r.nrhs)) + 64 * (int64_t(1) << 20);
 }
 
 struct Resident {
     std::unique_ptr<psx::Model> model;
     bool busy = false;
     Clock::time_point used;
+    int64_t bytes = 0;  // what `held` counts for it
 };
 
 }  // namespace
@@ -80,7 +82,7 @@ struct psx_server_s {
     int64_t evictable() const {
         int64_t b = 0;
         for (const auto& [id, p] : parts)
-            if (!p.busy) b += p.model->bytes;
+            if (!p.busy) b += p.bytes;
         return b;
     }
     // evict idle resident parts, least recently used first, until need fits (lock held)
@@ -90,7 +92,7 @@ struct psx_server_s {
             for (auto it = parts.begin(); it != parts.end(); ++it)
                 if (!it->second.busy && (victim == parts.end() || it->second.used < victim->second.used)) victim = it;
             if (victim == parts.end()) return;
-            held -= victim->second.model->bytes;
+            held -= victim->second.bytes;
             parts.erase(victim);
         }
     }
@@ -119,7 +121,7 @@ struct psx_server_s {
                 if (it != parts.end() && !it->second.busy) {
                     it->second.busy = true;
                     reused = true;
-                    r->estimate = std::max<int64_t>(0, r->estimate - it->second.model->bytes);
+                    r->estimate = std::max<int64_t>(0, r->estimate - it->second.bytes);
                 }
             }
             makeRoom(r->estimate);
@@ -167,10 +169,12 @@ struct psx_server_s {
             r.built = Clock::now();
             r.res.build_ms = reused ? 0 : msBetween(r.admitted, r.built);
             r.res.reused_part = reused ? 1 : 0;
-            r.res.device_bytes = M->bytes;
+            r.res.device_bytes = M->totalBytes();
             const int64_t n = M->nDof();
             if (n != r.nDof) throw std::invalid_argument("part does not match its load vectors");
-            // the request's vectors: f, u and x0 in HBM for the whole request
+            // the request's vectors: f, u and x0 (nrhs x n) in HBM for the whole request; all load
+            // cases run in one multi-instance solve
+            const size_t nk = size_t(n) * size_t(r.nrhs);
             struct Buffers {
                 hipStream_t s;
                 double* p[3] = {nullptr, nullptr, nullptr};
@@ -180,24 +184,28 @@ struct psx_server_s {
                     (void)hipStreamSynchronize(s);
                 }
             } buf{M->stream};
-            HIPCHK(hipMallocAsync(reinterpret_cast<void**>(&buf.p[0]), size_t(n) * 8, M->stream));
-            HIPCHK(hipMallocAsync(reinterpret_cast<void**>(&buf.p[1]), size_t(n) * 8, M->stream));
-            if (!r.x0.empty()) HIPCHK(hipMallocAsync(reinterpret_cast<void**>(&buf.p[2]), size_t(n) * 8, M->stream));
+            HIPCHK(hipMallocAsync(reinterpret_cast<void**>(&buf.p[0]), nk * 8, M->stream));
+            HIPCHK(hipMallocAsync(reinterpret_cast<void**>(&buf.p[1]), nk * 8, M->stream));
+            if (!r.x0.empty()) HIPCHK(hipMallocAsync(reinterpret_cast<void**>(&buf.p[2]), nk * 8, M->stream));
             double *df = buf.p[0], *du = buf.p[1], *dx = buf.p[2];
-            r.u.assign(size_t(r.nrhs) * n, 0.0);
+            HIPCHK(hipMemcpyAsync(df, r.f.data(), nk * 8, hipMemcpyHostToDevice, M->stream));
+            if (dx) {
+                if (r.x0PerRhs) HIPCHK(hipMemcpyAsync(dx, r.x0.data(), nk * 8, hipMemcpyHostToDevice, M->stream));
+                else
+                    for (int k = 0; k < r.nrhs; k++)
+                        HIPCHK(hipMemcpyAsync(dx + size_t(k) * n, r.x0.data(), size_t(n) * 8, hipMemcpyHostToDevice, M->stream));
+            }
+            std::vector<psx_solve_info> info(size_t(r.nrhs));
+            M->solveMulti(r.nrhs, df, dx, r.opts, du, info.data());
+            r.u.assign(nk, 0.0);
+            HIPCHK(hipMemcpyAsync(r.u.data(), du, nk * 8, hipMemcpyDeviceToHost, M->stream));
+            M->sync();
             r.res.converged = 1;
-            for (int k = 0; k < r.nrhs; k++) {
-                HIPCHK(hipMemcpyAsync(df, r.f.data() + size_t(k) * n, size_t(n) * 8, hipMemcpyHostToDevice, M->stream));
-                if (dx && (k == 0 || r.x0PerRhs))
-                    HIPCHK(hipMemcpyAsync(dx, r.x0.data() + (r.x0PerRhs ? size_t(k) * n : 0), size_t(n) * 8, hipMemcpyHostToDevice, M->stream));
-                psx_solve_info info{};
-                M->solveDevice(df, dx, r.opts, du, info);
-                HIPCHK(hipMemcpyAsync(r.u.data() + size_t(k) * n, du, size_t(n) * 8, hipMemcpyDeviceToHost, M->stream));
-                M->sync();
-                r.res.iterations += info.iterations;
-                r.res.max_iterations = std::max(r.res.max_iterations, info.iterations);
-                r.res.residual = std::max(r.res.residual, info.residual);
-                r.res.converged = r.res.converged && info.converged;
+            for (const psx_solve_info& i : info) {
+                r.res.iterations += i.iterations;
+                r.res.max_iterations = std::max(r.res.max_iterations, i.iterations);
+                r.res.residual = std::max(r.res.residual, i.residual);
+                r.res.converged = r.res.converged && i.converged;
             }
             r.res.solve_ms = msBetween(r.built, Clock::now());
             return PSX_OK;
@@ -208,10 +216,15 @@ struct psx_server_s {
             auto& p = parts.at(r.partId);
             p.busy = false;
             p.used = Clock::now();
+            // more load cases than before grow the part's work vectors
+            const int64_t now = p.model->totalBytes();
+            held += now - p.bytes;
+            p.bytes = now;
         } else if (own && r.partId >= 0 && r.res.status == PSX_OK && parts.find(r.partId) == parts.end()) {
             // keep the part resident (its bytes count against the budget until evicted)
-            held += own->bytes;
-            parts[r.partId] = Resident{std::move(own), false, Clock::now()};
+            const int64_t b = own->totalBytes();
+            held += b;
+            parts[r.partId] = Resident{std::move(own), false, Clock::now(), b};
         }
     }
 };

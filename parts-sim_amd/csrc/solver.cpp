@@ -220,6 +220,36 @@ int psx_solve(psx_model m, const double* f, const double* x0, const psx_solve_op
     });
 }
 
+int psx_solve_multi_device(psx_model m, int nrhs, const double* d_f, const double* d_x0, const psx_solve_opts* opts, double* d_u,
+                           psx_solve_info* info) {
+    return guarded([&] {
+        if (!m || nrhs < 1 || !d_f || !d_u || !info) throw std::invalid_argument("bad arguments");
+        const psx_solve_opts o = defaults(opts);
+        m->m->solveMulti(nrhs, d_f, d_x0, o, d_u, info);
+        HIPCHK(hipGetLastError());
+        return PSX_OK;
+    });
+}
+
+int psx_solve_multi(psx_model m, int nrhs, const double* f, const double* x0, const psx_solve_opts* opts, double* u, psx_solve_info* info) {
+    return guarded([&] {
+        if (!m || nrhs < 1 || !f || !u || !info) throw std::invalid_argument("bad arguments");
+        psx::Model& M = *m->m;
+        const int64_t n = M.nDof();
+        const size_t nk = size_t(n) * size_t(nrhs);
+        checkFinite(f, int64_t(nk), "Forces");
+        if (x0) checkFinite(x0, int64_t(nk), "Displacements");
+        const psx_solve_opts o = defaults(opts);
+        M.reserve(nrhs);
+        HIPCHK(hipMemcpyAsync(M.f, f, nk * 8, hipMemcpyHostToDevice, M.stream));
+        if (x0) HIPCHK(hipMemcpyAsync(M.x, x0, nk * 8, hipMemcpyHostToDevice, M.stream));
+        M.solveMulti(nrhs, M.f, x0 ? M.x : nullptr, o, M.u, info);
+        HIPCHK(hipMemcpyAsync(u, M.u, nk * 8, hipMemcpyDeviceToHost, M.stream));
+        M.sync();
+        return PSX_OK;
+    });
+}
+
 int psx_reactions(psx_model m, const double* u, const double* f, double out[3]) {
     return guarded([&] {
         if (!m || !u || !out) throw std::invalid_argument("bad arguments");
@@ -353,14 +383,14 @@ int psx_profile(psx_model m, int reps, char* buf, size_t len) {
                       "V-cycle FP64 %.4f ms  FP32 %.4f ms  FP32 degree-3 %.4f ms  FP32+BF16 matrix cores %.4f ms  FP32 stored own matrices %.4f ms\n",
                       tv64, tv32, tv3, tvbf, tvref);
         out += line;
-        hipEventDestroy(e0);
-        hipEventDestroy(e1);
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
         std::snprintf(buf, len, "%s", out.c_str());
         return PSX_OK;
     });
 }
 
-int64_t psx_device_bytes(psx_model m) { return m ? m->m->bytes : 0; }
+int64_t psx_device_bytes(psx_model m) { return m ? m->m->totalBytes() : 0; }
 
 }  // extern "C"
 

@@ -1,0 +1,71 @@
+"""Multi-instance solves (include/psx.h psx_solve_multi): several load cases of one part in one set
+of launches per V-cycle level. Each load case must give, bit for bit, the solution a lone solve gives
+(the instance's tiles, reductions and stop test are those of a single launch), and the lone solves
+are pinned to the oracle elsewhere; here the SIMP-block shape (every voxel its own stiffness,
+point loads) is also checked against the oracle's FP64 solve directly."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _model(P, w, coarsest=300):
+    return P.Model(w["dims"], w["density"], w["nu"], w["bc"], coarsest)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed", "ref"])
+def test_multi_equals_lone_solves(P, W, precision):
+    w = W.simp_block((24, 12, 12), seed=3, nrhs=5)
+    F = w["f"]
+    m = _model(P, w)
+    lone = [m.solve(F[k], tol=1e-9, precision=precision) for k in range(F.shape[0])]
+    U, infos = m.solve_multi(F, tol=1e-9, precision=precision)
+    for k in range(F.shape[0]):
+        print(precision, k, infos[k], flush=True)
+        assert infos[k]["converged"]
+        assert infos[k]["iterations"] == lone[k]["iterations"]
+        assert np.array_equal(U[k], lone[k]["u"])
+    m.close()
+
+
+def test_multi_zero_load_and_warm_start(P, W):
+    w = W.simp_block((20, 10, 10), seed=5, nrhs=3)
+    F = w["f"].copy()
+    F[1] = 0.0  # no load: u = 0 at once
+    m = _model(P, w)
+    U, infos = m.solve_multi(F, tol=1e-8, precision="mixed")
+    assert infos[1]["converged"] and infos[1]["iterations"] == 0 and not U[1].any()
+    assert infos[0]["converged"] and infos[2]["converged"]
+    # warm start from the solutions: every case is converged before the first iteration
+    U2, infos2 = m.solve_multi(F, tol=1e-6, X0=U, precision="mixed")
+    assert all(i["iterations"] == 0 and i["converged"] for i in infos2)
+    # a smaller batch after a larger one reuses the grown work vectors
+    U3, infos3 = m.solve_multi(F[2:3], tol=1e-8, precision="mixed")
+    assert np.array_equal(U3[0], U[2])
+    m.close()
+
+
+def test_multi_matches_oracle(P, W, cpu):
+    w = W.simp_block((16, 8, 8), seed=7, nrhs=3)
+    m = _model(P, w, 200)
+    U, infos = m.solve_multi(w["f"], tol=1e-10, precision="mixed")
+    ref = cpu.Model(w["dims"], w["density"], w["nu"], w["bc"], 200)
+    for k in range(3):
+        r = ref.solve(w["f"][k], tol=1e-12)
+        err = np.linalg.norm(U[k] - r["u"]) / np.linalg.norm(r["u"])
+        print(k, infos[k]["iterations"], r["iterations"], err, flush=True)
+        assert err < 1e-8
+    m.close()
+
+
+def test_multi_device_vectors(P, W):
+    w = W.simp_block((20, 10, 10), seed=9, nrhs=4)
+    F = w["f"]
+    m = _model(P, w)
+    U, _ = m.solve_multi(F, tol=1e-9, precision="mixed")
+    d_f = P.DeviceArray(F.size, F.reshape(-1))
+    d_u = P.DeviceArray(F.size)
+    infos = m.solve_multi_device(4, d_f, d_u, tol=1e-9, precision="mixed")
+    assert all(i["converged"] for i in infos)
+    assert np.array_equal(d_u.download().reshape(F.shape), U)
+    m.close()
