@@ -76,7 +76,10 @@ static_assert(sizeof(CgState) % sizeof(double) == 0, "CgState must tile in doubl
 
 enum ApplyMode { AM_APPLY = 0, AM_RAW = 1, AM_RESID = 2, AM_CHEB1 = 3, AM_CHEB2 = 4 };
 // operator forms (bit flags): own elements in child-scale form; BF16 matrix-core operator
-enum OpForm { OF_CHILD = 1, OF_MFMA = 2 };
+// OF_CHILD: every level that has the child-scale form uses it; OF_AUTO: only levels where it is the
+// cheaper form (most own elements two-valued; a level of general own elements streams its stored
+// matrices faster than it evaluates 8 children per element)
+enum OpForm { OF_CHILD = 1, OF_MFMA = 2, OF_AUTO = 4 };
 
 // y = K x on a level with the epilogue of `mode` (T = double or float):
 //   AM_APPLY  y = K x, held DOFs 0               AM_RAW   y = K x

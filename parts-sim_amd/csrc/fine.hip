@@ -93,10 +93,12 @@ template <class T, bool OWNC> struct Ring {
 
 // OWNC: the level has own elements in child-scale form (sum_c s_c M_c, crow = the corner-0 rows
 // of the 8 M_c, [c][row d][24]); an own voxel is always a differing voxel of its nodes
-template <class T, int MODE, bool OWNC>
-__global__ __launch_bounds__(NT) void kFine(DevLevel L, Sym<T> C, const T* __restrict__ crow, const T* __restrict__ ctab, const T* __restrict__ x, T* __restrict__ y,
-                                            const T* __restrict__ r, const T* __restrict__ zp, T c1, T c2, const int* stop, int zChunk, int gx, int gy,
-                                            int nTiles, int xcdOrder, double* __restrict__ dots, int64_t dotStride, const T* __restrict__ qdot) {
+#define PSX_FINE_PARAMS                                                                                                                    \
+    DevLevel L, Sym<T> C, const T *__restrict__ crow, const T *__restrict__ ctab, const T *__restrict__ x, T *__restrict__ y,                 \
+        const T *__restrict__ r, const T *__restrict__ zp, T c1, T c2, const int *stop, int zChunk, int gx, int gy, int nTiles, int xcdOrder, \
+        double *__restrict__ dots, int64_t dotStride, const T *__restrict__ qdot
+// DOT: the fused dot products are accumulated (dots != nullptr)
+template <class T, int MODE, bool OWNC, bool DOT> __device__ __forceinline__ void fineBody(PSX_FINE_PARAMS) {
     // multi-instance launches: blockIdx.y is the right-hand side (its vectors, stop flag, partials)
     if (stop) stop += int64_t(blockIdx.y) * CG_INTS;
     if (stop && __builtin_nontemporal_load(stop) != 0) return;
@@ -107,7 +109,7 @@ __global__ __launch_bounds__(NT) void kFine(DevLevel L, Sym<T> C, const T* __res
         if (r) r += io;
         if (zp) zp += io;
         if (qdot) qdot += io;
-        if (dots) dots += int64_t(blockIdx.y) * dotStride;
+        if (DOT) dots += int64_t(blockIdx.y) * dotStride;
     }
     // XCD-aware tile order: the hardware deals consecutive blocks round-robin to the 8 XCDs, so
     // block b takes tile (b % 8) * per + b / 8 and each XCD walks one contiguous run of tiles (x
@@ -126,22 +128,24 @@ __global__ __launch_bounds__(NT) void kFine(DevLevel L, Sym<T> C, const T* __res
     constexpr bool NEEDS_D = MODE == AM_CHEB2;
 
     constexpr int NPL = (PX * PY + NT - 1) / NT, NVL = (VX * VY + NT - 1) / NT;
-    // this thread's share of a node plane / voxel plane: x-y bounds and offsets fixed for the march
+    // this thread's share of a node plane / voxel plane: x-y bounds and offsets fixed for the march.
+    // Offsets within a plane are 32-bit (a plane holds < 2^31 / 3 nodes) and the plane's base is
+    // block-uniform, so loads and stores take a scalar base plus a 32-bit vector offset.
     bool pin[NPL], vin[NVL];
-    int64_t poff[NPL], voff[NVL];
+    int32_t poff[NPL], voff[NVL];
 #pragma unroll
     for (int q = 0; q < NPL; q++) {
         const int e = tid + q * NT;
         const int gi = i0 - 1 + e % PX, gj = j0 - 1 + e / PX;
         pin[q] = e < PX * PY && gi >= 0 && gj >= 0 && gi < L.NX && gj < L.NY;
-        poff[q] = pin[q] ? 3 * (gi + NX * gj) : 0;
+        poff[q] = pin[q] ? int32_t(3 * (gi + NX * gj)) : 0;
     }
 #pragma unroll
     for (int q = 0; q < NVL; q++) {
         const int e = tid + q * NT;
         const int vi = i0 - 1 + e % VX, vj = j0 - 1 + e / VX;
         vin[q] = e < VX * VY && vi >= 0 && vj >= 0 && vi < L.nx && vj < L.ny;
-        voff[q] = vin[q] ? vi + int64_t(L.nx) * vj : 0;
+        voff[q] = vin[q] ? int32_t(vi + int64_t(L.nx) * vj) : 0;
     }
     const int64_t nxy = int64_t(L.nx) * L.ny;
     auto fetchPlane = [&](int z, T (&px)[NPL][3]) {
@@ -195,18 +199,19 @@ __global__ __launch_bounds__(NT) void kFine(DevLevel L, Sym<T> C, const T* __res
         T inv[3], r[3], d[3], qd[3];
         uint32_t info;
     };
-    const int64_t col = mine ? i + NX * j : 0;
+    const int32_t col = mine ? int32_t(i + NX * j) : 0;
     auto fetchOwn = [&](int z, Own& o) {
-        const bool in = mine && z >= z0 && z < z1;
-        const int64_t nn = col + NXY * (in ? z : 0);
+        const bool zin = z >= z0 && z < z1, in = mine && zin;
+        const int64_t pb = NXY * (zin ? z : z0);  // block-uniform plane base
+        const int32_t c3 = 3 * col;
 #pragma unroll
         for (int d = 0; d < 3; d++) {
-            if constexpr (NEEDS_INV && OWNC) o.inv[d] = in ? invD[3 * nn + d] : T(0);
-            if constexpr (NEEDS_R) o.r[d] = in ? r[3 * nn + d] : T(0);
-            if constexpr (NEEDS_D) o.d[d] = in && zp ? zp[3 * nn + d] : T(0);
-            if constexpr (NEEDS_INV) o.qd[d] = in && qdot ? qdot[3 * nn + d] : T(0);
+            if constexpr (NEEDS_INV && OWNC) o.inv[d] = in ? (invD + 3 * pb)[c3 + d] : T(0);
+            if constexpr (NEEDS_R) o.r[d] = in ? (r + 3 * pb)[c3 + d] : T(0);
+            if constexpr (NEEDS_D) o.d[d] = in && zp ? (zp + 3 * pb)[c3 + d] : T(0);
+            if constexpr (NEEDS_INV) o.qd[d] = in && qdot ? (qdot + 3 * pb)[c3 + d] : T(0);
         }
-        o.info = in ? uint32_t(L.nodeInfo[nn]) : 0u;
+        o.info = in ? uint32_t((L.nodeInfo + pb)[col]) : 0u;
     };
 
     // prologue: node plane z0 - 1 in LDS; node plane z0 and voxel plane z0 - 2 in flight
@@ -222,7 +227,7 @@ __global__ __launch_bounds__(NT) void kFine(DevLevel L, Sym<T> C, const T* __res
     Own cur;
     cur.info = 0;
     T aN[3] = {0, 0, 0}, aC[3] = {0, 0, 0}, aP[3] = {0, 0, 0}, xcPrev[3] = {0, 0, 0}, xcPrev2[3] = {0, 0, 0};
-    double dot0 = 0, dot1 = 0;  // fused dot products (dots != nullptr)
+    double dot0 = 0, dot1 = 0;  // fused dot products (DOT)
     __syncthreads();
 
     for (int k = z0 - 1; k <= z1; k++) {
@@ -460,10 +465,11 @@ __global__ __launch_bounds__(NT) void kFine(DevLevel L, Sym<T> C, const T* __res
             }
             // epilogue for node k - 1 by the owning thread
             if (mine) {
-                const int64_t n = i + NX * j + NXY * (k - 1);
+                const int64_t pb3 = 3 * NXY * (k - 1);  // block-uniform plane base
+                const int32_t c3 = 3 * col;
                 if (L.diagAdd) {
 #pragma unroll
-                    for (int d = 0; d < 3; d++) out[d] += T(L.diagAdd[3 * n + d]) * xcPrev[d];
+                    for (int d = 0; d < 3; d++) out[d] += T((L.diagAdd + pb3)[c3 + d]) * xcPrev[d];
                 }
                 if constexpr (NEEDS_INV && !OWNC) {
                     // inverse diagonal from the voxel scales: every diagonal entry of Kb is the
@@ -471,15 +477,22 @@ __global__ __launch_bounds__(NT) void kFine(DevLevel L, Sym<T> C, const T* __res
                     T ssum = 0;
 #pragma unroll
                     for (int v = 0; v < 8; v++) ssum += T(sh.s[(v >> 2) ? v0 : vm][ty + ((v >> 1) & 1)][tx + (v & 1)]);
+                    if (L.diagAdd) {
 #pragma unroll
-                    for (int d = 0; d < 3; d++) {
-                        const T dg = C.v[0] * ssum + (L.diagAdd ? T(L.diagAdd[3 * n + d]) : T(0));
-                        cur.inv[d] = ((cur.info >> (8 + d)) & 1u) ? T(0) : T(1) / dg;
+                        for (int d = 0; d < 3; d++) {
+                            const T dg = C.v[0] * ssum + T((L.diagAdd + pb3)[c3 + d]);
+                            cur.inv[d] = ((cur.info >> (8 + d)) & 1u) ? T(0) : T(1) / dg;
+                        }
+                    } else {
+                        // one division for the three components (the same diagonal entry)
+                        const T iv = T(1) / (C.v[0] * ssum);
+#pragma unroll
+                        for (int d = 0; d < 3; d++) cur.inv[d] = ((cur.info >> (8 + d)) & 1u) ? T(0) : iv;
                     }
                 }
 #pragma unroll
                 for (int d = 0; d < 3; d++) {
-                    const int64_t q = 3 * n + d;
+                    const int32_t q = c3 + d;
                     const bool held = (cur.info >> (8 + d)) & 1u;
                     T yv;
                     if constexpr (MODE == AM_APPLY) yv = held ? T(0) : out[d];
@@ -487,8 +500,8 @@ __global__ __launch_bounds__(NT) void kFine(DevLevel L, Sym<T> C, const T* __res
                     else if constexpr (MODE == AM_RESID) yv = held ? T(0) : cur.r[d] - out[d];
                     else if constexpr (MODE == AM_CHEB1) yv = xcPrev[d] + c1 * cur.inv[d] * (cur.r[d] - out[d]);
                     else yv = xcPrev[d] + c1 * (xcPrev[d] - cur.d[d]) + c2 * cur.inv[d] * (cur.r[d] - out[d]);
-                    y[q] = yv;
-                    if (dots) {
+                    (y + pb3)[q] = yv;
+                    if constexpr (DOT) {
                         if constexpr (MODE == AM_APPLY) dot0 += double(xcPrev[d]) * double(yv);
                         if constexpr (NEEDS_INV) {
                             dot0 += double(cur.r[d]) * double(yv);
@@ -509,7 +522,7 @@ __global__ __launch_bounds__(NT) void kFine(DevLevel L, Sym<T> C, const T* __res
         }
         cur = nxt;
     }
-    if (dots) {
+    if constexpr (DOT) {
         // block partials in a fixed order: wavefront butterflies, then the waves in order
         __shared__ double wsum[2][NW];
 #pragma unroll
@@ -535,6 +548,18 @@ __global__ __launch_bounds__(NT) void kFine(DevLevel L, Sym<T> C, const T* __res
     }
 }
 
+#define PSX_FINE_FWD L, C, crow, ctab, x, y, r, zp, c1, c2, stop, zChunk, gx, gy, nTiles, xcdOrder, dots, dotStride, qdot
+template <class T, int MODE, bool OWNC, bool DOT> __global__ __launch_bounds__(NT) void kFine(PSX_FINE_PARAMS) {
+    fineBody<T, MODE, OWNC, DOT>(PSX_FINE_FWD);
+}
+// FP32 without own elements: held to 128 VGPRs (4 waves per SIMD instead of 3), which it fits
+// without spilling
+template <class T, int MODE, bool DOT>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void kFine4(PSX_FINE_PARAMS) {
+    fineBody<T, MODE, false, DOT>(PSX_FINE_FWD);
+}
+#undef PSX_FINE_FWD
+
 }  // namespace
 
 template <class T, bool OWNC>
@@ -551,12 +576,42 @@ static void launchFineT(const DevLevel& L, int mode, const Sym<T>& C, const T* c
         return e && e[0] == '1' ? 1 : 0;
     }();
 #define PSX_FINE_ARGS L, C, crow, ctab, x, y, r, d, T(c1), T(c2), stop, zChunk, gx, gy, nTiles, xcdOrder, dots, dotStride, qdot
-    switch (mode) {
-        case AM_APPLY: kFine<T, AM_APPLY, OWNC><<<g, b, 0, s>>>(PSX_FINE_ARGS); break;
-        case AM_RAW: kFine<T, AM_RAW, OWNC><<<g, b, 0, s>>>(PSX_FINE_ARGS); break;
-        case AM_RESID: kFine<T, AM_RESID, OWNC><<<g, b, 0, s>>>(PSX_FINE_ARGS); break;
-        case AM_CHEB1: kFine<T, AM_CHEB1, OWNC><<<g, b, 0, s>>>(PSX_FINE_ARGS); break;
-        default: kFine<T, AM_CHEB2, OWNC><<<g, b, 0, s>>>(PSX_FINE_ARGS); break;
+    // fused dots exist for APPLY (p.q) and the Chebyshev steps (r.z, z.q) only
+    const bool dot = dots && (mode == AM_APPLY || mode == AM_CHEB1 || mode == AM_CHEB2);
+    if constexpr (sizeof(T) == 4 && !OWNC) {
+        switch (mode) {
+            case AM_APPLY:
+                if (dot) kFine4<T, AM_APPLY, true><<<g, b, 0, s>>>(PSX_FINE_ARGS);
+                else kFine4<T, AM_APPLY, false><<<g, b, 0, s>>>(PSX_FINE_ARGS);
+                break;
+            case AM_RAW: kFine4<T, AM_RAW, false><<<g, b, 0, s>>>(PSX_FINE_ARGS); break;
+            case AM_RESID: kFine4<T, AM_RESID, false><<<g, b, 0, s>>>(PSX_FINE_ARGS); break;
+            case AM_CHEB1:
+                if (dot) kFine4<T, AM_CHEB1, true><<<g, b, 0, s>>>(PSX_FINE_ARGS);
+                else kFine4<T, AM_CHEB1, false><<<g, b, 0, s>>>(PSX_FINE_ARGS);
+                break;
+            default:
+                if (dot) kFine4<T, AM_CHEB2, true><<<g, b, 0, s>>>(PSX_FINE_ARGS);
+                else kFine4<T, AM_CHEB2, false><<<g, b, 0, s>>>(PSX_FINE_ARGS);
+                break;
+        }
+    } else {
+        switch (mode) {
+            case AM_APPLY:
+                if (dot) kFine<T, AM_APPLY, OWNC, true><<<g, b, 0, s>>>(PSX_FINE_ARGS);
+                else kFine<T, AM_APPLY, OWNC, false><<<g, b, 0, s>>>(PSX_FINE_ARGS);
+                break;
+            case AM_RAW: kFine<T, AM_RAW, OWNC, false><<<g, b, 0, s>>>(PSX_FINE_ARGS); break;
+            case AM_RESID: kFine<T, AM_RESID, OWNC, false><<<g, b, 0, s>>>(PSX_FINE_ARGS); break;
+            case AM_CHEB1:
+                if (dot) kFine<T, AM_CHEB1, OWNC, true><<<g, b, 0, s>>>(PSX_FINE_ARGS);
+                else kFine<T, AM_CHEB1, OWNC, false><<<g, b, 0, s>>>(PSX_FINE_ARGS);
+                break;
+            default:
+                if (dot) kFine<T, AM_CHEB2, OWNC, true><<<g, b, 0, s>>>(PSX_FINE_ARGS);
+                else kFine<T, AM_CHEB2, OWNC, false><<<g, b, 0, s>>>(PSX_FINE_ARGS);
+                break;
+        }
     }
 #undef PSX_FINE_ARGS
 }

@@ -70,6 +70,7 @@ public:
     struct Lvl {
         DevLevel dl{};
         void* aop = nullptr;  // BF16 matrix-core operands of Kb (levels without own matrices)
+        bool childCheap = false;  // child-scale form preferred (OF_AUTO): most own elements two-valued
         double *r64 = nullptr, *a64 = nullptr, *b64 = nullptr, *d64 = nullptr;
         float *r32 = nullptr, *a32 = nullptr, *b32 = nullptr, *d32 = nullptr;
     };
@@ -163,6 +164,9 @@ public:
             }
             D.childForm = L.childForm && L.nOwn > 0;
             if (D.childForm) {
+                int64_t two = 0;
+                for (int64_t e = 0; e < L.nOwn; e++) two += L.childTwo[size_t(e) * 4 + 2] >= 0 ? 1 : 0;
+                lv[l].childCheap = 2 * two >= L.nOwn;
                 D.childS = upload(L.childS.data(), int64_t(L.childS.size()));
                 double rows[8 * 72];
                 float rows32[8 * 72];
@@ -255,8 +259,10 @@ public:
         return !(e && e[0] == '0');
     }();
     // form: 0 = the reference's operator (own matrices as stored FP32 entries); OF_CHILD = own
-    // elements in child-scale form where the level has it (exact Galerkin products; the FP64 and
-    // mixed solves); OF_MFMA = the BF16 matrix-core operator (FP32 vectors) where the level allows it
+    // elements in child-scale form where the level has it (exact Galerkin products); OF_AUTO = the
+    // child-scale form only where it is the cheaper one (the FP64 and mixed solves: a V-cycle needs
+    // no exact coarse operators); OF_MFMA = the BF16 matrix-core operator (FP32 vectors) where the
+    // level allows it
     template <class T>
     // fd: a conjugate-gradient dot product over the level's DOFs (FD_PQ: x.y, FD_RZ: r.y and y.q),
     // fused into the operator kernel where it can be, else run after it; either way finished into
@@ -273,7 +279,7 @@ public:
         }
         if (!done) {
             if (!useFine) launchApply<T>(D, mode, xx, yy, rr, dd, c1, c2, stop, stream, ni);
-            else if (D.symOk && (H.levels[l].nOwn == 0 || ((form & OF_CHILD) && D.childForm))) {
+            else if (D.symOk && (H.levels[l].nOwn == 0 || (D.childForm && ((form & OF_CHILD) || ((form & OF_AUTO) && lv[l].childCheap))))) {
                 if (fd) {
                     const unsigned nb = launchFineDot<T>(D, mode, xx, yy, rr, dd, c1, c2, stop, *fd, stream, ni);
                     if (nb) {
@@ -447,7 +453,7 @@ public:
         const int batch = o.check_every > 0 ? o.check_every : 8;
         // the reference's algorithm keeps its operator; the FP64 and mixed solves use the exact
         // child-scale form, and the mixed smoother the matrix cores when asked (one right-hand side)
-        const int form = prec == PSX_PREC_REF ? 0 : (OF_CHILD | (prec == PSX_PREC_MIXED && o.mfma_smoother ? OF_MFMA : 0));
+        const int form = prec == PSX_PREC_REF ? 0 : (OF_AUTO | (prec == PSX_PREC_MIXED && o.mfma_smoother ? OF_MFMA : 0));
         const int degree = o.cheb_degree > 0 ? o.cheb_degree : 2;
         if (degree > CHEB_MAX) throw std::invalid_argument("Chebyshev degree above 6");
         if (prec == PSX_PREC_MIXED) {

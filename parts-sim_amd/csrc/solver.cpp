@@ -374,10 +374,10 @@ int psx_profile(psx_model m, int reps, char* buf, size_t len) {
                           l, (long long)L.nNodes, L.elemVox.size(), (long long)L.nOwn, t64, t32, tbf, c64, c32);
             out += line;
         }
-        const double tv64 = timeIt([&] { M.vcycle<double>(0, M.lv[0].r64, nullptr, psx::OF_CHILD); });
-        const double tv32 = timeIt([&] { M.vcycle<float>(0, M.lv[0].r32, nullptr, psx::OF_CHILD); });
-        const double tv3 = timeIt([&] { M.vcycle<float>(0, M.lv[0].r32, nullptr, psx::OF_CHILD, 3); });
-        const double tvbf = timeIt([&] { M.vcycle<float>(0, M.lv[0].r32, nullptr, psx::OF_CHILD | psx::OF_MFMA); });
+        const double tv64 = timeIt([&] { M.vcycle<double>(0, M.lv[0].r64, nullptr, psx::OF_AUTO); });
+        const double tv32 = timeIt([&] { M.vcycle<float>(0, M.lv[0].r32, nullptr, psx::OF_AUTO); });
+        const double tv3 = timeIt([&] { M.vcycle<float>(0, M.lv[0].r32, nullptr, psx::OF_AUTO, 3); });
+        const double tvbf = timeIt([&] { M.vcycle<float>(0, M.lv[0].r32, nullptr, psx::OF_AUTO | psx::OF_MFMA); });
         const double tvref = timeIt([&] { M.vcycle<float>(0, M.lv[0].r32, nullptr, 0); });
         std::snprintf(line, sizeof line,
                       "V-cycle FP64 %.4f ms  FP32 %.4f ms  FP32 degree-3 %.4f ms  FP32+BF16 matrix cores %.4f ms  FP32 stored own matrices %.4f ms\n",
