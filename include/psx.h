@@ -71,6 +71,8 @@ typedef struct {
     void* user;
     int cheb_degree;            /* Chebyshev smoother degree 1..6 (0 = 2, the reference's, voxel_fea.cpp:692-725) */
     int mfma_smoother;          /* PSX_PREC_MIXED: smooth with the BF16 matrix-core operator (0 = FP32) */
+    int eager;                  /* 1: launch every kernel from the host; 0: replay a captured HIP graph of
+                                   check_every iterations (the same kernels and arguments) */
 } psx_solve_opts;
 
 /* Replaces SolveResult's scalars (voxel_fea.hpp:119-124). */

@@ -34,7 +34,7 @@ PROGRESS = C.CFUNCTYPE(C.c_int, C.c_int, C.c_double, C.c_void_p)
 class SolveOpts(C.Structure):
     _fields_ = [("tol", C.c_double), ("max_iter", C.c_int), ("precision", C.c_int), ("check_every", C.c_int),
                 ("fold_every", C.c_int), ("on_progress", PROGRESS), ("user", C.c_void_p), ("cheb_degree", C.c_int),
-                ("mfma_smoother", C.c_int)]
+                ("mfma_smoother", C.c_int), ("eager", C.c_int)]
 
 
 class ServerOpts(C.Structure):
@@ -297,10 +297,10 @@ class Model:
         return z
 
     @staticmethod
-    def _opts(tol, max_iter, precision, check_every, fold_every, on_progress, cheb_degree=0, mfma_smoother=False):
+    def _opts(tol, max_iter, precision, check_every, fold_every, on_progress, cheb_degree=0, mfma_smoother=False, eager=False):
         cb = PROGRESS(0) if on_progress is None else PROGRESS(lambda it, rel, user: int(bool(on_progress(it, rel))))
         return SolveOpts(float(tol), int(max_iter), PREC[precision], int(check_every), int(fold_every), cb, None, int(cheb_degree),
-                         int(bool(mfma_smoother))), cb
+                         int(bool(mfma_smoother)), int(bool(eager))), cb
 
     @staticmethod
     def _info(info):
@@ -308,13 +308,13 @@ class Model:
                 "cancelled": bool(info.cancelled), "broken": bool(info.broken), "folds": info.folds, "device_ms": info.device_ms}
 
     def solve(self, f, tol=1e-6, max_iter=500, x0=None, precision="fp64", check_every=0, fold_every=0, on_progress=None, cheb_degree=0,
-              mfma_smoother=False):
+              mfma_smoother=False, eager=False):
         f = np.ascontiguousarray(f, np.float64)
         if f.size != self.n_dof or (x0 is not None and np.size(x0) != self.n_dof):
             raise ValueError("Force or displacement vector has the wrong size.")
         x0a = None if x0 is None else np.ascontiguousarray(x0, np.float64)
         u = np.zeros_like(f)
-        o, cb = self._opts(tol, max_iter, precision, check_every, fold_every, on_progress, cheb_degree, mfma_smoother)
+        o, cb = self._opts(tol, max_iter, precision, check_every, fold_every, on_progress, cheb_degree, mfma_smoother, eager)
         info = SolveInfo()
         _check(lib().psx_solve(self.h, _p(f), _p(x0a), C.byref(o), _p(u), C.byref(info)))
         out = self._info(info)
@@ -322,15 +322,16 @@ class Model:
         return out
 
     def solve_device(self, d_f, d_u, d_x0=None, tol=1e-6, max_iter=500, precision="fp64", check_every=0, fold_every=0, cheb_degree=0,
-                     mfma_smoother=False):
+                     mfma_smoother=False, eager=False):
         """Solve with HBM-resident vectors (DeviceArray or raw device addresses)."""
-        o, cb = self._opts(tol, max_iter, precision, check_every, fold_every, None, cheb_degree, mfma_smoother)
+        o, cb = self._opts(tol, max_iter, precision, check_every, fold_every, None, cheb_degree, mfma_smoother, eager)
         info = SolveInfo()
         addr = lambda a: None if a is None else C.c_void_p(int(a))
         _check(lib().psx_solve_device(self.h, addr(d_f), addr(d_x0), C.byref(o), addr(d_u), C.byref(info)))
         return self._info(info)
 
-    def solve_multi(self, F, tol=1e-6, max_iter=500, X0=None, precision="mixed", check_every=0, fold_every=0, on_progress=None, cheb_degree=0):
+    def solve_multi(self, F, tol=1e-6, max_iter=500, X0=None, precision="mixed", check_every=0, fold_every=0, on_progress=None, cheb_degree=0,
+                    eager=False):
         """Solve K U[k] = F[k] for nrhs load cases (F: nrhs x n_dof) in one set of launches per
         V-cycle level; returns U and one info dict per load case."""
         F = np.ascontiguousarray(F, np.float64)
@@ -341,14 +342,15 @@ class Model:
         if X0a is not None and X0a.size != F.size:
             raise ValueError("Start displacements must be nrhs x n_dof.")
         U = np.zeros_like(F)
-        o, cb = self._opts(tol, max_iter, precision, check_every, fold_every, on_progress, cheb_degree, False)
+        o, cb = self._opts(tol, max_iter, precision, check_every, fold_every, on_progress, cheb_degree, False, eager)
         infos = (SolveInfo * k)()
         _check(lib().psx_solve_multi(self.h, k, _p(F), _p(X0a), C.byref(o), _p(U), infos))
         return U, [self._info(infos[i]) for i in range(k)]
 
-    def solve_multi_device(self, nrhs, d_f, d_u, d_x0=None, tol=1e-6, max_iter=500, precision="mixed", check_every=0, fold_every=0, cheb_degree=0):
+    def solve_multi_device(self, nrhs, d_f, d_u, d_x0=None, tol=1e-6, max_iter=500, precision="mixed", check_every=0, fold_every=0, cheb_degree=0,
+                           eager=False):
         """solve_multi with HBM-resident nrhs x n_dof vectors (DeviceArray or raw device addresses)."""
-        o, cb = self._opts(tol, max_iter, precision, check_every, fold_every, None, cheb_degree, False)
+        o, cb = self._opts(tol, max_iter, precision, check_every, fold_every, None, cheb_degree, False, eager)
         infos = (SolveInfo * int(nrhs))()
         addr = lambda a: None if a is None else C.c_void_p(int(a))
         _check(lib().psx_solve_multi_device(self.h, int(nrhs), addr(d_f), addr(d_x0), C.byref(o), addr(d_u), infos))

@@ -36,6 +36,9 @@ struct DevLevel {
     const double* ctab64;
     const float* ctab32;
     int childForm;
+    // kFine sums the 8 voxels of every node directly (no stencil, no pair lists): set on levels
+    // without own elements where most nodes have differing voxels (per-voxel densities)
+    int dense;
     const double* invD64;      // inverse diagonal, 0 on held / inactive DOFs
     const float* invD32;
     const double* diagAdd;     // optional diagonal shift (nullptr when absent)

@@ -33,6 +33,7 @@ constexpr int VX = TX + 1, VY = TY + 1;  // voxel plane
 __device__ constexpr int cx(int b) { return (b == 1 || b == 2 || b == 5 || b == 6) ? 1 : 0; }
 __device__ constexpr int cy(int b) { return (b == 2 || b == 3 || b == 6 || b == 7) ? 1 : 0; }
 __device__ constexpr int cz(int b) { return b >= 4 ? 1 : 0; }
+__device__ constexpr int cornerOf(int x, int y, int z) { return (y ? (x ? 2 : 3) : (x ? 1 : 0)) + 4 * z; }
 // Symmetry classes (the host's elementClass, hierarchy.cpp): every element coefficient is
 // +-one of 22 values, at an index known at compile time once the loops are unrolled, so the values
 // stay in scalar registers for the whole kernel.
@@ -97,8 +98,11 @@ template <class T, bool OWNC> struct Ring {
     DevLevel L, Sym<T> C, const T *__restrict__ crow, const T *__restrict__ ctab, const T *__restrict__ x, T *__restrict__ y,                 \
         const T *__restrict__ r, const T *__restrict__ zp, T c1, T c2, const int *stop, int zChunk, int gx, int gy, int nTiles, int xcdOrder, \
         double *__restrict__ dots, int64_t dotStride, const T *__restrict__ qdot
-// DOT: the fused dot products are accumulated (dots != nullptr)
-template <class T, int MODE, bool OWNC, bool DOT> __device__ __forceinline__ void fineBody(PSX_FINE_PARAMS) {
+// DOT: the fused dot products are accumulated (dots != nullptr). DENSE (levels without own
+// elements): every node sums its 8 voxels, s_v E_v (x - x_node) with E_v the node's 3 rows of Kb in
+// voxel v (compile-time class coefficients), instead of the stencil and the pair lists; for per-voxel
+// densities, where nearly every node would list 7 or 8 pairs.
+template <class T, int MODE, bool OWNC, bool DOT, bool DENSE = false> __device__ __forceinline__ void fineBody(PSX_FINE_PARAMS) {
     // multi-instance launches: blockIdx.y is the right-hand side (its vectors, stop flag, partials)
     if (stop) stop += int64_t(blockIdx.y) * CG_INTS;
     if (stop && __builtin_nontemporal_load(stop) != 0) return;
@@ -246,7 +250,7 @@ template <class T, int MODE, bool OWNC, bool DOT> __device__ __forceinline__ voi
 #pragma unroll
             for (int q = 0; q < 9; q++)
 #pragma unroll
-                for (int c = 0; c < 3; c++) xm[q][c] = sh.x[sl][c][ty + q / 3][tx + q % 3];
+                for (int c = 0; c < 3; c++) xm[q][c] = (!DENSE || q == 4) ? sh.x[sl][c][ty + q / 3][tx + q % 3] : T(0);
         }
         // The stencil in class-sum form: with X_c(ox, oy) the plane's 3 x 3 values of component c and
         // oz = plane - node, every entry is D[|o_d|][nnz of the other two offsets] on the diagonal and
@@ -258,7 +262,7 @@ template <class T, int MODE, bool OWNC, bool DOT> __device__ __forceinline__ voi
         // P (x(z - 1) - 2 x(z) + x(z + 1)), P = the diagonal sum of a neighbour plane's blocks. A
         // rigid translation then cancels exactly and a rotation to element scale, which keeps the
         // FP32 operator accurate on the smooth, low-energy modes the conjugate gradients resolve last.
-        {
+        if constexpr (!DENSE) {
             const T D00 = C.v[14], D01 = C.v[15], D02 = C.v[16], D10 = C.v[17], D11 = C.v[18], D12 = C.v[19], E0 = C.v[20], E1 = C.v[21];
             (void)D00;
             // q = (ox + 1) + 3 (oy + 1)
@@ -305,7 +309,7 @@ template <class T, int MODE, bool OWNC, bool DOT> __device__ __forceinline__ voi
 
         // node k - 1 is complete: list its differing pairs (per-wave regions, voxel order)
         const bool done = k - 1 >= z0;
-        const uint32_t mask = done ? (cur.info & 0xffu) : 0u;
+        const uint32_t mask = (done && !DENSE) ? (cur.info & 0xffu) : 0u;
         const int cnt = __popc(mask);
         const uint64_t lt = (uint64_t(1) << lane) - 1;
         const uint64_t b0 = __ballot(cnt & 1), b1 = __ballot(cnt & 2), b2 = __ballot(cnt & 4), b3 = __ballot(cnt & 8);
@@ -313,7 +317,7 @@ template <class T, int MODE, bool OWNC, bool DOT> __device__ __forceinline__ voi
         // reference voxel: the lowest clear bit (all 8 differ only when all are own; voxel 0 then
         // reads scale 0)
         const int ref = mask == 0xffu ? 0 : __builtin_ctz(~mask);
-        {
+        if constexpr (!DENSE) {
             uint32_t m = mask;
             int at = waveOff;
             while (m) {
@@ -322,21 +326,49 @@ template <class T, int MODE, bool OWNC, bool DOT> __device__ __forceinline__ voi
                 sh.list[wid][at++] = uint16_t((tid << 6) | (ref << 3) | v);
             }
         }
-        if (lane == 0) sh.waveTot[k & 1][wid] = __popcll(b0) + 2 * __popcll(b1) + 4 * __popcll(b2) + 8 * __popcll(b3);
+        if (!DENSE && lane == 0) sh.waveTot[k & 1][wid] = __popcll(b0) + 2 * __popcll(b1) + 4 * __popcll(b2) + 8 * __popcll(b3);
         __syncthreads();
 
         if (done) {
             int tot[NW], total = 0, mineBase = 0;
 #pragma unroll
             for (int w = 0; w < NW; w++) {
-                tot[w] = sh.waveTot[k & 1][w];
+                tot[w] = DENSE ? 0 : sh.waveTot[k & 1][w];
                 mineBase += w < wid ? tot[w] : 0;
                 total += tot[w];
             }
             const int sm = (k + 2) & 3, s0 = (k + 3) & 3, sp = k & 3;  // node planes k - 2, k - 1, k
             const int vm = (k + 1) % 3, v0 = (k + 2) % 3;              // voxel planes k - 2, k - 1
             T out[3];
-            {
+            if constexpr (DENSE) {
+                // s_v E_v (x - x_node) over the node's 8 voxels; voxel v = (di, dj, dk) lies at node
+                // offset (di - 1, dj - 1, dk - 1) and has the node as its corner a = (1 - di, 1 - dj, 1 - dk)
+                T xn[3];
+#pragma unroll
+                for (int c = 0; c < 3; c++) xn[c] = sh.x[s0][c][ty + 1][tx + 1];
+                out[0] = out[1] = out[2] = T(0);
+                staticFor<8>([&](auto V) {
+                    constexpr int v = decltype(V)::value, di = v & 1, dj = (v >> 1) & 1, dk = v >> 2;
+                    constexpr int A = cornerOf(1 - di, 1 - dj, 1 - dk);
+                    const T sv = T(sh.s[dk ? v0 : vm][ty + dj][tx + di]);
+                    T acc[3] = {T(0), T(0), T(0)};
+                    staticFor<8>([&](auto Q) {
+                        constexpr int q = decltype(Q)::value;
+                        if constexpr (q != A) {
+                            constexpr int ox = di - 1 + cx(q), oy = dj - 1 + cy(q), oz = dk - 1 + cz(q);
+                            const int sl = oz < 0 ? sm : (oz == 0 ? s0 : sp);
+                            T dx[3];
+#pragma unroll
+                            for (int c = 0; c < 3; c++) dx[c] = sh.x[sl][c][ty + 1 + oy][tx + 1 + ox] - xn[c];
+                            staticFor<3>([&](auto Dd) {
+                                staticFor<3>([&](auto Cc) { acc[Dd] += coefMul<elementClass(A, Dd, q, Cc), T>(C, dx[Cc]); });
+                            });
+                        }
+                    });
+#pragma unroll
+                    for (int d = 0; d < 3; d++) out[d] += sv * acc[d];
+                });
+            } else {
                 // the centre column's second difference through the neighbour planes' diagonal sums
                 const T Pxy = C.v[15] + T(2) * (C.v[18] + C.v[16]) + T(4) * C.v[19], Pz = C.v[17] + T(4) * (C.v[18] + C.v[19]);
                 const T sref = T(sh.s[(ref >> 2) ? v0 : vm][ty + ((ref >> 1) & 1)][tx + (ref & 1)]);
@@ -511,6 +543,8 @@ template <class T, int MODE, bool OWNC, bool DOT> __device__ __forceinline__ voi
                 }
             }
         }
+        // DENSE reads node plane k - 2, which the next step's store replaces
+        if constexpr (DENSE) __syncthreads();
         // rotate: accumulators move up one node, the centre of plane k becomes node k's x
 #pragma unroll
         for (int d = 0; d < 3; d++) {
@@ -558,6 +592,10 @@ template <class T, int MODE, bool DOT>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void kFine4(PSX_FINE_PARAMS) {
     fineBody<T, MODE, false, DOT>(PSX_FINE_FWD);
 }
+// per-voxel form (levels without own elements)
+template <class T, int MODE, bool DOT> __global__ __launch_bounds__(NT) void kFineDense(PSX_FINE_PARAMS) {
+    fineBody<T, MODE, false, DOT, true>(PSX_FINE_FWD);
+}
 #undef PSX_FINE_FWD
 
 }  // namespace
@@ -578,7 +616,24 @@ static void launchFineT(const DevLevel& L, int mode, const Sym<T>& C, const T* c
 #define PSX_FINE_ARGS L, C, crow, ctab, x, y, r, d, T(c1), T(c2), stop, zChunk, gx, gy, nTiles, xcdOrder, dots, dotStride, qdot
     // fused dots exist for APPLY (p.q) and the Chebyshev steps (r.z, z.q) only
     const bool dot = dots && (mode == AM_APPLY || mode == AM_CHEB1 || mode == AM_CHEB2);
-    if constexpr (sizeof(T) == 4 && !OWNC) {
+    if (!OWNC && L.dense) {
+        switch (mode) {
+            case AM_APPLY:
+                if (dot) kFineDense<T, AM_APPLY, true><<<g, b, 0, s>>>(PSX_FINE_ARGS);
+                else kFineDense<T, AM_APPLY, false><<<g, b, 0, s>>>(PSX_FINE_ARGS);
+                break;
+            case AM_RAW: kFineDense<T, AM_RAW, false><<<g, b, 0, s>>>(PSX_FINE_ARGS); break;
+            case AM_RESID: kFineDense<T, AM_RESID, false><<<g, b, 0, s>>>(PSX_FINE_ARGS); break;
+            case AM_CHEB1:
+                if (dot) kFineDense<T, AM_CHEB1, true><<<g, b, 0, s>>>(PSX_FINE_ARGS);
+                else kFineDense<T, AM_CHEB1, false><<<g, b, 0, s>>>(PSX_FINE_ARGS);
+                break;
+            default:
+                if (dot) kFineDense<T, AM_CHEB2, true><<<g, b, 0, s>>>(PSX_FINE_ARGS);
+                else kFineDense<T, AM_CHEB2, false><<<g, b, 0, s>>>(PSX_FINE_ARGS);
+                break;
+        }
+    } else if constexpr (sizeof(T) == 4 && !OWNC) {
         switch (mode) {
             case AM_APPLY:
                 if (dot) kFine4<T, AM_APPLY, true><<<g, b, 0, s>>>(PSX_FINE_ARGS);

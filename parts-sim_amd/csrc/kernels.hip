@@ -236,88 +236,100 @@ __global__ __launch_bounds__(BLOCK) void kApply(DevLevel L, const T* __restrict_
 // this spreads one node's dependent loads over 8 lanes instead of one thread walking 8 voxels.
 template <class T, int MODE>
 __global__ __launch_bounds__(BLOCK) void kApplyOwn(DevLevel L, const T* __restrict__ x, T* __restrict__ y, const T* __restrict__ r, const T* __restrict__ zp,
-                                                   T c1, T c2, const int* stop) {
-    stop = instFlag(stop);
-    if (stopped(stop)) return;
-    {
-        const int64_t io = instOffset(L.nDof);
-        x += io;
-        y += io;
-        if (r) r += io;
-        if (zp) zp += io;
-    }
+                                                   T c1, T c2, const int* stop, int ninst) {
+    // Several right-hand sides: this lane's element rows (72 coefficients) are loaded once and
+    // applied to every instance in turn (instance-major vectors of nDof each, a stop flag per
+    // instance), so the own matrices stream from HBM once per launch instead of once per instance.
     __shared__ T Kb[576];
     for (int q = threadIdx.x; q < 576; q += BLOCK) Kb[q] = Prec<T>::Kb(L)[q];
     __syncthreads();
     const int64_t n = int64_t(blockIdx.x) * (BLOCK / 8) + threadIdx.x / 8;
     const int v = threadIdx.x & 7;
-    T b0 = 0, b1 = 0, b2 = 0;
+    const int64_t NX = L.NX, NXY = int64_t(L.NX) * L.NY;
+    bool has = false;
+    int32_t w = 0;
+    int64_t nb = 0;
+    int a = 0;
     if (n < L.nNodes) {
         const unsigned nn = unsigned(n), NXu = unsigned(L.NX), NYu = unsigned(L.NY);
         const int i = int(nn % NXu), j = int((nn / NXu) % NYu), k = int(nn / (NXu * NYu));
         const int di = v & 1, dj = (v >> 1) & 1, dk = v >> 2;
         const int ei = i - 1 + di, ej = j - 1 + dj, ek = k - 1 + dk;
         if (ei >= 0 && ej >= 0 && ek >= 0 && ei < L.nx && ej < L.ny && ek < L.nz) {
-            const int32_t w = L.words[ei + int64_t(L.nx) * (ej + int64_t(L.ny) * ek)];
-            if (w != 0) {
-                const int64_t NX = L.NX, NXY = int64_t(L.NX) * L.NY;
-                const int a = d_corner(1 - di, 1 - dj, 1 - dk);
-                const int64_t nb = ei + NX * ej + NXY * ek;
-                T xe[24];
+            w = L.words[ei + int64_t(L.nx) * (ej + int64_t(L.ny) * ek)];
+            has = w != 0;
+            a = d_corner(1 - di, 1 - dj, 1 - dk);
+            nb = ei + NX * ej + NXY * ek;
+        }
+    }
+    // the node's 3 rows of the voxel's element: s * Kb (scaled after the sum) or the own matrix
+    T K[72];
+    const T s = w > 0 ? T(__int_as_float(w)) : T(1);
+    if (has) {
+        if (w > 0) {
 #pragma unroll
-                for (int b = 0; b < 8; b++) {
-                    const T* xm = x + 3 * (nb + kCX(b) + kCY(b) * NX + kCZ(b) * NXY);
-                    xe[3 * b] = xm[0];
-                    xe[3 * b + 1] = xm[1];
-                    xe[3 * b + 2] = xm[2];
-                }
-                if (w > 0) {
-                    const T* K = Kb + 72 * a;
+            for (int c = 0; c < 72; c++) K[c] = Kb[72 * a + c];
+        } else {
+            const float4* __restrict__ K4 = reinterpret_cast<const float4*>(L.Kown + int64_t(w & 0x7fffffff) * 576 + 72 * a);
 #pragma unroll
-                    for (int c = 0; c < 24; c++) {
-                        b0 += K[c] * xe[c];
-                        b1 += K[24 + c] * xe[c];
-                        b2 += K[48 + c] * xe[c];
-                    }
-                    const T s = T(__int_as_float(w));
-                    b0 *= s;
-                    b1 *= s;
-                    b2 *= s;
-                } else {
-                    const float4* __restrict__ K4 = reinterpret_cast<const float4*>(L.Kown + int64_t(w & 0x7fffffff) * 576 + 72 * a);
-#pragma unroll
-                    for (int q = 0; q < 6; q++) {
-                        const float4 r0 = K4[q], r1 = K4[6 + q], r2 = K4[12 + q];
-                        b0 += T(r0.x) * xe[4 * q] + T(r0.y) * xe[4 * q + 1] + T(r0.z) * xe[4 * q + 2] + T(r0.w) * xe[4 * q + 3];
-                        b1 += T(r1.x) * xe[4 * q] + T(r1.y) * xe[4 * q + 1] + T(r1.z) * xe[4 * q + 2] + T(r1.w) * xe[4 * q + 3];
-                        b2 += T(r2.x) * xe[4 * q] + T(r2.y) * xe[4 * q + 1] + T(r2.z) * xe[4 * q + 2] + T(r2.w) * xe[4 * q + 3];
-                    }
-                }
+            for (int q = 0; q < 18; q++) {
+                const float4 r4 = K4[q];
+                K[4 * q] = T(r4.x);
+                K[4 * q + 1] = T(r4.y);
+                K[4 * q + 2] = T(r4.z);
+                K[4 * q + 3] = T(r4.w);
             }
         }
     }
-#pragma unroll
-    for (int o = 4; o >= 1; o >>= 1) {
-        b0 += __shfl_xor(b0, o, 64);
-        b1 += __shfl_xor(b1, o, 64);
-        b2 += __shfl_xor(b2, o, 64);
-    }
-    if (v != 0 || n >= L.nNodes) return;
-    T acc[3] = {b0, b1, b2};
-    if (L.diagAdd) {
-#pragma unroll
-        for (int d = 0; d < 3; d++) acc[d] += T(L.diagAdd[3 * n + d]) * x[3 * n + d];
-    }
     const T* __restrict__ invD = Prec<T>::invD(L);
+    for (int inst = 0; inst < ninst; inst++) {
+        if (stopped(stop ? stop + int64_t(inst) * CG_INTS : nullptr)) continue;  // uniform
+        const int64_t io = int64_t(inst) * L.nDof;
+        const T* __restrict__ xi = x + io;
+        T b0 = 0, b1 = 0, b2 = 0;
+        if (has) {
+            T xe[24];
 #pragma unroll
-    for (int d = 0; d < 3; d++) {
-        const int64_t q = 3 * n + d;
-        const T id = invD[q];
-        if constexpr (MODE == AM_APPLY) y[q] = id != T(0) ? acc[d] : T(0);
-        else if constexpr (MODE == AM_RAW) y[q] = acc[d];
-        else if constexpr (MODE == AM_RESID) y[q] = id != T(0) ? r[q] - acc[d] : T(0);
-        else if constexpr (MODE == AM_CHEB1) y[q] = x[q] + c1 * id * (r[q] - acc[d]);
-        else y[q] = x[q] + c1 * (zp ? x[q] - zp[q] : x[q]) + c2 * id * (r[q] - acc[d]);
+            for (int b = 0; b < 8; b++) {
+                const T* xm = xi + 3 * (nb + kCX(b) + kCY(b) * NX + kCZ(b) * NXY);
+                xe[3 * b] = xm[0];
+                xe[3 * b + 1] = xm[1];
+                xe[3 * b + 2] = xm[2];
+            }
+#pragma unroll
+            for (int c = 0; c < 24; c++) {
+                b0 += K[c] * xe[c];
+                b1 += K[24 + c] * xe[c];
+                b2 += K[48 + c] * xe[c];
+            }
+            b0 *= s;
+            b1 *= s;
+            b2 *= s;
+        }
+#pragma unroll
+        for (int o = 4; o >= 1; o >>= 1) {
+            b0 += __shfl_xor(b0, o, 64);
+            b1 += __shfl_xor(b1, o, 64);
+            b2 += __shfl_xor(b2, o, 64);
+        }
+        if (v == 0 && n < L.nNodes) {
+            T acc[3] = {b0, b1, b2};
+            if (L.diagAdd) {
+#pragma unroll
+                for (int d = 0; d < 3; d++) acc[d] += T(L.diagAdd[3 * n + d]) * xi[3 * n + d];
+            }
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                const int64_t q = 3 * n + d;
+                const T id = invD[q];
+                T* __restrict__ yq = y + io + q;
+                if constexpr (MODE == AM_APPLY) *yq = id != T(0) ? acc[d] : T(0);
+                else if constexpr (MODE == AM_RAW) *yq = acc[d];
+                else if constexpr (MODE == AM_RESID) *yq = id != T(0) ? r[io + q] - acc[d] : T(0);
+                else if constexpr (MODE == AM_CHEB1) *yq = xi[q] + c1 * id * (r[io + q] - acc[d]);
+                else *yq = xi[q] + c1 * (zp ? xi[q] - zp[io + q] : xi[q]) + c2 * id * (r[io + q] - acc[d]);
+            }
+        }
     }
 }
 
@@ -748,13 +760,14 @@ void launchApply(const DevLevel& L, int mode, const T* x, T* y, const T* r, cons
 
 template <class T>
 void launchApplyOwn(const DevLevel& L, int mode, const T* x, T* y, const T* r, const T* d, double c1, double c2, const int* stop, hipStream_t s, int ninst) {
-    const dim3 g(unsigned((L.nNodes + BLOCK / 8 - 1) / (BLOCK / 8)), ninst), b(BLOCK);
+    // every instance in one grid: the kernel loops over them (the element rows are read once)
+    const dim3 g(unsigned((L.nNodes + BLOCK / 8 - 1) / (BLOCK / 8))), b(BLOCK);
     switch (mode) {
-        case AM_APPLY: kApplyOwn<T, AM_APPLY><<<g, b, 0, s>>>(L, x, y, r, d, T(c1), T(c2), stop); break;
-        case AM_RAW: kApplyOwn<T, AM_RAW><<<g, b, 0, s>>>(L, x, y, r, d, T(c1), T(c2), stop); break;
-        case AM_RESID: kApplyOwn<T, AM_RESID><<<g, b, 0, s>>>(L, x, y, r, d, T(c1), T(c2), stop); break;
-        case AM_CHEB1: kApplyOwn<T, AM_CHEB1><<<g, b, 0, s>>>(L, x, y, r, d, T(c1), T(c2), stop); break;
-        default: kApplyOwn<T, AM_CHEB2><<<g, b, 0, s>>>(L, x, y, r, d, T(c1), T(c2), stop); break;
+        case AM_APPLY: kApplyOwn<T, AM_APPLY><<<g, b, 0, s>>>(L, x, y, r, d, T(c1), T(c2), stop, ninst); break;
+        case AM_RAW: kApplyOwn<T, AM_RAW><<<g, b, 0, s>>>(L, x, y, r, d, T(c1), T(c2), stop, ninst); break;
+        case AM_RESID: kApplyOwn<T, AM_RESID><<<g, b, 0, s>>>(L, x, y, r, d, T(c1), T(c2), stop, ninst); break;
+        case AM_CHEB1: kApplyOwn<T, AM_CHEB1><<<g, b, 0, s>>>(L, x, y, r, d, T(c1), T(c2), stop, ninst); break;
+        default: kApplyOwn<T, AM_CHEB2><<<g, b, 0, s>>>(L, x, y, r, d, T(c1), T(c2), stop, ninst); break;
     }
 }
 

@@ -90,6 +90,22 @@ public:
     // the number the current launches run
     int cap = 0;
     int ni = 1;
+    // the last captured batch of conjugate-gradient iterations and what it was captured for
+    struct GraphKey {
+        int prec = -1, degree = 0, form = 0, k = 0, batch = 0;
+        const double* df = nullptr;
+        double* du = nullptr;
+        bool operator==(const GraphKey& o) const {
+            return prec == o.prec && degree == o.degree && form == o.form && k == o.k && batch == o.batch && df == o.df && du == o.du;
+        }
+    };
+    hipGraphExec_t graph = nullptr;
+    GraphKey graphKey{};
+    void dropGraph() {
+        if (graph) (void)hipGraphExecDestroy(graph);
+        graph = nullptr;
+        graphKey = GraphKey{};
+    }
 
     explicit Model(const psx_model_desc& d) {
         int n = 0;
@@ -102,6 +118,7 @@ public:
     // Device buffers come from the stream-ordered allocator: plain hipFree waits for the whole
     // device, which would stall every other model's stream (concurrent serving requests).
     ~Model() {
+        dropGraph();
         if (stream) {
             for (void* ptr : allocs) (void)hipFreeAsync(ptr, stream);
             (void)hipStreamSynchronize(stream);
@@ -163,6 +180,14 @@ public:
                 D.nodeInfo = upload(info.data(), L.nNodes);
             }
             D.childForm = L.childForm && L.nOwn > 0;
+            {
+                // per-voxel form where most nodes have differing voxels (PSX_DENSE=0 / 1 forces it
+                // off / on for the levels without own elements)
+                int64_t irregular = 0;
+                for (int64_t nn = 0; nn < L.nNodes; nn++) irregular += L.diffMask[nn] != 0 ? 1 : 0;
+                const char* e = std::getenv("PSX_DENSE");
+                D.dense = L.nOwn == 0 && (e && e[0] ? e[0] == '1' : 2 * irregular > L.nNodes);
+            }
             if (D.childForm) {
                 int64_t two = 0;
                 for (int64_t e = 0; e < L.nOwn; e++) two += L.childTwo[size_t(e) * 4 + 2] >= 0 ? 1 : 0;
@@ -211,6 +236,7 @@ public:
     void reserve(int k) {
         if (k <= cap) return;
         sync();
+        dropGraph();  // its kernels point into the buffers about to be replaced
         for (void* ptr : work) HIPCHK(hipFreeAsync(ptr, stream));
         work.clear();
         workBytes = 0;
@@ -494,7 +520,9 @@ public:
             HIPCHK(hipMemsetAsync(q, 0, nk * 8, stream));
         }
         bool cancelled = false;
-        while (!allStopped) {
+        // one batch of iterations: every kernel returns at once for an instance whose state says
+        // stop, so the same launches serve every batch
+        auto enqueueBatch = [&] {
             for (int b = 0; b < batch; b++) {
                 if (prec == PSX_PREC_MIXED) {
                     const DotFusion fz{FD_RZ, q32, red}, fp{FD_PQ, nullptr, red};
@@ -522,6 +550,35 @@ public:
                 op<double>(0, AM_APPLY, p, q, nullptr, nullptr, 0, 0, stop, form, &fp);
                 launchUpdateXR<double>(du, p, r, q, n, st, red, stream, k);
             }
+        };
+        // the batch as a HIP graph, captured once and replayed (kernels only: no copies or memsets);
+        // it saves the host one launch per kernel, which bounds small parts
+        const GraphKey key{prec, degree, form, k, batch, df, du};
+        const bool useGraph = !o.eager && !graphsOff();
+        if (useGraph && !allStopped && !(graph && graphKey == key)) {
+            dropGraph();
+            HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed));
+            try {
+                enqueueBatch();
+            } catch (...) {
+                hipGraph_t g = nullptr;
+                (void)hipStreamEndCapture(stream, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            hipGraph_t g = nullptr;
+            HIPCHK(hipStreamEndCapture(stream, &g));
+            const hipError_t e = hipGraphInstantiate(&graph, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) {
+                graph = nullptr;
+                throw DeviceError(std::string("hipGraphInstantiate failed: ") + hipGetErrorString(e));
+            }
+            graphKey = key;
+        }
+        while (!allStopped) {
+            if (useGraph) HIPCHK(hipGraphLaunch(graph, stream));
+            else enqueueBatch();
             readState();
             allStopped = true;
             int itMax = 0;
@@ -565,6 +622,15 @@ public:
             info[i].converged = !cancelled && std::isfinite(info[i].residual) && info[i].residual <= o.tol * 10;
             info[i].device_ms = ms(t0);
         }
+    }
+
+    // PSX_GRAPHS=0 launches every kernel from the host
+    static bool graphsOff() {
+        static const bool off = [] {
+            const char* e = std::getenv("PSX_GRAPHS");
+            return e && e[0] == '0';
+        }();
+        return off;
     }
 
     static double ms(std::chrono::steady_clock::time_point t0) {

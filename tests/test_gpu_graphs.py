@@ -1,0 +1,38 @@
+"""Solves replay one captured HIP graph per batch of conjugate-gradient iterations (the same kernels
+with the same arguments as the eager launches, which read their scalars from the device-side state).
+Every replay must give the eager result bit for bit, with eager work and host synchronisations of the
+program between the replays, and a graph must follow the buffers it was captured for."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed", "ref"])
+def test_graph_replays_equal_eager(P, W, precision):
+    w = W.void_block((40, 16, 12), seed=21)
+    m = P.Model(w["dims"], w["density"], w["nu"], w["bc"], 300)
+    f = w["f"]
+    eager = m.solve(f, tol=1e-9, precision=precision, eager=True)
+    for rep in range(3):
+        # the second and third solves replay the graph the first one captured, with eager solves
+        # and reads in between
+        g = m.solve(f * (1 + rep), tol=1e-9, precision=precision, check_every=4)
+        e = m.solve(f * (1 + rep), tol=1e-9, precision=precision, check_every=4, eager=True)
+        print(precision, rep, g["iterations"], e["iterations"], flush=True)
+        assert g["iterations"] == e["iterations"]
+        assert np.array_equal(g["u"], e["u"])
+    assert np.array_equal(m.solve(f, tol=1e-9, precision=precision)["u"], eager["u"])
+    m.close()
+
+
+def test_graph_follows_grown_buffers(P, W):
+    w = W.simp_block((20, 10, 10), seed=2, nrhs=4)
+    m = P.Model(w["dims"], w["density"], w["nu"], w["bc"], 300)
+    F = w["f"]
+    one = m.solve(F[0], tol=1e-8, precision="mixed")
+    U, _ = m.solve_multi(F, tol=1e-8, precision="mixed")  # grows the work vectors: a new graph
+    again = m.solve(F[0], tol=1e-8, precision="mixed")
+    eager = m.solve(F[0], tol=1e-8, precision="mixed", eager=True)
+    assert np.array_equal(one["u"], eager["u"]) and np.array_equal(again["u"], eager["u"]) and np.array_equal(U[0], eager["u"])
+    m.close()

@@ -256,3 +256,27 @@ def test_stresses_match_oracle(models):
         assert a is None and np.array_equal(vm2, vm)
         t, to = g.stress_tensors(u, E, h), o.stress_tensors(u, E, h)
         assert np.abs(t.astype(float) - to).max() <= 1e-6 * float(np.abs(to).max())
+
+
+@pytest.mark.parametrize("forced", ["1", "0"])
+def test_dense_form_forced(P, W, cpu, forced, monkeypatch):
+    """The per-voxel operator form (every node sums its 8 voxels), forced on for parts where most
+    nodes are regular and off for the SIMP block, against the oracle: operator, V-cycle and solve."""
+    monkeypatch.setenv("PSX_DENSE", forced)
+    for w in [W.void_block((48, 16, 16), seed=3), W.cantilever(4), W.simp_block((32, 16, 16), nrhs=1)]:
+        g = P.Model(w["dims"], w["density"], w["nu"], w["bc"], 1200)
+        o = cpu.Model(w["dims"], w["density"], w["nu"], w["bc"], 1200)
+        n = g.level(0)["nDof"]
+        x = np.random.default_rng(5).standard_normal(n)
+        for raw in (False, True):
+            assert _rel(g.apply(x, 0, raw), o.apply(x, 0, raw)) < 1e-13
+        x32 = x.astype(np.float32)
+        assert _rel(g.apply32(x32, 0).astype(np.float64), o.apply32(x32, 0).astype(np.float64)) < 1e-5
+        r = np.random.default_rng(6).standard_normal(n)
+        assert _rel(g.precondition(r), o.precondition(r)) < 1e-4
+        f = w["f"] if w["f"].ndim == 1 else w["f"][0]
+        u = g.solve(f, tol=1e-10, precision="mixed")["u"]
+        uo = o.solve(f, tol=1e-12)["u"]
+        print(w["name"], forced, _rel(u, uo), flush=True)
+        assert _rel(u, uo) < 1e-7
+        g.close()
