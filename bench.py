@@ -4,12 +4,14 @@ mixed precision against plain FP64 MG-PCG, at <= 1e-6 relative displacement erro
 Workload (one GPU): the 512x128x128 void block (25 610 499 DOF): seeded spherical voids (radii
 U[2, 8] voxels, 20 % void fraction, void stiffness 1e-3), E = 70 GPa, nu = 0.33, x = 0 clamped,
 -z pressure of 5 MPa on the top face over x > 200 mm, 6-level geometric multigrid. Mixed precision:
-FP32 operator, FP32 V-cycle (Chebyshev smoothing, FP64 coarsest solve), FP64 dot products, FP64
-x and r with the true residual recomputed in FP64 at least every 20 iterations.
+FP32 operator, FP32 V-cycle (Chebyshev smoothing, FP64 coarsest solve), FP32 CG recurrences with FP64
+dot products, the FP64 solution and the true residual recomputed in FP64 at least every 20
+iterations. Both solves run to the reference solver's default tolerance, relative residual 1e-6.
 
 A step is one complete solve from zero with the vectors resident in HBM (the hierarchy is built
 once, as a served part's is). The FP64 reference solution is a plain FP64 solve to 1e-11 on the
-GPU; every timed solve's relative displacement error against it is checked (<= 1e-6) and reported.
+GPU (it levels off near 5e-10); every timed solve's relative displacement error against it is
+checked (<= 1e-6) and reported.
 The CPU baseline times the reference's own multithreaded solver (oracle/_ref, compiled from its
 sources) on the host cores over a bounded sample of the same solve.
 
@@ -171,7 +173,7 @@ def batched(args, P, W):
     d_u = P.DeviceArray(F.size)
     m.solve_multi_device(k, d_f, d_u, tol=1e-11, max_iter=3000, precision="fp64")
     U_ref = d_u.download().reshape(k, n)
-    tol = args.tol if args.tol != 1e-8 else 1e-6
+    tol = args.tol
 
     def errs(U):
         return [float(np.linalg.norm(U[c] - U_ref[c]) / np.linalg.norm(U_ref[c])) for c in range(k)]
@@ -231,7 +233,7 @@ def main():
     ap.add_argument("--steps", type=int, default=8)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--dims", default="512,128,128")
-    ap.add_argument("--tol", type=float, default=1e-8)
+    ap.add_argument("--tol", type=float, default=1e-6, help="relative residual tolerance (the reference solver's default, voxel_fea.hpp:113)")
     ap.add_argument("--degree", type=int, default=2, help="Chebyshev smoother degree")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-iterations", type=int, default=12)

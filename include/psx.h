@@ -147,6 +147,16 @@ int psx_solve_multi(psx_model m, int nrhs, const double* f, const double* x0, co
 int psx_solve_multi_device(psx_model m, int nrhs, const double* d_f, const double* d_x0, const psx_solve_opts* opts, double* d_u,
                            psx_solve_info* info);
 
+/* Grid enclosing a box with n voxels along its longest side plus pad voxels each side (gridForBox,
+ * voxelize.cpp:14-23): origin, voxel size h and dims. */
+int psx_grid_for_box(const double mn[3], const double mx[3], int n, int pad, double origin[3], double* h, int dims[3]);
+/* Filled fraction per voxel of a triangle mesh (voxelize, voxelize.cpp:25-128; the same fractions bit
+ * for bit): pos = 3 floats per vertex, index = 3 vertex indices per triangle, sub^3 samples per voxel,
+ * each solid when at least two of its three axis rays find it inside (even-odd). frac has
+ * dims[0] * dims[1] * dims[2] entries, index i + nx (j + ny k). Runs on the device. */
+int psx_voxelize(const float* pos, int64_t n_vertices, const uint32_t* index, int64_t n_tris, const double origin[3], double h, const int dims[3], int sub,
+                 float* frac);
+
 /* Support reactions: sum over held active DOFs of (K u)_raw - f (VoxelFEA::reactions,
  * voxel_fea.cpp:929-940). f may be NULL. */
 int psx_reactions(psx_model m, const double* u, const double* f, double out[3]);

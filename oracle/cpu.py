@@ -58,6 +58,8 @@ def lib(kind):
         "prune_floating": (C.c_int, [C.c_int, C.c_int, C.c_int, _f32p, _u8p, _f32p]),
         "tridiagonal_max": (dbl, [_f64p, C.c_int, _f64p]),
         "cheb_coefficients": (None, [dbl, _f64p]),
+        "grid_for_box": (None, [_f64p, _f64p, C.c_int, C.c_int, _f64p, C.POINTER(C.c_double), C.POINTER(C.c_int)]),
+        "voxelize": (C.c_int, [_f32p, i64, vp, i64, _f64p, dbl, C.POINTER(C.c_int), C.c_int, _f32p]),
     }
     if kind == "ref":
         sig["timed_solve"] = (dbl, [C.c_int, C.c_int, C.c_int, _f32p, dbl, _u8p, i64, _f64p, dbl, C.c_int, _f64p,
@@ -236,6 +238,29 @@ def prune_floating(dims, density, held, kind="orc"):
     out = np.zeros_like(density)
     removed = getattr(L, kind + "_prune_floating")(*[int(d) for d in dims], density, np.ascontiguousarray(held, np.uint8), out)
     return out, removed
+
+
+def grid_for_box(mn, mx, n, pad=0, kind="orc"):
+    """gridForBox (voxelize.cpp:14-23): (origin, h, dims)."""
+    origin = np.zeros(3)
+    h = C.c_double()
+    dims = (C.c_int * 3)()
+    getattr(lib(kind), kind + "_grid_for_box")(np.ascontiguousarray(mn, np.float64), np.ascontiguousarray(mx, np.float64), int(n), int(pad), origin,
+                                                C.byref(h), dims)
+    return origin, h.value, tuple(dims)
+
+
+def voxelize(pos, index, origin, h, dims, sub, kind="orc"):
+    """Filled fraction per voxel (voxelize.cpp:25-128), float32 nx * ny * nz."""
+    pos = np.ascontiguousarray(pos, np.float32).reshape(-1)
+    index = np.ascontiguousarray(index, np.uint32).reshape(-1)
+    frac = np.zeros(int(dims[0]) * int(dims[1]) * int(dims[2]), np.float32)
+    d = (C.c_int * 3)(*[int(v) for v in dims])
+    r = getattr(lib(kind), kind + "_voxelize")(pos, pos.size // 3, index.ctypes.data_as(C.c_void_p), index.size // 3,
+                                              np.ascontiguousarray(origin, np.float64), float(h), d, int(sub), frac)
+    if r != 0:
+        raise ValueError(getattr(lib(kind), kind + "_last_error")().decode())
+    return frac
 
 
 def principal(s, kind="orc"):

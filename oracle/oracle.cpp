@@ -22,6 +22,8 @@
 //                      bitwise u + iteration count vs oracle/_ref and the golden fixtures
 //   stresses           patch test (linear u -> constant sigma); M c / I; vs oracle/_ref
 //   prune_floating     known-answer 4x1x1 case (test_fea.cpp:115-122); brute force BFS; vs _ref
+//   voxelize           sphere volume within 1 % (test_model.cpp:52-77); an axis-aligned box's exact
+//                      fractions (closed form); point-in-mesh brute force per sample; vs _ref
 #include <algorithm>
 #include <array>
 #include <cmath>
@@ -1303,6 +1305,91 @@ static int pruneFloating(int nx, int ny, int nz, const float* density, const uin
     return removed;
 }
 
+// voxelize.cpp:14-23 (gridForBox): voxel size from the longest side, dims rounded up, box centred
+static void gridForBox(const double* mn, const double* mx, int n, int pad, double* origin, double* h, int* dims) {
+    const double size[3] = {mx[0] - mn[0], mx[1] - mn[1], mx[2] - mn[2]};
+    *h = std::max({size[0], size[1], size[2]}) / n;
+    for (int d = 0; d < 3; d++) {
+        dims[d] = std::max(1, int(std::ceil(size[d] / *h - 1e-6))) + 2 * pad;
+        origin[d] = mn[d] - (dims[d] * *h - size[d]) / 2;
+    }
+}
+
+// voxelize.cpp:25-128: sub^3 samples per voxel; along each axis a, rays through the sample centres
+// of the other two axes (offset by a small jitter) collect the depths where they cross triangles
+// (barycentric test in the triangle's projection, depth interpolated, stored as float), the sorted
+// depths pair up into inside spans (even-odd), and each sample inside a span gets a vote; a sample
+// is solid with at least two of its three votes, a voxel's fraction is its solid samples / sub^3.
+// Serial here: the reference's bands only partition rays, and every ray's hits are sorted before use.
+static void voxelize(const float* pos, const uint32_t* index, i64 nTri, const double* origin, double h, const int* dims, int sub, float* frac) {
+    const int S[3] = {dims[0] * sub, dims[1] * sub, dims[2] * sub};
+    const double ds = h / sub;
+    std::vector<uint8_t> votes(size_t(S[0]) * S[1] * S[2], 0);
+    const i64 strides[3] = {1, S[0], i64(S[0]) * S[1]};
+    const double jitter[3] = {0.000137 * ds, 0.000291 * ds, 0.000213 * ds};
+    for (int a = 0; a < 3; a++) {
+        const int u = (a + 1) % 3, v = (a + 2) % 3;
+        const int Nu = S[u], Nv = S[v], Na = S[a];
+        const double ou = origin[u] + jitter[u], ov = origin[v] + jitter[v];
+        std::vector<std::vector<float>> hits(size_t(Nu) * Nv);
+        for (i64 t = 0; t < nTri; t++) {
+            const size_t i0 = 3 * size_t(index[3 * t]), i1 = 3 * size_t(index[3 * t + 1]), i2 = 3 * size_t(index[3 * t + 2]);
+            const double av = pos[i0 + v], bv = pos[i1 + v], cv = pos[i2 + v];
+            const int sv0 = std::max(0, int(std::ceil((std::min({av, bv, cv}) - ov) / ds - 0.5)));
+            const int sv1 = std::min(Nv - 1, int(std::floor((std::max({av, bv, cv}) - ov) / ds - 0.5)));
+            if (sv0 > sv1) continue;
+            const double au = pos[i0 + u], aa = pos[i0 + a];
+            const double bu = pos[i1 + u], ba = pos[i1 + a];
+            const double cu = pos[i2 + u], ca = pos[i2 + a];
+            const double det = (bv - cv) * (au - cu) + (cu - bu) * (av - cv);
+            if (det == 0) continue;
+            const int su0 = std::max(0, int(std::ceil((std::min({au, bu, cu}) - ou) / ds - 0.5)));
+            const int su1 = std::min(Nu - 1, int(std::floor((std::max({au, bu, cu}) - ou) / ds - 0.5)));
+            if (su0 > su1) continue;
+            const double inv = 1 / det;
+            for (int sv = sv0; sv <= sv1; sv++) {
+                const double pv = ov + (sv + 0.5) * ds;
+                for (int su = su0; su <= su1; su++) {
+                    const double pu = ou + (su + 0.5) * ds;
+                    const double l1 = ((bv - cv) * (pu - cu) + (cu - bu) * (pv - cv)) * inv;
+                    if (l1 < 0) continue;
+                    const double l2 = ((cv - av) * (pu - cu) + (au - cu) * (pv - cv)) * inv;
+                    if (l2 < 0) continue;
+                    const double l3 = 1 - l1 - l2;
+                    if (l3 < 0) continue;
+                    hits[size_t(su) + size_t(Nu) * sv].push_back(float(l1 * aa + l2 * ba + l3 * ca));
+                }
+            }
+        }
+        const double oa = origin[a];
+        for (int sv = 0; sv < Nv; sv++)
+            for (int su = 0; su < Nu; su++) {
+                auto& hs = hits[size_t(su) + size_t(Nu) * sv];
+                if (hs.size() < 2) continue;
+                std::sort(hs.begin(), hs.end());
+                const i64 base = su * strides[u] + sv * strides[v];
+                for (size_t q = 0; q + 1 < hs.size(); q += 2) {
+                    const int s0 = std::max(0, int(std::ceil((hs[q] - oa) / ds - 0.5)));
+                    const int s1 = std::min(Na - 1, int(std::floor((hs[q + 1] - oa) / ds - 0.5)));
+                    for (int sa = s0; sa <= s1; sa++) votes[base + sa * strides[a]]++;
+                }
+            }
+    }
+    const int nx = dims[0], ny = dims[1], nz = dims[2];
+    const double inv = 1.0 / (sub * sub * sub);
+    for (i64 k = 0; k < nz; k++)
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++) {
+                int c = 0;
+                for (int dk = 0; dk < sub; dk++)
+                    for (int dj = 0; dj < sub; dj++) {
+                        const i64 row = i64(i) * sub + S[0] * (i64(j) * sub + dj + i64(S[1]) * (k * sub + dk));
+                        for (int di = 0; di < sub; di++) c += votes[row + di] >= 2;
+                    }
+                frac[i + size_t(nx) * (j + size_t(ny) * k)] = float(c * inv);
+            }
+}
+
 }  // namespace orc
 
 // ---------------------------------------------------------------------------------------------
@@ -1448,6 +1535,15 @@ void orc_smooth(void* h, int l, const float* r, float* z) {
     std::copy(r, r + L.nDof, L.r.begin());
     orc::chebyshev(L, true, M->chebDegree);
     std::copy(L.z.begin(), L.z.end(), z);
+}
+
+void orc_grid_for_box(const double* mn, const double* mx, int n, int pad, double* origin, double* h, int* dims) {
+    orc::gridForBox(mn, mx, n, pad, origin, h, dims);
+}
+int orc_voxelize(const float* pos, int64_t nVert, const uint32_t* index, int64_t nTri, const double* origin, double h, const int* dims, int sub, float* frac) {
+    (void)nVert;
+    orc::voxelize(pos, index, nTri, origin, h, dims, sub, frac);
+    return 0;
 }
 
 void orc_cheb_coefficients(double lmax, double* out3) {

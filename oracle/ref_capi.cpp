@@ -1,7 +1,8 @@
 // ORACLE — test infrastructure only, never the product.
 //
 // C ABI over the reference's own VoxelFEA, compiled from the reference sources where they lie
-// (oracle/Makefile: native/src/fea/hex8.cpp, native/src/fea/voxel_fea.cpp, native/src/util/parallel.cpp)
+// (oracle/Makefile: native/src/fea/hex8.cpp, native/src/fea/voxel_fea.cpp, native/src/util/parallel.cpp,
+// and for the voxelizer native/src/core/voxelize.cpp with the mesh and BVH sources it links against)
 // into oracle/_ref/libpsref.so. The tests use it to pin the restatement in oracle/oracle.cpp bit for
 // bit, and bench.py may time it as the CPU baseline. The entry points have the same shapes as
 // oracle.cpp's orc_* functions.
@@ -26,6 +27,7 @@
 #include "fea/voxel_fea.hpp"
 #undef private
 #include "fea/voxel_fea.cpp"
+#include "core/voxelize.hpp"
 
 static std::string g_err;
 
@@ -231,5 +233,33 @@ int ref_timed_phases(int nx, int ny, int nz, const float* density, double nu, co
 }
 
 unsigned ref_threads() { return ps::ThreadPool::instance().size(); }
+
+void ref_grid_for_box(const double* mn, const double* mx, int n, int pad, double* origin, double* h, int* dims) {
+    const ps::Grid g = ps::gridForBox({mn[0], mn[1], mn[2]}, {mx[0], mx[1], mx[2]}, n, pad);
+    for (int d = 0; d < 3; d++) {
+        origin[d] = g.origin[d];
+        dims[d] = g.dims[d];
+    }
+    *h = g.h;
+}
+
+int ref_voxelize(const float* pos, int64_t nVert, const uint32_t* index, int64_t nTri, const double* origin, double h, const int* dims, int sub, float* frac) {
+    try {
+        ps::Grid g;
+        for (int d = 0; d < 3; d++) {
+            g.origin[d] = origin[d];
+            g.dims[d] = dims[d];
+        }
+        g.h = h;
+        const std::vector<float> p(pos, pos + 3 * nVert);
+        const std::vector<uint32_t> ix(index, index + 3 * nTri);
+        const auto f = ps::voxelize(p, ix, g, sub);
+        std::memcpy(frac, f.data(), f.size() * sizeof(float));
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+}
 
 }  // extern "C"

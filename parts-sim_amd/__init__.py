@@ -96,6 +96,8 @@ SIGNATURES = {
     "psx_server_pending": (_i, [_vp]),
     "psx_server_destroy": (None, [_vp]),
     "psx_prune_floating": (_i, [_i, _i, _i, _vp, _vp, _vp]),
+    "psx_grid_for_box": (_i, [_vp, _vp, _i, _i, _vp, _vp, _vp]),
+    "psx_voxelize": (_i, [_vp, C.c_int64, _vp, C.c_int64, _vp, C.c_double, _vp, _i, _vp]),
     "psx_hex_element": (_i, [_d, _vp, _vp]),
     "psx_hierarchy_create": (_i, [C.POINTER(ModelDesc), C.POINTER(_vp)]),
     "psx_hierarchy_destroy": (None, [_vp]),
@@ -175,6 +177,30 @@ def prune_floating(dims, density, held):
     if r < 0:
         raise PsxError(-r, lib().psx_last_error().decode())
     return out, r
+
+
+def grid_for_box(mn, mx, n, pad=0):
+    """Grid enclosing the box [mn, mx] with n voxels along its longest side (gridForBox): (origin, h, dims)."""
+    mn = np.ascontiguousarray(mn, np.float64)
+    mx = np.ascontiguousarray(mx, np.float64)
+    origin = np.zeros(3)
+    h = C.c_double()
+    dims = np.zeros(3, np.int32)
+    _check(lib().psx_grid_for_box(_p(mn), _p(mx), int(n), int(pad), _p(origin), C.byref(h), _p(dims)))
+    return origin, h.value, tuple(int(v) for v in dims)
+
+
+def voxelize(pos, index, origin, h, dims, sub=2):
+    """Filled fraction per voxel of a triangle mesh, computed on the device (float32, i + nx (j + ny k))."""
+    pos = np.ascontiguousarray(pos, np.float32).reshape(-1)
+    index = np.ascontiguousarray(index, np.uint32).reshape(-1)
+    if pos.size % 3 or index.size % 3:
+        raise ValueError("Positions and triangle indices come in threes.")
+    d = np.ascontiguousarray([int(v) for v in dims], np.int32)
+    o = np.ascontiguousarray(origin, np.float64)
+    frac = np.zeros(int(d[0]) * int(d[1]) * int(d[2]), np.float32)
+    _check(lib().psx_voxelize(_p(pos), pos.size // 3, _p(index), index.size // 3, _p(o), float(h), _p(d), int(sub), _p(frac)))
+    return frac
 
 
 def hex_element(nu):

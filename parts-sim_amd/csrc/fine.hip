@@ -121,8 +121,14 @@ template <class T, int MODE, bool OWNC, bool DOT, bool DENSE = false> __device__
     const int per = gridDim.x / 8, t = xcdOrder ? int(blockIdx.x % 8) * per + int(blockIdx.x / 8) : int(blockIdx.x);
     if (t >= nTiles) return;
     __shared__ Ring<T, OWNC> sh;
-    const int tid = threadIdx.x, tx = tid % TX, ty = tid / TX, lane = tid & 63, wid = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int i0 = (t % gx) * TX, j0 = ((t / gx) % gy) * TY;
+    // A wavefront holds two node rows of the tile, so in the last tile row of a part (NY = 2^k + 1
+    // leaves one row there) the waves past the part's edge hold no node and skip the arithmetic. The
+    // last tile column (usually one node column) takes the transposed mapping, 8 columns of 8 rows
+    // per wavefront, for the same effect.
+    const bool transposed = L.NX - i0 <= TX / 4;
+    const int tx = transposed ? tid / TY : tid % TX, ty = transposed ? tid % TY : tid / TX;
     const int z0 = (t / (gx * gy)) * zChunk, z1 = min(L.NZ, z0 + zChunk);
     const int i = i0 + tx, j = j0 + ty;
     const bool mine = i < L.NX && j < L.NY;
@@ -244,8 +250,10 @@ template <class T, int MODE, bool OWNC, bool DOT, bool DENSE = false> __device__
         fetchOwn(k, nxt);
 
         // stencil rows fed by plane k: node k + 1, node k and node k - 1
+        // threads without a node of their own (the tiles past a part's last node column / row) skip
+        // the arithmetic: whole waves of them then issue only the plane loads and barriers
         T xm[9][3];
-        {
+        if (mine) {
             const int sl = k & 3;
 #pragma unroll
             for (int q = 0; q < 9; q++)
@@ -262,7 +270,7 @@ template <class T, int MODE, bool OWNC, bool DOT, bool DENSE = false> __device__
         // P (x(z - 1) - 2 x(z) + x(z + 1)), P = the diagonal sum of a neighbour plane's blocks. A
         // rigid translation then cancels exactly and a rotation to element scale, which keeps the
         // FP32 operator accurate on the smooth, low-energy modes the conjugate gradients resolve last.
-        if constexpr (!DENSE) {
+        if (!DENSE && mine) {
             const T D00 = C.v[14], D01 = C.v[15], D02 = C.v[16], D10 = C.v[17], D11 = C.v[18], D12 = C.v[19], E0 = C.v[20], E1 = C.v[21];
             (void)D00;
             // q = (ox + 1) + 3 (oy + 1)
@@ -323,7 +331,7 @@ template <class T, int MODE, bool OWNC, bool DOT, bool DENSE = false> __device__
             while (m) {
                 const int v = __builtin_ctz(m);
                 m &= m - 1;
-                sh.list[wid][at++] = uint16_t((tid << 6) | (ref << 3) | v);
+                sh.list[wid][at++] = uint16_t(((ty * TX + tx) << 6) | (ref << 3) | v);
             }
         }
         if (!DENSE && lane == 0) sh.waveTot[k & 1][wid] = __popcll(b0) + 2 * __popcll(b1) + 4 * __popcll(b2) + 8 * __popcll(b3);
@@ -339,14 +347,14 @@ template <class T, int MODE, bool OWNC, bool DOT, bool DENSE = false> __device__
             }
             const int sm = (k + 2) & 3, s0 = (k + 3) & 3, sp = k & 3;  // node planes k - 2, k - 1, k
             const int vm = (k + 1) % 3, v0 = (k + 2) % 3;              // voxel planes k - 2, k - 1
-            T out[3];
+            T out[3] = {T(0), T(0), T(0)};
             if constexpr (DENSE) {
+              if (mine) {
                 // s_v E_v (x - x_node) over the node's 8 voxels; voxel v = (di, dj, dk) lies at node
                 // offset (di - 1, dj - 1, dk - 1) and has the node as its corner a = (1 - di, 1 - dj, 1 - dk)
                 T xn[3];
 #pragma unroll
                 for (int c = 0; c < 3; c++) xn[c] = sh.x[s0][c][ty + 1][tx + 1];
-                out[0] = out[1] = out[2] = T(0);
                 staticFor<8>([&](auto V) {
                     constexpr int v = decltype(V)::value, di = v & 1, dj = (v >> 1) & 1, dk = v >> 2;
                     constexpr int A = cornerOf(1 - di, 1 - dj, 1 - dk);
@@ -368,6 +376,7 @@ template <class T, int MODE, bool OWNC, bool DOT, bool DENSE = false> __device__
 #pragma unroll
                     for (int d = 0; d < 3; d++) out[d] += sv * acc[d];
                 });
+              }
             } else {
                 // the centre column's second difference through the neighbour planes' diagonal sums
                 const T Pxy = C.v[15] + T(2) * (C.v[18] + C.v[16]) + T(4) * C.v[19], Pz = C.v[17] + T(4) * (C.v[18] + C.v[19]);
@@ -688,6 +697,9 @@ unsigned launchFineDot(const DevLevel& L, int mode, const T* x, T* y, const T* r
             break;
         }
     if (const char* e = std::getenv("PSX_ZCHUNK")) zChunk = std::max(2, std::atoi(e));
+    // (measured: near-equal chunks of 8-9 planes, which drop the one-plane last chunk of NZ = 2^k + 1,
+    // are 4 % slower on the 512x128x128 block than this rule's 8-plane chunks)
+    const int gz = (L.NZ + zChunk - 1) / zChunk;
     const bool fused = (fd.kind == FD_PQ && mode == AM_APPLY) || (fd.kind == FD_RZ && (mode == AM_CHEB1 || mode == AM_CHEB2));
     double* dots = fused ? fd.red.partials : nullptr;
     const T* qdot = fused && fd.kind == FD_RZ ? static_cast<const T*>(fd.q) : nullptr;
@@ -699,7 +711,7 @@ unsigned launchFineDot(const DevLevel& L, int mode, const T* x, T* y, const T* r
     } else {
         launchFineT<T, false>(L, mode, C, nullptr, nullptr, x, y, r, d, c1, c2, stop, zChunk, dots, fd.red.stride, qdot, s, ninst);
     }
-    return fused ? unsigned(gx * gy * ((L.NZ + zChunk - 1) / zChunk)) : 0u;
+    return fused ? unsigned(gx * gy * gz) : 0u;
 }
 
 template <class T>

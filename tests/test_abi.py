@@ -45,3 +45,15 @@ def test_server_without_device_raises(P):
     with pytest.raises(P.PsxError) as e:
         P.Server(max_inflight=2)
     assert e.value.code == P.ERR_NO_DEVICE
+
+
+def test_grid_for_box_matches_oracle(P, cpu):
+    import numpy as np
+
+    import meshes
+    for pos in (meshes.uv_sphere()[0], meshes.torus()[0], meshes.box((-3, -2, -1), (5, 2.5, 1.7))[0]):
+        mn, mx = meshes.bounds(pos)
+        for n, pad in ((10, 0), (37, 2), (128, 1)):
+            o, h, d = P.grid_for_box(mn, mx, n, pad)
+            oo, ho, do = cpu.grid_for_box(mn, mx, n, pad)
+            assert np.array_equal(o, oo) and h == ho and d == do

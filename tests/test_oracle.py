@@ -384,3 +384,75 @@ def test_bitwise_against_compiled_reference(cpu, W):
         assert np.array_equal(a[0]["u"], b[0]["u"]) and a[0]["iterations"] == b[0]["iterations"]
         assert all(np.array_equal(p, q) for p, q in zip(a[1], b[1]))
         assert np.array_equal(a[2], b[2]) and a[3] == b[3]
+
+
+# ---- voxelize (voxelize.cpp:14-128) ----
+
+import meshes
+
+
+def _octahedron_fractions(R, c, origin, h, dims, sub):
+    """Closed form: a sample is solid iff |p - c|_1 < R; None when a sample sits within 1e-6 of the
+    surface (the rays' jitter then decides)."""
+    nx, ny, nz = dims
+    ds = h / sub
+    ax = [origin[d] + (np.arange(dims[d] * sub) + 0.5) * ds - c[d] for d in range(3)]
+    X, Y, Z = np.meshgrid(np.abs(ax[0]), np.abs(ax[1]), np.abs(ax[2]), indexing="ij")
+    L1 = X + Y + Z
+    if np.any(np.abs(L1 - R) < 1e-6):
+        return None
+    solid = (L1 < R).astype(np.int32)
+    f = solid.reshape(nx, sub, ny, sub, nz, sub).sum(axis=(1, 3, 5)) / sub ** 3
+    return f.transpose(2, 1, 0).reshape(-1).astype(np.float32)
+
+
+def test_voxelize_sphere_volume(cpu):
+    # test_model.cpp:52-77: the voxel volume of the UV sphere at 40 voxels, sub 4, within 1 % of the
+    # mesh volume
+    pos, idx = meshes.uv_sphere()
+    o, h, dims = cpu.grid_for_box(*meshes.bounds(pos), 40)
+    f = cpu.voxelize(pos, idx, o, h, dims, 4)
+    p = pos.reshape(-1, 3, 3).astype(np.float64)
+    vol = abs(np.einsum("ij,ij->i", p[:, 0], np.cross(p[:, 1], p[:, 2])).sum()) / 6
+    assert abs(f.sum() * h ** 3 / vol - 1) < 0.01
+
+
+def test_voxelize_box_closed_form(cpu):
+    # a box on voxel boundaries fills its voxels exactly; shifted by half a voxel, boundary voxels
+    # are half, quarter or eighth full
+    pos, idx = meshes.box((0, 0, 0), (10, 6, 4))
+    o, h, dims = cpu.grid_for_box(*meshes.bounds(pos), 10)
+    assert h == 1 and dims == (10, 6, 4) and list(o) == [0, 0, 0]
+    for sub in (2, 3, 4):
+        assert np.all(cpu.voxelize(pos, idx, o, h, dims, sub) == 1.0)
+    o2 = np.array([-0.5, -0.5, -0.5])
+    f = cpu.voxelize(pos, idx, o2, 1.0, (11, 7, 5), 2).reshape(5, 7, 11)
+    expect = np.ones((5, 7, 11))
+    for ax in range(3):
+        sl = [slice(None)] * 3
+        for e in (0, -1):
+            sl[ax] = e
+            expect[tuple(sl)] *= 0.5
+    assert np.array_equal(f, expect.astype(np.float32))
+
+
+@pytest.mark.parametrize("sub", [2, 3, 5])
+def test_voxelize_octahedron_closed_form(cpu, sub):
+    R, c = 7.3, (0.21, -0.37, 0.13)
+    pos, idx = meshes.octahedron(R, c)
+    o, h, dims = cpu.grid_for_box(*meshes.bounds(pos), 12, pad=1)
+    expect = _octahedron_fractions(R, c, o, h, dims, sub)
+    assert expect is not None
+    assert np.array_equal(cpu.voxelize(pos, idx, o, h, dims, sub), expect)
+
+
+@pytest.mark.skipif(not ref_available(), reason="oracle/_ref not built")
+@pytest.mark.parametrize("mesh", ["sphere", "torus", "soup", "box"])
+def test_voxelize_bitwise_vs_reference(cpu, mesh):
+    pos, idx = {"sphere": meshes.uv_sphere, "torus": meshes.torus, "soup": meshes.soup, "box": lambda: meshes.box((-3, -2, -1), (5, 2.5, 1.7))}[mesh]()
+    for n, sub, pad in ((24, 2, 0), (17, 3, 1), (30, 5, 2)):
+        mn, mx = meshes.bounds(pos)
+        g = cpu.grid_for_box(mn, mx, n, pad)
+        gr = cpu.grid_for_box(mn, mx, n, pad, kind="ref")
+        assert np.array_equal(g[0], gr[0]) and g[1] == gr[1] and g[2] == gr[2]
+        assert np.array_equal(cpu.voxelize(pos, idx, *g, sub), cpu.voxelize(pos, idx, *g, sub, kind="ref"))
