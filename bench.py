@@ -235,6 +235,7 @@ def main():
     ap.add_argument("--dims", default="512,128,128")
     ap.add_argument("--tol", type=float, default=1e-6, help="relative residual tolerance (the reference solver's default, voxel_fea.hpp:113)")
     ap.add_argument("--degree", type=int, default=2, help="Chebyshev smoother degree")
+    ap.add_argument("--pre-degree", type=int, default=0, help="Chebyshev degree of the pre-smoothing (0 = --degree)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-iterations", type=int, default=12)
     ap.add_argument("--serve", type=int, default=0, help="serving workload: this many requests of the Poisson stream")
@@ -298,9 +299,9 @@ def main():
 
     # timed region: barrier-equivalent (the device is idle between calls: every solve ends on a
     # host read of its final residual) and one stream
-    lat64, err64, its64, info64 = run("fp64", max(3, args.steps // 2), 1, cheb_degree=args.degree)
+    lat64, err64, its64, info64 = run("fp64", max(3, args.steps // 2), 1, cheb_degree=args.degree, pre_degree=args.pre_degree)
     log("FP64 MG-PCG: %.2f ms/solve, %d its, err %.2e" % (1e3 * lat64.mean(), its64[0], err64.max()))
-    lat, err, its, info = run("mixed", args.steps, args.warmup, cheb_degree=args.degree)
+    lat, err, its, info = run("mixed", args.steps, args.warmup, cheb_degree=args.degree, pre_degree=args.pre_degree)
     log("mixed: %.2f ms/solve (p50 %.2f p99 %.2f), %d its, %d folds, err %.2e" % (1e3 * lat.mean(), 1e3 * np.percentile(lat, 50),
                                                                                    1e3 * np.percentile(lat, 99), its[0], info["folds"], err.max()))
     solves_per_s = len(lat) / lat.sum()

@@ -73,6 +73,8 @@ typedef struct {
     int mfma_smoother;          /* PSX_PREC_MIXED: smooth with the BF16 matrix-core operator (0 = FP32) */
     int eager;                  /* 1: launch every kernel from the host; 0: replay a captured HIP graph of
                                    check_every iterations (the same kernels and arguments) */
+    int pre_degree;             /* Chebyshev degree of the pre-smoothing steps (0 = cheb_degree); the
+                                   post-smoothing keeps cheb_degree */
 } psx_solve_opts;
 
 /* Replaces SolveResult's scalars (voxel_fea.hpp:119-124). */

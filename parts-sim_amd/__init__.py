@@ -34,7 +34,7 @@ PROGRESS = C.CFUNCTYPE(C.c_int, C.c_int, C.c_double, C.c_void_p)
 class SolveOpts(C.Structure):
     _fields_ = [("tol", C.c_double), ("max_iter", C.c_int), ("precision", C.c_int), ("check_every", C.c_int),
                 ("fold_every", C.c_int), ("on_progress", PROGRESS), ("user", C.c_void_p), ("cheb_degree", C.c_int),
-                ("mfma_smoother", C.c_int), ("eager", C.c_int)]
+                ("mfma_smoother", C.c_int), ("eager", C.c_int), ("pre_degree", C.c_int)]
 
 
 class ServerOpts(C.Structure):
@@ -323,10 +323,10 @@ class Model:
         return z
 
     @staticmethod
-    def _opts(tol, max_iter, precision, check_every, fold_every, on_progress, cheb_degree=0, mfma_smoother=False, eager=False):
+    def _opts(tol, max_iter, precision, check_every, fold_every, on_progress, cheb_degree=0, mfma_smoother=False, eager=False, pre_degree=0):
         cb = PROGRESS(0) if on_progress is None else PROGRESS(lambda it, rel, user: int(bool(on_progress(it, rel))))
         return SolveOpts(float(tol), int(max_iter), PREC[precision], int(check_every), int(fold_every), cb, None, int(cheb_degree),
-                         int(bool(mfma_smoother)), int(bool(eager))), cb
+                         int(bool(mfma_smoother)), int(bool(eager)), int(pre_degree)), cb
 
     @staticmethod
     def _info(info):
@@ -334,13 +334,13 @@ class Model:
                 "cancelled": bool(info.cancelled), "broken": bool(info.broken), "folds": info.folds, "device_ms": info.device_ms}
 
     def solve(self, f, tol=1e-6, max_iter=500, x0=None, precision="fp64", check_every=0, fold_every=0, on_progress=None, cheb_degree=0,
-              mfma_smoother=False, eager=False):
+              mfma_smoother=False, eager=False, pre_degree=0):
         f = np.ascontiguousarray(f, np.float64)
         if f.size != self.n_dof or (x0 is not None and np.size(x0) != self.n_dof):
             raise ValueError("Force or displacement vector has the wrong size.")
         x0a = None if x0 is None else np.ascontiguousarray(x0, np.float64)
         u = np.zeros_like(f)
-        o, cb = self._opts(tol, max_iter, precision, check_every, fold_every, on_progress, cheb_degree, mfma_smoother, eager)
+        o, cb = self._opts(tol, max_iter, precision, check_every, fold_every, on_progress, cheb_degree, mfma_smoother, eager, pre_degree)
         info = SolveInfo()
         _check(lib().psx_solve(self.h, _p(f), _p(x0a), C.byref(o), _p(u), C.byref(info)))
         out = self._info(info)
@@ -348,9 +348,9 @@ class Model:
         return out
 
     def solve_device(self, d_f, d_u, d_x0=None, tol=1e-6, max_iter=500, precision="fp64", check_every=0, fold_every=0, cheb_degree=0,
-                     mfma_smoother=False, eager=False):
+                     mfma_smoother=False, eager=False, pre_degree=0):
         """Solve with HBM-resident vectors (DeviceArray or raw device addresses)."""
-        o, cb = self._opts(tol, max_iter, precision, check_every, fold_every, None, cheb_degree, mfma_smoother, eager)
+        o, cb = self._opts(tol, max_iter, precision, check_every, fold_every, None, cheb_degree, mfma_smoother, eager, pre_degree)
         info = SolveInfo()
         addr = lambda a: None if a is None else C.c_void_p(int(a))
         _check(lib().psx_solve_device(self.h, addr(d_f), addr(d_x0), C.byref(o), addr(d_u), C.byref(info)))

@@ -98,8 +98,10 @@ template <class T> void launchApplyOwn(const DevLevel& L, int mode, const T* x, 
                                        hipStream_t s, int ninst = 1);
 // the same for levels whose elements are s * Kb, or own elements in child-scale form: z-marching
 // LDS planes, class-sum stencil in difference form + one lane per differing voxel (fine.hip)
+// mf: FP32 levels without own elements take the in-plane stencil on the matrix cores (BF16 operands,
+// FP32 accumulation; for the V-cycle's smoothing steps)
 template <class T> void launchFine(const DevLevel& L, int mode, const T* x, T* y, const T* r, const T* zp, double c1, double c2, const int* stop,
-                                   hipStream_t s, int ninst = 1);
+                                   hipStream_t s, int ninst = 1, bool mf = false);
 // Dot products fused into an operator launch (conjugate gradients), block partials into red:
 //   FD_PQ  (AM_APPLY)  p.q with p = x, q = y                         -> launchFinishPQ
 //   FD_RZ  (AM_CHEB1/2) r.z and z.q with z = y, q = qdot             -> launchFinishRZ
@@ -111,7 +113,7 @@ struct DotFusion {
 };
 // launchFine with a fused dot; returns the number of block partials written (0: not fused)
 template <class T> unsigned launchFineDot(const DevLevel& L, int mode, const T* x, T* y, const T* r, const T* zp, double c1, double c2, const int* stop,
-                                          const DotFusion& fd, hipStream_t s, int ninst = 1);
+                                          const DotFusion& fd, hipStream_t s, int ninst = 1, bool mf = false);
 void launchFinishPQ(Reduce red, unsigned nb, CgState* st, hipStream_t s, int ninst = 1);
 void launchFinishRZ(Reduce red, unsigned nb, CgState* st, hipStream_t s, int ninst = 1);
 // the same in BF16 on the matrix cores, FP32 vectors (mfma.hip); Aop from mfmaOperands()

@@ -26,6 +26,9 @@ namespace psx {
 
 namespace {
 
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
 constexpr int TX = 32, TY = 8, NT = TX * TY;
 constexpr int PX = TX + 2, PY = TY + 2;  // node plane with halo
 constexpr int VX = TX + 1, VY = TY + 1;  // voxel plane
@@ -102,7 +105,15 @@ template <class T, bool OWNC> struct Ring {
 // elements): every node sums its 8 voxels, s_v E_v (x - x_node) with E_v the node's 3 rows of Kb in
 // voxel v (compile-time class coefficients), instead of the stencil and the pair lists; for per-voxel
 // densities, where nearly every node would list 7 or 8 pairs.
-template <class T, int MODE, bool OWNC, bool DOT, bool DENSE = false> __device__ __forceinline__ void fineBody(PSX_FINE_PARAMS) {
+// MF (FP32 levels without own elements, the V-cycle's smoothing steps): the plane's in-plane stencil
+// sum runs on the matrix cores, BF16 operands with FP32 accumulation. Per wavefront, 16 nodes form the
+// columns of one v_mfma_f32_16x16x32_bf16: B = their 8 in-plane neighbour differences (3 components +
+// a zero each: K = 32), A = the stencil blocks for the three node planes the input plane feeds (rows
+// 4 g + i: plane group g = node plane mod 3, component i). The accumulator tiles stay in registers for
+// the whole march; a node plane's rows are complete one step after its upper neighbour plane and are
+// read out by lane shuffles, then cleared for the plane three steps on. The centre column term and
+// the differing-voxel pairs stay FP32, as in the VALU form.
+template <class T, int MODE, bool OWNC, bool DOT, bool DENSE = false, bool MF = false> __device__ __forceinline__ void fineBody(PSX_FINE_PARAMS) {
     // multi-instance launches: blockIdx.y is the right-hand side (its vectors, stop flag, partials)
     if (stop) stop += int64_t(blockIdx.y) * CG_INTS;
     if (stop && __builtin_nontemporal_load(stop) != 0) return;
@@ -224,6 +235,36 @@ template <class T, int MODE, bool OWNC, bool DOT, bool DENSE = false> __device__
         o.info = in ? uint32_t((L.nodeInfo + pb)[col]) : 0u;
     };
 
+    // matrix-core operands of the in-plane stencil: A[rot] (this lane: row l % 16 = 4 g + i, K slice
+    // 8 (l / 16) .. + 7 = neighbours 2 (l / 16), 2 (l / 16) + 1 of the in-plane list, components j) for
+    // the input plane's offsets oz = 0 / -1 / +1 to the node planes of groups rot, rot + 1, rot + 2
+    bf16x8 Acoef[3];
+    static_assert(NT / 64 == 4, "four 16-node groups per wavefront");
+    floatx4 macc[4];
+    if constexpr (MF) {
+        const int row = lane & 15, g = row >> 2, ci = row & 3, kb = lane >> 4;
+#pragma unroll
+        for (int rot = 0; rot < 3; rot++) {
+            const int oz = g == rot ? 0 : (g == (rot + 1) % 3 ? -1 : 1);
+#pragma unroll
+            for (int t = 0; t < 8; t++) {
+                const int nb = 2 * kb + (t >> 2), j = t & 3;
+                const int q = nb < 4 ? nb : nb + 1;  // in-plane neighbour, the centre (q = 4) skipped
+                const int ox = q % 3 - 1, oy = q / 3 - 1;
+                const int m = (ox + 1) + 3 * (oy + 1) + 9 * (oz + 1);
+                const float v = (g < 3 && ci < 3 && j < 3) ? L.St32[m * 9 + ci * 3 + j] : 0.f;
+                Acoef[rot][t] = (__bf16)v;
+            }
+        }
+#pragma unroll
+        for (int G = 0; G < 4; G++) macc[G] = floatx4{0.f, 0.f, 0.f, 0.f};
+    }
+    // position in the tile of local node o under this block's thread mapping
+    auto posOf = [&](int o, int& px, int& py) {
+        px = transposed ? o / TY : o % TX;
+        py = transposed ? o % TY : o / TX;
+    };
+
     // prologue: node plane z0 - 1 in LDS; node plane z0 and voxel plane z0 - 2 in flight
     T pn[NPL][3];
     int32_t vn[NVL];
@@ -258,7 +299,42 @@ template <class T, int MODE, bool OWNC, bool DOT, bool DENSE = false> __device__
 #pragma unroll
             for (int q = 0; q < 9; q++)
 #pragma unroll
-                for (int c = 0; c < 3; c++) xm[q][c] = (!DENSE || q == 4) ? sh.x[sl][c][ty + q / 3][tx + q % 3] : T(0);
+                for (int c = 0; c < 3; c++) xm[q][c] = ((!DENSE && !MF) || q == 4) ? sh.x[sl][c][ty + q / 3][tx + q % 3] : T(0);
+        }
+        T mres[3] = {T(0), T(0), T(0)};  // MF: the in-plane stencil sums of node k - 1
+        if constexpr (MF) {
+            const int sl = k & 3, kb = lane >> 4, rot = ((k % 3) + 3) % 3;
+            if (__ballot(mine) != 0) {  // a wavefront without nodes (past the part's edge) skips
+#pragma unroll
+                for (int G = 0; G < 4; G++) {
+                    int px, py;
+                    posOf(64 * wid + 16 * G + (lane & 15), px, py);
+                    const float c0 = sh.x[sl][0][py + 1][px + 1], c1v = sh.x[sl][1][py + 1][px + 1], c2v = sh.x[sl][2][py + 1][px + 1];
+                    bf16x8 b;
+#pragma unroll
+                    for (int h = 0; h < 2; h++) {
+                        const int nb = 2 * kb + h, q = nb < 4 ? nb : nb + 1;
+                        const int ox = q % 3 - 1, oy = q / 3 - 1;
+                        b[4 * h] = (__bf16)(sh.x[sl][0][py + 1 + oy][px + 1 + ox] - c0);
+                        b[4 * h + 1] = (__bf16)(sh.x[sl][1][py + 1 + oy][px + 1 + ox] - c1v);
+                        b[4 * h + 2] = (__bf16)(sh.x[sl][2][py + 1 + oy][px + 1 + ox] - c2v);
+                        b[4 * h + 3] = (__bf16)0.f;
+                    }
+                    macc[G] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(rot == 0 ? Acoef[0] : (rot == 1 ? Acoef[1] : Acoef[2]), b, macc[G], 0, 0, 0);
+                }
+                // node plane k - 1 (group (k - 1) mod 3) is complete: lanes 16 g + n hold node n's rows;
+                // the owner of node 16 G + n is lane 16 G + n of this wavefront
+                const int gout = ((k - 1) % 3 + 3) % 3, src = 16 * gout + (lane & 15);
+#pragma unroll
+                for (int G = 0; G < 4; G++) {
+#pragma unroll
+                    for (int c = 0; c < 3; c++) {
+                        const float v = __shfl(macc[G][c], src, 64);
+                        if ((lane >> 4) == G) mres[c] = T(v);
+                    }
+                    if ((lane >> 4) == gout) macc[G] = floatx4{0.f, 0.f, 0.f, 0.f};
+                }
+            }
         }
         // The stencil in class-sum form: with X_c(ox, oy) the plane's 3 x 3 values of component c and
         // oz = plane - node, every entry is D[|o_d|][nnz of the other two offsets] on the diagonal and
@@ -270,7 +346,7 @@ template <class T, int MODE, bool OWNC, bool DOT, bool DENSE = false> __device__
         // P (x(z - 1) - 2 x(z) + x(z + 1)), P = the diagonal sum of a neighbour plane's blocks. A
         // rigid translation then cancels exactly and a rotation to element scale, which keeps the
         // FP32 operator accurate on the smooth, low-energy modes the conjugate gradients resolve last.
-        if (!DENSE && mine) {
+        if (!DENSE && !MF && mine) {
             const T D00 = C.v[14], D01 = C.v[15], D02 = C.v[16], D10 = C.v[17], D11 = C.v[18], D12 = C.v[19], E0 = C.v[20], E1 = C.v[21];
             (void)D00;
             // q = (ox + 1) + 3 (oy + 1)
@@ -381,9 +457,10 @@ template <class T, int MODE, bool OWNC, bool DOT, bool DENSE = false> __device__
                 // the centre column's second difference through the neighbour planes' diagonal sums
                 const T Pxy = C.v[15] + T(2) * (C.v[18] + C.v[16]) + T(4) * C.v[19], Pz = C.v[17] + T(4) * (C.v[18] + C.v[19]);
                 const T sref = T(sh.s[(ref >> 2) ? v0 : vm][ty + ((ref >> 1) & 1)][tx + (ref & 1)]);
-                out[0] = sref * (aP[0] + Pxy * ((xcPrev2[0] + xm[4][0]) - T(2) * xcPrev[0]));
-                out[1] = sref * (aP[1] + Pxy * ((xcPrev2[1] + xm[4][1]) - T(2) * xcPrev[1]));
-                out[2] = sref * (aP[2] + Pz * ((xcPrev2[2] + xm[4][2]) - T(2) * xcPrev[2]));
+                const T s0 = MF ? mres[0] : aP[0], s1 = MF ? mres[1] : aP[1], s2 = MF ? mres[2] : aP[2];
+                out[0] = sref * (s0 + Pxy * ((xcPrev2[0] + xm[4][0]) - T(2) * xcPrev[0]));
+                out[1] = sref * (s1 + Pxy * ((xcPrev2[1] + xm[4][1]) - T(2) * xcPrev[1]));
+                out[2] = sref * (s2 + Pz * ((xcPrev2[2] + xm[4][2]) - T(2) * xcPrev[2]));
             }
             const int mine0 = mineBase + waveOff;
             for (int pass = 0; pass * NT < total; pass++) {
@@ -605,13 +682,24 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void kF
 template <class T, int MODE, bool DOT> __global__ __launch_bounds__(NT) void kFineDense(PSX_FINE_PARAMS) {
     fineBody<T, MODE, false, DOT, true>(PSX_FINE_FWD);
 }
+// in-plane stencil on the matrix cores (FP32 levels without own elements)
+template <int MODE, bool DOT>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void kFineMf(DevLevel L, Sym<float> C, const float* __restrict__ crow,
+                                                                                   const float* __restrict__ ctab, const float* __restrict__ x,
+                                                                                   float* __restrict__ y, const float* __restrict__ r,
+                                                                                   const float* __restrict__ zp, float c1, float c2, const int* stop,
+                                                                                   int zChunk, int gx, int gy, int nTiles, int xcdOrder,
+                                                                                   double* __restrict__ dots, int64_t dotStride,
+                                                                                   const float* __restrict__ qdot) {
+    fineBody<float, MODE, false, DOT, false, true>(PSX_FINE_FWD);
+}
 #undef PSX_FINE_FWD
 
 }  // namespace
 
 template <class T, bool OWNC>
 static void launchFineT(const DevLevel& L, int mode, const Sym<T>& C, const T* crow, const T* ctab, const T* x, T* y, const T* r, const T* d, double c1, double c2,
-                        const int* stop, int zChunk, double* dots, int64_t dotStride, const T* qdot, hipStream_t s, int ninst) {
+                        const int* stop, int zChunk, double* dots, int64_t dotStride, const T* qdot, hipStream_t s, int ninst, bool mf) {
     const int gx = (L.NX + TX - 1) / TX, gy = (L.NY + TY - 1) / TY, gz = (L.NZ + zChunk - 1) / zChunk;
     const int nTiles = gx * gy * gz;
     const dim3 g(unsigned(8 * ((nTiles + 7) / 8)), unsigned(ninst)), b(NT);  // whole rounds over the 8 XCDs
@@ -625,6 +713,27 @@ static void launchFineT(const DevLevel& L, int mode, const Sym<T>& C, const T* c
 #define PSX_FINE_ARGS L, C, crow, ctab, x, y, r, d, T(c1), T(c2), stop, zChunk, gx, gy, nTiles, xcdOrder, dots, dotStride, qdot
     // fused dots exist for APPLY (p.q) and the Chebyshev steps (r.z, z.q) only
     const bool dot = dots && (mode == AM_APPLY || mode == AM_CHEB1 || mode == AM_CHEB2);
+    if constexpr (sizeof(T) == 4 && !OWNC) {
+        if (mf && !L.dense) {
+            switch (mode) {
+                case AM_APPLY:
+                    if (dot) kFineMf<AM_APPLY, true><<<g, b, 0, s>>>(PSX_FINE_ARGS);
+                    else kFineMf<AM_APPLY, false><<<g, b, 0, s>>>(PSX_FINE_ARGS);
+                    break;
+                case AM_RAW: kFineMf<AM_RAW, false><<<g, b, 0, s>>>(PSX_FINE_ARGS); break;
+                case AM_RESID: kFineMf<AM_RESID, false><<<g, b, 0, s>>>(PSX_FINE_ARGS); break;
+                case AM_CHEB1:
+                    if (dot) kFineMf<AM_CHEB1, true><<<g, b, 0, s>>>(PSX_FINE_ARGS);
+                    else kFineMf<AM_CHEB1, false><<<g, b, 0, s>>>(PSX_FINE_ARGS);
+                    break;
+                default:
+                    if (dot) kFineMf<AM_CHEB2, true><<<g, b, 0, s>>>(PSX_FINE_ARGS);
+                    else kFineMf<AM_CHEB2, false><<<g, b, 0, s>>>(PSX_FINE_ARGS);
+                    break;
+            }
+            return;
+        }
+    }
     if (!OWNC && L.dense) {
         switch (mode) {
             case AM_APPLY:
@@ -682,7 +791,7 @@ static void launchFineT(const DevLevel& L, int mode, const Sym<T>& C, const T* c
 
 template <class T>
 unsigned launchFineDot(const DevLevel& L, int mode, const T* x, T* y, const T* r, const T* d, double c1, double c2, const int* stop, const DotFusion& fd,
-                       hipStream_t s, int ninst) {
+                       hipStream_t s, int ninst, bool mf) {
     Sym<T> C;
     for (int q = 0; q < 22; q++) C.v[q] = sizeof(T) == 8 ? T(L.sym64[q]) : T(L.sym32[q]);
     // the longest z-chunk (16, 8 or 4 planes) that still gives >= 4096 blocks (16 per CU); shorter
@@ -707,23 +816,25 @@ unsigned launchFineDot(const DevLevel& L, int mode, const T* x, T* y, const T* r
         const T *crow = nullptr, *ctab = nullptr;
         if constexpr (sizeof(T) == 8) crow = L.crow64, ctab = L.ctab64;
         else crow = L.crow32, ctab = L.ctab32;
-        launchFineT<T, true>(L, mode, C, crow, ctab, x, y, r, d, c1, c2, stop, zChunk, dots, fd.red.stride, qdot, s, ninst);
+        launchFineT<T, true>(L, mode, C, crow, ctab, x, y, r, d, c1, c2, stop, zChunk, dots, fd.red.stride, qdot, s, ninst, false);
     } else {
-        launchFineT<T, false>(L, mode, C, nullptr, nullptr, x, y, r, d, c1, c2, stop, zChunk, dots, fd.red.stride, qdot, s, ninst);
+        launchFineT<T, false>(L, mode, C, nullptr, nullptr, x, y, r, d, c1, c2, stop, zChunk, dots, fd.red.stride, qdot, s, ninst, mf);
     }
     return fused ? unsigned(gx * gy * gz) : 0u;
 }
 
 template <class T>
-void launchFine(const DevLevel& L, int mode, const T* x, T* y, const T* r, const T* d, double c1, double c2, const int* stop, hipStream_t s, int ninst) {
-    launchFineDot<T>(L, mode, x, y, r, d, c1, c2, stop, DotFusion{}, s, ninst);
+void launchFine(const DevLevel& L, int mode, const T* x, T* y, const T* r, const T* d, double c1, double c2, const int* stop, hipStream_t s, int ninst,
+                bool mf) {
+    launchFineDot<T>(L, mode, x, y, r, d, c1, c2, stop, DotFusion{}, s, ninst, mf);
 }
 
 template unsigned launchFineDot<double>(const DevLevel&, int, const double*, double*, const double*, const double*, double, double, const int*,
-                                        const DotFusion&, hipStream_t, int);
+                                        const DotFusion&, hipStream_t, int, bool);
 template unsigned launchFineDot<float>(const DevLevel&, int, const float*, float*, const float*, const float*, double, double, const int*,
-                                       const DotFusion&, hipStream_t, int);
-template void launchFine<double>(const DevLevel&, int, const double*, double*, const double*, const double*, double, double, const int*, hipStream_t, int);
-template void launchFine<float>(const DevLevel&, int, const float*, float*, const float*, const float*, double, double, const int*, hipStream_t, int);
+                                       const DotFusion&, hipStream_t, int, bool);
+template void launchFine<double>(const DevLevel&, int, const double*, double*, const double*, const double*, double, double, const int*, hipStream_t, int,
+                                 bool);
+template void launchFine<float>(const DevLevel&, int, const float*, float*, const float*, const float*, double, double, const int*, hipStream_t, int, bool);
 
 }  // namespace psx

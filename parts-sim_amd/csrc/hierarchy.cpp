@@ -13,12 +13,15 @@
 
 namespace psx {
 
+// host threads of one hierarchy build: PSX_HOST_THREADS, else OMP_NUM_THREADS (the process's CPU
+// share where a scheduler sets it), else the hardware's, at most 64
 int hostThreads() {
     static const int n = [] {
-        if (const char* e = std::getenv("PSX_HOST_THREADS")) {
-            const int v = std::atoi(e);
-            if (v > 0) return v;
-        }
+        for (const char* name : {"PSX_HOST_THREADS", "OMP_NUM_THREADS"})
+            if (const char* e = std::getenv(name)) {
+                const int v = std::atoi(e);
+                if (v > 0) return std::min(v, 64);
+            }
         return int(std::max(1u, std::min(64u, std::thread::hardware_concurrency())));
     }();
     return n;
@@ -288,57 +291,78 @@ static Level emptyLevel(int nx, int ny, int nz) {
 static void finishLevel(Level& L) {
     const i64 nE = i64(L.elemVox.size());
     L.active.assign(L.nNodes, 0);
-    std::vector<double> diag(L.nDof, 0.0);
-    for (i64 e = 0; e < nE; e++) {
-        const int v = L.elemVox[e];
-        const int i = v % L.nx, j = (v / L.nx) % L.ny, k = v / (L.nx * L.ny);
-        const float* Ko = L.own[e] >= 0 ? L.K.data() + i64(L.own[e]) * 576 : nullptr;
-        for (int a = 0; a < 8; a++) {
-            const i64 n = L.node(i + CX[a], j + CY[a], k + CZ[a]);
-            L.active[n] = 1;
-            for (int d = 0; d < 3; d++) {
-                const int r = 3 * a + d;
-                diag[3 * n + d] += Ko ? double(Ko[r * 25]) : L.rho[e] * L.Kb[r * 25];
-            }
-        }
-    }
-    if (!L.diagAdd.empty())
-        for (i64 q = 0; q < L.nDof; q++) diag[q] += L.diagAdd[q];
     L.fixed.assign(L.nDof, 0);
     L.invDiag.assign(L.nDof, 0.0);
-    i64 nfree = 0;
-    for (i64 q = 0; q < L.nDof; q++) {
-        if (!L.active[q / 3] || L.bc[q] || !(diag[q] > 0)) L.fixed[q] = 1;
-        else {
-            L.invDiag[q] = 1 / diag[q];
-            nfree++;
+    // per node, gathered over its voxels in increasing voxel (= element) order: the same sums as
+    // the reference's scatter over elements
+    std::vector<i64> freeRows(i64(L.NZ) * L.NY, 0);
+    parallelRange(i64(L.NZ) * L.NY, [&](i64 lo, i64 hi) {
+        for (i64 row = lo; row < hi; row++) {
+            const int j = int(row % L.NY), k = int(row / L.NY);
+            i64 nfree = 0;
+            for (int i = 0; i < L.NX; i++) {
+                const i64 n = L.node(i, j, k);
+                double diag[3] = {0, 0, 0};
+                bool act = false;
+                for (int dk = 0; dk < 2; dk++)
+                    for (int dj = 0; dj < 2; dj++)
+                        for (int di = 0; di < 2; di++) {
+                            const int vi = i - 1 + di, vj = j - 1 + dj, vk = k - 1 + dk;
+                            if (vi < 0 || vj < 0 || vk < 0 || vi >= L.nx || vj >= L.ny || vk >= L.nz) continue;
+                            const int e = L.emap[vi + i64(L.nx) * (vj + i64(L.ny) * vk)];
+                            if (e < 0) continue;
+                            act = true;
+                            const int a = corner(1 - di, 1 - dj, 1 - dk);
+                            const float* Ko = L.own[e] >= 0 ? L.K.data() + i64(L.own[e]) * 576 : nullptr;
+                            for (int d = 0; d < 3; d++) {
+                                const int r = 3 * a + d;
+                                diag[d] += Ko ? double(Ko[r * 25]) : L.rho[e] * L.Kb[r * 25];
+                            }
+                        }
+                L.active[n] = act ? 1 : 0;
+                for (int d = 0; d < 3; d++) {
+                    const i64 q = 3 * n + d;
+                    const double dg = L.diagAdd.empty() ? diag[d] : diag[d] + L.diagAdd[q];
+                    if (!act || L.bc[q] || !(dg > 0)) L.fixed[q] = 1;
+                    else {
+                        L.invDiag[q] = 1 / dg;
+                        nfree++;
+                    }
+                }
+            }
+            freeRows[row] = nfree;
         }
-    }
-    L.freeDof = nfree;
+    }, 16);
+    L.freeDof = 0;
+    for (i64 c : freeRows) L.freeDof += c;
     L.words.assign(L.nVox, 0);
     std::vector<float> vs(L.nVox, 0.f);
-    for (i64 e = 0; e < nE; e++) {
-        const int v = L.elemVox[e];
-        if (L.own[e] >= 0) {
-            L.words[v] = int32_t(0x80000000u | uint32_t(L.own[e]));
-            vs[v] = -1.f;
-        } else {
-            const float s = float(L.rho[e]);
-            std::memcpy(&L.words[v], &s, 4);
-            vs[v] = s;
-        }
-    }
-    L.nodeScale.assign(L.nNodes, 0.f);
-    for (int k = 1; k < L.nz; k++)
-        for (int j = 1; j < L.ny; j++)
-            for (int i = 1; i < L.nx; i++) {
-                const i64 v0 = (i - 1) + i64(L.nx) * ((j - 1) + i64(L.ny) * (k - 1));
-                const float s0 = vs[v0];
-                if (!(s0 > 0)) continue;
-                bool same = true;
-                for (int v = 1; v < 8 && same; v++) same = vs[v0 + (v & 1) + i64(L.nx) * (((v >> 1) & 1) + i64(L.ny) * (v >> 2))] == s0;
-                if (same) L.nodeScale[L.node(i, j, k)] = s0;
+    parallelRange(nE, [&](i64 lo, i64 hi) {
+        for (i64 e = lo; e < hi; e++) {
+            const int v = L.elemVox[e];
+            if (L.own[e] >= 0) {
+                L.words[v] = int32_t(0x80000000u | uint32_t(L.own[e]));
+                vs[v] = -1.f;
+            } else {
+                const float s = float(L.rho[e]);
+                std::memcpy(&L.words[v], &s, 4);
+                vs[v] = s;
             }
+        }
+    });
+    L.nodeScale.assign(L.nNodes, 0.f);
+    parallelRange(std::max(0, L.nz - 1), [&](i64 lo, i64 hi) {
+        for (int k = int(lo) + 1; k < int(hi) + 1; k++)
+            for (int j = 1; j < L.ny; j++)
+                for (int i = 1; i < L.nx; i++) {
+                    const i64 v0 = (i - 1) + i64(L.nx) * ((j - 1) + i64(L.ny) * (k - 1));
+                    const float s0 = vs[v0];
+                    if (!(s0 > 0)) continue;
+                    bool same = true;
+                    for (int v = 1; v < 8 && same; v++) same = vs[v0 + (v & 1) + i64(L.nx) * (((v >> 1) & 1) + i64(L.ny) * (v >> 2))] == s0;
+                    if (same) L.nodeScale[L.node(i, j, k)] = s0;
+                }
+    }, 1);
     // per node: the voxels (v = di + 2 dj + 4 dk at (i - 1 + di, j - 1 + dj, k - 1 + dk)) whose
     // scale differs from the node's reference voxel, the lowest-index voxel of the most common
     // scale (outside voxels count as 0, own-matrix voxels as distinct)
@@ -663,12 +687,30 @@ void buildHierarchy(Hierarchy& H, int nx, int ny, int nz, const float* density, 
     Level L0 = emptyLevel(nx, ny, nz);
     std::memcpy(L0.Kb, H.el.K, sizeof L0.Kb);
     L0.emap.assign(nVox, -1);
-    for (i64 v = 0; v < nVox; v++)
-        if (density[v] > 0) {
-            L0.emap[v] = int32_t(L0.elemVox.size());
-            L0.elemVox.push_back(int32_t(v));
-            L0.rho.push_back(density[v]);
-        }
+    {
+        // element numbering in voxel order: per-chunk counts, then offsets, then the fill
+        const int T = hostThreads();
+        std::vector<i64> cnt(T + 1, 0);
+        parallelRange(T, [&](i64 lo, i64 hi) {
+            for (i64 t = lo; t < hi; t++)
+                for (i64 v = nVox * t / T; v < nVox * (t + 1) / T; v++) cnt[t + 1] += density[v] > 0 ? 1 : 0;
+        }, 1);
+        for (int t = 0; t < T; t++) cnt[t + 1] += cnt[t];
+        L0.elemVox.resize(cnt[T]);
+        L0.rho.resize(cnt[T]);
+        parallelRange(T, [&](i64 lo, i64 hi) {
+            for (i64 t = lo; t < hi; t++) {
+                i64 at = cnt[t];
+                for (i64 v = nVox * t / T; v < nVox * (t + 1) / T; v++)
+                    if (density[v] > 0) {
+                        L0.emap[v] = int32_t(at);
+                        L0.elemVox[at] = int32_t(v);
+                        L0.rho[at] = density[v];
+                        at++;
+                    }
+            }
+        }, 1);
+    }
     L0.own.assign(L0.elemVox.size(), -1);
     L0.bc.assign(bc, bc + L0.nDof);
     if (diagAdd) L0.diagAdd.assign(diagAdd, diagAdd + L0.nDof);

@@ -90,13 +90,14 @@ public:
     // the number the current launches run
     int cap = 0;
     int ni = 1;
+    int preDegree = 0;  // Chebyshev degree of the V-cycle's pre-smoothing (0 = the post-smoothing's)
     // the last captured batch of conjugate-gradient iterations and what it was captured for
     struct GraphKey {
-        int prec = -1, degree = 0, form = 0, k = 0, batch = 0;
+        int prec = -1, degree = 0, pre = 0, form = 0, k = 0, batch = 0;
         const double* df = nullptr;
         double* du = nullptr;
         bool operator==(const GraphKey& o) const {
-            return prec == o.prec && degree == o.degree && form == o.form && k == o.k && batch == o.batch && df == o.df && du == o.du;
+            return prec == o.prec && degree == o.degree && pre == o.pre && form == o.form && k == o.k && batch == o.batch && df == o.df && du == o.du;
         }
     };
     hipGraphExec_t graph = nullptr;
@@ -296,28 +297,21 @@ public:
     void op(size_t l, int mode, const T* xx, T* yy, const T* rr, const T* dd, double c1, double c2, const int* stop, int form = 0,
             const DotFusion* fd = nullptr) {
         const DevLevel& D = lv[l].dl;
-        bool done = false;
-        if constexpr (sizeof(T) == 4) {
-            if ((form & OF_MFMA) && lv[l].aop && ni == 1) {
-                launchFineMfma(D, lv[l].aop, mode, xx, yy, rr, dd, c1, c2, stop, stream);
-                done = true;
-            }
-        }
-        if (!done) {
-            if (!useFine) launchApply<T>(D, mode, xx, yy, rr, dd, c1, c2, stop, stream, ni);
-            else if (D.symOk && (H.levels[l].nOwn == 0 || (D.childForm && ((form & OF_CHILD) || ((form & OF_AUTO) && lv[l].childCheap))))) {
-                if (fd) {
-                    const unsigned nb = launchFineDot<T>(D, mode, xx, yy, rr, dd, c1, c2, stop, *fd, stream, ni);
-                    if (nb) {
-                        if (fd->kind == FD_PQ) launchFinishPQ(fd->red, nb, st, stream, ni);
-                        else launchFinishRZ(fd->red, nb, st, stream, ni);
-                        return;
-                    }
-                } else {
-                    launchFine<T>(D, mode, xx, yy, rr, dd, c1, c2, stop, stream, ni);
+        // the matrix-core form: FP32 levels without own elements (the per-voxel form keeps FP32)
+        const bool mf = sizeof(T) == 4 && (form & OF_MFMA) && H.levels[l].nOwn == 0 && !D.dense;
+        if (!useFine) launchApply<T>(D, mode, xx, yy, rr, dd, c1, c2, stop, stream, ni);
+        else if (D.symOk && (H.levels[l].nOwn == 0 || (D.childForm && ((form & OF_CHILD) || ((form & OF_AUTO) && lv[l].childCheap))))) {
+            if (fd) {
+                const unsigned nb = launchFineDot<T>(D, mode, xx, yy, rr, dd, c1, c2, stop, *fd, stream, ni, mf);
+                if (nb) {
+                    if (fd->kind == FD_PQ) launchFinishPQ(fd->red, nb, st, stream, ni);
+                    else launchFinishRZ(fd->red, nb, st, stream, ni);
+                    return;
                 }
-            } else launchApplyOwn<T>(D, mode, xx, yy, rr, dd, c1, c2, stop, stream, ni);
-        }
+            } else {
+                launchFine<T>(D, mode, xx, yy, rr, dd, c1, c2, stop, stream, ni, mf);
+            }
+        } else launchApplyOwn<T>(D, mode, xx, yy, rr, dd, c1, c2, stop, stream, ni);
         if (fd) dotAfter<T>(*fd, D.nDof, xx, yy, rr);
     }
     template <class T> void dotAfter(const DotFusion& fd, int64_t n, const T* xx, const T* yy, const T* rr) {
@@ -404,7 +398,8 @@ public:
             z = y;
         };
         launchChebFirst<T>(D, rIn, z, c.c0, stop, stream, ni);
-        for (int j = 1; j < degree; j++) step(AM_CHEB2, c.c1[j], c.c2[j]);
+        const int pre = preDegree > 0 ? std::min(preDegree, degree) : degree;
+        for (int j = 1; j < pre; j++) step(AM_CHEB2, c.c1[j], c.c2[j]);
         zp = nullptr;
         T* t = freeBuf();
         op<T>(l, AM_RESID, z, t, rIn, nullptr, 0, 0, stop, form);
@@ -482,6 +477,11 @@ public:
         const int form = prec == PSX_PREC_REF ? 0 : (OF_AUTO | (prec == PSX_PREC_MIXED && o.mfma_smoother ? OF_MFMA : 0));
         const int degree = o.cheb_degree > 0 ? o.cheb_degree : 2;
         if (degree > CHEB_MAX) throw std::invalid_argument("Chebyshev degree above 6");
+        preDegree = o.pre_degree > 0 ? std::min(o.pre_degree, degree) : 0;
+        struct ResetPre {
+            int& p;
+            ~ResetPre() { p = 0; }
+        } resetPre{preDegree};
         if (prec == PSX_PREC_MIXED) {
             // FP32 units for the whole solve: each first residual scaled to order one
             launchMaxAbs64(r, n, &st->rmax, red, stream, k);
@@ -553,7 +553,7 @@ public:
         };
         // the batch as a HIP graph, captured once and replayed (kernels only: no copies or memsets);
         // it saves the host one launch per kernel, which bounds small parts
-        const GraphKey key{prec, degree, form, k, batch, df, du};
+        const GraphKey key{prec, degree, preDegree, form, k, batch, df, du};
         const bool useGraph = !o.eager && !graphsOff();
         if (useGraph && !allStopped && !(graph && graphKey == key)) {
             dropGraph();

@@ -357,21 +357,22 @@ int psx_profile(psx_model m, int reps, char* buf, size_t len) {
             HIPCHK(hipEventElapsedTime(&t, e0, e1));
             return double(t) / reps;
         };
-        char line[320];
+        char line[400];
         for (size_t l = 0; l < M.lv.size(); l++) {
             auto& V = M.lv[l];
             const psx::Level& L = M.H.levels[l];
             const double t64 = timeIt([&] { M.op<double>(l, psx::AM_APPLY, V.a64, V.b64, nullptr, nullptr, 0, 0, nullptr); });
             const double t32 = timeIt([&] { M.op<float>(l, psx::AM_APPLY, V.a32, V.b32, nullptr, nullptr, 0, 0, nullptr); });
-            double tbf = 0, c64 = 0, c32 = 0;
+            double tbf = 0, tmf = 0, c64 = 0, c32 = 0;
             if (V.aop) tbf = timeIt([&] { psx::launchFineMfma(V.dl, V.aop, psx::AM_APPLY, V.a32, V.b32, nullptr, nullptr, 0, 0, nullptr, M.stream); });
+            if (L.nOwn == 0 && !V.dl.dense) tmf = timeIt([&] { M.op<float>(l, psx::AM_APPLY, V.a32, V.b32, nullptr, nullptr, 0, 0, nullptr, psx::OF_MFMA); });
             if (V.dl.childForm) {
                 c64 = timeIt([&] { M.op<double>(l, psx::AM_APPLY, V.a64, V.b64, nullptr, nullptr, 0, 0, nullptr, psx::OF_CHILD); });
                 c32 = timeIt([&] { M.op<float>(l, psx::AM_APPLY, V.a32, V.b32, nullptr, nullptr, 0, 0, nullptr, psx::OF_CHILD); });
             }
             std::snprintf(line, sizeof line,
-                          "L%zu nodes=%lld elems=%zu own matrices=%lld: matvec64 %.4f ms  matvec32 %.4f ms  matvec-bf16 %.4f ms  child-form64 %.4f ms  child-form32 %.4f ms\n",
-                          l, (long long)L.nNodes, L.elemVox.size(), (long long)L.nOwn, t64, t32, tbf, c64, c32);
+                          "L%zu nodes=%lld elems=%zu own matrices=%lld: matvec64 %.4f ms  matvec32 %.4f ms  matvec-bf16 %.4f ms  matvec-mf %.4f ms  child-form64 %.4f ms  child-form32 %.4f ms\n",
+                          l, (long long)L.nNodes, L.elemVox.size(), (long long)L.nOwn, t64, t32, tbf, tmf, c64, c32);
             out += line;
         }
         const double tv64 = timeIt([&] { M.vcycle<double>(0, M.lv[0].r64, nullptr, psx::OF_AUTO); });
