@@ -123,9 +123,11 @@ int psx_apply32(psx_model m, int level, const float* x, float* y, int raw);
  * fp32 selects float x and y. No reference counterpart (the reference always streams its matrices). */
 int psx_apply_child(psx_model m, int level, int fp32, const void* x, void* y, int raw);
 
-/* y = K x with the BF16 matrix-core operator the mixed-precision V-cycle smooths with (levels
- * without own element matrices; FP32 vectors, BF16 products, FP32 accumulation). No reference
- * counterpart: the reference's smoother operator is FP32 (voxel_fea.cpp:393, fea.wgsl:173-206). */
+/* y = K x with the matrix-core operator the mixed-precision V-cycle smooths with when
+ * opts.mfma_smoother is set (levels without own element matrices; FP32 vectors, the in-plane stencil
+ * sums as BF16 products of neighbour differences with FP32 accumulation; levels of per-voxel
+ * densities keep their FP32 per-voxel form). No reference counterpart: the reference's smoother
+ * operator is FP32 (voxel_fea.cpp:393, fea.wgsl:173-206). */
 int psx_apply_bf16(psx_model m, int level, const float* x, float* y);
 
 /* One V-cycle z ~ K^-1 r, FP32 with r scaled to order one (GpuFeaSolver::precondition,

@@ -116,11 +116,6 @@ template <class T> unsigned launchFineDot(const DevLevel& L, int mode, const T* 
                                           const DotFusion& fd, hipStream_t s, int ninst = 1, bool mf = false);
 void launchFinishPQ(Reduce red, unsigned nb, CgState* st, hipStream_t s, int ninst = 1);
 void launchFinishRZ(Reduce red, unsigned nb, CgState* st, hipStream_t s, int ninst = 1);
-// the same in BF16 on the matrix cores, FP32 vectors (mfma.hip); Aop from mfmaOperands()
-void launchFineMfma(const DevLevel& L, const void* Aop, int mode, const float* x, float* y, const float* r, const float* zp, double c1, double c2,
-                    const int* stop, hipStream_t s);
-size_t mfmaOperandBytes();
-void mfmaOperands(const double* Kb, uint16_t* out);
 // z = c0 D^-1 r (first Chebyshev step from zero)
 template <class T> void launchChebFirst(const DevLevel& L, const T* r, T* z, double c0, const int* stop, hipStream_t s, int ninst = 1);
 // rc = P^T rf on the coarse level's free DOFs

@@ -684,7 +684,7 @@ template <class T, int MODE, bool DOT> __global__ __launch_bounds__(NT) void kFi
 }
 // in-plane stencil on the matrix cores (FP32 levels without own elements)
 template <int MODE, bool DOT>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void kFineMf(DevLevel L, Sym<float> C, const float* __restrict__ crow,
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void kFineMf(DevLevel L, Sym<float> C, const float* __restrict__ crow,
                                                                                    const float* __restrict__ ctab, const float* __restrict__ x,
                                                                                    float* __restrict__ y, const float* __restrict__ r,
                                                                                    const float* __restrict__ zp, float c1, float c2, const int* stop,

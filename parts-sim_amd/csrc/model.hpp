@@ -69,7 +69,6 @@ public:
 
     struct Lvl {
         DevLevel dl{};
-        void* aop = nullptr;  // BF16 matrix-core operands of Kb (levels without own matrices)
         bool childCheap = false;  // child-scale form preferred (OF_AUTO): most own elements two-valued
         double *r64 = nullptr, *a64 = nullptr, *b64 = nullptr, *d64 = nullptr;
         float *r32 = nullptr, *a32 = nullptr, *b32 = nullptr, *d32 = nullptr;
@@ -214,11 +213,6 @@ public:
                 const double dev = symmetryClasses(L.Kb, D.sym64);
                 for (int q = 0; q < 22; q++) D.sym32[q] = float(D.sym64[q]);
                 D.symOk = dev < 1e-12;
-            }
-            if (L.nOwn == 0) {
-                std::vector<uint16_t> a(mfmaOperandBytes() / 2);
-                mfmaOperands(L.Kb, a.data());
-                lv[l].aop = upload(a.data(), int64_t(a.size()));
             }
         }
         freeIdx = upload(H.freeIdx.data(), H.m);

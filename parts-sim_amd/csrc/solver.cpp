@@ -153,10 +153,10 @@ int psx_apply_bf16(psx_model m, int level, const float* x, float* y) {
         if (!m || !x || !y || level < 0 || level >= int(m->m->H.levels.size())) throw std::invalid_argument("bad arguments");
         psx::Model& M = *m->m;
         auto& V = M.lv[level];
-        if (!V.aop) throw std::invalid_argument("the level has own element matrices: no matrix-core operator");
+        if (M.H.levels[size_t(level)].nOwn > 0) throw std::invalid_argument("the level has own element matrices: no matrix-core operator");
         const int64_t n = V.dl.nDof;
         HIPCHK(hipMemcpyAsync(V.a32, x, size_t(n) * 4, hipMemcpyHostToDevice, M.stream));
-        psx::launchFineMfma(V.dl, V.aop, psx::AM_APPLY, V.a32, V.b32, nullptr, nullptr, 0, 0, nullptr, M.stream);
+        M.op<float>(size_t(level), psx::AM_APPLY, V.a32, V.b32, nullptr, nullptr, 0, 0, nullptr, psx::OF_MFMA);
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(y, V.b32, size_t(n) * 4, hipMemcpyDeviceToHost, M.stream));
         M.sync();
@@ -363,16 +363,15 @@ int psx_profile(psx_model m, int reps, char* buf, size_t len) {
             const psx::Level& L = M.H.levels[l];
             const double t64 = timeIt([&] { M.op<double>(l, psx::AM_APPLY, V.a64, V.b64, nullptr, nullptr, 0, 0, nullptr); });
             const double t32 = timeIt([&] { M.op<float>(l, psx::AM_APPLY, V.a32, V.b32, nullptr, nullptr, 0, 0, nullptr); });
-            double tbf = 0, tmf = 0, c64 = 0, c32 = 0;
-            if (V.aop) tbf = timeIt([&] { psx::launchFineMfma(V.dl, V.aop, psx::AM_APPLY, V.a32, V.b32, nullptr, nullptr, 0, 0, nullptr, M.stream); });
+            double tmf = 0, c64 = 0, c32 = 0;
             if (L.nOwn == 0 && !V.dl.dense) tmf = timeIt([&] { M.op<float>(l, psx::AM_APPLY, V.a32, V.b32, nullptr, nullptr, 0, 0, nullptr, psx::OF_MFMA); });
             if (V.dl.childForm) {
                 c64 = timeIt([&] { M.op<double>(l, psx::AM_APPLY, V.a64, V.b64, nullptr, nullptr, 0, 0, nullptr, psx::OF_CHILD); });
                 c32 = timeIt([&] { M.op<float>(l, psx::AM_APPLY, V.a32, V.b32, nullptr, nullptr, 0, 0, nullptr, psx::OF_CHILD); });
             }
             std::snprintf(line, sizeof line,
-                          "L%zu nodes=%lld elems=%zu own matrices=%lld: matvec64 %.4f ms  matvec32 %.4f ms  matvec-bf16 %.4f ms  matvec-mf %.4f ms  child-form64 %.4f ms  child-form32 %.4f ms\n",
-                          l, (long long)L.nNodes, L.elemVox.size(), (long long)L.nOwn, t64, t32, tbf, tmf, c64, c32);
+                          "L%zu nodes=%lld elems=%zu own matrices=%lld: matvec64 %.4f ms  matvec32 %.4f ms  matvec-bf16 %.4f ms  child-form64 %.4f ms  child-form32 %.4f ms\n",
+                          l, (long long)L.nNodes, L.elemVox.size(), (long long)L.nOwn, t64, t32, tmf, c64, c32);
             out += line;
         }
         const double tv64 = timeIt([&] { M.vcycle<double>(0, M.lv[0].r64, nullptr, psx::OF_AUTO); });

@@ -149,9 +149,10 @@ def test_coarsest_buffer_reuse_does_not_leak(models):
 
 
 def test_bf16_matrix_core_operator(P, W):
-    """The BF16 matrix-core operator of the mixed V-cycle against the FP64 operator: BF16 rounds
-    the displacement differences within each voxel (relative 2^-9), so the product is accurate to
-    about 1e-2 of |K| |x| for rough vectors and much better for smooth ones (difference form)."""
+    """The matrix-core operator of the mixed V-cycle against the FP64 operator: the in-plane stencil
+    sums are BF16 products of neighbour differences (relative 2^-9 each) with FP32 accumulation, so
+    the product is accurate to about 1e-2 of |K| |x| for rough vectors and for smooth ones (difference
+    form)."""
     w = W.void_block((40, 16, 12), seed=5)
     g = P.Model(w["dims"], w["density"], w["nu"], w["bc"], 300)
     n = g.n_dof
@@ -165,9 +166,21 @@ def test_bf16_matrix_core_operator(P, W):
         err = np.linalg.norm(yb - y64) / np.linalg.norm(y64)
         print("bf16 operator rel err", err, flush=True)
         assert err < tol
-    # the rigid translation is annihilated exactly (difference form), not to BF16 rounding
+    # a rigid translation: exactly zero at nodes whose voxels all carry one scale (every difference
+    # vanishes); at the part's faces and void surfaces the FP32 corrections of the differing voxels
+    # meet BF16 stencil sums, which leaves a residue at BF16 rounding relative to |K| |t|
     t = np.tile([1.0, -2.0, 0.5], n // 3)
-    assert np.abs(g.apply_bf16(t.astype(np.float32))).max() == 0
+    yt = g.apply_bf16(t.astype(np.float32))
+    scale = np.abs(g.apply(rough)).max() * np.abs(t).max() / np.abs(rough).max()
+    print("translation residue", np.abs(yt).max() / scale, flush=True)
+    assert np.abs(yt).max() <= 1e-2 * scale
+    # a solid block: every interior node is regular, so the translation vanishes there exactly
+    ws = W.void_block((24, 12, 10), seed=5, void_fraction=0.0)
+    gs = P.Model(ws["dims"], ws["density"], ws["nu"], ws["bc"], 300)
+    ts = np.tile([1.0, -2.0, 0.5], gs.n_dof // 3)
+    ys = gs.apply_bf16(ts.astype(np.float32)).reshape(ws["dims"][2] + 1, ws["dims"][1] + 1, ws["dims"][0] + 1, 3)
+    assert np.all(ys[1:-1, 1:-1, 1:-1] == 0)
+    gs.close()
 
 
 def test_mixed_iterations_close_to_fp64(models):
