@@ -59,6 +59,8 @@ def lib(kind):
         "tridiagonal_max": (dbl, [_f64p, C.c_int, _f64p]),
         "cheb_coefficients": (None, [dbl, _f64p]),
         "grid_for_box": (None, [_f64p, _f64p, C.c_int, C.c_int, _f64p, C.POINTER(C.c_double), C.POINTER(C.c_int)]),
+        "element_energies": (None, [vp, _f64p, _f64p]),
+        "optimize_topology": (C.c_int, [vp, _f32p, _u8p, _u8p, _f64p, dbl, dbl, dbl, dbl, dbl, dbl, C.c_int, i64, _f32p, _f64p]),
         "voxelize": (C.c_int, [_f32p, i64, vp, i64, _f64p, dbl, C.POINTER(C.c_int), C.c_int, _f32p]),
     }
     if kind == "ref":
@@ -68,6 +70,7 @@ def lib(kind):
         sig["timed_phases"] = (C.c_int, [C.c_int, C.c_int, C.c_int, _f32p, dbl, _u8p, i64, _f64p, dbl, C.c_int, _f64p,
                                          C.POINTER(C.c_int), C.POINTER(C.c_double), _f64p])
     else:
+        sig["topo_filter"] = (None, [vp, _f32p, dbl, C.c_int, _f64p, _f64p])
         sig["set_threads"] = (None, [C.c_int])
         sig["set_cheb_degree"] = (None, [vp, C.c_int])
         sig["smooth"] = (None, [vp, C.c_int, _f32p, _f32p])
@@ -212,6 +215,12 @@ class Model:
             raise ValueError(self._f("last_error")().decode())
         return {"u": u, "iterations": its.value, "residual": res.value, "converged": ok == 1}
 
+    def element_energies(self, u):
+        """u_e^T K0 u_e per finest-level element (voxel order)."""
+        out = np.zeros(int(np.count_nonzero(self.density > 0)))
+        self._f("element_energies")(self.h, np.ascontiguousarray(u, np.float64), out)
+        return out
+
     def reactions(self, u, f=None):
         R = np.zeros(3)
         fa = None if f is None else np.ascontiguousarray(f, np.float64)
@@ -238,6 +247,32 @@ def prune_floating(dims, density, held, kind="orc"):
     out = np.zeros_like(density)
     removed = getattr(L, kind + "_prune_floating")(*[int(d) for d in dims], density, np.ascontiguousarray(held, np.uint8), out)
     return out, removed
+
+
+def topo_filter(dims, fill, rmin, x, transpose=False):
+    """The density filter (topology.cpp:21-79) over the solid voxels of fill, or its transpose."""
+    fill = np.ascontiguousarray(fill, np.float32).reshape(-1)
+    x = np.ascontiguousarray(x, np.float64)
+    out = np.zeros_like(x)
+    d = np.ascontiguousarray([int(v) for v in dims], np.int32)
+    lib("orc").orc_topo_filter(d.ctypes.data_as(C.c_void_p), fill, float(rmin), int(bool(transpose)), x, out)
+    return out
+
+
+def optimize_topology(dims, fill, keep, bc, f, nu=0.3, vol_frac=0.4, penal=3.0, rmin=1.5, move=0.2, tol=0.01, max_iter=40, coarsest_max_dof=1100,
+                      kind="orc"):
+    """optimizeTopology (topology.cpp:114-183) with the VoxelFEA solve: (density, history rows)."""
+    fill = np.ascontiguousarray(fill, np.float32).reshape(-1)
+    density = np.zeros_like(fill)
+    hist = np.zeros(4 * max(1, int(max_iter)))
+    d = np.ascontiguousarray([int(v) for v in dims], np.int32)
+    n = getattr(lib(kind), kind + "_optimize_topology")(d.ctypes.data_as(C.c_void_p), fill, np.ascontiguousarray(keep, np.uint8).reshape(-1),
+                                                        np.ascontiguousarray(bc, np.uint8).reshape(-1), np.ascontiguousarray(f, np.float64).reshape(-1),
+                                                        float(nu), float(vol_frac), float(penal), float(rmin), float(move), float(tol), int(max_iter),
+                                                        int(coarsest_max_dof), density, hist)
+    if n < 0:
+        raise ValueError(getattr(lib(kind), kind + "_last_error")().decode())
+    return density, hist[:4 * n].reshape(n, 4)
 
 
 def grid_for_box(mn, mx, n, pad=0, kind="orc"):

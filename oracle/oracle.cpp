@@ -22,6 +22,8 @@
 //                      bitwise u + iteration count vs oracle/_ref and the golden fixtures
 //   stresses           patch test (linear u -> constant sigma); M c / I; vs oracle/_ref
 //   prune_floating     known-answer 4x1x1 case (test_fea.cpp:115-122); brute force BFS; vs _ref
+//   topology           energies sum to the compliance f.u; the filter keeps constants and its
+//                      transpose is its adjoint; the whole loop bitwise vs oracle/_ref
 //   voxelize           sphere volume within 1 % (test_model.cpp:52-77); an axis-aligned box's exact
 //                      fractions (closed form); point-in-mesh brute force per sample; vs _ref
 #include <algorithm>
@@ -1390,6 +1392,183 @@ static void voxelize(const float* pos, const uint32_t* index, i64 nTri, const do
             }
 }
 
+// ---- topology optimization (topology.cpp:21-101, 104-183) ----
+constexpr double TOPO_XMIN = 1e-3;
+
+// topology.cpp:92-112: u_e^T K0 u_e per finest-level element, elements in voxel order
+static void elementEnergies(const Model& M, const double* u, double* out) {
+    const Level& L = M.levels[0];
+    const double* K = M.el.K;
+    for (size_t e = 0; e < L.elems.size(); e++) {
+        double ue[24];
+        for (int a = 0; a < 8; a++) {
+            const i64 n = 3 * (L.base[e] + L.off[a]);
+            ue[3 * a] = u[n];
+            ue[3 * a + 1] = u[n + 1];
+            ue[3 * a + 2] = u[n + 2];
+        }
+        double sum = 0;
+        for (int r = 0; r < 24; r++) {
+            double t = 0;
+            for (int c = 0; c < 24; c++) t += K[r * 24 + c] * ue[c];
+            sum += ue[r] * t;
+        }
+        out[e] = sum;
+    }
+}
+
+// topology.cpp:21-52: weights max(0, rmin - distance) over the solid voxels within the ball, in the
+// offset order k, j, i; W = the row sums
+struct TopoFilter {
+    std::vector<int> start, nbr;
+    std::vector<double> wts, W;
+};
+static TopoFilter densityFilter(const int* dims, const std::vector<int>& ids, double rmin) {
+    const int nx = dims[0], ny = dims[1], nz = dims[2];
+    const int r = int(std::ceil(rmin)) - 1;
+    struct Off { int i, j, k; double w; };
+    std::vector<Off> offs;
+    for (int k = -r - 1; k <= r + 1; k++)
+        for (int j = -r - 1; j <= r + 1; j++)
+            for (int i = -r - 1; i <= r + 1; i++) {
+                const double w = rmin - std::sqrt(double(i * i + j * j + k * k));
+                if (w > 0) offs.push_back({i, j, k, w});
+            }
+    std::vector<int> pos(size_t(nx) * ny * nz, -1);
+    for (size_t q = 0; q < ids.size(); q++) pos[ids[q]] = int(q);
+    TopoFilter F;
+    F.start.assign(ids.size() + 1, 0);
+    for (size_t q = 0; q < ids.size(); q++) {
+        const int e = ids[q];
+        const int i = e % nx, j = (e / nx) % ny, k = e / (nx * ny);
+        for (const auto& o : offs) {
+            const int a = i + o.i, b = j + o.j, c = k + o.k;
+            if (a < 0 || b < 0 || c < 0 || a >= nx || b >= ny || c >= nz) continue;
+            const int p = pos[a + nx * (b + ny * c)];
+            if (p >= 0) {
+                F.nbr.push_back(p);
+                F.wts.push_back(o.w);
+            }
+        }
+        F.start[q + 1] = int(F.nbr.size());
+    }
+    F.W.assign(ids.size(), 0.0);
+    for (size_t q = 0; q < ids.size(); q++)
+        for (int t = F.start[q]; t < F.start[q + 1]; t++) F.W[q] += F.wts[t];
+    return F;
+}
+// topology.cpp:54-63 (filter) and 65-79 (its transpose, gathered)
+static void applyFilter(const TopoFilter& F, const double* x, double* out, size_t m) {
+    for (size_t q = 0; q < m; q++) {
+        double sum = 0;
+        for (int t = F.start[q]; t < F.start[q + 1]; t++) sum += F.wts[t] * x[F.nbr[t]];
+        out[q] = sum / F.W[q];
+    }
+}
+static void applyFilterT(const TopoFilter& F, const double* g, double* out, size_t m) {
+    for (size_t q = 0; q < m; q++) {
+        double sum = 0;
+        for (int t = F.start[q]; t < F.start[q + 1]; t++) {
+            const int p = F.nbr[t];
+            sum += F.wts[t] * g[p] / F.W[p];
+        }
+        out[q] = sum;
+    }
+}
+
+// topology.cpp:114-183 with the solve done by this restatement's VoxelFEA on the same density
+// (no pruning: every solid voxel keeps a density of at least TOPO_XMIN * fill). Returns the final
+// physical densities per voxel and the compliance / volume / change history.
+static int optimizeTopology(const int* dims, const float* fill, const uint8_t* keep, const uint8_t* bc, const double* f, double nu, double volFrac,
+                            double penal, double rmin, double move, double tol, int maxIter, i64 coarsestMaxDof, float* densityOut, double* history) {
+    const i64 nVox = i64(dims[0]) * dims[1] * dims[2];
+    std::vector<int> solid;
+    for (i64 e = 0; e < nVox; e++)
+        if (fill[e] > 0) solid.push_back(int(e));
+    const size_t m = solid.size();
+    const TopoFilter F = densityFilter(dims, solid, rmin);
+    std::vector<double> vol(m);
+    std::vector<uint8_t> isFree(m, 0);
+    double totalVol = 0, keptVol = 0;
+    for (size_t q = 0; q < m; q++) {
+        vol[q] = fill[solid[q]];
+        totalVol += vol[q];
+        if (keep[solid[q]]) keptVol += vol[q];
+        else isFree[q] = 1;
+    }
+    const double target = std::max(volFrac * totalVol, keptVol * 1.02);
+    const double start = std::min(1.0, std::max(TOPO_XMIN, (target - keptVol) / std::max(1e-12, totalVol - keptVol)));
+    std::vector<double> x(m), xPhys(m), dc(m), dcx(m), dv(m), dvx(m), xNew(m), phys(m);
+    for (size_t q = 0; q < m; q++) x[q] = isFree[q] ? start : 1;
+    std::vector<float> density(nVox, 0.f);
+    std::vector<double> u0, u;
+    const i64 nDof = 3 * i64(dims[0] + 1) * (dims[1] + 1) * (dims[2] + 1);
+    std::vector<double> byVoxel(nVox, 0.0);
+    double change = 1;
+    int iters = 0;
+    for (int it = 0; it < maxIter; it++) {
+        applyFilter(F, x.data(), xPhys.data(), m);
+        for (size_t q = 0; q < m; q++)
+            if (!isFree[q]) xPhys[q] = 1;
+        for (size_t q = 0; q < m; q++)
+            density[solid[q]] = float(std::min(1.0, fill[solid[q]] * (TOPO_XMIN + (1 - TOPO_XMIN) * std::pow(xPhys[q], penal))));
+        Model M;
+        build(M, dims[0], dims[1], dims[2], density.data(), nu, bc, coarsestMaxDof, nullptr);
+        u.assign(nDof, 0.0);
+        solve(M, f, u0.empty() ? nullptr : u0.data(), 1e-8, 1000, u.data());
+        const Level& L = M.levels[0];
+        std::vector<double> energy(L.elems.size());
+        elementEnergies(M, u.data(), energy.data());
+        std::fill(byVoxel.begin(), byVoxel.end(), 0.0);
+        for (size_t t = 0; t < L.elems.size(); t++) byVoxel[L.elems[t]] = energy[t];
+        double c = 0;
+        for (i64 i = 0; i < nDof; i++) c += f[i] * u[i];
+        double used = 0;
+        for (size_t q = 0; q < m; q++) {
+            dc[q] = -penal * (1 - TOPO_XMIN) * fill[solid[q]] * std::pow(xPhys[q], penal - 1) * byVoxel[solid[q]];
+            dv[q] = vol[q];
+            used += vol[q] * xPhys[q];
+        }
+        u0 = u;
+        history[4 * it] = it;
+        history[4 * it + 1] = c;
+        history[4 * it + 2] = used / totalVol;
+        history[4 * it + 3] = change;
+        iters = it + 1;
+        if (it > 4 && change < tol) break;
+        applyFilterT(F, dc.data(), dcx.data(), m);
+        applyFilterT(F, dv.data(), dvx.data(), m);
+        double l1 = 0, l2 = 1e9;
+        while ((l2 - l1) / (l1 + l2 + 1e-30) > 1e-4) {
+            const double lm = 0.5 * (l1 + l2);
+            for (size_t q = 0; q < m; q++) {
+                if (!isFree[q]) {
+                    xNew[q] = 1;
+                    continue;
+                }
+                const double B = std::sqrt(std::max(0.0, -dcx[q]) / (lm * dvx[q] + 1e-300));
+                xNew[q] = std::max(0.0, std::max(x[q] - move, std::min(1.0, std::min(x[q] + move, x[q] * B))));
+            }
+            applyFilter(F, xNew.data(), phys.data(), m);
+            double v = 0;
+            for (size_t q = 0; q < m; q++) v += vol[q] * (isFree[q] ? phys[q] : 1);
+            if (v > target) l1 = lm;
+            else l2 = lm;
+        }
+        change = 0;
+        for (size_t q = 0; q < m; q++) {
+            change = std::max(change, std::abs(xNew[q] - x[q]));
+            x[q] = xNew[q];
+        }
+    }
+    applyFilter(F, x.data(), xPhys.data(), m);
+    for (size_t q = 0; q < m; q++)
+        if (!isFree[q]) xPhys[q] = 1;
+    for (i64 e = 0; e < nVox; e++) densityOut[e] = 0.f;
+    for (size_t q = 0; q < m; q++) densityOut[solid[q]] = float(xPhys[q]);
+    return iters;
+}
+
 }  // namespace orc
 
 // ---------------------------------------------------------------------------------------------
@@ -1544,6 +1723,26 @@ int orc_voxelize(const float* pos, int64_t nVert, const uint32_t* index, int64_t
     (void)nVert;
     orc::voxelize(pos, index, nTri, origin, h, dims, sub, frac);
     return 0;
+}
+
+void orc_element_energies(void* h, const double* u, double* out) { orc::elementEnergies(*static_cast<orc::Model*>(h), u, out); }
+// the density filter over the solid voxels (fill > 0) and its transpose; x, out: one per solid voxel
+void orc_topo_filter(const int* dims, const float* fill, double rmin, int transpose, const double* x, double* out) {
+    std::vector<int> ids;
+    for (int64_t e = 0; e < int64_t(dims[0]) * dims[1] * dims[2]; e++)
+        if (fill[e] > 0) ids.push_back(int(e));
+    const auto F = orc::densityFilter(dims, ids, rmin);
+    if (transpose) orc::applyFilterT(F, x, out, ids.size());
+    else orc::applyFilter(F, x, out, ids.size());
+}
+int orc_optimize_topology(const int* dims, const float* fill, const uint8_t* keep, const uint8_t* bc, const double* f, double nu, double volFrac,
+                          double penal, double rmin, double move, double tol, int maxIter, int64_t coarsestMaxDof, float* density, double* history) {
+    try {
+        return orc::optimizeTopology(dims, fill, keep, bc, f, nu, volFrac, penal, rmin, move, tol, maxIter, coarsestMaxDof, density, history);
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
 }
 
 void orc_cheb_coefficients(double lmax, double* out3) {

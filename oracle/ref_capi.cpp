@@ -28,6 +28,7 @@
 #undef private
 #include "fea/voxel_fea.cpp"
 #include "core/voxelize.hpp"
+#include "fea/topology.hpp"
 
 static std::string g_err;
 
@@ -233,6 +234,56 @@ int ref_timed_phases(int nx, int ny, int nz, const float* density, double nu, co
 }
 
 unsigned ref_threads() { return ps::ThreadPool::instance().size(); }
+
+void ref_element_energies(void* h, const double* u, double* out) {
+    auto* M = static_cast<ps::VoxelFEA*>(h);
+    const std::vector<double> uv(u, u + M->nDof());
+    const auto e = ps::elementEnergies(*M, uv);
+    std::copy(e.begin(), e.end(), out);
+}
+
+// the reference's optimizeTopology with its own VoxelFEA as the solve (tol 1e-8, 1000 iterations,
+// warm-started as the loop asks); history: it, compliance, volume, change per iteration
+int ref_optimize_topology(const int* dims, const float* fill, const uint8_t* keep, const uint8_t* bc, const double* f, double nu, double volFrac,
+                          double penal, double rmin, double move, double tol, int maxIter, int64_t coarsestMaxDof, float* density, double* history) {
+    try {
+        const int64_t nE = int64_t(dims[0]) * dims[1] * dims[2], nD = 3 * int64_t(dims[0] + 1) * (dims[1] + 1) * (dims[2] + 1);
+        ps::TopologyOptions o;
+        o.dims = {dims[0], dims[1], dims[2]};
+        o.fill.assign(fill, fill + nE);
+        o.keep.assign(keep, keep + nE);
+        o.volFrac = volFrac;
+        o.penal = penal;
+        o.rmin = rmin;
+        o.move = move;
+        o.tol = tol;
+        o.maxIter = maxIter;
+        const std::vector<uint8_t> b(bc, bc + nD);
+        const std::vector<double> fv(f, f + nD);
+        std::unique_ptr<ps::VoxelFEA> fea;
+        o.solve = [&](const std::vector<float>& dens, const std::vector<double>* x0) {
+            fea = std::make_unique<ps::VoxelFEA>(o.dims, dens, nu, b, coarsestMaxDof, nullptr);
+            ps::SolveOptions so;
+            so.tol = 1e-8;
+            so.maxIter = 1000;
+            so.x0 = x0;
+            auto r = fea->solve(fv, so);
+            return ps::TopologySolve{std::move(r.u), fv, fea.get()};
+        };
+        const auto res = ps::optimizeTopology(o);
+        std::copy(res.density.begin(), res.density.end(), density);
+        for (size_t q = 0; q < res.history.size(); q++) {
+            history[4 * q] = res.history[q].it;
+            history[4 * q + 1] = res.history[q].compliance;
+            history[4 * q + 2] = res.history[q].volume;
+            history[4 * q + 3] = res.history[q].change;
+        }
+        return int(res.history.size());
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+}
 
 void ref_grid_for_box(const double* mn, const double* mx, int n, int pad, double* origin, double* h, int* dims) {
     const ps::Grid g = ps::gridForBox({mn[0], mn[1], mn[2]}, {mx[0], mx[1], mx[2]}, n, pad);

@@ -456,3 +456,56 @@ def test_voxelize_bitwise_vs_reference(cpu, mesh):
         gr = cpu.grid_for_box(mn, mx, n, pad, kind="ref")
         assert np.array_equal(g[0], gr[0]) and g[1] == gr[1] and g[2] == gr[2]
         assert np.array_equal(cpu.voxelize(pos, idx, *g, sub), cpu.voxelize(pos, idx, *g, sub, kind="ref"))
+
+
+# ---- topology optimization (topology.cpp:21-183) ----
+
+def _topo_case(W, dims=(24, 8, 6)):
+    nx, ny, nz = dims
+    bc = W.clamp_x0(dims)
+    NX, NY, NZ = nx + 1, ny + 1, nz + 1
+    f = np.zeros((NZ, NY, NX, 3))
+    f[:, :, nx, 2] = -1.0 / (NY * NZ)  # a downward load over the free end
+    fill = np.ones(nx * ny * nz, np.float32)
+    keep = np.zeros(nx * ny * nz, np.uint8)
+    keep.reshape(nz, ny, nx)[:, :, 0] = 1  # the clamped layer stays solid
+    keep.reshape(nz, ny, nx)[:, :, -1] = 1  # and the loaded one
+    return dims, fill, keep, bc, f.reshape(-1)
+
+
+def test_element_energies_sum_to_compliance(cpu, W):
+    dims, fill, keep, bc, f = _topo_case(W)
+    rho = np.random.default_rng(2).uniform(0.2, 1.0, fill.size).astype(np.float32)
+    m = cpu.Model(dims, rho, 0.3, bc, 300)
+    u = m.solve(f, tol=1e-12)["u"]
+    e = m.element_energies(u)
+    assert abs((rho.astype(np.float64) * e).sum() - f @ u) <= 1e-9 * abs(f @ u)
+    if ref_available():
+        r = cpu.Model(dims, rho, 0.3, bc, 300, kind="ref")
+        assert np.array_equal(e, r.element_energies(u))
+
+
+def test_density_filter_constant_and_adjoint(cpu):
+    dims = (9, 7, 5)
+    fill = (np.random.default_rng(1).uniform(0, 1, 9 * 7 * 5) > 0.3).astype(np.float32)
+    m = int((fill > 0).sum())
+    for rmin in (1.0, 1.5, 2.4):
+        assert np.allclose(cpu.topo_filter(dims, fill, rmin, np.full(m, 0.37)), 0.37, rtol=1e-14)
+        x, y = np.random.default_rng(3).standard_normal((2, m))
+        Fx = cpu.topo_filter(dims, fill, rmin, x)
+        Fty = cpu.topo_filter(dims, fill, rmin, y, transpose=True)
+        assert abs(Fx @ y - x @ Fty) <= 1e-12 * np.abs(x).sum() * np.abs(y).max()
+    # rmin <= 1: the ball holds only the voxel itself
+    x = np.random.default_rng(4).standard_normal(m)
+    assert np.array_equal(cpu.topo_filter(dims, fill, 1.0, x), x)
+
+
+@pytest.mark.skipif(not ref_available(), reason="oracle/_ref not built")
+def test_optimize_topology_bitwise_vs_reference(cpu, W):
+    dims, fill, keep, bc, f = _topo_case(W)
+    kw = dict(nu=0.3, vol_frac=0.4, penal=3.0, rmin=1.5, move=0.2, tol=0.01, max_iter=6, coarsest_max_dof=300)
+    d, h = cpu.optimize_topology(dims, fill, keep, bc, f, **kw)
+    dr, hr = cpu.optimize_topology(dims, fill, keep, bc, f, kind="ref", **kw)
+    print(h, flush=True)
+    assert np.array_equal(d, dr) and np.array_equal(h, hr)
+    assert h[-1, 2] < 0.45 and h[-1, 1] < h[1, 1] * 5
