@@ -161,6 +161,40 @@ int psx_grid_for_box(const double mn[3], const double mx[3], int n, int pad, dou
 int psx_voxelize(const float* pos, int64_t n_vertices, const uint32_t* index, int64_t n_tris, const double origin[3], double h, const int dims[3], int sub,
                  float* frac);
 
+/* ---- topology optimization (optimizeTopology, topology.cpp:104-183) ----
+ * SIMP densities on the voxels with fill > 0 (kept voxels stay 1), compliance minimised for a volume
+ * budget by optimality-criteria updates with a density filter of radius rmin voxels. Each iteration
+ * builds the part's hierarchy from fill * (xmin + (1 - xmin) x^penal), xmin = 1e-3, solves on the device
+ * (warm-started from the previous solution) and runs the energies, filter, sensitivities and the
+ * volume bisection as kernels on HBM-resident arrays. Zero fields take the defaults. */
+typedef int (*psx_topology_fn)(int it, double compliance, double volume, double change, void* user); /* nonzero stops */
+typedef struct {
+    double vol_frac;            /* volume budget as a fraction of the solid volume (0 = 0.4) */
+    double penal;               /* SIMP exponent (0 = 3) */
+    double rmin;                /* filter radius in voxels (0 = 1.5) */
+    double move;                /* largest change of a density per iteration (0 = 0.2) */
+    double tol;                 /* stop when the largest change is below this, after iteration 5 (0 = 0.01) */
+    int max_iter;               /* 0 = 40 */
+    int precision;              /* enum psx_precision of the static solves */
+    double solve_tol;           /* relative residual of the static solves (0 = 1e-8) */
+    int64_t coarsest_max_dof;   /* 0 = 1100 */
+    psx_topology_fn on_iter;    /* TopologyOptions::onIter (topology.hpp:48), or NULL */
+    void* user;
+} psx_topology_opts;
+/* dims, fill [nx*ny*nz], keep [nx*ny*nz] (1 = stays solid), bc [3*nodes], f [3*nodes], nu as for
+ * psx_create. density receives the final filtered densities per voxel (0 outside the part); history
+ * receives 4 doubles per iteration (iteration, compliance f.u, volume fraction, change) and must hold
+ * 4 * max_iter; iterations receives the count; kept_fraction (may be NULL) the kept share of the
+ * solid volume. */
+int psx_optimize_topology(const int dims[3], const float* fill, const uint8_t* keep, const uint8_t* bc, const double* f, double nu,
+                          const psx_topology_opts* opts, float* density, double* history, int* iterations, double* kept_fraction);
+/* Strain energies u_e^T K0 u_e of the finest level's elements (elementEnergies, topology.cpp:92-102),
+ * elements in voxel order, n_elems values, FP64. */
+int psx_element_energies(psx_model m, const double* u, double* out);
+/* The density filter (applyFilter, topology.cpp:54-63) or its transpose (applyFilterT, topology.cpp:
+ * 65-79) over the voxels with fill > 0 in voxel order: x and out hold one value per such voxel. */
+int psx_topology_filter(const int dims[3], const float* fill, double rmin, int transpose, const double* x, double* out);
+
 /* Support reactions: sum over held active DOFs of (K u)_raw - f (VoxelFEA::reactions,
  * voxel_fea.cpp:929-940). f may be NULL. */
 int psx_reactions(psx_model m, const double* u, const double* f, double out[3]);

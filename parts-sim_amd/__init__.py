@@ -37,6 +37,15 @@ class SolveOpts(C.Structure):
                 ("mfma_smoother", C.c_int), ("eager", C.c_int), ("pre_degree", C.c_int)]
 
 
+TOPO_ITER = C.CFUNCTYPE(C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_void_p)
+
+
+class TopologyOpts(C.Structure):
+    _fields_ = [("vol_frac", C.c_double), ("penal", C.c_double), ("rmin", C.c_double), ("move", C.c_double), ("tol", C.c_double),
+                ("max_iter", C.c_int), ("precision", C.c_int), ("solve_tol", C.c_double), ("coarsest_max_dof", C.c_int64),
+                ("on_iter", TOPO_ITER), ("user", C.c_void_p)]
+
+
 class ServerOpts(C.Structure):
     _fields_ = [("max_inflight", C.c_int), ("hbm_budget", C.c_int64)]
 
@@ -98,6 +107,9 @@ SIGNATURES = {
     "psx_prune_floating": (_i, [_i, _i, _i, _vp, _vp, _vp]),
     "psx_grid_for_box": (_i, [_vp, _vp, _i, _i, _vp, _vp, _vp]),
     "psx_voxelize": (_i, [_vp, C.c_int64, _vp, C.c_int64, _vp, C.c_double, _vp, _i, _vp]),
+    "psx_optimize_topology": (_i, [_vp, _vp, _vp, _vp, _vp, _d, C.POINTER(TopologyOpts), _vp, _vp, C.POINTER(_i), C.POINTER(_d)]),
+    "psx_element_energies": (_i, [_vp, _vp, _vp]),
+    "psx_topology_filter": (_i, [_vp, _vp, _d, _i, _vp, _vp]),
     "psx_hex_element": (_i, [_d, _vp, _vp]),
     "psx_hierarchy_create": (_i, [C.POINTER(ModelDesc), C.POINTER(_vp)]),
     "psx_hierarchy_destroy": (None, [_vp]),
@@ -201,6 +213,46 @@ def voxelize(pos, index, origin, h, dims, sub=2):
     frac = np.zeros(int(d[0]) * int(d[1]) * int(d[2]), np.float32)
     _check(lib().psx_voxelize(_p(pos), pos.size // 3, _p(index), index.size // 3, _p(o), float(h), _p(d), int(sub), _p(frac)))
     return frac
+
+
+def optimize_topology(dims, fill, keep, bc, f, nu, vol_frac=0.4, penal=3.0, rmin=1.5, move=0.2, tol=0.01, max_iter=40, precision="mixed",
+                      solve_tol=1e-8, coarsest_max_dof=1100, on_iter=None):
+    """SIMP topology optimization (optimizeTopology): the static solves, energies, filter and OC bisection
+    on the device. Returns (density per voxel, history rows [it, compliance, volume, change], kept fraction).
+    on_iter(it, compliance, volume, change) returning True stops the loop."""
+    d = np.ascontiguousarray([int(v) for v in dims], np.int32)
+    nV = int(d[0]) * int(d[1]) * int(d[2])
+    fill = np.ascontiguousarray(fill, np.float32).reshape(-1)
+    keep = np.ascontiguousarray(keep, np.uint8).reshape(-1)
+    bc = np.ascontiguousarray(bc, np.uint8).reshape(-1)
+    f = np.ascontiguousarray(f, np.float64).reshape(-1)
+    nDof = 3 * (int(d[0]) + 1) * (int(d[1]) + 1) * (int(d[2]) + 1)
+    if fill.size != nV or keep.size != nV or bc.size != nDof or f.size != nDof:
+        raise ValueError("fill and keep need one value per voxel, bc and f one per DOF")
+    o = TopologyOpts(vol_frac=vol_frac, penal=penal, rmin=rmin, move=move, tol=tol, max_iter=max_iter, precision=PREC[precision],
+                     solve_tol=solve_tol, coarsest_max_dof=coarsest_max_dof)
+    cb = None
+    if on_iter is not None:
+        cb = TOPO_ITER(lambda it, c, v, ch, _u: 1 if on_iter(it, c, v, ch) else 0)
+        o.on_iter = cb
+    density = np.zeros(nV, np.float32)
+    history = np.zeros(4 * max(1, int(max_iter)))
+    iters, kept = C.c_int(), C.c_double()
+    _check(lib().psx_optimize_topology(_p(d), _p(fill), _p(keep), _p(bc), _p(f), float(nu), C.byref(o), _p(density), _p(history),
+                                       C.byref(iters), C.byref(kept)))
+    return density, history[:4 * iters.value].reshape(-1, 4), kept.value
+
+
+def topology_filter(dims, fill, rmin, x, transpose=False):
+    """The density filter (or its transpose) over the voxels with fill > 0, on the device."""
+    d = np.ascontiguousarray([int(v) for v in dims], np.int32)
+    fill = np.ascontiguousarray(fill, np.float32).reshape(-1)
+    x = np.ascontiguousarray(x, np.float64).reshape(-1)
+    if x.size != int((fill > 0).sum()):
+        raise ValueError("x needs one value per voxel with fill > 0")
+    out = np.zeros_like(x)
+    _check(lib().psx_topology_filter(_p(d), _p(fill), float(rmin), int(bool(transpose)), _p(x), _p(out)))
+    return out
 
 
 def hex_element(nu):
@@ -387,6 +439,12 @@ class Model:
         fa = None if f is None else np.ascontiguousarray(f, np.float64)
         _check(lib().psx_reactions(self.h, _p(np.ascontiguousarray(u, np.float64)), _p(fa), _p(R)))
         return R
+
+    def element_energies(self, u):
+        """u_e^T K0 u_e of the finest level's elements in voxel order (FP64, on the device)."""
+        out = np.zeros(self.level(0)["nElems"])
+        _check(lib().psx_element_energies(self.h, _p(np.ascontiguousarray(u, np.float64)), _p(out)))
+        return out
 
     def stresses(self, u, E, h, principals=True):
         """Nodal von Mises (and P1, P3) of a solution u, float32 per node."""

@@ -3,11 +3,17 @@
 #pragma once
 
 #include <cmath>
+#include <memory>
 #include <stdexcept>
 #include <string>
 
 #include "../../include/psx.h"
 #include "model.hpp"
+
+// the opaque handle of include/psx.h: one device-resident model
+struct psx_model_s {
+    std::unique_ptr<psx::Model> m;
+};
 
 namespace psx::abi {
 inline thread_local std::string g_error;

@@ -22,10 +22,6 @@
 // =================================================================================================
 // C ABI
 
-struct psx_model_s {
-    std::unique_ptr<psx::Model> m;
-};
-
 using namespace psx::abi;
 
 extern "C" {
