@@ -167,7 +167,9 @@ int psx_voxelize(const float* pos, int64_t n_vertices, const uint32_t* index, in
  * builds the part's hierarchy from fill * (xmin + (1 - xmin) x^penal), xmin = 1e-3, solves on the device
  * (warm-started from the previous solution) and runs the energies, filter, sensitivities and the
  * volume bisection as kernels on HBM-resident arrays. Zero fields take the defaults. */
-typedef int (*psx_topology_fn)(int it, double compliance, double volume, double change, void* user); /* nonzero stops */
+/* nonzero stops; density = the iteration's filtered densities per voxel (TopologyIter::xPhys scattered to
+ * the voxels, 0 outside the part), valid during the call */
+typedef int (*psx_topology_fn)(int it, double compliance, double volume, double change, const float* density, void* user);
 typedef struct {
     double vol_frac;            /* volume budget as a fraction of the solid volume (0 = 0.4) */
     double penal;               /* SIMP exponent (0 = 3) */
@@ -180,6 +182,7 @@ typedef struct {
     int64_t coarsest_max_dof;   /* 0 = 1100 */
     psx_topology_fn on_iter;    /* TopologyOptions::onIter (topology.hpp:48), or NULL */
     void* user;
+    int solve_max_iter;         /* CG iteration cap of the static solves (0 = 1000) */
 } psx_topology_opts;
 /* dims, fill [nx*ny*nz], keep [nx*ny*nz] (1 = stays solid), bc [3*nodes], f [3*nodes], nu as for
  * psx_create. density receives the final filtered densities per voxel (0 outside the part); history

@@ -37,13 +37,13 @@ class SolveOpts(C.Structure):
                 ("mfma_smoother", C.c_int), ("eager", C.c_int), ("pre_degree", C.c_int)]
 
 
-TOPO_ITER = C.CFUNCTYPE(C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_void_p)
+TOPO_ITER = C.CFUNCTYPE(C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.POINTER(C.c_float), C.c_void_p)
 
 
 class TopologyOpts(C.Structure):
     _fields_ = [("vol_frac", C.c_double), ("penal", C.c_double), ("rmin", C.c_double), ("move", C.c_double), ("tol", C.c_double),
                 ("max_iter", C.c_int), ("precision", C.c_int), ("solve_tol", C.c_double), ("coarsest_max_dof", C.c_int64),
-                ("on_iter", TOPO_ITER), ("user", C.c_void_p)]
+                ("on_iter", TOPO_ITER), ("user", C.c_void_p), ("solve_max_iter", C.c_int)]
 
 
 class ServerOpts(C.Structure):
@@ -216,10 +216,11 @@ def voxelize(pos, index, origin, h, dims, sub=2):
 
 
 def optimize_topology(dims, fill, keep, bc, f, nu, vol_frac=0.4, penal=3.0, rmin=1.5, move=0.2, tol=0.01, max_iter=40, precision="mixed",
-                      solve_tol=1e-8, coarsest_max_dof=1100, on_iter=None):
+                      solve_tol=1e-8, coarsest_max_dof=1100, on_iter=None, solve_max_iter=1000):
     """SIMP topology optimization (optimizeTopology): the static solves, energies, filter and OC bisection
     on the device. Returns (density per voxel, history rows [it, compliance, volume, change], kept fraction).
-    on_iter(it, compliance, volume, change) returning True stops the loop."""
+    on_iter(it, compliance, volume, change, density) returning True stops the loop (density: the iteration's
+    filtered densities per voxel, a copy)."""
     d = np.ascontiguousarray([int(v) for v in dims], np.int32)
     nV = int(d[0]) * int(d[1]) * int(d[2])
     fill = np.ascontiguousarray(fill, np.float32).reshape(-1)
@@ -230,10 +231,10 @@ def optimize_topology(dims, fill, keep, bc, f, nu, vol_frac=0.4, penal=3.0, rmin
     if fill.size != nV or keep.size != nV or bc.size != nDof or f.size != nDof:
         raise ValueError("fill and keep need one value per voxel, bc and f one per DOF")
     o = TopologyOpts(vol_frac=vol_frac, penal=penal, rmin=rmin, move=move, tol=tol, max_iter=max_iter, precision=PREC[precision],
-                     solve_tol=solve_tol, coarsest_max_dof=coarsest_max_dof)
+                     solve_tol=solve_tol, coarsest_max_dof=coarsest_max_dof, solve_max_iter=solve_max_iter)
     cb = None
     if on_iter is not None:
-        cb = TOPO_ITER(lambda it, c, v, ch, _u: 1 if on_iter(it, c, v, ch) else 0)
+        cb = TOPO_ITER(lambda it, c, v, ch, dens, _u: 1 if on_iter(it, c, v, ch, np.ctypeslib.as_array(dens, (nV,)).copy()) else 0)
         o.on_iter = cb
     density = np.zeros(nV, np.float32)
     history = np.zeros(4 * max(1, int(max_iter)))

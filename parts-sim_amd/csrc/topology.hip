@@ -109,6 +109,12 @@ __global__ __launch_bounds__(BLOCK) void kSensitivity(int64_t m, const int32_t* 
     dc[q] = -penal * (1 - TOPO_XMIN) * fill[v] * pow(xPhys[q], penal - 1) * en;
 }
 
+// the filtered densities per voxel for the iteration callback (the voxels outside stay 0)
+__global__ __launch_bounds__(BLOCK) void kScatter(int64_t m, const int32_t* __restrict__ solid, const double* __restrict__ xPhys, float* __restrict__ out) {
+    const int64_t q = int64_t(blockIdx.x) * BLOCK + threadIdx.x;
+    if (q < m) out[solid[q]] = float(xPhys[q]);
+}
+
 // the optimality-criteria candidate for a multiplier lm
 __global__ __launch_bounds__(BLOCK) void kOc(int64_t m, const uint8_t* __restrict__ isFree, const double* __restrict__ x, const double* __restrict__ dcx,
                                              const double* __restrict__ dvx, double lm, double move, double* __restrict__ xNew) {
@@ -272,6 +278,7 @@ int psx_optimize_topology(const int dims[3], const float* fill, const uint8_t* k
         if (o.max_iter <= 0) o.max_iter = 40;
         if (o.solve_tol <= 0) o.solve_tol = 1e-8;
         if (o.coarsest_max_dof <= 0) o.coarsest_max_dof = 1100;
+        if (o.solve_max_iter <= 0) o.solve_max_iter = 1000;
         if (o.precision < PSX_PREC_FP64 || o.precision > PSX_PREC_MIXED) throw std::invalid_argument("unknown precision");
         const int64_t nV = int64_t(nx) * ny * nz, nDof = 3 * int64_t(nx + 1) * (ny + 1) * (nz + 1);
         abi::checkFinite(f, nDof, "Loads");
@@ -316,6 +323,8 @@ int psx_optimize_topology(const int dims[3], const float* fill, const uint8_t* k
         int32_t* ev = D.get<int32_t>(size_t(nV));
         double* energy = D.get<double>(size_t(nV));
         double* K0 = D.get<double>(576);
+        float* dShow = o.on_iter ? D.get<float>(size_t(nV)) : nullptr;
+        std::vector<float> hShow(o.on_iter ? size_t(nV) : 0);
         kFilter<2><<<grid(m), BLOCK, 0, D.s>>>(G, m, nullptr, nullptr, W);
         // dv = vol in every iteration, so its filtered transpose is computed once
         kFilter<1><<<grid(m), BLOCK, 0, D.s>>>(G, m, W, dVol, dvx);
@@ -351,7 +360,7 @@ int psx_optimize_topology(const int dims[3], const float* fill, const uint8_t* k
             HIPCHK(hipStreamSynchronize(D.s));
             psx_solve_opts so{};
             so.tol = o.solve_tol;
-            so.max_iter = 1000;
+            so.max_iter = o.solve_max_iter;
             so.precision = o.precision;
             psx_solve_info info{};
             M.solveDevice(dF, warm ? u : nullptr, so, u, info);
@@ -367,7 +376,13 @@ int psx_optimize_topology(const int dims[3], const float* fill, const uint8_t* k
             history[4 * it + 2] = used / totalVol;
             history[4 * it + 3] = change;
             iters = it + 1;
-            if (o.on_iter && o.on_iter(it, c, used / totalVol, change, o.user)) break;
+            if (o.on_iter) {
+                kScatter<<<grid(m), BLOCK, 0, D.s>>>(m, G.solid, xPhys, dShow);
+                HIPCHK(hipGetLastError());
+                HIPCHK(hipMemcpyAsync(hShow.data(), dShow, size_t(nV) * sizeof(float), hipMemcpyDeviceToHost, D.s));
+                HIPCHK(hipStreamSynchronize(D.s));
+                if (o.on_iter(it, c, used / totalVol, change, hShow.data(), o.user)) break;
+            }
             if (it > 4 && change < o.tol) break;
             kFilter<1><<<grid(m), BLOCK, 0, D.s>>>(G, m, W, dc, dcx);
             // optimality criteria: bisection on the volume multiplier, one candidate per step
