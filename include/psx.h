@@ -134,6 +134,11 @@ int psx_apply_bf16(psx_model m, int level, const float* x, float* y);
  * gpu_fea.cpp:431-450; VoxelFEA::precondition voxel_fea.cpp:830-861). */
 int psx_precondition(psx_model m, const double* r, double* z);
 int psx_precondition_device(psx_model m, const double* d_r, double* d_z);
+/* nvec V-cycles in one set of launches (a block preconditioner for LOBPCG, whose FullPreconditioner
+ * takes a block of residuals, eigen.cpp:508-553; the reference runs one V-cycle per vector): r and z
+ * hold nvec x n_dof values, vector-major, and each z equals psx_precondition of its r alone. */
+int psx_precondition_multi(psx_model m, int nvec, const double* r, double* z);
+int psx_precondition_multi_device(psx_model m, int nvec, const double* d_r, double* d_z);
 
 /* K u = f (GpuFeaSolver::solve gpu_fea.cpp:452-574, VoxelFEA::solve voxel_fea.cpp:863-927).
  * x0 may be NULL (zero start). opts may be NULL (defaults). */

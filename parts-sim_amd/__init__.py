@@ -89,6 +89,8 @@ SIGNATURES = {
     "psx_apply_child": (_i, [_vp, _i, _i, _vp, _vp, _i]),
     "psx_precondition": (_i, [_vp, _vp, _vp]),
     "psx_precondition_device": (_i, [_vp, _vp, _vp]),
+    "psx_precondition_multi": (_i, [_vp, _i, _vp, _vp]),
+    "psx_precondition_multi_device": (_i, [_vp, _i, _vp, _vp]),
     "psx_solve": (_i, [_vp, _vp, _vp, C.POINTER(SolveOpts), _vp, C.POINTER(SolveInfo)]),
     "psx_solve_device": (_i, [_vp, _vp, _vp, C.POINTER(SolveOpts), _vp, C.POINTER(SolveInfo)]),
     "psx_solve_multi": (_i, [_vp, _i, _vp, _vp, C.POINTER(SolveOpts), _vp, C.POINTER(SolveInfo)]),
@@ -374,6 +376,15 @@ class Model:
         z = np.zeros_like(r)
         _check(lib().psx_precondition(self.h, _p(r), _p(z)))
         return z
+
+    def precondition_multi(self, R):
+        """One V-cycle per row of R (nvec x n_dof) in one set of launches."""
+        R = np.ascontiguousarray(R, np.float64)
+        if R.ndim != 2 or R.shape[1] != self.n_dof:
+            raise ValueError("R must be nvec x n_dof")
+        Z = np.zeros_like(R)
+        _check(lib().psx_precondition_multi(self.h, R.shape[0], _p(R), _p(Z)))
+        return Z
 
     @staticmethod
     def _opts(tol, max_iter, precision, check_every, fold_every, on_progress, cheb_degree=0, mfma_smoother=False, eager=False, pre_degree=0):

@@ -415,6 +415,18 @@ public:
         launchFromF32(z32, dz, n, &st->rmax, stream, ni);
     }
 
+    // one V-cycle for each of k vectors (instance-major) in one set of launches; each result equals
+    // the vector's lone precondition32
+    void preconditionMulti(int k, const double* dr, double* dz) {
+        reserve(k);
+        ni = k;
+        struct Restore {
+            int& n;
+            ~Restore() { n = 1; }
+        } restore{ni};
+        precondition32(dr, dz, nullptr);
+    }
+
     // ----- conjugate gradients -----
 
     void readState() {

@@ -187,6 +187,35 @@ int psx_precondition(psx_model m, const double* r, double* z) {
     });
 }
 
+int psx_precondition_multi_device(psx_model m, int nvec, const double* d_r, double* d_z) {
+    return guarded([&] {
+        if (!m || !d_r || !d_z || nvec < 1) throw std::invalid_argument("bad arguments");
+        psx::Model& M = *m->m;
+        if (!M.smoothersReady) M.estimateSmoothers();
+        M.preconditionMulti(nvec, d_r, d_z);
+        HIPCHK(hipGetLastError());
+        M.sync();
+        return PSX_OK;
+    });
+}
+
+int psx_precondition_multi(psx_model m, int nvec, const double* r, double* z) {
+    return guarded([&] {
+        if (!m || !r || !z || nvec < 1) throw std::invalid_argument("bad arguments");
+        psx::Model& M = *m->m;
+        const int64_t n = M.nDof();
+        if (!M.smoothersReady) M.estimateSmoothers();
+        M.reserve(nvec);
+        const size_t bytes = size_t(n) * size_t(nvec) * 8;
+        HIPCHK(hipMemcpyAsync(M.f, r, bytes, hipMemcpyHostToDevice, M.stream));
+        M.preconditionMulti(nvec, M.f, M.u);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(z, M.u, bytes, hipMemcpyDeviceToHost, M.stream));
+        M.sync();
+        return PSX_OK;
+    });
+}
+
 int psx_solve_device(psx_model m, const double* d_f, const double* d_x0, const psx_solve_opts* opts, double* d_u, psx_solve_info* info) {
     return guarded([&] {
         if (!m || !d_f || !d_u) throw std::invalid_argument("bad arguments");

@@ -69,3 +69,17 @@ def test_multi_device_vectors(P, W):
     assert all(i["converged"] for i in infos)
     assert np.array_equal(d_u.download().reshape(F.shape), U)
     m.close()
+
+
+def test_block_precondition_equals_lone(P, W):
+    # the block V-cycle of a LOBPCG preconditioner: each vector's result is its lone V-cycle
+    for w in (W.simp_block((24, 12, 12), seed=3, nrhs=1), W.void_block((32, 16, 16), seed=2)):
+        m = _model(P, w)
+        R = np.random.default_rng(11).standard_normal((6, m.n_dof))
+        R[2] *= 1e-9  # the V-cycle scales each vector to order one on its own
+        Z = m.precondition_multi(R)
+        for k in range(R.shape[0]):
+            assert np.array_equal(Z[k], m.precondition(R[k])), k
+        # a smaller block after a larger one
+        assert np.array_equal(m.precondition_multi(R[4:6]), Z[4:6])
+        m.close()
