@@ -71,6 +71,8 @@ def lib(kind):
                                          C.POINTER(C.c_int), C.POINTER(C.c_double), _f64p])
     else:
         sig["topo_filter"] = (None, [vp, _f32p, dbl, C.c_int, _f64p, _f64p])
+        sig["break_test"] = (C.c_int, [vp, _f32p, dbl, _u8p, _f64p, dbl, dbl, dbl, C.c_int, C.c_int, dbl, C.c_int, i64, _f64p, vp, vp,
+                                       C.c_int, vp, C.c_int, vp, C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_int)])
         sig["set_threads"] = (None, [C.c_int])
         sig["set_cheb_degree"] = (None, [vp, C.c_int])
         sig["smooth"] = (None, [vp, C.c_int, _f32p, _f32p])
@@ -247,6 +249,42 @@ def prune_floating(dims, density, held, kind="orc"):
     out = np.zeros_like(density)
     removed = getattr(L, kind + "_prune_floating")(*[int(d) for d in dims], density, np.ascontiguousarray(held, np.uint8), out)
     return out, removed
+
+
+BREAK_REASONS = ("step limit", "separated", "unsupported load", "solver did not converge", "no stress")
+
+
+def break_test(dims, density, nu, bc, f, E, h, strength, principal=False, max_steps=5, tol=1e-6, max_iter=400, coarsest_max_dof=1100,
+               fields=False):
+    """breakTest (static_study.cpp:197-279) with the VoxelFEA solve: dict(reason, peak, removed, steps), each
+    step dict(lambda, max_disp, voxels, cracked, detached[, vm])."""
+    d = np.ascontiguousarray([int(v) for v in dims], np.int32)
+    nV = int(d[0]) * int(d[1]) * int(d[2])
+    nN = (int(d[0]) + 1) * (int(d[1]) + 1) * (int(d[2]) + 1)
+    scalars = np.zeros(3 * max(1, max_steps))
+    counts = np.zeros(2 * max(1, max_steps), np.int32)
+    cracked = np.zeros(max(1, max_steps) * nV, np.int32)
+    detached = np.zeros(max(1, max_steps) * nV, np.int32)
+    vm = np.zeros(max(1, max_steps) * nN, np.float32) if fields else None
+    peak, removed, reason = C.c_double(), C.c_int(), C.c_int()
+    n = lib("orc").orc_break_test(d.ctypes.data_as(C.c_void_p), np.ascontiguousarray(density, np.float32).reshape(-1), float(nu),
+                                  np.ascontiguousarray(bc, np.uint8).reshape(-1), np.ascontiguousarray(f, np.float64).reshape(-1), float(E),
+                                  float(h), float(strength), int(bool(principal)), int(max_steps), float(tol), int(max_iter),
+                                  int(coarsest_max_dof), scalars, _ptr(counts), _ptr(cracked), cracked.size, _ptr(detached), detached.size,
+                                  _ptr(vm), C.byref(peak), C.byref(removed), C.byref(reason))
+    if n < 0:
+        raise ValueError(lib("orc").orc_last_error().decode())
+    steps, ic, idt = [], 0, 0
+    for s in range(n):
+        nc, nd = int(counts[2 * s]), int(counts[2 * s + 1])
+        st = {"lambda": scalars[3 * s], "max_disp": scalars[3 * s + 1], "voxels": int(scalars[3 * s + 2]),
+              "cracked": cracked[ic:ic + nc].copy(), "detached": detached[idt:idt + nd].copy()}
+        if fields:
+            st["vm"] = vm[s * nN:(s + 1) * nN].copy()
+        steps.append(st)
+        ic += nc
+        idt += nd
+    return {"reason": BREAK_REASONS[reason.value], "peak": peak.value, "removed": removed.value, "steps": steps}
 
 
 def topo_filter(dims, fill, rmin, x, transpose=False):

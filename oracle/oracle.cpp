@@ -33,6 +33,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <numeric>
 #include <stdexcept>
 #include <string>
 #include <thread>
@@ -1569,6 +1570,108 @@ static int optimizeTopology(const int* dims, const float* fill, const uint8_t* k
     return iters;
 }
 
+// ---- break test (static_study.cpp:17-32, 138-143, 197-279) ----
+// The solve is this restatement's VoxelFEA solve (the CPU engine of buildAndSolve,
+// static_study.cpp:135-140) with the tolerance and iteration cap given.
+struct BreakStepOut {
+    double lambda = 0, maxDisp = 0;
+    int voxels = 0;
+    std::vector<int> cracked, detached;
+    std::vector<float> vm;  // nodal von Mises at lambda
+};
+enum BreakReason { BR_STEP_LIMIT = 0, BR_SEPARATED = 1, BR_UNSUPPORTED = 2, BR_NOT_CONVERGED = 3, BR_NO_STRESS = 4 };
+
+static double lostLoadFraction(const Level& L, const double* f) {
+    double lost = 0, total = 0;
+    for (size_t n = 0; n < L.activeNode.size(); n++) {
+        const double m = std::sqrt(f[3 * n] * f[3 * n] + f[3 * n + 1] * f[3 * n + 1] + f[3 * n + 2] * f[3 * n + 2]);
+        total += m;
+        if (!L.activeNode[n]) lost += m;
+    }
+    return total > 0 ? lost / total : 0;
+}
+
+static int breakTest(const int* dims, const float* density0, double nu, const uint8_t* bc, const double* f, double E, double h, double strength,
+                     bool principal, int maxSteps, double tol, int maxIter, i64 coarsestMaxDof, std::vector<BreakStepOut>& steps, double& peak,
+                     int& removed) {
+    if (!(strength > 0)) throw std::invalid_argument("Failure strength must be positive.");
+    const int nx = dims[0], ny = dims[1], nz = dims[2];
+    const i64 NX = nx + 1, NY = ny + 1, nV = i64(nx) * ny * nz;
+    const i64 nNodes = NX * NY * (nz + 1);
+    const i64 off[8] = {0, 1, 1 + NX, NX, NX * NY, 1 + NX * NY, 1 + NX + NX * NY, NX + NX * NY};
+    std::vector<uint8_t> held(nNodes);
+    for (i64 n = 0; n < nNodes; n++) held[n] = bc[3 * n] | bc[3 * n + 1] | bc[3 * n + 2];
+    std::vector<float> density(nV), next(nV), pruned(nV);
+    removed = pruneFloating(nx, ny, nz, density0, held.data(), density.data());
+    peak = 0;
+    steps.clear();
+    int reason = BR_STEP_LIMIT;
+    std::vector<double> x0;
+    for (int step = 0; step < maxSteps; step++) {
+        if (std::none_of(density.begin(), density.end(), [](float d) { return d > 0; })) {
+            reason = step > 0 ? BR_SEPARATED : BR_UNSUPPORTED;
+            break;
+        }
+        Model M;
+        build(M, nx, ny, nz, density.data(), nu, bc, coarsestMaxDof, nullptr);
+        const Level& L = M.levels[0];
+        std::vector<double> su(L.nDof);
+        const SolveOut so = solve(M, f, x0.empty() ? nullptr : x0.data(), tol, maxIter, su.data());
+        if (L.elems.empty() || lostLoadFraction(L, f) > 0.5) {
+            reason = step > 0 ? BR_SEPARATED : BR_UNSUPPORTED;
+            break;
+        }
+        if (!so.converged) {
+            reason = BR_NOT_CONVERGED;
+            break;
+        }
+        std::vector<double> u(su.size());
+        const double scale = 1 / (E * h);
+        for (size_t i = 0; i < u.size(); i++) u[i] = su[i] * scale;
+        std::vector<float> vm(L.nNodes), p1(L.nNodes), p3(L.nNodes);
+        stresses(M, u.data(), E, h, true, vm.data(), p1.data(), p3.data());
+        x0 = su;
+        const std::vector<float>& nodal = principal ? p1 : vm;
+        std::vector<float> measure(L.elems.size());
+        double smax = 0;
+        for (size_t q = 0; q < L.elems.size(); q++) {
+            float m = 0;
+            for (int a = 0; a < 8; a++) m = std::max(m, nodal[L.base[q] + off[a]]);
+            measure[q] = m;
+            smax = std::max(smax, double(m));
+        }
+        if (!(smax > 0)) {
+            reason = BR_NO_STRESS;
+            break;
+        }
+        BreakStepOut bs;
+        bs.lambda = strength / smax;
+        peak = std::max(peak, bs.lambda);
+        const size_t cap = std::max<size_t>(1, size_t(std::lround(0.02 * std::pow(double(L.elems.size()), 2.0 / 3))));
+        std::vector<int> order(L.elems.size());
+        std::iota(order.begin(), order.end(), 0);
+        std::sort(order.begin(), order.end(), [&](int a, int b) { return measure[a] > measure[b]; });
+        double best = 0;
+        for (size_t i = 0; i + 2 < u.size(); i += 3) best = std::max(best, u[i] * u[i] + u[i + 1] * u[i + 1] + u[i + 2] * u[i + 2]);
+        bs.maxDisp = std::sqrt(best) * bs.lambda;
+        for (int q : order) {
+            if (bs.cracked.size() >= cap || measure[q] < 0.95 * smax) break;
+            bs.cracked.push_back(L.elems[q]);
+        }
+        next = density;
+        for (int e : bs.cracked) next[e] = 0;
+        pruneFloating(nx, ny, nz, next.data(), held.data(), pruned.data());
+        for (i64 e = 0; e < nV; e++)
+            if (next[e] > 0 && !(pruned[e] > 0)) bs.detached.push_back(int(e));
+        density = pruned;
+        bs.vm = vm;
+        for (auto& v : bs.vm) v = float(v * bs.lambda);
+        bs.voxels = int(L.elems.size());
+        steps.push_back(std::move(bs));
+    }
+    return reason;
+}
+
 }  // namespace orc
 
 // ---------------------------------------------------------------------------------------------
@@ -1739,6 +1842,38 @@ int orc_optimize_topology(const int* dims, const float* fill, const uint8_t* kee
                           double penal, double rmin, double move, double tol, int maxIter, int64_t coarsestMaxDof, float* density, double* history) {
     try {
         return orc::optimizeTopology(dims, fill, keep, bc, f, nu, volFrac, penal, rmin, move, tol, maxIter, coarsestMaxDof, density, history);
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
+// the break sequence: returns the step count (or -1); per step scalars[3 s ..] = lambda, max
+// displacement, voxels; counts[2 s ..] = cracked, detached; the cracked and detached voxels of all
+// steps concatenated (capacity given); vm[s * nNodes ..] the step's nodal von Mises at lambda when not null
+int orc_break_test(const int* dims, const float* density, double nu, const uint8_t* bc, const double* f, double E, double h, double strength,
+                   int principal, int maxSteps, double tol, int maxIter, int64_t coarsestMaxDof, double* scalars, int* counts, int* cracked,
+                   int crackedCap, int* detached, int detachedCap, float* vm, double* peak, int* removed, int* reason) {
+    try {
+        std::vector<orc::BreakStepOut> steps;
+        *reason = orc::breakTest(dims, density, nu, bc, f, E, h, strength, principal != 0, maxSteps, tol, maxIter, coarsestMaxDof, steps, *peak,
+                                 *removed);
+        int nc = 0, nd = 0;
+        const size_t nNodes = size_t(dims[0] + 1) * (dims[1] + 1) * (dims[2] + 1);
+        for (size_t s = 0; s < steps.size(); s++) {
+            const auto& b = steps[s];
+            scalars[3 * s] = b.lambda;
+            scalars[3 * s + 1] = b.maxDisp;
+            scalars[3 * s + 2] = b.voxels;
+            counts[2 * s] = int(b.cracked.size());
+            counts[2 * s + 1] = int(b.detached.size());
+            for (int e : b.cracked)
+                if (nc < crackedCap) cracked[nc++] = e;
+            for (int e : b.detached)
+                if (nd < detachedCap) detached[nd++] = e;
+            if (vm) std::memcpy(vm + s * nNodes, b.vm.data(), nNodes * sizeof(float));
+        }
+        return int(steps.size());
     } catch (const std::exception& e) {
         g_err = e.what();
         return -1;

@@ -509,3 +509,67 @@ def test_optimize_topology_bitwise_vs_reference(cpu, W):
     print(h, flush=True)
     assert np.array_equal(d, dr) and np.array_equal(h, hr)
     assert h[-1, 2] < 0.45 and h[-1, 1] < h[1, 1] * 5
+
+
+def _break_case(W, dims=(20, 6, 6), seed=3):
+    nx, ny, nz = dims
+    bc = W.clamp_x0(dims)
+    NX, NY, NZ = nx + 1, ny + 1, nz + 1
+    f = np.zeros((NZ, NY, NX, 3))
+    f[:, :, nx, 2] = -2000.0 / (NY * NZ)  # 2 kN down over the free end
+    # stiffness varies per voxel, so no two corner stresses tie
+    density = np.random.default_rng(seed).uniform(0.6, 1.0, nx * ny * nz).astype(np.float32)
+    return dims, density, bc, f.reshape(-1)
+
+
+def test_break_first_step_restated(cpu, W):
+    # one step of breakTest (static_study.cpp:227-266) recomputed from the oracle's own pieces:
+    # solve, corner stresses, per-voxel worst corner, the 0.95 threshold and the cap, the prune
+    dims, density, bc, f = _break_case(W)
+    E, h, strength = 200e9, 1e-3, 150e6
+    for principal in (False, True):
+        r = cpu.break_test(dims, density, 0.3, bc, f, E, h, strength, principal=principal, max_steps=1, tol=1e-10)
+        assert r["removed"] == 0 and len(r["steps"]) == 1
+        st = r["steps"][0]
+        m = cpu.Model(dims, density, 0.3, bc, 1100)
+        u = m.solve(f, tol=1e-10, max_iter=400)["u"] * (1 / (E * h))
+        vm, p1, _ = m.stresses(u, E, h)
+        nodal = p1 if principal else vm
+        nx, ny, nz = dims
+        k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+        n0 = (i + (nx + 1) * (j + (ny + 1) * k)).reshape(-1)
+        offs = [0, 1, 1 + (nx + 1), nx + 1]
+        offs = offs + [o + (nx + 1) * (ny + 1) for o in offs]
+        measure = np.maximum(0, np.max(np.stack([nodal[n0 + o] for o in offs]), axis=0))
+        smax = float(measure.max())
+        assert st["lambda"] == strength / smax
+        cap = max(1, int(np.floor(0.02 * measure.size ** (2 / 3) + 0.5)))
+        cand = np.flatnonzero(measure >= 0.95 * smax)
+        order = cand[np.argsort(-measure[cand], kind="stable")]
+        got = np.asarray(st["cracked"])
+        # the worst voxels in non-increasing order of their measure; equal measures (neighbours share
+        # their worst corner) come in std::sort's order, which the standard leaves unspecified
+        assert len(got) == min(cap, cand.size) >= 1
+        assert np.all(np.diff(measure[got]) <= 0)
+        assert np.array_equal(np.sort(measure[got]), np.sort(measure[order[:len(got)]]))
+        if cand.size <= cap or measure[order[cap - 1]] > measure[order[cap]]:
+            assert set(got.tolist()) == set(order[:len(got)].tolist())
+        want = got.tolist()
+        assert st["max_disp"] == np.sqrt(np.max((u.reshape(-1, 3) ** 2).sum(axis=1))) * st["lambda"]
+        nxt = density.copy()
+        nxt[want] = 0
+        pruned, _ = cpu.prune_floating(dims, nxt, W.held_nodes(bc))
+        assert list(st["detached"]) == [int(e) for e in np.flatnonzero((nxt > 0) & ~(pruned > 0))]
+
+
+def test_break_sequence_invariants(cpu, W):
+    dims, density, bc, f = _break_case(W, (16, 4, 4), seed=5)
+    r = cpu.break_test(dims, density, 0.3, bc, f, 200e9, 1e-3, 150e6, max_steps=40, tol=1e-8)
+    steps = r["steps"]
+    print(r["reason"], len(steps), [len(s["cracked"]) for s in steps], flush=True)
+    # every step removes its cracked and detached voxels from the next model
+    for a, b in zip(steps, steps[1:]):
+        assert b["voxels"] == a["voxels"] - len(a["cracked"]) - len(a["detached"])
+    assert r["peak"] == max(s["lambda"] for s in steps)
+    # a clamped cantilever ends separated from its load within the step budget
+    assert r["reason"] == "separated"
