@@ -203,6 +203,65 @@ int psx_element_energies(psx_model m, const double* u, double* out);
  * 65-79) over the voxels with fill > 0 in voxel order: x and out hold one value per such voxel. */
 int psx_topology_filter(const int dims[3], const float* fill, double rmin, int transpose, const double* x, double* out);
 
+/* ---- break test (breakTest, static_study.cpp:197-279) ----
+ * Repeats: build and solve (warm-started from the last step's solution), nodal stresses, the failure
+ * measure of every voxel (its worst corner stress, von Mises or first principal), the load factor
+ * lambda = strength / largest measure, the crack (the worst voxels, at most round(0.02 n^(2/3)) of
+ * the n voxels, each within 95 % of the largest) and the voxels the crack detaches from the held
+ * nodes. Solves, stresses, measures, the largest displacement and the prune (a breadth-first search
+ * over face neighbours from the held voxels) run on the device; the hierarchy is built on the host. */
+enum psx_break_reason {
+    PSX_BREAK_STEP_LIMIT = 0,     /* "step limit" */
+    PSX_BREAK_SEPARATED = 1,      /* "separated" */
+    PSX_BREAK_UNSUPPORTED = 2,    /* "unsupported load" */
+    PSX_BREAK_NOT_CONVERGED = 3,  /* "solver did not converge" */
+    PSX_BREAK_NO_STRESS = 4,      /* "no stress" */
+    PSX_BREAK_STOPPED = 5         /* on_step returned nonzero (no reference counterpart) */
+};
+/* One step (BreakStep, static_study.hpp:52-60); the arrays are valid during the on_step call. */
+typedef struct {
+    int step;
+    double lambda;              /* strength / the step's largest measure */
+    double max_disp;            /* largest nodal displacement magnitude at lambda */
+    int voxels;                 /* solid voxels of the step's model */
+    const int32_t* cracked;     /* voxels cracked, worst first */
+    int n_cracked;
+    const int32_t* detached;    /* voxels cut off by the crack, increasing */
+    int n_detached;
+    const float* u;             /* 3 * nodes physical displacements at lambda (fields = 1, else NULL) */
+    const float* node_vm;       /* nodes, von Mises at lambda (fields = 1, else NULL) */
+    const uint8_t* active_node; /* nodes (fields = 1, else NULL) */
+} psx_break_step;
+typedef int (*psx_break_fn)(const psx_break_step* step, void* user); /* nonzero stops */
+typedef struct {
+    double strength;            /* failure stress, > 0 */
+    int principal;              /* 1: first principal stress, 0: von Mises */
+    int max_steps;
+    double E, h;                /* Young's modulus and voxel size (u_physical = u / (E h)) */
+    double tol;                 /* relative residual of the solves (0 = 1e-6, static_study.cpp:111) */
+    int max_iter;               /* 0 = 3000 (the GPU engine's, static_study.cpp:125) */
+    int precision;              /* enum psx_precision */
+    int tie_order;              /* 0: equal measures in the reference's std::sort order (the measures of
+                                   all voxels are sorted on the host, as the reference does); 1: equal
+                                   measures by increasing voxel index (only the voxels within 95 % of
+                                   the largest leave the device) */
+    int fields;                 /* 1: fill psx_break_step's u, node_vm and active_node */
+    psx_break_fn on_step;
+    void* user;
+} psx_break_opts;
+typedef struct {
+    int reason;                 /* enum psx_break_reason */
+    int steps;
+    int removed;                /* voxels the first prune removed */
+    double peak;                /* largest lambda */
+} psx_break_result;
+/* part: dims, density, nu, bc and coarsest_max_dof as for psx_create (diag_add must be NULL); f the
+ * 3 * nodes loads. */
+int psx_break_test(const psx_model_desc* part, const double* f, const psx_break_opts* opts, psx_break_result* out);
+/* pruneFloating (voxel_fea.cpp:1086-1119) on the device: d_density, d_out nx*ny*nz floats in HBM,
+ * d_held one byte per node; removed receives the count of solid voxels set to 0. */
+int psx_prune_floating_device(int nx, int ny, int nz, const float* d_density, const uint8_t* d_held, float* d_out, int* removed);
+
 /* Support reactions: sum over held active DOFs of (K u)_raw - f (VoxelFEA::reactions,
  * voxel_fea.cpp:929-940). f may be NULL. */
 int psx_reactions(psx_model m, const double* u, const double* f, double out[3]);

@@ -46,6 +46,28 @@ class TopologyOpts(C.Structure):
                 ("on_iter", TOPO_ITER), ("user", C.c_void_p), ("solve_max_iter", C.c_int)]
 
 
+class BreakStep(C.Structure):
+    _fields_ = [("step", C.c_int), ("lambda_", C.c_double), ("max_disp", C.c_double), ("voxels", C.c_int),
+                ("cracked", C.POINTER(C.c_int32)), ("n_cracked", C.c_int), ("detached", C.POINTER(C.c_int32)), ("n_detached", C.c_int),
+                ("u", C.POINTER(C.c_float)), ("node_vm", C.POINTER(C.c_float)), ("active_node", C.POINTER(C.c_uint8))]
+
+
+BREAK_FN = C.CFUNCTYPE(C.c_int, C.POINTER(BreakStep), C.c_void_p)
+
+
+class BreakOpts(C.Structure):
+    _fields_ = [("strength", C.c_double), ("principal", C.c_int), ("max_steps", C.c_int), ("E", C.c_double), ("h", C.c_double),
+                ("tol", C.c_double), ("max_iter", C.c_int), ("precision", C.c_int), ("tie_order", C.c_int), ("fields", C.c_int),
+                ("on_step", BREAK_FN), ("user", C.c_void_p)]
+
+
+class BreakResult(C.Structure):
+    _fields_ = [("reason", C.c_int), ("steps", C.c_int), ("removed", C.c_int), ("peak", C.c_double)]
+
+
+BREAK_REASONS = ("step limit", "separated", "unsupported load", "solver did not converge", "no stress", "stopped")
+
+
 class ServerOpts(C.Structure):
     _fields_ = [("max_inflight", C.c_int), ("hbm_budget", C.c_int64)]
 
@@ -112,6 +134,8 @@ SIGNATURES = {
     "psx_optimize_topology": (_i, [_vp, _vp, _vp, _vp, _vp, _d, C.POINTER(TopologyOpts), _vp, _vp, C.POINTER(_i), C.POINTER(_d)]),
     "psx_element_energies": (_i, [_vp, _vp, _vp]),
     "psx_topology_filter": (_i, [_vp, _vp, _d, _i, _vp, _vp]),
+    "psx_break_test": (_i, [C.POINTER(ModelDesc), _vp, C.POINTER(BreakOpts), C.POINTER(BreakResult)]),
+    "psx_prune_floating_device": (_i, [_i, _i, _i, _vp, _vp, _vp, C.POINTER(_i)]),
     "psx_hex_element": (_i, [_d, _vp, _vp]),
     "psx_hierarchy_create": (_i, [C.POINTER(ModelDesc), C.POINTER(_vp)]),
     "psx_hierarchy_destroy": (None, [_vp]),
@@ -256,6 +280,42 @@ def topology_filter(dims, fill, rmin, x, transpose=False):
     out = np.zeros_like(x)
     _check(lib().psx_topology_filter(_p(d), _p(fill), float(rmin), int(bool(transpose)), _p(x), _p(out)))
     return out
+
+
+def break_test(dims, density, nu, bc, f, E, h, strength, principal=False, max_steps=5, tol=1e-6, max_iter=3000, precision="mixed",
+               tie_order=0, fields=False, coarsest_max_dof=1100, on_step=None):
+    """The break test (breakTest) with the solves, stresses, measures and prunes on the device: dict(reason,
+    peak, removed, steps), each step dict(lambda, max_disp, voxels, cracked, detached[, u, vm, active]).
+    on_step(step_dict) returning True stops the sequence."""
+    dims = tuple(int(d) for d in dims)
+    density = np.ascontiguousarray(density, np.float32).reshape(-1)
+    bc = np.ascontiguousarray(bc, np.uint8).reshape(-1)
+    f = np.ascontiguousarray(f, np.float64).reshape(-1)
+    nN = (dims[0] + 1) * (dims[1] + 1) * (dims[2] + 1)
+    if density.size != dims[0] * dims[1] * dims[2] or bc.size != 3 * nN or f.size != 3 * nN:
+        raise ValueError("density needs one value per voxel, bc and f one per DOF")
+    desc = ModelDesc(nx=dims[0], ny=dims[1], nz=dims[2], density=density.ctypes.data, nu=float(nu), bc=bc.ctypes.data, diag_add=None,
+                     coarsest_max_dof=int(coarsest_max_dof), max_levels=0)
+    steps = []
+
+    def collect(ptr, _user):
+        b = ptr.contents
+        st = {"step": b.step, "lambda": b.lambda_, "max_disp": b.max_disp, "voxels": b.voxels,
+              "cracked": np.ctypeslib.as_array(b.cracked, (b.n_cracked,)).copy() if b.n_cracked else np.zeros(0, np.int32),
+              "detached": np.ctypeslib.as_array(b.detached, (b.n_detached,)).copy() if b.n_detached else np.zeros(0, np.int32)}
+        if fields:
+            st["u"] = np.ctypeslib.as_array(b.u, (3 * nN,)).copy()
+            st["vm"] = np.ctypeslib.as_array(b.node_vm, (nN,)).copy()
+            st["active"] = np.ctypeslib.as_array(b.active_node, (nN,)).copy()
+        steps.append(st)
+        return 1 if (on_step is not None and on_step(st)) else 0
+
+    cb = BREAK_FN(collect)
+    o = BreakOpts(strength=float(strength), principal=int(bool(principal)), max_steps=int(max_steps), E=float(E), h=float(h), tol=float(tol),
+                  max_iter=int(max_iter), precision=PREC[precision], tie_order=int(tie_order), fields=int(bool(fields)), on_step=cb, user=None)
+    res = BreakResult()
+    _check(lib().psx_break_test(C.byref(desc), _p(f), C.byref(o), C.byref(res)))
+    return {"reason": BREAK_REASONS[res.reason], "peak": res.peak, "removed": res.removed, "steps": steps}
 
 
 def hex_element(nu):
@@ -590,3 +650,31 @@ class Server:
 
     def __del__(self):
         self.close()
+
+
+def prune_floating_device(dims, density, held):
+    """pruneFloating on the device (psx_prune_floating_device) from host arrays: (density, removed)."""
+    nx, ny, nz = (int(d) for d in dims)
+    density = np.ascontiguousarray(density, np.float32).reshape(-1)
+    held = np.ascontiguousarray(held, np.uint8).reshape(-1)
+    if density.size != nx * ny * nz or held.size != (nx + 1) * (ny + 1) * (nz + 1):
+        raise ValueError("density needs one value per voxel and held one per node")
+    hip = _hiprt()
+    bufs = [C.c_void_p() for _ in range(3)]
+    sizes = (4 * density.size, held.size, 4 * density.size)
+    out = np.zeros_like(density)
+    try:
+        for b, n in zip(bufs, sizes):
+            if hip.hipMalloc(C.byref(b), max(8, n)) != 0:
+                raise PsxError(ERR_DEVICE, "hipMalloc failed")
+        if hip.hipMemcpy(bufs[0], C.c_void_p(density.ctypes.data), sizes[0], 1) or hip.hipMemcpy(bufs[1], C.c_void_p(held.ctypes.data), sizes[1], 1):
+            raise PsxError(ERR_DEVICE, "hipMemcpy H2D failed")
+        removed = C.c_int()
+        _check(lib().psx_prune_floating_device(nx, ny, nz, bufs[0], bufs[1], bufs[2], C.byref(removed)))
+        if hip.hipMemcpy(C.c_void_p(out.ctypes.data), bufs[2], sizes[2], 2) != 0:
+            raise PsxError(ERR_DEVICE, "hipMemcpy D2H failed")
+    finally:
+        for b in bufs:
+            if b.value:
+                hip.hipFree(b)
+    return out, removed.value

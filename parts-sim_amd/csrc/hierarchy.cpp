@@ -4,6 +4,7 @@
 #include "psx_core.hpp"
 
 #include <algorithm>
+#include <numeric>
 #include <atomic>
 #include <cmath>
 #include <cstdlib>
@@ -771,6 +772,18 @@ double tridiagonalMax(const std::vector<double>& a, const std::vector<double>& b
 // voxels not face-connected to a held node, removed (density 0): the input rule applied before
 // every break-step re-solve (voxel_fea.cpp:1086-1119). Seeds are the solid voxels with a held corner;
 // the fill walks the six face neighbours. Returns the number of voxels removed.
+std::vector<int32_t> crackSelect(const float* measure, size_t n, double smax, size_t cap) {
+    std::vector<int> order(n);
+    std::iota(order.begin(), order.end(), 0);
+    std::sort(order.begin(), order.end(), [&](int a, int b) { return measure[a] > measure[b]; });
+    std::vector<int32_t> out;
+    for (int q : order) {
+        if (out.size() >= cap || measure[q] < 0.95 * smax) break;
+        out.push_back(q);
+    }
+    return out;
+}
+
 int pruneFloating(int nx, int ny, int nz, const float* density, const uint8_t* heldNode, float* out) {
     const i64 NX = nx + 1, NY = ny + 1, nE = i64(nx) * ny * nz;
     std::vector<uint8_t> reached(nE, 0);

@@ -415,6 +415,16 @@ public:
         launchFromF32(z32, dz, n, &st->rmax, stream, ni);
     }
 
+    // the corner stress matrices scaled by E / h, uploaded into stressS
+    void stressMatrices(double E, double h) {
+        if (!(E > 0) || !(h > 0)) throw std::invalid_argument("E and h must be positive");
+        std::vector<double> S(8 * 144);
+        for (int a = 0; a < 8; a++)
+            for (int q = 0; q < 144; q++) S[size_t(a) * 144 + size_t(q)] = H.el.corner[a][q] * E / h;
+        HIPCHK(hipMemcpyAsync(stressS, S.data(), S.size() * 8, hipMemcpyHostToDevice, stream));
+        sync();
+    }
+
     // one V-cycle for each of k vectors (instance-major) in one set of launches; each result equals
     // the vector's lone precondition32
     void preconditionMulti(int k, const double* dr, double* dz) {

@@ -94,6 +94,11 @@ std::vector<double> lanczosStart(const Level& L);
 // largest eigenvalue of a symmetric tridiagonal (Sturm-count bisection; voxel_fea.cpp:506-530)
 double tridiagonalMax(const std::vector<double>& a, const std::vector<double>& b);
 
+// The crack selection of one break step (static_study.cpp:245-258): the element indices by decreasing
+// measure as std::sort orders all n of them (equal measures in that sort's order), the first cap of
+// those whose measure is at least 0.95 smax.
+std::vector<int32_t> crackSelect(const float* measure, size_t n, double smax, size_t cap);
+
 // solid voxels not face-connected to a held node set to 0 (voxel_fea.cpp:1086-1119); heldNode per
 // node; returns the count removed
 int pruneFloating(int nx, int ny, int nz, const float* density, const uint8_t* heldNode, float* out);

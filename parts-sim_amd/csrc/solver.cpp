@@ -297,22 +297,11 @@ int psx_reactions(psx_model m, const double* u, const double* f, double out[3]) 
     });
 }
 
-namespace {
-// the corner stress matrices scaled by E / h, uploaded into the model's buffer
-void stressMatrices(psx::Model& M, double E, double h) {
-    if (!(E > 0) || !(h > 0)) throw std::invalid_argument("E and h must be positive");
-    std::vector<double> S(8 * 144);
-    for (int a = 0; a < 8; a++)
-        for (int q = 0; q < 144; q++) S[a * 144 + q] = M.H.el.corner[a][q] * E / h;
-    HIPCHK(hipMemcpyAsync(M.stressS, S.data(), S.size() * 8, hipMemcpyHostToDevice, M.stream));
-}
-}  // namespace
-
 int psx_stresses_device(psx_model m, const double* d_u, double E, double h, int principals, float* d_vm, float* d_p1, float* d_p3) {
     return guarded([&] {
         if (!m || !d_u || !d_vm || (principals && (!d_p1 || !d_p3))) throw std::invalid_argument("bad arguments");
         psx::Model& M = *m->m;
-        stressMatrices(M, E, h);
+        M.stressMatrices(E, h);
         psx::launchStresses(M.lv[0].dl, M.stressS, d_u, principals, d_vm, d_p1, d_p3, M.stream);
         HIPCHK(hipGetLastError());
         M.sync();
@@ -330,7 +319,7 @@ int psx_stresses(psx_model m, const double* u, double E, double h, int principal
         double* du = M.lv[0].b64;
         float* out = reinterpret_cast<float*>(M.lv[0].a64);
         HIPCHK(hipMemcpyAsync(du, u, size_t(D.nDof) * 8, hipMemcpyHostToDevice, M.stream));
-        stressMatrices(M, E, h);
+        M.stressMatrices(E, h);
         psx::launchStresses(D, M.stressS, du, principals, out, out + nn, out + 2 * nn, M.stream);
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(vm, out, size_t(nn) * 4, hipMemcpyDeviceToHost, M.stream));
@@ -353,7 +342,7 @@ int psx_stress_tensors(psx_model m, const double* u, double E, double h, float* 
         // contiguous, so the tensor goes through the CG vector p (nDof doubles = 6 nNodes floats)
         float* dout = reinterpret_cast<float*>(M.p);
         HIPCHK(hipMemcpyAsync(du, u, size_t(D.nDof) * 8, hipMemcpyHostToDevice, M.stream));
-        stressMatrices(M, E, h);
+        M.stressMatrices(E, h);
         psx::launchStressTensors(D, M.stressS, du, dout, M.stream);
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(out, dout, size_t(D.nNodes) * 6 * 4, hipMemcpyDeviceToHost, M.stream));
