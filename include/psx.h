@@ -57,6 +57,11 @@ typedef struct {
     const double* diag_add;     /* optional [3*nodes] shift added to K's diagonal, or NULL */
     int64_t coarsest_max_dof;   /* coarsen until free DOFs <= this (default 1100, gpu_fea.hpp:15) */
     int max_levels;             /* 0 = the reference's cap of 12 (voxel_fea.cpp:176) */
+    int build_on_device;        /* 1: build the hierarchy (levels, Galerkin coarse elements, coarsest
+                                   inverse) with device kernels instead of on the host CPU; the level
+                                   data are the host build's bit for bit, the coarsest inverse agrees
+                                   to rounding. No counterpart in the reference (its VoxelFEA builds on
+                                   the CPU, voxel_fea.cpp:138-183). */
 } psx_model_desc;
 
 /* Replaces SolveOptions (voxel_fea.hpp:112-117). */
@@ -107,6 +112,17 @@ void psx_destroy(psx_model m);
 
 int psx_num_levels(psx_model m);
 int psx_get_level_info(psx_model m, int level, psx_level_info* out);
+/* The level data a model's kernels read, downloaded from the device (any pointer may be NULL):
+ * held-or-inactive mask per DOF (inverse diagonal 0), inverse diagonal [n_dof], node scale and node
+ * info word [n_nodes], voxel words [nx*ny*nz], uniform element [576], own matrices [n_own*576];
+ * child-scale form (levels with it): child scales [n_own*8] and two-scale records [n_own*4]; the
+ * diagonal shift [n_dof] (levels that have one). The coarsest level's free-DOF map [m] and dense
+ * inverse [m*m]. No counterpart in the reference: they let a test compare a device-built hierarchy
+ * with the host build (VoxelFEA::Level, voxel_fea.hpp:37-78, is the reference's form of the data). */
+int psx_model_level(psx_model m, int level, uint8_t* fixed, double* inv_diag, float* node_scale, int32_t* words, double* kb, float* own,
+                    uint16_t* node_info);
+int psx_model_level_child(psx_model m, int level, float* child_s, int32_t* child_two, double* diag_add);
+int psx_model_coarse(psx_model m, int64_t* mdense, int32_t* free_idx, double* ainv);
 /* Smoother eigenvalue estimates per level (the reference stores them in Level::lmax,
  * gpu_fea.cpp:412-417); set_lmax overrides one (e.g. the CPU model's value). */
 int psx_estimate_smoothers(psx_model m);

@@ -25,7 +25,8 @@ class PsxError(RuntimeError):
 
 class ModelDesc(C.Structure):
     _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("density", C.c_void_p), ("nu", C.c_double),
-                ("bc", C.c_void_p), ("diag_add", C.c_void_p), ("coarsest_max_dof", C.c_int64), ("max_levels", C.c_int)]
+                ("bc", C.c_void_p), ("diag_add", C.c_void_p), ("coarsest_max_dof", C.c_int64), ("max_levels", C.c_int),
+                ("build_on_device", C.c_int)]
 
 
 PROGRESS = C.CFUNCTYPE(C.c_int, C.c_int, C.c_double, C.c_void_p)
@@ -142,6 +143,9 @@ SIGNATURES = {
     "psx_hierarchy_num_levels": (_i, [_vp]),
     "psx_hierarchy_level_info": (_i, [_vp, _i, C.POINTER(LevelInfo)]),
     "psx_hierarchy_level": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "psx_model_level": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "psx_model_level_child": (_i, [_vp, _i, _vp, _vp, _vp]),
+    "psx_model_coarse": (_i, [_vp, C.POINTER(C.c_int64), _vp, _vp]),
     "psx_hierarchy_coarse": (_i, [_vp, C.POINTER(_i64), _vp, _vp]),
 }
 
@@ -184,7 +188,7 @@ def device_name():
     return buf.value.decode()
 
 
-def _desc(dims, density, nu, bc, coarsest_max_dof, diag_add, max_levels, keep):
+def _desc(dims, density, nu, bc, coarsest_max_dof, diag_add, max_levels, keep, device_build=False):
     density = np.ascontiguousarray(density, np.float32).reshape(-1)
     bc = np.ascontiguousarray(bc, np.uint8).reshape(-1)
     nx, ny, nz = (int(v) for v in dims)
@@ -193,7 +197,7 @@ def _desc(dims, density, nu, bc, coarsest_max_dof, diag_add, max_levels, keep):
     da = None if diag_add is None else np.ascontiguousarray(diag_add, np.float64)
     keep.extend([density, bc, da])
     return ModelDesc(nx, ny, nz, density.ctypes.data, float(nu), bc.ctypes.data, None if da is None else da.ctypes.data,
-                     int(coarsest_max_dof), int(max_levels))
+                     int(coarsest_max_dof), int(max_levels), int(bool(device_build)))
 
 
 def _level_info(fn, h, l):
@@ -373,9 +377,11 @@ class Model:
     """One part on the GPU: its hierarchy in HBM and the solver around it
     (VoxelFEA + GpuFeaSolver of the reference)."""
 
-    def __init__(self, dims, density, nu, bc, coarsest_max_dof=1100, diag_add=None, max_levels=0):
+    def __init__(self, dims, density, nu, bc, coarsest_max_dof=1100, diag_add=None, max_levels=0, device_build=False):
+        """device_build: build the hierarchy with the device kernels (the host build's level data bit
+        for bit) instead of on the host CPU."""
         keep = []
-        d = _desc(dims, density, nu, bc, coarsest_max_dof, diag_add, max_levels, keep)
+        d = _desc(dims, density, nu, bc, coarsest_max_dof, diag_add, max_levels, keep, device_build)
         h = C.c_void_p()
         _check(lib().psx_create(C.byref(d), C.byref(h)))
         self.h = h
@@ -396,6 +402,39 @@ class Model:
 
     def level(self, l):
         return _level_info(lib().psx_get_level_info, self.h, l)
+
+    def level_arrays(self, l):
+        """The level data the kernels read, downloaded from the device (the host Hierarchy's fields,
+        plus the node info words, the child-scale form and the diagonal shift where the level has them)."""
+        info = self.level(l)
+        nvox = int(np.prod(info["dims"]))
+        fixed = np.zeros(info["nDof"], np.uint8)
+        inv = np.zeros(info["nDof"])
+        ns = np.zeros(info["nNodes"], np.float32)
+        words = np.zeros(nvox, np.int32)
+        kb = np.zeros(576)
+        own = np.zeros(max(1, info["nOwn"]) * 576, np.float32)
+        ninfo = np.zeros(info["nNodes"], np.uint16)
+        _check(lib().psx_model_level(self.h, l, _p(fixed), _p(inv), _p(ns), _p(words), _p(kb), _p(own), _p(ninfo)))
+        out = {"fixed": fixed, "invDiag": inv, "nodeScale": ns, "words": words, "Kb": kb.reshape(24, 24),
+               "own": own[:info["nOwn"] * 576].reshape(-1, 24, 24), "nodeInfo": ninfo}
+        cs = np.zeros(max(1, info["nOwn"]) * 8, np.float32)
+        two = np.zeros(max(1, info["nOwn"]) * 4, np.int32)
+        if lib().psx_model_level_child(self.h, l, _p(cs), _p(two), None) == 0:
+            out["childS"], out["childTwo"] = cs[:info["nOwn"] * 8], two[:info["nOwn"] * 4]
+        da = np.zeros(info["nDof"])
+        if lib().psx_model_level_child(self.h, l, None, None, _p(da)) == 0:
+            out["diagAdd"] = da
+        return out
+
+    def coarse(self):
+        """The coarsest level's free-DOF map and dense inverse as the device holds them."""
+        m = C.c_int64()
+        _check(lib().psx_model_coarse(self.h, C.byref(m), None, None))
+        idx = np.zeros(max(1, m.value), np.int32)
+        ainv = np.zeros(max(1, m.value * m.value))
+        _check(lib().psx_model_coarse(self.h, C.byref(m), _p(idx), _p(ainv)))
+        return idx[:m.value], ainv[:m.value * m.value].reshape(m.value, m.value)
 
     def estimate_smoothers(self):
         _check(lib().psx_estimate_smoothers(self.h))
@@ -609,10 +648,11 @@ class Server:
         self._keep = {}
 
     def submit(self, dims, density, nu, bc, f, x0=None, part_id=-1, coarsest_max_dof=1100, tol=1e-6, max_iter=500, precision="fp64",
-               cheb_degree=0, diag_add=None):
-        """Queues one part with its load cases f (nrhs x nDof or nDof); returns a ticket."""
+               cheb_degree=0, diag_add=None, device_build=False):
+        """Queues one part with its load cases f (nrhs x nDof or nDof); returns a ticket.
+        device_build: the request's hierarchy is built by the device kernels."""
         keep = []
-        d = _desc(dims, density, nu, bc, coarsest_max_dof, diag_add, 0, keep)
+        d = _desc(dims, density, nu, bc, coarsest_max_dof, diag_add, 0, keep, device_build)
         f = np.ascontiguousarray(f, np.float64)
         n = 3 * int(np.prod([int(v) + 1 for v in dims]))
         nrhs = f.size // n

@@ -291,6 +291,7 @@ static Level emptyLevel(int nx, int ny, int nz) {
 // (voxel_fea.cpp:208-266)
 static void finishLevel(Level& L) {
     const i64 nE = i64(L.elemVox.size());
+    L.nElem = nE;
     L.active.assign(L.nNodes, 0);
     L.fixed.assign(L.nDof, 0);
     L.invDiag.assign(L.nDof, 0.0);
@@ -394,7 +395,7 @@ static void finishLevel(Level& L) {
 }
 
 // interpolation weight of coarse corner b at fine corner a of child c (trilinear)
-static double childWeight(int c, int a, int b) {
+double childWeight(int c, int a, int b) {
     const int cb[3] = {c & 1, (c >> 1) & 1, c >> 2};
     const int ab[3] = {CX[a], CY[a], CZ[a]}, bb[3] = {CX[b], CY[b], CZ[b]};
     double w = 1;
@@ -429,6 +430,28 @@ static void galerkinChild(double* out, const double* K, int c) {
             }
 }
 
+void childProducts(const double* Kb, double* M, double* KbCoarse) {
+    std::fill(M, M + 8 * 576, 0.0);
+    for (int c = 0; c < 8; c++) galerkinChild(M + c * 576, Kb, c);
+    for (int u = 0; u < 576; u++) {
+        double s = 0;
+        for (int c = 0; c < 8; c++) s += M[c * 576 + u];
+        KbCoarse[u] = s;
+    }
+}
+
+void childTable(const double* M, double* tab) {
+    // corner-0 rows of M_B for every subset B, columns of corners 1..7 (the rows sum to zero over
+    // the corners, so corner 0's columns are not needed): [256][3][21], padded to 64
+    std::fill(tab, tab + 256 * 64, 0.0);
+    for (int B = 1; B < 256; B++)
+        for (int c = 0; c < 8; c++) {
+            if (!((B >> c) & 1)) continue;
+            for (int d = 0; d < 3; d++)
+                for (int u = 0; u < 21; u++) tab[B * 64 + d * 21 + u] += M[size_t(c) * 576 + size_t(d) * 24 + 3 + u];
+        }
+}
+
 // fine level F -> coarse level (voxel_fea.cpp:268-386): children of each coarse voxel, exact
 // Galerkin element matrices, held coarse DOFs and the lumped diagonal shift
 static Level coarsen(const Level& F) {
@@ -459,13 +482,8 @@ static Level coarsen(const Level& F) {
         for (i64 q = 0; q < nF; q++) kids[fill[C.emap[parent[q]]]++] = int32_t(q);
     }
     // the fine uniform element's Galerkin product per child position; their sum is the coarse Kb
-    std::vector<double> M(8 * 576, 0.0);
-    for (int c = 0; c < 8; c++) galerkinChild(M.data() + c * 576, F.Kb, c);
-    for (int u = 0; u < 576; u++) {
-        double s = 0;
-        for (int c = 0; c < 8; c++) s += M[c * 576 + u];
-        C.Kb[u] = s;
-    }
+    std::vector<double> M(8 * 576);
+    childProducts(F.Kb, M.data(), C.Kb);
     C.rho.assign(nE, 0.0);
     C.own.assign(nE, -1);
     int nOwn = 0;
@@ -533,15 +551,8 @@ static Level coarsen(const Level& F) {
             std::memcpy(&t[1], &b, 4);
             t[2] = two && mask ? mask : -1;
         }
-        // corner-0 rows of M_B for every subset B, columns of corners 1..7 (the rows sum to zero
-        // over the corners, so corner 0's columns are not needed): [256][3][21], padded to 64
         C.childTab.assign(256 * 64, 0.0);
-        for (int B = 1; B < 256; B++)
-            for (int c = 0; c < 8; c++) {
-                if (!((B >> c) & 1)) continue;
-                for (int d = 0; d < 3; d++)
-                    for (int u = 0; u < 21; u++) C.childTab[B * 64 + d * 21 + u] += M[size_t(c) * 576 + size_t(d) * 24 + 3 + u];
-            }
+        childTable(M.data(), C.childTab.data());
     }
     // a coarse DOF is held when it interpolates onto a held fine DOF
     C.bc.assign(C.nDof, 0);

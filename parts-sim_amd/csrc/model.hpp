@@ -60,6 +60,14 @@ inline Cheb chebyshev(double lmax, int degree) {
     return c;
 }
 
+class Model;
+// builds the hierarchy of d with the device kernels of build.hip on m's stream (sets m.H's level
+// sizes, uniform elements and coarsest map, and every device buffer of m.lv)
+void buildOnDevice(Model& m, const psx_model_desc& d);
+// the Lanczos start vector of a level on the device: the LCG sequence of lanczosStart with held
+// DOFs 0, normalised (tmp: one device double)
+void launchLanczosStart(const double* invD, int64_t n, double* v, double* tmp, Reduce red, hipStream_t s);
+
 class Model {
 public:
     Hierarchy H;
@@ -107,17 +115,34 @@ public:
         graphKey = GraphKey{};
     }
 
+    // hierarchy built by the device kernels (build.hip): H holds the level sizes and uniform
+    // elements only; the level-0 held DOFs stay on the device (bc0) until a host-side caller needs
+    // them (hostLevel0)
+    bool deviceBuilt = false;
+    uint8_t* bc0 = nullptr;
+
     explicit Model(const psx_model_desc& d) {
         int n = 0;
         if (hipGetDeviceCount(&n) != hipSuccess || n == 0) throw NoDevice("no usable GPU was found");
         HIPCHK(hipGetDevice(&device));
+        if (d.build_on_device) {
+            HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+            try {
+                buildOnDevice(*this, d);
+            } catch (...) {
+                release();
+                throw;
+            }
+            return;
+        }
         buildHierarchy(H, d.nx, d.ny, d.nz, d.density, d.nu, d.bc, d.coarsest_max_dof > 0 ? d.coarsest_max_dof : 1100, d.diag_add, d.max_levels);
         HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         upload();
     }
     // Device buffers come from the stream-ordered allocator: plain hipFree waits for the whole
     // device, which would stall every other model's stream (concurrent serving requests).
-    ~Model() {
+    ~Model() { release(); }
+    void release() {
         dropGraph();
         if (stream) {
             for (void* ptr : allocs) (void)hipFreeAsync(ptr, stream);
@@ -129,6 +154,10 @@ public:
         }
         if (stHost) (void)hipHostFree(stHost);
         if (stream) (void)hipStreamDestroy(stream);
+        allocs.clear();
+        work.clear();
+        stHost = nullptr;
+        stream = nullptr;
     }
 
     template <class T> T* alloc(int64_t count, bool perRhs = false) {
@@ -158,16 +187,7 @@ public:
             D.nDof = L.nDof;
             D.words = upload(L.words.data(), L.nVox);
             D.Kown = upload(L.K.data(), int64_t(L.K.size()));
-            float kb32[576];
-            for (int u = 0; u < 576; u++) kb32[u] = float(L.Kb[u]);
-            D.Kb64 = upload(L.Kb, 576);
-            D.Kb32 = upload(kb32, 576);
-            double S[243];
-            float S32[243];
-            stencil(L.Kb, S);
-            for (int u = 0; u < 243; u++) S32[u] = float(S[u]);
-            D.St64 = upload(S, 243);
-            D.St32 = upload(S32, 243);
+            uploadConstants(l);
             D.nodeScale = upload(L.nodeScale.data(), L.nNodes);
             {
                 std::vector<uint16_t> info(L.nNodes);
@@ -193,27 +213,13 @@ public:
                 for (int64_t e = 0; e < L.nOwn; e++) two += L.childTwo[size_t(e) * 4 + 2] >= 0 ? 1 : 0;
                 lv[l].childCheap = 2 * two >= L.nOwn;
                 D.childS = upload(L.childS.data(), int64_t(L.childS.size()));
-                double rows[8 * 72];
-                float rows32[8 * 72];
-                for (int c = 0; c < 8; c++)
-                    for (int u = 0; u < 72; u++) rows[c * 72 + u] = L.childM[size_t(c) * 576 + u];  // rows 0..2 of M_c
-                for (int u = 0; u < 8 * 72; u++) rows32[u] = float(rows[u]);
-                D.crow64 = upload(rows, 8 * 72);
-                D.crow32 = upload(rows32, 8 * 72);
                 D.childTwo = upload(L.childTwo.data(), int64_t(L.childTwo.size()));
-                std::vector<float> tab32(L.childTab.begin(), L.childTab.end());
-                D.ctab64 = upload(L.childTab.data(), int64_t(L.childTab.size()));
-                D.ctab32 = upload(tab32.data(), int64_t(tab32.size()));
+                uploadChildConstants(l);
             }
             std::vector<float> inv32(L.invDiag.begin(), L.invDiag.end());
             D.invD64 = upload(L.invDiag.data(), L.nDof);
             D.invD32 = upload(inv32.data(), L.nDof);
             D.diagAdd = L.diagAdd.empty() ? nullptr : upload(L.diagAdd.data(), L.nDof);
-            {
-                const double dev = symmetryClasses(L.Kb, D.sym64);
-                for (int q = 0; q < 22; q++) D.sym32[q] = float(D.sym64[q]);
-                D.symOk = dev < 1e-12;
-            }
         }
         freeIdx = upload(H.freeIdx.data(), H.m);
         {
@@ -222,9 +228,70 @@ public:
             denseIdx = upload(dense.data(), int64_t(dense.size()));
         }
         Ainv = upload(H.Ainv.data(), H.m * H.m);
+        finishSetup();
+    }
+    void finishSetup() {
         stressS = alloc<double>(8 * 144);
         red.counter = alloc<unsigned>(1);
         reserve(1);
+    }
+    // the level's uniform element, its stencil and symmetry classes (from H.levels[l].Kb)
+    void uploadConstants(size_t l) {
+        const Level& L = H.levels[l];
+        DevLevel& D = lv[l].dl;
+        float kb32[576];
+        for (int u = 0; u < 576; u++) kb32[u] = float(L.Kb[u]);
+        D.Kb64 = upload(L.Kb, 576);
+        D.Kb32 = upload(kb32, 576);
+        double S[243];
+        float S32[243];
+        stencil(L.Kb, S);
+        for (int u = 0; u < 243; u++) S32[u] = float(S[u]);
+        D.St64 = upload(S, 243);
+        D.St32 = upload(S32, 243);
+        const double dev = symmetryClasses(L.Kb, D.sym64);
+        for (int q = 0; q < 22; q++) D.sym32[q] = float(D.sym64[q]);
+        D.symOk = dev < 1e-12;
+    }
+    // child-scale form: corner-0 rows of the 8 Galerkin products and the subset table (from
+    // H.levels[l].childM / childTab)
+    void uploadChildConstants(size_t l) {
+        const Level& L = H.levels[l];
+        DevLevel& D = lv[l].dl;
+        double rows[8 * 72];
+        float rows32[8 * 72];
+        for (int c = 0; c < 8; c++)
+            for (int u = 0; u < 72; u++) rows[c * 72 + u] = L.childM[size_t(c) * 576 + u];  // rows 0..2 of M_c
+        for (int u = 0; u < 8 * 72; u++) rows32[u] = float(rows[u]);
+        D.crow64 = upload(rows, 8 * 72);
+        D.crow32 = upload(rows32, 8 * 72);
+        std::vector<float> tab32(L.childTab.begin(), L.childTab.end());
+        D.ctab64 = upload(L.childTab.data(), int64_t(L.childTab.size()));
+        D.ctab32 = upload(tab32.data(), int64_t(tab32.size()));
+    }
+    // level 0's held DOFs and active nodes on the host (reactions); a device-built model downloads
+    // them on first use
+    const Level& hostLevel0() {
+        Level& L = H.levels[0];
+        if (!deviceBuilt || !L.active.empty()) return L;
+        std::vector<int32_t> words(size_t(L.nVox));
+        L.bc.resize(size_t(L.nDof));
+        HIPCHK(hipMemcpyAsync(words.data(), lv[0].dl.words, size_t(L.nVox) * 4, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipMemcpyAsync(L.bc.data(), bc0, size_t(L.nDof), hipMemcpyDeviceToHost, stream));
+        sync();
+        L.active.assign(size_t(L.nNodes), 0);
+        for (int k = 0; k < L.NZ; k++)
+            for (int j = 0; j < L.NY; j++)
+                for (int i = 0; i < L.NX; i++) {
+                    bool act = false;
+                    for (int v = 0; v < 8 && !act; v++) {
+                        const int vi = i - 1 + (v & 1), vj = j - 1 + ((v >> 1) & 1), vk = k - 1 + (v >> 2);
+                        if (vi < 0 || vj < 0 || vk < 0 || vi >= L.nx || vj >= L.ny || vk >= L.nz) continue;
+                        act = words[size_t(vi + int64_t(L.nx) * (vj + int64_t(L.ny) * vk))] != 0;
+                    }
+                    L.active[size_t(L.node(i, j, k))] = act ? 1 : 0;
+                }
+        return L;
     }
 
     // work vectors for k right-hand sides (grown, never shrunk; the contents do not survive)
@@ -325,9 +392,13 @@ public:
             if (L.lmax > 0) continue;
             const DevLevel& D = lv[l].dl;
             const int64_t n = L.nDof;
-            const auto v0 = lanczosStart(L);
             double *v = lv[l].a64, *vPrev = lv[l].d64, *w = lv[l].b64, *xx = lv[l].r64;
-            HIPCHK(hipMemcpyAsync(v, v0.data(), size_t(n) * 8, hipMemcpyHostToDevice, stream));
+            if (deviceBuilt) {
+                launchLanczosStart(D.invD64, n, v, &st->spare[0], red, stream);
+            } else {
+                const auto v0 = lanczosStart(L);
+                HIPCHK(hipMemcpyAsync(v, v0.data(), size_t(n) * 8, hipMemcpyHostToDevice, stream));
+            }
             HIPCHK(hipMemsetAsync(vPrev, 0, size_t(n) * 8, stream));
             for (int j = 0; j < 10; j++) {
                 launchLzX(v, D.invD64, xx, n, stream);

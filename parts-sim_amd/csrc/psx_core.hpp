@@ -55,6 +55,7 @@ struct Level {
     std::vector<int32_t> childTwo;
     std::vector<double> childTab;
     std::vector<int32_t> words;     // device voxel words
+    i64 nElem = 0;                  // elements (non-empty voxels)
     double lmax = 0;
     i64 node(int i, int j, int k) const { return i + i64(NX) * (j + i64(NY) * k); }
 };
@@ -74,6 +75,15 @@ struct Hierarchy {
 // solve (0 or more than 3000 free DOFs).
 void buildHierarchy(Hierarchy& H, int nx, int ny, int nz, const float* density, double nu, const uint8_t* bc, i64 coarsestMaxDof,
                     const double* diagAdd, int maxLevels);
+
+// Galerkin products of a level's uniform element Kb per child position c = cx + 2 cy + 4 cz
+// (M: 8 x 576, M_c = P_c^T Kb P_c) and their sum, the coarser level's uniform element
+void childProducts(const double* Kb, double* M, double* KbCoarse);
+// corner-0 rows of sum_{c in B} M_c for every subset B of the children, columns of corners 1..7:
+// [256][64]
+void childTable(const double* M, double* tab);
+// trilinear weight of coarse corner b at fine corner a of child position c
+double childWeight(int c, int a, int b);
 
 // The 27-point stencil of a uniform node for element matrix K: S[m][d][c] couples DOF d of the node
 // with DOF c of neighbour m = ox + 3 oy + 9 oz (offsets -1..1 shifted to 0..2).

@@ -29,7 +29,7 @@ double msBetween(Clock::time_point a, Clock::time_point b) { return std::chrono:
 struct Request {
     int64_t ticket = 0;
     // the part, copied
-    int nx = 0, ny = 0, nz = 0, maxLevels = 0;
+    int nx = 0, ny = 0, nz = 0, maxLevels = 0, onDevice = 0;
     double nu = 0.3;
     int64_t coarsest = 0;
     std::vector<float> density;
@@ -161,6 +161,7 @@ struct psx_server_s {
                 d.diag_add = r.diagAdd.empty() ? nullptr : r.diagAdd.data();
                 d.coarsest_max_dof = r.coarsest;
                 d.max_levels = r.maxLevels;
+                d.build_on_device = r.onDevice;
                 own = std::make_unique<psx::Model>(d);
                 own->estimateSmoothers();
                 own->sync();
@@ -264,6 +265,7 @@ int psx_server_submit(psx_server s, const psx_request* req, int64_t* ticket) {
         r->nu = d.nu;
         r->coarsest = d.coarsest_max_dof > 0 ? d.coarsest_max_dof : 1100;
         r->maxLevels = d.max_levels;
+        r->onDevice = d.build_on_device;
         const int64_t nVox = int64_t(d.nx) * d.ny * d.nz, nNodes = int64_t(d.nx + 1) * (d.ny + 1) * (d.nz + 1);
         r->nDof = 3 * nNodes;
         r->density.assign(d.density, d.density + nVox);

@@ -68,9 +68,64 @@ int psx_get_level_info(psx_model m, int level, psx_level_info* out) {
         out->n_nodes = L.nNodes;
         out->n_dof = L.nDof;
         out->free_dof = L.freeDof;
-        out->n_elems = int64_t(L.elemVox.size());
+        out->n_elems = L.nElem;
         out->n_own = L.nOwn;
         out->lmax = L.lmax;
+        return PSX_OK;
+    });
+}
+
+int psx_model_level(psx_model m, int level, uint8_t* fixed, double* inv_diag, float* node_scale, int32_t* words, double* kb, float* own,
+                    uint16_t* node_info) {
+    return guarded([&] {
+        if (!m || level < 0 || level >= int(m->m->H.levels.size())) throw std::invalid_argument("bad level");
+        psx::Model& M = *m->m;
+        const psx::Level& L = M.H.levels[size_t(level)];
+        const psx::DevLevel& D = M.lv[size_t(level)].dl;
+        auto get = [&](void* dst, const void* src, size_t bytes) {
+            if (dst && bytes) HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, M.stream));
+        };
+        std::vector<double> inv(fixed && !inv_diag ? size_t(L.nDof) : 0);
+        double* invOut = inv_diag ? inv_diag : (fixed ? inv.data() : nullptr);
+        get(invOut, D.invD64, size_t(L.nDof) * 8);
+        get(node_scale, D.nodeScale, size_t(L.nNodes) * 4);
+        get(words, D.words, size_t(L.nVox) * 4);
+        get(kb, D.Kb64, 576 * 8);
+        get(own, D.Kown, size_t(L.nOwn) * 576 * 4);
+        get(node_info, D.nodeInfo, size_t(L.nNodes) * 2);
+        M.sync();
+        if (fixed)
+            for (int64_t q = 0; q < L.nDof; q++) fixed[q] = invOut[q] == 0 ? 1 : 0;
+        return PSX_OK;
+    });
+}
+
+int psx_model_level_child(psx_model m, int level, float* child_s, int32_t* child_two, double* diag_add) {
+    return guarded([&] {
+        if (!m || level < 0 || level >= int(m->m->H.levels.size())) throw std::invalid_argument("bad level");
+        psx::Model& M = *m->m;
+        const psx::Level& L = M.H.levels[size_t(level)];
+        const psx::DevLevel& D = M.lv[size_t(level)].dl;
+        if ((child_s || child_two) && !D.childForm) throw std::invalid_argument("the level has no child-scale form");
+        if (diag_add && !D.diagAdd) throw std::invalid_argument("the level has no diagonal shift");
+        if (child_s) HIPCHK(hipMemcpyAsync(child_s, D.childS, size_t(L.nOwn) * 8 * 4, hipMemcpyDeviceToHost, M.stream));
+        if (child_two) HIPCHK(hipMemcpyAsync(child_two, D.childTwo, size_t(L.nOwn) * 4 * 4, hipMemcpyDeviceToHost, M.stream));
+        if (diag_add) HIPCHK(hipMemcpyAsync(diag_add, D.diagAdd, size_t(L.nDof) * 8, hipMemcpyDeviceToHost, M.stream));
+        M.sync();
+        return PSX_OK;
+    });
+}
+
+int psx_model_coarse(psx_model m, int64_t* mdense, int32_t* free_idx, double* ainv) {
+    return guarded([&] {
+        if (!m) throw std::invalid_argument("null model");
+        psx::Model& M = *m->m;
+        if (mdense) *mdense = M.H.m;
+        if (free_idx) std::copy(M.H.freeIdx.begin(), M.H.freeIdx.end(), free_idx);
+        if (ainv) {
+            HIPCHK(hipMemcpyAsync(ainv, M.Ainv, size_t(M.H.m * M.H.m) * 8, hipMemcpyDeviceToHost, M.stream));
+            M.sync();
+        }
         return PSX_OK;
     });
 }
@@ -279,7 +334,7 @@ int psx_reactions(psx_model m, const double* u, const double* f, double out[3]) 
     return guarded([&] {
         if (!m || !u || !out) throw std::invalid_argument("bad arguments");
         psx::Model& M = *m->m;
-        const psx::Level& L = M.H.levels[0];
+        const psx::Level& L = M.hostLevel0();
         const int64_t n = L.nDof;
         std::vector<double> y(n);
         HIPCHK(hipMemcpyAsync(M.x, u, size_t(n) * 8, hipMemcpyHostToDevice, M.stream));
@@ -385,7 +440,7 @@ int psx_profile(psx_model m, int reps, char* buf, size_t len) {
             }
             std::snprintf(line, sizeof line,
                           "L%zu nodes=%lld elems=%zu own matrices=%lld: matvec64 %.4f ms  matvec32 %.4f ms  matvec-bf16 %.4f ms  child-form64 %.4f ms  child-form32 %.4f ms\n",
-                          l, (long long)L.nNodes, L.elemVox.size(), (long long)L.nOwn, t64, t32, tmf, c64, c32);
+                          l, (long long)L.nNodes, size_t(L.nElem), (long long)L.nOwn, t64, t32, tmf, c64, c32);
             out += line;
         }
         const double tv64 = timeIt([&] { M.vcycle<double>(0, M.lv[0].r64, nullptr, psx::OF_AUTO); });
@@ -460,7 +515,7 @@ int psx_hierarchy_level_info(psx_hierarchy h, int level, psx_level_info* out) {
         out->n_nodes = L.nNodes;
         out->n_dof = L.nDof;
         out->free_dof = L.freeDof;
-        out->n_elems = int64_t(L.elemVox.size());
+        out->n_elems = L.nElem;
         out->n_own = L.nOwn;
         out->lmax = L.lmax;
         return PSX_OK;
