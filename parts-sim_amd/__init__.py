@@ -648,9 +648,9 @@ class Server:
         self._keep = {}
 
     def submit(self, dims, density, nu, bc, f, x0=None, part_id=-1, coarsest_max_dof=1100, tol=1e-6, max_iter=500, precision="fp64",
-               cheb_degree=0, diag_add=None, device_build=False):
+               cheb_degree=0, diag_add=None, device_build=True):
         """Queues one part with its load cases f (nrhs x nDof or nDof); returns a ticket.
-        device_build: the request's hierarchy is built by the device kernels."""
+        device_build: the request's hierarchy is built by the device kernels (False: on the host)."""
         keep = []
         d = _desc(dims, density, nu, bc, coarsest_max_dof, diag_add, 0, keep, device_build)
         f = np.ascontiguousarray(f, np.float64)

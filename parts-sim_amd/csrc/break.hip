@@ -344,9 +344,9 @@ int psx_break_test(const psx_model_desc* part, const double* f, const psx_break_
             }
             psx_model_desc desc = *part;
             desc.density = density.data();
-            desc.build_on_device = 0;  // the step reads the host level's element list and active nodes
+            desc.build_on_device = 1;
             Model M(desc);
-            const Level& L = M.H.levels[0];
+            const Level& L = M.hostLevel0();
             psx_solve_opts so{};
             so.tol = o.tol;
             so.max_iter = o.max_iter;

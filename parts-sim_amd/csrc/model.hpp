@@ -269,8 +269,9 @@ public:
         D.ctab64 = upload(L.childTab.data(), int64_t(L.childTab.size()));
         D.ctab32 = upload(tab32.data(), int64_t(tab32.size()));
     }
-    // level 0's held DOFs and active nodes on the host (reactions); a device-built model downloads
-    // them on first use
+    // level 0's held DOFs, active nodes and element list (element -> voxel, in voxel order) on the
+    // host (reactions, the break and topology loops); a device-built model downloads its voxel words
+    // and held DOFs on first use and derives them
     const Level& hostLevel0() {
         Level& L = H.levels[0];
         if (!deviceBuilt || !L.active.empty()) return L;
@@ -279,6 +280,10 @@ public:
         HIPCHK(hipMemcpyAsync(words.data(), lv[0].dl.words, size_t(L.nVox) * 4, hipMemcpyDeviceToHost, stream));
         HIPCHK(hipMemcpyAsync(L.bc.data(), bc0, size_t(L.nDof), hipMemcpyDeviceToHost, stream));
         sync();
+        L.elemVox.clear();
+        L.elemVox.reserve(size_t(L.nElem));
+        for (int64_t v = 0; v < L.nVox; v++)
+            if (words[size_t(v)] != 0) L.elemVox.push_back(int32_t(v));
         L.active.assign(size_t(L.nNodes), 0);
         for (int k = 0; k < L.NZ; k++)
             for (int j = 0; j < L.NY; j++)

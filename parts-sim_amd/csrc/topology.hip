@@ -216,7 +216,7 @@ int psx_element_energies(psx_model m, const double* u, double* out) {
     return abi::guarded([&] {
         if (!m || !u || !out) throw std::invalid_argument("bad arguments");
         Model& M = *m->m;
-        const Level& L = M.H.levels[0];
+        const Level& L = M.hostLevel0();
         const int64_t nE = int64_t(L.elemVox.size());
         Dev D;
         HIPCHK(hipStreamCreateWithFlags(&D.s, hipStreamNonBlocking));
@@ -349,8 +349,9 @@ int psx_optimize_topology(const int dims[3], const float* fill, const uint8_t* k
             desc.nu = nu;
             desc.bc = bc;
             desc.coarsest_max_dof = o.coarsest_max_dof;
+            desc.build_on_device = 1;
             Model M(desc);
-            const Level& L0 = M.H.levels[0];
+            const Level& L0 = M.hostLevel0();
             const int64_t nE = int64_t(L0.elemVox.size());
             std::fill(hElemOf.begin(), hElemOf.end(), -1);
             for (int64_t e = 0; e < nE; e++) hElemOf[size_t(L0.elemVox[size_t(e)])] = int32_t(e);

@@ -18,17 +18,18 @@ def _loads(w, nrhs, seed):
     return np.stack([f * (1 + k) + 0.01 * np.abs(f).max() * rng.standard_normal(f.size) * (f != 0) for k in range(nrhs)])
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_served_results_equal_lone_solves(P, W, precision):
+@pytest.mark.parametrize("precision,device_build", [("fp64", True), ("mixed", True), ("mixed", False)])
+def test_served_results_equal_lone_solves(P, W, precision, device_build):
     parts = _parts(W)
     alone = []
     for q, (w, nrhs) in enumerate(parts):
-        m = P.Model(w["dims"], w["density"], w["nu"], w["bc"], 300)
+        m = P.Model(w["dims"], w["density"], w["nu"], w["bc"], 300, device_build=device_build)
         F = _loads(w, nrhs, q)
         alone.append(np.stack([m.solve(F[k], tol=1e-9, precision=precision)["u"] for k in range(nrhs)]))
         m.close()
     srv = P.Server(max_inflight=3)
-    tickets = [srv.submit(w["dims"], w["density"], w["nu"], w["bc"], _loads(w, nrhs, q), coarsest_max_dof=300, tol=1e-9, precision=precision)
+    tickets = [srv.submit(w["dims"], w["density"], w["nu"], w["bc"], _loads(w, nrhs, q), coarsest_max_dof=300, tol=1e-9, precision=precision,
+                          device_build=device_build)
                for q, (w, nrhs) in enumerate(parts)]
     for q, t in enumerate(tickets):
         u, res = srv.wait(t)
@@ -42,7 +43,7 @@ def test_served_results_equal_lone_solves(P, W, precision):
 def test_resident_part_and_admission(P, W):
     w, _ = _parts(W)[0]
     f = _loads(w, 2, 5)
-    m = P.Model(w["dims"], w["density"], w["nu"], w["bc"], 300)
+    m = P.Model(w["dims"], w["density"], w["nu"], w["bc"], 300, device_build=True)
     lone = [m.solve(f[k], tol=1e-9, precision="mixed")["u"] for k in range(2)]
     # a budget smaller than any request: requests run one at a time, in order
     srv = P.Server(max_inflight=4, hbm_budget=1)
