@@ -55,9 +55,9 @@ __device__ __forceinline__ double waveSum(double v) {
     return v;
 }
 
-// sums v[0..NV) over the block; the result is valid in thread 0
-template <int NV> __device__ __forceinline__ void blockSum(double (&v)[NV]) {
-    __shared__ double sh[NV][BLOCK / 64];
+// sums v[0..NV) over the block of NTH threads; the result is valid in thread 0
+template <int NV, int NTH = BLOCK> __device__ __forceinline__ void blockSum(double (&v)[NV]) {
+    __shared__ double sh[NV][NTH / 64];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
     for (int k = 0; k < NV; k++) v[k] = waveSum(v[k]);
@@ -67,7 +67,7 @@ template <int NV> __device__ __forceinline__ void blockSum(double (&v)[NV]) {
     __syncthreads();
     if (wid == 0) {
 #pragma unroll
-        for (int k = 0; k < NV; k++) v[k] = waveSum(lane < BLOCK / 64 ? sh[k][lane] : 0.0);
+        for (int k = 0; k < NV; k++) v[k] = waveSum(lane < NTH / 64 ? sh[k][lane] : 0.0);
     }
     __syncthreads();
 }
@@ -84,8 +84,10 @@ template <int NV> __device__ __forceinline__ void blockPartials(double (&v)[NV],
 }
 
 // guard: skip when *guard != 0 (the producer kernel returned early for the same reason)
-// one block per instance (gridDim.x = instances)
-template <int NV, class Fin> __global__ __launch_bounds__(BLOCK) void kFinish(Reduce red, unsigned nb, const int* guard, Fin fin) {
+// one block of FIN_BLOCK threads per instance (gridDim.x = instances): the partials are read by one
+// compute unit, so the wider block keeps more loads in flight
+constexpr int FIN_BLOCK = 1024;
+template <int NV, class Fin> __global__ __launch_bounds__(FIN_BLOCK) void kFinish(Reduce red, unsigned nb, const int* guard, Fin fin) {
     const int inst = blockIdx.x;
     if (guard && guard[int64_t(inst) * CG_INTS]) return;
     red.partials += int64_t(inst) * red.stride;
@@ -98,16 +100,16 @@ template <int NV, class Fin> __global__ __launch_bounds__(BLOCK) void kFinish(Re
         double a[U];
 #pragma unroll
         for (int u = 0; u < U; u++) a[u] = 0;
-        for (unsigned b0 = 0; b0 < nb; b0 += BLOCK * U) {
+        for (unsigned b0 = 0; b0 < nb; b0 += FIN_BLOCK * U) {
 #pragma unroll
             for (int u = 0; u < U; u++) {
-                const unsigned b = b0 + u * BLOCK + threadIdx.x;
+                const unsigned b = b0 + u * FIN_BLOCK + threadIdx.x;
                 if (b < nb) a[u] += red.partials[k * nb + b];
             }
         }
         s[k] = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
     }
-    blockSum<NV>(s);
+    blockSum<NV, FIN_BLOCK>(s);
     if (threadIdx.x == 0) fin(s, inst);
 }
 
@@ -394,8 +396,9 @@ __global__ __launch_bounds__(BLOCK) void kProlong(DevLevel F, DevLevel C, const 
     zf += instOffset(F.nDof);
     const int64_t fn = int64_t(blockIdx.x) * BLOCK + threadIdx.x;
     if (fn >= F.nNodes) return;
-    const T* __restrict__ finv = Prec<T>::invD(F);
-    const bool f0 = finv[3 * fn] != T(0), f1 = finv[3 * fn + 1] != T(0), f2 = finv[3 * fn + 2] != T(0);
+    // held / inactive DOFs from the node info word (bits 8 + d: inverse diagonal 0)
+    const uint32_t info = F.nodeInfo[fn];
+    const bool f0 = !(info & 0x100u), f1 = !(info & 0x200u), f2 = !(info & 0x400u);
     if (!(f0 || f1 || f2)) return;
     const unsigned u = unsigned(fn), FNX = unsigned(F.NX), FNY = unsigned(F.NY);
     const int i = int(u % FNX), j = int((u / FNX) % FNY), k = int(u / (FNX * FNY));
@@ -522,15 +525,20 @@ __global__ __launch_bounds__(BLOCK) void kUpdateP(const T* __restrict__ z, T* __
 
 // mixed precision with FP32 recurrences: the correction x32 and the residual r32 in units of
 // 1 / rmax; the correction joins the FP64 solution at every fold
+// The fold kernels run in every iteration and return at once unless a fold is due, so they use a
+// small grid (grid-stride loops): an early return then costs a few hundred workgroups, not one per
+// 256 DOFs.
+constexpr unsigned FOLD_GRID = 1024;
 __global__ __launch_bounds__(BLOCK) void kFoldX(double* __restrict__ u, float* __restrict__ x, int64_t n, const CgState* st) {
     st += blockIdx.y;
     if (st->noFold) return;
-    const int64_t i = int64_t(blockIdx.x) * BLOCK + threadIdx.x;
-    if (i >= n) return;
     u += instOffset(n);
     x += instOffset(n);
-    u[i] += double(x[i]) * st->rmax;
-    x[i] = 0.f;
+    const double rmax = st->rmax;
+    for (int64_t i = int64_t(blockIdx.x) * BLOCK + threadIdx.x; i < n; i += int64_t(gridDim.x) * BLOCK) {
+        u[i] += double(x[i]) * rmax;
+        x[i] = 0.f;
+    }
 }
 
 __global__ __launch_bounds__(BLOCK) void kFoldR(const double* __restrict__ f, const double* __restrict__ Kx, const double* __restrict__ invD,
@@ -543,11 +551,19 @@ __global__ __launch_bounds__(BLOCK) void kFoldR(const double* __restrict__ f, co
     r32 += instOffset(n);
     const double inv = 1.0 / st->rmax;
     double v[1] = {0};
-    TILE_LOOP(
-        const double ri = invD[i] != 0 ? f[i] - Kx[i] : 0.0;
-        r64[i] = ri;
-        r32[i] = float(ri * inv);
-        v[0] += ri * ri;)
+    // tiles of RED_TILE elements, block b taking tiles b, b + gridDim.x, ...
+    for (int64_t t0 = int64_t(blockIdx.x) * RED_TILE; t0 < n; t0 += int64_t(gridDim.x) * RED_TILE) {
+#pragma unroll
+        for (int k = 0; k < RED_TILE / BLOCK; k++) {
+            const int64_t i = t0 + threadIdx.x + k * BLOCK;
+            if (i < n) {
+                const double ri = invD[i] != 0 ? f[i] - Kx[i] : 0.0;
+                r64[i] = ri;
+                r32[i] = float(ri * inv);
+                v[0] += ri * ri;
+            }
+        }
+    }
     blockPartials<1>(v, red);
 }
 
@@ -793,49 +809,51 @@ void launchCoarse(const int32_t* freeIdx, const int32_t* dense, int64_t m, int64
 template <class T> void launchDotPQ(const T* p, const T* q, int64_t n, CgState* st, Reduce red, hipStream_t s, int ninst) {
     const unsigned g = reduceGrid(n);
     kDotPQ<T><<<dim3(g, ninst), BLOCK, 0, s>>>(p, q, n, st, red);
-    kFinish<1, FinPQ><<<ninst, BLOCK, 0, s>>>(red, g, &st->stop, FinPQ{st});
+    kFinish<1, FinPQ><<<ninst, FIN_BLOCK, 0, s>>>(red, g, &st->stop, FinPQ{st});
 }
 template <class T> void launchUpdateXR(T* x, const T* p, T* r, const T* q, int64_t n, CgState* st, Reduce red, hipStream_t s, int ninst) {
     const unsigned g = reduceGrid(n);
     kUpdateXR<T><<<dim3(g, ninst), BLOCK, 0, s>>>(x, p, r, q, n, st, red);
-    kFinish<1, FinXR><<<ninst, BLOCK, 0, s>>>(red, g, &st->stop, FinXR{st});
+    kFinish<1, FinXR><<<ninst, FIN_BLOCK, 0, s>>>(red, g, &st->stop, FinXR{st});
 }
 template <class T> void launchDotsRZ(const T* r, const T* z, const T* q, int64_t n, CgState* st, Reduce red, hipStream_t s, int ninst) {
     const unsigned g = reduceGrid(n);
     kDotsRZ<T><<<dim3(g, ninst), BLOCK, 0, s>>>(r, z, q, n, st, red);
-    kFinish<2, FinRZ><<<ninst, BLOCK, 0, s>>>(red, g, &st->stop, FinRZ{st});
+    kFinish<2, FinRZ><<<ninst, FIN_BLOCK, 0, s>>>(red, g, &st->stop, FinRZ{st});
 }
 template <class T> void launchUpdateP(const T* z, T* p, int64_t n, CgState* st, hipStream_t s, int ninst) {
     kUpdateP<T><<<dim3(flatGrid(n), ninst), BLOCK, 0, s>>>(z, p, n, st);
 }
 
 // finishers for dot products whose block partials another kernel wrote ([k * nb + block])
-void launchFinishPQ(Reduce red, unsigned nb, CgState* st, hipStream_t s, int ninst) { kFinish<1, FinPQ><<<ninst, BLOCK, 0, s>>>(red, nb, &st->stop, FinPQ{st}); }
-void launchFinishRZ(Reduce red, unsigned nb, CgState* st, hipStream_t s, int ninst) { kFinish<2, FinRZ><<<ninst, BLOCK, 0, s>>>(red, nb, &st->stop, FinRZ{st}); }
+void launchFinishPQ(Reduce red, unsigned nb, CgState* st, hipStream_t s, int ninst) { kFinish<1, FinPQ><<<ninst, FIN_BLOCK, 0, s>>>(red, nb, &st->stop, FinPQ{st}); }
+void launchFinishRZ(Reduce red, unsigned nb, CgState* st, hipStream_t s, int ninst) { kFinish<2, FinRZ><<<ninst, FIN_BLOCK, 0, s>>>(red, nb, &st->stop, FinRZ{st}); }
 
 void launchUpdateXR32Fold(float* x, const float* p, float* r, const float* q, int64_t n, CgState* st, Reduce red, hipStream_t s, int ninst) {
     const unsigned g = reduceGrid(n);
     kUpdateXR<float><<<dim3(g, ninst), BLOCK, 0, s>>>(x, p, r, q, n, st, red);
-    kFinish<1, FinXRFold><<<ninst, BLOCK, 0, s>>>(red, g, &st->stop, FinXRFold{st});
+    kFinish<1, FinXRFold><<<ninst, FIN_BLOCK, 0, s>>>(red, g, &st->stop, FinXRFold{st});
 }
-void launchFoldX(double* u, float* x, int64_t n, CgState* st, hipStream_t s, int ninst) { kFoldX<<<dim3(flatGrid(n), ninst), BLOCK, 0, s>>>(u, x, n, st); }
+void launchFoldX(double* u, float* x, int64_t n, CgState* st, hipStream_t s, int ninst) {
+    kFoldX<<<dim3(std::min(flatGrid(n), FOLD_GRID), ninst), BLOCK, 0, s>>>(u, x, n, st);
+}
 
 void launchFoldR(const double* f, const double* Kx, const double* invD, double* r64, float* r32, int64_t n, CgState* st, Reduce red, hipStream_t s,
                  int ninst) {
-    const unsigned g = reduceGrid(n);
+    const unsigned g = std::min(reduceGrid(n), FOLD_GRID);
     kFoldR<<<dim3(g, ninst), BLOCK, 0, s>>>(f, Kx, invD, r64, r32, n, st, red);
-    kFinish<1, FinFoldR><<<ninst, BLOCK, 0, s>>>(red, g, &st->noFold, FinFoldR{st});
+    kFinish<1, FinFoldR><<<ninst, FIN_BLOCK, 0, s>>>(red, g, &st->noFold, FinFoldR{st});
 }
 
 void launchResidual64(const double* f, const double* Kx, const double* invD, double* r, int64_t n, double* out, Reduce red, hipStream_t s, int ninst) {
     const unsigned g = reduceGrid(n);
     kResidual64<<<dim3(g, ninst), BLOCK, 0, s>>>(f, Kx, invD, r, n, red);
-    kFinish<1, FinStore><<<ninst, BLOCK, 0, s>>>(red, g, nullptr, FinStore{out});
+    kFinish<1, FinStore><<<ninst, FIN_BLOCK, 0, s>>>(red, g, nullptr, FinStore{out});
 }
 void launchNormFree64(const double* f, const double* invD, int64_t n, double* out, Reduce red, hipStream_t s, int ninst) {
     const unsigned g = reduceGrid(n);
     kNormFree64<<<dim3(g, ninst), BLOCK, 0, s>>>(f, invD, n, red);
-    kFinish<1, FinStore><<<ninst, BLOCK, 0, s>>>(red, g, nullptr, FinStore{out});
+    kFinish<1, FinStore><<<ninst, FIN_BLOCK, 0, s>>>(red, g, nullptr, FinStore{out});
 }
 void launchMaskCopy64(const double* x0, const double* invD, double* x, int64_t n, hipStream_t s, int ninst) {
     kMaskCopy64<<<dim3(flatGrid(n), ninst), BLOCK, 0, s>>>(x0, invD, x, n);
@@ -855,12 +873,12 @@ void launchLzX(const double* v, const double* invD, double* x, int64_t n, hipStr
 void launchLzW(double* w, const double* invD, const double* vPrev, const double* v, int64_t n, CgState* st, int j, Reduce red, hipStream_t s) {
     const unsigned g = reduceGrid(n);
     kLzW<<<g, BLOCK, 0, s>>>(w, invD, vPrev, v, n, st, j, red);
-    kFinish<1, FinLzA><<<1, BLOCK, 0, s>>>(red, g, nullptr, FinLzA{st, j});
+    kFinish<1, FinLzA><<<1, FIN_BLOCK, 0, s>>>(red, g, nullptr, FinLzA{st, j});
 }
 void launchLzAxpy(double* w, const double* v, int64_t n, CgState* st, int j, Reduce red, hipStream_t s) {
     const unsigned g = reduceGrid(n);
     kLzAxpy<<<g, BLOCK, 0, s>>>(w, v, n, st, j, red);
-    kFinish<1, FinLzB><<<1, BLOCK, 0, s>>>(red, g, nullptr, FinLzB{st, j});
+    kFinish<1, FinLzB><<<1, FIN_BLOCK, 0, s>>>(red, g, nullptr, FinLzB{st, j});
 }
 void launchLzNext(const double* w, double* v, double* vPrev, int64_t n, CgState* st, int j, hipStream_t s) {
     kLzNext<<<flatGrid(n), BLOCK, 0, s>>>(w, v, vPrev, n, st, j);
