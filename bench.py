@@ -18,6 +18,7 @@ sources) on the host cores over a bounded sample of the same solve.
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--dims 512,128,128] [--no-cpu]
     python bench.py --batched [--nrhs 16]      (16 load cases of the 128x64x64 SIMP block, one solve)
     python bench.py --serve N [--rate 200]     (the serving stream through the request loop)
+    python bench.py --build-only [--steps K]   (hierarchy build on the device against the host build)
 """
 import argparse
 import json
@@ -227,6 +228,29 @@ def batched(args, P, W):
     print(json.dumps(out), flush=True)
 
 
+def build_only(args, P, w, coarsest):
+    """Hierarchy build (levels, Galerkin coarse elements, coarsest inverse) plus the Lanczos smoother
+    estimates of one part, from host arrays to a model ready to solve: device kernels against the host
+    CPU build, --steps times each after one warm-up."""
+    out = {}
+    for name, dev in (("device", True), ("host", False)):
+        ts = []
+        for s in range(args.steps + 1):
+            t = time.perf_counter()
+            m = P.Model(w["dims"], w["density"], w["nu"], w["bc"], coarsest, device_build=dev)
+            m.estimate_smoothers()
+            dt = time.perf_counter() - t
+            levels = m.num_levels
+            m.close()
+            if s:
+                ts.append(dt)
+        out[name] = {"ms_mean": round(1e3 * float(np.mean(ts)), 2), "ms_min": round(1e3 * float(np.min(ts)), 2)}
+        log("%s build + smoother estimates: %.1f ms mean, %.1f ms best (%d levels)" % (name, 1e3 * np.mean(ts), 1e3 * np.min(ts), levels))
+    print(json.dumps({"metric": "hierarchy build + smoother estimates (ms)", "workload": w["name"], "dof": 3 * int(np.prod([d + 1 for d in w["dims"]])),
+                      "steps": args.steps, "device": out["device"], "host": out["host"],
+                      "speedup": round(out["host"]["ms_mean"] / out["device"]["ms_mean"], 2)}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -243,6 +267,8 @@ def main():
     ap.add_argument("--inflight", type=int, default=4, help="serving: requests on the device at once")
     ap.add_argument("--batched", action="store_true", help="batched load cases: the 128x64x64 SIMP block, --nrhs cases in one multi-instance solve")
     ap.add_argument("--nrhs", type=int, default=16)
+    ap.add_argument("--build-only", action="store_true",
+                    help="hierarchy build + smoother estimates of the workload, on the device and on the host, --steps times each")
     ap.add_argument("--kernels-only", action="store_true",
                     help="counter runs: three 1 GiB device copies (the calibration kernel), then the per-level operator and V-cycle timings")
     args = ap.parse_args()
@@ -263,6 +289,9 @@ def main():
     w = W.void_block(dims, seed=1)
     n_dof = 3 * W.nodes(dims)
     log("workload", w["name"], "DOF", n_dof, "void fraction %.3f" % w["void_fraction"])
+    if args.build_only:
+        build_only(args, P, w, coarsest)
+        return
     # hierarchy build + smoother estimates: on the device (the model the solves use) and, for
     # comparison, on the host CPU
     t0 = time.perf_counter()

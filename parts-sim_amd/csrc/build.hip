@@ -9,7 +9,7 @@
 // depends on an ordering (element numbers, own-matrix numbers, the coarsest free-DOF numbering) comes
 // from a deterministic scan in voxel order. The one exception is the coarsest inverse: the host
 // factors A = L L^T and forms L^-T L^-1; here A^-1 comes from m Gauss-Jordan sweeps of the symmetric
-// matrix (one launch per pivot, the whole matrix updated per launch), with the host's pivot
+// matrix (one launch per two pivots, the whole matrix updated per launch), with the host's pivot
 // regularisation (a pivot at most 1e-10 of the largest diagonal entry is replaced by that entry,
 // voxel_fea.cpp:644-650). Both are the same matrix to rounding.
 //
@@ -41,29 +41,39 @@ __device__ constexpr int dCX(int b) { return (b == 1 || b == 2 || b == 5 || b ==
 __device__ constexpr int dCY(int b) { return (b == 2 || b == 3 || b == 6 || b == 7) ? 1 : 0; }
 __device__ constexpr int dCZ(int b) { return b >= 4 ? 1 : 0; }
 
-// one atomic per wavefront: adds the wave's sum of v to *c
-__device__ __forceinline__ void waveAdd(u64* c, u64 v) {
+// The counting kernels run on a fixed grid (grid-stride loops) and add one block sum per counter
+// with one atomic: a few thousand same-address atomics instead of one per wavefront.
+constexpr unsigned COUNT_GRID = 2048;
+inline unsigned gridCount(int64_t n) { return std::min(grid1(n), COUNT_GRID); }
+// adds the block's sum of v to *c (every thread of the block calls it)
+__device__ __forceinline__ void blockAdd(u64* c, u64 v) {
+    __shared__ u64 sh[BB / 64];
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
-    if ((threadIdx.x & 63) == 0 && v) atomicAdd(c, v);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const u64 t = sh[0] + sh[1] + sh[2] + sh[3];
+        if (t) atomicAdd(c, t);
+    }
 }
 
 // ------------------------------------------------------------------------------------------------
 // level 0: density check and voxel words
 
 __global__ void kWords0(const float* dens, int64_t n, int32_t* words, u64* cnt) {
-    const int64_t v = int64_t(blockIdx.x) * BB + threadIdx.x;
     u64 e = 0, bad = 0;
-    if (v < n) {
+    for (int64_t v = int64_t(blockIdx.x) * BB + threadIdx.x; v < n; v += int64_t(gridDim.x) * BB) {
         const float d = dens[v];
-        if (!(isfinite(d) && d >= 0.f && d <= 1.f)) bad = 1;
+        if (!(isfinite(d) && d >= 0.f && d <= 1.f)) bad++;
         else if (d > 0.f) {
             words[v] = __float_as_int(d);
-            e = 1;
+            e++;
         }
     }
-    waveAdd(&cnt[0], e);
-    waveAdd(&cnt[1], bad);
+    blockAdd(&cnt[0], e);
+    blockAdd(&cnt[1], bad);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -76,10 +86,9 @@ struct Grid {
 // per coarse voxel: empty (no child), uniform (8 children, all s * Kb with one s: the word is s),
 // or own (flag 1; the word is set by kAssignOwn)
 __global__ void kClassify(Grid F, Grid C, const int32_t* fw, int32_t* cw, uint8_t* ownFlag, u64* cnt) {
-    const int64_t V = int64_t(blockIdx.x) * BB + threadIdx.x;
     const int64_t nV = int64_t(C.nx) * C.ny * C.nz;
     u64 e = 0;
-    if (V < nV) {
+    for (int64_t V = int64_t(blockIdx.x) * BB + threadIdx.x; V < nV; V += int64_t(gridDim.x) * BB) {
         const int I = int(V % C.nx), J = int((V / C.nx) % C.ny), K = int(V / (int64_t(C.nx) * C.ny));
         int count = 0;
         bool uniform = true, first = true;
@@ -100,9 +109,9 @@ __global__ void kClassify(Grid F, Grid C, const int32_t* fw, int32_t* cw, uint8_
         uniform = uniform && count == 8;
         ownFlag[V] = count > 0 && !uniform ? 1 : 0;
         cw[V] = count > 0 && uniform ? __float_as_int(s) : 0;
-        e = count > 0 ? 1 : 0;
+        e += count > 0 ? 1 : 0;
     }
-    waveAdd(&cnt[0], e);
+    blockAdd(&cnt[0], e);
 }
 
 // own flags per block of SCAN voxels
@@ -150,9 +159,8 @@ __global__ void kAssignOwn(const uint8_t* flag, int64_t n, const int32_t* blockO
 // child-scale form (the finer level has no own elements): the 8 child scales of each own element
 // (0 for an absent child) and its two-scale record [a, b, mask of the b children or -1, 0]
 __global__ void kChildForm(Grid F, Grid C, const int32_t* fw, const int32_t* ownVox, int64_t nOwn, float* childS, int32_t* childTwo, u64* cnt) {
-    const int64_t e = int64_t(blockIdx.x) * BB + threadIdx.x;
     u64 two = 0;
-    if (e < nOwn) {
+    for (int64_t e = int64_t(blockIdx.x) * BB + threadIdx.x; e < nOwn; e += int64_t(gridDim.x) * BB) {
         const int64_t V = ownVox[e];
         const int I = int(V % C.nx), J = int((V / C.nx) % C.ny), K = int(V / (int64_t(C.nx) * C.ny));
         float cs[8];
@@ -177,9 +185,9 @@ __global__ void kChildForm(Grid F, Grid C, const int32_t* fw, const int32_t* own
         childTwo[4 * e + 1] = __float_as_int(b);
         childTwo[4 * e + 2] = t2;
         childTwo[4 * e + 3] = 0;
-        two = t2 >= 0 ? 1 : 0;
+        two += t2 >= 0 ? 1 : 0;
     }
-    waveAdd(&cnt[0], two);
+    blockAdd(&cnt[0], two);
 }
 
 // own coarse element = sum over its children in child order of s M_c (child s * Kb) or
@@ -290,10 +298,9 @@ __global__ void kCoarseDiagAdd(Grid F, Grid C, const double* fda, double* cda) {
 
 __global__ void kNodes(Grid G, const int32_t* words, const float* Kown, const double* Kb, const uint8_t* bc, const double* diagAdd, double* invD64,
                        float* invD32, uint16_t* info, float* nodeScale, u64* cnt) {
-    const int64_t n = int64_t(blockIdx.x) * BB + threadIdx.x;
     const int64_t nN = int64_t(G.NX) * G.NY * G.NZ;
     u64 nfree = 0, irregular = 0;
-    if (n < nN) {
+    for (int64_t n = int64_t(blockIdx.x) * BB + threadIdx.x; n < nN; n += int64_t(gridDim.x) * BB) {
         const int i = int(n % G.NX), j = int((n / G.NX) % G.NY), k = int(n / (int64_t(G.NX) * G.NY));
         double diag[3] = {0, 0, 0};
         bool act = false;
@@ -341,7 +348,7 @@ __global__ void kNodes(Grid G, const int32_t* words, const float* Kown, const do
         for (int v = 0; v < 8; v++)
             if (!(sv[v] == sv[ref] && sv[v] >= 0)) m |= uint16_t(1u << v);
         info[n] = uint16_t(word | m);
-        irregular = m ? 1 : 0;
+        irregular += m ? 1 : 0;
         float ns = 0.f;
         if (i >= 1 && j >= 1 && k >= 1 && i < G.nx && j < G.ny && k < G.nz && sv[0] > 0) {
             bool same = true;
@@ -350,8 +357,8 @@ __global__ void kNodes(Grid G, const int32_t* words, const float* Kown, const do
         }
         nodeScale[n] = ns;
     }
-    waveAdd(&cnt[0], nfree);
-    waveAdd(&cnt[1], irregular);
+    blockAdd(&cnt[0], nfree);
+    blockAdd(&cnt[1], irregular);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -417,6 +424,31 @@ __global__ void kSweep(const double* A, double* B, int64_t m, int64_t k, const d
     else if (i == k) v = akj / d;
     else if (j == k) v = aik / d;
     else v = A[t] - aik * akj / d;
+    B[t] = v;
+}
+
+// two sweeps in one launch (pivots k and k + 1): every thread evaluates the entries of sweep_k(A)
+// it needs with kSweep's expressions, then sweep_(k+1) of those; bitwise the two launches' result
+__global__ void kSweep2(const double* A, double* B, int64_t m, int64_t k, const double* maxDiag) {
+    const int64_t t = int64_t(blockIdx.x) * BB + threadIdx.x;
+    if (t >= m * m) return;
+    const int64_t i = t / m, j = t % m, k1 = k + 1;
+    const double md = *maxDiag;
+    double d0 = A[k * m + k];
+    if (!(d0 > 1e-10 * md)) d0 = md;
+    auto s1 = [&](int64_t a, int64_t b) -> double {
+        if (a == k && b == k) return -1 / d0;
+        if (a == k) return A[k * m + b] / d0;
+        if (b == k) return A[a * m + k] / d0;
+        return A[a * m + b] - A[a * m + k] * A[k * m + b] / d0;
+    };
+    double d1 = s1(k1, k1);
+    if (!(d1 > 1e-10 * md)) d1 = md;
+    double v;
+    if (i == k1 && j == k1) v = -1 / d1;
+    else if (i == k1) v = s1(k1, j) / d1;
+    else if (j == k1) v = s1(i, k1) / d1;
+    else v = s1(i, j) - s1(i, k1) * s1(k1, j) / d1;
     B[t] = v;
 }
 
@@ -532,7 +564,7 @@ void buildOnDevice(Model& M, const psx_model_desc& d) {
         float* dens = tmp.get<float>(L.nVox);
         HIPCHK(hipMemcpyAsync(dens, d.density, size_t(L.nVox) * 4, hipMemcpyHostToDevice, s));
         int32_t* words = M.alloc<int32_t>(L.nVox);
-        kWords0<<<grid1(L.nVox), BB, 0, s>>>(dens, L.nVox, words, cnt);
+        kWords0<<<gridCount(L.nVox), BB, 0, s>>>(dens, L.nVox, words, cnt);
         D.words = words;
         counters(2);
         if (hc[1]) throw std::invalid_argument("Voxel density must be between zero and one.");
@@ -555,7 +587,7 @@ void buildOnDevice(Model& M, const psx_model_desc& d) {
             float* invD32 = M.alloc<float>(L.nDof);
             uint16_t* info = M.alloc<uint16_t>(L.nNodes);
             float* nodeScale = M.alloc<float>(L.nNodes);
-            kNodes<<<grid1(L.nNodes), BB, 0, s>>>(gridOf(L), D.words, D.Kown, D.Kb64, bcL, D.diagAdd, invD64, invD32, info, nodeScale, cnt);
+            kNodes<<<gridCount(L.nNodes), BB, 0, s>>>(gridOf(L), D.words, D.Kown, D.Kb64, bcL, D.diagAdd, invD64, invD32, info, nodeScale, cnt);
             D.invD64 = invD64;
             D.invD32 = invD32;
             D.nodeInfo = info;
@@ -582,7 +614,7 @@ void buildOnDevice(Model& M, const psx_model_desc& d) {
         HIPCHK(hipMemcpyAsync(dM, Mc.data(), Mc.size() * 8, hipMemcpyHostToDevice, s));
         int32_t* cw = M.alloc<int32_t>(C.nVox);
         uint8_t* flag = tmp.get<uint8_t>(C.nVox);
-        kClassify<<<grid1(C.nVox), BB, 0, s>>>(gridOf(Fl), gridOf(C), DF.words, cw, flag, cnt);
+        kClassify<<<gridCount(C.nVox), BB, 0, s>>>(gridOf(Fl), gridOf(C), DF.words, cw, flag, cnt);
         const int64_t nb = (C.nVox + SCAN - 1) / SCAN;
         int32_t* bcount = tmp.get<int32_t>(nb);
         kCountFlags<<<unsigned(nb), BB, 0, s>>>(flag, C.nVox, bcount);
@@ -614,7 +646,7 @@ void buildOnDevice(Model& M, const psx_model_desc& d) {
             if (nOwn > 0) {
                 float* cs = M.alloc<float>(nOwn * 8);
                 int32_t* two = M.alloc<int32_t>(nOwn * 4);
-                kChildForm<<<grid1(nOwn), BB, 0, s>>>(gridOf(Fl), gridOf(C), DF.words, ownVox, nOwn, cs, two, cnt);
+                kChildForm<<<gridCount(nOwn), BB, 0, s>>>(gridOf(Fl), gridOf(C), DF.words, ownVox, nOwn, cs, two, cnt);
                 DC.childS = cs;
                 DC.childTwo = two;
                 M.uploadChildConstants(l + 1);
@@ -658,8 +690,9 @@ void buildOnDevice(Model& M, const psx_model_desc& d) {
         HIPCHK(hipMemsetAsync(A, 0, size_t(m * m) * 8, s));
         kAssemble<<<grid1(m), BB, 0, s>>>(gridOf(L), D.words, D.Kown, D.Kb64, M.freeIdx, M.denseIdx, m, D.diagAdd, A);
         kMaxDiag<<<1, BB, 0, s>>>(A, m, md);
-        for (i64 k = 0; k < m; k++) {
-            kSweep<<<grid1(m * m), BB, 0, s>>>(A, B, m, k, md);
+        for (i64 k = 0; k < m; k += 2) {
+            if (k + 1 < m) kSweep2<<<grid1(m * m), BB, 0, s>>>(A, B, m, k, md);
+            else kSweep<<<grid1(m * m), BB, 0, s>>>(A, B, m, k, md);
             std::swap(A, B);
         }
         double* Ainv = M.alloc<double>(m * m);
