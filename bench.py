@@ -267,6 +267,8 @@ def main():
     ap.add_argument("--inflight", type=int, default=4, help="serving: requests on the device at once")
     ap.add_argument("--batched", action="store_true", help="batched load cases: the 128x64x64 SIMP block, --nrhs cases in one multi-instance solve")
     ap.add_argument("--nrhs", type=int, default=16)
+    ap.add_argument("--profile-solves", default="", choices=["", "mixed", "fp64", "ref"],
+                    help="profiling runs: build on the device, then --steps solves of this precision only (no reference solve, no CPU leg)")
     ap.add_argument("--build-only", action="store_true",
                     help="hierarchy build + smoother estimates of the workload, on the device and on the host, --steps times each")
     ap.add_argument("--kernels-only", action="store_true",
@@ -292,17 +294,26 @@ def main():
     if args.build_only:
         build_only(args, P, w, coarsest)
         return
-    # hierarchy build + smoother estimates: on the device (the model the solves use) and, for
-    # comparison, on the host CPU
-    t0 = time.perf_counter()
-    m = P.Model(w["dims"], w["density"], w["nu"], w["bc"], coarsest, device_build=True)
-    m.estimate_smoothers()
-    setup_s = time.perf_counter() - t0
+    if args.profile_solves:
+        m = P.Model(w["dims"], w["density"], w["nu"], w["bc"], coarsest, device_build=True)
+        m.estimate_smoothers()
+        d_f, d_u = P.DeviceArray(n_dof, w["f"]), P.DeviceArray(n_dof)
+        for k in range(args.steps):
+            t = time.perf_counter()
+            info = m.solve_device(d_f, d_u, tol=args.tol, max_iter=2000, precision=args.profile_solves, check_every=8)
+            log("%s solve %d: %d iterations, %.2f ms" % (args.profile_solves, k, info["iterations"], 1e3 * (time.perf_counter() - t)))
+        return
+    # hierarchy build + smoother estimates: on the host CPU (first, so that it also carries the
+    # process's GPU start-up) and on the device (the model the solves use)
     t0 = time.perf_counter()
     mh = P.Model(w["dims"], w["density"], w["nu"], w["bc"], coarsest)
     mh.estimate_smoothers()
     setup_host_s = time.perf_counter() - t0
     mh.close()
+    t0 = time.perf_counter()
+    m = P.Model(w["dims"], w["density"], w["nu"], w["bc"], coarsest, device_build=True)
+    m.estimate_smoothers()
+    setup_s = time.perf_counter() - t0
     log("setup %.3f s on the device (host build %.3f s), levels %d, device MB %.0f" % (setup_s, setup_host_s, m.num_levels, m.device_bytes / 2 ** 20))
     if args.kernels_only:
         a, b = P.DeviceArray(1 << 27), P.DeviceArray(1 << 27)

@@ -55,6 +55,9 @@ GpuFeaSolver::GpuFeaSolver(VoxelFEA& fea) : fea_(fea) {
     // stop at the same level as the CPU model: its coarsest free-DOF count is below every finer one
     d.coarsest_max_dof = fea.levels.back().freeDof;
     d.max_levels = int(fea.levels.size());
+    // the CPU model is built already; the device builds its own copy of the levels (the same data
+    // bit for bit) instead of a second host build
+    d.build_on_device = 1;
     impl_ = std::make_unique<Impl>();
     check(psx_create(&d, &impl_->m));
     if (psx_num_levels(impl_->m) != int(fea.levels.size())) throw std::runtime_error("GPU hierarchy does not match the CPU model");
