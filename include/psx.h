@@ -355,6 +355,74 @@ int psx_hex_element(double nu, double* K, double* corner_stress);
  * static_study.cpp:17-21). Returns the number removed, or a negative status (pruneFloating,
  * voxel_fea.cpp:1086-1119). */
 int psx_prune_floating(int nx, int ny, int nz, const float* density, const uint8_t* held, float* out);
+
+/* ---- Input contract of a CAD part (host; the fill fractions come from psx_voxelize) ----
+ * A surface mesh as the reference's Part holds it (native/src/core/mesh.hpp:37-57): float xyz per
+ * vertex and 3 vertex indices per outward triangle; triangle normals and areas are derived as
+ * computeTriangleData does (mesh.cpp:315-330), the bounding box as boundingBox (mesh.cpp:427-439). */
+typedef struct {
+    const float* vertices;      /* 3 * n_vertices */
+    int64_t n_vertices;
+    const uint32_t* tris;       /* 3 * n_tris */
+    int64_t n_tris;
+} psx_mesh;
+/* Replaces typicalWallThickness(part, samples) (native/src/core/voxelize.cpp:190-217): the 20th
+ * percentile of the wall thickness at `samples` area-weighted triangles, or +inf (1500 = default). */
+int psx_wall_thickness(const psx_mesh* mesh, int samples, double* out);
+/* Replaces thinWallDensity(part, grid, maxThickness) (voxelize.cpp:219-276): [nx*ny*nz]. */
+int psx_thin_wall_density(const psx_mesh* mesh, const double origin[3], double h, const int dims[3], double max_thickness, float* out);
+typedef struct {
+    double wall_thickness;      /* typicalWallThickness of the part */
+    int sub;                    /* voxelizer samples per voxel edge */
+    int voxel_count, thin_voxels;
+    int used_shell;             /* the thin-wall layer was added (wall < 2.5 h) */
+} psx_structural_info;
+/* Replaces StructuralModel(part, resolution) (native/src/fea/structural.cpp:19-64) on the grid
+ * psx_grid_for_box(bbox, resolution) gives: voxel densities [nx*ny*nz] (fill fraction >= 0.3, thin
+ * walls as a connected layer), active and surface nodes [(nx+1)(ny+1)(nz+1)] (either may be NULL).
+ * Needs the device (the voxelizer). */
+int psx_structural_model(const psx_mesh* mesh, const double origin[3], double h, const int dims[3], float* density, uint8_t* active_node,
+                         uint8_t* surface_node, psx_structural_info* info);
+/* A selected surface area (structural.hpp:18-21): triangles, optionally clipped to a sphere. */
+typedef struct {
+    const int32_t* tris;
+    int64_t n_tris;
+    int has_clip;
+    double clip_center[3];
+    double clip_radius;
+} psx_patch;
+typedef struct {                /* Fixture (structural.hpp:23-26) */
+    const char* name;
+    const psx_patch* patches;
+    int n_patches;
+} psx_fixture;
+enum psx_load_type { PSX_LOAD_FORCE = 0, PSX_LOAD_PRESSURE = 1, PSX_LOAD_WIND = 2 };
+typedef struct {                /* Load (structural.hpp:28-35) */
+    int type;                   /* enum psx_load_type */
+    const char* name;
+    double magnitude;           /* N (force), MPa (pressure) */
+    double dir[3];              /* force direction (normalised here) */
+    const psx_patch* patches;
+    int n_patches;
+    const float* forces;        /* wind: N per part triangle (xyz), n_forces = 3 * n_tris */
+    int64_t n_forces;
+} psx_load;
+typedef struct {                /* the Material fields assemble validates and uses (materials.hpp) */
+    double E, nu, yield, uts, density;
+    int brittle;
+} psx_material;
+typedef struct {
+    int fixed_nodes;
+    double total[3];            /* applied load, N */
+    int n_warnings;
+} psx_assembly_info;
+/* Replaces StructuralModel::assemble(fixtures, loads, gravity, material, toMeters)
+ * (structural.cpp:107-209) for the model of `density` on the grid: held DOFs bc and nodal forces f
+ * [3*(nx+1)(ny+1)(nz+1)]; warnings newline-separated into the buffer (may be NULL). Errors as the
+ * reference's invalid_argument (material, units, magnitudes, directions, wind force sizes). */
+int psx_assemble(const psx_mesh* mesh, const double origin[3], double h, const int dims[3], const float* density, const psx_fixture* fixtures,
+                 int n_fixtures, const psx_load* loads, int n_loads, int gravity, const psx_material* material, double to_meters, uint8_t* bc,
+                 double* f, psx_assembly_info* info, char* warnings, size_t warnings_len);
 /* Builds the hierarchy on the host only (VoxelFEA ctor, voxel_fea.cpp:138-183). */
 int psx_hierarchy_create(const psx_model_desc* desc, psx_hierarchy* out);
 void psx_hierarchy_destroy(psx_hierarchy h);

@@ -385,3 +385,111 @@ def ref_timed_phases(dims, density, nu, bc, f, coarsest_max_dof, tol, max_iter):
 
 def ref_threads():
     return int(lib("ref").ref_threads())
+
+
+# ---- input contract of a CAD part: the reference's own buildPart, typicalWallThickness,
+# thinWallDensity, StructuralModel and assemble (oracle/_ref only) ----
+
+def _ref_struct():
+    L = lib("ref")
+    if not getattr(L, "_struct_set", False):
+        i64p = C.POINTER(C.c_int64)
+        L.ref_build_part.restype = C.c_int
+        L.ref_build_part.argtypes = [_f32p, C.c_int64, np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS"), C.c_int64, i64p, i64p]
+        L.ref_part_mesh.restype = None
+        L.ref_part_mesh.argtypes = [C.c_int, _f32p, np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS"), _f32p, _f32p]
+        L.ref_wall_thickness.restype = C.c_double
+        L.ref_wall_thickness.argtypes = [C.c_int, C.c_int]
+        L.ref_thin_wall_density.restype = None
+        L.ref_thin_wall_density.argtypes = [C.c_int, _f64p, C.c_double, np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS"), C.c_double, _f32p]
+        L.ref_structural_model.restype = C.c_int
+        L.ref_structural_model.argtypes = [C.c_int, C.c_int, _f64p, C.POINTER(C.c_double), np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS"),
+                                           C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_double), C.c_void_p]
+        L.ref_assemble.restype = C.c_int
+        L.ref_assemble.argtypes = [C.c_int] + [C.c_void_p] * 4 + [C.c_int, C.c_void_p, C.c_int] + [C.c_void_p] * 6 + [C.c_int, C.c_void_p, C.c_double,
+                                   _u8p, _f64p, C.POINTER(C.c_int), _f64p, C.c_char_p, C.c_size_t]
+        L._struct_set = True
+    return L
+
+
+def ref_build_part(positions, index):
+    """buildPart (weld, orient, refine, triangle data); returns (handle, vertices, tris, tri normals, tri areas)."""
+    L = _ref_struct()
+    pos = np.ascontiguousarray(positions, np.float32).reshape(-1)
+    idx = np.ascontiguousarray(index, np.uint32).reshape(-1)
+    nv, nt = C.c_int64(), C.c_int64()
+    hnd = L.ref_build_part(pos, pos.size, idx, idx.size, C.byref(nv), C.byref(nt))
+    if hnd < 0:
+        raise ValueError(L.ref_last_error().decode())
+    V, T = np.zeros(3 * nv.value, np.float32), np.zeros(3 * nt.value, np.uint32)
+    N, A = np.zeros(3 * nt.value, np.float32), np.zeros(nt.value, np.float32)
+    L.ref_part_mesh(hnd, V, T, N, A)
+    return hnd, V, T, N, A
+
+
+def ref_wall_thickness(part, samples=1500):
+    return float(_ref_struct().ref_wall_thickness(int(part), int(samples)))
+
+
+def ref_thin_wall_density(part, origin, h, dims, max_thickness):
+    d = np.ascontiguousarray(dims, np.int32)
+    out = np.zeros(int(np.prod(d)), np.float32)
+    _ref_struct().ref_thin_wall_density(int(part), np.ascontiguousarray(origin, np.float64), float(h), d, float(max_thickness), out)
+    return out
+
+
+def ref_structural_model(part, resolution):
+    """StructuralModel(part, resolution): (model handle, origin, h, dims, density, active, surface, wall thickness, (voxels, thin))."""
+    L = _ref_struct()
+    o, h, d = np.zeros(3), C.c_double(), np.zeros(3, np.int32)
+    wall = C.c_double()
+    hnd = L.ref_structural_model(int(part), int(resolution), o, C.byref(h), d, None, None, None, C.byref(wall), None)
+    if hnd < 0:
+        raise ValueError(L.ref_last_error().decode())
+    nv, nn = int(np.prod(d)), int(np.prod(d + 1))
+    dens, act, surf, cnt = np.zeros(nv, np.float32), np.zeros(nn, np.uint8), np.zeros(nn, np.uint8), np.zeros(2, np.int32)
+    hnd = L.ref_structural_model(int(part), int(resolution), o, C.byref(h), d, dens.ctypes.data, act.ctypes.data, surf.ctypes.data, C.byref(wall),
+                                 cnt.ctypes.data)
+    return hnd, o, h.value, d, dens, act, surf, wall.value, (int(cnt[0]), int(cnt[1]))
+
+
+def ref_assemble(model, n_nodes, fixtures, loads, gravity, material, to_meters):
+    """StructuralModel::assemble with the same fixture / load description parts_sim_amd.assemble takes."""
+    L = _ref_struct()
+    ntris, tris, has, clip = [], [], [], []
+
+    def add(plist):
+        for p in plist:
+            t = np.asarray(p["tris"], np.int32)
+            ntris.append(t.size)
+            tris.append(t)
+            c = p.get("clip")
+            has.append(1 if c is not None else 0)
+            clip.extend(list(c[0]) + [c[1]] if c is not None else [0, 0, 0, 0])
+    for pl in fixtures:
+        add(pl)
+    for l in loads:
+        add(l.get("patches", []))
+    nt = np.ascontiguousarray(ntris if ntris else [0], np.int64)
+    tt = np.ascontiguousarray(np.concatenate(tris) if tris else np.zeros(1, np.int32), np.int32)
+    hc = np.ascontiguousarray(has if has else [0], np.int32)
+    cl = np.ascontiguousarray(clip if clip else [0.0] * 4, np.float64)
+    fnp = np.ascontiguousarray([len(pl) for pl in fixtures] or [0], np.int32)
+    types = {"force": 0, "pressure": 1, "wind": 2}
+    lt = np.ascontiguousarray([types[l["type"]] for l in loads] or [0], np.int32)
+    lm = np.ascontiguousarray([float(l.get("magnitude", 0.0)) for l in loads] or [0.0], np.float64)
+    lddir = np.ascontiguousarray([x for l in loads for x in l.get("dir", (0.0, -1.0, 0.0))] or [0.0] * 3, np.float64)
+    lnp = np.ascontiguousarray([len(l.get("patches", [])) for l in loads] or [0], np.int32)
+    wf = [np.asarray(l["forces"], np.float32).reshape(-1) if l["type"] == "wind" else np.zeros(0, np.float32) for l in loads]
+    wn = np.ascontiguousarray([w.size for w in wf] or [0], np.int64)
+    wf = np.ascontiguousarray(np.concatenate(wf) if wf and sum(w.size for w in wf) else np.zeros(1, np.float32), np.float32)
+    mt = material
+    mat = np.ascontiguousarray([mt["E"], mt["nu"], mt["yield"], mt["uts"], mt["density"], float(bool(mt.get("brittle", False)))], np.float64)
+    bc, f, tot, fixed = np.zeros(3 * n_nodes, np.uint8), np.zeros(3 * n_nodes), np.zeros(3), C.c_int()
+    buf = C.create_string_buffer(8192)
+    rc = L.ref_assemble(int(model), nt.ctypes.data, tt.ctypes.data, hc.ctypes.data, cl.ctypes.data, len(fixtures), fnp.ctypes.data, len(loads),
+                        lt.ctypes.data, lm.ctypes.data, lddir.ctypes.data, lnp.ctypes.data, wf.ctypes.data, wn.ctypes.data, int(bool(gravity)), mat.ctypes.data,
+                        float(to_meters), bc, f, C.byref(fixed), tot, buf, 8192)
+    if rc < 0:
+        raise ValueError(L.ref_last_error().decode())
+    return bc, f, {"fixed_nodes": fixed.value, "total": list(tot)}, [w for w in buf.value.decode().split("\n") if w]

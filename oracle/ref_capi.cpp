@@ -2,7 +2,8 @@
 //
 // C ABI over the reference's own VoxelFEA, compiled from the reference sources where they lie
 // (oracle/Makefile: native/src/fea/hex8.cpp, native/src/fea/voxel_fea.cpp, native/src/util/parallel.cpp,
-// and for the voxelizer native/src/core/voxelize.cpp with the mesh and BVH sources it links against)
+// and for the voxelizer native/src/core/voxelize.cpp with the mesh and BVH sources it links against;
+// for the input contract of a CAD part native/src/fea/structural.cpp and native/src/core/materials.cpp)
 // into oracle/_ref/libpsref.so. The tests use it to pin the restatement in oracle/oracle.cpp bit for
 // bit, and bench.py may time it as the CPU baseline. The entry points have the same shapes as
 // oracle.cpp's orc_* functions.
@@ -28,6 +29,7 @@
 #undef private
 #include "fea/voxel_fea.cpp"
 #include "core/voxelize.hpp"
+#include "fea/structural.hpp"
 #include "fea/topology.hpp"
 
 static std::string g_err;
@@ -306,6 +308,125 @@ int ref_voxelize(const float* pos, int64_t nVert, const uint32_t* index, int64_t
         const std::vector<uint32_t> ix(index, index + 3 * nTri);
         const auto f = ps::voxelize(p, ix, g, sub);
         std::memcpy(frac, f.data(), f.size() * sizeof(float));
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
+// ---- input contract of a CAD part: Parts built by the reference's buildPart (weld, orient, refine,
+// triangle data), whose final vertices and triangles the tests hand to the product as well ----
+static std::vector<std::shared_ptr<ps::Part>> g_parts;
+
+int ref_build_part(const float* pos, int64_t nFloats, const uint32_t* index, int64_t nIndex, int64_t* nVert, int64_t* nTri) {
+    try {
+        ps::MeshSource src;
+        src.positions.assign(pos, pos + nFloats);
+        src.index.assign(index, index + nIndex);
+        g_parts.push_back(ps::buildPart(src));
+        *nVert = g_parts.back()->nVert;
+        *nTri = g_parts.back()->nTri;
+        return int(g_parts.size()) - 1;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
+void ref_part_mesh(int part, float* vertices, uint32_t* tris, float* normal, float* area) {
+    const ps::Part& p = *g_parts.at(size_t(part));
+    std::copy(p.vertices.begin(), p.vertices.end(), vertices);
+    std::copy(p.tris.begin(), p.tris.end(), tris);
+    if (normal) std::copy(p.triNormal.begin(), p.triNormal.end(), normal);
+    if (area) std::copy(p.triArea.begin(), p.triArea.end(), area);
+}
+
+double ref_wall_thickness(int part, int samples) {
+    ps::Part& p = *g_parts.at(size_t(part));
+    p.wallThickness.reset();
+    return ps::typicalWallThickness(p, samples);
+}
+
+void ref_thin_wall_density(int part, const double* origin, double h, const int* dims, double maxThickness, float* out) {
+    ps::Grid g;
+    for (int d = 0; d < 3; d++) g.origin[d] = origin[d], g.dims[d] = dims[d];
+    g.h = h;
+    const auto v = ps::thinWallDensity(*g_parts.at(size_t(part)), g, maxThickness);
+    std::copy(v.begin(), v.end(), out);
+}
+
+static std::vector<std::unique_ptr<ps::StructuralModel>> g_models;
+
+// StructuralModel(part, resolution): the grid first (density etc. NULL), then the arrays
+int ref_structural_model(int part, int resolution, double* origin, double* h, int* dims, float* density, uint8_t* active, uint8_t* surface, double* wall,
+                         int* counts) {
+    try {
+        g_parts.at(size_t(part))->wallThickness.reset();
+        g_models.push_back(std::make_unique<ps::StructuralModel>(g_parts.at(size_t(part)), resolution));
+        const ps::StructuralModel& m = *g_models.back();
+        for (int d = 0; d < 3; d++) origin[d] = m.grid.origin[d], dims[d] = m.grid.dims[d];
+        *h = m.grid.h;
+        if (density) std::copy(m.density.begin(), m.density.end(), density);
+        if (active) std::copy(m.activeNode.begin(), m.activeNode.end(), active);
+        if (surface) std::copy(m.surfaceNode.begin(), m.surfaceNode.end(), surface);
+        if (wall) *wall = m.wallThickness;
+        if (counts) counts[0] = m.voxelCount, counts[1] = m.thinVoxels;
+        return int(g_models.size()) - 1;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
+// assemble on a model from ref_structural_model. Patches flat: patch p has ntris[p] triangles (from
+// tris, in order), clip[4 p .. 4 p + 3] = centre, radius when hasClip[p]; fixture q takes fixNP[q]
+// patches in order, then load q takes loadNP[q]; wind load q takes windN[q] forces from windForces
+int ref_assemble(int model, const int64_t* ntris, const int32_t* tris, const int* hasClip, const double* clip, int nFix, const int* fixNP, int nLoad,
+                 const int* loadType, const double* loadMag, const double* loadDir, const int* loadNP, const float* windForces, const int64_t* windN, int gravity,
+                 const double* mat, double toMeters, uint8_t* bc, double* f, int* fixedNodes, double* total, char* warnings, size_t wlen) {
+    try {
+        const ps::StructuralModel& m = *g_models.at(size_t(model));
+        int pi = 0;
+        int64_t at = 0;
+        auto takePatches = [&](int n) {
+            std::vector<ps::Patch> out;
+            for (int q = 0; q < n; q++, pi++) {
+                ps::Patch p;
+                p.tris.assign(tris + at, tris + at + ntris[pi]);
+                at += ntris[pi];
+                if (hasClip[pi]) p.clip = ps::Clip{{clip[4 * pi], clip[4 * pi + 1], clip[4 * pi + 2]}, clip[4 * pi + 3]};
+                out.push_back(std::move(p));
+            }
+            return out;
+        };
+        std::vector<ps::Fixture> fx;
+        for (int q = 0; q < nFix; q++) fx.push_back({"F" + std::to_string(q), takePatches(fixNP[q])});
+        std::vector<ps::Load> lds;
+        int64_t wat = 0;
+        for (int q = 0; q < nLoad; q++) {
+            ps::Load l;
+            l.type = ps::Load::Type(loadType[q]);
+            l.name = "L" + std::to_string(q);
+            l.magnitude = loadMag[q];
+            l.dir = {loadDir[3 * q], loadDir[3 * q + 1], loadDir[3 * q + 2]};
+            l.patches = takePatches(loadNP[q]);
+            if (l.type == ps::Load::Wind) {
+                l.forces.assign(windForces + wat, windForces + wat + windN[q]);
+                wat += windN[q];
+            }
+            lds.push_back(std::move(l));
+        }
+        ps::Material mt;
+        mt.E = mat[0], mt.nu = mat[1], mt.yield = mat[2], mt.uts = mat[3], mt.density = mat[4], mt.brittle = mat[5] != 0;
+        const ps::Assembly a = m.assemble(fx, lds, gravity != 0, mt, toMeters);
+        std::copy(a.bc.begin(), a.bc.end(), bc);
+        std::copy(a.f.begin(), a.f.end(), f);
+        *fixedNodes = a.fixedNodes;
+        for (int d = 0; d < 3; d++) total[d] = a.total[d];
+        std::string all;
+        for (const auto& w : a.warnings) all += w + "\n";
+        std::snprintf(warnings, wlen, "%s", all.c_str());
         return 0;
     } catch (const std::exception& e) {
         g_err = e.what();

@@ -94,6 +94,37 @@ class LevelInfo(C.Structure):
                 ("free_dof", C.c_int64), ("n_elems", C.c_int64), ("n_own", C.c_int64), ("lmax", C.c_double)]
 
 
+class PsxMesh(C.Structure):
+    _fields_ = [("vertices", C.c_void_p), ("n_vertices", C.c_int64), ("tris", C.c_void_p), ("n_tris", C.c_int64)]
+
+
+class PsxPatch(C.Structure):
+    _fields_ = [("tris", C.c_void_p), ("n_tris", C.c_int64), ("has_clip", C.c_int), ("clip_center", C.c_double * 3), ("clip_radius", C.c_double)]
+
+
+class PsxFixture(C.Structure):
+    _fields_ = [("name", C.c_char_p), ("patches", C.POINTER(PsxPatch)), ("n_patches", C.c_int)]
+
+
+class PsxLoad(C.Structure):
+    _fields_ = [("type", C.c_int), ("name", C.c_char_p), ("magnitude", C.c_double), ("dir", C.c_double * 3), ("patches", C.POINTER(PsxPatch)),
+                ("n_patches", C.c_int), ("forces", C.c_void_p), ("n_forces", C.c_int64)]
+
+
+class PsxMaterial(C.Structure):
+    _fields_ = [("E", C.c_double), ("nu", C.c_double), ("yield_", C.c_double), ("uts", C.c_double), ("density", C.c_double), ("brittle", C.c_int)]
+
+
+class StructuralInfo(C.Structure):
+    _fields_ = [("wall_thickness", C.c_double), ("sub", C.c_int), ("voxel_count", C.c_int), ("thin_voxels", C.c_int), ("used_shell", C.c_int)]
+
+
+class AssemblyInfo(C.Structure):
+    _fields_ = [("fixed_nodes", C.c_int), ("total", C.c_double * 3), ("n_warnings", C.c_int)]
+
+
+LOAD_TYPES = {"force": 0, "pressure": 1, "wind": 2}
+
 # every entry point declared in include/psx.h: name -> (restype, argtypes)
 _vp, _i, _d, _i64 = C.c_void_p, C.c_int, C.c_double, C.c_int64
 SIGNATURES = {
@@ -147,6 +178,11 @@ SIGNATURES = {
     "psx_model_level_child": (_i, [_vp, _i, _vp, _vp, _vp]),
     "psx_model_coarse": (_i, [_vp, C.POINTER(C.c_int64), _vp, _vp]),
     "psx_hierarchy_coarse": (_i, [_vp, C.POINTER(_i64), _vp, _vp]),
+    "psx_wall_thickness": (_i, [C.POINTER(PsxMesh), _i, C.POINTER(_d)]),
+    "psx_thin_wall_density": (_i, [C.POINTER(PsxMesh), _vp, _d, _vp, _d, _vp]),
+    "psx_structural_model": (_i, [C.POINTER(PsxMesh), _vp, _d, _vp, _vp, _vp, _vp, C.POINTER(StructuralInfo)]),
+    "psx_assemble": (_i, [C.POINTER(PsxMesh), _vp, _d, _vp, _vp, C.POINTER(PsxFixture), _i, C.POINTER(PsxLoad), _i, _i, C.POINTER(PsxMaterial), _d,
+                          _vp, _vp, C.POINTER(AssemblyInfo), C.c_char_p, C.c_size_t]),
 }
 
 _lib = None
@@ -243,6 +279,103 @@ def voxelize(pos, index, origin, h, dims, sub=2):
     frac = np.zeros(int(d[0]) * int(d[1]) * int(d[2]), np.float32)
     _check(lib().psx_voxelize(_p(pos), pos.size // 3, _p(index), index.size // 3, _p(o), float(h), _p(d), int(sub), _p(frac)))
     return frac
+
+
+def _mesh(vertices, tris, keep):
+    v = np.ascontiguousarray(vertices, np.float32).reshape(-1)
+    t = np.ascontiguousarray(tris, np.uint32).reshape(-1)
+    if v.size % 3 or t.size % 3:
+        raise ValueError("Positions and triangle indices come in threes.")
+    keep.extend([v, t])
+    return PsxMesh(v.ctypes.data, v.size // 3, t.ctypes.data, t.size // 3)
+
+
+def wall_thickness(vertices, tris, samples=1500):
+    """typicalWallThickness of a surface mesh: the 20th percentile of the sampled wall thickness, or inf."""
+    keep, out = [], C.c_double()
+    m = _mesh(vertices, tris, keep)
+    _check(lib().psx_wall_thickness(C.byref(m), int(samples), C.byref(out)))
+    return out.value
+
+
+def thin_wall_density(vertices, tris, origin, h, dims, max_thickness):
+    """thinWallDensity: the density thin walls give each voxel of the grid (float32, 0 where none)."""
+    keep = []
+    m = _mesh(vertices, tris, keep)
+    d = np.ascontiguousarray([int(x) for x in dims], np.int32)
+    o = np.ascontiguousarray(origin, np.float64)
+    out = np.zeros(int(np.prod(d)), np.float32)
+    _check(lib().psx_thin_wall_density(C.byref(m), _p(o), float(h), _p(d), float(max_thickness), _p(out)))
+    return out
+
+
+def structural_model(vertices, tris, origin, h, dims):
+    """StructuralModel's voxel densities, active and surface nodes and counts on the given grid (the
+    fill fractions from the device voxelizer)."""
+    keep = []
+    m = _mesh(vertices, tris, keep)
+    d = np.ascontiguousarray([int(x) for x in dims], np.int32)
+    o = np.ascontiguousarray(origin, np.float64)
+    dens = np.zeros(int(np.prod(d)), np.float32)
+    nn = int(np.prod(d + 1))
+    act, surf = np.zeros(nn, np.uint8), np.zeros(nn, np.uint8)
+    info = StructuralInfo()
+    _check(lib().psx_structural_model(C.byref(m), _p(o), float(h), _p(d), _p(dens), _p(act), _p(surf), C.byref(info)))
+    return dens, act, surf, {k: getattr(info, k) for k, _ in StructuralInfo._fields_}
+
+
+def _patches(plist, keep):
+    arr = (PsxPatch * max(1, len(plist)))()
+    for q, p in enumerate(plist):
+        t = np.ascontiguousarray(p["tris"], np.int32)
+        keep.append(t)
+        arr[q].tris, arr[q].n_tris = t.ctypes.data, t.size
+        if p.get("clip") is not None:
+            c, r = p["clip"]
+            arr[q].has_clip = 1
+            arr[q].clip_center = (C.c_double * 3)(*[float(x) for x in c])
+            arr[q].clip_radius = float(r)
+    keep.append(arr)
+    return arr
+
+
+def assemble(vertices, tris, origin, h, dims, density, fixtures, loads, gravity=False, material=None, to_meters=1e-3):
+    """StructuralModel::assemble: held DOFs and nodal forces of fixtures (lists of patches) and loads
+    (dicts: type force / pressure / wind, magnitude, dir, patches, forces) on the grid's nodes.
+    A patch is {"tris": triangle indices, "clip": None or ((x, y, z), radius)}. material: dict with
+    E, nu, yield, uts, density, brittle. Returns (bc, f, info, warnings)."""
+    keep = []
+    m = _mesh(vertices, tris, keep)
+    d = np.ascontiguousarray([int(x) for x in dims], np.int32)
+    o = np.ascontiguousarray(origin, np.float64)
+    dens = np.ascontiguousarray(density, np.float32).reshape(-1)
+    fx = (PsxFixture * max(1, len(fixtures)))()
+    for q, pl in enumerate(fixtures):
+        fx[q].name = b"F%d" % q
+        fx[q].patches = _patches(pl, keep)
+        fx[q].n_patches = len(pl)
+    ld = (PsxLoad * max(1, len(loads)))()
+    for q, l in enumerate(loads):
+        ld[q].type = LOAD_TYPES[l["type"]]
+        ld[q].name = b"L%d" % q
+        ld[q].magnitude = float(l.get("magnitude", 0.0))
+        ld[q].dir = (C.c_double * 3)(*[float(x) for x in l.get("dir", (0.0, -1.0, 0.0))])
+        ld[q].patches = _patches(l.get("patches", []), keep)
+        ld[q].n_patches = len(l.get("patches", []))
+        if l.get("forces") is not None:
+            fw = np.ascontiguousarray(l["forces"], np.float32).reshape(-1)
+            keep.append(fw)
+            ld[q].forces, ld[q].n_forces = fw.ctypes.data, fw.size
+    mt = material or {"E": 200.0, "nu": 0.3, "yield": 250.0, "uts": 400.0, "density": 7850.0, "brittle": False}
+    mat = PsxMaterial(mt["E"], mt["nu"], mt["yield"], mt["uts"], mt["density"], int(bool(mt.get("brittle", False))))
+    nn = int(np.prod(d + 1))
+    bc, f = np.zeros(3 * nn, np.uint8), np.zeros(3 * nn)
+    info = AssemblyInfo()
+    buf = C.create_string_buffer(8192)
+    _check(lib().psx_assemble(C.byref(m), _p(o), float(h), _p(d), _p(dens), fx, len(fixtures), ld, len(loads), int(bool(gravity)), C.byref(mat),
+                              float(to_meters), _p(bc), _p(f), C.byref(info), buf, 8192))
+    out = {"fixed_nodes": info.fixed_nodes, "total": list(info.total), "n_warnings": info.n_warnings}
+    return bc, f, out, [w for w in buf.value.decode().split("\n") if w]
 
 
 def optimize_topology(dims, fill, keep, bc, f, nu, vol_frac=0.4, penal=3.0, rmin=1.5, move=0.2, tol=0.01, max_iter=40, precision="mixed",

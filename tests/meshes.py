@@ -64,3 +64,12 @@ def soup(n=300, seed=1):
 def bounds(pos):
     p = np.asarray(pos, np.float32).reshape(-1, 3)
     return p.min(0).astype(np.float64), p.max(0).astype(np.float64)
+
+
+def hollow_box(lo, hi, t):
+    """A box with walls of thickness t: the outer box outward, the inner cavity inward (its triangles
+    reversed), so every wall has an opposite face facing the other way."""
+    v0, i0 = box(lo, hi)
+    v1, i1 = box(tuple(a + t for a in lo), tuple(b - t for b in hi))
+    i1 = i1.reshape(-1, 3)[:, ::-1].reshape(-1) + v0.size // 3
+    return np.concatenate([v0, v1]), np.concatenate([i0, i1]).astype(np.uint32)
