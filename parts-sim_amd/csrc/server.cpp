@@ -3,8 +3,9 @@
 // Requests queue in arrival order. A pool of max_inflight workers admits the queue head as soon as a
 // worker is free and the head's estimated device footprint fits the budget beside what running
 // requests (and resident parts) hold; resident parts are evicted, least recently used first, to make
-// room. An admitted request builds its hierarchy on the host (or takes its resident part), uploads
-// it and solves its load cases on the model's own HIP stream, so the kernels of concurrent requests
+// room. An admitted request builds its hierarchy with the device kernels (or on the host when the
+// request asks for it, or takes its resident part) and solves its load cases on the model's own HIP
+// stream, so the kernels of concurrent requests
 // overlap on the device while each request's arithmetic stays exactly that of a lone solve.
 #include <hip/hip_runtime_api.h>
 
@@ -55,7 +56,14 @@ struct Request {
 // (about 144 B per DOF), and room for own element matrices
 int64_t footprint(const Request& r) {
     const int64_t n = r.nDof;
-    return n * (40 + 144 * int64_t(r.nrhs)) + 64 * (int64_t(1) << 20);
+    int64_t b = n * (40 + 144 * int64_t(r.nrhs)) + 64 * (int64_t(1) << 20);
+    if (r.onDevice) {
+        // device build: the uploaded density and held DOFs, per-level flags, and the coarsest level's
+        // two m x m sweep buffers beside its inverse (m <= the coarsest free-DOF target, at most 3000)
+        const int64_t m = std::min<int64_t>(std::max<int64_t>(r.coarsest, 1200), 3000);
+        b += 3 * n + 3 * 8 * m * m;
+    }
+    return b;
 }
 
 struct Resident {
