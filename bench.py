@@ -167,7 +167,7 @@ def batched(args, P, W):
     w = W.simp_block(dims, seed=4, nrhs=args.nrhs)
     F = w["f"]
     k, n = F.shape
-    m = P.Model(w["dims"], w["density"], w["nu"], w["bc"], 1200)
+    m = P.Model(w["dims"], w["density"], w["nu"], w["bc"], 1200, device_build=True)
     m.estimate_smoothers()
     log("workload", w["name"], "DOF per case", n, "levels", m.num_levels)
     d_f = P.DeviceArray(F.size, F.reshape(-1))
