@@ -580,7 +580,10 @@ int psx_structural_model(const psx_mesh* mesh, const double origin[3], double h,
         while (sub > 2 && double(nx) * ny * nz * sub * sub * sub > 48e6) sub--;
         const size_t nVox = size_t(nx) * ny * nz;
         std::vector<float> frac(nVox);
-        if (psx_voxelize(M.V, M.nV, M.T, M.nT, origin, h, dims, sub, frac.data()) != PSX_OK) throw psx::DeviceError(psx_last_error());
+        const int vrc = psx_voxelize(M.V, M.nV, M.T, M.nT, origin, h, dims, sub, frac.data());
+        if (vrc == PSX_ERR_ARGUMENT) throw std::invalid_argument(psx_last_error());
+        if (vrc == PSX_ERR_NO_DEVICE) throw psx::NoDevice(psx_last_error());
+        if (vrc != PSX_OK) throw psx::DeviceError(psx_last_error());
         // walls under ~1.5 voxels thick get a connected layer with their true cross-section
         std::vector<float> shell;
         if (wall < 2.5 * h) shell = psx::thinWallDensity(M, bvh, origin, h, dims, 1.5 * h);
