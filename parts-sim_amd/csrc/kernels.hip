@@ -418,9 +418,16 @@ __global__ __launch_bounds__(BLOCK) void kProlong(DevLevel F, DevLevel C, const 
             }
         }
     }
-    if (f0) zf[3 * fn] += s0;
-    if (f1) zf[3 * fn + 1] += s1;
-    if (f2) zf[3 * fn + 2] += s2;
+    // the node's three components in one load and one store (held components written back as read)
+    struct alignas(sizeof(T)) V3 {
+        T a, b, c;
+    };
+    V3* __restrict__ dst = reinterpret_cast<V3*>(zf + 3 * fn);
+    V3 v = *dst;
+    v.a = f0 ? v.a + s0 : v.a;
+    v.b = f1 ? v.b + s1 : v.b;
+    v.c = f2 ? v.c + s2 : v.c;
+    *dst = v;
 }
 
 // z = Ainv r on the coarsest level's free DOFs and 0 on every held one (the whole vector is
