@@ -362,23 +362,36 @@ __global__ __launch_bounds__(BLOCK) void kRestrict(DevLevel F, DevLevel C, const
     const T i0 = cinv[3 * cn], i1 = cinv[3 * cn + 1], i2 = cinv[3 * cn + 2];
     T s0 = 0, s1 = 0, s2 = 0;
     if (i0 != T(0) || i1 != T(0) || i2 != T(0)) {
+        // the 27 fine nodes around 2I, all loads issued together: a neighbour outside the fine grid
+        // (2I itself is outside for the last coarse node of an odd fine count) reads the nearest
+        // fine node, clamped per axis, with weight 0 — an exact zero added to a sum that starts at
+        // +0 leaves its bits unchanged, so the result is the sum over the in-range nodes in order
+        struct alignas(sizeof(T)) V3 {
+            T a, b, c;
+        };
+        const int64_t FNX = F.NX, FNXY = int64_t(F.NX) * F.NY;
+#pragma unroll
         for (int dk = -1; dk <= 1; dk++) {
             const int kf = 2 * K + dk;
-            if (kf < 0 || kf >= F.NZ) continue;
+            const bool kin = kf >= 0 && kf < F.NZ;
+            const int kc = min(max(kf, 0), F.NZ - 1);
             const T wk = dk == 0 ? T(1) : T(0.5);
+#pragma unroll
             for (int dj = -1; dj <= 1; dj++) {
                 const int jf = 2 * J + dj;
-                if (jf < 0 || jf >= F.NY) continue;
+                const bool jin = kin && jf >= 0 && jf < F.NY;
+                const int jc = min(max(jf, 0), F.NY - 1);
                 const T wj = dj == 0 ? wk : T(0.5) * wk;
 #pragma unroll
                 for (int di = -1; di <= 1; di++) {
                     const int fi = 2 * I + di;
-                    if (fi < 0 || fi >= F.NX) continue;
-                    const T w = di == 0 ? wj : T(0.5) * wj;
-                    const T* p = rf + 3 * (fi + int64_t(F.NX) * (jf + int64_t(F.NY) * kf));
-                    s0 += w * p[0];
-                    s1 += w * p[1];
-                    s2 += w * p[2];
+                    const bool in = jin && fi >= 0 && fi < F.NX;
+                    const int ic = min(max(fi, 0), F.NX - 1);
+                    const T w = in ? (di == 0 ? wj : T(0.5) * wj) : T(0);
+                    const V3 v = *reinterpret_cast<const V3*>(rf + 3 * (ic + FNX * jc + FNXY * kc));
+                    s0 += w * v.a;
+                    s1 += w * v.b;
+                    s2 += w * v.c;
                 }
             }
         }
