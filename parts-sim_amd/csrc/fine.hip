@@ -225,12 +225,36 @@ template <class T, int MODE, bool OWNC, bool DOT, bool DENSE = false, bool MF = 
         const bool zin = z >= z0 && z < z1, in = mine && zin;
         const int64_t pb = NXY * (zin ? z : z0);  // block-uniform plane base
         const int32_t c3 = 3 * col;
+        // a node's three components in one load (the address is in range for every thread: plane z0
+        // and column 0 stand in for a thread without a node), zero where the thread has none
+        struct alignas(sizeof(T)) V3 {
+            T a, b, c;
+        };
+        auto ld3 = [&](const T* base, T (&dst)[3]) {
+            const V3 v = *reinterpret_cast<const V3*>(base + 3 * pb + c3);
+            dst[0] = in ? v.a : T(0);
+            dst[1] = in ? v.b : T(0);
+            dst[2] = in ? v.c : T(0);
+        };
+        auto zero3 = [](T (&dst)[3]) { dst[0] = dst[1] = dst[2] = T(0); };
+        if constexpr (DOT) {
+            // the fused-dot variants sit at the 128-VGPR limit: component loads schedule better there
+            // (measured: the three-component loads make the Chebyshev step with fused dots 15 % slower)
 #pragma unroll
-        for (int d = 0; d < 3; d++) {
-            if constexpr (NEEDS_INV && OWNC) o.inv[d] = in ? (invD + 3 * pb)[c3 + d] : T(0);
-            if constexpr (NEEDS_R) o.r[d] = in ? (r + 3 * pb)[c3 + d] : T(0);
-            if constexpr (NEEDS_D) o.d[d] = in && zp ? (zp + 3 * pb)[c3 + d] : T(0);
-            if constexpr (NEEDS_INV) o.qd[d] = in && qdot ? (qdot + 3 * pb)[c3 + d] : T(0);
+            for (int d = 0; d < 3; d++) {
+                if constexpr (NEEDS_INV && OWNC) o.inv[d] = in ? (invD + 3 * pb)[c3 + d] : T(0);
+                if constexpr (NEEDS_R) o.r[d] = in ? (r + 3 * pb)[c3 + d] : T(0);
+                if constexpr (NEEDS_D) o.d[d] = in && zp ? (zp + 3 * pb)[c3 + d] : T(0);
+                if constexpr (NEEDS_INV) o.qd[d] = in && qdot ? (qdot + 3 * pb)[c3 + d] : T(0);
+            }
+        } else {
+            if constexpr (NEEDS_INV && OWNC) ld3(invD, o.inv);
+            if constexpr (NEEDS_R) ld3(r, o.r);
+            if constexpr (NEEDS_D) {
+                if (zp) ld3(zp, o.d);
+                else zero3(o.d);
+            }
+            if constexpr (NEEDS_INV) zero3(o.qd);
         }
         o.info = in ? uint32_t((L.nodeInfo + pb)[col]) : 0u;
     };
