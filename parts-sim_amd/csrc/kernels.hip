@@ -320,17 +320,26 @@ __global__ __launch_bounds__(BLOCK) void kApplyOwn(DevLevel L, const T* __restri
 #pragma unroll
                 for (int d = 0; d < 3; d++) acc[d] += T(L.diagAdd[3 * n + d]) * xi[3 * n + d];
             }
+            // the node's operands and result as three-component loads and one store
+            struct alignas(sizeof(T)) V3 {
+                T c[3];
+            };
+            const V3 idv = *reinterpret_cast<const V3*>(invD + 3 * n);
+            V3 xv{}, rv{}, zv{}, out;
+            if constexpr (MODE == AM_CHEB1 || MODE == AM_CHEB2) xv = *reinterpret_cast<const V3*>(xi + 3 * n);
+            if constexpr (MODE == AM_RESID || MODE == AM_CHEB1 || MODE == AM_CHEB2) rv = *reinterpret_cast<const V3*>(r + io + 3 * n);
+            if constexpr (MODE == AM_CHEB2)
+                if (zp) zv = *reinterpret_cast<const V3*>(zp + io + 3 * n);
 #pragma unroll
             for (int d = 0; d < 3; d++) {
-                const int64_t q = 3 * n + d;
-                const T id = invD[q];
-                T* __restrict__ yq = y + io + q;
-                if constexpr (MODE == AM_APPLY) *yq = id != T(0) ? acc[d] : T(0);
-                else if constexpr (MODE == AM_RAW) *yq = acc[d];
-                else if constexpr (MODE == AM_RESID) *yq = id != T(0) ? r[io + q] - acc[d] : T(0);
-                else if constexpr (MODE == AM_CHEB1) *yq = xi[q] + c1 * id * (r[io + q] - acc[d]);
-                else *yq = xi[q] + c1 * (zp ? xi[q] - zp[io + q] : xi[q]) + c2 * id * (r[io + q] - acc[d]);
+                const T id = idv.c[d];
+                if constexpr (MODE == AM_APPLY) out.c[d] = id != T(0) ? acc[d] : T(0);
+                else if constexpr (MODE == AM_RAW) out.c[d] = acc[d];
+                else if constexpr (MODE == AM_RESID) out.c[d] = id != T(0) ? rv.c[d] - acc[d] : T(0);
+                else if constexpr (MODE == AM_CHEB1) out.c[d] = xv.c[d] + c1 * id * (rv.c[d] - acc[d]);
+                else out.c[d] = xv.c[d] + c1 * (zp ? xv.c[d] - zv.c[d] : xv.c[d]) + c2 * id * (rv.c[d] - acc[d]);
             }
+            *reinterpret_cast<V3*>(y + io + 3 * n) = out;
         }
     }
 }
