@@ -92,7 +92,11 @@ def test_device_build_reactions_and_errors(P, W):
     with pytest.raises(P.PsxError) as e:
         P.Model(w["dims"], w["density"], 0.6, w["bc"], 100, device_build=True)
     assert e.value.code == 1
-    # every DOF held: no coarse direct solve
-    with pytest.raises(P.PsxError) as e:
-        P.Model(w["dims"], w["density"], w["nu"], np.ones_like(w["bc"]), 100, device_build=True)
-    assert e.value.code == 3
+    # every DOF held, or a single level with more than 3000 free DOFs: no coarse direct solve, as
+    # the host build reports it
+    w5 = W.cantilever(5)  # 5400 free DOFs on its finest level
+    for part, bc, levels in ((w, np.ones_like(w["bc"]), 0), (w5, w5["bc"], 1)):
+        for dev in (False, True):
+            with pytest.raises(P.PsxError) as e:
+                P.Model(part["dims"], part["density"], part["nu"], bc, 100, None, levels, device_build=dev)
+            assert e.value.code == 3, (levels, dev)
