@@ -141,10 +141,14 @@ template <class T> void launchUpdateP(const T* z, T* p, int64_t n, CgState* st, 
 void launchStresses(const DevLevel& L, const double* S, const double* u, int principals, float* vm, float* p1, float* p3, hipStream_t s);
 void launchStressTensors(const DevLevel& L, const double* S, const double* u, float* out, hipStream_t s);
 // mixed with FP32 recurrences: x32 += alpha p, r32 -= alpha q (fold decision); at a fold u += rmax x32
-void launchUpdateXR32Fold(float* x, const float* p, float* r, const float* q, int64_t n, CgState* st, Reduce red, hipStream_t s, int ninst = 1);
+// z0 (may be null): also z0 = c0 D^-1 r from the new r (invD32: the level's FP32 inverse diagonal),
+// the next V-cycle's first Chebyshev step; launchFoldR rewrites it when it replaces r32
+void launchUpdateXR32Fold(float* x, const float* p, float* r, const float* q, int64_t n, CgState* st, Reduce red, hipStream_t s, int ninst = 1,
+                          float* z0 = nullptr, const float* invD32 = nullptr, double c0 = 0);
 void launchFoldX(double* u, float* x, int64_t n, CgState* st, hipStream_t s, int ninst = 1);
 // fold, part 2 (after Kx = K u in FP64): r64 = f - Kx, r32 = r64 / rmax, true residual, stop test
-void launchFoldR(const double* f, const double* Kx, const double* invD, double* r64, float* r32, int64_t n, CgState* st, Reduce red, hipStream_t s, int ninst = 1);
+void launchFoldR(const double* f, const double* Kx, const double* invD, double* r64, float* r32, int64_t n, CgState* st, Reduce red, hipStream_t s, int ninst = 1,
+                 float* z0 = nullptr, const float* invD32 = nullptr, double c0 = 0);
 
 // r = f - Kx on free DOFs (Kx already masked), out[0] += |r|^2 over the grid (FP64)
 void launchResidual64(const double* f, const double* Kx, const double* invD, double* r, int64_t n, double* out, Reduce red, hipStream_t s, int ninst = 1);

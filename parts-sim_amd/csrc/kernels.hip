@@ -501,9 +501,11 @@ __global__ __launch_bounds__(BLOCK) void kDotPQ(const T* __restrict__ p, const T
     blockPartials<1>(v, red);
 }
 
+// z0 (when not null): the next V-cycle's first Chebyshev step z0 = c0 D^-1 r from the new r, as
+// kChebFirst computes it (the V-cycle then starts from it instead of reading r again)
 template <class T>
 __global__ __launch_bounds__(BLOCK) void kUpdateXR(T* __restrict__ x, const T* __restrict__ p, T* __restrict__ r, const T* __restrict__ q, int64_t n,
-                                                   CgState* st, Reduce red) {
+                                                   CgState* st, Reduce red, T* __restrict__ z0, const T* __restrict__ invD, T c0) {
     st += blockIdx.y;
     if (st->stop) return;
     {
@@ -512,6 +514,7 @@ __global__ __launch_bounds__(BLOCK) void kUpdateXR(T* __restrict__ x, const T* _
         p += io;
         r += io;
         q += io;
+        if (z0) z0 += io;
     }
     const T a = T(st->alpha);
     double v[1] = {0};
@@ -519,6 +522,7 @@ __global__ __launch_bounds__(BLOCK) void kUpdateXR(T* __restrict__ x, const T* _
         x[i] += a * p[i];
         const T ri = r[i] - a * q[i];
         r[i] = ri;
+        if (z0) z0[i] = c0 * invD[i] * ri;
         v[0] += double(ri) * double(ri);)
     blockPartials<1>(v, red);
 }
@@ -570,14 +574,17 @@ __global__ __launch_bounds__(BLOCK) void kFoldX(double* __restrict__ u, float* _
     }
 }
 
+// z0 (when not null): the first Chebyshev step from the new r32, as kUpdateXR writes it
 __global__ __launch_bounds__(BLOCK) void kFoldR(const double* __restrict__ f, const double* __restrict__ Kx, const double* __restrict__ invD,
-                                                double* __restrict__ r64, float* __restrict__ r32, int64_t n, CgState* st, Reduce red) {
+                                                double* __restrict__ r64, float* __restrict__ r32, int64_t n, CgState* st, Reduce red,
+                                                float* __restrict__ z0, const float* __restrict__ invD32, float c0) {
     st += blockIdx.y;
     if (st->noFold) return;
     f += instOffset(n);
     Kx += instOffset(n);
     r64 += instOffset(n);
     r32 += instOffset(n);
+    if (z0) z0 += instOffset(n);
     const double inv = 1.0 / st->rmax;
     double v[1] = {0};
     // tiles of RED_TILE elements, block b taking tiles b, b + gridDim.x, ...
@@ -588,7 +595,9 @@ __global__ __launch_bounds__(BLOCK) void kFoldR(const double* __restrict__ f, co
             if (i < n) {
                 const double ri = invD[i] != 0 ? f[i] - Kx[i] : 0.0;
                 r64[i] = ri;
-                r32[i] = float(ri * inv);
+                const float rf = float(ri * inv);
+                r32[i] = rf;
+                if (z0) z0[i] = c0 * invD32[i] * rf;
                 v[0] += ri * ri;
             }
         }
@@ -842,7 +851,7 @@ template <class T> void launchDotPQ(const T* p, const T* q, int64_t n, CgState* 
 }
 template <class T> void launchUpdateXR(T* x, const T* p, T* r, const T* q, int64_t n, CgState* st, Reduce red, hipStream_t s, int ninst) {
     const unsigned g = reduceGrid(n);
-    kUpdateXR<T><<<dim3(g, ninst), BLOCK, 0, s>>>(x, p, r, q, n, st, red);
+    kUpdateXR<T><<<dim3(g, ninst), BLOCK, 0, s>>>(x, p, r, q, n, st, red, nullptr, nullptr, T(0));
     kFinish<1, FinXR><<<ninst, FIN_BLOCK, 0, s>>>(red, g, &st->stop, FinXR{st});
 }
 template <class T> void launchDotsRZ(const T* r, const T* z, const T* q, int64_t n, CgState* st, Reduce red, hipStream_t s, int ninst) {
@@ -858,9 +867,10 @@ template <class T> void launchUpdateP(const T* z, T* p, int64_t n, CgState* st, 
 void launchFinishPQ(Reduce red, unsigned nb, CgState* st, hipStream_t s, int ninst) { kFinish<1, FinPQ><<<ninst, FIN_BLOCK, 0, s>>>(red, nb, &st->stop, FinPQ{st}); }
 void launchFinishRZ(Reduce red, unsigned nb, CgState* st, hipStream_t s, int ninst) { kFinish<2, FinRZ><<<ninst, FIN_BLOCK, 0, s>>>(red, nb, &st->stop, FinRZ{st}); }
 
-void launchUpdateXR32Fold(float* x, const float* p, float* r, const float* q, int64_t n, CgState* st, Reduce red, hipStream_t s, int ninst) {
+void launchUpdateXR32Fold(float* x, const float* p, float* r, const float* q, int64_t n, CgState* st, Reduce red, hipStream_t s, int ninst, float* z0,
+                          const float* invD32, double c0) {
     const unsigned g = reduceGrid(n);
-    kUpdateXR<float><<<dim3(g, ninst), BLOCK, 0, s>>>(x, p, r, q, n, st, red);
+    kUpdateXR<float><<<dim3(g, ninst), BLOCK, 0, s>>>(x, p, r, q, n, st, red, z0, invD32, float(c0));
     kFinish<1, FinXRFold><<<ninst, FIN_BLOCK, 0, s>>>(red, g, &st->stop, FinXRFold{st});
 }
 void launchFoldX(double* u, float* x, int64_t n, CgState* st, hipStream_t s, int ninst) {
@@ -868,9 +878,9 @@ void launchFoldX(double* u, float* x, int64_t n, CgState* st, hipStream_t s, int
 }
 
 void launchFoldR(const double* f, const double* Kx, const double* invD, double* r64, float* r32, int64_t n, CgState* st, Reduce red, hipStream_t s,
-                 int ninst) {
+                 int ninst, float* z0, const float* invD32, double c0) {
     const unsigned g = std::min(reduceGrid(n), FOLD_GRID);
-    kFoldR<<<dim3(g, ninst), BLOCK, 0, s>>>(f, Kx, invD, r64, r32, n, st, red);
+    kFoldR<<<dim3(g, ninst), BLOCK, 0, s>>>(f, Kx, invD, r64, r32, n, st, red, z0, invD32, float(c0));
     kFinish<1, FinFoldR><<<ninst, FIN_BLOCK, 0, s>>>(red, g, &st->noFold, FinFoldR{st});
 }
 

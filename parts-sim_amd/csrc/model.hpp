@@ -442,7 +442,10 @@ public:
 
     // one V-cycle from rIn on level l (voxel_fea.cpp:817-828); returns the buffer holding z
     // rz: the conjugate gradients' r.z and z.q (q = the previous K p), fused into the last step
-    template <class T> T* vcycle(size_t l, const T* rIn, const int* stop, int form = 0, int degree = 2, const DotFusion* rz = nullptr) {
+    // firstReady: buffer a of level l already holds the first Chebyshev step c0 D^-1 rIn (the
+    // conjugate gradients' residual update writes it), so the cycle starts from it
+    template <class T>
+    T* vcycle(size_t l, const T* rIn, const int* stop, int form = 0, int degree = 2, const DotFusion* rz = nullptr, bool firstReady = false) {
         Vecs<T> V = vecs<T>(l);
         const DevLevel& D = lv[l].dl;
         if (l + 1 == lv.size()) {
@@ -467,7 +470,7 @@ public:
             zp = z;
             z = y;
         };
-        launchChebFirst<T>(D, rIn, z, c.c0, stop, stream, ni);
+        if (!firstReady) launchChebFirst<T>(D, rIn, z, c.c0, stop, stream, ni);
         const int pre = preDegree > 0 ? std::min(preDegree, degree) : degree;
         for (int j = 1; j < pre; j++) step(AM_CHEB2, c.c1[j], c.c2[j]);
         zp = nullptr;
@@ -574,6 +577,9 @@ public:
             int& p;
             ~ResetPre() { p = 0; }
         } resetPre{preDegree};
+        // mixed: the residual updates also write the next V-cycle's first Chebyshev step on level 0
+        const bool fuseFirst = prec == PSX_PREC_MIXED && lv.size() > 1;
+        const double c0first = fuseFirst ? chebyshev(H.levels[0].lmax, degree).c0 : 0.0;
         if (prec == PSX_PREC_MIXED) {
             // FP32 units for the whole solve: each first residual scaled to order one
             launchMaxAbs64(r, n, &st->rmax, red, stream, k);
@@ -607,6 +613,8 @@ public:
             HIPCHK(hipMemsetAsync(q32, 0, nk * 4, stream));
             HIPCHK(hipMemsetAsync(x32, 0, nk * 4, stream));
             launchToF32(r, r32, n, &st->rmax, stream, k);
+            // the first V-cycle's first Chebyshev step; later ones come from the residual updates
+            if (fuseFirst) launchChebFirst<float>(D0, r32, lv[0].a32, c0first, stop, stream, k);
         } else {
             HIPCHK(hipMemsetAsync(p, 0, nk * 8, stream));
             HIPCHK(hipMemsetAsync(q, 0, nk * 8, stream));
@@ -618,16 +626,16 @@ public:
             for (int b = 0; b < batch; b++) {
                 if (prec == PSX_PREC_MIXED) {
                     const DotFusion fz{FD_RZ, q32, red}, fp{FD_PQ, nullptr, red};
-                    const float* z = vcycle<float>(0, r32, stop, form, degree, &fz);
+                    const float* z = vcycle<float>(0, r32, stop, form, degree, &fz, fuseFirst);
                     launchUpdateP<float>(z, p32, n, st, stream, k);
                     // the outer operator stays FP32 (the matrix-core form is for the smoother only)
                     op<float>(0, AM_APPLY, p32, q32, nullptr, nullptr, 0, 0, stop, form & ~OF_MFMA, &fp);
-                    launchUpdateXR32Fold(x32, p32, r32, q32, n, st, red, stream, k);
+                    launchUpdateXR32Fold(x32, p32, r32, q32, n, st, red, stream, k, fuseFirst ? lv[0].a32 : nullptr, D0.invD32, c0first);
                     // reliable update: the correction joins the FP64 solution and the FP64 true
                     // residual replaces the recurrence's
                     launchFoldX(du, x32, n, st, stream, k);
                     op<double>(0, AM_APPLY, du, q, nullptr, nullptr, 0, 0, &st->noFold);
-                    launchFoldR(df, q, D0.invD64, r, r32, n, st, red, stream, k);
+                    launchFoldR(df, q, D0.invD64, r, r32, n, st, red, stream, k, fuseFirst ? lv[0].a32 : nullptr, D0.invD32, c0first);
                     continue;
                 }
                 const double* z;

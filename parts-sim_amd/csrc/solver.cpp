@@ -142,6 +142,7 @@ int psx_set_lmax(psx_model m, int level, double lmax) {
     return guarded([&] {
         if (!m || level < 0 || level >= int(m->m->H.levels.size()) || !(lmax > 0)) throw std::invalid_argument("bad level or lmax");
         m->m->H.levels[level].lmax = lmax;
+        m->m->dropGraph();  // a captured iteration batch holds the old Chebyshev coefficients
         return PSX_OK;
     });
 }
