@@ -36,3 +36,21 @@ def test_graph_follows_grown_buffers(P, W):
     eager = m.solve(F[0], tol=1e-8, precision="mixed", eager=True)
     assert np.array_equal(one["u"], eager["u"]) and np.array_equal(again["u"], eager["u"]) and np.array_equal(U[0], eager["u"])
     m.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_graph_follows_new_smoother_estimates(P, W, precision):
+    # a captured batch holds the Chebyshev coefficients of its levels: after psx_set_lmax the next
+    # solve must use the new ones, exactly as an eager solve does
+    w = W.void_block((40, 16, 12), seed=22)
+    m = P.Model(w["dims"], w["density"], w["nu"], w["bc"], 300, device_build=True)
+    f = w["f"]
+    before = m.solve(f, tol=1e-9, precision=precision)
+    lm = m.estimate_smoothers()
+    m.set_lmax(0, 1.3 * lm[0])
+    after = m.solve(f, tol=1e-9, precision=precision)
+    eager = m.solve(f, tol=1e-9, precision=precision, eager=True)
+    print(precision, before["iterations"], after["iterations"], eager["iterations"], flush=True)
+    assert after["iterations"] == eager["iterations"] and np.array_equal(after["u"], eager["u"])
+    assert not np.array_equal(after["u"], before["u"])
+    m.close()
