@@ -422,30 +422,33 @@ __global__ __launch_bounds__(BLOCK) void kProlong(DevLevel F, DevLevel C, const 
     const uint32_t info = F.nodeInfo[fn];
     const bool f0 = !(info & 0x100u), f1 = !(info & 0x200u), f2 = !(info & 0x400u);
     if (!(f0 || f1 || f2)) return;
+    // a node's three components in one load (dwordx3 / 3 x dwordx2)
+    struct alignas(sizeof(T)) V3 {
+        T a, b, c;
+    };
+    // the fine node itself, loaded ahead of the gathers (held components are written back as read)
+    V3* __restrict__ dst = reinterpret_cast<V3*>(zf + 3 * fn);
+    V3 v = *dst;
     const unsigned u = unsigned(fn), FNX = unsigned(F.NX), FNY = unsigned(F.NY);
     const int i = int(u % FNX), j = int((u / FNX) % FNY), k = int(u / (FNX * FNY));
     const int ni = (i & 1) ? 2 : 1, nj = (j & 1) ? 2 : 1, nk = (k & 1) ? 2 : 1;
     const T w = T(1.0 / (ni * nj * nk));
+    // the lowest coarse corner once in 64 bits, the other corners at 32-bit row / plane strides in
+    // nodes (a plane holds < 2^31 / 3 nodes)
+    const V3* __restrict__ base =
+        reinterpret_cast<const V3*>(zc) + ((i >> 1) + int64_t(C.NX) * ((j >> 1) + int64_t(C.NY) * (k >> 1)));
+    const int32_t oy = int32_t(C.NX), oz = int32_t(C.NX) * int32_t(C.NY);
     T s0 = 0, s1 = 0, s2 = 0;
     for (int c = 0; c < nk; c++) {
-        const int K = (k >> 1) + c;
         for (int b = 0; b < nj; b++) {
-            const int J = (j >> 1) + b;
             for (int a = 0; a < ni; a++) {
-                const int I = (i >> 1) + a;
-                const T* p = zc + 3 * (I + int64_t(C.NX) * (J + int64_t(C.NY) * K));
-                s0 += w * p[0];
-                s1 += w * p[1];
-                s2 += w * p[2];
+                const V3 q = base[a + b * oy + c * oz];
+                s0 += w * q.a;
+                s1 += w * q.b;
+                s2 += w * q.c;
             }
         }
     }
-    // the node's three components in one load and one store (held components written back as read)
-    struct alignas(sizeof(T)) V3 {
-        T a, b, c;
-    };
-    V3* __restrict__ dst = reinterpret_cast<V3*>(zf + 3 * fn);
-    V3 v = *dst;
     v.a = f0 ? v.a + s0 : v.a;
     v.b = f1 ? v.b + s1 : v.b;
     v.c = f2 ? v.c + s2 : v.c;
