@@ -378,26 +378,31 @@ __global__ __launch_bounds__(BLOCK) void kRestrict(DevLevel F, DevLevel C, const
         struct alignas(sizeof(T)) V3 {
             T a, b, c;
         };
-        const int64_t FNX = F.NX, FNXY = int64_t(F.NX) * F.NY;
+        // a 64-bit base per fine plane, 32-bit offsets in nodes within it (a plane holds < 2^31 / 3
+        // nodes)
+        const int64_t FNXY = int64_t(F.NX) * F.NY;
+        const V3* __restrict__ rv = reinterpret_cast<const V3*>(rf);
 #pragma unroll
         for (int dk = -1; dk <= 1; dk++) {
             const int kf = 2 * K + dk;
             const bool kin = kf >= 0 && kf < F.NZ;
             const int kc = min(max(kf, 0), F.NZ - 1);
             const T wk = dk == 0 ? T(1) : T(0.5);
+            const V3* __restrict__ plane = rv + FNXY * kc;
 #pragma unroll
             for (int dj = -1; dj <= 1; dj++) {
                 const int jf = 2 * J + dj;
                 const bool jin = kin && jf >= 0 && jf < F.NY;
                 const int jc = min(max(jf, 0), F.NY - 1);
                 const T wj = dj == 0 ? wk : T(0.5) * wk;
+                const int32_t row = int32_t(F.NX) * jc;
 #pragma unroll
                 for (int di = -1; di <= 1; di++) {
                     const int fi = 2 * I + di;
                     const bool in = jin && fi >= 0 && fi < F.NX;
                     const int ic = min(max(fi, 0), F.NX - 1);
                     const T w = in ? (di == 0 ? wj : T(0.5) * wj) : T(0);
-                    const V3 v = *reinterpret_cast<const V3*>(rf + 3 * (ic + FNX * jc + FNXY * kc));
+                    const V3 v = plane[row + ic];
                     s0 += w * v.a;
                     s1 += w * v.b;
                     s2 += w * v.c;
