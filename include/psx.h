@@ -277,6 +277,14 @@ int psx_break_test(const psx_model_desc* part, const double* f, const psx_break_
 /* pruneFloating (voxel_fea.cpp:1086-1119) on the device: d_density, d_out nx*ny*nz floats in HBM,
  * d_held one byte per node; removed receives the count of solid voxels set to 0. */
 int psx_prune_floating_device(int nx, int ny, int nz, const float* d_density, const uint8_t* d_held, float* d_out, int* removed);
+/* The same prune without allocating: no counterpart in the reference. It is for callers that cannot
+ * allocate device or pinned memory (code recording a stream into a graph, tight serving loops,
+ * harnesses that forbid it). Its search scratch is a static device array, one copy per GPU held until
+ * the process ends (about 34 MB), so parts of at most 8388608 (2^23) voxels are accepted and larger
+ * ones fail with PSX_ERR_ARGUMENT. Work runs on the null stream and the call returns when it is done.
+ * Calls that overlap on one GPU share its scratch and race. The result equals
+ * psx_prune_floating_device's bit for bit. */
+int psx_prune_floating_device_noalloc(int nx, int ny, int nz, const float* d_density, const uint8_t* d_held, float* d_out, int* removed);
 
 /* Support reactions: sum over held active DOFs of (K u)_raw - f (VoxelFEA::reactions,
  * voxel_fea.cpp:929-940). f may be NULL. */

@@ -168,6 +168,7 @@ SIGNATURES = {
     "psx_topology_filter": (_i, [_vp, _vp, _d, _i, _vp, _vp]),
     "psx_break_test": (_i, [C.POINTER(ModelDesc), _vp, C.POINTER(BreakOpts), C.POINTER(BreakResult)]),
     "psx_prune_floating_device": (_i, [_i, _i, _i, _vp, _vp, _vp, C.POINTER(_i)]),
+    "psx_prune_floating_device_noalloc": (_i, [_i, _i, _i, _vp, _vp, _vp, C.POINTER(_i)]),
     "psx_hex_element": (_i, [_d, _vp, _vp]),
     "psx_hierarchy_create": (_i, [C.POINTER(ModelDesc), C.POINTER(_vp)]),
     "psx_hierarchy_destroy": (None, [_vp]),
@@ -825,8 +826,9 @@ class Server:
         self.close()
 
 
-def prune_floating_device(dims, density, held):
-    """pruneFloating on the device (psx_prune_floating_device) from host arrays: (density, removed)."""
+def prune_floating_device(dims, density, held, no_alloc=False):
+    """pruneFloating on the device (psx_prune_floating_device) from host arrays: (density, removed).
+    no_alloc: the allocation-free entry point (psx_prune_floating_device_noalloc, static scratch)."""
     nx, ny, nz = (int(d) for d in dims)
     density = np.ascontiguousarray(density, np.float32).reshape(-1)
     held = np.ascontiguousarray(held, np.uint8).reshape(-1)
@@ -843,7 +845,8 @@ def prune_floating_device(dims, density, held):
         if hip.hipMemcpy(bufs[0], C.c_void_p(density.ctypes.data), sizes[0], 1) or hip.hipMemcpy(bufs[1], C.c_void_p(held.ctypes.data), sizes[1], 1):
             raise PsxError(ERR_DEVICE, "hipMemcpy H2D failed")
         removed = C.c_int()
-        _check(lib().psx_prune_floating_device(nx, ny, nz, bufs[0], bufs[1], bufs[2], C.byref(removed)))
+        fn = lib().psx_prune_floating_device_noalloc if no_alloc else lib().psx_prune_floating_device
+        _check(fn(nx, ny, nz, bufs[0], bufs[1], bufs[2], C.byref(removed)))
         if hip.hipMemcpy(C.c_void_p(out.ctypes.data), bufs[2], sizes[2], 2) != 0:
             raise PsxError(ERR_DEVICE, "hipMemcpy D2H failed")
     finally:

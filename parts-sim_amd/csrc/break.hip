@@ -14,7 +14,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <numeric>
+#include <string>
 #include <stdexcept>
 #include <vector>
 
@@ -52,14 +54,19 @@ __global__ __launch_bounds__(BLOCK) void kPruneSeed(Grid3 g, const float* __rest
     queue[atomicAdd(&cursor[0], 1)] = int32_t(e);
 }
 
+// the level-end marks form a ring: level l reads marks[l], marks[l + 1] and its mark kernel writes
+// marks[l + 2], so a ring of 64 covers any batch of levels queued between two host checks
+constexpr int MARK_RING = 64;
+static_assert(MARK_RING >= 2 * (BFS_BATCH + 2), "the ring must hold a batch of levels");
+
 // marks[level + 1] = the queue's end
 __global__ void kPruneMark(int* __restrict__ marks, int level, const int* __restrict__ cursor) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) marks[level + 1] = cursor[0];
+    if (threadIdx.x == 0 && blockIdx.x == 0) marks[(level + 1) % MARK_RING] = cursor[0];
 }
 
 __global__ __launch_bounds__(BLOCK) void kPruneExpand(Grid3 g, const float* __restrict__ dens, uint32_t* __restrict__ bits, int32_t* __restrict__ queue,
                                                       int* __restrict__ cursor, const int* __restrict__ marks, int level) {
-    const int lo = marks[level], hi = marks[level + 1];
+    const int lo = marks[level % MARK_RING], hi = marks[(level + 1) % MARK_RING];
     const int64_t sxy = int64_t(g.nx) * g.ny;
     for (int64_t t = lo + int64_t(blockIdx.x) * BLOCK + threadIdx.x; t < hi; t += int64_t(gridDim.x) * BLOCK) {
         const int64_t e = queue[t];
@@ -93,63 +100,81 @@ __global__ __launch_bounds__(BLOCK) void kPruneFinish(int64_t nV, const float* _
     if (solid && reached) atomicAdd(&cursor[2], 1);
 }
 
-// the prune's scratch: bit per voxel, the queue, the level marks and the counters
+__global__ __launch_bounds__(BLOCK) void kZeroWords(uint32_t* __restrict__ p, int64_t n) {
+    for (int64_t t = int64_t(blockIdx.x) * BLOCK + threadIdx.x; t < n; t += int64_t(gridDim.x) * BLOCK) p[t] = 0u;
+}
+
+// where the prune's scratch lives: a bit per voxel, the queue (one entry per voxel), the ring of
+// level marks and the counters
+struct PruneScratch {
+    uint32_t* bits = nullptr;
+    int32_t* queue = nullptr;
+    int* marks = nullptr;   // MARK_RING entries
+    int* cursor = nullptr;  // [0] queue end, [1] removed, [2] kept, [3] lost entries
+};
+
+// the breadth-first search on scratch S; host receives the cursor and two marks (4 ints, pinned or
+// not). out = the pruned densities; returns the count removed; kept receives the solid voxels left;
+// lostHost (with lost, nV entries of device scratch) the removed voxels in increasing order
+int runPrune(hipStream_t s, const PruneScratch& S, int64_t nV, int* host, Grid3 g, const float* dens, const uint8_t* held, float* out, int* kept,
+             int32_t* lost = nullptr, std::vector<int32_t>* lostHost = nullptr) {
+    // cleared by kernels (no memset nodes, for callers that record the stream)
+    kZeroWords<<<unsigned(std::min<int64_t>(2048, (nV / 32 + BLOCK) / BLOCK + 1)), BLOCK, 0, s>>>(S.bits, (nV + 31) / 32);
+    kZeroWords<<<1, BLOCK, 0, s>>>(reinterpret_cast<uint32_t*>(S.cursor), 4);
+    kZeroWords<<<1, BLOCK, 0, s>>>(reinterpret_cast<uint32_t*>(S.marks), MARK_RING);
+    kPruneSeed<<<grid(nV), BLOCK, 0, s>>>(g, dens, held, S.bits, S.queue, S.cursor);
+    kPruneMark<<<1, 64, 0, s>>>(S.marks, 0, S.cursor);
+    // level l visits queue[marks[l], marks[l + 1]); every voxel enters the queue once, so at most
+    // nV levels are non-empty
+    int64_t level = 0;
+    for (;;) {
+        for (int b = 0; b < BFS_BATCH && level < nV; b++, level++) {
+            kPruneExpand<<<BFS_BLOCKS, BLOCK, 0, s>>>(g, dens, S.bits, S.queue, S.cursor, S.marks, int(level));
+            kPruneMark<<<1, 64, 0, s>>>(S.marks, int(level + 1), S.cursor);
+        }
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(host, S.marks + level % MARK_RING, 4, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(host + 1, S.marks + (level + 1) % MARK_RING, 4, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        if (host[0] == host[1] || level >= nV) break;  // the last level added nothing
+    }
+    kPruneFinish<<<grid(nV), BLOCK, 0, s>>>(nV, dens, S.bits, out, S.cursor, lost);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(host, S.cursor, 16, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (kept) *kept = host[2];
+    if (lost && lostHost) {
+        lostHost->resize(size_t(host[3]));
+        if (host[3] > 0) HIPCHK(hipMemcpyAsync(lostHost->data(), lost, size_t(host[3]) * 4, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        std::sort(lostHost->begin(), lostHost->end());
+    }
+    return host[1];
+}
+
+// the prune's scratch in stream-ordered allocations of the caller's stream
 struct Pruner {
     hipStream_t s;
     int64_t nV;
-    uint32_t* bits = nullptr;
-    int32_t* queue = nullptr;
-    int* marks = nullptr;
-    int* cursor = nullptr;  // [0] queue end, [1] removed, [2] kept, [3] lost entries
-    int* host = nullptr;    // pinned copies of the cursor and two marks
+    PruneScratch S;
+    int* host = nullptr;  // pinned copies of the cursor and two marks
     Pruner(hipStream_t st, int64_t n) : s(st), nV(n) {
-        HIPCHK(hipMallocAsync(reinterpret_cast<void**>(&bits), size_t((n + 31) / 32) * 4, s));
-        HIPCHK(hipMallocAsync(reinterpret_cast<void**>(&queue), size_t(n) * 4, s));
-        HIPCHK(hipMallocAsync(reinterpret_cast<void**>(&marks), size_t(n + 2) * 4, s));
-        HIPCHK(hipMallocAsync(reinterpret_cast<void**>(&cursor), 16, s));
+        HIPCHK(hipMallocAsync(reinterpret_cast<void**>(&S.bits), size_t((n + 31) / 32) * 4, s));
+        HIPCHK(hipMallocAsync(reinterpret_cast<void**>(&S.queue), size_t(n) * 4, s));
+        HIPCHK(hipMallocAsync(reinterpret_cast<void**>(&S.marks), size_t(MARK_RING) * 4, s));
+        HIPCHK(hipMallocAsync(reinterpret_cast<void**>(&S.cursor), 16, s));
         HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&host), 32, hipHostMallocDefault));
     }
     ~Pruner() {
-        if (bits) (void)hipFreeAsync(bits, s);
-        if (queue) (void)hipFreeAsync(queue, s);
-        if (marks) (void)hipFreeAsync(marks, s);
-        if (cursor) (void)hipFreeAsync(cursor, s);
+        if (S.bits) (void)hipFreeAsync(S.bits, s);
+        if (S.queue) (void)hipFreeAsync(S.queue, s);
+        if (S.marks) (void)hipFreeAsync(S.marks, s);
+        if (S.cursor) (void)hipFreeAsync(S.cursor, s);
         (void)hipStreamSynchronize(s);
         if (host) (void)hipHostFree(host);
     }
-    // out = pruned dens; returns the count removed; kept receives the solid voxels left; lostHost
-    // (with lost, nV entries of device scratch) the removed voxels in increasing order
     int run(Grid3 g, const float* dens, const uint8_t* held, float* out, int* kept, int32_t* lost = nullptr, std::vector<int32_t>* lostHost = nullptr) {
-        HIPCHK(hipMemsetAsync(bits, 0, size_t((nV + 31) / 32) * 4, s));
-        HIPCHK(hipMemsetAsync(cursor, 0, 16, s));
-        HIPCHK(hipMemsetAsync(marks, 0, 4, s));
-        kPruneSeed<<<grid(nV), BLOCK, 0, s>>>(g, dens, held, bits, queue, cursor);
-        kPruneMark<<<1, 64, 0, s>>>(marks, 0, cursor);
-        // level l visits queue[marks[l], marks[l + 1]); every voxel enters the queue once, so at most
-        // nV levels are non-empty
-        int level = 0;
-        for (;;) {
-            for (int b = 0; b < BFS_BATCH && level < nV; b++, level++) {
-                kPruneExpand<<<BFS_BLOCKS, BLOCK, 0, s>>>(g, dens, bits, queue, cursor, marks, level);
-                kPruneMark<<<1, 64, 0, s>>>(marks, level + 1, cursor);
-            }
-            HIPCHK(hipGetLastError());
-            HIPCHK(hipMemcpyAsync(host, marks + level, 8, hipMemcpyDeviceToHost, s));
-            HIPCHK(hipStreamSynchronize(s));
-            if (host[0] == host[1] || level >= nV) break;  // the last level added nothing
-        }
-        kPruneFinish<<<grid(nV), BLOCK, 0, s>>>(nV, dens, bits, out, cursor, lost);
-        HIPCHK(hipGetLastError());
-        HIPCHK(hipMemcpyAsync(host, cursor, 16, hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
-        if (kept) *kept = host[2];
-        if (lost && lostHost) {
-            lostHost->resize(size_t(host[3]));
-            if (host[3] > 0) HIPCHK(hipMemcpyAsync(lostHost->data(), lost, size_t(host[3]) * 4, hipMemcpyDeviceToHost, s));
-            HIPCHK(hipStreamSynchronize(s));
-            std::sort(lostHost->begin(), lostHost->end());
-        }
-        return host[1];
+        return runPrune(s, S, nV, host, g, dens, held, out, kept, lost, lostHost);
     }
 };
 
@@ -260,9 +285,64 @@ double lostLoadFraction(const Level& L, const double* f) {
 
 }  // namespace
 
+// Scratch of the allocation-free prune (psx_prune_floating_device_noalloc), one copy per GPU that runs
+// it, held until the process ends: 64 words of counters, the ring of level marks (64 words), then the
+// visited bits and the queue of a part of at most PRUNE_NOALLOC_MAX_VOXELS voxels. Every call on a
+// GPU uses that GPU's copy, so calls that overlap on one GPU race.
+constexpr int64_t PRUNE_NOALLOC_MAX_VOXELS = int64_t(1) << 23;
+constexpr int64_t NOALLOC_MARKS = 64, NOALLOC_BITS = 128, NOALLOC_QUEUE = NOALLOC_BITS + PRUNE_NOALLOC_MAX_VOXELS / 32;
+constexpr int64_t NOALLOC_WORDS = NOALLOC_QUEUE + PRUNE_NOALLOC_MAX_VOXELS;
+static_assert(NOALLOC_MARKS >= MARK_RING && NOALLOC_BITS - NOALLOC_MARKS >= MARK_RING, "ring and counters fit their slots");
+static __device__ __align__(256) uint32_t scratch[NOALLOC_WORDS];
+
+namespace {
+
+constexpr int NOALLOC_DEVICES = 64;
+std::mutex g_scratchMu;
+uint32_t* g_scratchPtr[NOALLOC_DEVICES] = {};
+
+// the device address of the current GPU's scratch copy, looked up on the first call on that GPU (the
+// array's name in host code is a host-side stand-in, never a device address)
+uint32_t* scratchOnCurrentDevice() {
+    int dev = -1;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NOALLOC_DEVICES) throw std::invalid_argument("device index beyond the allocation-free prune's table");
+    std::lock_guard<std::mutex> lk(g_scratchMu);
+    if (!g_scratchPtr[dev]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) throw DeviceError(std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+        g_scratchPtr[dev] = static_cast<uint32_t*>(ptr);
+    }
+    return g_scratchPtr[dev];
+}
+
+}  // namespace
+
 }  // namespace psx
 
 extern "C" {
+
+int psx_prune_floating_device_noalloc(int nx, int ny, int nz, const float* d_density, const uint8_t* d_held, float* d_out, int* removed) {
+    using namespace psx;
+    return abi::guarded([&] {
+        if (!d_density || !d_held || !d_out || nx < 1 || ny < 1 || nz < 1) throw std::invalid_argument("bad arguments");
+        const int64_t nV = int64_t(nx) * ny * nz;
+        if (nV > PRUNE_NOALLOC_MAX_VOXELS)
+            throw std::invalid_argument("grid of " + std::to_string(nV) + " voxels exceeds the allocation-free prune's limit of " +
+                                        std::to_string(PRUNE_NOALLOC_MAX_VOXELS) + " voxels");
+        uint32_t* base = scratchOnCurrentDevice();
+        PruneScratch S;
+        S.cursor = reinterpret_cast<int*>(base);
+        S.marks = reinterpret_cast<int*>(base + NOALLOC_MARKS);
+        S.bits = base + NOALLOC_BITS;
+        S.queue = reinterpret_cast<int32_t*>(base + NOALLOC_QUEUE);
+        int host[4] = {0, 0, 0, 0};  // read back through pageable copies: no pinned allocation
+        const int r = runPrune(nullptr, S, nV, host, Grid3{nx, ny, nz}, d_density, d_held, d_out, nullptr);
+        if (removed) *removed = r;
+        return PSX_OK;
+    });
+}
 
 int psx_prune_floating_device(int nx, int ny, int nz, const float* d_density, const uint8_t* d_held, float* d_out, int* removed) {
     using namespace psx;
