@@ -62,6 +62,8 @@ def lib(kind):
         "element_energies": (None, [vp, _f64p, _f64p]),
         "optimize_topology": (C.c_int, [vp, _f32p, _u8p, _u8p, _f64p, dbl, dbl, dbl, dbl, dbl, dbl, C.c_int, i64, _f32p, _f64p]),
         "voxelize": (C.c_int, [_f32p, i64, vp, i64, _f64p, dbl, C.POINTER(C.c_int), C.c_int, _f32p]),
+        "break_test": (C.c_int, [vp, _f32p, dbl, _u8p, _f64p, dbl, dbl, dbl, C.c_int, C.c_int, dbl, C.c_int, i64, _f64p, vp, vp,
+                                 C.c_int, vp, C.c_int, vp, C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     }
     if kind == "ref":
         sig["timed_solve"] = (dbl, [C.c_int, C.c_int, C.c_int, _f32p, dbl, _u8p, i64, _f64p, dbl, C.c_int, _f64p,
@@ -71,8 +73,6 @@ def lib(kind):
                                          C.POINTER(C.c_int), C.POINTER(C.c_double), _f64p])
     else:
         sig["topo_filter"] = (None, [vp, _f32p, dbl, C.c_int, _f64p, _f64p])
-        sig["break_test"] = (C.c_int, [vp, _f32p, dbl, _u8p, _f64p, dbl, dbl, dbl, C.c_int, C.c_int, dbl, C.c_int, i64, _f64p, vp, vp,
-                                       C.c_int, vp, C.c_int, vp, C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_int)])
         sig["set_threads"] = (None, [C.c_int])
         sig["set_cheb_degree"] = (None, [vp, C.c_int])
         sig["smooth"] = (None, [vp, C.c_int, _f32p, _f32p])
@@ -255,9 +255,10 @@ BREAK_REASONS = ("step limit", "separated", "unsupported load", "solver did not 
 
 
 def break_test(dims, density, nu, bc, f, E, h, strength, principal=False, max_steps=5, tol=1e-6, max_iter=400, coarsest_max_dof=1100,
-               fields=False):
+               fields=False, kind="orc"):
     """breakTest (static_study.cpp:197-279) with the VoxelFEA solve: dict(reason, peak, removed, steps), each
-    step dict(lambda, max_disp, voxels, cracked, detached[, vm])."""
+    step dict(lambda, max_disp, voxels, cracked, detached[, vm]). kind "ref" runs the reference's own
+    breakTest without a GPU, which fixes tol 1e-6, max_iter 400 and coarsest_max_dof 1100."""
     d = np.ascontiguousarray([int(v) for v in dims], np.int32)
     nV = int(d[0]) * int(d[1]) * int(d[2])
     nN = (int(d[0]) + 1) * (int(d[1]) + 1) * (int(d[2]) + 1)
@@ -267,13 +268,13 @@ def break_test(dims, density, nu, bc, f, E, h, strength, principal=False, max_st
     detached = np.zeros(max(1, max_steps) * nV, np.int32)
     vm = np.zeros(max(1, max_steps) * nN, np.float32) if fields else None
     peak, removed, reason = C.c_double(), C.c_int(), C.c_int()
-    n = lib("orc").orc_break_test(d.ctypes.data_as(C.c_void_p), np.ascontiguousarray(density, np.float32).reshape(-1), float(nu),
+    n = getattr(lib(kind), kind + "_break_test")(d.ctypes.data_as(C.c_void_p), np.ascontiguousarray(density, np.float32).reshape(-1), float(nu),
                                   np.ascontiguousarray(bc, np.uint8).reshape(-1), np.ascontiguousarray(f, np.float64).reshape(-1), float(E),
                                   float(h), float(strength), int(bool(principal)), int(max_steps), float(tol), int(max_iter),
                                   int(coarsest_max_dof), scalars, _ptr(counts), _ptr(cracked), cracked.size, _ptr(detached), detached.size,
                                   _ptr(vm), C.byref(peak), C.byref(removed), C.byref(reason))
     if n < 0:
-        raise ValueError(lib("orc").orc_last_error().decode())
+        raise ValueError(getattr(lib(kind), kind + "_last_error")().decode())
     steps, ic, idt = [], 0, 0
     for s in range(n):
         nc, nd = int(counts[2 * s]), int(counts[2 * s + 1])

@@ -3,7 +3,9 @@
 // C ABI over the reference's own VoxelFEA, compiled from the reference sources where they lie
 // (oracle/Makefile: native/src/fea/hex8.cpp, native/src/fea/voxel_fea.cpp, native/src/util/parallel.cpp,
 // and for the voxelizer native/src/core/voxelize.cpp with the mesh and BVH sources it links against;
-// for the input contract of a CAD part native/src/fea/structural.cpp and native/src/core/materials.cpp)
+// for the input contract of a CAD part native/src/fea/structural.cpp and native/src/core/materials.cpp;
+// for the break test native/src/fea/static_study.cpp with the no-GPU builds of native/src/gpu/gpu.cpp
+// and native/src/gpu/gpu_fea.cpp, gpu_fea.cpp:654-663)
 // into oracle/_ref/libpsref.so. The tests use it to pin the restatement in oracle/oracle.cpp bit for
 // bit, and bench.py may time it as the CPU baseline. The entry points have the same shapes as
 // oracle.cpp's orc_* functions.
@@ -31,6 +33,7 @@
 #include "core/voxelize.hpp"
 #include "fea/structural.hpp"
 #include "fea/topology.hpp"
+#include "fea/static_study.hpp"
 
 static std::string g_err;
 
@@ -428,6 +431,56 @@ int ref_assemble(int model, const int64_t* ntris, const int32_t* tris, const int
         for (const auto& w : a.warnings) all += w + "\n";
         std::snprintf(warnings, wlen, "%s", all.c_str());
         return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
+
+// breakTest (static_study.cpp:197-279) as the reference runs it without a GPU (useGPU = false): each
+// step's VoxelFEA with coarsestMaxDof 1100 solved to 1e-6 in at most 400 iterations
+// (static_study.cpp:67-68, 83, 117-120). Same output shape as orc_break_test; tol, maxIter and
+// coarsestMaxDof must be those fixed values (the reference takes no options for them).
+int ref_break_test(const int* dims, const float* density, double nu, const uint8_t* bc, const double* f, double E, double h, double strength,
+                   int principal, int maxSteps, double tol, int maxIter, int64_t coarsestMaxDof, double* scalars, int* counts, int* cracked,
+                   int crackedCap, int* detached, int detachedCap, float* vm, double* peak, int* removed, int* reason) {
+    try {
+        if (tol != 1e-6 || maxIter != 400 || coarsestMaxDof != 1100)
+            throw std::invalid_argument("the reference's CPU break test solves to 1e-6 in 400 iterations with coarsestMaxDof 1100");
+        ps::StructuralInput in;
+        in.dims = {dims[0], dims[1], dims[2]};
+        const int64_t nV = int64_t(dims[0]) * dims[1] * dims[2], nNodes = int64_t(dims[0] + 1) * (dims[1] + 1) * (dims[2] + 1);
+        in.density.assign(density, density + nV);
+        in.bc.assign(bc, bc + 3 * nNodes);
+        in.f.assign(f, f + 3 * nNodes);
+        in.nu = nu;
+        in.E = E;
+        in.h = h;
+        in.useGPU = false;
+        std::vector<ps::BreakStep> steps;
+        const ps::BreakResult r = ps::breakTest(in, strength, principal != 0, maxSteps, [&](ps::BreakStep&& b) { steps.push_back(std::move(b)); }, nullptr);
+        static const char* names[] = {"step limit", "separated", "unsupported load", "solver did not converge", "no stress"};
+        *reason = -1;
+        for (int k = 0; k < 5; k++)
+            if (r.reason == names[k]) *reason = k;
+        *peak = r.peak;
+        *removed = r.removed;
+        int nc = 0, nd = 0;
+        for (size_t s = 0; s < steps.size(); s++) {
+            const auto& b = steps[s];
+            scalars[3 * s] = b.lambda;
+            scalars[3 * s + 1] = b.maxDisp;
+            scalars[3 * s + 2] = b.voxels;
+            counts[2 * s] = int(b.cracked.size());
+            counts[2 * s + 1] = int(b.detached.size());
+            for (int e : b.cracked)
+                if (nc < crackedCap) cracked[nc++] = e;
+            for (int e : b.detached)
+                if (nd < detachedCap) detached[nd++] = e;
+            if (vm) std::memcpy(vm + s * size_t(nNodes), b.nodeVM.data(), size_t(nNodes) * sizeof(float));
+        }
+        return int(steps.size());
     } catch (const std::exception& e) {
         g_err = e.what();
         return -1;

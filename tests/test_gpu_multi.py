@@ -83,3 +83,32 @@ def test_block_precondition_equals_lone(P, W):
         # a smaller block after a larger one
         assert np.array_equal(m.precondition_multi(R[4:6]), Z[4:6])
         m.close()
+
+
+def test_block_precondition_matches_oracle_per_vector(P, W, cpu):
+    # LOBPCG's block preconditioner as the reference's eigen solver drives it (eigen.cpp:508-553):
+    # a block of free-DOF vectors scattered to full length (held DOFs 0), one V-cycle per vector
+    # (cpuPreconditioner), gathered back. The device's block V-cycle against the oracle's
+    # VoxelFEA::precondition of each vector, to the FP32 V-cycle's bar; one part carries a diagonal
+    # shift, as a shifted eigen model's does.
+    can = W.cantilever(4)
+    can["diag_add"] = 0.01 * np.random.default_rng(5).random(3 * W.nodes(can["dims"]))
+    for w in (W.void_block((32, 16, 16), seed=2), can):
+        da = w.get("diag_add")
+        m = P.Model(w["dims"], w["density"], w["nu"], w["bc"], 300, diag_add=da)
+        o = cpu.Model(w["dims"], w["density"], w["nu"], w["bc"], 300, diag_add=da)
+        fixed = o.level_arrays(0)[0]
+        free = np.flatnonzero(fixed == 0)
+        rng = np.random.default_rng(13)
+        block = rng.standard_normal((5, free.size))
+        block[3] *= 1e-7
+        full = np.zeros((5, m.n_dof))
+        full[:, free] = block
+        Z = m.precondition_multi(full)
+        for k in range(5):
+            zo = o.precondition(full[k])
+            err = np.linalg.norm(Z[k] - zo) / np.linalg.norm(zo)
+            print(w["name"], k, err, flush=True)
+            assert err < 1e-4
+            assert not Z[k][fixed != 0].any()
+        m.close()

@@ -562,6 +562,50 @@ def test_break_first_step_restated(cpu, W):
         assert list(st["detached"]) == [int(e) for e in np.flatnonzero((nxt > 0) & ~(pruned > 0))]
 
 
+def _measures(dims, density, nodal):
+    """Each voxel's worst corner value (static_study.cpp:239-247), 0 for empty voxels."""
+    nx, ny, nz = dims
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    n0 = (i + (nx + 1) * (j + (ny + 1) * k)).reshape(-1)
+    offs = [0, 1, 1 + (nx + 1), nx + 1]
+    offs = offs + [o + (nx + 1) * (ny + 1) for o in offs]
+    m = np.maximum(0, np.max(np.stack([nodal[n0 + o] for o in offs]), axis=0))
+    return np.where(density > 0, m, -1.0)
+
+
+@pytest.mark.skipif(not ref_available(), reason="oracle/_ref not built")
+@pytest.mark.parametrize("case", ["uniform", "random"])
+def test_break_sequence_bitwise_vs_reference_breaktest(cpu, W, case):
+    # the reference's own breakTest (static_study.cpp:197-279, CPU engine: coarsest 1100, tol 1e-6,
+    # 400 iterations) against the restatement: every step's lambda, displacement, crack list in order,
+    # detached voxels and von Mises field, bit for bit. Neighbouring voxels share their worst corner,
+    # so equal measures straddle the crack size; std::sort's order among them must be the reference's.
+    dims, density, bc, f = _break_case(W, (20, 6, 6), seed=3)
+    if case == "uniform":
+        density = np.ones_like(density)
+    kw = dict(principal=False, max_steps=6, tol=1e-6, max_iter=400, coarsest_max_dof=1100, fields=True)
+    E, h, strength = 200e9, 1e-3, 150e6
+    a = cpu.break_test(dims, density, 0.3, bc, f, E, h, strength, **kw)
+    b = cpu.break_test(dims, density, 0.3, bc, f, E, h, strength, kind="ref", **kw)
+    print(case, a["reason"], len(a["steps"]), [list(s["cracked"]) for s in a["steps"]], flush=True)
+    assert a["reason"] == b["reason"] and a["removed"] == b["removed"] and a["peak"] == b["peak"]
+    assert len(a["steps"]) == len(b["steps"]) >= 3
+    straddled = 0
+    dens = density.copy()
+    for sa, sb in zip(a["steps"], b["steps"]):
+        assert sa["lambda"] == sb["lambda"] and sa["max_disp"] == sb["max_disp"] and sa["voxels"] == sb["voxels"]
+        assert np.array_equal(sa["cracked"], sb["cracked"]) and np.array_equal(sa["detached"], sb["detached"])
+        assert np.array_equal(sa["vm"], sb["vm"])
+        # equal measures on both sides of the crack size: the tie order decided which were cracked
+        m = _measures(dims, dens, sa["vm"])
+        last = m[sa["cracked"][-1]]
+        straddled += int(np.count_nonzero(m == last) > np.count_nonzero(m[sa["cracked"]] == last))
+        dens[sa["cracked"]] = 0
+        dens[sa["detached"]] = 0
+    if case == "uniform":
+        assert straddled >= 1
+
+
 def test_break_sequence_invariants(cpu, W):
     dims, density, bc, f = _break_case(W, (16, 4, 4), seed=5)
     r = cpu.break_test(dims, density, 0.3, bc, f, 200e9, 1e-3, 150e6, max_steps=40, tol=1e-8)
