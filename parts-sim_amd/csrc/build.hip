@@ -527,7 +527,7 @@ void launchLanczosStart(const double* invD, int64_t n, double* v, double* tmp, R
 }
 
 void buildOnDevice(Model& M, const psx_model_desc& d) {
-    if (d.nx < 1 || d.ny < 1 || d.nz < 1) throw std::invalid_argument("The structural grid must have three positive dimensions.");
+    checkGridLimits(d.nx, d.ny, d.nz);
     if (!d.density || !d.bc) throw std::invalid_argument("density and bc are required");
     const int maxLevels = d.max_levels <= 0 || d.max_levels > 12 ? 12 : d.max_levels;
     const i64 coarsestMaxDof = d.coarsest_max_dof > 0 ? d.coarsest_max_dof : 1100;

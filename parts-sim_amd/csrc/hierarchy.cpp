@@ -10,6 +10,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <stdexcept>
+#include <string>
 #include <thread>
 
 namespace psx {
@@ -685,9 +686,17 @@ static void coarseInverse(Hierarchy& H) {
     }, 8);
 }
 
+void checkGridLimits(int nx, int ny, int nz) {
+    if (nx < 1 || ny < 1 || nz < 1) throw std::invalid_argument("The structural grid must have three positive dimensions.");
+    const i64 NX = i64(nx) + 1, NY = i64(ny) + 1, NZ = i64(nz) + 1, lim = INT32_MAX;
+    if (3 * NX * NY > lim || i64(nx) * ny > lim || NX * NY * NZ > lim - 2 || i64(nx) * ny * nz > lim - 2)
+        throw std::invalid_argument("The structural grid of " + std::to_string(nx) + " x " + std::to_string(ny) + " x " + std::to_string(nz) +
+                                    " voxels exceeds the 32-bit node and plane indices of the device kernels.");
+}
+
 void buildHierarchy(Hierarchy& H, int nx, int ny, int nz, const float* density, double nu, const uint8_t* bc, i64 coarsestMaxDof,
                     const double* diagAdd, int maxLevels) {
-    if (nx < 1 || ny < 1 || nz < 1) throw std::invalid_argument("The structural grid must have three positive dimensions.");
+    checkGridLimits(nx, ny, nz);
     if (!density || !bc) throw std::invalid_argument("density and bc are required");
     const i64 nVox = i64(nx) * ny * nz;
     for (i64 e = 0; e < nVox; e++)

@@ -69,6 +69,12 @@ struct Hierarchy {
     std::vector<double> Ainv;        // m x m inverse of the coarsest free-DOF matrix (row-major)
 };
 
+// Index widths the device kernels rely on: a node plane's component offsets 3 (NX NY) fit a signed
+// 32-bit offset from a 64-bit plane base, a voxel plane nx ny fits an int, and node and voxel indices
+// fit signed 32-bit values (queue entries, voxel words, decoded node indices). Throws
+// std::invalid_argument for a grid beyond them, before any array is read.
+void checkGridLimits(int nx, int ny, int nz);
+
 // Builds the hierarchy (voxel_fea.cpp:138-183): coarsens until the free DOFs are at most
 // coarsestMaxDof, 12 levels, or a single voxel; then factors the coarsest level. Throws
 // std::invalid_argument on bad input and std::runtime_error when the coarsest level has no direct

@@ -93,14 +93,17 @@ struct psx_server_s {
             if (!p.busy) b += p.bytes;
         return b;
     }
-    // evict idle resident parts, least recently used first, until need fits (lock held)
-    void makeRoom(int64_t need) {
+    // evict idle resident parts, least recently used first, until need fits (lock held); the evicted
+    // models move to `out`, whose owner destroys them after unlocking (a model's teardown waits on its
+    // stream, which must not hold up other workers' admissions and completions)
+    void makeRoom(int64_t need, std::vector<std::unique_ptr<psx::Model>>& out) {
         while (held + need > budget) {
             auto victim = parts.end();
             for (auto it = parts.begin(); it != parts.end(); ++it)
                 if (!it->second.busy && (victim == parts.end() || it->second.used < victim->second.used)) victim = it;
             if (victim == parts.end()) return;
             held -= victim->second.bytes;
+            out.push_back(std::move(victim->second.model));
             parts.erase(victim);
         }
     }
@@ -132,11 +135,13 @@ struct psx_server_s {
                     r->estimate = std::max<int64_t>(0, r->estimate - it->second.bytes);
                 }
             }
-            makeRoom(r->estimate);
+            std::vector<std::unique_ptr<psx::Model>> evicted;
+            makeRoom(r->estimate, evicted);
             held += r->estimate;
             running++;
             r->admitted = Clock::now();
             lk.unlock();
+            evicted.clear();
             run(*r, reused);
             lk.lock();
             held -= r->estimate;

@@ -49,6 +49,7 @@ int psx_create(const psx_model_desc* desc, psx_model* out) {
     return guarded([&] {
         if (!desc || !out) throw std::invalid_argument("desc and out are required");
         *out = nullptr;
+        psx::checkGridLimits(desc->nx, desc->ny, desc->nz);  // before the device is touched
         auto h = std::make_unique<psx_model_s>();
         h->m = std::make_unique<psx::Model>(*desc);
         *out = h.release();

@@ -41,10 +41,13 @@ struct Axis {
 __device__ __forceinline__ double min3(double x, double y, double z) { return x < y ? (x < z ? x : z) : (y < z ? y : z); }
 __device__ __forceinline__ double max3(double x, double y, double z) { return x > y ? (x > z ? x : z) : (y > z ? y : z); }
 
-// FILL = false: count the crossings per ray; true: store their depths at fill[ray]++
+// FILL = false: count the crossings per ray; true: store their depths at cnt[ray]++, within the ray's
+// slice [.., end[ray + 1]) that the count pass sized (a crossing the count pass did not see is not
+// stored but counted in overflow, which the host turns into an error)
 template <bool FILL>
 __global__ __launch_bounds__(BLOCK) void kRaster(const float* __restrict__ pos, const uint32_t* __restrict__ index, int64_t nTri, Axis X,
-                                                 uint32_t* __restrict__ cnt, float* __restrict__ hits) {
+                                                 uint32_t* __restrict__ cnt, float* __restrict__ hits, const uint32_t* __restrict__ end,
+                                                 uint32_t* __restrict__ overflow) {
     const int64_t t = int64_t(blockIdx.x) * BLOCK + threadIdx.x;
     if (t >= nTri) return;
     const int rowLo = int(blockIdx.y) * ROWS, rowHi = min(X.Nv - 1, rowLo + ROWS - 1);
@@ -73,8 +76,13 @@ __global__ __launch_bounds__(BLOCK) void kRaster(const float* __restrict__ pos, 
             const double l3 = 1 - l1 - l2;
             if (l3 < 0) continue;
             const int64_t ray = int64_t(su) + int64_t(X.Nu) * sv;
-            if constexpr (FILL) hits[atomicAdd(&cnt[ray], 1u)] = float(l1 * aa + l2 * ba + l3 * ca);
-            else atomicAdd(&cnt[ray], 1u);
+            if constexpr (FILL) {
+                const uint32_t slot = atomicAdd(&cnt[ray], 1u);
+                if (slot < end[ray + 1]) hits[slot] = float(l1 * aa + l2 * ba + l3 * ca);
+                else atomicAdd(overflow, 1u);
+            } else {
+                atomicAdd(&cnt[ray], 1u);
+            }
         }
     }
 }
@@ -191,6 +199,8 @@ int psx_voxelize(const float* pos, int64_t n_vertices, const uint32_t* index, in
         size_t scanBytes = 0;
         HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, scanBytes, cnt, off, int(maxRays + 1), B.s));
         void* scanTmp = B.get<uint8_t>(scanBytes);
+        uint32_t* overflow = B.get<uint32_t>(1);
+        HIPCHK(hipMemsetAsync(overflow, 0, sizeof(uint32_t), B.s));
         uint32_t* total = nullptr;
         HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&total), sizeof(uint32_t), hipHostMallocDefault));
         struct Pinned {
@@ -216,18 +226,21 @@ int psx_voxelize(const float* pos, int64_t n_vertices, const uint32_t* index, in
             if (nRays + 1 > int64_t(UINT32_MAX) / 2 || (X.Nv + ROWS - 1) / ROWS > 65535) throw std::invalid_argument("sample grid too large");
             HIPCHK(hipMemsetAsync(cnt, 0, size_t(nRays + 1) * sizeof(uint32_t), B.s));
             const dim3 g(unsigned((n_tris + BLOCK - 1) / BLOCK), unsigned((X.Nv + ROWS - 1) / ROWS));
-            if (n_tris) kRaster<false><<<g, BLOCK, 0, B.s>>>(dPos, dIdx, n_tris, X, cnt, nullptr);
+            if (n_tris) kRaster<false><<<g, BLOCK, 0, B.s>>>(dPos, dIdx, n_tris, X, cnt, nullptr, nullptr, nullptr);
             HIPCHK(hipcub::DeviceScan::ExclusiveSum(scanTmp, scanBytes, cnt, off, int(nRays + 1), B.s));
             HIPCHK(hipMemcpyAsync(total, off + nRays, sizeof(uint32_t), hipMemcpyDeviceToHost, B.s));
             HIPCHK(hipStreamSynchronize(B.s));
             float* hits = B.get<float>(size_t(*total));
             HIPCHK(hipMemcpyAsync(cnt, off, size_t(nRays + 1) * sizeof(uint32_t), hipMemcpyDeviceToDevice, B.s));
-            if (n_tris) kRaster<true><<<g, BLOCK, 0, B.s>>>(dPos, dIdx, n_tris, X, cnt, hits);
+            if (n_tris) kRaster<true><<<g, BLOCK, 0, B.s>>>(dPos, dIdx, n_tris, X, cnt, hits, off, overflow);
             kSpans<<<unsigned((nRays + BLOCK - 1) / BLOCK), BLOCK, 0, B.s>>>(off, hits, X, votes);
             HIPCHK(hipGetLastError());
         }
         kFractions<<<unsigned((nVox + BLOCK - 1) / BLOCK), BLOCK, 0, B.s>>>(votes, dims[0], dims[1], dims[2], sub, S[0], S[1], dFrac);
         HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(total, overflow, sizeof(uint32_t), hipMemcpyDeviceToHost, B.s));
+        HIPCHK(hipStreamSynchronize(B.s));
+        if (*total != 0) throw DeviceError("voxelizer: the fill pass found " + std::to_string(*total) + " ray crossings the count pass did not");
         HIPCHK(hipMemcpyAsync(frac, dFrac, size_t(nVox) * sizeof(float), hipMemcpyDeviceToHost, B.s));
         HIPCHK(hipStreamSynchronize(B.s));
         return PSX_OK;

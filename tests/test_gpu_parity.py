@@ -293,3 +293,29 @@ def test_dense_form_forced(P, W, cpu, forced, monkeypatch):
         print(w["name"], forced, _rel(u, uo), flush=True)
         assert _rel(u, uo) < 1e-7
         g.close()
+
+
+@pytest.mark.parametrize("dims", [(21, 13, 11), (23, 9, 7), (37, 23, 11)])
+def test_precondition_odd_grids_match_oracle(P, W, cpu, dims):
+    # grids with odd cell counts on the levels they coarsen: the fine node 2I of a coarse node on the
+    # last coarse plane lies outside the fine grid, so restriction (the 3 x 3 x 3 gather around 2I)
+    # and prolongation (trilinear from 1, 2, 4 or 8 coarse nodes) take their one-sided branches
+    # (voxel_fea.cpp:739-815). One V-cycle against the oracle's at every depth from 2 levels (the
+    # transfer between the fine level and the coarsest) to the full hierarchy; a wrong transfer
+    # weight on a boundary plane moves z by far more than the FP32 V-cycle's 1e-4.
+    nx, ny, nz = dims
+    rng = np.random.default_rng(sum(dims))
+    density = rng.uniform(0.2, 1.0, nx * ny * nz).astype(np.float32)
+    density[rng.uniform(0, 1, density.size) < 0.15] = 0.0  # empty voxels: mixed coarse elements
+    density.reshape(nz, ny, nx)[:, :, 0] = 1.0  # the clamped face stays attached
+    bc = W.clamp_x0(dims)
+    for coarsest in (3000, 300, 60):
+        g = P.Model(dims, density, 0.3, bc, coarsest)
+        o = cpu.Model(dims, density, 0.3, bc, coarsest)
+        assert g.num_levels == o.num_levels
+        for k in range(2):
+            r = np.random.default_rng(k).standard_normal(g.n_dof)
+            z, zo = g.precondition(r), o.precondition(r)
+            print(dims, coarsest, g.num_levels, [g.level(l)["dims"] for l in range(g.num_levels)], k, _rel(z, zo), flush=True)
+            assert _rel(z, zo) < 1e-4
+        g.close()

@@ -21,3 +21,22 @@ def test_prune_floating_matches_golden_and_oracle(P, cpu):
         a, ra = P.prune_floating(dims, d, held)
         b, rb = cpu.prune_floating(dims, d, held)
         assert np.array_equal(a, b) and ra == rb
+
+
+def test_grids_beyond_the_kernels_index_widths_are_rejected(P):
+    # the device kernels index node planes with 32-bit offsets (3 NX NY components) and nodes and
+    # voxels with signed 32-bit values: both build paths reject a grid beyond them before reading
+    # any array (stand-in pointers), with the argument error and no device work
+    import ctypes as C
+    dummy = C.c_void_p(256)
+    for dims in [(40000, 20000, 1), (50000, 1, 50000), (2000, 2000, 600), (1 << 16, 1 << 15, 1)]:
+        for device_build in (0, 1):
+            d = P.ModelDesc(dims[0], dims[1], dims[2], dummy, 0.3, dummy, None, 1100, 0, device_build)
+            h = C.c_void_p()
+            code = P.lib().psx_create(C.byref(d), C.byref(h))
+            assert code == P.ERR_ARGUMENT, (dims, code)
+            assert "32-bit" in P.lib().psx_last_error().decode()
+        code = P.lib().psx_hierarchy_create(C.byref(d), C.byref(h))
+        assert code == P.ERR_ARGUMENT and "32-bit" in P.lib().psx_last_error().decode()
+    # the largest configuration the bench solves stays accepted by the rule
+    assert 3 * 513 * 129 < 2 ** 31 and 513 * 129 * 129 < 2 ** 31 - 2
