@@ -1,6 +1,8 @@
-// The device-availability queries of the reference's native/src/gpu/gpu.hpp (gpu.hpp:22-26,
-// gpu.cpp:250-285) over the HIP runtime, for a build whose GPU solver is the psx library. The
-// PARTS_SIM_NO_GPU switch keeps its meaning (gpu.cpp:252): set, it reports no GPU.
+// Everything the reference's native/src/gpu/gpu.hpp declares outside its WebGPU section (gpu.hpp:22-31,
+// gpu.cpp:250-285), for a build whose GPU solver is the psx library: the device queries over the HIP
+// runtime, and the two WebGPU-only queries answered as a build without WebGPU answers them (no 16-bit
+// shader floats, no embedded WGSL), so callers such as the airflow solver (airflow.cpp:25) still link.
+// The PARTS_SIM_NO_GPU switch keeps its meaning (gpu.cpp:252): set, it reports no GPU.
 #include "../gpu/gpu.hpp"
 
 #include <cstdlib>
@@ -22,5 +24,9 @@ std::string gpuName() {
     if (psx_device_name(buf, sizeof buf) != PSX_OK) return "";
     return buf;
 }
+
+bool gpuShaderF16() { return false; }
+
+const char* wgslSource(const std::string&) { return ""; }
 
 }  // namespace ps
