@@ -94,10 +94,7 @@ def serve(args, P, W):
     parts, results, lock = {}, {}, threading.Lock()
     done_evt = {r["id"]: threading.Event() for r in reqs}
 
-    def loads(w, nrhs, seed):
-        rng = np.random.default_rng(seed)
-        f = w["f"]
-        return np.stack([f * (1 + 0.5 * k) + 0.05 * np.abs(f).max() * rng.standard_normal(f.size) * (f != 0) for k in range(nrhs)])
+    loads = W.serving_loads
 
     def waiter(r, ticket):
         u, res = srv.wait(ticket)

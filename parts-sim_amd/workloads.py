@@ -158,6 +158,14 @@ def serving_stream(n_requests, seed=5, rate=200.0, edge_min=32, edge_max=256, wa
     return out
 
 
+def serving_loads(w, nrhs, seed):
+    """The load cases of a served request on part ``w``: the part's pressure load scaled by 1, 1.5, 2, ...
+    plus seeded noise of 5 % of its largest nodal force on the loaded DOFs (nrhs x nDof)."""
+    rng = np.random.default_rng(seed)
+    f = w["f"]
+    return np.stack([f * (1 + 0.5 * k) + 0.05 * np.abs(f).max() * rng.standard_normal(f.size) * (f != 0) for k in range(nrhs)])
+
+
 def remove_voxels(density, fraction, seed):
     """Sets ``fraction`` of the solid voxels (density > 0) to 0, chosen with the given seed."""
     rng = np.random.default_rng(seed)
