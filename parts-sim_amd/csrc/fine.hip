@@ -75,6 +75,43 @@ template <class F, int... I> __device__ __forceinline__ void staticForImpl(F&& f
 }
 template <int N, class F> __device__ __forceinline__ void staticFor(F&& f) { staticForImpl(f, std::make_integer_sequence<int, N>{}); }
 
+// Raw buffer accesses over one plane: the descriptor's byte count is the plane's, so a lane whose
+// offset is OUT_OF_RANGE (a position outside the part) reads 0 and its store is dropped, and a plane
+// outside the grid (count 0) reads 0 everywhere, without predicated branches. Plane byte counts stay
+// below 2^31 (checkGridLimits bounds a node plane at 2^31 / 24 nodes).
+constexpr int32_t OUT_OF_RANGE = int32_t(0x80000000u);
+constexpr int RSRC_WORD3 = 0x00020000;  // raw buffer, 32-bit data (gfx9 family)
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t planeRsrc(const void* base, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, int(bytes), RSRC_WORD3);
+}
+// a node's three components at byte offset off
+__device__ __forceinline__ void bload3(__amdgpu_buffer_rsrc_t rs, int32_t off, float (&d)[3]) {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b96(rs, off, 0, 0);
+    d[0] = __uint_as_float(v[0]);
+    d[1] = __uint_as_float(v[1]);
+    d[2] = __uint_as_float(v[2]);
+}
+__device__ __forceinline__ void bload3(__amdgpu_buffer_rsrc_t rs, int32_t off, double (&d)[3]) {
+    const auto a = __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0);
+    const auto b = __builtin_amdgcn_raw_buffer_load_b64(rs, off + 16, 0, 0);
+    d[0] = __hiloint2double(int(a[1]), int(a[0]));
+    d[1] = __hiloint2double(int(a[3]), int(a[2]));
+    d[2] = __hiloint2double(int(b[1]), int(b[0]));
+}
+__device__ __forceinline__ void bstore3(__amdgpu_buffer_rsrc_t rs, int32_t off, const float (&d)[3]) {
+    typedef unsigned u3 __attribute__((ext_vector_type(3)));
+    const u3 v = {__float_as_uint(d[0]), __float_as_uint(d[1]), __float_as_uint(d[2])};
+    __builtin_amdgcn_raw_buffer_store_b96(v, rs, off, 0, 0);
+}
+__device__ __forceinline__ void bstore3(__amdgpu_buffer_rsrc_t rs, int32_t off, const double (&d)[3]) {
+    typedef unsigned u4 __attribute__((ext_vector_type(4)));
+    typedef unsigned u2 __attribute__((ext_vector_type(2)));
+    const u4 a = {unsigned(__double2loint(d[0])), unsigned(__double2hiint(d[0])), unsigned(__double2loint(d[1])), unsigned(__double2hiint(d[1]))};
+    const u2 b = {unsigned(__double2loint(d[2])), unsigned(__double2hiint(d[2]))};
+    __builtin_amdgcn_raw_buffer_store_b128(a, rs, off, 0, 0);
+    __builtin_amdgcn_raw_buffer_store_b64(b, rs, off + 16, 0, 0);
+}
+
 template <class T> struct FineP;
 template <> struct FineP<double> {
     __device__ static const double* invD(const DevLevel& L) { return L.invD64; }
@@ -149,37 +186,31 @@ template <class T, int MODE, bool OWNC, bool DOT, bool DENSE = false, bool MF = 
     constexpr bool NEEDS_D = MODE == AM_CHEB2;
 
     constexpr int NPL = (PX * PY + NT - 1) / NT, NVL = (VX * VY + NT - 1) / NT;
-    // this thread's share of a node plane / voxel plane: x-y bounds and offsets fixed for the march.
-    // Offsets within a plane are 32-bit (a plane holds < 2^31 / 3 nodes) and the plane's base is
-    // block-uniform, so loads and stores take a scalar base plus a 32-bit vector offset.
-    bool pin[NPL], vin[NVL];
+    // this thread's share of a node plane / voxel plane: byte offsets within the plane, fixed for the
+    // march, OUT_OF_RANGE for positions outside the part (their loads return 0); the plane's
+    // descriptor carries its block-uniform base
     int32_t poff[NPL], voff[NVL];
 #pragma unroll
     for (int q = 0; q < NPL; q++) {
         const int e = tid + q * NT;
         const int gi = i0 - 1 + e % PX, gj = j0 - 1 + e / PX;
-        pin[q] = e < PX * PY && gi >= 0 && gj >= 0 && gi < L.NX && gj < L.NY;
-        poff[q] = pin[q] ? int32_t(3 * (gi + NX * gj)) : 0;
+        const bool in = e < PX * PY && gi >= 0 && gj >= 0 && gi < L.NX && gj < L.NY;
+        poff[q] = in ? int32_t(int64_t(sizeof(T)) * 3 * (gi + NX * gj)) : OUT_OF_RANGE;
     }
 #pragma unroll
     for (int q = 0; q < NVL; q++) {
         const int e = tid + q * NT;
         const int vi = i0 - 1 + e % VX, vj = j0 - 1 + e / VX;
-        vin[q] = e < VX * VY && vi >= 0 && vj >= 0 && vi < L.nx && vj < L.ny;
-        voff[q] = vin[q] ? int32_t(vi + int64_t(L.nx) * vj) : 0;
+        const bool in = e < VX * VY && vi >= 0 && vj >= 0 && vi < L.nx && vj < L.ny;
+        voff[q] = in ? int32_t(4 * (vi + int64_t(L.nx) * vj)) : OUT_OF_RANGE;
     }
     const int64_t nxy = int64_t(L.nx) * L.ny;
+    const uint32_t planeBytes = uint32_t(3 * NXY * int64_t(sizeof(T)));
     auto fetchPlane = [&](int z, T (&px)[NPL][3]) {
         const bool zin = z >= 0 && z < L.NZ;
-        const T* base = x + 3 * NXY * (zin ? z : 0);
+        const auto rs = planeRsrc(x + 3 * NXY * (zin ? z : 0), zin ? planeBytes : 0u);
 #pragma unroll
-        for (int q = 0; q < NPL; q++) {
-            const bool in = pin[q] && zin;
-            const T* p = base + poff[q];
-            px[q][0] = in ? p[0] : T(0);
-            px[q][1] = in ? p[1] : T(0);
-            px[q][2] = in ? p[2] : T(0);
-        }
+        for (int q = 0; q < NPL; q++) bload3(rs, poff[q], px[q]);
     };
     auto storePlane = [&](int slot, const T (&px)[NPL][3]) {
 #pragma unroll
@@ -195,13 +226,9 @@ template <class T, int MODE, bool OWNC, bool DOT, bool DENSE = false, bool MF = 
     // voxel words in flight: the word itself (scale bits, or the own index with the sign bit)
     auto fetchVox = [&](int k, int32_t (&pv)[NVL]) {
         const bool kin = k >= 0 && k < L.nz;
-        const int32_t* base = L.words + nxy * (kin ? k : 0);
+        const auto rs = planeRsrc(L.words + nxy * (kin ? k : 0), kin ? uint32_t(4 * nxy) : 0u);
 #pragma unroll
-        for (int q = 0; q < NVL; q++) {
-            const bool in = vin[q] && kin;
-            const int32_t w = base[voff[q]];
-            pv[q] = in ? w : 0;
-        }
+        for (int q = 0; q < NVL; q++) pv[q] = int32_t(__builtin_amdgcn_raw_buffer_load_b32(rs, voff[q], 0, 0));
     };
     auto storeVox = [&](int slot, const int32_t (&pv)[NVL]) {
 #pragma unroll
@@ -221,42 +248,20 @@ template <class T, int MODE, bool OWNC, bool DOT, bool DENSE = false, bool MF = 
         uint32_t info;
     };
     const int32_t col = mine ? int32_t(i + NX * j) : 0;
+    const int32_t noff = mine ? int32_t(int64_t(sizeof(T)) * 3 * col) : OUT_OF_RANGE;  // the node's vector entries
+    const int32_t ioff = mine ? int32_t(2 * col) : OUT_OF_RANGE;                          // its info word
     auto fetchOwn = [&](int z, Own& o) {
-        const bool zin = z >= z0 && z < z1, in = mine && zin;
+        const bool zin = z >= z0 && z < z1;
         const int64_t pb = NXY * (zin ? z : z0);  // block-uniform plane base
-        const int32_t c3 = 3 * col;
-        // a node's three components in one load (the address is in range for every thread: plane z0
-        // and column 0 stand in for a thread without a node), zero where the thread has none
-        struct alignas(sizeof(T)) V3 {
-            T a, b, c;
-        };
-        auto ld3 = [&](const T* base, T (&dst)[3]) {
-            const V3 v = *reinterpret_cast<const V3*>(base + 3 * pb + c3);
-            dst[0] = in ? v.a : T(0);
-            dst[1] = in ? v.b : T(0);
-            dst[2] = in ? v.c : T(0);
-        };
-        auto zero3 = [](T (&dst)[3]) { dst[0] = dst[1] = dst[2] = T(0); };
-        if constexpr (DOT) {
-            // the fused-dot variants sit at the 128-VGPR limit: component loads schedule better there
-            // (measured: the three-component loads make the Chebyshev step with fused dots 15 % slower)
-#pragma unroll
-            for (int d = 0; d < 3; d++) {
-                if constexpr (NEEDS_INV && OWNC) o.inv[d] = in ? (invD + 3 * pb)[c3 + d] : T(0);
-                if constexpr (NEEDS_R) o.r[d] = in ? (r + 3 * pb)[c3 + d] : T(0);
-                if constexpr (NEEDS_D) o.d[d] = in && zp ? (zp + 3 * pb)[c3 + d] : T(0);
-                if constexpr (NEEDS_INV) o.qd[d] = in && qdot ? (qdot + 3 * pb)[c3 + d] : T(0);
-            }
-        } else {
-            if constexpr (NEEDS_INV && OWNC) ld3(invD, o.inv);
-            if constexpr (NEEDS_R) ld3(r, o.r);
-            if constexpr (NEEDS_D) {
-                if (zp) ld3(zp, o.d);
-                else zero3(o.d);
-            }
-            if constexpr (NEEDS_INV) zero3(o.qd);
+        const uint32_t vb = zin ? planeBytes : 0u;
+        if constexpr (NEEDS_INV && OWNC) bload3(planeRsrc(invD + 3 * pb, vb), noff, o.inv);
+        if constexpr (NEEDS_R) bload3(planeRsrc(r + 3 * pb, vb), noff, o.r);
+        if constexpr (NEEDS_D) bload3(planeRsrc(zp ? zp + 3 * pb : x, zp ? vb : 0u), noff, o.d);
+        if constexpr (NEEDS_INV) {
+            if constexpr (DOT) bload3(planeRsrc(qdot ? qdot + 3 * pb : x, qdot ? vb : 0u), noff, o.qd);
+            else o.qd[0] = o.qd[1] = o.qd[2] = T(0);
         }
-        o.info = in ? uint32_t((L.nodeInfo + pb)[col]) : 0u;
+        o.info = __builtin_amdgcn_raw_buffer_load_b16(planeRsrc(L.nodeInfo + pb, zin ? uint32_t(2 * NXY) : 0u), ioff, 0, 0);
     };
 
     // matrix-core operands of the in-plane stencil: A[rot] (this lane: row l % 16 = 4 g + i, K slice
@@ -632,25 +637,24 @@ template <class T, int MODE, bool OWNC, bool DOT, bool DENSE = false, bool MF = 
                         for (int d = 0; d < 3; d++) cur.inv[d] = ((cur.info >> (8 + d)) & 1u) ? T(0) : iv;
                     }
                 }
+                T yv[3];
 #pragma unroll
                 for (int d = 0; d < 3; d++) {
-                    const int32_t q = c3 + d;
                     const bool held = (cur.info >> (8 + d)) & 1u;
-                    T yv;
-                    if constexpr (MODE == AM_APPLY) yv = held ? T(0) : out[d];
-                    else if constexpr (MODE == AM_RAW) yv = out[d];
-                    else if constexpr (MODE == AM_RESID) yv = held ? T(0) : cur.r[d] - out[d];
-                    else if constexpr (MODE == AM_CHEB1) yv = xcPrev[d] + c1 * cur.inv[d] * (cur.r[d] - out[d]);
-                    else yv = xcPrev[d] + c1 * (xcPrev[d] - cur.d[d]) + c2 * cur.inv[d] * (cur.r[d] - out[d]);
-                    (y + pb3)[q] = yv;
+                    if constexpr (MODE == AM_APPLY) yv[d] = held ? T(0) : out[d];
+                    else if constexpr (MODE == AM_RAW) yv[d] = out[d];
+                    else if constexpr (MODE == AM_RESID) yv[d] = held ? T(0) : cur.r[d] - out[d];
+                    else if constexpr (MODE == AM_CHEB1) yv[d] = xcPrev[d] + c1 * cur.inv[d] * (cur.r[d] - out[d]);
+                    else yv[d] = xcPrev[d] + c1 * (xcPrev[d] - cur.d[d]) + c2 * cur.inv[d] * (cur.r[d] - out[d]);
                     if constexpr (DOT) {
-                        if constexpr (MODE == AM_APPLY) dot0 += double(xcPrev[d]) * double(yv);
+                        if constexpr (MODE == AM_APPLY) dot0 += double(xcPrev[d]) * double(yv[d]);
                         if constexpr (NEEDS_INV) {
-                            dot0 += double(cur.r[d]) * double(yv);
-                            dot1 += double(yv) * double(cur.qd[d]);
+                            dot0 += double(cur.r[d]) * double(yv[d]);
+                            dot1 += double(yv[d]) * double(cur.qd[d]);
                         }
                     }
                 }
+                bstore3(planeRsrc(y + pb3, planeBytes), noff, yv);
             }
         }
         // DENSE reads node plane k - 2, which the next step's store replaces

@@ -689,7 +689,9 @@ static void coarseInverse(Hierarchy& H) {
 void checkGridLimits(int nx, int ny, int nz) {
     if (nx < 1 || ny < 1 || nz < 1) throw std::invalid_argument("The structural grid must have three positive dimensions.");
     const i64 NX = i64(nx) + 1, NY = i64(ny) + 1, NZ = i64(nz) + 1, lim = INT32_MAX;
-    if (3 * NX * NY > lim || i64(nx) * ny > lim || NX * NY * NZ > lim - 2 || i64(nx) * ny * nz > lim - 2)
+    // a node plane's FP64 components (24 bytes a node) and a voxel plane's words (4 bytes) stay within a
+    // 32-bit byte offset: the operator kernels address a plane through one buffer descriptor
+    if (24 * NX * NY > lim || 4 * i64(nx) * ny > lim || NX * NY * NZ > lim - 2 || i64(nx) * ny * nz > lim - 2)
         throw std::invalid_argument("The structural grid of " + std::to_string(nx) + " x " + std::to_string(ny) + " x " + std::to_string(nz) +
                                     " voxels exceeds the 32-bit node and plane indices of the device kernels.");
 }

@@ -69,9 +69,9 @@ struct Hierarchy {
     std::vector<double> Ainv;        // m x m inverse of the coarsest free-DOF matrix (row-major)
 };
 
-// Index widths the device kernels rely on: a node plane's component offsets 3 (NX NY) fit a signed
-// 32-bit offset from a 64-bit plane base, a voxel plane nx ny fits an int, and node and voxel indices
-// fit signed 32-bit values (queue entries, voxel words, decoded node indices). Throws
+// Index widths the device kernels rely on: a node plane's FP64 components (24 NX NY bytes) and a voxel
+// plane's words (4 nx ny bytes) fit a signed 32-bit byte offset from a 64-bit plane base, and node and
+// voxel indices fit signed 32-bit values (queue entries, voxel words, decoded node indices). Throws
 // std::invalid_argument for a grid beyond them, before any array is read.
 void checkGridLimits(int nx, int ny, int nz);
 

@@ -29,7 +29,7 @@ def test_grids_beyond_the_kernels_index_widths_are_rejected(P):
     # any array (stand-in pointers), with the argument error and no device work
     import ctypes as C
     dummy = C.c_void_p(256)
-    for dims in [(40000, 20000, 1), (50000, 1, 50000), (2000, 2000, 600), (1 << 16, 1 << 15, 1)]:
+    for dims in [(40000, 20000, 1), (50000, 1, 50000), (2000, 2000, 600), (1 << 16, 1 << 15, 1), (10000, 9000, 2)]:
         for device_build in (0, 1):
             d = P.ModelDesc(dims[0], dims[1], dims[2], dummy, 0.3, dummy, None, 1100, 0, device_build)
             h = C.c_void_p()
