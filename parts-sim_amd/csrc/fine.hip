@@ -424,13 +424,16 @@ template <class T, int MODE, bool OWNC, bool DOT, bool DENSE = false, bool MF = 
         const bool done = k - 1 >= z0;
         const uint32_t mask = (done && !DENSE) ? (cur.info & 0xffu) : 0u;
         const int cnt = __popc(mask);
-        const uint64_t lt = (uint64_t(1) << lane) - 1;
-        const uint64_t b0 = __ballot(cnt & 1), b1 = __ballot(cnt & 2), b2 = __ballot(cnt & 4), b3 = __ballot(cnt & 8);
-        const int waveOff = __popcll(b0 & lt) + 2 * __popcll(b1 & lt) + 4 * __popcll(b2 & lt) + 8 * __popcll(b3 & lt);
         // reference voxel: the lowest clear bit (all 8 differ only when all are own; voxel 0 then
         // reads scale 0)
         const int ref = mask == 0xffu ? 0 : __builtin_ctz(~mask);
-        if constexpr (!DENSE) {
+        int waveOff = 0, waveTotal = 0;
+        // wave-uniform: a wave whose nodes list nothing (solid regions) skips the prefix counts
+        if (!DENSE && __ballot(cnt != 0) != 0) {
+            const uint64_t lt = (uint64_t(1) << lane) - 1;
+            const uint64_t b0 = __ballot(cnt & 1), b1 = __ballot(cnt & 2), b2 = __ballot(cnt & 4), b3 = __ballot(cnt & 8);
+            waveOff = __popcll(b0 & lt) + 2 * __popcll(b1 & lt) + 4 * __popcll(b2 & lt) + 8 * __popcll(b3 & lt);
+            waveTotal = __popcll(b0) + 2 * __popcll(b1) + 4 * __popcll(b2) + 8 * __popcll(b3);
             uint32_t m = mask;
             int at = waveOff;
             while (m) {
@@ -439,7 +442,7 @@ template <class T, int MODE, bool OWNC, bool DOT, bool DENSE = false, bool MF = 
                 sh.list[wid][at++] = uint16_t(((ty * TX + tx) << 6) | (ref << 3) | v);
             }
         }
-        if (!DENSE && lane == 0) sh.waveTot[k & 1][wid] = __popcll(b0) + 2 * __popcll(b1) + 4 * __popcll(b2) + 8 * __popcll(b3);
+        if (!DENSE && lane == 0) sh.waveTot[k & 1][wid] = waveTotal;
         __syncthreads();
 
         if (done) {
