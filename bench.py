@@ -211,6 +211,17 @@ def serve(args, P, W):
 
 def batched(args, P, W):
     dims = tuple(int(v) for v in args.dims.split(",")) if args.dims != "512,128,128" else (128, 64, 64)
+    if args.profile_solves:
+        # profiling runs: --steps multi-instance solves of one precision only
+        w = W.simp_block(dims, seed=4, nrhs=args.nrhs)
+        m = P.Model(w["dims"], w["density"], w["nu"], w["bc"], 1200, device_build=True)
+        m.estimate_smoothers()
+        d_f, d_u = P.DeviceArray(w["f"].size, w["f"].reshape(-1)), P.DeviceArray(w["f"].size)
+        for k in range(args.steps):
+            t = time.perf_counter()
+            infos = m.solve_multi_device(args.nrhs, d_f, d_u, tol=args.tol, max_iter=3000, precision=args.profile_solves, check_every=8)
+            log("%s batched solve %d: %d iterations, %.2f ms" % (args.profile_solves, k, max(i["iterations"] for i in infos), 1e3 * (time.perf_counter() - t)))
+        return
     print(json.dumps(batched_leg(P, W, dims, args.nrhs, args.steps, args.warmup, args.tol)), flush=True)
 
 
@@ -243,13 +254,18 @@ def batched_leg(P, W, dims, nrhs, steps, warmup, tol):
                 lat.append(time.perf_counter() - t)
         return np.array(lat), out
 
-    def multi(precision):
-        return m.solve_multi_device(k, d_f, d_u, tol=tol, max_iter=3000, precision=precision, check_every=8)
+    def multi(precision, mfma=0):
+        return m.solve_multi_device(k, d_f, d_u, tol=tol, max_iter=3000, precision=precision, check_every=8, mfma_smoother=mfma)
 
     lat_b, infos = timed(lambda: multi("mixed"), steps, warmup)
     e_b = errs(d_u.download().reshape(k, n))
     if not all(i["converged"] for i in infos) or max(e_b) > 1e-6:
         raise RuntimeError("batched mixed solve missed the accuracy bar")
+    # the coarse levels' own element products on the matrix cores (BF16 MFMA over the 16 cases)
+    lat_mf, infos_mf = timed(lambda: multi("mixed", 2), steps, warmup)
+    e_mf = errs(d_u.download().reshape(k, n))
+    if not all(i["converged"] for i in infos_mf) or max(e_mf) > 1e-6:
+        raise RuntimeError("batched mixed solve with the matrix-core own elements missed the accuracy bar")
     lat_b64, infos64 = timed(lambda: multi("fp64"), max(2, steps // 2), 1)
     e_b64 = errs(d_u.download().reshape(k, n))
     # one load case after another (the reference's way), same device vectors
@@ -279,7 +295,10 @@ def batched_leg(P, W, dims, nrhs, steps, warmup, tol):
            "sequential_mixed": {"solves_per_s": round(k / lat_s.mean(), 3), "ms": round(1e3 * lat_s.mean(), 3), "iterations_max": int(max(its_s))},
            "sequential_fp64": {"solves_per_s": round(k / lat_s64.mean(), 3), "ms": round(1e3 * lat_s64.mean(), 3), "iterations_max": int(max(its_s64))},
            "speedup_vs_sequential_fp64": round(lat_s64.mean() / lat_b.mean(), 3),
-           "speedup_vs_sequential_mixed": round(lat_s.mean() / lat_b.mean(), 3)}
+           "speedup_vs_sequential_mixed": round(lat_s.mean() / lat_b.mean(), 3),
+           "batched_mfma_own": {"solves_per_s": round(k / lat_mf.mean(), 3), "ms": round(1e3 * lat_mf.mean(), 3), "rel_error_max": max(e_mf),
+                                "iterations_max": int(max(i["iterations"] for i in infos_mf)),
+                                "form": "coarse levels' own element products as BF16 MFMA (v_mfma_f32_16x16x32_bf16) over the 16 cases"}}
     m.close()
     return out
 

@@ -75,7 +75,12 @@ typedef struct {
     psx_progress_fn on_progress;
     void* user;
     int cheb_degree;            /* Chebyshev smoother degree 1..6 (0 = 2, the reference's, voxel_fea.cpp:692-725) */
-    int mfma_smoother;          /* PSX_PREC_MIXED: smooth with the BF16 matrix-core operator (0 = FP32) */
+    int mfma_smoother;          /* PSX_PREC_MIXED, bit flags (0 = FP32 VALU everywhere): 1 = the level-0 in-plane
+                                   stencil on the matrix cores (BF16 neighbour differences, one load case);
+                                   2 = levels with stored own element matrices apply them on the matrix
+                                   cores (one v_mfma_f32_16x16x32_bf16 per node and voxel over 16 load
+                                   cases: BF16 element rows and element-corner differences, FP32
+                                   accumulation) */
     int eager;                  /* 1: launch every kernel from the host; 0: replay a captured HIP graph of
                                    check_every iterations (the same kernels and arguments) */
     int pre_degree;             /* Chebyshev degree of the pre-smoothing steps (0 = cheb_degree); the

@@ -623,7 +623,7 @@ class Model:
     def _opts(tol, max_iter, precision, check_every, fold_every, on_progress, cheb_degree=0, mfma_smoother=False, eager=False, pre_degree=0):
         cb = PROGRESS(0) if on_progress is None else PROGRESS(lambda it, rel, user: int(bool(on_progress(it, rel))))
         return SolveOpts(float(tol), int(max_iter), PREC[precision], int(check_every), int(fold_every), cb, None, int(cheb_degree),
-                         int(bool(mfma_smoother)), int(bool(eager)), int(pre_degree)), cb
+                         int(mfma_smoother), int(bool(eager)), int(pre_degree)), cb
 
     @staticmethod
     def _info(info):
@@ -654,7 +654,7 @@ class Model:
         return self._info(info)
 
     def solve_multi(self, F, tol=1e-6, max_iter=500, X0=None, precision="mixed", check_every=0, fold_every=0, on_progress=None, cheb_degree=0,
-                    eager=False):
+                    eager=False, mfma_smoother=0):
         """Solve K U[k] = F[k] for nrhs load cases (F: nrhs x n_dof) in one set of launches per
         V-cycle level; returns U and one info dict per load case."""
         F = np.ascontiguousarray(F, np.float64)
@@ -665,15 +665,15 @@ class Model:
         if X0a is not None and X0a.size != F.size:
             raise ValueError("Start displacements must be nrhs x n_dof.")
         U = np.zeros_like(F)
-        o, cb = self._opts(tol, max_iter, precision, check_every, fold_every, on_progress, cheb_degree, False, eager)
+        o, cb = self._opts(tol, max_iter, precision, check_every, fold_every, on_progress, cheb_degree, mfma_smoother, eager)
         infos = (SolveInfo * k)()
         _check(lib().psx_solve_multi(self.h, k, _p(F), _p(X0a), C.byref(o), _p(U), infos))
         return U, [self._info(infos[i]) for i in range(k)]
 
     def solve_multi_device(self, nrhs, d_f, d_u, d_x0=None, tol=1e-6, max_iter=500, precision="mixed", check_every=0, fold_every=0, cheb_degree=0,
-                           eager=False):
+                           eager=False, mfma_smoother=0):
         """solve_multi with HBM-resident nrhs x n_dof vectors (DeviceArray or raw device addresses)."""
-        o, cb = self._opts(tol, max_iter, precision, check_every, fold_every, None, cheb_degree, False, eager)
+        o, cb = self._opts(tol, max_iter, precision, check_every, fold_every, None, cheb_degree, mfma_smoother, eager)
         infos = (SolveInfo * int(nrhs))()
         addr = lambda a: None if a is None else C.c_void_p(int(a))
         _check(lib().psx_solve_multi_device(self.h, int(nrhs), addr(d_f), addr(d_x0), C.byref(o), addr(d_u), infos))

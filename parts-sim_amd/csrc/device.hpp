@@ -82,7 +82,8 @@ enum ApplyMode { AM_APPLY = 0, AM_RAW = 1, AM_RESID = 2, AM_CHEB1 = 3, AM_CHEB2 
 // OF_CHILD: every level that has the child-scale form uses it; OF_AUTO: only levels where it is the
 // cheaper form (most own elements two-valued; a level of general own elements streams its stored
 // matrices faster than it evaluates 8 children per element)
-enum OpForm { OF_CHILD = 1, OF_MFMA = 2, OF_AUTO = 4 };
+// OF_MFMA_OWN: FP32 levels with stored own matrices take the matrix-core element product
+enum OpForm { OF_CHILD = 1, OF_MFMA = 2, OF_AUTO = 4, OF_MFMA_OWN = 8 };
 
 // y = K x on a level with the epilogue of `mode` (T = double or float):
 //   AM_APPLY  y = K x, held DOFs 0               AM_RAW   y = K x
@@ -96,6 +97,11 @@ template <class T> void launchApply(const DevLevel& L, int mode, const T* x, T* 
 // the same with 8 lanes per node (one per voxel): levels with own matrices
 template <class T> void launchApplyOwn(const DevLevel& L, int mode, const T* x, T* y, const T* r, const T* zp, double c1, double c2, const int* stop,
                                        hipStream_t s, int ninst = 1);
+// the same product on the matrix cores (FP32 vectors, BF16 operands on the element-corner differences,
+// FP32 accumulation; one wavefront per node, 16 instances per pass): the mixed V-cycle's smoothing on
+// levels with own matrices when asked (OF_MFMA_OWN)
+void launchApplyOwnMf(const DevLevel& L, int mode, const float* x, float* y, const float* r, const float* zp, double c1, double c2, const int* stop,
+                      hipStream_t s, int ninst = 1);
 // the same for levels whose elements are s * Kb, or own elements in child-scale form: z-marching
 // LDS planes, class-sum stencil in difference form + one lane per differing voxel (fine.hip)
 // mf: FP32 levels without own elements take the in-plane stencil on the matrix cores (BF16 operands,

@@ -344,6 +344,105 @@ __global__ __launch_bounds__(BLOCK) void kApplyOwn(DevLevel L, const T* __restri
     }
 }
 
+// The own-element product on the matrix cores (FP32 vectors: the mixed V-cycle's smoothing steps on
+// levels with own element matrices), one wavefront per node and up to 16 instances per pass. For each
+// of the node's 8 voxels one v_mfma_f32_16x16x32_bf16 accumulates D += A B: A = the node's 3 rows of the
+// voxel's element matrix (s Kb or the own matrix; rows 3..15 and columns 24..31 zero) and B = the
+// element's 24 corner values minus the node's own, one instance per column. The element rows sum to
+// zero over a rigid translation, so the differences carry the product (the BF16 rounding acts on them,
+// not on the values). BF16 operands, FP32 accumulation. Lane l holds A row l % 16 and B column l % 16
+// over the K slice 8 (l / 16) .. + 7; after the 8 voxels lanes 0..15 hold D rows 0..3 of column l, the
+// node's 3 rows for instance l. An instance's column is computed the same way for any instance count,
+// so a batch equals its lone solves bit for bit.
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float floatx4_t __attribute__((ext_vector_type(4)));
+
+template <int MODE>
+__global__ __launch_bounds__(BLOCK) void kApplyOwnMf(DevLevel L, const float* __restrict__ x, float* __restrict__ y, const float* __restrict__ r,
+                                                    const float* __restrict__ zp, float c1, float c2, const int* stop, int ninst) {
+    __shared__ float Kb[576];
+    for (int q = threadIdx.x; q < 576; q += BLOCK) Kb[q] = L.Kb32[q];
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const int64_t n = int64_t(blockIdx.x) * (BLOCK / 64) + (threadIdx.x >> 6);
+    if (n >= L.nNodes) return;  // the whole wavefront
+    const unsigned nn = unsigned(n), NXu = unsigned(L.NX), NYu = unsigned(L.NY);
+    const int i = int(nn % NXu), j = int((nn / NXu) % NYu), k = int(nn / (NXu * NYu));
+    const int64_t NX = L.NX, NXY = int64_t(L.NX) * L.NY;
+    const int col = lane & 15, kb = lane >> 4;
+    for (int g0 = 0; g0 < ninst; g0 += 16) {
+        const int inst = g0 + col;
+        const bool act = inst < ninst && !stopped(stop ? stop + int64_t(inst) * CG_INTS : nullptr);
+        const int64_t io = int64_t(act ? inst : 0) * L.nDof;
+        const float* __restrict__ xi = x + io;
+        float xn[3] = {0.f, 0.f, 0.f};
+        if (act) {
+            xn[0] = xi[3 * n];
+            xn[1] = xi[3 * n + 1];
+            xn[2] = xi[3 * n + 2];
+        }
+        floatx4_t acc = {0.f, 0.f, 0.f, 0.f};
+        for (int v = 0; v < 8; v++) {  // wave-uniform: one node per wavefront
+            const int di = v & 1, dj = (v >> 1) & 1, dk = v >> 2;
+            const int ei = i - 1 + di, ej = j - 1 + dj, ek = k - 1 + dk;
+            if (ei < 0 || ej < 0 || ek < 0 || ei >= L.nx || ej >= L.ny || ek >= L.nz) continue;
+            const int32_t w = L.words[ei + int64_t(L.nx) * (ej + int64_t(L.ny) * ek)];
+            if (w == 0) continue;
+            const int a = d_corner(1 - di, 1 - dj, 1 - dk);
+            float ta[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+            if (col < 3 && kb < 3) {
+                const int at = (3 * a + col) * 24 + 8 * kb;  // 16-byte aligned (rows of 24 floats)
+                if (w > 0) {
+                    const float s = __int_as_float(w);
+#pragma unroll
+                    for (int t = 0; t < 8; t++) ta[t] = s * Kb[at + t];
+                } else {
+                    const float4* __restrict__ K4 = reinterpret_cast<const float4*>(L.Kown + int64_t(w & 0x7fffffff) * 576 + at);
+                    const float4 p = K4[0], q = K4[1];
+                    ta[0] = p.x, ta[1] = p.y, ta[2] = p.z, ta[3] = p.w, ta[4] = q.x, ta[5] = q.y, ta[6] = q.z, ta[7] = q.w;
+                }
+            }
+            float tb[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+            if (act && kb < 3) {
+                const int64_t nb = ei + NX * ej + NXY * ek;  // the element's corner-0 node
+#pragma unroll
+                for (int t = 0; t < 8; t++) {
+                    // K index 8 kb + t = corner q, component cc
+                    const int kk = 8 * kb + t, q = kk / 3, cc = kk - 3 * q;
+                    const int64_t node = nb + kCX(q) + kCY(q) * NX + kCZ(q) * NXY;
+                    tb[t] = xi[3 * node + cc] - xn[cc];
+                }
+            }
+            bf16x8_t A, B;
+#pragma unroll
+            for (int t = 0; t < 8; t++) {
+                A[t] = (__bf16)ta[t];
+                B[t] = (__bf16)tb[t];
+            }
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B, acc, 0, 0, 0);
+        }
+        if (kb == 0 && act) {
+            float out[3] = {acc[0], acc[1], acc[2]};
+            if (L.diagAdd) {
+#pragma unroll
+                for (int d = 0; d < 3; d++) out[d] += float(L.diagAdd[3 * n + d]) * xn[d];
+            }
+            const float* __restrict__ idv = L.invD32 + 3 * n;
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                const float id = idv[d];
+                float o;
+                if constexpr (MODE == AM_APPLY) o = id != 0.f ? out[d] : 0.f;
+                else if constexpr (MODE == AM_RAW) o = out[d];
+                else if constexpr (MODE == AM_RESID) o = id != 0.f ? r[io + 3 * n + d] - out[d] : 0.f;
+                else if constexpr (MODE == AM_CHEB1) o = xn[d] + c1 * id * (r[io + 3 * n + d] - out[d]);
+                else o = xn[d] + c1 * (zp ? xn[d] - zp[io + 3 * n + d] : xn[d]) + c2 * id * (r[io + 3 * n + d] - out[d]);
+                y[io + 3 * n + d] = o;
+            }
+        }
+    }
+}
+
 template <class T> __global__ __launch_bounds__(BLOCK) void kChebFirst(const T* __restrict__ invD, const T* __restrict__ r, T* __restrict__ z, T c0, int64_t n,
                                                                        const int* stop) {
     stop = instFlag(stop);
@@ -830,6 +929,19 @@ void launchApplyOwn(const DevLevel& L, int mode, const T* x, T* y, const T* r, c
         case AM_RESID: kApplyOwn<T, AM_RESID><<<g, b, 0, s>>>(L, x, y, r, d, T(c1), T(c2), stop, ninst); break;
         case AM_CHEB1: kApplyOwn<T, AM_CHEB1><<<g, b, 0, s>>>(L, x, y, r, d, T(c1), T(c2), stop, ninst); break;
         default: kApplyOwn<T, AM_CHEB2><<<g, b, 0, s>>>(L, x, y, r, d, T(c1), T(c2), stop, ninst); break;
+    }
+}
+
+void launchApplyOwnMf(const DevLevel& L, int mode, const float* x, float* y, const float* r, const float* d, double c1, double c2, const int* stop,
+                      hipStream_t s, int ninst) {
+    const dim3 g(unsigned((L.nNodes + BLOCK / 64 - 1) / (BLOCK / 64))), b(BLOCK);
+    const float f1 = float(c1), f2 = float(c2);
+    switch (mode) {
+        case AM_APPLY: kApplyOwnMf<AM_APPLY><<<g, b, 0, s>>>(L, x, y, r, d, f1, f2, stop, ninst); break;
+        case AM_RAW: kApplyOwnMf<AM_RAW><<<g, b, 0, s>>>(L, x, y, r, d, f1, f2, stop, ninst); break;
+        case AM_RESID: kApplyOwnMf<AM_RESID><<<g, b, 0, s>>>(L, x, y, r, d, f1, f2, stop, ninst); break;
+        case AM_CHEB1: kApplyOwnMf<AM_CHEB1><<<g, b, 0, s>>>(L, x, y, r, d, f1, f2, stop, ninst); break;
+        default: kApplyOwnMf<AM_CHEB2><<<g, b, 0, s>>>(L, x, y, r, d, f1, f2, stop, ninst); break;
     }
 }
 

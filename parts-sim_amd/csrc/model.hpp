@@ -377,7 +377,12 @@ public:
             } else {
                 launchFine<T>(D, mode, xx, yy, rr, dd, c1, c2, stop, stream, ni, mf);
             }
-        } else launchApplyOwn<T>(D, mode, xx, yy, rr, dd, c1, c2, stop, stream, ni);
+        } else if constexpr (sizeof(T) == 4) {
+            if (form & OF_MFMA_OWN) launchApplyOwnMf(D, mode, xx, yy, rr, dd, c1, c2, stop, stream, ni);
+            else launchApplyOwn<T>(D, mode, xx, yy, rr, dd, c1, c2, stop, stream, ni);
+        } else {
+            launchApplyOwn<T>(D, mode, xx, yy, rr, dd, c1, c2, stop, stream, ni);
+        }
         if (fd) dotAfter<T>(*fd, D.nDof, xx, yy, rr);
     }
     template <class T> void dotAfter(const DotFusion& fd, int64_t n, const T* xx, const T* yy, const T* rr) {
@@ -569,7 +574,10 @@ public:
         const int batch = o.check_every > 0 ? o.check_every : 8;
         // the reference's algorithm keeps its operator; the FP64 and mixed solves use the exact
         // child-scale form, and the mixed smoother the matrix cores when asked (one right-hand side)
-        const int form = prec == PSX_PREC_REF ? 0 : (OF_AUTO | (prec == PSX_PREC_MIXED && o.mfma_smoother ? OF_MFMA : 0));
+        // mfma_smoother bit 0: the level-0 in-plane stencil on the matrix cores; bit 1: own-element levels
+        const int form = prec == PSX_PREC_REF ? 0
+                                              : (OF_AUTO | (prec == PSX_PREC_MIXED && (o.mfma_smoother & 1) ? OF_MFMA : 0) |
+                                                 (prec == PSX_PREC_MIXED && (o.mfma_smoother & 2) ? OF_MFMA_OWN : 0));
         const int degree = o.cheb_degree > 0 ? o.cheb_degree : 2;
         if (degree > CHEB_MAX) throw std::invalid_argument("Chebyshev degree above 6");
         preDegree = o.pre_degree > 0 ? std::min(o.pre_degree, degree) : 0;
@@ -629,7 +637,7 @@ public:
                     const float* z = vcycle<float>(0, r32, stop, form, degree, &fz, fuseFirst);
                     launchUpdateP<float>(z, p32, n, st, stream, k);
                     // the outer operator stays FP32 (the matrix-core form is for the smoother only)
-                    op<float>(0, AM_APPLY, p32, q32, nullptr, nullptr, 0, 0, stop, form & ~OF_MFMA, &fp);
+                    op<float>(0, AM_APPLY, p32, q32, nullptr, nullptr, 0, 0, stop, form & ~(OF_MFMA | OF_MFMA_OWN), &fp);
                     launchUpdateXR32Fold(x32, p32, r32, q32, n, st, red, stream, k, fuseFirst ? lv[0].a32 : nullptr, D0.invD32, c0first);
                     // reliable update: the correction joins the FP64 solution and the FP64 true
                     // residual replaces the recurrence's

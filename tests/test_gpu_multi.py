@@ -112,3 +112,27 @@ def test_block_precondition_matches_oracle_per_vector(P, W, cpu):
             assert err < 1e-4
             assert not Z[k][fixed != 0].any()
         m.close()
+
+
+@pytest.mark.parametrize("nrhs", [5, 16])
+def test_matrix_core_own_elements(P, W, cpu, nrhs):
+    # own-element levels on the matrix cores (mfma_smoother bit 2): each load case's column of the
+    # contraction is computed the same way for any batch, so a batch equals its lone solves bit for
+    # bit; the BF16 smoother changes only the preconditioner, so the solution still meets the FP64
+    # oracle at a tight tolerance
+    w = W.simp_block((24, 12, 12), seed=3, nrhs=nrhs)
+    F = w["f"]
+    m = _model(P, w)
+    U, infos = m.solve_multi(F, tol=1e-10, precision="mixed", mfma_smoother=2)
+    ref = cpu.Model(w["dims"], w["density"], w["nu"], w["bc"], 300)
+    for k in (0, nrhs - 1):
+        lone = m.solve(F[k], tol=1e-10, precision="mixed", mfma_smoother=2)
+        print(nrhs, k, infos[k]["iterations"], lone["iterations"], flush=True)
+        assert infos[k]["converged"] and infos[k]["iterations"] == lone["iterations"]
+        assert np.array_equal(U[k], lone["u"])
+        r = ref.solve(F[k], tol=1e-12)
+        assert np.linalg.norm(U[k] - r["u"]) / np.linalg.norm(r["u"]) < 1e-8
+    # the FP32 VALU smoother on the same batch: iteration counts within a few
+    V, infos32 = m.solve_multi(F, tol=1e-10, precision="mixed")
+    assert max(abs(a["iterations"] - b["iterations"]) for a, b in zip(infos, infos32)) <= 3
+    m.close()
