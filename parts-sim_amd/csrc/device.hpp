@@ -16,6 +16,10 @@ struct DevLevel {
     int64_t nNodes, nDof;
     const int32_t* words;      // per voxel: 0 empty, >0 float bits of s (s * Kb), <0 0x80000000 | own index
     const float* Kown;         // own element matrices, 576 floats each
+    // the same in FP16 for the mixed V-cycle's smoothing, each element divided by a power of two
+    // (ownScale) that puts its largest entry in [2^13, 2^14) (null on levels without own matrices)
+    const uint16_t* KownH;
+    const float* ownScale;
     const double* Kb64;        // the level's uniform element matrix
     const float* Kb32;
     const double* St64;        // 27-point stencil of Kb: [27 neighbours][3 rows][3 cols]
@@ -95,8 +99,11 @@ enum OpForm { OF_CHILD = 1, OF_MFMA = 2, OF_AUTO = 4, OF_MFMA_OWN = 8 };
 template <class T> void launchApply(const DevLevel& L, int mode, const T* x, T* y, const T* r, const T* zp, double c1, double c2, const int* stop,
                                     hipStream_t s, int ninst = 1);
 // the same with 8 lanes per node (one per voxel): levels with own matrices
+// half: read the FP16 own matrices (FP32 vectors only)
 template <class T> void launchApplyOwn(const DevLevel& L, int mode, const T* x, T* y, const T* r, const T* zp, double c1, double c2, const int* stop,
-                                       hipStream_t s, int ninst = 1);
+                                       hipStream_t s, int ninst = 1, bool half = false);
+// FP16 copies of n own matrices with their per-element power-of-two scales
+void launchOwnToHalf(const float* Kown, int64_t n, uint16_t* half, float* scale, hipStream_t s);
 // the same product on the matrix cores (FP32 vectors, BF16 operands on the element-corner differences,
 // FP32 accumulation; one wavefront per node, 16 instances per pass): the mixed V-cycle's smoothing on
 // levels with own matrices when asked (OF_MFMA_OWN)

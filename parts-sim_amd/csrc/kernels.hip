@@ -13,6 +13,7 @@
 //
 // Every solve kernel also serves multi-instance launches (gridDim.y right-hand sides of one level):
 // blockIdx.y selects the instance's slice of each vector and its CgState.
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
 #include "device.hpp"
@@ -236,7 +237,7 @@ __global__ __launch_bounds__(BLOCK) void kApply(DevLevel L, const T* __restrict_
 // (3 rows x 24 columns from s * Kb in LDS or from the voxel's own matrix), the 8 partial rows are
 // summed with shuffles inside the aligned 8-lane group, lane 0 writes. Coarse levels are small:
 // this spreads one node's dependent loads over 8 lanes instead of one thread walking 8 voxels.
-template <class T, int MODE>
+template <class T, int MODE, bool H16 = false>
 __global__ __launch_bounds__(BLOCK) void kApplyOwn(DevLevel L, const T* __restrict__ x, T* __restrict__ y, const T* __restrict__ r, const T* __restrict__ zp,
                                                    T c1, T c2, const int* stop, int ninst) {
     // Several right-hand sides: this lane's element rows (72 coefficients) are loaded once and
@@ -266,11 +267,27 @@ __global__ __launch_bounds__(BLOCK) void kApplyOwn(DevLevel L, const T* __restri
     }
     // the node's 3 rows of the voxel's element: s * Kb (scaled after the sum) or the own matrix
     T K[72];
-    const T s = w > 0 ? T(__int_as_float(w)) : T(1);
+    // s scales the summed rows: s for s * Kb, 1 for a stored own matrix, its power-of-two scale for an
+    // FP16 one
+    T s = w > 0 ? T(__int_as_float(w)) : T(1);
     if (has) {
         if (w > 0) {
 #pragma unroll
             for (int c = 0; c < 72; c++) K[c] = Kb[72 * a + c];
+        } else if constexpr (H16) {
+            const int64_t e = int64_t(w & 0x7fffffff);
+            const uint4* __restrict__ H4 = reinterpret_cast<const uint4*>(L.KownH + e * 576 + 72 * a);  // 144-byte rows: 16-byte aligned
+            s = T(L.ownScale[e]);
+#pragma unroll
+            for (int q = 0; q < 9; q++) {
+                const uint4 u = H4[q];
+                const uint32_t wd[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+                for (int h = 0; h < 4; h++) {
+                    K[8 * q + 2 * h] = T(__half2float(__ushort_as_half(uint16_t(wd[h] & 0xffffu))));
+                    K[8 * q + 2 * h + 1] = T(__half2float(__ushort_as_half(uint16_t(wd[h] >> 16))));
+                }
+            }
         } else {
             const float4* __restrict__ K4 = reinterpret_cast<const float4*>(L.Kown + int64_t(w & 0x7fffffff) * 576 + 72 * a);
 #pragma unroll
@@ -441,6 +458,23 @@ __global__ __launch_bounds__(BLOCK) void kApplyOwnMf(DevLevel L, const float* __
             }
         }
     }
+}
+
+// FP16 copy of the own matrices, one wavefront per element: the element's largest magnitude sets a
+// power-of-two scale 2^e with max / 2^e in (2^13, 2^14] (the rule of the reference's GPU solver,
+// gpu_fea.cpp:291-299, per element instead of per level so soft elements keep their precision)
+__global__ __launch_bounds__(64) void kOwnToHalf(const float* __restrict__ Kown, int64_t n, uint16_t* __restrict__ half, float* __restrict__ scale) {
+    const int64_t e = blockIdx.x;
+    if (e >= n) return;
+    const float* __restrict__ K = Kown + e * 576;
+    float m = 0.f;
+    for (int q = threadIdx.x; q < 576; q += 64) m = fmaxf(m, fabsf(K[q]));
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+    const float sc = m > 0.f ? exp2f(ceilf(log2f(m / 16384.f))) : 1.f;
+    const float inv = 1.f / sc;  // exact: a power of two
+    for (int q = threadIdx.x; q < 576; q += 64) half[e * 576 + q] = __half_as_ushort(__float2half_rn(K[q] * inv));
+    if (threadIdx.x == 0) scale[e] = sc;
 }
 
 template <class T> __global__ __launch_bounds__(BLOCK) void kChebFirst(const T* __restrict__ invD, const T* __restrict__ r, T* __restrict__ z, T c0, int64_t n,
@@ -920,9 +954,22 @@ void launchApply(const DevLevel& L, int mode, const T* x, T* y, const T* r, cons
 }
 
 template <class T>
-void launchApplyOwn(const DevLevel& L, int mode, const T* x, T* y, const T* r, const T* d, double c1, double c2, const int* stop, hipStream_t s, int ninst) {
+void launchApplyOwn(const DevLevel& L, int mode, const T* x, T* y, const T* r, const T* d, double c1, double c2, const int* stop, hipStream_t s, int ninst,
+                    bool half) {
     // every instance in one grid: the kernel loops over them (the element rows are read once)
     const dim3 g(unsigned((L.nNodes + BLOCK / 8 - 1) / (BLOCK / 8))), b(BLOCK);
+    if constexpr (sizeof(T) == 4) {
+        if (half && L.KownH) {
+            switch (mode) {
+                case AM_APPLY: kApplyOwn<T, AM_APPLY, true><<<g, b, 0, s>>>(L, x, y, r, d, T(c1), T(c2), stop, ninst); break;
+                case AM_RAW: kApplyOwn<T, AM_RAW, true><<<g, b, 0, s>>>(L, x, y, r, d, T(c1), T(c2), stop, ninst); break;
+                case AM_RESID: kApplyOwn<T, AM_RESID, true><<<g, b, 0, s>>>(L, x, y, r, d, T(c1), T(c2), stop, ninst); break;
+                case AM_CHEB1: kApplyOwn<T, AM_CHEB1, true><<<g, b, 0, s>>>(L, x, y, r, d, T(c1), T(c2), stop, ninst); break;
+                default: kApplyOwn<T, AM_CHEB2, true><<<g, b, 0, s>>>(L, x, y, r, d, T(c1), T(c2), stop, ninst); break;
+            }
+            return;
+        }
+    }
     switch (mode) {
         case AM_APPLY: kApplyOwn<T, AM_APPLY><<<g, b, 0, s>>>(L, x, y, r, d, T(c1), T(c2), stop, ninst); break;
         case AM_RAW: kApplyOwn<T, AM_RAW><<<g, b, 0, s>>>(L, x, y, r, d, T(c1), T(c2), stop, ninst); break;
@@ -930,6 +977,10 @@ void launchApplyOwn(const DevLevel& L, int mode, const T* x, T* y, const T* r, c
         case AM_CHEB1: kApplyOwn<T, AM_CHEB1><<<g, b, 0, s>>>(L, x, y, r, d, T(c1), T(c2), stop, ninst); break;
         default: kApplyOwn<T, AM_CHEB2><<<g, b, 0, s>>>(L, x, y, r, d, T(c1), T(c2), stop, ninst); break;
     }
+}
+
+void launchOwnToHalf(const float* Kown, int64_t n, uint16_t* half, float* scale, hipStream_t s) {
+    if (n > 0) kOwnToHalf<<<unsigned(n), 64, 0, s>>>(Kown, n, half, scale);
 }
 
 void launchApplyOwnMf(const DevLevel& L, int mode, const float* x, float* y, const float* r, const float* d, double c1, double c2, const int* stop,
@@ -1046,7 +1097,7 @@ void launchLzNext(const double* w, double* v, double* vPrev, int64_t n, CgState*
 #define PSX_INSTANTIATE(T)                                                                                                                   \
     template void launchApply<T>(const DevLevel&, int, const T*, T*, const T*, const T*, double, double, const int*, hipStream_t, int);     \
     template void launchChebFirst<T>(const DevLevel&, const T*, T*, double, const int*, hipStream_t, int);                                  \
-    template void launchApplyOwn<T>(const DevLevel&, int, const T*, T*, const T*, const T*, double, double, const int*, hipStream_t, int);  \
+    template void launchApplyOwn<T>(const DevLevel&, int, const T*, T*, const T*, const T*, double, double, const int*, hipStream_t, int, bool); \
     template void launchRestrict<T>(const DevLevel&, const DevLevel&, const T*, T*, const int*, hipStream_t, int);                         \
     template void launchProlong<T>(const DevLevel&, const DevLevel&, const T*, T*, const int*, hipStream_t, int);                          \
     template void launchCoarse<T>(const int32_t*, const int32_t*, int64_t, int64_t, const double*, const T*, T*, const int*, hipStream_t, int); \

@@ -129,6 +129,7 @@ public:
             HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
             try {
                 buildOnDevice(*this, d);
+                halfOwn();
             } catch (...) {
                 release();
                 throw;
@@ -138,6 +139,21 @@ public:
         buildHierarchy(H, d.nx, d.ny, d.nz, d.density, d.nu, d.bc, d.coarsest_max_dof > 0 ? d.coarsest_max_dof : 1100, d.diag_add, d.max_levels);
         HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         upload();
+        halfOwn();
+    }
+    // FP16 copies of the stored own matrices (the mixed V-cycle's smoothing reads them, OF_AUTO)
+    void halfOwn() {
+        for (size_t l = 0; l < lv.size(); l++) {
+            DevLevel& D = lv[l].dl;
+            const int64_t nOwn = H.levels[l].nOwn;
+            if (nOwn <= 0 || !D.Kown) continue;
+            uint16_t* kh = alloc<uint16_t>(nOwn * 576);
+            float* sc = alloc<float>(nOwn);
+            launchOwnToHalf(D.Kown, nOwn, kh, sc, stream);
+            HIPCHK(hipGetLastError());
+            D.KownH = kh;
+            D.ownScale = sc;
+        }
     }
     // Device buffers come from the stream-ordered allocator: plain hipFree waits for the whole
     // device, which would stall every other model's stream (concurrent serving requests).
@@ -379,7 +395,7 @@ public:
             }
         } else if constexpr (sizeof(T) == 4) {
             if (form & OF_MFMA_OWN) launchApplyOwnMf(D, mode, xx, yy, rr, dd, c1, c2, stop, stream, ni);
-            else launchApplyOwn<T>(D, mode, xx, yy, rr, dd, c1, c2, stop, stream, ni);
+            else launchApplyOwn<T>(D, mode, xx, yy, rr, dd, c1, c2, stop, stream, ni, (form & OF_AUTO) != 0);
         } else {
             launchApplyOwn<T>(D, mode, xx, yy, rr, dd, c1, c2, stop, stream, ni);
         }
