@@ -134,14 +134,16 @@ def W_nodes(dims):
     return int(np.prod([int(d) + 1 for d in dims]))
 
 
-def serving_leg(P, W, n_requests, rate, inflight, tol):
+def serving_leg(P, W, n_requests, rate, inflight, tol, pack_max=0, pack_max_dof=0, edge_max=256):
     """Serving workload: a seeded Poisson stream of void-block parts (edges log-uniform 32-256 voxels,
     aspect 1-4, nu U[0.28, 0.36], 1-8 load cases, 30 % warm re-solves of an earlier part with 1 % of
     its solid voxels removed and floating voxels pruned, warm-started from the earlier solution)
-    through the device-resident request loop; latency is arrival -> result. Returns the summary."""
+    through the device-resident request loop; latency is arrival -> result. Parts of at most
+    pack_max_dof DOFs (0: the server's default) run in its shared-launch pack of up to pack_max
+    requests (1: no packing). Returns the summary."""
     import threading
-    reqs = W.serving_stream(n_requests, seed=5, rate=rate)
-    srv = P.Server(max_inflight=inflight)
+    reqs = W.serving_stream(n_requests, seed=5, rate=rate, edge_max=edge_max)
+    srv = P.Server(max_inflight=inflight, pack_max=pack_max, pack_max_dof=pack_max_dof)
     parts, results, lock = {}, {}, threading.Lock()
     done_evt = {r["id"]: threading.Event() for r in reqs}
 
@@ -201,12 +203,13 @@ def serving_leg(P, W, n_requests, rate, inflight, tol):
            "mean_build_ms": round(float(np.mean([res["build_ms"] for r, res, t, u in results.values()])), 1),
            "mean_solve_ms": round(float(np.mean([res["solve_ms"] for r, res, t, u in results.values()])), 1),
            "mean_queue_ms": round(float(np.mean([res["queue_ms"] for r, res, t, u in results.values()])), 1),
-           "dof_total": int(sum(3 * W.nodes(r["dims"]) * r["nrhs"] for r in reqs)), "tol": tol}
+           "packed_requests": int(sum(res["packed"] for r, res, t, u in results.values())), "pack_max": pack_max, "pack_max_dof": pack_max_dof,
+           "dof_total": int(sum(3 * W.nodes(r["dims"]) * r["nrhs"] for r in reqs)), "tol": tol, "edge_max": edge_max}
     return out
 
 
 def serve(args, P, W):
-    print(json.dumps(serving_leg(P, W, args.serve, args.rate, args.inflight, args.tol)), flush=True)
+    print(json.dumps(serving_leg(P, W, args.serve, args.rate, args.inflight, args.tol, args.pack_max, args.pack_max_dof, args.edge_max)), flush=True)
 
 
 def batched(args, P, W):
@@ -340,7 +343,10 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=40, help="requests of the default line's serving leg")
     ap.add_argument("--serve", type=int, default=0, help="serving workload: this many requests of the Poisson stream")
     ap.add_argument("--rate", type=float, default=200.0, help="serving arrival rate (requests/s)")
-    ap.add_argument("--inflight", type=int, default=4, help="serving: requests on the device at once")
+    ap.add_argument("--inflight", type=int, default=4, help="serving: admission workers (requests built or solved alone at once)")
+    ap.add_argument("--pack-max", type=int, default=0, help="serving: requests in the shared-launch pack (0 = the server's 16, 1 = no packing)")
+    ap.add_argument("--edge-max", type=int, default=256, help="serving: longest part edge of the stream (voxels)")
+    ap.add_argument("--pack-max-dof", type=int, default=0, help="serving: largest part (DOFs) that joins the pack (0 = the server's 100 000)")
     ap.add_argument("--batched", action="store_true", help="batched load cases: the 128x64x64 SIMP block, --nrhs cases in one multi-instance solve")
     ap.add_argument("--nrhs", type=int, default=16)
     ap.add_argument("--profile-solves", default="", choices=["", "mixed", "fp64", "ref"],
@@ -458,7 +464,8 @@ def main():
         log("C3 as specified (degree 3, BF16 smoother): %.2f ms/solve, %d its" % (1e3 * lat3.mean(), its3[0]))
         m.close()
         for name, fn in (("batched", lambda: batched_leg(P, W, (128, 64, 64), 16, max(3, args.steps // 2), 1, args.tol)),
-                         ("serving", lambda: serving_leg(P, W, args.serve_requests, args.rate, args.inflight, args.tol))):
+                         ("serving", lambda: serving_leg(P, W, args.serve_requests, args.rate, args.inflight, args.tol, args.pack_max,
+                                                         args.pack_max_dof))):
             t = time.perf_counter()
             try:
                 extras[name] = fn()

@@ -177,6 +177,18 @@ int psx_solve_multi(psx_model m, int nrhs, const double* f, const double* x0, co
 int psx_solve_multi_device(psx_model m, int nrhs, const double* d_f, const double* d_x0, const psx_solve_opts* opts, double* d_u,
                            psx_solve_info* info);
 
+/* Load cases of several models in one set of launches: model i solves nrhs[i] load cases (d_f[i],
+ * d_u[i] and, when d_x0 and d_x0[i] are not NULL, d_x0[i]: nrhs[i] x its n_dof, device memory), and
+ * each step of an iteration is one launch for every model whose run takes it (the grids of the
+ * models' own launches side by side). Each model's solutions, iterations and residuals are bit for
+ * bit those of psx_solve_multi_device on it alone; runs leave the pack as they stop. info receives
+ * sum(nrhs) entries, model-major. At most 64 distinct models; opts->on_progress is not called. No
+ * reference counterpart: the reference solves one part at a time (gpu_fea.cpp:452-574); for the
+ * many small parts of a serving stream, whose kernels cannot fill the device, this pays one launch
+ * per step for the whole pack. */
+int psx_solve_packed(int n_models, const psx_model* models, const int* nrhs, const double* const* d_f, const double* const* d_x0,
+                     const psx_solve_opts* opts, double* const* d_u, psx_solve_info* info);
+
 /* Grid enclosing a box with n voxels along its longest side plus pad voxels each side (gridForBox,
  * voxelize.cpp:14-23): origin, voxel size h and dims. */
 int psx_grid_for_box(const double mn[3], const double mx[3], int n, int pad, double origin[3], double* h, int dims[3]);
@@ -316,13 +328,17 @@ int64_t psx_device_bytes(psx_model m);
  * No reference counterpart: the reference solves one part at a time behind a global GPU lock
  * (gpu.hpp:43, gpu_fea.cpp:256-578). A server admits queued solve requests in arrival order as
  * slots (max_inflight) and device memory (hbm_budget) free up; each admitted request builds its
- * part's hierarchy on the host, uploads it and solves its load cases on its own HIP stream, so
- * requests that arrive while others run overlap on the device. Every result is the same as the
- * request's solve alone (per-request deterministic reductions, no shared device state). */
+ * part's hierarchy (on the device unless asked otherwise). Parts of at most pack_max_dof DOFs join
+ * the pack, whose runs advance together with one shared launch per step for all of them (as
+ * psx_solve_packed); larger parts solve on their own HIP stream, overlapping with the rest. Every
+ * result is the same as the request's solve alone (per-request deterministic reductions, no shared
+ * device state). */
 typedef struct psx_server_s* psx_server;
 typedef struct {
-    int max_inflight;           /* requests on the device at once (0 = 4) */
+    int max_inflight;           /* admission workers: requests built or solved alone at once (0 = 4) */
     int64_t hbm_budget;         /* device bytes requests may hold at once (0 = 80 % of the free memory) */
+    int pack_max;               /* requests in the pack at once (0 = 16, 1 = no packing, at most 64) */
+    int64_t pack_max_dof;       /* largest part that joins the pack, in DOFs (0 = 100 000) */
 } psx_server_opts;
 typedef struct {
     psx_model_desc part;        /* copied at submit */
@@ -346,6 +362,7 @@ typedef struct {
     double total_ms;            /* submit -> done */
     int64_t device_bytes;       /* the model's device memory */
     int reused_part;            /* the model was resident */
+    int packed;                 /* solved in the pack (shared launches) */
 } psx_result;
 int psx_server_create(const psx_server_opts* opts, psx_server* out);
 /* Queues a request (non-blocking); ticket identifies it for psx_server_wait. */

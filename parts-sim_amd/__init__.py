@@ -70,7 +70,7 @@ BREAK_REASONS = ("step limit", "separated", "unsupported load", "solver did not 
 
 
 class ServerOpts(C.Structure):
-    _fields_ = [("max_inflight", C.c_int), ("hbm_budget", C.c_int64)]
+    _fields_ = [("max_inflight", C.c_int), ("hbm_budget", C.c_int64), ("pack_max", C.c_int), ("pack_max_dof", C.c_int64)]
 
 
 class Request(C.Structure):
@@ -81,7 +81,7 @@ class Request(C.Structure):
 class Result(C.Structure):
     _fields_ = [("status", C.c_int), ("iterations", C.c_int), ("max_iterations", C.c_int), ("converged", C.c_int), ("residual", C.c_double),
                 ("queue_ms", C.c_double), ("build_ms", C.c_double), ("solve_ms", C.c_double), ("total_ms", C.c_double),
-                ("device_bytes", C.c_int64), ("reused_part", C.c_int)]
+                ("device_bytes", C.c_int64), ("reused_part", C.c_int), ("packed", C.c_int)]
 
 
 class SolveInfo(C.Structure):
@@ -149,6 +149,7 @@ SIGNATURES = {
     "psx_solve_device": (_i, [_vp, _vp, _vp, C.POINTER(SolveOpts), _vp, C.POINTER(SolveInfo)]),
     "psx_solve_multi": (_i, [_vp, _i, _vp, _vp, C.POINTER(SolveOpts), _vp, C.POINTER(SolveInfo)]),
     "psx_solve_multi_device": (_i, [_vp, _i, _vp, _vp, C.POINTER(SolveOpts), _vp, C.POINTER(SolveInfo)]),
+    "psx_solve_packed": (_i, [_i, _vp, _vp, _vp, _vp, C.POINTER(SolveOpts), _vp, C.POINTER(SolveInfo)]),
     "psx_reactions": (_i, [_vp, _vp, _vp, _vp]),
     "psx_stresses": (_i, [_vp, _vp, _d, _d, _i, _vp, _vp, _vp]),
     "psx_stresses_device": (_i, [_vp, _vp, _d, _d, _i, _vp, _vp, _vp]),
@@ -775,9 +776,11 @@ class Server:
     """A device-resident request loop (include/psx.h psx_server_*): requests are admitted as slots and
     device memory free up and run concurrently, each on its own stream."""
 
-    def __init__(self, max_inflight=4, hbm_budget=0):
+    def __init__(self, max_inflight=4, hbm_budget=0, pack_max=0, pack_max_dof=0):
+        """pack_max: requests solved together in the pack (0 = 16, 1 = no packing); pack_max_dof: the
+        largest part (DOFs) that joins it (0 = 100 000)."""
         h = C.c_void_p()
-        _check(lib().psx_server_create(C.byref(ServerOpts(int(max_inflight), int(hbm_budget))), C.byref(h)))
+        _check(lib().psx_server_create(C.byref(ServerOpts(int(max_inflight), int(hbm_budget), int(pack_max), int(pack_max_dof))), C.byref(h)))
         self.h = h
         self._keep = {}
 
@@ -824,6 +827,28 @@ class Server:
 
     def __del__(self):
         self.close()
+
+
+def solve_packed(models, d_f, d_u, nrhs, d_x0=None, tol=1e-6, max_iter=500, precision="mixed", check_every=0, fold_every=0, cheb_degree=0,
+                 eager=False, mfma_smoother=0):
+    """psx_solve_packed: the load cases of several models in shared launches. models: Model list;
+    d_f, d_u (and d_x0, entries may be None): per model a device array of nrhs[i] x its nDof; returns
+    per model the list of its load cases' info dicts (each as that model's solve_multi_device alone)."""
+    n = len(models)
+    o, _ = Model._opts(tol, max_iter, precision, check_every, fold_every, None, cheb_degree, mfma_smoother, eager)
+    hs = (C.c_void_p * n)(*[m.h.value if isinstance(m.h, C.c_void_p) else m.h for m in models])
+    ks = (C.c_int * n)(*[int(k) for k in nrhs])
+    addr = lambda a: None if a is None else int(a)
+    fs = (C.c_void_p * n)(*[addr(a) for a in d_f])
+    us = (C.c_void_p * n)(*[addr(a) for a in d_u])
+    xs = None if d_x0 is None else (C.c_void_p * n)(*[addr(a) for a in d_x0])
+    infos = (SolveInfo * int(sum(ks)))()
+    _check(lib().psx_solve_packed(n, hs, ks, fs, xs, C.byref(o), us, infos))
+    out, at = [], 0
+    for k in ks:
+        out.append([Model._info(infos[at + i]) for i in range(k)])
+        at += k
+    return out
 
 
 def prune_floating_device(dims, density, held, no_alloc=False):

@@ -18,9 +18,11 @@
 
 #include <algorithm>
 #include <cstdlib>
+#include <type_traits>
 #include <utility>
 
 #include "device.hpp"
+#include "pack_dev.hpp"
 
 namespace psx {
 
@@ -150,23 +152,24 @@ template <class T, bool OWNC> struct Ring {
 // the whole march; a node plane's rows are complete one step after its upper neighbour plane and are
 // read out by lane shuffles, then cleared for the plane three steps on. The centre column term and
 // the differing-voxel pairs stay FP32, as in the VALU form.
-template <class T, int MODE, bool OWNC, bool DOT, bool DENSE = false, bool MF = false> __device__ __forceinline__ void fineBody(PSX_FINE_PARAMS) {
-    // multi-instance launches: blockIdx.y is the right-hand side (its vectors, stop flag, partials)
-    if (stop) stop += int64_t(blockIdx.y) * CG_INTS;
+template <class T, int MODE, bool OWNC, bool DOT, bool DENSE = false, bool MF = false>
+__device__ __forceinline__ void fineBody(VBlk vb, PSX_FINE_PARAMS) {
+    // multi-instance launches: the block's y is the right-hand side (its vectors, stop flag, partials)
+    if (stop) stop += int64_t(vb.y) * CG_INTS;
     if (stop && __builtin_nontemporal_load(stop) != 0) return;
     {
-        const int64_t io = int64_t(blockIdx.y) * L.nDof;
+        const int64_t io = int64_t(vb.y) * L.nDof;
         x += io;
         y += io;
         if (r) r += io;
         if (zp) zp += io;
         if (qdot) qdot += io;
-        if (DOT) dots += int64_t(blockIdx.y) * dotStride;
+        if (DOT) dots += int64_t(vb.y) * dotStride;
     }
     // XCD-aware tile order: the hardware deals consecutive blocks round-robin to the 8 XCDs, so
     // block b takes tile (b % 8) * per + b / 8 and each XCD walks one contiguous run of tiles (x
     // fastest), whose halos then meet in that XCD's L2
-    const int per = gridDim.x / 8, t = xcdOrder ? int(blockIdx.x % 8) * per + int(blockIdx.x / 8) : int(blockIdx.x);
+    const int per = int(vb.gx / 8), t = xcdOrder ? int(vb.x % 8) * per + int(vb.x / 8) : int(vb.x);
     if (t >= nTiles) return;
     __shared__ Ring<T, OWNC> sh;
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
@@ -700,18 +703,23 @@ template <class T, int MODE, bool OWNC, bool DOT, bool DENSE = false, bool MF = 
 }
 
 #define PSX_FINE_FWD L, C, crow, ctab, x, y, r, zp, c1, c2, stop, zChunk, gx, gy, nTiles, xcdOrder, dots, dotStride, qdot
+// the kernel bodies for packed launches: WPE_ as the named kernel's waves-per-SIMD attribute
+template <class T, int MODE, bool OWNC, bool DOT, bool DENSE, bool MF, int WPE_> struct Fine {
+    static constexpr int NTH = NT, WPE = WPE_;
+    __device__ static void run(VBlk vb, PSX_FINE_PARAMS) { fineBody<T, MODE, OWNC, DOT, DENSE, MF>(vb, PSX_FINE_FWD); }
+};
 template <class T, int MODE, bool OWNC, bool DOT> __global__ __launch_bounds__(NT) void kFine(PSX_FINE_PARAMS) {
-    fineBody<T, MODE, OWNC, DOT>(PSX_FINE_FWD);
+    fineBody<T, MODE, OWNC, DOT>(ownBlock(), PSX_FINE_FWD);
 }
 // FP32 without own elements: held to 128 VGPRs (4 waves per SIMD instead of 3), which it fits
 // without spilling
 template <class T, int MODE, bool DOT>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void kFine4(PSX_FINE_PARAMS) {
-    fineBody<T, MODE, false, DOT>(PSX_FINE_FWD);
+    fineBody<T, MODE, false, DOT>(ownBlock(), PSX_FINE_FWD);
 }
 // per-voxel form (levels without own elements)
 template <class T, int MODE, bool DOT> __global__ __launch_bounds__(NT) void kFineDense(PSX_FINE_PARAMS) {
-    fineBody<T, MODE, false, DOT, true>(PSX_FINE_FWD);
+    fineBody<T, MODE, false, DOT, true>(ownBlock(), PSX_FINE_FWD);
 }
 // in-plane stencil on the matrix cores (FP32 levels without own elements)
 template <int MODE, bool DOT>
@@ -722,9 +730,27 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void kF
                                                                                    int zChunk, int gx, int gy, int nTiles, int xcdOrder,
                                                                                    double* __restrict__ dots, int64_t dotStride,
                                                                                    const float* __restrict__ qdot) {
-    fineBody<float, MODE, false, DOT, false, true>(PSX_FINE_FWD);
+    fineBody<float, MODE, false, DOT, false, true>(ownBlock(), PSX_FINE_FWD);
 }
 #undef PSX_FINE_FWD
+
+// launches (or records) the operator kernel of one form, mode and dot choice
+template <class T, int MODE, bool OWNC, bool DOT, bool DENSE, bool MF>
+void fineGo(dim3 g, hipStream_t s, const DevLevel& L, const Sym<T>& C, const T* crow, const T* ctab, const T* x, T* y, const T* r, const T* zp, T c1, T c2,
+            const int* stop, int zChunk, int gx, int gy, int nTiles, int xcdOrder, double* dots, int64_t dotStride, const T* qdot) {
+#define PSX_FINE_ARGS L, C, crow, ctab, x, y, r, zp, c1, c2, stop, zChunk, gx, gy, nTiles, xcdOrder, dots, dotStride, qdot
+    if constexpr (MF) {
+        static_assert(sizeof(T) == 4 && !OWNC && !DENSE, "matrix-core form: FP32 levels without own elements");
+        go<Fine<float, MODE, false, DOT, false, true, 3>>(kFineMf<MODE, DOT>, g, s, PSX_FINE_ARGS);
+    } else if constexpr (DENSE) {
+        go<Fine<T, MODE, false, DOT, true, false, 0>>(kFineDense<T, MODE, DOT>, g, s, PSX_FINE_ARGS);
+    } else if constexpr (sizeof(T) == 4 && !OWNC) {
+        go<Fine<T, MODE, false, DOT, false, false, 4>>(kFine4<T, MODE, DOT>, g, s, PSX_FINE_ARGS);
+    } else {
+        go<Fine<T, MODE, OWNC, DOT, false, false, 0>>(kFine<T, MODE, OWNC, DOT>, g, s, PSX_FINE_ARGS);
+    }
+#undef PSX_FINE_ARGS
+}
 
 }  // namespace
 
@@ -733,7 +759,7 @@ static void launchFineT(const DevLevel& L, int mode, const Sym<T>& C, const T* c
                         const int* stop, int zChunk, double* dots, int64_t dotStride, const T* qdot, hipStream_t s, int ninst, bool mf) {
     const int gx = (L.NX + TX - 1) / TX, gy = (L.NY + TY - 1) / TY, gz = (L.NZ + zChunk - 1) / zChunk;
     const int nTiles = gx * gy * gz;
-    const dim3 g(unsigned(8 * ((nTiles + 7) / 8)), unsigned(ninst)), b(NT);  // whole rounds over the 8 XCDs
+    const dim3 g(unsigned(8 * ((nTiles + 7) / 8)), unsigned(ninst));  // whole rounds over the 8 XCDs
     // measured on the 512x128x128 void block: the XCD-contiguous order is 8 % slower than the
     // launch order for this kernel (its halos are served from the shared last-level cache), so it
     // is off unless PSX_XCD_ORDER=1
@@ -741,83 +767,45 @@ static void launchFineT(const DevLevel& L, int mode, const Sym<T>& C, const T* c
         const char* e = std::getenv("PSX_XCD_ORDER");
         return e && e[0] == '1' ? 1 : 0;
     }();
-#define PSX_FINE_ARGS L, C, crow, ctab, x, y, r, d, T(c1), T(c2), stop, zChunk, gx, gy, nTiles, xcdOrder, dots, dotStride, qdot
     // fused dots exist for APPLY (p.q) and the Chebyshev steps (r.z, z.q) only
     const bool dot = dots && (mode == AM_APPLY || mode == AM_CHEB1 || mode == AM_CHEB2);
-    if constexpr (sizeof(T) == 4 && !OWNC) {
-        if (mf && !L.dense) {
-            switch (mode) {
-                case AM_APPLY:
-                    if (dot) kFineMf<AM_APPLY, true><<<g, b, 0, s>>>(PSX_FINE_ARGS);
-                    else kFineMf<AM_APPLY, false><<<g, b, 0, s>>>(PSX_FINE_ARGS);
-                    break;
-                case AM_RAW: kFineMf<AM_RAW, false><<<g, b, 0, s>>>(PSX_FINE_ARGS); break;
-                case AM_RESID: kFineMf<AM_RESID, false><<<g, b, 0, s>>>(PSX_FINE_ARGS); break;
-                case AM_CHEB1:
-                    if (dot) kFineMf<AM_CHEB1, true><<<g, b, 0, s>>>(PSX_FINE_ARGS);
-                    else kFineMf<AM_CHEB1, false><<<g, b, 0, s>>>(PSX_FINE_ARGS);
-                    break;
-                default:
-                    if (dot) kFineMf<AM_CHEB2, true><<<g, b, 0, s>>>(PSX_FINE_ARGS);
-                    else kFineMf<AM_CHEB2, false><<<g, b, 0, s>>>(PSX_FINE_ARGS);
-                    break;
+    const bool useMf = sizeof(T) == 4 && !OWNC && mf && !L.dense, dense = !OWNC && L.dense;
+    auto form = [&](auto m, auto dt) {
+        constexpr int M = decltype(m)::value;
+        constexpr bool DT = decltype(dt)::value;
+        const T f1 = T(c1), f2 = T(c2);
+        if constexpr (sizeof(T) == 4 && !OWNC) {
+            if (useMf) {
+                fineGo<T, M, false, DT, false, true>(g, s, L, C, crow, ctab, x, y, r, d, f1, f2, stop, zChunk, gx, gy, nTiles, xcdOrder, dots, dotStride, qdot);
+                return;
             }
-            return;
         }
+        if constexpr (!OWNC) {
+            if (dense) {
+                fineGo<T, M, false, DT, true, false>(g, s, L, C, crow, ctab, x, y, r, d, f1, f2, stop, zChunk, gx, gy, nTiles, xcdOrder, dots, dotStride, qdot);
+                return;
+            }
+        }
+        fineGo<T, M, OWNC, DT, false, false>(g, s, L, C, crow, ctab, x, y, r, d, f1, f2, stop, zChunk, gx, gy, nTiles, xcdOrder, dots, dotStride, qdot);
+    };
+    using yes = std::true_type;
+    using no = std::false_type;
+    switch (mode) {
+        case AM_APPLY:
+            if (dot) form(std::integral_constant<int, AM_APPLY>{}, yes{});
+            else form(std::integral_constant<int, AM_APPLY>{}, no{});
+            break;
+        case AM_RAW: form(std::integral_constant<int, AM_RAW>{}, no{}); break;
+        case AM_RESID: form(std::integral_constant<int, AM_RESID>{}, no{}); break;
+        case AM_CHEB1:
+            if (dot) form(std::integral_constant<int, AM_CHEB1>{}, yes{});
+            else form(std::integral_constant<int, AM_CHEB1>{}, no{});
+            break;
+        default:
+            if (dot) form(std::integral_constant<int, AM_CHEB2>{}, yes{});
+            else form(std::integral_constant<int, AM_CHEB2>{}, no{});
+            break;
     }
-    if (!OWNC && L.dense) {
-        switch (mode) {
-            case AM_APPLY:
-                if (dot) kFineDense<T, AM_APPLY, true><<<g, b, 0, s>>>(PSX_FINE_ARGS);
-                else kFineDense<T, AM_APPLY, false><<<g, b, 0, s>>>(PSX_FINE_ARGS);
-                break;
-            case AM_RAW: kFineDense<T, AM_RAW, false><<<g, b, 0, s>>>(PSX_FINE_ARGS); break;
-            case AM_RESID: kFineDense<T, AM_RESID, false><<<g, b, 0, s>>>(PSX_FINE_ARGS); break;
-            case AM_CHEB1:
-                if (dot) kFineDense<T, AM_CHEB1, true><<<g, b, 0, s>>>(PSX_FINE_ARGS);
-                else kFineDense<T, AM_CHEB1, false><<<g, b, 0, s>>>(PSX_FINE_ARGS);
-                break;
-            default:
-                if (dot) kFineDense<T, AM_CHEB2, true><<<g, b, 0, s>>>(PSX_FINE_ARGS);
-                else kFineDense<T, AM_CHEB2, false><<<g, b, 0, s>>>(PSX_FINE_ARGS);
-                break;
-        }
-    } else if constexpr (sizeof(T) == 4 && !OWNC) {
-        switch (mode) {
-            case AM_APPLY:
-                if (dot) kFine4<T, AM_APPLY, true><<<g, b, 0, s>>>(PSX_FINE_ARGS);
-                else kFine4<T, AM_APPLY, false><<<g, b, 0, s>>>(PSX_FINE_ARGS);
-                break;
-            case AM_RAW: kFine4<T, AM_RAW, false><<<g, b, 0, s>>>(PSX_FINE_ARGS); break;
-            case AM_RESID: kFine4<T, AM_RESID, false><<<g, b, 0, s>>>(PSX_FINE_ARGS); break;
-            case AM_CHEB1:
-                if (dot) kFine4<T, AM_CHEB1, true><<<g, b, 0, s>>>(PSX_FINE_ARGS);
-                else kFine4<T, AM_CHEB1, false><<<g, b, 0, s>>>(PSX_FINE_ARGS);
-                break;
-            default:
-                if (dot) kFine4<T, AM_CHEB2, true><<<g, b, 0, s>>>(PSX_FINE_ARGS);
-                else kFine4<T, AM_CHEB2, false><<<g, b, 0, s>>>(PSX_FINE_ARGS);
-                break;
-        }
-    } else {
-        switch (mode) {
-            case AM_APPLY:
-                if (dot) kFine<T, AM_APPLY, OWNC, true><<<g, b, 0, s>>>(PSX_FINE_ARGS);
-                else kFine<T, AM_APPLY, OWNC, false><<<g, b, 0, s>>>(PSX_FINE_ARGS);
-                break;
-            case AM_RAW: kFine<T, AM_RAW, OWNC, false><<<g, b, 0, s>>>(PSX_FINE_ARGS); break;
-            case AM_RESID: kFine<T, AM_RESID, OWNC, false><<<g, b, 0, s>>>(PSX_FINE_ARGS); break;
-            case AM_CHEB1:
-                if (dot) kFine<T, AM_CHEB1, OWNC, true><<<g, b, 0, s>>>(PSX_FINE_ARGS);
-                else kFine<T, AM_CHEB1, OWNC, false><<<g, b, 0, s>>>(PSX_FINE_ARGS);
-                break;
-            default:
-                if (dot) kFine<T, AM_CHEB2, OWNC, true><<<g, b, 0, s>>>(PSX_FINE_ARGS);
-                else kFine<T, AM_CHEB2, OWNC, false><<<g, b, 0, s>>>(PSX_FINE_ARGS);
-                break;
-        }
-    }
-#undef PSX_FINE_ARGS
 }
 
 template <class T>

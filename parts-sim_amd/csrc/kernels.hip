@@ -12,11 +12,17 @@
 // vector length, so every sum is bitwise reproducible and independent of what else runs on the GPU.
 //
 // Every solve kernel also serves multi-instance launches (gridDim.y right-hand sides of one level):
-// blockIdx.y selects the instance's slice of each vector and its CgState.
+// the block's y index selects the instance's slice of each vector and its CgState. The solve path's
+// kernels are bodies (run(VBlk, ...)) behind named kernels, so packed launches (pack_dev.hpp) can
+// run them for several models at once.
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
+#include <stdexcept>
+#include <type_traits>
+
 #include "device.hpp"
+#include "pack_dev.hpp"
 
 namespace psx {
 
@@ -44,8 +50,8 @@ template <> struct Prec<float> {
 
 __device__ __forceinline__ bool stopped(const int* stop) { return stop && __builtin_nontemporal_load(stop) != 0; }
 // multi-instance launches: the instance's stop flag and the start of its slice of a vector
-__device__ __forceinline__ const int* instFlag(const int* f) { return f ? f + int64_t(blockIdx.y) * CG_INTS : nullptr; }
-__device__ __forceinline__ int64_t instOffset(int64_t n) { return int64_t(blockIdx.y) * n; }
+__device__ __forceinline__ const int* instFlag(VBlk vb, const int* f) { return f ? f + int64_t(vb.y) * CG_INTS : nullptr; }
+__device__ __forceinline__ int64_t instOffset(VBlk vb, int64_t n) { return int64_t(vb.y) * n; }
 
 // ------------------------------------------------------------------------------------------------
 // reductions
@@ -77,19 +83,26 @@ template <int NV, int NTH = BLOCK> __device__ __forceinline__ void blockSum(doub
 // atomic per block), then a one-block kernel adds the partials in block order and applies fin().
 // The kernel boundary makes the partials visible; the order is fixed, so results are bitwise
 // reproducible.
-template <int NV> __device__ __forceinline__ void blockPartials(double (&v)[NV], Reduce red) {
+template <int NV> __device__ __forceinline__ void blockPartials(VBlk vb, double (&v)[NV], Reduce red) {
     blockSum<NV>(v);
     if (threadIdx.x == 0)
 #pragma unroll
-        for (int k = 0; k < NV; k++) red.partials[int64_t(blockIdx.y) * red.stride + k * gridDim.x + blockIdx.x] = v[k];
+        for (int k = 0; k < NV; k++) red.partials[int64_t(vb.y) * red.stride + k * vb.gx + vb.x] = v[k];
 }
 
 // guard: skip when *guard != 0 (the producer kernel returned early for the same reason)
 // one block of FIN_BLOCK threads per instance (gridDim.x = instances): the partials are read by one
 // compute unit, so the wider block keeps more loads in flight
 constexpr int FIN_BLOCK = 1024;
+template <int NV, class Fin> struct Finish {
+    static constexpr int NTH = FIN_BLOCK, WPE = 0;
+    __device__ static void run(VBlk vb, Reduce red, unsigned nb, const int* guard, Fin fin);
+};
 template <int NV, class Fin> __global__ __launch_bounds__(FIN_BLOCK) void kFinish(Reduce red, unsigned nb, const int* guard, Fin fin) {
-    const int inst = blockIdx.x;
+    Finish<NV, Fin>::run(ownBlock(), red, nb, guard, fin);
+}
+template <int NV, class Fin> __device__ void Finish<NV, Fin>::run(VBlk vb, Reduce red, unsigned nb, const int* guard, Fin fin) {
+    const int inst = vb.x;
     if (guard && guard[int64_t(inst) * CG_INTS]) return;
     red.partials += int64_t(inst) * red.stride;
     // 8 independent partial sums per thread keep 8 loads in flight; they are combined in a fixed
@@ -122,7 +135,7 @@ inline unsigned reduceGrid(int64_t n) {
 }
 #define TILE_LOOP(body)                                                                  \
     _Pragma("unroll") for (int k_ = 0; k_ < RED_TILE / BLOCK; k_++) {                    \
-        const int64_t i = int64_t(blockIdx.x) * RED_TILE + threadIdx.x + k_ * BLOCK;    \
+        const int64_t i = int64_t(vb.x) * RED_TILE + threadIdx.x + k_ * BLOCK;          \
         if (i < n) { body }                                                              \
     }
 inline unsigned flatGrid(int64_t n) { return unsigned((n + BLOCK - 1) / BLOCK); }
@@ -202,19 +215,29 @@ __device__ __forceinline__ void applyNode(const DevLevel& L, const T* __restrict
     }
 }
 
+template <class T, int MODE> struct Apply {
+    static constexpr int NTH = BLOCK, WPE = 0;
+    __device__ static void run(VBlk vb, DevLevel L, const T* __restrict__ x, T* __restrict__ y, const T* __restrict__ r, const T* __restrict__ zp, T c1, T c2,
+                               const int* stop);
+};
 template <class T, int MODE>
 __global__ __launch_bounds__(BLOCK) void kApply(DevLevel L, const T* __restrict__ x, T* __restrict__ y, const T* __restrict__ r, const T* __restrict__ zp,
                                                  T c1, T c2, const int* stop) {
-    stop = instFlag(stop);
+    Apply<T, MODE>::run(ownBlock(), L, x, y, r, zp, c1, c2, stop);
+}
+template <class T, int MODE>
+__device__ void Apply<T, MODE>::run(VBlk vb, DevLevel L, const T* __restrict__ x, T* __restrict__ y, const T* __restrict__ r, const T* __restrict__ zp,
+                                    T c1, T c2, const int* stop) {
+    stop = instFlag(vb, stop);
     if (stopped(stop)) return;
     {
-        const int64_t io = instOffset(L.nDof);
+        const int64_t io = instOffset(vb, L.nDof);
         x += io;
         y += io;
         if (r) r += io;
         if (zp) zp += io;
     }
-    const int64_t n = int64_t(blockIdx.x) * BLOCK + threadIdx.x;
+    const int64_t n = int64_t(vb.x) * BLOCK + threadIdx.x;
     if (n >= L.nNodes) return;
     const unsigned nn = unsigned(n), NX = unsigned(L.NX), NY = unsigned(L.NY);
     const int i = int(nn % NX), j = int((nn / NX) % NY), k = int(nn / (NX * NY));
@@ -237,16 +260,26 @@ __global__ __launch_bounds__(BLOCK) void kApply(DevLevel L, const T* __restrict_
 // (3 rows x 24 columns from s * Kb in LDS or from the voxel's own matrix), the 8 partial rows are
 // summed with shuffles inside the aligned 8-lane group, lane 0 writes. Coarse levels are small:
 // this spreads one node's dependent loads over 8 lanes instead of one thread walking 8 voxels.
+template <class T, int MODE, bool H16> struct ApplyOwn {
+    static constexpr int NTH = BLOCK, WPE = 0;
+    __device__ static void run(VBlk vb, DevLevel L, const T* __restrict__ x, T* __restrict__ y, const T* __restrict__ r, const T* __restrict__ zp, T c1, T c2,
+                               const int* stop, int ninst);
+};
 template <class T, int MODE, bool H16 = false>
 __global__ __launch_bounds__(BLOCK) void kApplyOwn(DevLevel L, const T* __restrict__ x, T* __restrict__ y, const T* __restrict__ r, const T* __restrict__ zp,
                                                    T c1, T c2, const int* stop, int ninst) {
+    ApplyOwn<T, MODE, H16>::run(ownBlock(), L, x, y, r, zp, c1, c2, stop, ninst);
+}
+template <class T, int MODE, bool H16>
+__device__ void ApplyOwn<T, MODE, H16>::run(VBlk vb, DevLevel L, const T* __restrict__ x, T* __restrict__ y, const T* __restrict__ r,
+                                            const T* __restrict__ zp, T c1, T c2, const int* stop, int ninst) {
     // Several right-hand sides: this lane's element rows (72 coefficients) are loaded once and
     // applied to every instance in turn (instance-major vectors of nDof each, a stop flag per
     // instance), so the own matrices stream from HBM once per launch instead of once per instance.
     __shared__ T Kb[576];
     for (int q = threadIdx.x; q < 576; q += BLOCK) Kb[q] = Prec<T>::Kb(L)[q];
     __syncthreads();
-    const int64_t n = int64_t(blockIdx.x) * (BLOCK / 8) + threadIdx.x / 8;
+    const int64_t n = int64_t(vb.x) * (BLOCK / 8) + threadIdx.x / 8;
     const int v = threadIdx.x & 7;
     const int64_t NX = L.NX, NXY = int64_t(L.NX) * L.NY;
     bool has = false;
@@ -374,14 +407,24 @@ __global__ __launch_bounds__(BLOCK) void kApplyOwn(DevLevel L, const T* __restri
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float floatx4_t __attribute__((ext_vector_type(4)));
 
+template <int MODE> struct ApplyOwnMf {
+    static constexpr int NTH = BLOCK, WPE = 0;
+    __device__ static void run(VBlk vb, DevLevel L, const float* __restrict__ x, float* __restrict__ y, const float* __restrict__ r,
+                               const float* __restrict__ zp, float c1, float c2, const int* stop, int ninst);
+};
 template <int MODE>
 __global__ __launch_bounds__(BLOCK) void kApplyOwnMf(DevLevel L, const float* __restrict__ x, float* __restrict__ y, const float* __restrict__ r,
                                                     const float* __restrict__ zp, float c1, float c2, const int* stop, int ninst) {
+    ApplyOwnMf<MODE>::run(ownBlock(), L, x, y, r, zp, c1, c2, stop, ninst);
+}
+template <int MODE>
+__device__ void ApplyOwnMf<MODE>::run(VBlk vb, DevLevel L, const float* __restrict__ x, float* __restrict__ y, const float* __restrict__ r,
+                                      const float* __restrict__ zp, float c1, float c2, const int* stop, int ninst) {
     __shared__ float Kb[576];
     for (int q = threadIdx.x; q < 576; q += BLOCK) Kb[q] = L.Kb32[q];
     __syncthreads();
     const int lane = threadIdx.x & 63;
-    const int64_t n = int64_t(blockIdx.x) * (BLOCK / 64) + (threadIdx.x >> 6);
+    const int64_t n = int64_t(vb.x) * (BLOCK / 64) + (threadIdx.x >> 6);
     if (n >= L.nNodes) return;  // the whole wavefront
     const unsigned nn = unsigned(n), NXu = unsigned(L.NX), NYu = unsigned(L.NY);
     const int i = int(nn % NXu), j = int((nn / NXu) % NYu), k = int(nn / (NXu * NYu));
@@ -477,26 +520,39 @@ __global__ __launch_bounds__(64) void kOwnToHalf(const float* __restrict__ Kown,
     if (threadIdx.x == 0) scale[e] = sc;
 }
 
+template <class T> struct ChebFirst {
+    static constexpr int NTH = BLOCK, WPE = 0;
+    __device__ static void run(VBlk vb, const T* __restrict__ invD, const T* __restrict__ r, T* __restrict__ z, T c0, int64_t n, const int* stop) {
+        stop = instFlag(vb, stop);
+        if (stopped(stop)) return;
+        const int64_t q = int64_t(vb.x) * BLOCK + threadIdx.x;
+        if (q >= n) return;
+        const int64_t io = instOffset(vb, n);
+        z[io + q] = c0 * invD[q] * r[io + q];
+    }
+};
 template <class T> __global__ __launch_bounds__(BLOCK) void kChebFirst(const T* __restrict__ invD, const T* __restrict__ r, T* __restrict__ z, T c0, int64_t n,
                                                                        const int* stop) {
-    stop = instFlag(stop);
-    if (stopped(stop)) return;
-    const int64_t q = int64_t(blockIdx.x) * BLOCK + threadIdx.x;
-    if (q >= n) return;
-    const int64_t io = instOffset(n);
-    z[io + q] = c0 * invD[q] * r[io + q];
+    ChebFirst<T>::run(ownBlock(), invD, r, z, c0, n, stop);
 }
 
 // ------------------------------------------------------------------------------------------------
 // grid transfers (voxel_fea.cpp:738-815): full weighting P^T and trilinear P, coarse node I <-> fine 2I
 
+template <class T> struct Restrict {
+    static constexpr int NTH = BLOCK, WPE = 0;
+    __device__ static void run(VBlk vb, DevLevel F, DevLevel C, const T* __restrict__ rf, T* __restrict__ rc, const int* stop);
+};
 template <class T>
 __global__ __launch_bounds__(BLOCK) void kRestrict(DevLevel F, DevLevel C, const T* __restrict__ rf, T* __restrict__ rc, const int* stop) {
-    stop = instFlag(stop);
+    Restrict<T>::run(ownBlock(), F, C, rf, rc, stop);
+}
+template <class T> __device__ void Restrict<T>::run(VBlk vb, DevLevel F, DevLevel C, const T* __restrict__ rf, T* __restrict__ rc, const int* stop) {
+    stop = instFlag(vb, stop);
     if (stopped(stop)) return;
-    rf += instOffset(F.nDof);
-    rc += instOffset(C.nDof);
-    const int64_t cn = int64_t(blockIdx.x) * BLOCK + threadIdx.x;
+    rf += instOffset(vb, F.nDof);
+    rc += instOffset(vb, C.nDof);
+    const int64_t cn = int64_t(vb.x) * BLOCK + threadIdx.x;
     if (cn >= C.nNodes) return;
     const unsigned u = unsigned(cn), CNX = unsigned(C.NX), CNY = unsigned(C.NY);
     const int I = int(u % CNX), J = int((u / CNX) % CNY), K = int(u / (CNX * CNY));
@@ -548,13 +604,20 @@ __global__ __launch_bounds__(BLOCK) void kRestrict(DevLevel F, DevLevel C, const
     rc[3 * cn + 2] = i2 != T(0) ? s2 : T(0);
 }
 
+template <class T> struct Prolong {
+    static constexpr int NTH = BLOCK, WPE = 0;
+    __device__ static void run(VBlk vb, DevLevel F, DevLevel C, const T* __restrict__ zc, T* __restrict__ zf, const int* stop);
+};
 template <class T>
 __global__ __launch_bounds__(BLOCK) void kProlong(DevLevel F, DevLevel C, const T* __restrict__ zc, T* __restrict__ zf, const int* stop) {
-    stop = instFlag(stop);
+    Prolong<T>::run(ownBlock(), F, C, zc, zf, stop);
+}
+template <class T> __device__ void Prolong<T>::run(VBlk vb, DevLevel F, DevLevel C, const T* __restrict__ zc, T* __restrict__ zf, const int* stop) {
+    stop = instFlag(vb, stop);
     if (stopped(stop)) return;
-    zc += instOffset(C.nDof);
-    zf += instOffset(F.nDof);
-    const int64_t fn = int64_t(blockIdx.x) * BLOCK + threadIdx.x;
+    zc += instOffset(vb, C.nDof);
+    zf += instOffset(vb, F.nDof);
+    const int64_t fn = int64_t(vb.x) * BLOCK + threadIdx.x;
     if (fn >= F.nNodes) return;
     // held / inactive DOFs from the node info word (bits 8 + d: inverse diagonal 0)
     const uint32_t info = F.nodeInfo[fn];
@@ -597,18 +660,28 @@ __global__ __launch_bounds__(BLOCK) void kProlong(DevLevel F, DevLevel C, const 
 // written, so nothing left in the buffer by another use leaks into the cycle): r gathered into LDS,
 // one wavefront per DOF
 constexpr int COARSE_MAX = 3072;
+template <class T> struct Coarse {
+    static constexpr int NTH = BLOCK, WPE = 0;
+    __device__ static void run(VBlk vb, const int32_t* __restrict__ freeIdx, const int32_t* __restrict__ dense, int m, int64_t nDof,
+                               const double* __restrict__ Ainv, const T* __restrict__ r, T* __restrict__ z, const int* stop);
+};
 template <class T>
 __global__ __launch_bounds__(BLOCK) void kCoarse(const int32_t* __restrict__ freeIdx, const int32_t* __restrict__ dense, int m, int64_t nDof,
                                                  const double* __restrict__ Ainv, const T* __restrict__ r, T* __restrict__ z, const int* stop) {
-    stop = instFlag(stop);
+    Coarse<T>::run(ownBlock(), freeIdx, dense, m, nDof, Ainv, r, z, stop);
+}
+template <class T>
+__device__ void Coarse<T>::run(VBlk vb, const int32_t* __restrict__ freeIdx, const int32_t* __restrict__ dense, int m, int64_t nDof,
+                               const double* __restrict__ Ainv, const T* __restrict__ r, T* __restrict__ z, const int* stop) {
+    stop = instFlag(vb, stop);
     if (stopped(stop)) return;
-    r += instOffset(nDof);
-    z += instOffset(nDof);
+    r += instOffset(vb, nDof);
+    z += instOffset(vb, nDof);
     __shared__ double rl[COARSE_MAX];
     for (int q = threadIdx.x; q < m; q += BLOCK) rl[q] = double(r[freeIdx[q]]);
     __syncthreads();
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    const int64_t q = int64_t(blockIdx.x) * (BLOCK / 64) + wid;  // one wavefront per DOF
+    const int64_t q = int64_t(vb.x) * (BLOCK / 64) + wid;  // one wavefront per DOF
     if (q >= nDof) return;
     const int row = dense[q];
     double s = 0;
@@ -631,26 +704,42 @@ __global__ __launch_bounds__(BLOCK) void kCoarse(const int32_t* __restrict__ fre
 // ------------------------------------------------------------------------------------------------
 // conjugate-gradient vector steps
 
+template <class T> struct DotPQ {
+    static constexpr int NTH = BLOCK, WPE = 0;
+    __device__ static void run(VBlk vb, const T* __restrict__ p, const T* __restrict__ q, int64_t n, CgState* st, Reduce red) {
+        st += vb.y;
+        if (st->stop) return;
+        p += instOffset(vb, n);
+        q += instOffset(vb, n);
+        double v[1] = {0};
+        TILE_LOOP(v[0] += double(p[i]) * double(q[i]);)
+        blockPartials<1>(vb, v, red);
+    }
+};
 template <class T>
 __global__ __launch_bounds__(BLOCK) void kDotPQ(const T* __restrict__ p, const T* __restrict__ q, int64_t n, CgState* st, Reduce red) {
-    st += blockIdx.y;
-    if (st->stop) return;
-    p += instOffset(n);
-    q += instOffset(n);
-    double v[1] = {0};
-    TILE_LOOP(v[0] += double(p[i]) * double(q[i]);)
-    blockPartials<1>(v, red);
+    DotPQ<T>::run(ownBlock(), p, q, n, st, red);
 }
 
 // z0 (when not null): the next V-cycle's first Chebyshev step z0 = c0 D^-1 r from the new r, as
 // kChebFirst computes it (the V-cycle then starts from it instead of reading r again)
+template <class T> struct UpdateXR {
+    static constexpr int NTH = BLOCK, WPE = 0;
+    __device__ static void run(VBlk vb, T* __restrict__ x, const T* __restrict__ p, T* __restrict__ r, const T* __restrict__ q, int64_t n, CgState* st,
+                               Reduce red, T* __restrict__ z0, const T* __restrict__ invD, T c0);
+};
 template <class T>
 __global__ __launch_bounds__(BLOCK) void kUpdateXR(T* __restrict__ x, const T* __restrict__ p, T* __restrict__ r, const T* __restrict__ q, int64_t n,
                                                    CgState* st, Reduce red, T* __restrict__ z0, const T* __restrict__ invD, T c0) {
-    st += blockIdx.y;
+    UpdateXR<T>::run(ownBlock(), x, p, r, q, n, st, red, z0, invD, c0);
+}
+template <class T>
+__device__ void UpdateXR<T>::run(VBlk vb, T* __restrict__ x, const T* __restrict__ p, T* __restrict__ r, const T* __restrict__ q, int64_t n, CgState* st,
+                                 Reduce red, T* __restrict__ z0, const T* __restrict__ invD, T c0) {
+    st += vb.y;
     if (st->stop) return;
     {
-        const int64_t io = instOffset(n);
+        const int64_t io = instOffset(vb, n);
         x += io;
         p += io;
         r += io;
@@ -665,36 +754,48 @@ __global__ __launch_bounds__(BLOCK) void kUpdateXR(T* __restrict__ x, const T* _
         r[i] = ri;
         if (z0) z0[i] = c0 * invD[i] * ri;
         v[0] += double(ri) * double(ri);)
-    blockPartials<1>(v, red);
+    blockPartials<1>(vb, v, red);
 }
 
+template <class T> struct DotsRZ {
+    static constexpr int NTH = BLOCK, WPE = 0;
+    __device__ static void run(VBlk vb, const T* __restrict__ r, const T* __restrict__ z, const T* __restrict__ q, int64_t n, CgState* st, Reduce red) {
+        st += vb.y;
+        if (st->stop) return;
+        {
+            const int64_t io = instOffset(vb, n);
+            r += io;
+            z += io;
+            q += io;
+        }
+        double v[2] = {0, 0};
+        TILE_LOOP(
+            const double zi = double(z[i]);
+            v[0] += double(r[i]) * zi;
+            v[1] += zi * double(q[i]);)
+        blockPartials<2>(vb, v, red);
+    }
+};
 template <class T>
 __global__ __launch_bounds__(BLOCK) void kDotsRZ(const T* __restrict__ r, const T* __restrict__ z, const T* __restrict__ q, int64_t n, CgState* st,
                                                  Reduce red) {
-    st += blockIdx.y;
-    if (st->stop) return;
-    {
-        const int64_t io = instOffset(n);
-        r += io;
-        z += io;
-        q += io;
-    }
-    double v[2] = {0, 0};
-    TILE_LOOP(
-        const double zi = double(z[i]);
-        v[0] += double(r[i]) * zi;
-        v[1] += zi * double(q[i]);)
-    blockPartials<2>(v, red);
+    DotsRZ<T>::run(ownBlock(), r, z, q, n, st, red);
 }
 
+template <class T> struct UpdateP {
+    static constexpr int NTH = BLOCK, WPE = 0;
+    __device__ static void run(VBlk vb, const T* __restrict__ z, T* __restrict__ p, int64_t n, const CgState* st) {
+        st += vb.y;
+        if (st->stop) return;
+        const int64_t i = int64_t(vb.x) * BLOCK + threadIdx.x;
+        if (i >= n) return;
+        const int64_t io = instOffset(vb, n);
+        p[io + i] = z[io + i] + T(st->beta) * p[io + i];
+    }
+};
 template <class T>
 __global__ __launch_bounds__(BLOCK) void kUpdateP(const T* __restrict__ z, T* __restrict__ p, int64_t n, const CgState* st) {
-    st += blockIdx.y;
-    if (st->stop) return;
-    const int64_t i = int64_t(blockIdx.x) * BLOCK + threadIdx.x;
-    if (i >= n) return;
-    const int64_t io = instOffset(n);
-    p[io + i] = z[io + i] + T(st->beta) * p[io + i];
+    UpdateP<T>::run(ownBlock(), z, p, n, st);
 }
 
 // mixed precision with FP32 recurrences: the correction x32 and the residual r32 in units of
@@ -703,33 +804,49 @@ __global__ __launch_bounds__(BLOCK) void kUpdateP(const T* __restrict__ z, T* __
 // small grid (grid-stride loops): an early return then costs a few hundred workgroups, not one per
 // 256 DOFs.
 constexpr unsigned FOLD_GRID = 1024;
-__global__ __launch_bounds__(BLOCK) void kFoldX(double* __restrict__ u, float* __restrict__ x, int64_t n, const CgState* st) {
-    st += blockIdx.y;
-    if (st->noFold) return;
-    u += instOffset(n);
-    x += instOffset(n);
-    const double rmax = st->rmax;
-    for (int64_t i = int64_t(blockIdx.x) * BLOCK + threadIdx.x; i < n; i += int64_t(gridDim.x) * BLOCK) {
-        u[i] += double(x[i]) * rmax;
-        x[i] = 0.f;
+struct FoldX {
+    static constexpr int NTH = BLOCK, WPE = 0;
+    __device__ static void run(VBlk vb, double* __restrict__ u, float* __restrict__ x, int64_t n, const CgState* st) {
+        st += vb.y;
+        if (st->noFold) return;
+        u += instOffset(vb, n);
+        x += instOffset(vb, n);
+        const double rmax = st->rmax;
+        for (int64_t i = int64_t(vb.x) * BLOCK + threadIdx.x; i < n; i += int64_t(vb.gx) * BLOCK) {
+            u[i] += double(x[i]) * rmax;
+            x[i] = 0.f;
+        }
     }
+};
+__global__ __launch_bounds__(BLOCK) void kFoldX(double* __restrict__ u, float* __restrict__ x, int64_t n, const CgState* st) {
+    FoldX::run(ownBlock(), u, x, n, st);
 }
 
 // z0 (when not null): the first Chebyshev step from the new r32, as kUpdateXR writes it
+struct FoldR {
+    static constexpr int NTH = BLOCK, WPE = 0;
+    __device__ static void run(VBlk vb, const double* __restrict__ f, const double* __restrict__ Kx, const double* __restrict__ invD, double* __restrict__ r64,
+                               float* __restrict__ r32, int64_t n, CgState* st, Reduce red, float* __restrict__ z0, const float* __restrict__ invD32,
+                               float c0);
+};
 __global__ __launch_bounds__(BLOCK) void kFoldR(const double* __restrict__ f, const double* __restrict__ Kx, const double* __restrict__ invD,
                                                 double* __restrict__ r64, float* __restrict__ r32, int64_t n, CgState* st, Reduce red,
                                                 float* __restrict__ z0, const float* __restrict__ invD32, float c0) {
-    st += blockIdx.y;
+    FoldR::run(ownBlock(), f, Kx, invD, r64, r32, n, st, red, z0, invD32, c0);
+}
+__device__ void FoldR::run(VBlk vb, const double* __restrict__ f, const double* __restrict__ Kx, const double* __restrict__ invD, double* __restrict__ r64,
+                           float* __restrict__ r32, int64_t n, CgState* st, Reduce red, float* __restrict__ z0, const float* __restrict__ invD32, float c0) {
+    st += vb.y;
     if (st->noFold) return;
-    f += instOffset(n);
-    Kx += instOffset(n);
-    r64 += instOffset(n);
-    r32 += instOffset(n);
-    if (z0) z0 += instOffset(n);
+    f += instOffset(vb, n);
+    Kx += instOffset(vb, n);
+    r64 += instOffset(vb, n);
+    r32 += instOffset(vb, n);
+    if (z0) z0 += instOffset(vb, n);
     const double inv = 1.0 / st->rmax;
     double v[1] = {0};
-    // tiles of RED_TILE elements, block b taking tiles b, b + gridDim.x, ...
-    for (int64_t t0 = int64_t(blockIdx.x) * RED_TILE; t0 < n; t0 += int64_t(gridDim.x) * RED_TILE) {
+    // tiles of RED_TILE elements, block b taking tiles b, b + (blocks of the launch), ...
+    for (int64_t t0 = int64_t(vb.x) * RED_TILE; t0 < n; t0 += int64_t(vb.gx) * RED_TILE) {
 #pragma unroll
         for (int k = 0; k < RED_TILE / BLOCK; k++) {
             const int64_t i = t0 + threadIdx.x + k * BLOCK;
@@ -743,38 +860,46 @@ __global__ __launch_bounds__(BLOCK) void kFoldR(const double* __restrict__ f, co
             }
         }
     }
-    blockPartials<1>(v, red);
+    blockPartials<1>(vb, v, red);
 }
 
 __global__ __launch_bounds__(BLOCK) void kResidual64(const double* __restrict__ f, const double* __restrict__ Kx, const double* __restrict__ invD,
                                                      double* __restrict__ r, int64_t n, Reduce red) {
-    f += instOffset(n);
-    Kx += instOffset(n);
-    r += instOffset(n);
+    const VBlk vb = ownBlock();
+    f += instOffset(vb, n);
+    Kx += instOffset(vb, n);
+    r += instOffset(vb, n);
     double v[1] = {0};
     TILE_LOOP(
         const double ri = invD[i] != 0 ? f[i] - Kx[i] : 0.0;
         r[i] = ri;
         v[0] += ri * ri;)
-    blockPartials<1>(v, red);
+    blockPartials<1>(vb, v, red);
 }
 
 __global__ __launch_bounds__(BLOCK) void kNormFree64(const double* __restrict__ f, const double* __restrict__ invD, int64_t n, Reduce red) {
-    f += instOffset(n);
+    const VBlk vb = ownBlock();
+    f += instOffset(vb, n);
     double v[1] = {0};
     TILE_LOOP(if (invD[i] != 0) v[0] += f[i] * f[i];)
-    blockPartials<1>(v, red);
+    blockPartials<1>(vb, v, red);
 }
 
 __global__ __launch_bounds__(BLOCK) void kMaskCopy64(const double* __restrict__ x0, const double* __restrict__ invD, double* __restrict__ x, int64_t n) {
-    const int64_t i = int64_t(blockIdx.x) * BLOCK + threadIdx.x;
+    const VBlk vb = ownBlock();
+    const int64_t i = int64_t(vb.x) * BLOCK + threadIdx.x;
     if (i >= n) return;
-    const int64_t io = instOffset(n);
+    const int64_t io = instOffset(vb, n);
     x[io + i] = (x0 && invD[i] != 0) ? x0[io + i] : 0.0;
 }
 
-__global__ __launch_bounds__(BLOCK) void kMaxAbs64(const double* __restrict__ v, int64_t n, Reduce red) {
-    v += instOffset(n);
+struct MaxAbs64 {
+    static constexpr int NTH = BLOCK, WPE = 0;
+    __device__ static void run(VBlk vb, const double* __restrict__ v, int64_t n, Reduce red);
+};
+__global__ __launch_bounds__(BLOCK) void kMaxAbs64(const double* __restrict__ v, int64_t n, Reduce red) { MaxAbs64::run(ownBlock(), v, n, red); }
+__device__ void MaxAbs64::run(VBlk vb, const double* __restrict__ v, int64_t n, Reduce red) {
+    v += instOffset(vb, n);
     double m = 0;
     TILE_LOOP(m = fmax(m, fabs(v[i]));)
     __shared__ double sh[BLOCK / 64];
@@ -784,14 +909,19 @@ __global__ __launch_bounds__(BLOCK) void kMaxAbs64(const double* __restrict__ v,
     if (threadIdx.x == 0) {
         double b = sh[0];
         for (int w = 1; w < BLOCK / 64; w++) b = fmax(b, sh[w]);
-        red.partials[int64_t(blockIdx.y) * red.stride + blockIdx.x] = b;
+        red.partials[int64_t(vb.y) * red.stride + vb.x] = b;
     }
 }
 
 // one block per instance; out steps by CgState
-__global__ __launch_bounds__(BLOCK) void kMaxFinish(Reduce red, unsigned nb, double* out) {
-    red.partials += int64_t(blockIdx.x) * red.stride;
-    out += int64_t(blockIdx.x) * CG_DBLS;
+struct MaxFinish {
+    static constexpr int NTH = BLOCK, WPE = 0;
+    __device__ static void run(VBlk vb, Reduce red, unsigned nb, double* out);
+};
+__global__ __launch_bounds__(BLOCK) void kMaxFinish(Reduce red, unsigned nb, double* out) { MaxFinish::run(ownBlock(), red, nb, out); }
+__device__ void MaxFinish::run(VBlk vb, Reduce red, unsigned nb, double* out) {
+    red.partials += int64_t(vb.x) * red.stride;
+    out += int64_t(vb.x) * CG_DBLS;
     double m = 0;
     for (unsigned b = threadIdx.x; b < nb; b += BLOCK) m = fmax(m, red.partials[b]);
     __shared__ double sh[BLOCK / 64];
@@ -806,19 +936,31 @@ __global__ __launch_bounds__(BLOCK) void kMaxFinish(Reduce red, unsigned nb, dou
 }
 
 // scaleFrom points into CgState (steps by instance)
+struct ToF32 {
+    static constexpr int NTH = BLOCK, WPE = 0;
+    __device__ static void run(VBlk vb, const double* __restrict__ x, float* __restrict__ y, int64_t n, const double* scaleFrom) {
+        const int64_t i = int64_t(vb.x) * BLOCK + threadIdx.x;
+        if (i >= n) return;
+        const int64_t io = instOffset(vb, n);
+        const double s = scaleFrom[int64_t(vb.y) * CG_DBLS];
+        y[io + i] = s > 0 ? float(x[io + i] * (1.0 / s)) : 0.f;
+    }
+};
 __global__ __launch_bounds__(BLOCK) void kToF32(const double* __restrict__ x, float* __restrict__ y, int64_t n, const double* scaleFrom) {
-    const int64_t i = int64_t(blockIdx.x) * BLOCK + threadIdx.x;
-    if (i >= n) return;
-    const int64_t io = instOffset(n);
-    const double s = scaleFrom[int64_t(blockIdx.y) * CG_DBLS];
-    y[io + i] = s > 0 ? float(x[io + i] * (1.0 / s)) : 0.f;
+    ToF32::run(ownBlock(), x, y, n, scaleFrom);
 }
 
+struct FromF32 {
+    static constexpr int NTH = BLOCK, WPE = 0;
+    __device__ static void run(VBlk vb, const float* __restrict__ z, double* __restrict__ y, int64_t n, const double* scaleFrom) {
+        const int64_t i = int64_t(vb.x) * BLOCK + threadIdx.x;
+        if (i >= n) return;
+        const int64_t io = instOffset(vb, n);
+        y[io + i] = double(z[io + i]) * scaleFrom[int64_t(vb.y) * CG_DBLS];
+    }
+};
 __global__ __launch_bounds__(BLOCK) void kFromF32(const float* __restrict__ z, double* __restrict__ y, int64_t n, const double* scaleFrom) {
-    const int64_t i = int64_t(blockIdx.x) * BLOCK + threadIdx.x;
-    if (i >= n) return;
-    const int64_t io = instOffset(n);
-    y[io + i] = double(z[io + i]) * scaleFrom[int64_t(blockIdx.y) * CG_DBLS];
+    FromF32::run(ownBlock(), z, y, n, scaleFrom);
 }
 
 // Lanczos on D^-1/2 K D^-1/2
@@ -830,23 +972,25 @@ __global__ __launch_bounds__(BLOCK) void kLzX(const double* __restrict__ v, cons
 
 __global__ __launch_bounds__(BLOCK) void kLzW(double* __restrict__ w, const double* __restrict__ invD, const double* __restrict__ vPrev,
                                               const double* __restrict__ v, int64_t n, CgState* st, int j, Reduce red) {
+    const VBlk vb = ownBlock();
     const double b = j > 0 ? st->lz[16 + j - 1] : 0.0;
     double s[1] = {0};
     TILE_LOOP(
         const double wi = sqrt(invD[i]) * w[i] - b * vPrev[i];
         w[i] = wi;
         s[0] += wi * v[i];)
-    blockPartials<1>(s, red);
+    blockPartials<1>(vb, s, red);
 }
 
 __global__ __launch_bounds__(BLOCK) void kLzAxpy(double* __restrict__ w, const double* __restrict__ v, int64_t n, CgState* st, int j, Reduce red) {
+    const VBlk vb = ownBlock();
     const double a = st->lz[j];
     double s[1] = {0};
     TILE_LOOP(
         const double wi = w[i] - a * v[i];
         w[i] = wi;
         s[0] += wi * wi;)
-    blockPartials<1>(s, red);
+    blockPartials<1>(vb, s, red);
 }
 
 __global__ __launch_bounds__(BLOCK) void kLzNext(const double* __restrict__ w, double* __restrict__ v, double* __restrict__ vPrev, int64_t n,
@@ -936,6 +1080,16 @@ struct FinLzB {
     __device__ void operator()(const double* s, int) const { st->lz[16 + j] = sqrt(s[0]); }
 };
 
+// one finish block per instance
+template <int NV, class Fin> void finish(int ninst, hipStream_t s, Reduce red, unsigned nb, const int* guard, Fin fin) {
+    go<Finish<NV, Fin>>(kFinish<NV, Fin>, dim3(unsigned(ninst)), s, red, nb, guard, fin);
+}
+// launchers whose kernels a packed solve never records (set-up and final steps) refuse to run while
+// this thread records, so nothing is issued out of order
+void notRecording() {
+    if (packRecorder()) throw std::logic_error("a set-up kernel was launched while a packed batch was being recorded");
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------------------------------------
@@ -943,13 +1097,17 @@ struct FinLzB {
 
 template <class T>
 void launchApply(const DevLevel& L, int mode, const T* x, T* y, const T* r, const T* d, double c1, double c2, const int* stop, hipStream_t s, int ninst) {
-    const dim3 g(flatGrid(L.nNodes), ninst), b(BLOCK);
+    const dim3 g(flatGrid(L.nNodes), ninst);
+    auto one = [&](auto m) {
+        constexpr int M = decltype(m)::value;
+        go<Apply<T, M>>(kApply<T, M>, g, s, L, x, y, r, d, T(c1), T(c2), stop);
+    };
     switch (mode) {
-        case AM_APPLY: kApply<T, AM_APPLY><<<g, b, 0, s>>>(L, x, y, r, d, T(c1), T(c2), stop); break;
-        case AM_RAW: kApply<T, AM_RAW><<<g, b, 0, s>>>(L, x, y, r, d, T(c1), T(c2), stop); break;
-        case AM_RESID: kApply<T, AM_RESID><<<g, b, 0, s>>>(L, x, y, r, d, T(c1), T(c2), stop); break;
-        case AM_CHEB1: kApply<T, AM_CHEB1><<<g, b, 0, s>>>(L, x, y, r, d, T(c1), T(c2), stop); break;
-        default: kApply<T, AM_CHEB2><<<g, b, 0, s>>>(L, x, y, r, d, T(c1), T(c2), stop); break;
+        case AM_APPLY: one(std::integral_constant<int, AM_APPLY>{}); break;
+        case AM_RAW: one(std::integral_constant<int, AM_RAW>{}); break;
+        case AM_RESID: one(std::integral_constant<int, AM_RESID>{}); break;
+        case AM_CHEB1: one(std::integral_constant<int, AM_CHEB1>{}); break;
+        default: one(std::integral_constant<int, AM_CHEB2>{}); break;
     }
 }
 
@@ -957,140 +1115,156 @@ template <class T>
 void launchApplyOwn(const DevLevel& L, int mode, const T* x, T* y, const T* r, const T* d, double c1, double c2, const int* stop, hipStream_t s, int ninst,
                     bool half) {
     // every instance in one grid: the kernel loops over them (the element rows are read once)
-    const dim3 g(unsigned((L.nNodes + BLOCK / 8 - 1) / (BLOCK / 8))), b(BLOCK);
+    const dim3 g(unsigned((L.nNodes + BLOCK / 8 - 1) / (BLOCK / 8)));
+    auto one = [&](auto m, auto h) {
+        constexpr int M = decltype(m)::value;
+        constexpr bool H = decltype(h)::value;
+        go<ApplyOwn<T, M, H>>(kApplyOwn<T, M, H>, g, s, L, x, y, r, d, T(c1), T(c2), stop, ninst);
+    };
+    auto modes = [&](auto h) {
+        switch (mode) {
+            case AM_APPLY: one(std::integral_constant<int, AM_APPLY>{}, h); break;
+            case AM_RAW: one(std::integral_constant<int, AM_RAW>{}, h); break;
+            case AM_RESID: one(std::integral_constant<int, AM_RESID>{}, h); break;
+            case AM_CHEB1: one(std::integral_constant<int, AM_CHEB1>{}, h); break;
+            default: one(std::integral_constant<int, AM_CHEB2>{}, h); break;
+        }
+    };
     if constexpr (sizeof(T) == 4) {
         if (half && L.KownH) {
-            switch (mode) {
-                case AM_APPLY: kApplyOwn<T, AM_APPLY, true><<<g, b, 0, s>>>(L, x, y, r, d, T(c1), T(c2), stop, ninst); break;
-                case AM_RAW: kApplyOwn<T, AM_RAW, true><<<g, b, 0, s>>>(L, x, y, r, d, T(c1), T(c2), stop, ninst); break;
-                case AM_RESID: kApplyOwn<T, AM_RESID, true><<<g, b, 0, s>>>(L, x, y, r, d, T(c1), T(c2), stop, ninst); break;
-                case AM_CHEB1: kApplyOwn<T, AM_CHEB1, true><<<g, b, 0, s>>>(L, x, y, r, d, T(c1), T(c2), stop, ninst); break;
-                default: kApplyOwn<T, AM_CHEB2, true><<<g, b, 0, s>>>(L, x, y, r, d, T(c1), T(c2), stop, ninst); break;
-            }
+            modes(std::true_type{});
             return;
         }
     }
-    switch (mode) {
-        case AM_APPLY: kApplyOwn<T, AM_APPLY><<<g, b, 0, s>>>(L, x, y, r, d, T(c1), T(c2), stop, ninst); break;
-        case AM_RAW: kApplyOwn<T, AM_RAW><<<g, b, 0, s>>>(L, x, y, r, d, T(c1), T(c2), stop, ninst); break;
-        case AM_RESID: kApplyOwn<T, AM_RESID><<<g, b, 0, s>>>(L, x, y, r, d, T(c1), T(c2), stop, ninst); break;
-        case AM_CHEB1: kApplyOwn<T, AM_CHEB1><<<g, b, 0, s>>>(L, x, y, r, d, T(c1), T(c2), stop, ninst); break;
-        default: kApplyOwn<T, AM_CHEB2><<<g, b, 0, s>>>(L, x, y, r, d, T(c1), T(c2), stop, ninst); break;
-    }
+    modes(std::false_type{});
 }
 
 void launchOwnToHalf(const float* Kown, int64_t n, uint16_t* half, float* scale, hipStream_t s) {
+    notRecording();
     if (n > 0) kOwnToHalf<<<unsigned(n), 64, 0, s>>>(Kown, n, half, scale);
 }
 
 void launchApplyOwnMf(const DevLevel& L, int mode, const float* x, float* y, const float* r, const float* d, double c1, double c2, const int* stop,
                       hipStream_t s, int ninst) {
-    const dim3 g(unsigned((L.nNodes + BLOCK / 64 - 1) / (BLOCK / 64))), b(BLOCK);
+    const dim3 g(unsigned((L.nNodes + BLOCK / 64 - 1) / (BLOCK / 64)));
     const float f1 = float(c1), f2 = float(c2);
+    auto one = [&](auto m) {
+        constexpr int M = decltype(m)::value;
+        go<ApplyOwnMf<M>>(kApplyOwnMf<M>, g, s, L, x, y, r, d, f1, f2, stop, ninst);
+    };
     switch (mode) {
-        case AM_APPLY: kApplyOwnMf<AM_APPLY><<<g, b, 0, s>>>(L, x, y, r, d, f1, f2, stop, ninst); break;
-        case AM_RAW: kApplyOwnMf<AM_RAW><<<g, b, 0, s>>>(L, x, y, r, d, f1, f2, stop, ninst); break;
-        case AM_RESID: kApplyOwnMf<AM_RESID><<<g, b, 0, s>>>(L, x, y, r, d, f1, f2, stop, ninst); break;
-        case AM_CHEB1: kApplyOwnMf<AM_CHEB1><<<g, b, 0, s>>>(L, x, y, r, d, f1, f2, stop, ninst); break;
-        default: kApplyOwnMf<AM_CHEB2><<<g, b, 0, s>>>(L, x, y, r, d, f1, f2, stop, ninst); break;
+        case AM_APPLY: one(std::integral_constant<int, AM_APPLY>{}); break;
+        case AM_RAW: one(std::integral_constant<int, AM_RAW>{}); break;
+        case AM_RESID: one(std::integral_constant<int, AM_RESID>{}); break;
+        case AM_CHEB1: one(std::integral_constant<int, AM_CHEB1>{}); break;
+        default: one(std::integral_constant<int, AM_CHEB2>{}); break;
     }
 }
 
 template <class T> void launchChebFirst(const DevLevel& L, const T* r, T* z, double c0, const int* stop, hipStream_t s, int ninst) {
     const T* invD = sizeof(T) == 8 ? reinterpret_cast<const T*>(L.invD64) : reinterpret_cast<const T*>(L.invD32);
-    kChebFirst<T><<<dim3(flatGrid(L.nDof), ninst), BLOCK, 0, s>>>(invD, r, z, T(c0), L.nDof, stop);
+    go<ChebFirst<T>>(kChebFirst<T>, dim3(flatGrid(L.nDof), ninst), s, invD, r, z, T(c0), L.nDof, stop);
 }
 
 template <class T> void launchRestrict(const DevLevel& F, const DevLevel& C, const T* rf, T* rc, const int* stop, hipStream_t s, int ninst) {
-    kRestrict<T><<<dim3(flatGrid(C.nNodes), ninst), BLOCK, 0, s>>>(F, C, rf, rc, stop);
+    go<Restrict<T>>(kRestrict<T>, dim3(flatGrid(C.nNodes), ninst), s, F, C, rf, rc, stop);
 }
 
 template <class T> void launchProlong(const DevLevel& F, const DevLevel& C, const T* zc, T* zf, const int* stop, hipStream_t s, int ninst) {
-    kProlong<T><<<dim3(flatGrid(F.nNodes), ninst), BLOCK, 0, s>>>(F, C, zc, zf, stop);
+    go<Prolong<T>>(kProlong<T>, dim3(flatGrid(F.nNodes), ninst), s, F, C, zc, zf, stop);
 }
 
 template <class T>
 void launchCoarse(const int32_t* freeIdx, const int32_t* dense, int64_t m, int64_t nDof, const double* Ainv, const T* r, T* z, const int* stop, hipStream_t s,
                   int ninst) {
-    kCoarse<T><<<dim3(unsigned((nDof + BLOCK / 64 - 1) / (BLOCK / 64)), ninst), BLOCK, 0, s>>>(freeIdx, dense, int(m), nDof, Ainv, r, z, stop);
+    go<Coarse<T>>(kCoarse<T>, dim3(unsigned((nDof + BLOCK / 64 - 1) / (BLOCK / 64)), ninst), s, freeIdx, dense, int(m), nDof, Ainv, r, z, stop);
 }
 
 template <class T> void launchDotPQ(const T* p, const T* q, int64_t n, CgState* st, Reduce red, hipStream_t s, int ninst) {
     const unsigned g = reduceGrid(n);
-    kDotPQ<T><<<dim3(g, ninst), BLOCK, 0, s>>>(p, q, n, st, red);
-    kFinish<1, FinPQ><<<ninst, FIN_BLOCK, 0, s>>>(red, g, &st->stop, FinPQ{st});
+    go<DotPQ<T>>(kDotPQ<T>, dim3(g, ninst), s, p, q, n, st, red);
+    finish<1, FinPQ>(ninst, s, red, g, &st->stop, FinPQ{st});
 }
 template <class T> void launchUpdateXR(T* x, const T* p, T* r, const T* q, int64_t n, CgState* st, Reduce red, hipStream_t s, int ninst) {
     const unsigned g = reduceGrid(n);
-    kUpdateXR<T><<<dim3(g, ninst), BLOCK, 0, s>>>(x, p, r, q, n, st, red, nullptr, nullptr, T(0));
-    kFinish<1, FinXR><<<ninst, FIN_BLOCK, 0, s>>>(red, g, &st->stop, FinXR{st});
+    go<UpdateXR<T>>(kUpdateXR<T>, dim3(g, ninst), s, x, p, r, q, n, st, red, nullptr, nullptr, T(0));
+    finish<1, FinXR>(ninst, s, red, g, &st->stop, FinXR{st});
 }
 template <class T> void launchDotsRZ(const T* r, const T* z, const T* q, int64_t n, CgState* st, Reduce red, hipStream_t s, int ninst) {
     const unsigned g = reduceGrid(n);
-    kDotsRZ<T><<<dim3(g, ninst), BLOCK, 0, s>>>(r, z, q, n, st, red);
-    kFinish<2, FinRZ><<<ninst, FIN_BLOCK, 0, s>>>(red, g, &st->stop, FinRZ{st});
+    go<DotsRZ<T>>(kDotsRZ<T>, dim3(g, ninst), s, r, z, q, n, st, red);
+    finish<2, FinRZ>(ninst, s, red, g, &st->stop, FinRZ{st});
 }
 template <class T> void launchUpdateP(const T* z, T* p, int64_t n, CgState* st, hipStream_t s, int ninst) {
-    kUpdateP<T><<<dim3(flatGrid(n), ninst), BLOCK, 0, s>>>(z, p, n, st);
+    go<UpdateP<T>>(kUpdateP<T>, dim3(flatGrid(n), ninst), s, z, p, n, st);
 }
 
 // finishers for dot products whose block partials another kernel wrote ([k * nb + block])
-void launchFinishPQ(Reduce red, unsigned nb, CgState* st, hipStream_t s, int ninst) { kFinish<1, FinPQ><<<ninst, FIN_BLOCK, 0, s>>>(red, nb, &st->stop, FinPQ{st}); }
-void launchFinishRZ(Reduce red, unsigned nb, CgState* st, hipStream_t s, int ninst) { kFinish<2, FinRZ><<<ninst, FIN_BLOCK, 0, s>>>(red, nb, &st->stop, FinRZ{st}); }
+void launchFinishPQ(Reduce red, unsigned nb, CgState* st, hipStream_t s, int ninst) { finish<1, FinPQ>(ninst, s, red, nb, &st->stop, FinPQ{st}); }
+void launchFinishRZ(Reduce red, unsigned nb, CgState* st, hipStream_t s, int ninst) { finish<2, FinRZ>(ninst, s, red, nb, &st->stop, FinRZ{st}); }
 
 void launchUpdateXR32Fold(float* x, const float* p, float* r, const float* q, int64_t n, CgState* st, Reduce red, hipStream_t s, int ninst, float* z0,
                           const float* invD32, double c0) {
     const unsigned g = reduceGrid(n);
-    kUpdateXR<float><<<dim3(g, ninst), BLOCK, 0, s>>>(x, p, r, q, n, st, red, z0, invD32, float(c0));
-    kFinish<1, FinXRFold><<<ninst, FIN_BLOCK, 0, s>>>(red, g, &st->stop, FinXRFold{st});
+    go<UpdateXR<float>>(kUpdateXR<float>, dim3(g, ninst), s, x, p, r, q, n, st, red, z0, invD32, float(c0));
+    finish<1, FinXRFold>(ninst, s, red, g, &st->stop, FinXRFold{st});
 }
 void launchFoldX(double* u, float* x, int64_t n, CgState* st, hipStream_t s, int ninst) {
-    kFoldX<<<dim3(std::min(flatGrid(n), FOLD_GRID), ninst), BLOCK, 0, s>>>(u, x, n, st);
+    go<FoldX>(kFoldX, dim3(std::min(flatGrid(n), FOLD_GRID), ninst), s, u, x, n, st);
 }
 
 void launchFoldR(const double* f, const double* Kx, const double* invD, double* r64, float* r32, int64_t n, CgState* st, Reduce red, hipStream_t s,
                  int ninst, float* z0, const float* invD32, double c0) {
     const unsigned g = std::min(reduceGrid(n), FOLD_GRID);
-    kFoldR<<<dim3(g, ninst), BLOCK, 0, s>>>(f, Kx, invD, r64, r32, n, st, red, z0, invD32, float(c0));
-    kFinish<1, FinFoldR><<<ninst, FIN_BLOCK, 0, s>>>(red, g, &st->noFold, FinFoldR{st});
+    go<FoldR>(kFoldR, dim3(g, ninst), s, f, Kx, invD, r64, r32, n, st, red, z0, invD32, float(c0));
+    finish<1, FinFoldR>(ninst, s, red, g, &st->noFold, FinFoldR{st});
 }
 
 void launchResidual64(const double* f, const double* Kx, const double* invD, double* r, int64_t n, double* out, Reduce red, hipStream_t s, int ninst) {
+    notRecording();
     const unsigned g = reduceGrid(n);
     kResidual64<<<dim3(g, ninst), BLOCK, 0, s>>>(f, Kx, invD, r, n, red);
     kFinish<1, FinStore><<<ninst, FIN_BLOCK, 0, s>>>(red, g, nullptr, FinStore{out});
 }
 void launchNormFree64(const double* f, const double* invD, int64_t n, double* out, Reduce red, hipStream_t s, int ninst) {
+    notRecording();
     const unsigned g = reduceGrid(n);
     kNormFree64<<<dim3(g, ninst), BLOCK, 0, s>>>(f, invD, n, red);
     kFinish<1, FinStore><<<ninst, FIN_BLOCK, 0, s>>>(red, g, nullptr, FinStore{out});
 }
 void launchMaskCopy64(const double* x0, const double* invD, double* x, int64_t n, hipStream_t s, int ninst) {
+    notRecording();
     kMaskCopy64<<<dim3(flatGrid(n), ninst), BLOCK, 0, s>>>(x0, invD, x, n);
 }
 void launchMaxAbs64(const double* v, int64_t n, double* out, Reduce red, hipStream_t s, int ninst) {
     const unsigned g = reduceGrid(n);
-    kMaxAbs64<<<dim3(g, ninst), BLOCK, 0, s>>>(v, n, red);
-    kMaxFinish<<<ninst, BLOCK, 0, s>>>(red, g, out);
+    go<MaxAbs64>(kMaxAbs64, dim3(g, ninst), s, v, n, red);
+    go<MaxFinish>(kMaxFinish, dim3(unsigned(ninst)), s, red, g, out);
 }
 void launchToF32(const double* x, float* y, int64_t n, const double* scaleFrom, hipStream_t s, int ninst) {
-    kToF32<<<dim3(flatGrid(n), ninst), BLOCK, 0, s>>>(x, y, n, scaleFrom);
+    go<ToF32>(kToF32, dim3(flatGrid(n), ninst), s, x, y, n, scaleFrom);
 }
 void launchFromF32(const float* z, double* y, int64_t n, const double* scaleFrom, hipStream_t s, int ninst) {
-    kFromF32<<<dim3(flatGrid(n), ninst), BLOCK, 0, s>>>(z, y, n, scaleFrom);
+    go<FromF32>(kFromF32, dim3(flatGrid(n), ninst), s, z, y, n, scaleFrom);
 }
-void launchLzX(const double* v, const double* invD, double* x, int64_t n, hipStream_t s) { kLzX<<<flatGrid(n), BLOCK, 0, s>>>(v, invD, x, n); }
+void launchLzX(const double* v, const double* invD, double* x, int64_t n, hipStream_t s) {
+    notRecording();
+    kLzX<<<flatGrid(n), BLOCK, 0, s>>>(v, invD, x, n);
+}
 void launchLzW(double* w, const double* invD, const double* vPrev, const double* v, int64_t n, CgState* st, int j, Reduce red, hipStream_t s) {
+    notRecording();
     const unsigned g = reduceGrid(n);
     kLzW<<<g, BLOCK, 0, s>>>(w, invD, vPrev, v, n, st, j, red);
     kFinish<1, FinLzA><<<1, FIN_BLOCK, 0, s>>>(red, g, nullptr, FinLzA{st, j});
 }
 void launchLzAxpy(double* w, const double* v, int64_t n, CgState* st, int j, Reduce red, hipStream_t s) {
+    notRecording();
     const unsigned g = reduceGrid(n);
     kLzAxpy<<<g, BLOCK, 0, s>>>(w, v, n, st, j, red);
     kFinish<1, FinLzB><<<1, FIN_BLOCK, 0, s>>>(red, g, nullptr, FinLzB{st, j});
 }
 void launchLzNext(const double* w, double* v, double* vPrev, int64_t n, CgState* st, int j, hipStream_t s) {
+    notRecording();
     kLzNext<<<flatGrid(n), BLOCK, 0, s>>>(w, v, vPrev, n, st, j);
 }
 

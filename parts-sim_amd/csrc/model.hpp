@@ -546,144 +546,44 @@ public:
 
     void solveDevice(const double* df, const double* dx0, const psx_solve_opts& o, double* du, psx_solve_info& info) { solveMulti(1, df, dx0, o, du, &info); }
 
+    // One conjugate-gradient run over k right-hand sides, in phases: begin (start vector, initial
+    // residual and device state), batches of iterations (enqueueBatch, then poll), end (the last fold
+    // and the true residual). solveMulti runs them on the model's stream; a packed solve
+    // (packed.hpp) runs the batches of several models' runs as shared launches.
+    struct Run {
+        int k = 1;
+        const double* df = nullptr;
+        const double* dx0 = nullptr;
+        double* du = nullptr;
+        psx_solve_opts o{};
+        psx_solve_info* info = nullptr;
+        std::chrono::steady_clock::time_point t0;
+        std::vector<double> bnorm;
+        int prec = 0, batch = 8, form = 0, degree = 2;
+        bool fuseFirst = false;
+        double c0first = 0;
+        bool allStopped = true, cancelled = false;
+    };
+
     // k independent right-hand sides in one set of launches: df, du (and dx0 when not null) hold k
     // vectors of nDof, instance-major; every kernel runs all k (blockIdx.y) and each instance stops
     // on its own state, so the result of each right-hand side is the one a lone solve gives
     void solveMulti(int k, const double* df, const double* dx0, const psx_solve_opts& o, double* du, psx_solve_info* info) {
-        const auto t0 = std::chrono::steady_clock::now();
-        if (k < 1) throw std::invalid_argument("no right-hand sides");
-        if (!smoothersReady) estimateSmoothers();
-        reserve(k);
-        ni = k;
-        struct Restore {
-            int& n;
-            ~Restore() { n = 1; }
-        } restore{ni};
-        const int64_t n = nDof();
-        const size_t nk = size_t(n) * size_t(k);
-        const DevLevel& D0 = lv[0].dl;
-        for (int i = 0; i < k; i++) info[i] = psx_solve_info{};
-        // x = x0 on free DOFs; bnorm = |f_free|; r = f - K x (FP64)
-        launchMaskCopy64(dx0, D0.invD64, du, n, stream, k);
-        launchNormFree64(df, D0.invD64, n, &st->bnorm, red, stream, k);
-        op<double>(0, AM_APPLY, du, q, nullptr, nullptr, 0, 0, nullptr);
-        launchResidual64(df, q, D0.invD64, r, n, &st->rr, red, stream, k);
-        readState();
-        std::vector<double> bnorm(k), rel(k), rmax(k, 1.0);
-        bool anyLoad = false;
-        for (int i = 0; i < k; i++) {
-            bnorm[i] = std::sqrt(stHost[i].bnorm);
-            rel[i] = bnorm[i] > 0 ? std::sqrt(stHost[i].rr) / bnorm[i] : 0;
-            anyLoad = anyLoad || bnorm[i] > 0;
-            // no load on the free DOFs: u = 0 (voxel_fea.cpp:872-876)
-            if (bnorm[i] == 0) HIPCHK(hipMemsetAsync(du + size_t(i) * n, 0, size_t(n) * 8, stream));
-        }
-        if (!anyLoad) {
-            sync();
-            for (int i = 0; i < k; i++) {
-                info[i].converged = 1;
-                info[i].device_ms = ms(t0);
-            }
-            return;
-        }
-        const int prec = o.precision;
-        const int batch = o.check_every > 0 ? o.check_every : 8;
-        // the reference's algorithm keeps its operator; the FP64 and mixed solves use the exact
-        // child-scale form, and the mixed smoother the matrix cores when asked (one right-hand side)
-        // mfma_smoother bit 0: the level-0 in-plane stencil on the matrix cores; bit 1: own-element levels
-        const int form = prec == PSX_PREC_REF ? 0
-                                              : (OF_AUTO | (prec == PSX_PREC_MIXED && (o.mfma_smoother & 1) ? OF_MFMA : 0) |
-                                                 (prec == PSX_PREC_MIXED && (o.mfma_smoother & 2) ? OF_MFMA_OWN : 0));
-        const int degree = o.cheb_degree > 0 ? o.cheb_degree : 2;
-        if (degree > CHEB_MAX) throw std::invalid_argument("Chebyshev degree above 6");
-        preDegree = o.pre_degree > 0 ? std::min(o.pre_degree, degree) : 0;
-        struct ResetPre {
-            int& p;
-            ~ResetPre() { p = 0; }
-        } resetPre{preDegree};
-        // mixed: the residual updates also write the next V-cycle's first Chebyshev step on level 0
-        const bool fuseFirst = prec == PSX_PREC_MIXED && lv.size() > 1;
-        const double c0first = fuseFirst ? chebyshev(H.levels[0].lmax, degree).c0 : 0.0;
-        if (prec == PSX_PREC_MIXED) {
-            // FP32 units for the whole solve: each first residual scaled to order one
-            launchMaxAbs64(r, n, &st->rmax, red, stream, k);
-            readState();
-            for (int i = 0; i < k; i++) rmax[i] = stHost[i].rmax > 0 ? stHost[i].rmax : 1;
-        }
-        std::vector<CgState> s0(k);
-        bool allStopped = true;
-        for (int i = 0; i < k; i++) {
-            CgState& c = s0[i];
-            c = CgState{};
-            c.rz = 1;
-            c.bnorm = bnorm[i];
-            c.tol = o.tol;
-            c.maxIter = o.max_iter;
-            c.scale = prec == PSX_PREC_MIXED ? rmax[i] : 1;
-            c.rmax = rmax[i];
-            c.rel = rel[i];
-            c.trueRel = rel[i];
-            c.peak = rel[i];
-            c.noFold = 1;
-            c.foldEvery = o.fold_every > 0 ? o.fold_every : 20;
-            c.stop = (bnorm[i] == 0 || rel[i] <= o.tol || o.max_iter <= 0) ? 1 : 0;
-            allStopped = allStopped && c.stop;
-        }
-        HIPCHK(hipMemcpyAsync(st, s0.data(), sizeof(CgState) * size_t(k), hipMemcpyHostToDevice, stream));
-        int* stop = &st->stop;
-        float* r32 = lv[0].r32;
-        if (prec == PSX_PREC_MIXED) {
-            HIPCHK(hipMemsetAsync(p32, 0, nk * 4, stream));
-            HIPCHK(hipMemsetAsync(q32, 0, nk * 4, stream));
-            HIPCHK(hipMemsetAsync(x32, 0, nk * 4, stream));
-            launchToF32(r, r32, n, &st->rmax, stream, k);
-            // the first V-cycle's first Chebyshev step; later ones come from the residual updates
-            if (fuseFirst) launchChebFirst<float>(D0, r32, lv[0].a32, c0first, stop, stream, k);
-        } else {
-            HIPCHK(hipMemsetAsync(p, 0, nk * 8, stream));
-            HIPCHK(hipMemsetAsync(q, 0, nk * 8, stream));
-        }
-        bool cancelled = false;
-        // one batch of iterations: every kernel returns at once for an instance whose state says
-        // stop, so the same launches serve every batch
-        auto enqueueBatch = [&] {
-            for (int b = 0; b < batch; b++) {
-                if (prec == PSX_PREC_MIXED) {
-                    const DotFusion fz{FD_RZ, q32, red}, fp{FD_PQ, nullptr, red};
-                    const float* z = vcycle<float>(0, r32, stop, form, degree, &fz, fuseFirst);
-                    launchUpdateP<float>(z, p32, n, st, stream, k);
-                    // the outer operator stays FP32 (the matrix-core form is for the smoother only)
-                    op<float>(0, AM_APPLY, p32, q32, nullptr, nullptr, 0, 0, stop, form & ~(OF_MFMA | OF_MFMA_OWN), &fp);
-                    launchUpdateXR32Fold(x32, p32, r32, q32, n, st, red, stream, k, fuseFirst ? lv[0].a32 : nullptr, D0.invD32, c0first);
-                    // reliable update: the correction joins the FP64 solution and the FP64 true
-                    // residual replaces the recurrence's
-                    launchFoldX(du, x32, n, st, stream, k);
-                    op<double>(0, AM_APPLY, du, q, nullptr, nullptr, 0, 0, &st->noFold);
-                    launchFoldR(df, q, D0.invD64, r, r32, n, st, red, stream, k, fuseFirst ? lv[0].a32 : nullptr, D0.invD32, c0first);
-                    continue;
-                }
-                const double* z;
-                const DotFusion fz{FD_RZ, q, red}, fp{FD_PQ, nullptr, red};
-                if (prec == PSX_PREC_FP64) z = vcycle<double>(0, r, stop, form, degree, &fz);
-                else {
-                    precondition32(r, lv[0].b64, stop, degree);
-                    z = lv[0].b64;
-                    launchDotsRZ<double>(r, z, q, n, st, red, stream, k);
-                }
-                launchUpdateP<double>(z, p, n, st, stream, k);
-                op<double>(0, AM_APPLY, p, q, nullptr, nullptr, 0, 0, stop, form, &fp);
-                launchUpdateXR<double>(du, p, r, q, n, st, red, stream, k);
-            }
-        };
+        Run R;
+        struct Reset {
+            Model& m;
+            ~Reset() { m.resetRun(); }
+        } reset{*this};
+        if (!begin(R, k, df, dx0, o, du, info)) return;
         // the batch as a HIP graph, captured once and replayed (kernels only: no copies or memsets);
         // it saves the host one launch per kernel, which bounds small parts
-        const GraphKey key{prec, degree, preDegree, form, k, batch, df, du};
+        const GraphKey key{R.prec, R.degree, preDegree, R.form, k, R.batch, df, du};
         const bool useGraph = !o.eager && !graphsOff();
-        if (useGraph && !allStopped && !(graph && graphKey == key)) {
+        if (useGraph && !(graph && graphKey == key)) {
             dropGraph();
             HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed));
             try {
-                enqueueBatch();
+                enqueueBatch(R);
             } catch (...) {
                 hipGraph_t g = nullptr;
                 (void)hipStreamEndCapture(stream, &g);
@@ -700,30 +600,194 @@ public:
             }
             graphKey = key;
         }
-        while (!allStopped) {
+        for (;;) {
             if (useGraph) HIPCHK(hipGraphLaunch(graph, stream));
-            else enqueueBatch();
+            else enqueueBatch(R);
             readState();
-            allStopped = true;
-            int itMax = 0;
-            double relMax = 0;
-            for (int i = 0; i < k; i++) {
-                allStopped = allStopped && stHost[i].stop;
-                itMax = std::max(itMax, stHost[i].iters);
-                relMax = std::max(relMax, stHost[i].rel);
-            }
-            if (allStopped) break;
-            if (o.on_progress && o.on_progress(itMax, relMax, o.user)) {
-                cancelled = true;
-                break;
-            }
+            if (poll(R, true)) break;
         }
-        if (prec == PSX_PREC_MIXED) {
+        end(R);
+    }
+
+    // the run's launch counts back to one right-hand side and the pre-smoothing to the post-smoothing's
+    void resetRun() {
+        ni = 1;
+        preDegree = 0;
+    }
+
+    // x = x0, the initial residual and the per-instance state; false when no instance iterates (info
+    // is then complete)
+    bool begin(Run& R, int k, const double* df, const double* dx0, const psx_solve_opts& o, double* du, psx_solve_info* info) {
+        R = Run{};
+        R.t0 = std::chrono::steady_clock::now();
+        R.k = k;
+        R.df = df;
+        R.dx0 = dx0;
+        R.du = du;
+        R.o = o;
+        R.info = info;
+        if (k < 1) throw std::invalid_argument("no right-hand sides");
+        if (!smoothersReady) estimateSmoothers();
+        reserve(k);
+        ni = k;
+        const int64_t n = nDof();
+        const size_t nk = size_t(n) * size_t(k);
+        const DevLevel& D0 = lv[0].dl;
+        for (int i = 0; i < k; i++) info[i] = psx_solve_info{};
+        // x = x0 on free DOFs; bnorm = |f_free|; r = f - K x (FP64)
+        launchMaskCopy64(dx0, D0.invD64, du, n, stream, k);
+        launchNormFree64(df, D0.invD64, n, &st->bnorm, red, stream, k);
+        op<double>(0, AM_APPLY, du, q, nullptr, nullptr, 0, 0, nullptr);
+        launchResidual64(df, q, D0.invD64, r, n, &st->rr, red, stream, k);
+        readState();
+        std::vector<double> rel(k), rmax(k, 1.0);
+        R.bnorm.assign(k, 0.0);
+        bool anyLoad = false;
+        for (int i = 0; i < k; i++) {
+            R.bnorm[i] = std::sqrt(stHost[i].bnorm);
+            rel[i] = R.bnorm[i] > 0 ? std::sqrt(stHost[i].rr) / R.bnorm[i] : 0;
+            anyLoad = anyLoad || R.bnorm[i] > 0;
+            // no load on the free DOFs: u = 0 (voxel_fea.cpp:872-876)
+            if (R.bnorm[i] == 0) HIPCHK(hipMemsetAsync(du + size_t(i) * n, 0, size_t(n) * 8, stream));
+        }
+        if (!anyLoad) {
+            sync();
+            for (int i = 0; i < k; i++) {
+                info[i].converged = 1;
+                info[i].device_ms = ms(R.t0);
+            }
+            return false;
+        }
+        R.prec = o.precision;
+        R.batch = o.check_every > 0 ? o.check_every : 8;
+        // the reference's algorithm keeps its operator; the FP64 and mixed solves use the exact
+        // child-scale form, and the mixed smoother the matrix cores when asked (one right-hand side)
+        // mfma_smoother bit 0: the level-0 in-plane stencil on the matrix cores; bit 1: own-element levels
+        R.form = R.prec == PSX_PREC_REF ? 0
+                                        : (OF_AUTO | (R.prec == PSX_PREC_MIXED && (o.mfma_smoother & 1) ? OF_MFMA : 0) |
+                                           (R.prec == PSX_PREC_MIXED && (o.mfma_smoother & 2) ? OF_MFMA_OWN : 0));
+        R.degree = o.cheb_degree > 0 ? o.cheb_degree : 2;
+        if (R.degree > CHEB_MAX) throw std::invalid_argument("Chebyshev degree above 6");
+        preDegree = o.pre_degree > 0 ? std::min(o.pre_degree, R.degree) : 0;
+        // mixed: the residual updates also write the next V-cycle's first Chebyshev step on level 0
+        R.fuseFirst = R.prec == PSX_PREC_MIXED && lv.size() > 1;
+        R.c0first = R.fuseFirst ? chebyshev(H.levels[0].lmax, R.degree).c0 : 0.0;
+        if (R.prec == PSX_PREC_MIXED) {
+            // FP32 units for the whole solve: each first residual scaled to order one
+            launchMaxAbs64(r, n, &st->rmax, red, stream, k);
+            readState();
+            for (int i = 0; i < k; i++) rmax[i] = stHost[i].rmax > 0 ? stHost[i].rmax : 1;
+        }
+        std::vector<CgState> s0(k);
+        R.allStopped = true;
+        for (int i = 0; i < k; i++) {
+            CgState& c = s0[i];
+            c = CgState{};
+            c.rz = 1;
+            c.bnorm = R.bnorm[i];
+            c.tol = o.tol;
+            c.maxIter = o.max_iter;
+            c.scale = R.prec == PSX_PREC_MIXED ? rmax[i] : 1;
+            c.rmax = rmax[i];
+            c.rel = rel[i];
+            c.trueRel = rel[i];
+            c.peak = rel[i];
+            c.noFold = 1;
+            c.foldEvery = o.fold_every > 0 ? o.fold_every : 20;
+            c.stop = (R.bnorm[i] == 0 || rel[i] <= o.tol || o.max_iter <= 0) ? 1 : 0;
+            R.allStopped = R.allStopped && c.stop;
+        }
+        HIPCHK(hipMemcpyAsync(st, s0.data(), sizeof(CgState) * size_t(k), hipMemcpyHostToDevice, stream));
+        if (R.prec == PSX_PREC_MIXED) {
+            HIPCHK(hipMemsetAsync(p32, 0, nk * 4, stream));
+            HIPCHK(hipMemsetAsync(q32, 0, nk * 4, stream));
+            HIPCHK(hipMemsetAsync(x32, 0, nk * 4, stream));
+            launchToF32(r, lv[0].r32, n, &st->rmax, stream, k);
+            // the first V-cycle's first Chebyshev step; later ones come from the residual updates
+            if (R.fuseFirst) launchChebFirst<float>(D0, lv[0].r32, lv[0].a32, R.c0first, &st->stop, stream, k);
+        } else {
+            HIPCHK(hipMemsetAsync(p, 0, nk * 8, stream));
+            HIPCHK(hipMemsetAsync(q, 0, nk * 8, stream));
+        }
+        if (R.allStopped) {
+            end(R);
+            return false;
+        }
+        // the state and the cleared vectors are in place before any other stream runs the batches
+        sync();
+        return true;
+    }
+
+    // one batch of iterations: every kernel returns at once for an instance whose state says stop,
+    // so the same launches serve every batch (kernels only: it is captured and recorded)
+    void enqueueBatch(const Run& R) {
+        const int64_t n = nDof();
+        const int k = R.k;
+        const DevLevel& D0 = lv[0].dl;
+        int* stop = &st->stop;
+        float* r32 = lv[0].r32;
+        const int prec = R.prec, form = R.form, degree = R.degree;
+        for (int b = 0; b < R.batch; b++) {
+            if (prec == PSX_PREC_MIXED) {
+                const DotFusion fz{FD_RZ, q32, red}, fp{FD_PQ, nullptr, red};
+                const float* z = vcycle<float>(0, r32, stop, form, degree, &fz, R.fuseFirst);
+                launchUpdateP<float>(z, p32, n, st, stream, k);
+                // the outer operator stays FP32 (the matrix-core form is for the smoother only)
+                op<float>(0, AM_APPLY, p32, q32, nullptr, nullptr, 0, 0, stop, form & ~(OF_MFMA | OF_MFMA_OWN), &fp);
+                launchUpdateXR32Fold(x32, p32, r32, q32, n, st, red, stream, k, R.fuseFirst ? lv[0].a32 : nullptr, D0.invD32, R.c0first);
+                // reliable update: the correction joins the FP64 solution and the FP64 true
+                // residual replaces the recurrence's
+                launchFoldX(R.du, x32, n, st, stream, k);
+                op<double>(0, AM_APPLY, R.du, q, nullptr, nullptr, 0, 0, &st->noFold);
+                launchFoldR(R.df, q, D0.invD64, r, r32, n, st, red, stream, k, R.fuseFirst ? lv[0].a32 : nullptr, D0.invD32, R.c0first);
+                continue;
+            }
+            const double* z;
+            const DotFusion fz{FD_RZ, q, red}, fp{FD_PQ, nullptr, red};
+            if (prec == PSX_PREC_FP64) z = vcycle<double>(0, r, stop, form, degree, &fz);
+            else {
+                precondition32(r, lv[0].b64, stop, degree);
+                z = lv[0].b64;
+                launchDotsRZ<double>(r, z, q, n, st, red, stream, k);
+            }
+            launchUpdateP<double>(z, p, n, st, stream, k);
+            op<double>(0, AM_APPLY, p, q, nullptr, nullptr, 0, 0, stop, form, &fp);
+            launchUpdateXR<double>(R.du, p, r, q, n, st, red, stream, k);
+        }
+    }
+
+    // after a batch (the state already read back): true when every instance has stopped or the
+    // progress callback (when asked) cancels
+    bool poll(Run& R, bool progress) {
+        R.allStopped = true;
+        int itMax = 0;
+        double relMax = 0;
+        for (int i = 0; i < R.k; i++) {
+            R.allStopped = R.allStopped && stHost[i].stop;
+            itMax = std::max(itMax, stHost[i].iters);
+            relMax = std::max(relMax, stHost[i].rel);
+        }
+        if (R.allStopped) return true;
+        if (progress && R.o.on_progress && R.o.on_progress(itMax, relMax, R.o.user)) {
+            R.cancelled = true;
+            return true;
+        }
+        return false;
+    }
+
+    void end(Run& R) {
+        const int k = R.k;
+        const int64_t n = nDof();
+        const DevLevel& D0 = lv[0].dl;
+        psx_solve_info* info = R.info;
+        // host source of the flag copies below: alive until readState has waited for them
+        std::vector<CgState> s0(k);
+        if (R.prec == PSX_PREC_MIXED) {
             // a cancelled run still folds in its last correction
             for (int i = 0; i < k; i++) s0[i].noFold = 0;
             for (int i = 0; i < k; i++)
                 HIPCHK(hipMemcpyAsync(&st[i].noFold, &s0[i].noFold, sizeof(int), hipMemcpyHostToDevice, stream));
-            launchFoldX(du, x32, n, st, stream, k);
+            launchFoldX(R.du, x32, n, st, stream, k);
         }
         readState();
         for (int i = 0; i < k; i++) {
@@ -732,19 +796,19 @@ public:
             info[i].folds = stHost[i].folds;
         }
         // the final true residual against the equilibrium equations (voxel_fea.cpp:920-925)
-        op<double>(0, AM_APPLY, du, q, nullptr, nullptr, 0, 0, nullptr);
-        launchResidual64(df, q, D0.invD64, r, n, &st->rr, red, stream, k);
+        op<double>(0, AM_APPLY, R.du, q, nullptr, nullptr, 0, 0, nullptr);
+        launchResidual64(R.df, q, D0.invD64, r, n, &st->rr, red, stream, k);
         readState();
         for (int i = 0; i < k; i++) {
-            if (bnorm[i] == 0) {
+            if (R.bnorm[i] == 0) {
                 info[i].converged = 1;
-                info[i].device_ms = ms(t0);
+                info[i].device_ms = ms(R.t0);
                 continue;
             }
-            info[i].cancelled = cancelled;
-            info[i].residual = std::sqrt(stHost[i].rr) / bnorm[i];
-            info[i].converged = !cancelled && std::isfinite(info[i].residual) && info[i].residual <= o.tol * 10;
-            info[i].device_ms = ms(t0);
+            info[i].cancelled = R.cancelled;
+            info[i].residual = std::sqrt(stHost[i].rr) / R.bnorm[i];
+            info[i].converged = !R.cancelled && std::isfinite(info[i].residual) && info[i].residual <= R.o.tol * 10;
+            info[i].device_ms = ms(R.t0);
         }
     }
 

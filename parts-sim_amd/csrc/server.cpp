@@ -4,9 +4,12 @@
 // worker is free and the head's estimated device footprint fits the budget beside what running
 // requests (and resident parts) hold; resident parts are evicted, least recently used first, to make
 // room. An admitted request builds its hierarchy with the device kernels (or on the host when the
-// request asks for it, or takes its resident part) and solves its load cases on the model's own HIP
-// stream, so the kernels of concurrent requests
-// overlap on the device while each request's arithmetic stays exactly that of a lone solve.
+// request asks for it, or takes its resident part) and begins its run. A small part (at most
+// pack_max_dof DOFs) then joins the pack: one thread advances every member's run together, each
+// step of a batch one shared launch for all members (packed.hpp), members joining and leaving at
+// batch boundaries. A larger part solves its load cases on the model's own HIP stream, so its
+// kernels overlap with the pack's and other requests'. Either way each request's arithmetic is
+// exactly that of a lone solve.
 #include <hip/hip_runtime_api.h>
 
 #include <chrono>
@@ -19,6 +22,7 @@
 
 #include "abi.hpp"
 #include "model.hpp"
+#include "packed.hpp"
 
 using namespace psx::abi;
 using Clock = std::chrono::steady_clock;
@@ -73,18 +77,51 @@ struct Resident {
     int64_t bytes = 0;  // what `held` counts for it
 };
 
+// a request between admission and completion: its model and device vectors
+struct Job {
+    Request* r = nullptr;
+    bool reused = false;
+    psx::Model* M = nullptr;
+    std::unique_ptr<psx::Model> own;
+    double* buf[3] = {nullptr, nullptr, nullptr};  // f, u, x0 (nrhs x n each) on M's stream
+    std::vector<psx_solve_info> info;
+    psx::Model::Run run;
+    void freeBuffers() {
+        if (!M || !(buf[0] || buf[1] || buf[2])) return;
+        for (double*& q : buf)
+            if (q) {
+                (void)hipFreeAsync(q, M->stream);
+                q = nullptr;
+            }
+        (void)hipStreamSynchronize(M->stream);
+    }
+    ~Job() { freeBuffers(); }
+};
+
 }  // namespace
 
 struct psx_server_s {
     std::mutex mu;
-    std::condition_variable wake, finished;
+    std::condition_variable wake, finished, packWake;
     std::deque<Request*> queue;
     std::map<int64_t, std::unique_ptr<Request>> requests;
     std::map<int64_t, Resident> parts;
     std::vector<std::thread> workers;
+    std::thread packer;
     int64_t budget = 0, held = 0, nextTicket = 1;
     int running = 0, device = 0;
-    bool stopping = false;
+    bool stopping = false, packStop = false;
+    // packing: requests of at most packDof DOFs run in the shared-launch pack (up to packMax at once).
+    // Measured on the 40-request stream of void-block parts (edges 32-256, 200 requests/s): packing
+    // parts up to 100 000 DOFs leaves solves/s and latencies where the lone streams have them, while
+    // packing parts up to 3 * 2^20 DOFs kept the solves/s but raised the median latency from about
+    // 0.2 s to 0.75 s (the pack's members advance in lockstep, each step as long as its largest
+    // member's kernel, beside the big parts' streams); so only tiny, launch-bound parts are packed by
+    // default.
+    int packMax = 16;
+    int64_t packDof = 100000;
+    int packed = 0;  // jobs in the pack or its inbox
+    std::vector<std::unique_ptr<Job>> inbox;
 
     // bytes held by resident parts that are not in use (evictable)
     int64_t evictable() const {
@@ -140,93 +177,141 @@ struct psx_server_s {
             held += r->estimate;
             running++;
             r->admitted = Clock::now();
+            // small parts go to the pack while it has room
+            const bool pack = packMax > 1 && r->nDof <= packDof && packed < packMax;
+            if (pack) packed++;
             lk.unlock();
             evicted.clear();
-            run(*r, reused);
+            auto job = std::make_unique<Job>();
+            job->r = r;
+            job->reused = reused;
+            bool iterate = false;
+            r->res.status = guarded([&] {
+                iterate = prepare(*job);
+                if (iterate && !pack) {
+                    solveLone(*job);
+                    iterate = false;
+                }
+                return PSX_OK;
+            });
+            if (r->res.status != PSX_OK) {
+                r->error = g_error;
+                iterate = false;
+            }
+            if (iterate) {
+                // the pack thread runs its batches from here on
+                r->res.packed = 1;
+                lk.lock();
+                inbox.push_back(std::move(job));
+                packWake.notify_one();
+                continue;
+            }
+            if (pack) {
+                lk.lock();
+                packed--;
+                lk.unlock();
+            }
+            complete(std::move(job));
             lk.lock();
-            held -= r->estimate;
-            running--;
-            r->finished = Clock::now();
-            r->res.queue_ms = msBetween(r->submitted, r->admitted);
-            r->res.total_ms = msBetween(r->submitted, r->finished);
-            r->done = true;
-            finished.notify_all();
-            wake.notify_all();
         }
     }
 
-    // build (or take) the model and solve every load case; the lock is not held
-    void run(Request& r, bool reused) {
-        psx::Model* M = nullptr;
-        std::unique_ptr<psx::Model> own;
-        r.res.status = guarded([&] {
-            if (reused) {
-                std::lock_guard<std::mutex> g(mu);
-                M = parts.at(r.partId).model.get();
-            } else {
-                psx_model_desc d{};
-                d.nx = r.nx;
-                d.ny = r.ny;
-                d.nz = r.nz;
-                d.density = r.density.data();
-                d.nu = r.nu;
-                d.bc = r.bc.data();
-                d.diag_add = r.diagAdd.empty() ? nullptr : r.diagAdd.data();
-                d.coarsest_max_dof = r.coarsest;
-                d.max_levels = r.maxLevels;
-                d.build_on_device = r.onDevice;
-                own = std::make_unique<psx::Model>(d);
-                own->estimateSmoothers();
-                own->sync();
-                M = own.get();
-            }
-            r.built = Clock::now();
-            r.res.build_ms = reused ? 0 : msBetween(r.admitted, r.built);
-            r.res.reused_part = reused ? 1 : 0;
-            r.res.device_bytes = M->totalBytes();
-            const int64_t n = M->nDof();
-            if (n != r.nDof) throw std::invalid_argument("part does not match its load vectors");
-            // the request's vectors: f, u and x0 (nrhs x n) in HBM for the whole request; all load
-            // cases run in one multi-instance solve
-            const size_t nk = size_t(n) * size_t(r.nrhs);
-            struct Buffers {
-                hipStream_t s;
-                double* p[3] = {nullptr, nullptr, nullptr};
-                ~Buffers() {
-                    for (double* q : p)
-                        if (q) (void)hipFreeAsync(q, s);
-                    (void)hipStreamSynchronize(s);
+    // builds (or takes) the model, uploads the load cases and begins the run; false when the run
+    // needs no iterations (its results are then final)
+    bool prepare(Job& j) {
+        Request& r = *j.r;
+        if (j.reused) {
+            std::lock_guard<std::mutex> g(mu);
+            j.M = parts.at(r.partId).model.get();
+        } else {
+            psx_model_desc d{};
+            d.nx = r.nx;
+            d.ny = r.ny;
+            d.nz = r.nz;
+            d.density = r.density.data();
+            d.nu = r.nu;
+            d.bc = r.bc.data();
+            d.diag_add = r.diagAdd.empty() ? nullptr : r.diagAdd.data();
+            d.coarsest_max_dof = r.coarsest;
+            d.max_levels = r.maxLevels;
+            d.build_on_device = r.onDevice;
+            j.own = std::make_unique<psx::Model>(d);
+            j.own->estimateSmoothers();
+            j.own->sync();
+            j.M = j.own.get();
+        }
+        psx::Model* M = j.M;
+        r.built = Clock::now();
+        r.res.build_ms = j.reused ? 0 : msBetween(r.admitted, r.built);
+        r.res.reused_part = j.reused ? 1 : 0;
+        r.res.device_bytes = M->totalBytes();
+        const int64_t n = M->nDof();
+        if (n != r.nDof) throw std::invalid_argument("part does not match its load vectors");
+        // the request's vectors: f, u and x0 (nrhs x n) in HBM for the whole request; all load
+        // cases run in one multi-instance solve
+        const size_t nk = size_t(n) * size_t(r.nrhs);
+        HIPCHK(hipMallocAsync(reinterpret_cast<void**>(&j.buf[0]), nk * 8, M->stream));
+        HIPCHK(hipMallocAsync(reinterpret_cast<void**>(&j.buf[1]), nk * 8, M->stream));
+        if (!r.x0.empty()) HIPCHK(hipMallocAsync(reinterpret_cast<void**>(&j.buf[2]), nk * 8, M->stream));
+        double *df = j.buf[0], *du = j.buf[1], *dx = j.buf[2];
+        HIPCHK(hipMemcpyAsync(df, r.f.data(), nk * 8, hipMemcpyHostToDevice, M->stream));
+        if (dx) {
+            if (r.x0PerRhs) HIPCHK(hipMemcpyAsync(dx, r.x0.data(), nk * 8, hipMemcpyHostToDevice, M->stream));
+            else
+                for (int k = 0; k < r.nrhs; k++)
+                    HIPCHK(hipMemcpyAsync(dx + size_t(k) * n, r.x0.data(), size_t(n) * 8, hipMemcpyHostToDevice, M->stream));
+        }
+        j.info.assign(size_t(r.nrhs), psx_solve_info{});
+        try {
+            if (M->begin(j.run, r.nrhs, df, dx, r.opts, du, j.info.data())) return true;
+        } catch (...) {
+            M->resetRun();
+            throw;
+        }
+        M->resetRun();
+        return false;
+    }
+
+    // the run's batches on the model's own stream
+    void solveLone(Job& j) {
+        psx::Model* M = j.M;
+        struct Reset {
+            psx::Model* m;
+            ~Reset() { m->resetRun(); }
+        } reset{M};
+        for (;;) {
+            M->enqueueBatch(j.run);
+            M->readState();
+            if (M->poll(j.run, false)) break;
+        }
+        M->end(j.run);
+    }
+
+    // results to the host, the model kept resident or freed, the request done
+    void complete(std::unique_ptr<Job> j) {
+        Request& r = *j->r;
+        if (r.res.status == PSX_OK) {
+            r.res.status = guarded([&] {
+                const size_t nk = size_t(j->M->nDof()) * size_t(r.nrhs);
+                r.u.assign(nk, 0.0);
+                HIPCHK(hipMemcpyAsync(r.u.data(), j->buf[1], nk * 8, hipMemcpyDeviceToHost, j->M->stream));
+                j->M->sync();
+                r.res.converged = 1;
+                for (const psx_solve_info& i : j->info) {
+                    r.res.iterations += i.iterations;
+                    r.res.max_iterations = std::max(r.res.max_iterations, i.iterations);
+                    r.res.residual = std::max(r.res.residual, i.residual);
+                    r.res.converged = r.res.converged && i.converged;
                 }
-            } buf{M->stream};
-            HIPCHK(hipMallocAsync(reinterpret_cast<void**>(&buf.p[0]), nk * 8, M->stream));
-            HIPCHK(hipMallocAsync(reinterpret_cast<void**>(&buf.p[1]), nk * 8, M->stream));
-            if (!r.x0.empty()) HIPCHK(hipMallocAsync(reinterpret_cast<void**>(&buf.p[2]), nk * 8, M->stream));
-            double *df = buf.p[0], *du = buf.p[1], *dx = buf.p[2];
-            HIPCHK(hipMemcpyAsync(df, r.f.data(), nk * 8, hipMemcpyHostToDevice, M->stream));
-            if (dx) {
-                if (r.x0PerRhs) HIPCHK(hipMemcpyAsync(dx, r.x0.data(), nk * 8, hipMemcpyHostToDevice, M->stream));
-                else
-                    for (int k = 0; k < r.nrhs; k++)
-                        HIPCHK(hipMemcpyAsync(dx + size_t(k) * n, r.x0.data(), size_t(n) * 8, hipMemcpyHostToDevice, M->stream));
-            }
-            std::vector<psx_solve_info> info(size_t(r.nrhs));
-            M->solveMulti(r.nrhs, df, dx, r.opts, du, info.data());
-            r.u.assign(nk, 0.0);
-            HIPCHK(hipMemcpyAsync(r.u.data(), du, nk * 8, hipMemcpyDeviceToHost, M->stream));
-            M->sync();
-            r.res.converged = 1;
-            for (const psx_solve_info& i : info) {
-                r.res.iterations += i.iterations;
-                r.res.max_iterations = std::max(r.res.max_iterations, i.iterations);
-                r.res.residual = std::max(r.res.residual, i.residual);
-                r.res.converged = r.res.converged && i.converged;
-            }
-            r.res.solve_ms = msBetween(r.built, Clock::now());
-            return PSX_OK;
-        });
-        if (r.res.status != PSX_OK) r.error = g_error;
+                r.res.solve_ms = msBetween(r.built, Clock::now());
+                return PSX_OK;
+            });
+            if (r.res.status != PSX_OK) r.error = g_error;
+        }
+        j->freeBuffers();
         std::lock_guard<std::mutex> g(mu);
-        if (reused) {
+        j->M = nullptr;  // a resident model may be evicted once the lock is released
+        if (j->reused) {
             auto& p = parts.at(r.partId);
             p.busy = false;
             p.used = Clock::now();
@@ -234,11 +319,77 @@ struct psx_server_s {
             const int64_t now = p.model->totalBytes();
             held += now - p.bytes;
             p.bytes = now;
-        } else if (own && r.partId >= 0 && r.res.status == PSX_OK && parts.find(r.partId) == parts.end()) {
+        } else if (j->own && r.partId >= 0 && r.res.status == PSX_OK && parts.find(r.partId) == parts.end()) {
             // keep the part resident (its bytes count against the budget until evicted)
-            const int64_t b = own->totalBytes();
+            const int64_t b = j->own->totalBytes();
             held += b;
-            parts[r.partId] = Resident{std::move(own), false, Clock::now(), b};
+            parts[r.partId] = Resident{std::move(j->own), false, Clock::now(), b};
+        }
+        held -= r.estimate;
+        running--;
+        r.finished = Clock::now();
+        r.res.queue_ms = msBetween(r.submitted, r.admitted);
+        r.res.total_ms = msBetween(r.submitted, r.finished);
+        r.done = true;
+        finished.notify_all();
+        wake.notify_all();
+        // j, with its model when that is not kept, is destroyed after the lock is released (a model's
+        // teardown waits on its stream)
+    }
+
+    // the pack: members join from the inbox at batch boundaries and leave when their runs stop
+    void packLoop() {
+        if (hipSetDevice(device) != hipSuccess) return;
+        std::unique_ptr<psx::PackedSolve> pk;
+        std::map<psx::Model*, std::unique_ptr<Job>> jobs;
+        const bool graphs = !psx::Model::graphsOff();
+        std::unique_lock<std::mutex> lk(mu);
+        for (;;) {
+            packWake.wait(lk, [&] { return packStop || !inbox.empty() || !jobs.empty(); });
+            if (packStop && inbox.empty() && jobs.empty()) return;
+            std::vector<std::unique_ptr<Job>> in = std::move(inbox);
+            inbox.clear();
+            lk.unlock();
+            std::vector<std::unique_ptr<Job>> out;
+            std::string err;
+            const int code = guarded([&] {
+                if (!pk) pk = std::make_unique<psx::PackedSolve>();
+                for (auto& j : in) {
+                    auto mb = std::make_unique<psx::PackedSolve::Member>();
+                    mb->m = j->M;
+                    mb->run = j->run;
+                    psx::Model* key = j->M;
+                    jobs[key] = std::move(j);
+                    pk->add(std::move(mb));
+                }
+                in.clear();
+                for (auto& mb : pk->step(graphs)) {
+                    auto it = jobs.find(mb->m);
+                    out.push_back(std::move(it->second));
+                    jobs.erase(it);
+                }
+                return PSX_OK;
+            });
+            if (code != PSX_OK) {
+                // a failed batch fails every member
+                err = g_error;
+                if (pk) pk->release();
+                pk.reset();
+                for (auto& j : in) out.push_back(std::move(j));
+                for (auto& [m, j] : jobs) out.push_back(std::move(j));
+                jobs.clear();
+                for (auto& j : out) {
+                    j->r->res.status = code;
+                    j->r->error = err;
+                }
+            }
+            for (auto& j : out) {
+                lk.lock();
+                packed--;
+                lk.unlock();
+                complete(std::move(j));
+            }
+            lk.lock();
         }
     }
 };
@@ -254,6 +405,8 @@ int psx_server_create(const psx_server_opts* opts, psx_server* out) {
         auto s = std::make_unique<psx_server_s>();
         HIPCHK(hipGetDevice(&s->device));
         const int slots = opts && opts->max_inflight > 0 ? opts->max_inflight : 4;
+        if (opts && opts->pack_max > 0) s->packMax = std::min(opts->pack_max, psx::PACK_MAX);
+        if (opts && opts->pack_max_dof > 0) s->packDof = opts->pack_max_dof;
         if (opts && opts->hbm_budget > 0) s->budget = opts->hbm_budget;
         else {
             size_t freeB = 0, total = 0;
@@ -261,6 +414,7 @@ int psx_server_create(const psx_server_opts* opts, psx_server* out) {
             s->budget = int64_t(0.8 * double(freeB));
         }
         for (int k = 0; k < slots; k++) s->workers.emplace_back([p = s.get()] { p->work(); });
+        if (s->packMax > 1) s->packer = std::thread([p = s.get()] { p->packLoop(); });
         *out = s.release();
         return PSX_OK;
     });
@@ -350,6 +504,13 @@ void psx_server_destroy(psx_server s) {
         s->finished.notify_all();
     }
     for (auto& t : s->workers) t.join();
+    // the workers have handed over every request they admitted: the pack finishes them, then stops
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        s->packStop = true;
+        s->packWake.notify_all();
+    }
+    if (s->packer.joinable()) s->packer.join();
     delete s;
 }
 

@@ -15,6 +15,7 @@
 #include "../../include/psx.h"
 #include "device.hpp"
 #include "abi.hpp"
+#include "packed.hpp"
 #include "model.hpp"
 #include "psx_core.hpp"
 
@@ -309,6 +310,42 @@ int psx_solve_multi_device(psx_model m, int nrhs, const double* d_f, const doubl
         const psx_solve_opts o = defaults(opts);
         m->m->solveMulti(nrhs, d_f, d_x0, o, d_u, info);
         HIPCHK(hipGetLastError());
+        return PSX_OK;
+    });
+}
+
+int psx_solve_packed(int n_models, const psx_model* models, const int* nrhs, const double* const* d_f, const double* const* d_x0,
+                     const psx_solve_opts* opts, double* const* d_u, psx_solve_info* info) {
+    return guarded([&] {
+        if (n_models < 1 || !models || !nrhs || !d_f || !d_u || !info) throw std::invalid_argument("bad arguments");
+        if (n_models > psx::PACK_MAX) throw std::invalid_argument("a pack holds at most 64 models");
+        std::vector<psx::Model*> ms(size_t(n_models), nullptr);
+        for (int i = 0; i < n_models; i++) {
+            if (!models[i] || nrhs[i] < 1 || !d_f[i] || !d_u[i]) throw std::invalid_argument("bad arguments");
+            ms[size_t(i)] = models[i]->m.get();
+            for (int j = 0; j < i; j++)
+                if (ms[size_t(j)] == ms[size_t(i)]) throw std::invalid_argument("a model can be in a pack once");
+        }
+        psx_solve_opts o = defaults(opts);
+        o.on_progress = nullptr;
+        // every model's run settings are reset however this ends
+        struct Reset {
+            std::vector<psx::Model*>& ms;
+            ~Reset() {
+                for (psx::Model* m : ms) m->resetRun();
+            }
+        } reset{ms};
+        psx::PackedSolve pk;
+        size_t at = 0;
+        for (int i = 0; i < n_models; i++) {
+            auto mb = std::make_unique<psx::PackedSolve::Member>();
+            mb->m = ms[size_t(i)];
+            if (mb->m->begin(mb->run, nrhs[i], d_f[i], d_x0 ? d_x0[i] : nullptr, o, d_u[i], info + at)) pk.add(std::move(mb));
+            else mb->m->resetRun();
+            at += size_t(nrhs[i]);
+        }
+        const bool graphs = !o.eager && !psx::Model::graphsOff();
+        while (pk.size() > 0) pk.step(graphs);
         return PSX_OK;
     });
 }

@@ -18,8 +18,9 @@ def _loads(w, nrhs, seed):
     return np.stack([f * (1 + k) + 0.01 * np.abs(f).max() * rng.standard_normal(f.size) * (f != 0) for k in range(nrhs)])
 
 
-@pytest.mark.parametrize("precision,device_build", [("fp64", True), ("mixed", True), ("mixed", False)])
-def test_served_results_equal_lone_solves(P, W, precision, device_build):
+# pack_max 0: the small parts run in the shared-launch pack; 1: each on its own stream
+@pytest.mark.parametrize("precision,device_build,pack_max", [("fp64", True, 0), ("mixed", True, 0), ("mixed", False, 0), ("mixed", True, 1)])
+def test_served_results_equal_lone_solves(P, W, precision, device_build, pack_max):
     parts = _parts(W)
     alone = []
     for q, (w, nrhs) in enumerate(parts):
@@ -27,7 +28,7 @@ def test_served_results_equal_lone_solves(P, W, precision, device_build):
         F = _loads(w, nrhs, q)
         alone.append(np.stack([m.solve(F[k], tol=1e-9, precision=precision)["u"] for k in range(nrhs)]))
         m.close()
-    srv = P.Server(max_inflight=3)
+    srv = P.Server(max_inflight=3, pack_max=pack_max)
     tickets = [srv.submit(w["dims"], w["density"], w["nu"], w["bc"], _loads(w, nrhs, q), coarsest_max_dof=300, tol=1e-9, precision=precision,
                           device_build=device_build)
                for q, (w, nrhs) in enumerate(parts)]
@@ -35,6 +36,7 @@ def test_served_results_equal_lone_solves(P, W, precision, device_build):
         u, res = srv.wait(t)
         print(parts[q][0]["name"], res, flush=True)
         assert res["status"] == 0 and res["converged"]
+        assert res["packed"] == (1 if pack_max != 1 else 0)
         assert np.array_equal(u, alone[q])
     assert srv.pending() == 0
     srv.close()

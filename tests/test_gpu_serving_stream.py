@@ -3,8 +3,9 @@ warm re-solves of an earlier part with 1 % of its solid voxels removed and the f
 pruned, warm-started from the earlier solution, as breakTest re-solves, static_study.cpp:219, 235)
 through the request loop, on a bounded prefix of the stream with edges capped at 128 voxels.
 
-Every served result must equal the same request solved alone on its own model, bit for bit, and the
-three smallest parts' served solutions must match the oracle's FP64 VoxelFEA::solve to 1e-6."""
+Parts of at most 3 * 2^20 DOFs run in the server's shared-launch pack here (pack_max_dof), the
+larger ones on their own streams. Every served result must equal the same request solved alone on its own model, bit for bit,
+and the three smallest parts' served solutions must match the oracle's FP64 VoxelFEA::solve to 1e-6."""
 import numpy as np
 import pytest
 
@@ -34,7 +35,7 @@ def test_serving_stream_prefix(P, W, cpu):
     cold = [r for r in reqs if not r["warm"]]
     assert len(warm) >= 4 and len(cold) >= 8
     parts = {r["id"]: W.void_block(r["dims"], seed=r["part_seed"], nu=r["nu"], E=r["E"]) for r in cold}
-    srv = P.Server(max_inflight=4)
+    srv = P.Server(max_inflight=4, pack_max_dof=3 << 20)
     served, results = {}, {}
     # the cold requests together, then the warm re-solves together (each needs its parent's result)
     for phase in (cold, warm):
@@ -50,6 +51,9 @@ def test_serving_stream_prefix(P, W, cpu):
             served[r["id"]] = u
             results[r["id"]] = (args[r["id"]], res)
     srv.close()
+    packed = sum(res["packed"] for _, res in results.values())
+    print("packed", packed, "of", len(results), flush=True)
+    assert packed >= 8
     # each alone: its own model, the same multi-instance solve
     for r in reqs:
         (dims, dens, nu, bc, F, x0), res = results[r["id"]]
