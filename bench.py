@@ -212,6 +212,48 @@ def serve(args, P, W):
     print(json.dumps(serving_leg(P, W, args.serve, args.rate, args.inflight, args.tol, args.pack_max, args.pack_max_dof, args.edge_max)), flush=True)
 
 
+def packed_leg(P, W, n_parts, tol, reps=3, eager=False):
+    """Packed solves in isolation: the first n_parts cold parts of the serving stream (seed 5) of at
+    most 3 * 2^20 DOFs, each with its 1-8 load cases, solved one model after another
+    (psx_solve_multi_device, each model's own graph) and all together (psx_solve_packed, shared
+    launches). Hierarchies are built and the smoothers estimated before the timed region; the
+    packed results must equal the lone ones bit for bit."""
+    reqs = [r for r in W.serving_stream(400, seed=5) if not r["warm"] and 3 * W.nodes(r["dims"]) <= 3 << 20][:n_parts]
+    parts = []
+    for r in reqs:
+        w = W.void_block(r["dims"], seed=r["part_seed"], nu=r["nu"], E=r["E"])
+        parts.append((w, W.serving_loads(w, r["nrhs"], r["part_seed"])))
+    models = [P.Model(w["dims"], w["density"], w["nu"], w["bc"], 1200, device_build=True) for w, _ in parts]
+    for m in models:
+        m.estimate_smoothers()
+    d_f = [P.DeviceArray(F.size, F.reshape(-1)) for _, F in parts]
+    d_u = [P.DeviceArray(F.size) for _, F in parts]
+    nr = [F.shape[0] for _, F in parts]
+    kw = dict(tol=tol, max_iter=2000, precision="mixed", eager=eager)
+    lone_t, pack_t, same = [], [], True
+    for rep in range(reps + 1):
+        t = time.perf_counter()
+        for m, f, u, k in zip(models, d_f, d_u, nr):
+            m.solve_multi_device(k, f, u, **kw)
+        lone_t.append(time.perf_counter() - t)
+        U = [u.download() for u in d_u]
+        t = time.perf_counter()
+        infos = P.solve_packed(models, d_f, d_u, nr, **kw)
+        pack_t.append(time.perf_counter() - t)
+        same = same and all(np.array_equal(a, u.download()) for a, u in zip(U, d_u))
+        if not all(i["converged"] for inf in infos for i in inf):
+            raise RuntimeError("a packed solve did not converge")
+    for m in models:
+        m.close()
+    lone, pack = float(np.mean(lone_t[1:])), float(np.mean(pack_t[1:]))
+    cases = sum(nr)
+    return {"metric": "packed solves: load cases/s of small serving parts in shared launches vs one model after another",
+            "parts": len(parts), "load_cases": cases, "dof_max": int(max(3 * W.nodes(w["dims"]) for w, _ in parts)),
+            "lone_sequential_ms": round(1e3 * lone, 2), "packed_ms": round(1e3 * pack, 2), "speedup": round(lone / pack, 3),
+            "solves_per_s_lone": round(cases / lone, 1), "solves_per_s_packed": round(cases / pack, 1),
+            "bitwise_equal": bool(same), "eager": eager, "tol": tol}
+
+
 def batched(args, P, W):
     dims = tuple(int(v) for v in args.dims.split(",")) if args.dims != "512,128,128" else (128, 64, 64)
     if args.profile_solves:
@@ -347,12 +389,14 @@ def main():
     ap.add_argument("--pack-max", type=int, default=0, help="serving: requests in the shared-launch pack (0 = the server's 16, 1 = no packing)")
     ap.add_argument("--edge-max", type=int, default=256, help="serving: longest part edge of the stream (voxels)")
     ap.add_argument("--pack-max-dof", type=int, default=0, help="serving: largest part (DOFs) that joins the pack (0 = the server's 100 000)")
+    ap.add_argument("--packed", type=int, default=0, help="packed solves: this many small parts of the serving stream, lone vs one pack")
     ap.add_argument("--batched", action="store_true", help="batched load cases: the 128x64x64 SIMP block, --nrhs cases in one multi-instance solve")
     ap.add_argument("--nrhs", type=int, default=16)
     ap.add_argument("--profile-solves", default="", choices=["", "mixed", "fp64", "ref"],
                     help="profiling runs: build on the device, then --steps solves of this precision only (no reference solve, no CPU leg)")
     ap.add_argument("--build-only", action="store_true",
                     help="hierarchy build + smoother estimates of the workload, on the device and on the host, --steps times each")
+    ap.add_argument("--reps", type=int, default=50, help="back-to-back launches per kernel of the per-level timings (the roofline figure)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="counter runs: three 1 GiB device copies (the calibration kernel), then the per-level operator and V-cycle timings")
     args = ap.parse_args()
@@ -366,6 +410,9 @@ def main():
         return
     if args.batched:
         batched(args, P, W)
+        return
+    if args.packed:
+        print(json.dumps(packed_leg(P, W, args.packed, args.tol, max(1, args.steps // 2), eager=bool(os.environ.get("PSX_GRAPHS") == "0"))), flush=True)
         return
 
     dims = tuple(int(v) for v in args.dims.split(","))
@@ -401,7 +448,8 @@ def main():
         a, b = P.DeviceArray(1 << 27), P.DeviceArray(1 << 27)
         for _ in range(3):
             b.copy_from(a)
-        print(m.profile(reps=5), flush=True)
+        # the same launches and repetitions as the bench line's roofline figure
+        print(m.profile(reps=args.reps), flush=True)
         return
     d_f = P.DeviceArray(n_dof, w["f"])
     d_u = P.DeviceArray(n_dof)
@@ -441,7 +489,7 @@ def main():
 
     # roofline of the dominant kernel: the fine-level FP32 operator y = K x, timed with HIP events on
     # the model's stream over back-to-back launches
-    prof = m.profile(reps=50)
+    prof = m.profile(reps=args.reps)
     line0 = [l for l in prof.splitlines() if l.startswith("L0 ")][0]
     t_apply32 = float(line0.split("matvec32")[1].split("ms")[0]) * 1e-3
     nodes = W.nodes(dims)
@@ -465,7 +513,8 @@ def main():
         m.close()
         for name, fn in (("batched", lambda: batched_leg(P, W, (128, 64, 64), 16, max(3, args.steps // 2), 1, args.tol)),
                          ("serving", lambda: serving_leg(P, W, args.serve_requests, args.rate, args.inflight, args.tol, args.pack_max,
-                                                         args.pack_max_dof))):
+                                                         args.pack_max_dof)),
+                         ("packed", lambda: packed_leg(P, W, 16, args.tol))):
             t = time.perf_counter()
             try:
                 extras[name] = fn()
