@@ -39,6 +39,11 @@ struct DevLevel {
     const int32_t* childTwo;  // 4 per own element, 16-byte aligned
     const double* ctab64;
     const float* ctab32;
+    // the subset table in FP16 for the mixed V-cycle's smoothing: row B (64 entries) is ctab's row B over
+    // ctabRowScale[B], the power of two that puts the row's largest entry in (2^13, 2^14] (null on levels
+    // without the child-scale form)
+    const uint16_t* ctab16;
+    const float* ctabRowScale;
     int childForm;
     // kFine sums the 8 voxels of every node directly (no stencil, no pair lists): set on levels
     // without own elements where most nodes have differing voxels (per-voxel densities)
@@ -113,8 +118,10 @@ void launchApplyOwnMf(const DevLevel& L, int mode, const float* x, float* y, con
 // LDS planes, class-sum stencil in difference form + one lane per differing voxel (fine.hip)
 // mf: FP32 levels without own elements take the in-plane stencil on the matrix cores (BF16 operands,
 // FP32 accumulation; for the V-cycle's smoothing steps)
+// half: FP32 levels with child-scale own elements read the FP16 subset table (the mixed V-cycle's
+// smoothing)
 template <class T> void launchFine(const DevLevel& L, int mode, const T* x, T* y, const T* r, const T* zp, double c1, double c2, const int* stop,
-                                   hipStream_t s, int ninst = 1, bool mf = false);
+                                   hipStream_t s, int ninst = 1, bool mf = false, bool half = false);
 // Dot products fused into an operator launch (conjugate gradients), block partials into red:
 //   FD_PQ  (AM_APPLY)  p.q with p = x, q = y                         -> launchFinishPQ
 //   FD_RZ  (AM_CHEB1/2) r.z and z.q with z = y, q = qdot             -> launchFinishRZ
@@ -126,7 +133,7 @@ struct DotFusion {
 };
 // launchFine with a fused dot; returns the number of block partials written (0: not fused)
 template <class T> unsigned launchFineDot(const DevLevel& L, int mode, const T* x, T* y, const T* r, const T* zp, double c1, double c2, const int* stop,
-                                          const DotFusion& fd, hipStream_t s, int ninst = 1, bool mf = false);
+                                          const DotFusion& fd, hipStream_t s, int ninst = 1, bool mf = false, bool half = false);
 void launchFinishPQ(Reduce red, unsigned nb, CgState* st, hipStream_t s, int ninst = 1);
 void launchFinishRZ(Reduce red, unsigned nb, CgState* st, hipStream_t s, int ninst = 1);
 // z = c0 D^-1 r (first Chebyshev step from zero)

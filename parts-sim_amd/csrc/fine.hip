@@ -129,6 +129,7 @@ template <class T, bool OWNC> struct Ring {
     T x[4][3][PY][PX];                 // node planes k-2 .. k+1, one array per component
     float s[3][VY][VX];                // voxel-scale planes (0 = empty, outside or own matrix)
     int own[OWNC ? 3 : 1][VY][VX];     // own-matrix index of the voxel, -1 for s * Kb
+    int4 two[OWNC ? 3 : 1][OWNC ? VY : 1][OWNC ? VX : 1];  // the own voxel's two-scale record (childTwo)
     T pres[3][NT];                     // results of one pass of pairs
     uint16_t list[NW][WAVE_PAIRS];     // pairs: local node << 6 | reference voxel << 3 | voxel
     int waveTot[2][NW];
@@ -152,7 +153,10 @@ template <class T, bool OWNC> struct Ring {
 // the whole march; a node plane's rows are complete one step after its upper neighbour plane and are
 // read out by lane shuffles, then cleared for the plane three steps on. The centre column term and
 // the differing-voxel pairs stay FP32, as in the VALU form.
-template <class T, int MODE, bool OWNC, bool DOT, bool DENSE = false, bool MF = false>
+// H16 (FP32 levels with child-scale own elements, the mixed V-cycle's smoothing): the two-scale
+// elements read the subset table in FP16 (ctab16, each row times its power-of-two ctabRowScale), which
+// fits the compute unit's cache where the FP32 table does not
+template <class T, int MODE, bool OWNC, bool DOT, bool DENSE = false, bool MF = false, bool H16 = false>
 __device__ __forceinline__ void fineBody(VBlk vb, PSX_FINE_PARAMS) {
     // multi-instance launches: the block's y is the right-hand side (its vectors, stop flag, partials)
     if (stop) stop += int64_t(vb.y) * CG_INTS;
@@ -243,6 +247,25 @@ __device__ __forceinline__ void fineBody(VBlk vb, PSX_FINE_PARAMS) {
             }
         }
     };
+    // the two-scale records of a voxel plane's own voxels, loaded as soon as the plane's words are
+    // known and written to LDS just before the step's barrier: the latency hides behind the stencil
+    // instead of sitting in every own pair's pass (range-checked: other voxels read 0)
+    const auto twoRsrc = planeRsrc(L.childTwo, OWNC && L.childTwo ? 0x7fffffffu : 0u);
+    auto fetchTwo = [&](const int32_t (&pv)[NVL], int4 (&tw)[NVL]) {
+#pragma unroll
+        for (int q = 0; q < NVL; q++) {
+            const int32_t off = pv[q] < 0 ? int32_t(16 * (pv[q] & 0x7fffffff)) : OUT_OF_RANGE;
+            const auto u = __builtin_amdgcn_raw_buffer_load_b128(twoRsrc, off, 0, 0);
+            tw[q] = int4{int(u[0]), int(u[1]), int(u[2]), int(u[3])};
+        }
+    };
+    auto storeTwo = [&](int slot, const int4 (&tw)[NVL]) {
+#pragma unroll
+        for (int q = 0; q < NVL; q++) {
+            const int e = tid + q * NT;
+            if (e < VX * VY) sh.two[slot][e / VX][e % VX] = tw[q];
+        }
+    };
     // the epilogue's own operands of node (i, j, z), fetched one step before use; info = the node's
     // differing-voxel bits (0..7) and held-DOF bits (8..10)
     constexpr bool NEEDS_INV = MODE == AM_CHEB1 || MODE == AM_CHEB2;
@@ -317,6 +340,8 @@ __device__ __forceinline__ void fineBody(VBlk vb, PSX_FINE_PARAMS) {
         // stores of the previous step's fetches, then the next fetches
         storePlane((k + 1) & 3, pn);
         storeVox((k + 2) % 3, vn);  // voxel plane k - 1
+        int4 twv[NVL];
+        if constexpr (OWNC) fetchTwo(vn, twv);
         fetchPlane(k + 2, pn);
         fetchVox(k, vn);
         Own nxt;
@@ -446,6 +471,7 @@ __device__ __forceinline__ void fineBody(VBlk vb, PSX_FINE_PARAMS) {
             }
         }
         if (!DENSE && lane == 0) sh.waveTot[k & 1][wid] = waveTotal;
+        if constexpr (OWNC) storeTwo((k + 2) % 3, twv);
         __syncthreads();
 
         if (done) {
@@ -556,7 +582,7 @@ __device__ __forceinline__ void fineBody(VBlk vb, PSX_FINE_PARAMS) {
                             // own element sum_c s_c M_c, mirrored: child c' of the mirrored element
                             // is child c' ^ (7 ^ v) of the original
                             const int mb = 7 ^ v;
-                            const int4 tw = *reinterpret_cast<const int4*>(L.childTwo + 4 * int64_t(oi));
+                            const int4 tw = sh.two[dk ? v0 : vm][oy0 + dj][ox0 + di];
                             T xs[8][3];
 #pragma unroll
                             for (int q = 1; q < 8; q++)
@@ -568,15 +594,27 @@ __device__ __forceinline__ void fineBody(VBlk vb, PSX_FINE_PARAMS) {
                                 uint32_t Bp = 0;
 #pragma unroll
                                 for (int c = 0; c < 8; c++) Bp |= ((uint32_t(tw.z) >> (c ^ mb)) & 1u) << c;
-                                const T* __restrict__ tab = ctab + 64 * Bp;
                                 T u[3];
                                 uniformRows(u);
+                                if constexpr (H16) {
+                                    const _Float16* __restrict__ tab = reinterpret_cast<const _Float16*>(L.ctab16) + 64 * Bp;
+                                    const T scale = T(L.ctabRowScale[Bp]);
 #pragma unroll
-                                for (int d = 0; d < 3; d++) {
-                                    T t = 0;
+                                    for (int d = 0; d < 3; d++) {
+                                        T t = 0;
 #pragma unroll
-                                    for (int k2 = 0; k2 < 21; k2++) t += tab[d * 21 + k2] * xs[1 + k2 / 3][k2 % 3];
-                                    res[d] = (sa - sref) * u[d] + (sb - sa) * sg[d] * t;
+                                        for (int k2 = 0; k2 < 21; k2++) t += T(float(tab[d * 21 + k2])) * xs[1 + k2 / 3][k2 % 3];
+                                        res[d] = (sa - sref) * u[d] + (sb - sa) * sg[d] * (t * scale);
+                                    }
+                                } else {
+                                    const T* __restrict__ tab = ctab + 64 * Bp;
+#pragma unroll
+                                    for (int d = 0; d < 3; d++) {
+                                        T t = 0;
+#pragma unroll
+                                        for (int k2 = 0; k2 < 21; k2++) t += tab[d * 21 + k2] * xs[1 + k2 / 3][k2 % 3];
+                                        res[d] = (sa - sref) * u[d] + (sb - sa) * sg[d] * t;
+                                    }
                                 }
                             } else {
                                 const float* cs = L.childS + 8 * int64_t(oi);
@@ -704,12 +742,24 @@ __device__ __forceinline__ void fineBody(VBlk vb, PSX_FINE_PARAMS) {
 
 #define PSX_FINE_FWD L, C, crow, ctab, x, y, r, zp, c1, c2, stop, zChunk, gx, gy, nTiles, xcdOrder, dots, dotStride, qdot
 // the kernel bodies for packed launches: WPE_ as the named kernel's waves-per-SIMD attribute
-template <class T, int MODE, bool OWNC, bool DOT, bool DENSE, bool MF, int WPE_> struct Fine {
+template <class T, int MODE, bool OWNC, bool DOT, bool DENSE, bool MF, int WPE_, bool H16 = false> struct Fine {
     static constexpr int NTH = NT, WPE = WPE_;
-    __device__ static void run(VBlk vb, PSX_FINE_PARAMS) { fineBody<T, MODE, OWNC, DOT, DENSE, MF>(vb, PSX_FINE_FWD); }
+    __device__ static void run(VBlk vb, PSX_FINE_PARAMS) { fineBody<T, MODE, OWNC, DOT, DENSE, MF, H16>(vb, PSX_FINE_FWD); }
 };
 template <class T, int MODE, bool OWNC, bool DOT> __global__ __launch_bounds__(NT) void kFine(PSX_FINE_PARAMS) {
     fineBody<T, MODE, OWNC, DOT>(ownBlock(), PSX_FINE_FWD);
+}
+// FP32 child-scale levels reading the FP16 subset table: held to 168 VGPRs (3 waves per SIMD, as the
+// FP32-table kernels reach by themselves; the later Chebyshev step would otherwise take 174 and 2 waves)
+template <int MODE, bool DOT>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void kFineH(DevLevel L, Sym<float> C, const float* __restrict__ crow,
+                                                                                  const float* __restrict__ ctab, const float* __restrict__ x,
+                                                                                  float* __restrict__ y, const float* __restrict__ r,
+                                                                                  const float* __restrict__ zp, float c1, float c2, const int* stop,
+                                                                                  int zChunk, int gx, int gy, int nTiles, int xcdOrder,
+                                                                                  double* __restrict__ dots, int64_t dotStride,
+                                                                                  const float* __restrict__ qdot) {
+    fineBody<float, MODE, true, DOT, false, false, true>(ownBlock(), PSX_FINE_FWD);
 }
 // FP32 without own elements: held to 128 VGPRs (4 waves per SIMD instead of 3), which it fits
 // without spilling
@@ -735,7 +785,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void kF
 #undef PSX_FINE_FWD
 
 // launches (or records) the operator kernel of one form, mode and dot choice
-template <class T, int MODE, bool OWNC, bool DOT, bool DENSE, bool MF>
+template <class T, int MODE, bool OWNC, bool DOT, bool DENSE, bool MF, bool H16 = false>
 void fineGo(dim3 g, hipStream_t s, const DevLevel& L, const Sym<T>& C, const T* crow, const T* ctab, const T* x, T* y, const T* r, const T* zp, T c1, T c2,
             const int* stop, int zChunk, int gx, int gy, int nTiles, int xcdOrder, double* dots, int64_t dotStride, const T* qdot) {
 #define PSX_FINE_ARGS L, C, crow, ctab, x, y, r, zp, c1, c2, stop, zChunk, gx, gy, nTiles, xcdOrder, dots, dotStride, qdot
@@ -746,6 +796,9 @@ void fineGo(dim3 g, hipStream_t s, const DevLevel& L, const Sym<T>& C, const T* 
         go<Fine<T, MODE, false, DOT, true, false, 0>>(kFineDense<T, MODE, DOT>, g, s, PSX_FINE_ARGS);
     } else if constexpr (sizeof(T) == 4 && !OWNC) {
         go<Fine<T, MODE, false, DOT, false, false, 4>>(kFine4<T, MODE, DOT>, g, s, PSX_FINE_ARGS);
+    } else if constexpr (H16) {
+        static_assert(sizeof(T) == 4 && OWNC, "FP16 subset table: FP32 child-scale levels");
+        go<Fine<float, MODE, true, DOT, false, false, 3, true>>(kFineH<MODE, DOT>, g, s, PSX_FINE_ARGS);
     } else {
         go<Fine<T, MODE, OWNC, DOT, false, false, 0>>(kFine<T, MODE, OWNC, DOT>, g, s, PSX_FINE_ARGS);
     }
@@ -756,7 +809,8 @@ void fineGo(dim3 g, hipStream_t s, const DevLevel& L, const Sym<T>& C, const T* 
 
 template <class T, bool OWNC>
 static void launchFineT(const DevLevel& L, int mode, const Sym<T>& C, const T* crow, const T* ctab, const T* x, T* y, const T* r, const T* d, double c1, double c2,
-                        const int* stop, int zChunk, double* dots, int64_t dotStride, const T* qdot, hipStream_t s, int ninst, bool mf) {
+                        const int* stop, int zChunk, double* dots, int64_t dotStride, const T* qdot, hipStream_t s, int ninst, bool mf,
+                        bool half = false) {
     const int gx = (L.NX + TX - 1) / TX, gy = (L.NY + TY - 1) / TY, gz = (L.NZ + zChunk - 1) / zChunk;
     const int nTiles = gx * gy * gz;
     const dim3 g(unsigned(8 * ((nTiles + 7) / 8)), unsigned(ninst));  // whole rounds over the 8 XCDs
@@ -770,6 +824,7 @@ static void launchFineT(const DevLevel& L, int mode, const Sym<T>& C, const T* c
     // fused dots exist for APPLY (p.q) and the Chebyshev steps (r.z, z.q) only
     const bool dot = dots && (mode == AM_APPLY || mode == AM_CHEB1 || mode == AM_CHEB2);
     const bool useMf = sizeof(T) == 4 && !OWNC && mf && !L.dense, dense = !OWNC && L.dense;
+    const bool useH16 = sizeof(T) == 4 && OWNC && half && L.ctab16;
     auto form = [&](auto m, auto dt) {
         constexpr int M = decltype(m)::value;
         constexpr bool DT = decltype(dt)::value;
@@ -783,6 +838,13 @@ static void launchFineT(const DevLevel& L, int mode, const Sym<T>& C, const T* c
         if constexpr (!OWNC) {
             if (dense) {
                 fineGo<T, M, false, DT, true, false>(g, s, L, C, crow, ctab, x, y, r, d, f1, f2, stop, zChunk, gx, gy, nTiles, xcdOrder, dots, dotStride, qdot);
+                return;
+            }
+        }
+        if constexpr (sizeof(T) == 4 && OWNC) {
+            if (useH16) {
+                fineGo<T, M, OWNC, DT, false, false, true>(g, s, L, C, crow, ctab, x, y, r, d, f1, f2, stop, zChunk, gx, gy, nTiles, xcdOrder, dots, dotStride,
+                                                           qdot);
                 return;
             }
         }
@@ -810,7 +872,7 @@ static void launchFineT(const DevLevel& L, int mode, const Sym<T>& C, const T* c
 
 template <class T>
 unsigned launchFineDot(const DevLevel& L, int mode, const T* x, T* y, const T* r, const T* d, double c1, double c2, const int* stop, const DotFusion& fd,
-                       hipStream_t s, int ninst, bool mf) {
+                       hipStream_t s, int ninst, bool mf, bool half) {
     Sym<T> C;
     for (int q = 0; q < 22; q++) C.v[q] = sizeof(T) == 8 ? T(L.sym64[q]) : T(L.sym32[q]);
     // the longest z-chunk (16, 8 or 4 planes) that still gives >= 4096 blocks (16 per CU); shorter
@@ -838,7 +900,7 @@ unsigned launchFineDot(const DevLevel& L, int mode, const T* x, T* y, const T* r
         const T *crow = nullptr, *ctab = nullptr;
         if constexpr (sizeof(T) == 8) crow = L.crow64, ctab = L.ctab64;
         else crow = L.crow32, ctab = L.ctab32;
-        launchFineT<T, true>(L, mode, C, crow, ctab, x, y, r, d, c1, c2, stop, zChunk, dots, fd.red.stride, qdot, s, ninst, false);
+        launchFineT<T, true>(L, mode, C, crow, ctab, x, y, r, d, c1, c2, stop, zChunk, dots, fd.red.stride, qdot, s, ninst, false, half);
     } else {
         launchFineT<T, false>(L, mode, C, nullptr, nullptr, x, y, r, d, c1, c2, stop, zChunk, dots, fd.red.stride, qdot, s, ninst, mf);
     }
@@ -847,16 +909,17 @@ unsigned launchFineDot(const DevLevel& L, int mode, const T* x, T* y, const T* r
 
 template <class T>
 void launchFine(const DevLevel& L, int mode, const T* x, T* y, const T* r, const T* d, double c1, double c2, const int* stop, hipStream_t s, int ninst,
-                bool mf) {
-    launchFineDot<T>(L, mode, x, y, r, d, c1, c2, stop, DotFusion{}, s, ninst, mf);
+                bool mf, bool half) {
+    launchFineDot<T>(L, mode, x, y, r, d, c1, c2, stop, DotFusion{}, s, ninst, mf, half);
 }
 
 template unsigned launchFineDot<double>(const DevLevel&, int, const double*, double*, const double*, const double*, double, double, const int*,
-                                        const DotFusion&, hipStream_t, int, bool);
+                                        const DotFusion&, hipStream_t, int, bool, bool);
 template unsigned launchFineDot<float>(const DevLevel&, int, const float*, float*, const float*, const float*, double, double, const int*,
-                                       const DotFusion&, hipStream_t, int, bool);
+                                       const DotFusion&, hipStream_t, int, bool, bool);
 template void launchFine<double>(const DevLevel&, int, const double*, double*, const double*, const double*, double, double, const int*, hipStream_t, int,
-                                 bool);
-template void launchFine<float>(const DevLevel&, int, const float*, float*, const float*, const float*, double, double, const int*, hipStream_t, int, bool);
+                                 bool, bool);
+template void launchFine<float>(const DevLevel&, int, const float*, float*, const float*, const float*, double, double, const int*, hipStream_t, int, bool,
+                                bool);
 
 }  // namespace psx

@@ -284,6 +284,22 @@ public:
         std::vector<float> tab32(L.childTab.begin(), L.childTab.end());
         D.ctab64 = upload(L.childTab.data(), int64_t(L.childTab.size()));
         D.ctab32 = upload(tab32.data(), int64_t(tab32.size()));
+        // FP16 copy, each row of 64 with its own power-of-two scale (the rule of the FP16 own matrices)
+        const size_t nRows = L.childTab.size() / 64;
+        std::vector<uint16_t> tab16(L.childTab.size());
+        std::vector<float> rowScale(nRows);
+        for (size_t b = 0; b < nRows; b++) {
+            double mx = 0;
+            for (size_t i = 0; i < 64; i++) mx = std::max(mx, std::abs(L.childTab[64 * b + i]));
+            const double sc = mx > 0 ? std::exp2(std::ceil(std::log2(mx / 16384.0))) : 1.0;
+            rowScale[b] = float(sc);
+            for (size_t i = 0; i < 64; i++) {
+                const _Float16 h = static_cast<_Float16>(L.childTab[64 * b + i] / sc);
+                std::memcpy(&tab16[64 * b + i], &h, sizeof h);
+            }
+        }
+        D.ctab16 = upload(tab16.data(), int64_t(tab16.size()));
+        D.ctabRowScale = upload(rowScale.data(), int64_t(nRows));
     }
     // level 0's held DOFs, active nodes and element list (element -> voxel, in voxel order) on the
     // host (reactions, the break and topology loops); a device-built model downloads its voxel words
@@ -383,15 +399,17 @@ public:
         const bool mf = sizeof(T) == 4 && (form & OF_MFMA) && H.levels[l].nOwn == 0 && !D.dense;
         if (!useFine) launchApply<T>(D, mode, xx, yy, rr, dd, c1, c2, stop, stream, ni);
         else if (D.symOk && (H.levels[l].nOwn == 0 || (D.childForm && ((form & OF_CHILD) || ((form & OF_AUTO) && lv[l].childCheap))))) {
+            // the mixed V-cycle's smoothing (OF_AUTO, FP32) reads the child-scale subset table in FP16
+            const bool half = (form & OF_AUTO) != 0;
             if (fd) {
-                const unsigned nb = launchFineDot<T>(D, mode, xx, yy, rr, dd, c1, c2, stop, *fd, stream, ni, mf);
+                const unsigned nb = launchFineDot<T>(D, mode, xx, yy, rr, dd, c1, c2, stop, *fd, stream, ni, mf, half);
                 if (nb) {
                     if (fd->kind == FD_PQ) launchFinishPQ(fd->red, nb, st, stream, ni);
                     else launchFinishRZ(fd->red, nb, st, stream, ni);
                     return;
                 }
             } else {
-                launchFine<T>(D, mode, xx, yy, rr, dd, c1, c2, stop, stream, ni, mf);
+                launchFine<T>(D, mode, xx, yy, rr, dd, c1, c2, stop, stream, ni, mf, half);
             }
         } else if constexpr (sizeof(T) == 4) {
             if (form & OF_MFMA_OWN) launchApplyOwnMf(D, mode, xx, yy, rr, dd, c1, c2, stop, stream, ni);

@@ -82,6 +82,8 @@ __device__ __forceinline__ DevLevel globalArg(DevLevel v) {
     v.childTwo = asGlobal(v.childTwo);
     v.ctab64 = asGlobal(v.ctab64);
     v.ctab32 = asGlobal(v.ctab32);
+    v.ctab16 = asGlobal(v.ctab16);
+    v.ctabRowScale = asGlobal(v.ctabRowScale);
     v.invD64 = asGlobal(v.invD64);
     v.invD32 = asGlobal(v.invD32);
     v.diagAdd = asGlobal(v.diagAdd);
