@@ -97,3 +97,43 @@ def test_packed_single_and_errors(P, W):
     assert e.value.code == P.ERR_ARGUMENT
     for m in models:
         m.close()
+
+
+def test_packed_edge_members(P, W):
+    """Members whose runs need no iteration (a zero load, a start at the solution) and a part with a
+    diagonal shift, packed with ordinary members: each equals its own solve."""
+    parts = _parts(W)[:3]
+    w = W.cantilever(4)
+    shift = np.full(3 * W.nodes(w["dims"]), 1e6)
+    m_shift = P.Model(w["dims"], w["density"], w["nu"], w["bc"], 300, diag_add=shift)
+    models = _models(P, parts) + [m_shift]
+    parts = parts + [(w, w["f"][None, :])]
+    F0 = parts[0][1].copy()
+    F0[1] = 0.0  # a zero load case beside loaded ones
+    parts[0] = (parts[0][0], F0)
+    exact = _lone(P, models, parts, tol=1e-10, precision="mixed")
+    # model 1 starts at its solution: no iteration; the others from zero
+    x0s = [None, exact[1][0], None, None]
+    lone = _lone(P, models, parts, x0s=x0s, tol=1e-8, precision="mixed")
+    packed = _packed(P, models, parts, x0s=x0s, tol=1e-8, precision="mixed")
+    assert lone[1][1][0]["iterations"] == 0 and lone[0][1][1]["iterations"] == 0
+    _same(lone, packed)
+    for m in models:
+        m.close()
+
+
+def test_packed_argument_errors(P, W):
+    parts = _parts(W)[:1]
+    models = _models(P, parts)
+    F = parts[0][1]
+    d_f, d_u = P.DeviceArray(F.size, F.reshape(-1)), P.DeviceArray(F.size)
+    with pytest.raises(P.PsxError) as e:
+        P.solve_packed(models, [d_f], [d_u], [0])  # no load cases
+    assert e.value.code == P.ERR_ARGUMENT
+    with pytest.raises(P.PsxError) as e:
+        P.solve_packed(models * 65, [d_f] * 65, [d_u] * 65, [F.shape[0]] * 65)
+    assert e.value.code == P.ERR_ARGUMENT and "64" in str(e.value)
+    # the model still solves normally afterwards
+    _same(_lone(P, models, parts, tol=1e-8, precision="mixed"), _packed(P, models, parts, tol=1e-8, precision="mixed"))
+    for m in models:
+        m.close()
