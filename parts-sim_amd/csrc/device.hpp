@@ -138,8 +138,11 @@ void launchFinishPQ(Reduce red, unsigned nb, CgState* st, hipStream_t s, int nin
 void launchFinishRZ(Reduce red, unsigned nb, CgState* st, hipStream_t s, int ninst = 1);
 // z = c0 D^-1 r (first Chebyshev step from zero)
 template <class T> void launchChebFirst(const DevLevel& L, const T* r, T* z, double c0, const int* stop, hipStream_t s, int ninst = 1);
-// rc = P^T rf on the coarse level's free DOFs
-template <class T> void launchRestrict(const DevLevel& F, const DevLevel& C, const T* rf, T* rc, const int* stop, hipStream_t s, int ninst = 1);
+// rc = P^T rf on the coarse level's free DOFs; z0 (when not null) = c0 D_C^-1 rc, the coarse level's
+// first Chebyshev step
+template <class T>
+void launchRestrict(const DevLevel& F, const DevLevel& C, const T* rf, T* rc, const int* stop, hipStream_t s, int ninst = 1, T* z0 = nullptr,
+                    double c0 = 0);
 // zf += P zc on the fine level's free DOFs
 template <class T> void launchProlong(const DevLevel& F, const DevLevel& C, const T* zc, T* zf, const int* stop, hipStream_t s, int ninst = 1);
 // coarsest level: z[freeIdx] = Ainv r[freeIdx], z = 0 on held DOFs (dense[dof] = dense index or -1)

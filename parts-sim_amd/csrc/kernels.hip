@@ -539,19 +539,26 @@ template <class T> __global__ __launch_bounds__(BLOCK) void kChebFirst(const T* 
 // ------------------------------------------------------------------------------------------------
 // grid transfers (voxel_fea.cpp:738-815): full weighting P^T and trilinear P, coarse node I <-> fine 2I
 
+// z0 (when not null): also the coarse level's first Chebyshev step z0 = c0 D^-1 rc, as kChebFirst
+// computes it (the coarse cycle then starts from it)
 template <class T> struct Restrict {
     static constexpr int NTH = BLOCK, WPE = 0;
-    __device__ static void run(VBlk vb, DevLevel F, DevLevel C, const T* __restrict__ rf, T* __restrict__ rc, const int* stop);
+    __device__ static void run(VBlk vb, DevLevel F, DevLevel C, const T* __restrict__ rf, T* __restrict__ rc, const int* stop, T* __restrict__ z0,
+                               T c0);
 };
 template <class T>
-__global__ __launch_bounds__(BLOCK) void kRestrict(DevLevel F, DevLevel C, const T* __restrict__ rf, T* __restrict__ rc, const int* stop) {
-    Restrict<T>::run(ownBlock(), F, C, rf, rc, stop);
+__global__ __launch_bounds__(BLOCK) void kRestrict(DevLevel F, DevLevel C, const T* __restrict__ rf, T* __restrict__ rc, const int* stop,
+                                                    T* __restrict__ z0, T c0) {
+    Restrict<T>::run(ownBlock(), F, C, rf, rc, stop, z0, c0);
 }
-template <class T> __device__ void Restrict<T>::run(VBlk vb, DevLevel F, DevLevel C, const T* __restrict__ rf, T* __restrict__ rc, const int* stop) {
+template <class T>
+__device__ void Restrict<T>::run(VBlk vb, DevLevel F, DevLevel C, const T* __restrict__ rf, T* __restrict__ rc, const int* stop, T* __restrict__ z0,
+                                 T c0) {
     stop = instFlag(vb, stop);
     if (stopped(stop)) return;
     rf += instOffset(vb, F.nDof);
     rc += instOffset(vb, C.nDof);
+    if (z0) z0 += instOffset(vb, C.nDof);
     const int64_t cn = int64_t(vb.x) * BLOCK + threadIdx.x;
     if (cn >= C.nNodes) return;
     const unsigned u = unsigned(cn), CNX = unsigned(C.NX), CNY = unsigned(C.NY);
@@ -599,9 +606,15 @@ template <class T> __device__ void Restrict<T>::run(VBlk vb, DevLevel F, DevLeve
             }
         }
     }
-    rc[3 * cn] = i0 != T(0) ? s0 : T(0);
-    rc[3 * cn + 1] = i1 != T(0) ? s1 : T(0);
-    rc[3 * cn + 2] = i2 != T(0) ? s2 : T(0);
+    const T v0 = i0 != T(0) ? s0 : T(0), v1 = i1 != T(0) ? s1 : T(0), v2 = i2 != T(0) ? s2 : T(0);
+    rc[3 * cn] = v0;
+    rc[3 * cn + 1] = v1;
+    rc[3 * cn + 2] = v2;
+    if (z0) {
+        z0[3 * cn] = c0 * i0 * v0;
+        z0[3 * cn + 1] = c0 * i1 * v1;
+        z0[3 * cn + 2] = c0 * i2 * v2;
+    }
 }
 
 template <class T> struct Prolong {
@@ -1166,8 +1179,9 @@ template <class T> void launchChebFirst(const DevLevel& L, const T* r, T* z, dou
     go<ChebFirst<T>>(kChebFirst<T>, dim3(flatGrid(L.nDof), ninst), s, invD, r, z, T(c0), L.nDof, stop);
 }
 
-template <class T> void launchRestrict(const DevLevel& F, const DevLevel& C, const T* rf, T* rc, const int* stop, hipStream_t s, int ninst) {
-    go<Restrict<T>>(kRestrict<T>, dim3(flatGrid(C.nNodes), ninst), s, F, C, rf, rc, stop);
+template <class T>
+void launchRestrict(const DevLevel& F, const DevLevel& C, const T* rf, T* rc, const int* stop, hipStream_t s, int ninst, T* z0, double c0) {
+    go<Restrict<T>>(kRestrict<T>, dim3(flatGrid(C.nNodes), ninst), s, F, C, rf, rc, stop, z0, T(c0));
 }
 
 template <class T> void launchProlong(const DevLevel& F, const DevLevel& C, const T* zc, T* zf, const int* stop, hipStream_t s, int ninst) {
@@ -1272,7 +1286,7 @@ void launchLzNext(const double* w, double* v, double* vPrev, int64_t n, CgState*
     template void launchApply<T>(const DevLevel&, int, const T*, T*, const T*, const T*, double, double, const int*, hipStream_t, int);     \
     template void launchChebFirst<T>(const DevLevel&, const T*, T*, double, const int*, hipStream_t, int);                                  \
     template void launchApplyOwn<T>(const DevLevel&, int, const T*, T*, const T*, const T*, double, double, const int*, hipStream_t, int, bool); \
-    template void launchRestrict<T>(const DevLevel&, const DevLevel&, const T*, T*, const int*, hipStream_t, int);                         \
+    template void launchRestrict<T>(const DevLevel&, const DevLevel&, const T*, T*, const int*, hipStream_t, int, T*, double);           \
     template void launchProlong<T>(const DevLevel&, const DevLevel&, const T*, T*, const int*, hipStream_t, int);                          \
     template void launchCoarse<T>(const int32_t*, const int32_t*, int64_t, int64_t, const double*, const T*, T*, const int*, hipStream_t, int); \
     template void launchDotPQ<T>(const T*, const T*, int64_t, CgState*, Reduce, hipStream_t, int);                                        \

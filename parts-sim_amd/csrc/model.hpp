@@ -516,8 +516,11 @@ public:
         T* t = freeBuf();
         op<T>(l, AM_RESID, z, t, rIn, nullptr, 0, 0, stop, form);
         Vecs<T> W = vecs<T>(l + 1);
-        launchRestrict<T>(D, C, t, W.r, stop, stream, ni);
-        const T* zc = vcycle<T>(l + 1, W.r, stop, form, degree);
+        // the restriction also writes the next level's first Chebyshev step (its buffer a), unless that
+        // level is the coarsest
+        const bool smoothNext = l + 2 < lv.size();
+        launchRestrict<T>(D, C, t, W.r, stop, stream, ni, smoothNext ? W.a : nullptr, smoothNext ? chebyshev(H.levels[l + 1].lmax, degree).c0 : 0.0);
+        const T* zc = vcycle<T>(l + 1, W.r, stop, form, degree, nullptr, smoothNext);
         launchProlong<T>(D, C, zc, z, stop, stream, ni);
         step(AM_CHEB1, c.c0, 0, degree == 1 ? rz : nullptr);
         for (int j = 1; j < degree; j++) step(AM_CHEB2, c.c1[j], c.c2[j], j + 1 == degree ? rz : nullptr);
