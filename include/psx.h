@@ -75,8 +75,10 @@ typedef struct {
     psx_progress_fn on_progress;
     void* user;
     int cheb_degree;            /* Chebyshev smoother degree 1..6 (0 = 2, the reference's, voxel_fea.cpp:692-725) */
-    int mfma_smoother;          /* PSX_PREC_MIXED, bit flags (0 = FP32 VALU everywhere): 1 = the level-0 in-plane
-                                   stencil on the matrix cores (BF16 neighbour differences, one load case);
+    int mfma_smoother;          /* PSX_PREC_MIXED, bit flags (0 = FP32 VALU everywhere): 1 = the V-cycle's levels
+                                   without own element matrices as the element-by-element product on the
+                                   matrix cores (two BF16 pieces of x - c and of Kb, FP32 accumulation);
+                                   4 = the outer FP32 operator as the same product with three pieces each;
                                    2 = levels with stored own element matrices apply them on the matrix
                                    cores (one v_mfma_f32_16x16x32_bf16 per node and voxel over 16 load
                                    cases: BF16 element rows and element-corner differences, FP32
@@ -144,12 +146,13 @@ int psx_apply32(psx_model m, int level, const float* x, float* y, int raw);
  * fp32 selects float x and y. No reference counterpart (the reference always streams its matrices). */
 int psx_apply_child(psx_model m, int level, int fp32, const void* x, void* y, int raw);
 
-/* y = K x with the matrix-core operator the mixed-precision V-cycle smooths with when
- * opts.mfma_smoother is set (levels without own element matrices; FP32 vectors, the in-plane stencil
- * sums as BF16 products of neighbour differences with FP32 accumulation; levels of per-voxel
- * densities keep their FP32 per-voxel form). No reference counterpart: the reference's smoother
- * operator is FP32 (voxel_fea.cpp:393, fea.wgsl:173-206). */
-int psx_apply_bf16(psx_model m, int level, const float* x, float* y);
+/* y = K x with the matrix-core operator of the mixed-precision solve (levels without own element
+ * matrices, FP32 vectors): the element-by-element product sum_e s_e Kb (x_e - c), c one node's value per
+ * 32 x 8 x zChunk tile, as v_mfma_f32_16x16x32_bf16 products of `pieces` BF16 pieces of x - c and of Kb
+ * with FP32 accumulation. pieces = 2 (hi + lo: the V-cycle's smoothing, opts.mfma_smoother bit 0) or 3
+ * (the outer operator, bit 2). No reference counterpart: the reference's operator is FP32
+ * (voxel_fea.cpp:393, fea.wgsl:173-206). */
+int psx_apply_mfma(psx_model m, int level, int pieces, const float* x, float* y);
 
 /* One V-cycle z ~ K^-1 r, FP32 with r scaled to order one (GpuFeaSolver::precondition,
  * gpu_fea.cpp:431-450; VoxelFEA::precondition voxel_fea.cpp:830-861). */
@@ -171,8 +174,7 @@ int psx_solve_device(psx_model m, const double* d_f, const double* d_x0, const p
  * its own state, so each solution equals the one psx_solve gives for it alone. f, u (and x0 when not
  * NULL) are nrhs x n_dof, load-case major; info receives nrhs entries. The reference solves its load
  * cases one after another (GpuFeaSolver::solve per case, gpu_fea.cpp:452-574; the topology and
- * eigen loops call it per vector, eigen.cpp:508-553). The BF16 matrix-core smoother is for
- * nrhs = 1 only (opts->mfma_smoother is ignored above). */
+ * eigen loops call it per vector, eigen.cpp:508-553). */
 int psx_solve_multi(psx_model m, int nrhs, const double* f, const double* x0, const psx_solve_opts* opts, double* u, psx_solve_info* info);
 int psx_solve_multi_device(psx_model m, int nrhs, const double* d_f, const double* d_x0, const psx_solve_opts* opts, double* d_u,
                            psx_solve_info* info);

@@ -139,7 +139,7 @@ SIGNATURES = {
     "psx_set_lmax": (_i, [_vp, _i, _d]),
     "psx_apply": (_i, [_vp, _i, _vp, _vp, _i]),
     "psx_apply32": (_i, [_vp, _i, _vp, _vp, _i]),
-    "psx_apply_bf16": (_i, [_vp, _i, _vp, _vp]),
+    "psx_apply_mfma": (_i, [_vp, _i, _i, _vp, _vp]),
     "psx_apply_child": (_i, [_vp, _i, _i, _vp, _vp, _i]),
     "psx_precondition": (_i, [_vp, _vp, _vp]),
     "psx_precondition_device": (_i, [_vp, _vp, _vp]),
@@ -599,10 +599,12 @@ class Model:
         _check(lib().psx_apply_child(self.h, int(level), int(fp32), _p(x), _p(y), int(raw)))
         return y
 
-    def apply_bf16(self, x, level=0):
+    def apply_mfma(self, x, level=0, pieces=2):
+        """y = K x (FP32) as the element-by-element matrix-core product with `pieces` BF16 pieces of the
+        operands (2: the mixed V-cycle's smoothing; 3: the outer operator's option)."""
         x = np.ascontiguousarray(x, np.float32)
         y = np.zeros_like(x)
-        _check(lib().psx_apply_bf16(self.h, int(level), _p(x), _p(y)))
+        _check(lib().psx_apply_mfma(self.h, int(level), int(pieces), _p(x), _p(y)))
         return y
 
     def precondition(self, r):

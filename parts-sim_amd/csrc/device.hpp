@@ -24,6 +24,10 @@ struct DevLevel {
     const float* Kb32;
     const double* St64;        // 27-point stencil of Kb: [27 neighbours][3 rows][3 cols]
     const float* St32;
+    // Kb as the matrix-core A operands of fine_el.hip, [half][piece][lane] x 8 BF16: Kb split into three
+    // BF16 pieces (hi, mid, lo), output corners 4 half .. 4 half + 3 (numbered x + 2 y + 4 z) in rows
+    // 4 corner + component, input corner pairs in the K slices
+    const uint4* elA;
     const float* nodeScale;    // s for nodes whose 8 voxels are s * Kb, else 0
     const uint16_t* nodeInfo;  // per node: bit v (0..7) set when voxel v's scale differs from the
                                // reference voxel's (the lowest clear bit); bit 8 + d when DOF d is
@@ -92,7 +96,9 @@ enum ApplyMode { AM_APPLY = 0, AM_RAW = 1, AM_RESID = 2, AM_CHEB1 = 3, AM_CHEB2 
 // cheaper form (most own elements two-valued; a level of general own elements streams its stored
 // matrices faster than it evaluates 8 children per element)
 // OF_MFMA_OWN: FP32 levels with stored own matrices take the matrix-core element product
-enum OpForm { OF_CHILD = 1, OF_MFMA = 2, OF_AUTO = 4, OF_MFMA_OWN = 8 };
+// OF_MFMA: FP32 levels without own elements take the element-by-element product on the matrix cores
+// with two BF16 pieces of x - c and of Kb (fine_el.hip); OF_EL3: the same with three pieces each
+enum OpForm { OF_CHILD = 1, OF_MFMA = 2, OF_AUTO = 4, OF_MFMA_OWN = 8, OF_EL3 = 16 };
 
 // y = K x on a level with the epilogue of `mode` (T = double or float):
 //   AM_APPLY  y = K x, held DOFs 0               AM_RAW   y = K x
@@ -116,12 +122,16 @@ void launchApplyOwnMf(const DevLevel& L, int mode, const float* x, float* y, con
                       hipStream_t s, int ninst = 1);
 // the same for levels whose elements are s * Kb, or own elements in child-scale form: z-marching
 // LDS planes, class-sum stencil in difference form + one lane per differing voxel (fine.hip)
-// mf: FP32 levels without own elements take the in-plane stencil on the matrix cores (BF16 operands,
-// FP32 accumulation; for the V-cycle's smoothing steps)
+// el: FP32 levels without own elements take the element-by-element product on the matrix cores with
+// el (2 or 3) BF16 pieces of the operands (fine_el.hip); 0 = the VALU forms
 // half: FP32 levels with child-scale own elements read the FP16 subset table (the mixed V-cycle's
 // smoothing)
 template <class T> void launchFine(const DevLevel& L, int mode, const T* x, T* y, const T* r, const T* zp, double c1, double c2, const int* stop,
-                                   hipStream_t s, int ninst = 1, bool mf = false, bool half = false);
+                                   hipStream_t s, int ninst = 1, int el = 0, bool half = false);
+// the element-by-element matrix-core operator itself (fine_el.hip), on launchFine's tiles (zChunk:
+// planes per block; dots: fused dot partials per tile, as launchFineDot writes them)
+void launchFineEl(const DevLevel& L, int mode, const float* x, float* y, const float* r, const float* zp, double c1, double c2, const int* stop, int zChunk,
+                  double* dots, int64_t dotStride, const float* qdot, hipStream_t s, int ninst, int pieces);
 // Dot products fused into an operator launch (conjugate gradients), block partials into red:
 //   FD_PQ  (AM_APPLY)  p.q with p = x, q = y                         -> launchFinishPQ
 //   FD_RZ  (AM_CHEB1/2) r.z and z.q with z = y, q = qdot             -> launchFinishRZ
@@ -133,7 +143,7 @@ struct DotFusion {
 };
 // launchFine with a fused dot; returns the number of block partials written (0: not fused)
 template <class T> unsigned launchFineDot(const DevLevel& L, int mode, const T* x, T* y, const T* r, const T* zp, double c1, double c2, const int* stop,
-                                          const DotFusion& fd, hipStream_t s, int ninst = 1, bool mf = false, bool half = false);
+                                          const DotFusion& fd, hipStream_t s, int ninst = 1, int el = 0, bool half = false);
 void launchFinishPQ(Reduce red, unsigned nb, CgState* st, hipStream_t s, int ninst = 1);
 void launchFinishRZ(Reduce red, unsigned nb, CgState* st, hipStream_t s, int ninst = 1);
 // z = c0 D^-1 r (first Chebyshev step from zero)

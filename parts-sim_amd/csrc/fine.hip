@@ -23,13 +23,11 @@
 
 #include "device.hpp"
 #include "pack_dev.hpp"
+#include "stencil.hpp"
 
 namespace psx {
 
 namespace {
-
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 constexpr int TX = 32, TY = 8, NT = TX * TY;
 constexpr int PX = TX + 2, PY = TY + 2;  // node plane with halo
@@ -77,43 +75,6 @@ template <class F, int... I> __device__ __forceinline__ void staticForImpl(F&& f
 }
 template <int N, class F> __device__ __forceinline__ void staticFor(F&& f) { staticForImpl(f, std::make_integer_sequence<int, N>{}); }
 
-// Raw buffer accesses over one plane: the descriptor's byte count is the plane's, so a lane whose
-// offset is OUT_OF_RANGE (a position outside the part) reads 0 and its store is dropped, and a plane
-// outside the grid (count 0) reads 0 everywhere, without predicated branches. Plane byte counts stay
-// below 2^31 (checkGridLimits bounds a node plane at 2^31 / 24 nodes).
-constexpr int32_t OUT_OF_RANGE = int32_t(0x80000000u);
-constexpr int RSRC_WORD3 = 0x00020000;  // raw buffer, 32-bit data (gfx9 family)
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t planeRsrc(const void* base, uint32_t bytes) {
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, int(bytes), RSRC_WORD3);
-}
-// a node's three components at byte offset off
-__device__ __forceinline__ void bload3(__amdgpu_buffer_rsrc_t rs, int32_t off, float (&d)[3]) {
-    const auto v = __builtin_amdgcn_raw_buffer_load_b96(rs, off, 0, 0);
-    d[0] = __uint_as_float(v[0]);
-    d[1] = __uint_as_float(v[1]);
-    d[2] = __uint_as_float(v[2]);
-}
-__device__ __forceinline__ void bload3(__amdgpu_buffer_rsrc_t rs, int32_t off, double (&d)[3]) {
-    const auto a = __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0);
-    const auto b = __builtin_amdgcn_raw_buffer_load_b64(rs, off + 16, 0, 0);
-    d[0] = __hiloint2double(int(a[1]), int(a[0]));
-    d[1] = __hiloint2double(int(a[3]), int(a[2]));
-    d[2] = __hiloint2double(int(b[1]), int(b[0]));
-}
-__device__ __forceinline__ void bstore3(__amdgpu_buffer_rsrc_t rs, int32_t off, const float (&d)[3]) {
-    typedef unsigned u3 __attribute__((ext_vector_type(3)));
-    const u3 v = {__float_as_uint(d[0]), __float_as_uint(d[1]), __float_as_uint(d[2])};
-    __builtin_amdgcn_raw_buffer_store_b96(v, rs, off, 0, 0);
-}
-__device__ __forceinline__ void bstore3(__amdgpu_buffer_rsrc_t rs, int32_t off, const double (&d)[3]) {
-    typedef unsigned u4 __attribute__((ext_vector_type(4)));
-    typedef unsigned u2 __attribute__((ext_vector_type(2)));
-    const u4 a = {unsigned(__double2loint(d[0])), unsigned(__double2hiint(d[0])), unsigned(__double2loint(d[1])), unsigned(__double2hiint(d[1]))};
-    const u2 b = {unsigned(__double2loint(d[2])), unsigned(__double2hiint(d[2]))};
-    __builtin_amdgcn_raw_buffer_store_b128(a, rs, off, 0, 0);
-    __builtin_amdgcn_raw_buffer_store_b64(b, rs, off + 16, 0, 0);
-}
-
 template <class T> struct FineP;
 template <> struct FineP<double> {
     __device__ static const double* invD(const DevLevel& L) { return L.invD64; }
@@ -145,18 +106,10 @@ template <class T, bool OWNC> struct Ring {
 // elements): every node sums its 8 voxels, s_v E_v (x - x_node) with E_v the node's 3 rows of Kb in
 // voxel v (compile-time class coefficients), instead of the stencil and the pair lists; for per-voxel
 // densities, where nearly every node would list 7 or 8 pairs.
-// MF (FP32 levels without own elements, the V-cycle's smoothing steps): the plane's in-plane stencil
-// sum runs on the matrix cores, BF16 operands with FP32 accumulation. Per wavefront, 16 nodes form the
-// columns of one v_mfma_f32_16x16x32_bf16: B = their 8 in-plane neighbour differences (3 components +
-// a zero each: K = 32), A = the stencil blocks for the three node planes the input plane feeds (rows
-// 4 g + i: plane group g = node plane mod 3, component i). The accumulator tiles stay in registers for
-// the whole march; a node plane's rows are complete one step after its upper neighbour plane and are
-// read out by lane shuffles, then cleared for the plane three steps on. The centre column term and
-// the differing-voxel pairs stay FP32, as in the VALU form.
 // H16 (FP32 levels with child-scale own elements, the mixed V-cycle's smoothing): the two-scale
 // elements read the subset table in FP16 (ctab16, each row times its power-of-two ctabRowScale), which
 // fits the compute unit's cache where the FP32 table does not
-template <class T, int MODE, bool OWNC, bool DOT, bool DENSE = false, bool MF = false, bool H16 = false>
+template <class T, int MODE, bool OWNC, bool DOT, bool DENSE = false, bool H16 = false>
 __device__ __forceinline__ void fineBody(VBlk vb, PSX_FINE_PARAMS) {
     // multi-instance launches: the block's y is the right-hand side (its vectors, stop flag, partials)
     if (stop) stop += int64_t(vb.y) * CG_INTS;
@@ -290,36 +243,6 @@ __device__ __forceinline__ void fineBody(VBlk vb, PSX_FINE_PARAMS) {
         o.info = __builtin_amdgcn_raw_buffer_load_b16(planeRsrc(L.nodeInfo + pb, zin ? uint32_t(2 * NXY) : 0u), ioff, 0, 0);
     };
 
-    // matrix-core operands of the in-plane stencil: A[rot] (this lane: row l % 16 = 4 g + i, K slice
-    // 8 (l / 16) .. + 7 = neighbours 2 (l / 16), 2 (l / 16) + 1 of the in-plane list, components j) for
-    // the input plane's offsets oz = 0 / -1 / +1 to the node planes of groups rot, rot + 1, rot + 2
-    bf16x8 Acoef[3];
-    static_assert(NT / 64 == 4, "four 16-node groups per wavefront");
-    floatx4 macc[4];
-    if constexpr (MF) {
-        const int row = lane & 15, g = row >> 2, ci = row & 3, kb = lane >> 4;
-#pragma unroll
-        for (int rot = 0; rot < 3; rot++) {
-            const int oz = g == rot ? 0 : (g == (rot + 1) % 3 ? -1 : 1);
-#pragma unroll
-            for (int t = 0; t < 8; t++) {
-                const int nb = 2 * kb + (t >> 2), j = t & 3;
-                const int q = nb < 4 ? nb : nb + 1;  // in-plane neighbour, the centre (q = 4) skipped
-                const int ox = q % 3 - 1, oy = q / 3 - 1;
-                const int m = (ox + 1) + 3 * (oy + 1) + 9 * (oz + 1);
-                const float v = (g < 3 && ci < 3 && j < 3) ? L.St32[m * 9 + ci * 3 + j] : 0.f;
-                Acoef[rot][t] = (__bf16)v;
-            }
-        }
-#pragma unroll
-        for (int G = 0; G < 4; G++) macc[G] = floatx4{0.f, 0.f, 0.f, 0.f};
-    }
-    // position in the tile of local node o under this block's thread mapping
-    auto posOf = [&](int o, int& px, int& py) {
-        px = transposed ? o / TY : o % TX;
-        py = transposed ? o % TY : o / TX;
-    };
-
     // prologue: node plane z0 - 1 in LDS; node plane z0 and voxel plane z0 - 2 in flight
     T pn[NPL][3];
     int32_t vn[NVL];
@@ -356,42 +279,7 @@ __device__ __forceinline__ void fineBody(VBlk vb, PSX_FINE_PARAMS) {
 #pragma unroll
             for (int q = 0; q < 9; q++)
 #pragma unroll
-                for (int c = 0; c < 3; c++) xm[q][c] = ((!DENSE && !MF) || q == 4) ? sh.x[sl][c][ty + q / 3][tx + q % 3] : T(0);
-        }
-        T mres[3] = {T(0), T(0), T(0)};  // MF: the in-plane stencil sums of node k - 1
-        if constexpr (MF) {
-            const int sl = k & 3, kb = lane >> 4, rot = ((k % 3) + 3) % 3;
-            if (__ballot(mine) != 0) {  // a wavefront without nodes (past the part's edge) skips
-#pragma unroll
-                for (int G = 0; G < 4; G++) {
-                    int px, py;
-                    posOf(64 * wid + 16 * G + (lane & 15), px, py);
-                    const float c0 = sh.x[sl][0][py + 1][px + 1], c1v = sh.x[sl][1][py + 1][px + 1], c2v = sh.x[sl][2][py + 1][px + 1];
-                    bf16x8 b;
-#pragma unroll
-                    for (int h = 0; h < 2; h++) {
-                        const int nb = 2 * kb + h, q = nb < 4 ? nb : nb + 1;
-                        const int ox = q % 3 - 1, oy = q / 3 - 1;
-                        b[4 * h] = (__bf16)(sh.x[sl][0][py + 1 + oy][px + 1 + ox] - c0);
-                        b[4 * h + 1] = (__bf16)(sh.x[sl][1][py + 1 + oy][px + 1 + ox] - c1v);
-                        b[4 * h + 2] = (__bf16)(sh.x[sl][2][py + 1 + oy][px + 1 + ox] - c2v);
-                        b[4 * h + 3] = (__bf16)0.f;
-                    }
-                    macc[G] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(rot == 0 ? Acoef[0] : (rot == 1 ? Acoef[1] : Acoef[2]), b, macc[G], 0, 0, 0);
-                }
-                // node plane k - 1 (group (k - 1) mod 3) is complete: lanes 16 g + n hold node n's rows;
-                // the owner of node 16 G + n is lane 16 G + n of this wavefront
-                const int gout = ((k - 1) % 3 + 3) % 3, src = 16 * gout + (lane & 15);
-#pragma unroll
-                for (int G = 0; G < 4; G++) {
-#pragma unroll
-                    for (int c = 0; c < 3; c++) {
-                        const float v = __shfl(macc[G][c], src, 64);
-                        if ((lane >> 4) == G) mres[c] = T(v);
-                    }
-                    if ((lane >> 4) == gout) macc[G] = floatx4{0.f, 0.f, 0.f, 0.f};
-                }
-            }
+                for (int c = 0; c < 3; c++) xm[q][c] = (!DENSE || q == 4) ? sh.x[sl][c][ty + q / 3][tx + q % 3] : T(0);
         }
         // The stencil in class-sum form: with X_c(ox, oy) the plane's 3 x 3 values of component c and
         // oz = plane - node, every entry is D[|o_d|][nnz of the other two offsets] on the diagonal and
@@ -403,7 +291,7 @@ __device__ __forceinline__ void fineBody(VBlk vb, PSX_FINE_PARAMS) {
         // P (x(z - 1) - 2 x(z) + x(z + 1)), P = the diagonal sum of a neighbour plane's blocks. A
         // rigid translation then cancels exactly and a rotation to element scale, which keeps the
         // FP32 operator accurate on the smooth, low-energy modes the conjugate gradients resolve last.
-        if (!DENSE && !MF && mine) {
+        if (!DENSE && mine) {
             const T D00 = C.v[14], D01 = C.v[15], D02 = C.v[16], D10 = C.v[17], D11 = C.v[18], D12 = C.v[19], E0 = C.v[20], E1 = C.v[21];
             (void)D00;
             // q = (ox + 1) + 3 (oy + 1)
@@ -518,10 +406,9 @@ __device__ __forceinline__ void fineBody(VBlk vb, PSX_FINE_PARAMS) {
                 // the centre column's second difference through the neighbour planes' diagonal sums
                 const T Pxy = C.v[15] + T(2) * (C.v[18] + C.v[16]) + T(4) * C.v[19], Pz = C.v[17] + T(4) * (C.v[18] + C.v[19]);
                 const T sref = T(sh.s[(ref >> 2) ? v0 : vm][ty + ((ref >> 1) & 1)][tx + (ref & 1)]);
-                const T s0 = MF ? mres[0] : aP[0], s1 = MF ? mres[1] : aP[1], s2 = MF ? mres[2] : aP[2];
-                out[0] = sref * (s0 + Pxy * ((xcPrev2[0] + xm[4][0]) - T(2) * xcPrev[0]));
-                out[1] = sref * (s1 + Pxy * ((xcPrev2[1] + xm[4][1]) - T(2) * xcPrev[1]));
-                out[2] = sref * (s2 + Pz * ((xcPrev2[2] + xm[4][2]) - T(2) * xcPrev[2]));
+                out[0] = sref * (aP[0] + Pxy * ((xcPrev2[0] + xm[4][0]) - T(2) * xcPrev[0]));
+                out[1] = sref * (aP[1] + Pxy * ((xcPrev2[1] + xm[4][1]) - T(2) * xcPrev[1]));
+                out[2] = sref * (aP[2] + Pz * ((xcPrev2[2] + xm[4][2]) - T(2) * xcPrev[2]));
             }
             const int mine0 = mineBase + waveOff;
             for (int pass = 0; pass * NT < total; pass++) {
@@ -742,9 +629,9 @@ __device__ __forceinline__ void fineBody(VBlk vb, PSX_FINE_PARAMS) {
 
 #define PSX_FINE_FWD L, C, crow, ctab, x, y, r, zp, c1, c2, stop, zChunk, gx, gy, nTiles, xcdOrder, dots, dotStride, qdot
 // the kernel bodies for packed launches: WPE_ as the named kernel's waves-per-SIMD attribute
-template <class T, int MODE, bool OWNC, bool DOT, bool DENSE, bool MF, int WPE_, bool H16 = false> struct Fine {
+template <class T, int MODE, bool OWNC, bool DOT, bool DENSE, int WPE_, bool H16 = false> struct Fine {
     static constexpr int NTH = NT, WPE = WPE_;
-    __device__ static void run(VBlk vb, PSX_FINE_PARAMS) { fineBody<T, MODE, OWNC, DOT, DENSE, MF, H16>(vb, PSX_FINE_FWD); }
+    __device__ static void run(VBlk vb, PSX_FINE_PARAMS) { fineBody<T, MODE, OWNC, DOT, DENSE, H16>(vb, PSX_FINE_FWD); }
 };
 template <class T, int MODE, bool OWNC, bool DOT> __global__ __launch_bounds__(NT) void kFine(PSX_FINE_PARAMS) {
     fineBody<T, MODE, OWNC, DOT>(ownBlock(), PSX_FINE_FWD);
@@ -759,7 +646,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void kF
                                                                                   int zChunk, int gx, int gy, int nTiles, int xcdOrder,
                                                                                   double* __restrict__ dots, int64_t dotStride,
                                                                                   const float* __restrict__ qdot) {
-    fineBody<float, MODE, true, DOT, false, false, true>(ownBlock(), PSX_FINE_FWD);
+    fineBody<float, MODE, true, DOT, false, true>(ownBlock(), PSX_FINE_FWD);
 }
 // FP32 without own elements: held to 128 VGPRs (4 waves per SIMD instead of 3), which it fits
 // without spilling
@@ -771,36 +658,22 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void kF
 template <class T, int MODE, bool DOT> __global__ __launch_bounds__(NT) void kFineDense(PSX_FINE_PARAMS) {
     fineBody<T, MODE, false, DOT, true>(ownBlock(), PSX_FINE_FWD);
 }
-// in-plane stencil on the matrix cores (FP32 levels without own elements)
-template <int MODE, bool DOT>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void kFineMf(DevLevel L, Sym<float> C, const float* __restrict__ crow,
-                                                                                   const float* __restrict__ ctab, const float* __restrict__ x,
-                                                                                   float* __restrict__ y, const float* __restrict__ r,
-                                                                                   const float* __restrict__ zp, float c1, float c2, const int* stop,
-                                                                                   int zChunk, int gx, int gy, int nTiles, int xcdOrder,
-                                                                                   double* __restrict__ dots, int64_t dotStride,
-                                                                                   const float* __restrict__ qdot) {
-    fineBody<float, MODE, false, DOT, false, true>(ownBlock(), PSX_FINE_FWD);
-}
 #undef PSX_FINE_FWD
 
 // launches (or records) the operator kernel of one form, mode and dot choice
-template <class T, int MODE, bool OWNC, bool DOT, bool DENSE, bool MF, bool H16 = false>
+template <class T, int MODE, bool OWNC, bool DOT, bool DENSE, bool H16 = false>
 void fineGo(dim3 g, hipStream_t s, const DevLevel& L, const Sym<T>& C, const T* crow, const T* ctab, const T* x, T* y, const T* r, const T* zp, T c1, T c2,
             const int* stop, int zChunk, int gx, int gy, int nTiles, int xcdOrder, double* dots, int64_t dotStride, const T* qdot) {
 #define PSX_FINE_ARGS L, C, crow, ctab, x, y, r, zp, c1, c2, stop, zChunk, gx, gy, nTiles, xcdOrder, dots, dotStride, qdot
-    if constexpr (MF) {
-        static_assert(sizeof(T) == 4 && !OWNC && !DENSE, "matrix-core form: FP32 levels without own elements");
-        go<Fine<float, MODE, false, DOT, false, true, 3>>(kFineMf<MODE, DOT>, g, s, PSX_FINE_ARGS);
-    } else if constexpr (DENSE) {
-        go<Fine<T, MODE, false, DOT, true, false, 0>>(kFineDense<T, MODE, DOT>, g, s, PSX_FINE_ARGS);
+    if constexpr (DENSE) {
+        go<Fine<T, MODE, false, DOT, true, 0>>(kFineDense<T, MODE, DOT>, g, s, PSX_FINE_ARGS);
     } else if constexpr (sizeof(T) == 4 && !OWNC) {
-        go<Fine<T, MODE, false, DOT, false, false, 4>>(kFine4<T, MODE, DOT>, g, s, PSX_FINE_ARGS);
+        go<Fine<T, MODE, false, DOT, false, 4>>(kFine4<T, MODE, DOT>, g, s, PSX_FINE_ARGS);
     } else if constexpr (H16) {
         static_assert(sizeof(T) == 4 && OWNC, "FP16 subset table: FP32 child-scale levels");
-        go<Fine<float, MODE, true, DOT, false, false, 3, true>>(kFineH<MODE, DOT>, g, s, PSX_FINE_ARGS);
+        go<Fine<float, MODE, true, DOT, false, 3, true>>(kFineH<MODE, DOT>, g, s, PSX_FINE_ARGS);
     } else {
-        go<Fine<T, MODE, OWNC, DOT, false, false, 0>>(kFine<T, MODE, OWNC, DOT>, g, s, PSX_FINE_ARGS);
+        go<Fine<T, MODE, OWNC, DOT, false, 0>>(kFine<T, MODE, OWNC, DOT>, g, s, PSX_FINE_ARGS);
     }
 #undef PSX_FINE_ARGS
 }
@@ -809,7 +682,7 @@ void fineGo(dim3 g, hipStream_t s, const DevLevel& L, const Sym<T>& C, const T* 
 
 template <class T, bool OWNC>
 static void launchFineT(const DevLevel& L, int mode, const Sym<T>& C, const T* crow, const T* ctab, const T* x, T* y, const T* r, const T* d, double c1, double c2,
-                        const int* stop, int zChunk, double* dots, int64_t dotStride, const T* qdot, hipStream_t s, int ninst, bool mf,
+                        const int* stop, int zChunk, double* dots, int64_t dotStride, const T* qdot, hipStream_t s, int ninst, int el,
                         bool half = false) {
     const int gx = (L.NX + TX - 1) / TX, gy = (L.NY + TY - 1) / TY, gz = (L.NZ + zChunk - 1) / zChunk;
     const int nTiles = gx * gy * gz;
@@ -823,32 +696,33 @@ static void launchFineT(const DevLevel& L, int mode, const Sym<T>& C, const T* c
     }();
     // fused dots exist for APPLY (p.q) and the Chebyshev steps (r.z, z.q) only
     const bool dot = dots && (mode == AM_APPLY || mode == AM_CHEB1 || mode == AM_CHEB2);
-    const bool useMf = sizeof(T) == 4 && !OWNC && mf && !L.dense, dense = !OWNC && L.dense;
+    const bool dense = !OWNC && L.dense;
+    if constexpr (sizeof(T) == 4 && !OWNC) {
+        // the element-by-element product on the matrix cores (fine_el.hip)
+        if (el > 0) {
+            launchFineEl(L, mode, x, y, r, d, c1, c2, stop, zChunk, dots, dotStride, qdot, s, ninst, el);
+            return;
+        }
+    }
     const bool useH16 = sizeof(T) == 4 && OWNC && half && L.ctab16;
     auto form = [&](auto m, auto dt) {
         constexpr int M = decltype(m)::value;
         constexpr bool DT = decltype(dt)::value;
         const T f1 = T(c1), f2 = T(c2);
-        if constexpr (sizeof(T) == 4 && !OWNC) {
-            if (useMf) {
-                fineGo<T, M, false, DT, false, true>(g, s, L, C, crow, ctab, x, y, r, d, f1, f2, stop, zChunk, gx, gy, nTiles, xcdOrder, dots, dotStride, qdot);
-                return;
-            }
-        }
         if constexpr (!OWNC) {
             if (dense) {
-                fineGo<T, M, false, DT, true, false>(g, s, L, C, crow, ctab, x, y, r, d, f1, f2, stop, zChunk, gx, gy, nTiles, xcdOrder, dots, dotStride, qdot);
+                fineGo<T, M, false, DT, true>(g, s, L, C, crow, ctab, x, y, r, d, f1, f2, stop, zChunk, gx, gy, nTiles, xcdOrder, dots, dotStride, qdot);
                 return;
             }
         }
         if constexpr (sizeof(T) == 4 && OWNC) {
             if (useH16) {
-                fineGo<T, M, OWNC, DT, false, false, true>(g, s, L, C, crow, ctab, x, y, r, d, f1, f2, stop, zChunk, gx, gy, nTiles, xcdOrder, dots, dotStride,
+                fineGo<T, M, OWNC, DT, false, true>(g, s, L, C, crow, ctab, x, y, r, d, f1, f2, stop, zChunk, gx, gy, nTiles, xcdOrder, dots, dotStride,
                                                            qdot);
                 return;
             }
         }
-        fineGo<T, M, OWNC, DT, false, false>(g, s, L, C, crow, ctab, x, y, r, d, f1, f2, stop, zChunk, gx, gy, nTiles, xcdOrder, dots, dotStride, qdot);
+        fineGo<T, M, OWNC, DT, false>(g, s, L, C, crow, ctab, x, y, r, d, f1, f2, stop, zChunk, gx, gy, nTiles, xcdOrder, dots, dotStride, qdot);
     };
     using yes = std::true_type;
     using no = std::false_type;
@@ -872,7 +746,7 @@ static void launchFineT(const DevLevel& L, int mode, const Sym<T>& C, const T* c
 
 template <class T>
 unsigned launchFineDot(const DevLevel& L, int mode, const T* x, T* y, const T* r, const T* d, double c1, double c2, const int* stop, const DotFusion& fd,
-                       hipStream_t s, int ninst, bool mf, bool half) {
+                       hipStream_t s, int ninst, int el, bool half) {
     Sym<T> C;
     for (int q = 0; q < 22; q++) C.v[q] = sizeof(T) == 8 ? T(L.sym64[q]) : T(L.sym32[q]);
     // the longest z-chunk (16, 8 or 4 planes) that still gives >= 4096 blocks (16 per CU); shorter
@@ -902,24 +776,24 @@ unsigned launchFineDot(const DevLevel& L, int mode, const T* x, T* y, const T* r
         else crow = L.crow32, ctab = L.ctab32;
         launchFineT<T, true>(L, mode, C, crow, ctab, x, y, r, d, c1, c2, stop, zChunk, dots, fd.red.stride, qdot, s, ninst, false, half);
     } else {
-        launchFineT<T, false>(L, mode, C, nullptr, nullptr, x, y, r, d, c1, c2, stop, zChunk, dots, fd.red.stride, qdot, s, ninst, mf);
+        launchFineT<T, false>(L, mode, C, nullptr, nullptr, x, y, r, d, c1, c2, stop, zChunk, dots, fd.red.stride, qdot, s, ninst, el);
     }
     return fused ? unsigned(gx * gy * gz) : 0u;
 }
 
 template <class T>
 void launchFine(const DevLevel& L, int mode, const T* x, T* y, const T* r, const T* d, double c1, double c2, const int* stop, hipStream_t s, int ninst,
-                bool mf, bool half) {
-    launchFineDot<T>(L, mode, x, y, r, d, c1, c2, stop, DotFusion{}, s, ninst, mf, half);
+                int el, bool half) {
+    launchFineDot<T>(L, mode, x, y, r, d, c1, c2, stop, DotFusion{}, s, ninst, el, half);
 }
 
 template unsigned launchFineDot<double>(const DevLevel&, int, const double*, double*, const double*, const double*, double, double, const int*,
-                                        const DotFusion&, hipStream_t, int, bool, bool);
+                                        const DotFusion&, hipStream_t, int, int, bool);
 template unsigned launchFineDot<float>(const DevLevel&, int, const float*, float*, const float*, const float*, double, double, const int*,
-                                       const DotFusion&, hipStream_t, int, bool, bool);
+                                       const DotFusion&, hipStream_t, int, int, bool);
 template void launchFine<double>(const DevLevel&, int, const double*, double*, const double*, const double*, double, double, const int*, hipStream_t, int,
-                                 bool, bool);
-template void launchFine<float>(const DevLevel&, int, const float*, float*, const float*, const float*, double, double, const int*, hipStream_t, int, bool,
+                                 int, bool);
+template void launchFine<float>(const DevLevel&, int, const float*, float*, const float*, const float*, double, double, const int*, hipStream_t, int, int,
                                 bool);
 
 }  // namespace psx

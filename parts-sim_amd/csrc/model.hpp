@@ -268,6 +268,45 @@ public:
         const double dev = symmetryClasses(L.Kb, D.sym64);
         for (int q = 0; q < 22; q++) D.sym32[q] = float(D.sym64[q]);
         D.symOk = dev < 1e-12;
+        const auto A = elementOperands(L.Kb);
+        D.elA = reinterpret_cast<const uint4*>(upload(A.data(), int64_t(A.size())));
+    }
+    // Kb as the A operands of the element-by-element matrix-core product (fine_el.hip): per half h
+    // (output corners 4 h .. 4 h + 3, corners numbered x + 2 y + 4 z), piece p and lane l, 8 BF16:
+    // row l % 16 = 4 (corner - 4 h) + component, K slice 8 (l / 16) .. + 7 = input corners
+    // 2 (l / 16), 2 (l / 16) + 1 with components 0..3 (the fourth zero). The pieces split each entry
+    // v in turn, hi = bf16(v), mid = bf16(v - hi), lo = bf16(v - hi - mid), rounded to nearest.
+    static std::vector<uint32_t> elementOperands(const double* Kb) {
+        auto bf = [](double v) -> uint32_t {
+            const float f = float(v);
+            uint32_t u;
+            std::memcpy(&u, &f, 4);
+            return (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
+        };
+        auto val = [](uint32_t b) {
+            const uint32_t u = b << 16;
+            float f;
+            std::memcpy(&f, &u, 4);
+            return double(f);
+        };
+        auto corner = [](int k) { return ((k >> 1) & 1 ? ((k & 1) ? 2 : 3) : ((k & 1) ? 1 : 0)) + 4 * (k >> 2); };
+        std::vector<uint32_t> out(size_t(2) * 3 * 64 * 4);
+        for (int h = 0; h < 2; h++)
+            for (int l = 0; l < 64; l++) {
+                const int m = l & 15, kg = l >> 4, d = m & 3, ao = corner(4 * h + (m >> 2));
+                uint32_t pc[3][8];
+                for (int t = 0; t < 8; t++) {
+                    const int c = t & 3, ai = corner(2 * kg + (t >> 2));
+                    double v = (d < 3 && c < 3) ? Kb[(3 * ao + d) * 24 + 3 * ai + c] : 0.0;
+                    for (int p = 0; p < 3; p++) {
+                        pc[p][t] = bf(v);
+                        v -= val(pc[p][t]);
+                    }
+                }
+                for (int p = 0; p < 3; p++)
+                    for (int w = 0; w < 4; w++) out[((size_t(h) * 3 + p) * 64 + l) * 4 + w] = pc[p][2 * w] | (pc[p][2 * w + 1] << 16);
+            }
+        return out;
     }
     // child-scale form: corner-0 rows of the 8 Galerkin products and the subset table (from
     // H.levels[l].childM / childTab)
@@ -395,21 +434,21 @@ public:
     void op(size_t l, int mode, const T* xx, T* yy, const T* rr, const T* dd, double c1, double c2, const int* stop, int form = 0,
             const DotFusion* fd = nullptr) {
         const DevLevel& D = lv[l].dl;
-        // the matrix-core form: FP32 levels without own elements (the per-voxel form keeps FP32)
-        const bool mf = sizeof(T) == 4 && (form & OF_MFMA) && H.levels[l].nOwn == 0 && !D.dense;
+        // the element-by-element matrix-core form: FP32 levels without own elements
+        const int el = sizeof(T) == 4 && H.levels[l].nOwn == 0 ? ((form & OF_EL3) ? 3 : (form & OF_MFMA) ? 2 : 0) : 0;
         if (!useFine) launchApply<T>(D, mode, xx, yy, rr, dd, c1, c2, stop, stream, ni);
         else if (D.symOk && (H.levels[l].nOwn == 0 || (D.childForm && ((form & OF_CHILD) || ((form & OF_AUTO) && lv[l].childCheap))))) {
             // the mixed V-cycle's smoothing (OF_AUTO, FP32) reads the child-scale subset table in FP16
             const bool half = (form & OF_AUTO) != 0;
             if (fd) {
-                const unsigned nb = launchFineDot<T>(D, mode, xx, yy, rr, dd, c1, c2, stop, *fd, stream, ni, mf, half);
+                const unsigned nb = launchFineDot<T>(D, mode, xx, yy, rr, dd, c1, c2, stop, *fd, stream, ni, el, half);
                 if (nb) {
                     if (fd->kind == FD_PQ) launchFinishPQ(fd->red, nb, st, stream, ni);
                     else launchFinishRZ(fd->red, nb, st, stream, ni);
                     return;
                 }
             } else {
-                launchFine<T>(D, mode, xx, yy, rr, dd, c1, c2, stop, stream, ni, mf, half);
+                launchFine<T>(D, mode, xx, yy, rr, dd, c1, c2, stop, stream, ni, el, half);
             }
         } else if constexpr (sizeof(T) == 4) {
             if (form & OF_MFMA_OWN) launchApplyOwnMf(D, mode, xx, yy, rr, dd, c1, c2, stop, stream, ni);
@@ -683,10 +722,13 @@ public:
         R.batch = o.check_every > 0 ? o.check_every : 8;
         // the reference's algorithm keeps its operator; the FP64 and mixed solves use the exact
         // child-scale form, and the mixed smoother the matrix cores when asked (one right-hand side)
-        // mfma_smoother bit 0: the level-0 in-plane stencil on the matrix cores; bit 1: own-element levels
+        // mfma_smoother bit 0: the V-cycle's levels without own elements as the element-by-element
+        // matrix-core product (two BF16 pieces); bit 1: own-element levels; bit 2: the outer FP32
+        // operator as the three-piece element product
+        const bool mx = R.prec == PSX_PREC_MIXED;
         R.form = R.prec == PSX_PREC_REF ? 0
-                                        : (OF_AUTO | (R.prec == PSX_PREC_MIXED && (o.mfma_smoother & 1) ? OF_MFMA : 0) |
-                                           (R.prec == PSX_PREC_MIXED && (o.mfma_smoother & 2) ? OF_MFMA_OWN : 0));
+                                        : (OF_AUTO | (mx && (o.mfma_smoother & 1) ? OF_MFMA : 0) | (mx && (o.mfma_smoother & 2) ? OF_MFMA_OWN : 0) |
+                                           (mx && (o.mfma_smoother & 4) ? OF_EL3 : 0));
         R.degree = o.cheb_degree > 0 ? o.cheb_degree : 2;
         if (R.degree > CHEB_MAX) throw std::invalid_argument("Chebyshev degree above 6");
         preDegree = o.pre_degree > 0 ? std::min(o.pre_degree, R.degree) : 0;
@@ -751,9 +793,9 @@ public:
         for (int b = 0; b < R.batch; b++) {
             if (prec == PSX_PREC_MIXED) {
                 const DotFusion fz{FD_RZ, q32, red}, fp{FD_PQ, nullptr, red};
-                const float* z = vcycle<float>(0, r32, stop, form, degree, &fz, R.fuseFirst);
+                const float* z = vcycle<float>(0, r32, stop, form & ~OF_EL3, degree, &fz, R.fuseFirst);
                 launchUpdateP<float>(z, p32, n, st, stream, k);
-                // the outer operator stays FP32 (the matrix-core form is for the smoother only)
+                // the outer operator: the FP32 VALU form, or the three-piece element product (OF_EL3)
                 op<float>(0, AM_APPLY, p32, q32, nullptr, nullptr, 0, 0, stop, form & ~(OF_MFMA | OF_MFMA_OWN), &fp);
                 launchUpdateXR32Fold(x32, p32, r32, q32, n, st, red, stream, k, R.fuseFirst ? lv[0].a32 : nullptr, D0.invD32, R.c0first);
                 // reliable update: the correction joins the FP64 solution and the FP64 true

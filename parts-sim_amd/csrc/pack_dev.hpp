@@ -74,6 +74,7 @@ __device__ __forceinline__ DevLevel globalArg(DevLevel v) {
     v.Kb32 = asGlobal(v.Kb32);
     v.St64 = asGlobal(v.St64);
     v.St32 = asGlobal(v.St32);
+    v.elA = asGlobal(v.elA);
     v.nodeScale = asGlobal(v.nodeScale);
     v.nodeInfo = asGlobal(v.nodeInfo);
     v.childS = asGlobal(v.childS);

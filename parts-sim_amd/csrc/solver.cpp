@@ -202,15 +202,16 @@ int psx_apply_child(psx_model m, int level, int fp32, const void* x, void* y, in
     });
 }
 
-int psx_apply_bf16(psx_model m, int level, const float* x, float* y) {
+int psx_apply_mfma(psx_model m, int level, int pieces, const float* x, float* y) {
     return guarded([&] {
         if (!m || !x || !y || level < 0 || level >= int(m->m->H.levels.size())) throw std::invalid_argument("bad arguments");
+        if (pieces != 2 && pieces != 3) throw std::invalid_argument("pieces must be 2 or 3");
         psx::Model& M = *m->m;
         auto& V = M.lv[level];
         if (M.H.levels[size_t(level)].nOwn > 0) throw std::invalid_argument("the level has own element matrices: no matrix-core operator");
         const int64_t n = V.dl.nDof;
         HIPCHK(hipMemcpyAsync(V.a32, x, size_t(n) * 4, hipMemcpyHostToDevice, M.stream));
-        M.op<float>(size_t(level), psx::AM_APPLY, V.a32, V.b32, nullptr, nullptr, 0, 0, nullptr, psx::OF_MFMA);
+        M.op<float>(size_t(level), psx::AM_APPLY, V.a32, V.b32, nullptr, nullptr, 0, 0, nullptr, pieces == 3 ? psx::OF_EL3 : psx::OF_MFMA);
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(y, V.b32, size_t(n) * 4, hipMemcpyDeviceToHost, M.stream));
         M.sync();
@@ -465,21 +466,25 @@ int psx_profile(psx_model m, int reps, char* buf, size_t len) {
             HIPCHK(hipEventElapsedTime(&t, e0, e1));
             return double(t) / reps;
         };
-        char line[400];
+        char line[512];
         for (size_t l = 0; l < M.lv.size(); l++) {
             auto& V = M.lv[l];
             const psx::Level& L = M.H.levels[l];
             const double t64 = timeIt([&] { M.op<double>(l, psx::AM_APPLY, V.a64, V.b64, nullptr, nullptr, 0, 0, nullptr); });
             const double t32 = timeIt([&] { M.op<float>(l, psx::AM_APPLY, V.a32, V.b32, nullptr, nullptr, 0, 0, nullptr); });
-            double tmf = 0, c64 = 0, c32 = 0;
-            if (L.nOwn == 0 && !V.dl.dense) tmf = timeIt([&] { M.op<float>(l, psx::AM_APPLY, V.a32, V.b32, nullptr, nullptr, 0, 0, nullptr, psx::OF_MFMA); });
+            double tmf = 0, tmf3 = 0, c64 = 0, c32 = 0;
+            if (L.nOwn == 0) {
+                tmf = timeIt([&] { M.op<float>(l, psx::AM_APPLY, V.a32, V.b32, nullptr, nullptr, 0, 0, nullptr, psx::OF_MFMA); });
+                tmf3 = timeIt([&] { M.op<float>(l, psx::AM_APPLY, V.a32, V.b32, nullptr, nullptr, 0, 0, nullptr, psx::OF_EL3); });
+            }
             if (V.dl.childForm) {
                 c64 = timeIt([&] { M.op<double>(l, psx::AM_APPLY, V.a64, V.b64, nullptr, nullptr, 0, 0, nullptr, psx::OF_CHILD); });
                 c32 = timeIt([&] { M.op<float>(l, psx::AM_APPLY, V.a32, V.b32, nullptr, nullptr, 0, 0, nullptr, psx::OF_CHILD); });
             }
             std::snprintf(line, sizeof line,
-                          "L%zu nodes=%lld elems=%zu own matrices=%lld: matvec64 %.4f ms  matvec32 %.4f ms  matvec-bf16 %.4f ms  child-form64 %.4f ms  child-form32 %.4f ms\n",
-                          l, (long long)L.nNodes, size_t(L.nElem), (long long)L.nOwn, t64, t32, tmf, c64, c32);
+                          "L%zu nodes=%lld elems=%zu own matrices=%lld: matvec64 %.4f ms  matvec32 %.4f ms  matvec-mfma2 %.4f ms  matvec-mfma3 %.4f ms  "
+                          "child-form64 %.4f ms  child-form32 %.4f ms\n",
+                          l, (long long)L.nNodes, size_t(L.nElem), (long long)L.nOwn, t64, t32, tmf, tmf3, c64, c32);
             out += line;
         }
         const double tv64 = timeIt([&] { M.vcycle<double>(0, M.lv[0].r64, nullptr, psx::OF_AUTO); });
@@ -488,7 +493,7 @@ int psx_profile(psx_model m, int reps, char* buf, size_t len) {
         const double tvbf = timeIt([&] { M.vcycle<float>(0, M.lv[0].r32, nullptr, psx::OF_AUTO | psx::OF_MFMA); });
         const double tvref = timeIt([&] { M.vcycle<float>(0, M.lv[0].r32, nullptr, 0); });
         std::snprintf(line, sizeof line,
-                      "V-cycle FP64 %.4f ms  FP32 %.4f ms  FP32 degree-3 %.4f ms  FP32+BF16 matrix cores %.4f ms  FP32 stored own matrices %.4f ms\n",
+                      "V-cycle FP64 %.4f ms  FP32 %.4f ms  FP32 degree-3 %.4f ms  FP32 with the matrix-core element product %.4f ms  FP32 stored own matrices %.4f ms\n",
                       tv64, tv32, tv3, tvbf, tvref);
         out += line;
         (void)hipEventDestroy(e0);

@@ -137,3 +137,22 @@ def test_matrix_core_own_elements(P, W, cpu, nrhs):
     V, infos32 = m.solve_multi(F, tol=1e-10, precision="mixed")
     assert max(abs(a["iterations"] - b["iterations"]) for a, b in zip(infos, infos32)) <= 6
     m.close()
+
+
+def test_matrix_core_element_product_batch_equals_lone(P, W, cpu):
+    """The element-by-element matrix-core operator in the V-cycle and as the outer operator
+    (mfma_smoother 1 | 4) over a batch of load cases: each block works on one case's vectors with that
+    case's reference value, so every case equals its lone solve bit for bit, and the FP64 oracle to
+    1e-8."""
+    w = W.simp_block((24, 12, 12), seed=5, nrhs=5)
+    F = w["f"]
+    m = _model(P, w)
+    U, infos = m.solve_multi(F, tol=1e-10, precision="mixed", mfma_smoother=5)
+    ref = cpu.Model(w["dims"], w["density"], w["nu"], w["bc"], 300)
+    for k in (0, 2, 4):
+        lone = m.solve(F[k], tol=1e-10, precision="mixed", mfma_smoother=5)
+        assert infos[k]["converged"] and infos[k]["iterations"] == lone["iterations"]
+        assert np.array_equal(U[k], lone["u"])
+        r = ref.solve(F[k], tol=1e-12)
+        assert np.linalg.norm(U[k] - r["u"]) / np.linalg.norm(r["u"]) < 1e-8
+    m.close()

@@ -148,39 +148,46 @@ def test_coarsest_buffer_reuse_does_not_leak(models):
     assert s["converged"]
 
 
-def test_bf16_matrix_core_operator(P, W):
-    """The matrix-core operator of the mixed V-cycle against the FP64 operator: the in-plane stencil
-    sums are BF16 products of neighbour differences (relative 2^-9 each) with FP32 accumulation, so
-    the product is accurate to about 1e-2 of |K| |x| for rough vectors and for smooth ones (difference
-    form)."""
-    w = W.void_block((40, 16, 12), seed=5)
-    g = P.Model(w["dims"], w["density"], w["nu"], w["bc"], 300)
-    n = g.n_dof
-    nx, ny, nz = w["dims"]
-    rough = np.random.default_rng(4).standard_normal(n)
-    Z, Y, X = np.meshgrid(np.arange(nz + 1), np.arange(ny + 1), np.arange(nx + 1), indexing="ij")
-    smooth = np.stack([np.sin(X / 9.0) + 0.1 * Y, np.cos(Y / 5.0) * Z / 7.0, 0.02 * X * Z], -1).reshape(-1)
-    for x, tol in ((rough, 2e-2), (smooth, 2e-2)):
-        y64 = g.apply(x)
-        yb = g.apply_bf16(x.astype(np.float32)).astype(np.float64)
-        err = np.linalg.norm(yb - y64) / np.linalg.norm(y64)
-        print("bf16 operator rel err", err, flush=True)
-        assert err < tol
-    # a rigid translation: exactly zero at nodes whose voxels all carry one scale (every difference
-    # vanishes); at the part's faces and void surfaces the FP32 corrections of the differing voxels
-    # meet BF16 stencil sums, which leaves a residue at BF16 rounding relative to |K| |t|
-    t = np.tile([1.0, -2.0, 0.5], n // 3)
-    yt = g.apply_bf16(t.astype(np.float32))
-    scale = np.abs(g.apply(rough)).max() * np.abs(t).max() / np.abs(rough).max()
-    print("translation residue", np.abs(yt).max() / scale, flush=True)
-    assert np.abs(yt).max() <= 1e-2 * scale
-    # a solid block: every interior node is regular, so the translation vanishes there exactly
-    ws = W.void_block((24, 12, 10), seed=5, void_fraction=0.0)
-    gs = P.Model(ws["dims"], ws["density"], ws["nu"], ws["bc"], 300)
-    ts = np.tile([1.0, -2.0, 0.5], gs.n_dof // 3)
-    ys = gs.apply_bf16(ts.astype(np.float32)).reshape(ws["dims"][2] + 1, ws["dims"][1] + 1, ws["dims"][0] + 1, 3)
-    assert np.all(ys[1:-1, 1:-1, 1:-1] == 0)
-    gs.close()
+@pytest.mark.parametrize("pieces", [2, 3])
+def test_element_matrix_core_operator(P, W, pieces):
+    """The element-by-element matrix-core operator against the FP64 operator. Each voxel's product
+    Kb (x_e - c) is a sum of BF16 piece products with FP32 accumulation (c: one node's value per tile):
+    two pieces (the mixed V-cycle's smoothing) carry x - c to about 2^-17, three to FP32 rounding, so
+    the product is within 2e-5 (two) or 3e-7 (three) of K x for rough vectors and within 2e-4 (two)
+    or 3e-6 (three) for smooth ones, relative in the 2-norm (measured: 6e-6 / 7e-8 and 6e-5 / 8e-7;
+    the FP32 VALU operator: 1e-7 and 1.3e-6). Parts: voids at 1e-3 stiffness, an odd grid with empty
+    voxels (tiles with one node column and one node row), per-voxel SIMP densities."""
+    parts = [W.void_block((40, 16, 12), seed=5), W.void_block((21, 13, 11), seed=2, void_density=0.0, void_fraction=0.3),
+             W.simp_block((32, 16, 16), seed=1, nrhs=1)]
+    for w in parts:
+        g = P.Model(w["dims"], w["density"], w["nu"], w["bc"], 300)
+        n = g.n_dof
+        nx, ny, nz = w["dims"]
+        rough = np.random.default_rng(4).standard_normal(n)
+        Z, Y, X = np.meshgrid(np.arange(nz + 1), np.arange(ny + 1), np.arange(nx + 1), indexing="ij")
+        smooth = np.stack([np.sin(X / 9.0) + 0.1 * Y, np.cos(Y / 5.0) * Z / 7.0, 0.02 * X * Z], -1).reshape(-1)
+        for x, tol in ((rough, 2e-5 if pieces == 2 else 3e-7), (smooth, 2e-4 if pieces == 2 else 3e-6)):
+            y64 = g.apply(x)
+            ym = g.apply_mfma(x.astype(np.float32), 0, pieces).astype(np.float64)
+            err = np.linalg.norm(ym - y64) / np.linalg.norm(y64)
+            print(w["dims"], pieces, "pieces: rel err", err, flush=True)
+            assert err < tol
+        # a rigid translation is the reference value itself in every tile: exactly zero everywhere,
+        # void surfaces and the part's faces included
+        t = np.tile([1.0, -2.0, 0.5], n // 3)
+        assert np.all(g.apply_mfma(t.astype(np.float32), 0, pieces) == 0)
+        g.close()
+
+
+def test_element_matrix_core_operator_rejects(P, W):
+    w = W.void_block((16, 8, 8), seed=1)
+    g = P.Model(w["dims"], w["density"], w["nu"], w["bc"], 60)
+    with pytest.raises(Exception):
+        g.apply_mfma(np.zeros(g.n_dof, np.float32), 0, 4)
+    if g.num_levels > 1 and g.level(1)["nOwn"] > 0:
+        with pytest.raises(Exception):
+            g.apply_mfma(np.zeros(g.level(1)["nDof"], np.float32), 1, 2)
+    g.close()
 
 
 def test_mixed_iterations_close_to_fp64(models):
@@ -242,13 +249,22 @@ def test_degree3_smoother_matches_oracle(P, W, cpu, precision):
         assert abs(s["iterations"] - so["iterations"]) <= 2
 
 
-def test_mixed_with_matrix_core_smoother(P, W):
-    w = W.void_block((48, 16, 16), seed=3)
-    g = P.Model(w["dims"], w["density"], w["nu"], w["bc"], 1200)
-    a = g.solve(w["f"], tol=1e-10, precision="mixed", mfma_smoother=True)
-    b = g.solve(w["f"], tol=1e-10, precision="fp64")
-    print("mfma smoother its", a["iterations"], "fp64", b["iterations"], flush=True)
-    assert a["converged"] and _rel(a["u"], b["u"]) < 1e-7
+@pytest.mark.parametrize("form", [1, 4, 5])
+def test_mixed_with_matrix_core_operators(P, W, form):
+    """Mixed solves with the matrix-core element product in the V-cycle (form 1), as the outer
+    operator (form 4) and both (5): the FP64 solution to 1e-7 and iteration counts within 2 of the
+    FP32 VALU form's (void block; odd grid with empty voxels; per-voxel densities)."""
+    for w in (W.void_block((48, 16, 16), seed=3), W.void_block((21, 13, 11), seed=2, void_density=0.0, void_fraction=0.3),
+              W.simp_block((32, 16, 16), seed=1, nrhs=1)):
+        f = w["f"] if w["f"].ndim == 1 else w["f"][0]
+        g = P.Model(w["dims"], w["density"], w["nu"], w["bc"], 1200)
+        a = g.solve(f, tol=1e-10, precision="mixed", mfma_smoother=form)
+        v = g.solve(f, tol=1e-10, precision="mixed")
+        b = g.solve(f, tol=1e-10, precision="fp64")
+        print(w["dims"], "form", form, "its", a["iterations"], "valu", v["iterations"], "fp64", b["iterations"], flush=True)
+        assert a["converged"] and _rel(a["u"], b["u"]) < 1e-7
+        assert abs(a["iterations"] - v["iterations"]) <= 2
+        g.close()
 
 
 def test_stresses_match_oracle(models):
