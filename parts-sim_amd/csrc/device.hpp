@@ -25,8 +25,8 @@ struct DevLevel {
     const double* St64;        // 27-point stencil of Kb: [27 neighbours][3 rows][3 cols]
     const float* St32;
     // Kb as the matrix-core A operands of fine_el.hip, [half][piece][lane] x 8 BF16: Kb split into three
-    // BF16 pieces (hi, mid, lo), output corners 4 half .. 4 half + 3 (numbered x + 2 y + 4 z) in rows
-    // 4 corner + component, input corner pairs in the K slices
+    // BF16 pieces (hi, mid, lo); rows 4 component + output corner (x + 2 y, at z offset half), input
+    // corner pairs in the K slices
     const uint4* elA;
     const float* nodeScale;    // s for nodes whose 8 voxels are s * Kb, else 0
     const uint16_t* nodeInfo;  // per node: bit v (0..7) set when voxel v's scale differs from the
@@ -128,10 +128,10 @@ void launchApplyOwnMf(const DevLevel& L, int mode, const float* x, float* y, con
 // smoothing)
 template <class T> void launchFine(const DevLevel& L, int mode, const T* x, T* y, const T* r, const T* zp, double c1, double c2, const int* stop,
                                    hipStream_t s, int ninst = 1, int el = 0, bool half = false);
-// the element-by-element matrix-core operator itself (fine_el.hip), on launchFine's tiles (zChunk:
-// planes per block; dots: fused dot partials per tile, as launchFineDot writes them)
-void launchFineEl(const DevLevel& L, int mode, const float* x, float* y, const float* r, const float* zp, double c1, double c2, const int* stop, int zChunk,
-                  double* dots, int64_t dotStride, const float* qdot, hipStream_t s, int ninst, int pieces);
+// the element-by-element matrix-core operator itself (fine_el.hip); dots: fused dot partials, one per
+// tile (the kinds of launchFineDot); returns the number of partials written (0: no fused dot)
+unsigned launchFineEl(const DevLevel& L, int mode, const float* x, float* y, const float* r, const float* zp, double c1, double c2, const int* stop,
+                      double* dots, int64_t dotStride, const float* qdot, hipStream_t s, int ninst, int pieces);
 // Dot products fused into an operator launch (conjugate gradients), block partials into red:
 //   FD_PQ  (AM_APPLY)  p.q with p = x, q = y                         -> launchFinishPQ
 //   FD_RZ  (AM_CHEB1/2) r.z and z.q with z = y, q = qdot             -> launchFinishRZ

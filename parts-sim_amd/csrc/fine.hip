@@ -682,7 +682,7 @@ void fineGo(dim3 g, hipStream_t s, const DevLevel& L, const Sym<T>& C, const T* 
 
 template <class T, bool OWNC>
 static void launchFineT(const DevLevel& L, int mode, const Sym<T>& C, const T* crow, const T* ctab, const T* x, T* y, const T* r, const T* d, double c1, double c2,
-                        const int* stop, int zChunk, double* dots, int64_t dotStride, const T* qdot, hipStream_t s, int ninst, int el,
+                        const int* stop, int zChunk, double* dots, int64_t dotStride, const T* qdot, hipStream_t s, int ninst,
                         bool half = false) {
     const int gx = (L.NX + TX - 1) / TX, gy = (L.NY + TY - 1) / TY, gz = (L.NZ + zChunk - 1) / zChunk;
     const int nTiles = gx * gy * gz;
@@ -697,13 +697,6 @@ static void launchFineT(const DevLevel& L, int mode, const Sym<T>& C, const T* c
     // fused dots exist for APPLY (p.q) and the Chebyshev steps (r.z, z.q) only
     const bool dot = dots && (mode == AM_APPLY || mode == AM_CHEB1 || mode == AM_CHEB2);
     const bool dense = !OWNC && L.dense;
-    if constexpr (sizeof(T) == 4 && !OWNC) {
-        // the element-by-element product on the matrix cores (fine_el.hip)
-        if (el > 0) {
-            launchFineEl(L, mode, x, y, r, d, c1, c2, stop, zChunk, dots, dotStride, qdot, s, ninst, el);
-            return;
-        }
-    }
     const bool useH16 = sizeof(T) == 4 && OWNC && half && L.ctab16;
     auto form = [&](auto m, auto dt) {
         constexpr int M = decltype(m)::value;
@@ -747,6 +740,13 @@ static void launchFineT(const DevLevel& L, int mode, const Sym<T>& C, const T* c
 template <class T>
 unsigned launchFineDot(const DevLevel& L, int mode, const T* x, T* y, const T* r, const T* d, double c1, double c2, const int* stop, const DotFusion& fd,
                        hipStream_t s, int ninst, int el, bool half) {
+    const bool fused = (fd.kind == FD_PQ && mode == AM_APPLY) || (fd.kind == FD_RZ && (mode == AM_CHEB1 || mode == AM_CHEB2));
+    double* dots = fused ? fd.red.partials : nullptr;
+    const T* qdot = fused && fd.kind == FD_RZ ? static_cast<const T*>(fd.q) : nullptr;
+    if constexpr (sizeof(T) == 4) {
+        // the element-by-element product on the matrix cores (fine_el.hip), on its own tiles
+        if (el > 0 && !L.childForm) return launchFineEl(L, mode, x, y, r, d, c1, c2, stop, dots, fd.red.stride, qdot, s, ninst, el);
+    }
     Sym<T> C;
     for (int q = 0; q < 22; q++) C.v[q] = sizeof(T) == 8 ? T(L.sym64[q]) : T(L.sym32[q]);
     // the longest z-chunk (16, 8 or 4 planes) that still gives >= 4096 blocks (16 per CU); shorter
@@ -767,16 +767,13 @@ unsigned launchFineDot(const DevLevel& L, int mode, const T* x, T* y, const T* r
     // (measured: near-equal chunks of 8-9 planes, which drop the one-plane last chunk of NZ = 2^k + 1,
     // are 4 % slower on the 512x128x128 block than this rule's 8-plane chunks)
     const int gz = (L.NZ + zChunk - 1) / zChunk;
-    const bool fused = (fd.kind == FD_PQ && mode == AM_APPLY) || (fd.kind == FD_RZ && (mode == AM_CHEB1 || mode == AM_CHEB2));
-    double* dots = fused ? fd.red.partials : nullptr;
-    const T* qdot = fused && fd.kind == FD_RZ ? static_cast<const T*>(fd.q) : nullptr;
     if (L.childForm) {
         const T *crow = nullptr, *ctab = nullptr;
         if constexpr (sizeof(T) == 8) crow = L.crow64, ctab = L.ctab64;
         else crow = L.crow32, ctab = L.ctab32;
-        launchFineT<T, true>(L, mode, C, crow, ctab, x, y, r, d, c1, c2, stop, zChunk, dots, fd.red.stride, qdot, s, ninst, false, half);
+        launchFineT<T, true>(L, mode, C, crow, ctab, x, y, r, d, c1, c2, stop, zChunk, dots, fd.red.stride, qdot, s, ninst, half);
     } else {
-        launchFineT<T, false>(L, mode, C, nullptr, nullptr, x, y, r, d, c1, c2, stop, zChunk, dots, fd.red.stride, qdot, s, ninst, el);
+        launchFineT<T, false>(L, mode, C, nullptr, nullptr, x, y, r, d, c1, c2, stop, zChunk, dots, fd.red.stride, qdot, s, ninst);
     }
     return fused ? unsigned(gx * gy * gz) : 0u;
 }

@@ -9,19 +9,24 @@
 // weight is below 2^-8 per piece order are left out (two pieces each: hi.hi + hi.lo + lo.hi, an
 // error of about 2^-17 |Kb| |x'|; three: six pairs, about 2^-25).
 //
-// Layout of the work: a block of 256 threads owns the same 32 x 8 column of nodes as kFine and
-// marches along z one element layer (the voxels between node planes k and k + 1) per step:
-//   1. the layer's 33 x 9 elements, 16 per matrix instruction: columns = elements, K = the 8 corners
-//      x 4 components (the fourth zero) read as two adjacent nodes per lane from the LDS node planes
-//      (a node's 3 components + a zero, as 4 BF16: one 16-byte read gives the lane its K slice),
-//      rows = 4 output corners x 4 components (two instructions per piece pair cover the 8 corners);
-//   2. the products go to LDS, [component][corner][element];
-//   3. each node sums the corner rows of its 4 elements in the layer, times their voxel scales, in a
-//      fixed order: the lower corners complete node k, the upper ones start node k + 1.
+// Layout of the work: a block of 4 wavefronts owns 60 x 8 nodes in x-y and marches along z one
+// element layer (the voxels between node planes k and k + 1) per step. A wavefront owns a strip of
+// 15 x 8 nodes and computes the 16 x 9 elements around it, one element row (16 elements along x) per
+// matrix-instruction group:
+//   columns = the 16 elements; K = their 8 corners x 4 components (the fourth zero), read as two
+//   adjacent nodes per lane from the LDS node planes (4 BF16 per node: one 16-byte read); rows =
+//   4 components x 4 corners of one z side (two instructions per piece pair cover both sides), so
+//   lane (element n, component c) holds in its 4 accumulators the element's 4 corners of that side.
+// Each lane scales its element's corners by the voxel scale, and the node sums follow without a trip
+// through LDS: the x neighbour's corner arrives by a row shift (DPP row_shr:1: lane n - 1 to lane n,
+// the 16 lanes of a row being the 16 elements of one component), the y neighbour's from the previous
+// element row's group and the z neighbour's from the previous layer, both in registers. Node planes
+// are the only shared data: one barrier per layer.
 // Nothing depends on which voxels differ in scale, so void surfaces and per-voxel densities cost the
 // same as solid regions (no pair lists).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdlib>
 #include <type_traits>
 
@@ -36,22 +41,27 @@ namespace {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
-constexpr int TX = 32, TY = 8, NT = TX * TY, NW = NT / 64;
-constexpr int PX = TX + 2, PY = TY + 2, NP = PX * PY;  // node plane with halo (34 x 10)
-constexpr int EX = TX + 1, EY = TY + 1, NE = EX * EY;  // element layer (33 x 9)
-constexpr int NGMAX = (NE + 15) / 16;                  // matrix-instruction groups of 16 elements
-constexpr int EPAD = NGMAX * 16;
-constexpr int GPW = (NGMAX + NW - 1) / NW;              // groups per wavefront at most (a contiguous run)
-constexpr int NVL = (NE + NT - 1) / NT;                 // voxels staged per thread
+constexpr int NT = 256, NW = NT / 64;
+constexpr int SX = 15;                                  // nodes per wavefront strip in x (16 elements)
+constexpr int BX = SX * NW, R = 8;                       // block tile: 60 x 8 nodes
+constexpr int PX = BX + 2, PY = R + 2, NP = PX * PY;     // node plane with halo (62 x 10)
+constexpr int NQ = (NP + NT - 1) / NT;                   // nodes staged per thread
+constexpr int VX = BX + 1, VY = R + 1, NV = VX * VY;     // voxel layer of the block (61 x 9)
+constexpr int NVQ = (NV + NT - 1) / NT;                  // voxels staged per thread
+constexpr int EPI = 4;                                   // node rows per epilogue pass
 
 template <int XP> struct ElShared {
-    uint2 xq[2][XP][NP];     // node planes k, k + 1: the pieces of x - c, 4 BF16 per node
-    float s[3][NE];          // voxel scales of element layers k - 1 .. k + 1 (0: empty or outside)
-    float4 e[8][EPAD];       // the layer's products Kb (x_e - c): [corner][element][component, 0]
+    uint2 xq[3][XP][NP];  // node planes k .. k + 2 (ring): the pieces of x - c, 4 BF16 per node
+    float xf[3][R * BX * 3];  // the same planes' tile nodes (no halo) in FP32: the epilogue's own x
+    float s[2][NV];       // voxel scales of element layers k, k + 1 (0: empty or outside)
 };
 
 __device__ __forceinline__ uint32_t bfBits(float v) { return uint32_t(__builtin_bit_cast(uint16_t, (__bf16)v)); }
 __device__ __forceinline__ float bfValue(uint32_t b) { return __uint_as_float(b << 16); }
+// lane n of each 16-lane row receives v of lane n - 1 (lane 0 of a row: 0)
+__device__ __forceinline__ float fromLeft(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x111, 0xf, 0xf, true));
+}
 
 #define PSX_EL_PARAMS                                                                                                                          \
     DevLevel L, const float *__restrict__ x, float *__restrict__ y, const float *__restrict__ r, const float *__restrict__ zp, float c1, float c2, \
@@ -72,42 +82,32 @@ template <int MODE, bool DOT, int XP, int AP> __device__ __forceinline__ void el
     const int t = int(vb.x);
     if (t >= nTiles) return;
     __shared__ ElShared<XP> sh;
-    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    const int i0 = (t % gx) * TX, j0 = ((t / gx) % gy) * TY;
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6, n = lane & 15, g = lane >> 4;
+    const int X0 = (t % gx) * BX, Y0 = ((t / gx) % gy) * R;
     const int z0 = (t / (gx * gy)) * zChunk, z1 = min(L.NZ, z0 + zChunk);
-    // the last tile column of a part (usually one node column) takes the transposed mapping, 8 columns
-    // of 8 rows per wavefront, so that whole wavefronts past the part's edge skip the node work
-    const bool transposed = L.NX - i0 <= TX / 4;
-    const int tx = transposed ? tid / TY : tid % TX, ty = transposed ? tid % TY : tid / TX;
-    const int i = i0 + tx, j = j0 + ty;
-    const bool mine = i < L.NX && j < L.NY;
     const int64_t NX = L.NX, NXY = int64_t(L.NX) * L.NY;
     constexpr bool NEEDS_R = MODE == AM_RESID || MODE == AM_CHEB1 || MODE == AM_CHEB2;
     constexpr bool NEEDS_D = MODE == AM_CHEB2;
     constexpr bool NEEDS_INV = MODE == AM_CHEB1 || MODE == AM_CHEB2;
+    // this wavefront's strip: nodes Xw .. Xw + 14; lane n's element is voxel Xw - 1 + n, its node
+    // (after the row shift) Xw - 1 + n, so lanes 1..15 own the strip's nodes; lane group g is the
+    // component c in the products and the epilogue (g = 3 holds none)
+    const int Xw = X0 + SX * wid;
+    const int rows = min(R, L.NY - Y0);
+    const bool waveActive = Xw < L.NX;
+    const int Xn = Xw - 1 + n;
+    const bool owner = n >= 1 && Xn < L.NX && g < 3;
 
     // the block's reference value c: the node at the tile's centre (uniform loads)
     float cref[3];
     {
-        const int ic = min(i0 + TX / 2, L.NX - 1), jc = min(j0 + TY / 2, L.NY - 1), zc = min(z0 + zChunk / 2, L.NZ - 1);
+        const int ic = min(X0 + BX / 2, L.NX - 1), jc = min(Y0 + R / 2, L.NY - 1), zc = min(z0 + zChunk / 2, L.NZ - 1);
         const int64_t n3 = 3 * (ic + NX * jc + NXY * zc);
         cref[0] = x[n3];
         cref[1] = x[n3 + 1];
         cref[2] = x[n3 + 2];
     }
-
-    // the tile's element layer: ncx x ncy elements (local element (ex, ey) = voxel (i0 - 1 + ex,
-    // j0 - 1 + ey)), flattened as ex + EX ey, or ey + EY ex in a transposed tile (a part's last tile
-    // column: few columns), into groups of 16; the elements past ncx in a row of a tile at the x edge
-    // are computed and never gathered
-    const int ncx = min(TX, L.NX - i0) + 1, ncy = min(TY, L.NY - j0) + 1;
-    const int nEl = transposed ? EY * ncx : EX * ncy, nGroups = (nEl + 15) >> 4;
-    auto elemXY = [&](int e, int& ex, int& ey) {
-        if (transposed) ex = e / EY, ey = e % EY;
-        else ex = e % EX, ey = e / EX;
-    };
-
-    // matrix-core operands of Kb (see DevLevel::elA): A[h][piece] for output corners 4 h .. 4 h + 3
+    // matrix-core operands of Kb (see DevLevel::elA): A[h][piece] for the corners at z offset h
     bf16x8 A[2][AP];
 #pragma unroll
     for (int h = 0; h < 2; h++)
@@ -119,62 +119,34 @@ template <int MODE, bool DOT, int XP, int AP> __device__ __forceinline__ void el
             A[h][p] = __builtin_bit_cast(bf16x8, w);
         }
 
-    // this wavefront's element groups: the contiguous run wid * GPW + u, u < myGroups; for each, the
-    // lane's node pair in the node plane (its K slice: corners at y offset (lane / 16) & 1, z offset
-    // lane / 32)
-    const int myGroups = max(0, min(GPW, nGroups - wid * GPW));
-    int bOff[GPW];
+    // staging: node e of the plane with halo (local (e % PX, e / PX) = node (X0 - 1, Y0 - 1) + that),
+    // e = tid + q NT; offsets OUT_OF_RANGE outside the part (loads return 0)
+    int32_t poff[NQ];
 #pragma unroll
-    for (int u = 0; u < GPW; u++) {
-        const int e = 16 * (wid * GPW + u) + (lane & 15);
-        int ex, ey;
-        elemXY(e < nEl ? e : 0, ex, ey);
-        bOff[u] = (ey + ((lane >> 4) & 1)) * PX + ex;
-    }
-    const int zOff = lane >> 5;  // the lane's corners lie on node plane k + zOff
-
-    // staging: each thread stages its own node (interior position of the plane) and threads
-    // 0 .. NP - NT - 1 one halo node each; offsets OUT_OF_RANGE outside the part (loads return 0)
-    int lpos[2];
-    int32_t poff[2];
-    {
-        lpos[0] = (ty + 1) * PX + tx + 1;
-        const int h = tid;
-        int hx, hy;
-        if (h < PX) hx = h, hy = 0;
-        else if (h < 2 * PX) hx = h - PX, hy = PY - 1;
-        else if (h < 2 * PX + TY) hx = 0, hy = h - 2 * PX + 1;
-        else hx = PX - 1, hy = h - 2 * PX - TY + 1;
-        lpos[1] = h < NP - NT ? hy * PX + hx : -1;
-#pragma unroll
-        for (int q = 0; q < 2; q++) {
-            const int lp = lpos[q] < 0 ? 0 : lpos[q];
-            const int gi = i0 - 1 + lp % PX, gj = j0 - 1 + lp / PX;
-            const bool in = lpos[q] >= 0 && gi >= 0 && gj >= 0 && gi < L.NX && gj < L.NY;
-            poff[q] = in ? int32_t(12 * (gi + NX * gj)) : OUT_OF_RANGE;
-        }
-    }
-    int32_t voff[NVL];
-#pragma unroll
-    for (int q = 0; q < NVL; q++) {
+    for (int q = 0; q < NQ; q++) {
         const int e = tid + q * NT;
-        const int vi = i0 - 1 + e % EX, vj = j0 - 1 + e / EX;
-        const bool in = e < NE && vi >= 0 && vj >= 0 && vi < L.nx && vj < L.ny;
-        voff[q] = in ? int32_t(4 * (vi + int64_t(L.nx) * vj)) : OUT_OF_RANGE;
+        const int gi = X0 - 1 + e % PX, gj = Y0 - 1 + e / PX;
+        const bool in = e < NP && gi >= 0 && gj >= 0 && gi < L.NX && gj < L.NY;
+        poff[q] = in ? int32_t(12 * (gi + NX * gj)) : OUT_OF_RANGE;
     }
     const uint32_t planeBytes = uint32_t(12 * NXY);
-    const int64_t nxy = int64_t(L.nx) * L.ny;
-    auto fetchPlane = [&](int z, float (&px)[2][3]) {
+    auto fetchPlane = [&](int z, float (&px)[NQ][3]) {
         const bool zin = z >= 0 && z < L.NZ;
         const auto rs = planeRsrc(x + 3 * NXY * (zin ? z : 0), zin ? planeBytes : 0u);
 #pragma unroll
-        for (int q = 0; q < 2; q++) bload3(rs, poff[q], px[q]);
+        for (int q = 0; q < NQ; q++) bload3(rs, poff[q], px[q]);
     };
-    auto storePlane = [&](int z, const float (&px)[2][3]) {
-        const int sl = z & 1;
+    auto storePlane = [&](int z, const float (&px)[NQ][3]) {
+        const int sl = (z + 3) % 3;
 #pragma unroll
-        for (int q = 0; q < 2; q++) {
-            if (lpos[q] < 0) continue;
+        for (int q = 0; q < NQ; q++) {
+            const int e = tid + q * NT;
+            if (e >= NP) continue;
+            const int lx = e % PX - 1, ly = e / PX - 1;
+            if (lx >= 0 && lx < BX && ly >= 0 && ly < R) {
+#pragma unroll
+                for (int c = 0; c < 3; c++) sh.xf[sl][(ly * BX + lx) * 3 + c] = px[q][c];
+            }
             float v[3];
 #pragma unroll
             for (int c = 0; c < 3; c++) v[c] = px[q][c] - cref[c];
@@ -186,62 +158,74 @@ template <int MODE, bool DOT, int XP, int AP> __device__ __forceinline__ void el
                     b[c] = bfBits(v[c]);
                     v[c] -= bfValue(b[c]);
                 }
-                sh.xq[sl][p][lpos[q]] = uint2{b[0] | (b[1] << 16), b[2]};
+                sh.xq[sl][p][e] = uint2{b[0] | (b[1] << 16), b[2]};
             }
         }
     };
-    auto fetchVox = [&](int k, float (&pv)[NVL]) {
+    // the voxel scales of a layer: voxel e of the block's layer (local (e % VX, e / VX) = voxel
+    // (X0 - 1, Y0 - 1) + that), e = tid + q NT
+    const int64_t nxy = int64_t(L.nx) * L.ny;
+    int32_t voff[NVQ];
+#pragma unroll
+    for (int q = 0; q < NVQ; q++) {
+        const int e = tid + q * NT;
+        const int vi = X0 - 1 + e % VX, vj = Y0 - 1 + e / VX;
+        const bool in = e < NV && vi >= 0 && vj >= 0 && vi < L.nx && vj < L.ny;
+        voff[q] = in ? int32_t(4 * (vi + int64_t(L.nx) * vj)) : OUT_OF_RANGE;
+    }
+    auto fetchVox = [&](int k, float (&pv)[NVQ]) {
         const bool kin = k >= 0 && k < L.nz;
         const auto rs = planeRsrc(L.words + nxy * (kin ? k : 0), kin ? uint32_t(4 * nxy) : 0u);
 #pragma unroll
-        for (int q = 0; q < NVL; q++) {
+        for (int q = 0; q < NVQ; q++) {
             const int32_t w = int32_t(__builtin_amdgcn_raw_buffer_load_b32(rs, voff[q], 0, 0));
             pv[q] = w > 0 ? __int_as_float(w) : 0.f;
         }
     };
-    auto storeVox = [&](int k, const float (&pv)[NVL]) {
-        const int sl = (k + 3) % 3;
+    auto storeVox = [&](int k, const float (&pv)[NVQ]) {
 #pragma unroll
-        for (int q = 0; q < NVL; q++) {
+        for (int q = 0; q < NVQ; q++) {
             const int e = tid + q * NT;
-            if (e < NE) sh.s[sl][e] = pv[q];
+            if (e < NV) sh.s[(k + 2) & 1][e] = pv[q];
         }
     };
-    // the epilogue's own operands of node (i, j, z), fetched one step before use
-    struct Own {
-        float x[3], r[3], d[3], qd[3];
-        uint32_t info;
+    // the epilogue's operands of node (Xn, Y0 + j, z), component g, fetched before the layer's products
+    const int32_t eoff = owner ? int32_t(4 * (3 * (Xn + NX * Y0) + g)) : OUT_OF_RANGE;  // + 12 NX j
+    const int32_t ioff = owner ? int32_t(2 * (Xn + NX * Y0)) : OUT_OF_RANGE;           // + 2 NX j
+    const int32_t rowF = int32_t(12 * NX), rowI = int32_t(2 * NX);
+    struct Epi {
+        float x[EPI], r[EPI], d[EPI], qd[EPI];
+        uint32_t info[EPI];
     };
-    const int32_t col = mine ? int32_t(i + NX * j) : 0;
-    const int32_t noff = mine ? 12 * col : OUT_OF_RANGE;
-    const int32_t ioff = mine ? 2 * col : OUT_OF_RANGE;
-    auto fetchOwn = [&](int z, Own& o) {
+    auto fetchEpi = [&](int z, int j0, Epi& o) {
         const bool zin = z >= z0 && z < z1;
-        const int64_t pb = NXY * (zin ? z : z0);
+        const int64_t pb = 3 * NXY * (zin ? z : z0);
         const uint32_t vbytes = zin ? planeBytes : 0u;
-        bload3(planeRsrc(x + 3 * pb, vbytes), noff, o.x);
-        if constexpr (NEEDS_R) bload3(planeRsrc(r + 3 * pb, vbytes), noff, o.r);
-        if constexpr (NEEDS_D) bload3(planeRsrc(zp ? zp + 3 * pb : x, zp ? vbytes : 0u), noff, o.d);
-        if constexpr (NEEDS_INV) {
-            if constexpr (DOT) bload3(planeRsrc(qdot ? qdot + 3 * pb : x, qdot ? vbytes : 0u), noff, o.qd);
-            else o.qd[0] = o.qd[1] = o.qd[2] = 0.f;
-        }
-        o.info = __builtin_amdgcn_raw_buffer_load_b16(planeRsrc(L.nodeInfo + pb, zin ? uint32_t(2 * NXY) : 0u), ioff, 0, 0);
-    };
-    // the node's 4 elements in a layer: local element (tx + dx, ty + dy) holds the node as its corner
-    // (1 - dx, 1 - dy, .)
-    int eIdx[4], sIdx[4];
+        const auto rr = planeRsrc(r ? r + pb : x, r ? vbytes : 0u);
+        const auto rd = planeRsrc(zp ? zp + pb : x, zp ? vbytes : 0u);
+        const auto rq = planeRsrc(qdot ? qdot + pb : x, qdot ? vbytes : 0u);
+        const auto ri = planeRsrc(L.nodeInfo + NXY * (zin ? z : z0), zin ? uint32_t(2 * NXY) : 0u);
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-        const int dx = q & 1, dy = q >> 1;
-        eIdx[q] = transposed ? (ty + dy) + EY * (tx + dx) : (tx + dx) + EX * (ty + dy);
-        sIdx[q] = (tx + dx) + EX * (ty + dy);
-    }
+        for (int jj = 0; jj < EPI; jj++) {
+            const int j = j0 + jj;
+            const bool jin = j < rows && owner;
+            const int32_t fo = jin ? eoff + j * rowF : OUT_OF_RANGE, io = jin ? ioff + j * rowI : OUT_OF_RANGE;
+            o.x[jj] = jin ? sh.xf[(z + 3) % 3][(j * BX + SX * wid + n - 1) * 3 + g] : 0.f;
+            if constexpr (NEEDS_R) o.r[jj] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rr, fo, 0, 0));
+            if constexpr (NEEDS_D) o.d[jj] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rd, fo, 0, 0));
+            if constexpr (NEEDS_INV && DOT) o.qd[jj] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rq, fo, 0, 0));
+            o.info[jj] = __builtin_amdgcn_raw_buffer_load_b16(ri, io, 0, 0);
+        }
+    };
+    // the lane's operand reads: node pair (local x SX wid + n, + 1) of local row i + (g & 1) in node
+    // plane k + (g >> 1)
+    const int bBase = (g & 1) * PX + SX * wid + n, zOff = g >> 1;
 
-    // prologue: node planes z0 - 1, z0 and voxel layer z0 - 1 in LDS; plane z0 + 1 and layer z0 in flight
-    float pn[2][3], vn[NVL];
+    // prologue: node planes z0 - 1, z0 and voxel layer z0 - 1 in LDS; plane z0 + 1 and layer z0 in
+    // flight
+    float pn[NQ][3], vn[NVQ];
     {
-        float pa[2][3], pb[2][3], va[NVL];
+        float pa[NQ][3], pb[NQ][3], va[NVQ];
         fetchPlane(z0 - 1, pa);
         fetchPlane(z0, pb);
         fetchVox(z0 - 1, va);
@@ -251,144 +235,139 @@ template <int MODE, bool DOT, int XP, int AP> __device__ __forceinline__ void el
         storePlane(z0, pb);
         storeVox(z0 - 1, va);
     }
-    Own cur{};  // node z0 - 1 has no epilogue; node z0's operands arrive in the first step
-    float carry[3] = {0.f, 0.f, 0.f};
+    float carry[R], syPrev[R];
+#pragma unroll
+    for (int j = 0; j < R; j++) carry[j] = syPrev[j] = 0.f;
     double dot0 = 0, dot1 = 0;
+    const float k00 = L.sym32[0];
     __syncthreads();
 
     for (int k = z0 - 1; k < z1; k++) {
-        // 1. the layer's element products: the wavefront's groups, every operand read first and the
-        //    piece products of all groups interleaved (independent accumulators)
-        {
-            // node plane k + zOff sits in slot (k + zOff) & 1
-            const uint2* __restrict__ xs = &sh.xq[(k + zOff) & 1][0][0];
-            auto run = [&](auto cnt) {
-                constexpr int CNT = decltype(cnt)::value;
-                bf16x8 B[CNT > 0 ? CNT : 1][XP];
+        // node planes k (lower corners) and k + 1 (upper corners) of the strip's rows, component g
+        float lo[R], up[R], sy[R];
 #pragma unroll
-                for (int u = 0; u < CNT; u++)
+        for (int j = 0; j < R; j++) lo[j] = up[j] = sy[j] = 0.f;
+        if (waveActive) {
+            const uint2* __restrict__ xs = &sh.xq[(k + zOff + 3) % 3][0][0];
+            const float* __restrict__ sv = &sh.s[(k + 2) & 1][SX * wid + n];
+            // element row Y0 - 1 + i: its operands (two node pairs per lane and piece) and scale
+            auto fetch = [&](int i, bf16x8 (&B)[XP], float& sc) {
 #pragma unroll
-                    for (int p = 0; p < XP; p++) {
-                        const uint2 lo = xs[p * NP + bOff[u]], hi = xs[p * NP + bOff[u] + 1];
-                        typedef unsigned u4 __attribute__((ext_vector_type(4)));
-                        const u4 w = {lo.x, lo.y, hi.x, hi.y};
-                        B[u][p] = __builtin_bit_cast(bf16x8, w);
+                for (int p = 0; p < XP; p++) {
+                    const uint2 a = xs[p * NP + bBase + i * PX], b = xs[p * NP + bBase + i * PX + 1];
+                    typedef unsigned u4 __attribute__((ext_vector_type(4)));
+                    const u4 w = {a.x, a.y, b.x, b.y};
+                    B[p] = __builtin_bit_cast(bf16x8, w);
+                }
+                sc = sv[i * VX];
+            };
+            // its products: C[h] = Kb (x_e - c) at the corners of z side h, the smaller piece
+            // products first
+            auto product = [&](const bf16x8 (&B)[XP], floatx4 (&C)[2]) {
+#pragma unroll
+                for (int h = 0; h < 2; h++) C[h] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int ord = (AP > XP ? AP : XP) - 1; ord >= 0; ord--)
+#pragma unroll
+                    for (int a = 0; a < AP; a++) {
+                        const int b = ord - a;
+                        if (b >= 0 && b < XP) {
+#pragma unroll
+                            for (int h = 0; h < 2; h++) C[h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[h][a], B[b], C[h], 0, 0, 0);
+                        }
                     }
-                const int g = lane >> 4;
+            };
+            // its node sums: lane (n, g) holds the element's corners (x, y) = (r & 1, r >> 1) of
+            // component g; the x-upper corners belong to the next lane's node
+            auto gather = [&](int i, const floatx4 (&C)[2], float s) {
 #pragma unroll
                 for (int h = 0; h < 2; h++) {
-                    floatx4 acc[CNT > 0 ? CNT : 1];
-#pragma unroll
-                    for (int u = 0; u < CNT; u++) acc[u] = floatx4{0.f, 0.f, 0.f, 0.f};
-                    // the smaller piece products first
-#pragma unroll
-                    for (int ord = (AP > XP ? AP : XP) - 1; ord >= 0; ord--)
-#pragma unroll
-                        for (int a = 0; a < AP; a++) {
-                            const int b = ord - a;
-                            if (b >= 0 && b < XP) {
-#pragma unroll
-                                for (int u = 0; u < CNT; u++) acc[u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[h][a], B[u][b], acc[u], 0, 0, 0);
-                            }
-                        }
-#pragma unroll
-                    for (int u = 0; u < CNT; u++) sh.e[4 * h + g][16 * (wid * GPW + u) + (lane & 15)] = float4{acc[u][0], acc[u][1], acc[u][2], acc[u][3]};
+                    float(&acc)[R] = h ? up : lo;
+                    if (i >= 1) {  // y-lower corners: node row Y0 + i - 1
+                        acc[i - 1] += s * C[h][0];
+                        acc[i - 1] += fromLeft(s * C[h][1]);
+                    }
+                    if (i < R) {  // y-upper corners: node row Y0 + i
+                        acc[i] += s * C[h][2];
+                        acc[i] += fromLeft(s * C[h][3]);
+                    }
+                }
+                if constexpr (NEEDS_INV) {
+                    const float sx = s + fromLeft(s);  // the node's two voxels in x of this element row
+                    if (i >= 1) sy[i - 1] += sx;
+                    if (i < R) sy[i] += sx;
                 }
             };
-            static_assert(GPW == 5, "group runs of at most 5");
-            switch (myGroups) {
-                case 5: run(std::integral_constant<int, 5>{}); break;
-                case 4: run(std::integral_constant<int, 4>{}); break;
-                case 3: run(std::integral_constant<int, 3>{}); break;
-                case 2: run(std::integral_constant<int, 2>{}); break;
-                case 1: run(std::integral_constant<int, 1>{}); break;
-                default: break;
+            // software pipeline over the element rows (rows past the part's last node row are computed
+            // and not stored): the reads of row i + 1 and the sums of row i - 1 around the products
+            // of row i
+            bf16x8 B[2][XP];
+            floatx4 C[2][2];
+            float sc[3];
+            fetch(0, B[0], sc[0]);
+#pragma unroll
+            for (int i = 0; i <= R; i++) {
+                if (i < R) fetch(i + 1, B[(i + 1) & 1], sc[(i + 1) % 3]);
+                product(B[i & 1], C[i & 1]);
+                if (i >= 1) gather(i - 1, C[(i - 1) & 1], sc[(i - 1) % 3]);
             }
+            gather(R, C[R & 1], sc[R % 3]);
         }
-        Own nxt;
-        fetchOwn(k + 1, nxt);
-        __syncthreads();
-
-        // 2. node k: the upper corners of layer k - 1 (carry) and the lower corners of layer k;
-        //    node k + 1 starts with the upper corners of layer k
-        if (mine) {
-            const int sl = (k + 3) % 3;
-            float lower[3] = {0.f, 0.f, 0.f}, upper[3] = {0.f, 0.f, 0.f};
+        // node plane k is complete: the upper corners of layer k - 1 and the lower ones of layer k
+        if (k >= z0 && owner) {
+            const int64_t pb = 3 * NXY * k;
+            const auto ry = planeRsrc(y + pb, planeBytes);
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const int dx = q & 1, dy = q >> 1;
-                const int kl = (1 - dx) + 2 * (1 - dy);  // corner x + 2 y + 4 z of the node in element q
-                const float s = sh.s[sl][sIdx[q]];
-                const float4 lo = sh.e[kl][eIdx[q]], up = sh.e[kl + 4][eIdx[q]];
-                lower[0] += s * lo.x;
-                lower[1] += s * lo.y;
-                lower[2] += s * lo.z;
-                upper[0] += s * up.x;
-                upper[1] += s * up.y;
-                upper[2] += s * up.z;
-            }
-            if (k >= z0) {
-                float out[3];
+            for (int j0 = 0; j0 < R; j0 += EPI) {
+                // one pass of EPI rows: its operands, then its results (the scheduling barrier keeps the
+                // next pass's loads after this pass's stores, which bounds the registers)
+                __builtin_amdgcn_sched_barrier(0);
+                Epi ep;
+                fetchEpi(k, j0, ep);
 #pragma unroll
-                for (int c = 0; c < 3; c++) out[c] = carry[c] + lower[c];
-                const int64_t pb3 = 3 * NXY * k;
-                const int32_t c3 = 3 * col;
-                if (L.diagAdd) {
-#pragma unroll
-                    for (int d = 0; d < 3; d++) out[d] += float((L.diagAdd + pb3)[c3 + d]) * cur.x[d];
-                }
-                float inv[3] = {0.f, 0.f, 0.f};
-                if constexpr (NEEDS_INV) {
-                    // inverse diagonal from the voxel scales: diag = Kb_00 sum_v s_v (+ the shift), in
-                    // kFine's voxel order
-                    const int sm = (k + 2) % 3;
-                    float ssum = 0.f;
-#pragma unroll
-                    for (int v = 0; v < 8; v++) ssum += sh.s[(v >> 2) ? sl : sm][(tx + (v & 1)) + EX * (ty + ((v >> 1) & 1))];
-                    const float k00 = L.sym32[0];
-                    if (L.diagAdd) {
-#pragma unroll
-                        for (int d = 0; d < 3; d++) {
-                            const float dg = k00 * ssum + float((L.diagAdd + pb3)[c3 + d]);
-                            inv[d] = ((cur.info >> (8 + d)) & 1u) ? 0.f : 1.f / dg;
-                        }
-                    } else {
-                        const float iv = 1.f / (k00 * ssum);
-#pragma unroll
-                        for (int d = 0; d < 3; d++) inv[d] = ((cur.info >> (8 + d)) & 1u) ? 0.f : iv;
+                for (int jj = 0; jj < EPI; jj++) {
+                    const int j = j0 + jj;
+                    if (j >= rows) break;
+                    float out = carry[j] + lo[j];
+                    const int32_t fo = eoff + j * rowF;
+                    // the optional diagonal shift of this DOF (fo / 4: the DOF within the plane)
+                    const float dA = L.diagAdd ? float(L.diagAdd[pb + (fo >> 2)]) : 0.f;
+                    if (L.diagAdd) out += dA * ep.x[jj];
+                    const bool held = (ep.info[jj] >> (8 + g)) & 1u;
+                    float inv = 0.f;
+                    if constexpr (NEEDS_INV) {
+                        // inverse diagonal from the voxel scales: diag = Kb_00 sum_v s_v (+ the shift)
+                        const float ssum = syPrev[j] + sy[j];
+                        inv = held ? 0.f : 1.f / (L.diagAdd ? k00 * ssum + dA : k00 * ssum);
                     }
-                }
-                float yv[3];
-#pragma unroll
-                for (int d = 0; d < 3; d++) {
-                    const bool held = (cur.info >> (8 + d)) & 1u;
-                    if constexpr (MODE == AM_APPLY) yv[d] = held ? 0.f : out[d];
-                    else if constexpr (MODE == AM_RAW) yv[d] = out[d];
-                    else if constexpr (MODE == AM_RESID) yv[d] = held ? 0.f : cur.r[d] - out[d];
-                    else if constexpr (MODE == AM_CHEB1) yv[d] = cur.x[d] + c1 * inv[d] * (cur.r[d] - out[d]);
-                    else yv[d] = cur.x[d] + c1 * (cur.x[d] - cur.d[d]) + c2 * inv[d] * (cur.r[d] - out[d]);
+                    float yv;
+                    if constexpr (MODE == AM_APPLY) yv = held ? 0.f : out;
+                    else if constexpr (MODE == AM_RAW) yv = out;
+                    else if constexpr (MODE == AM_RESID) yv = held ? 0.f : ep.r[jj] - out;
+                    else if constexpr (MODE == AM_CHEB1) yv = ep.x[jj] + c1 * inv * (ep.r[jj] - out);
+                    else yv = ep.x[jj] + c1 * (ep.x[jj] - ep.d[jj]) + c2 * inv * (ep.r[jj] - out);
                     if constexpr (DOT) {
-                        if constexpr (MODE == AM_APPLY) dot0 += double(cur.x[d]) * double(yv[d]);
+                        if constexpr (MODE == AM_APPLY) dot0 += double(ep.x[jj]) * double(yv);
                         if constexpr (NEEDS_INV) {
-                            dot0 += double(cur.r[d]) * double(yv[d]);
-                            dot1 += double(yv[d]) * double(cur.qd[d]);
+                            dot0 += double(ep.r[jj]) * double(yv);
+                            dot1 += double(yv) * double(ep.qd[jj]);
                         }
                     }
+                    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(yv), ry, fo, 0, 0);
                 }
-                bstore3(planeRsrc(y + pb3, planeBytes), noff, yv);
             }
-#pragma unroll
-            for (int c = 0; c < 3; c++) carry[c] = upper[c];
         }
-
-        // 3. staging: node plane k + 2 into the slot of plane k (read only by this step's products,
-        //    before its first barrier) and voxel layer k + 1 into that of layer k - 2, then the next
-        //    fetches
+#pragma unroll
+        for (int j = 0; j < R; j++) {
+            carry[j] = up[j];
+            syPrev[j] = sy[j];
+        }
+        // staging: node plane k + 2 into the slot of plane k - 1 and voxel layer k + 1 into that of
+        // layer k - 1 (both read before the last barrier), then the next fetches
         storePlane(k + 2, pn);
         storeVox(k + 1, vn);
         fetchPlane(k + 3, pn);
         fetchVox(k + 2, vn);
-        cur = nxt;
         __syncthreads();
     }
     if constexpr (DOT) {
@@ -419,19 +398,32 @@ template <int MODE, bool DOT, int XP, int AP> __device__ __forceinline__ void el
 
 #define PSX_EL_FWD L, x, y, r, zp, c1, c2, stop, zChunk, gx, gy, nTiles, dots, dotStride, qdot
 template <int MODE, bool DOT, int XP, int AP> struct FineEl {
-    static constexpr int NTH = NT, WPE = 0;
+    static constexpr int NTH = NT, WPE = 3;
     __device__ static void run(VBlk vb, PSX_EL_PARAMS) { elBody<MODE, DOT, XP, AP>(vb, PSX_EL_FWD); }
 };
-template <int MODE, bool DOT, int XP, int AP> __global__ __launch_bounds__(NT) void kFineEl(PSX_EL_PARAMS) {
+// held to 168 registers: 3 wavefronts per SIMD
+template <int MODE, bool DOT, int XP, int AP> __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void kFineEl(PSX_EL_PARAMS) {
     elBody<MODE, DOT, XP, AP>(ownBlock(), PSX_EL_FWD);
 }
 
 }  // namespace
 
-// pieces: 2 = BF16 hi + lo of x - c and of Kb (the V-cycle's smoothing), 3 = three of each
-void launchFineEl(const DevLevel& L, int mode, const float* x, float* y, const float* r, const float* zp, double c1, double c2, const int* stop, int zChunk,
-                  double* dots, int64_t dotStride, const float* qdot, hipStream_t s, int ninst, int pieces) {
-    const int gx = (L.NX + TX - 1) / TX, gy = (L.NY + TY - 1) / TY, gz = (L.NZ + zChunk - 1) / zChunk;
+// pieces: 2 = BF16 hi + lo of x - c and of Kb (the V-cycle's smoothing), 3 = three of each (y = K x only)
+unsigned launchFineEl(const DevLevel& L, int mode, const float* x, float* y, const float* r, const float* zp, double c1, double c2, const int* stop,
+                      double* dots, int64_t dotStride, const float* qdot, hipStream_t s, int ninst, int pieces) {
+    const int gx = (L.NX + BX - 1) / BX, gy = (L.NY + R - 1) / R;
+    // the longest z-chunk (16, 8 or 4 planes) that still gives >= 2048 blocks (8 per CU), 2 planes for
+    // small levels; it ignores the instance count, so that an instance's tiles (and its fused dot
+    // partials) are those of a lone launch
+    int zChunk = 4;
+    for (int c : {16, 8})
+        if (int64_t(gx) * gy * ((L.NZ + c - 1) / c) >= 2048) {
+            zChunk = c;
+            break;
+        }
+    if (zChunk == 4 && int64_t(gx) * gy * ((L.NZ + 3) / 4) < 1024) zChunk = 2;
+    if (const char* e = std::getenv("PSX_EL_ZCHUNK")) zChunk = std::max(2, std::atoi(e));
+    const int gz = (L.NZ + zChunk - 1) / zChunk;
     const int nTiles = gx * gy * gz;
     const dim3 g(unsigned(8 * ((nTiles + 7) / 8)), unsigned(ninst));
     const bool dot = dots && (mode == AM_APPLY || mode == AM_CHEB1 || mode == AM_CHEB2);
@@ -441,9 +433,15 @@ void launchFineEl(const DevLevel& L, int mode, const float* x, float* y, const f
         constexpr bool DT = decltype(dt)::value;
         go<FineEl<M, DT, P, P>>(kFineEl<M, DT, P, P>, g, s, L, x, y, r, zp, f1, f2, stop, zChunk, gx, gy, nTiles, dots, dotStride, qdot);
     };
+    // three pieces for the outer operator (y = K x); the smoothing steps take two
     auto byPieces = [&](auto m, auto dt) {
-        if (pieces >= 3) form(m, dt, std::integral_constant<int, 3>{});
-        else form(m, dt, std::integral_constant<int, 2>{});
+        if constexpr (decltype(m)::value == AM_APPLY) {
+            if (pieces >= 3) {
+                form(m, dt, std::integral_constant<int, 3>{});
+                return;
+            }
+        }
+        form(m, dt, std::integral_constant<int, 2>{});
     };
     using yes = std::true_type;
     using no = std::false_type;
@@ -463,6 +461,7 @@ void launchFineEl(const DevLevel& L, int mode, const float* x, float* y, const f
             else byPieces(std::integral_constant<int, AM_CHEB2>{}, no{});
             break;
     }
+    return dot ? unsigned(nTiles) : 0u;
 }
 
 }  // namespace psx

@@ -272,10 +272,11 @@ public:
         D.elA = reinterpret_cast<const uint4*>(upload(A.data(), int64_t(A.size())));
     }
     // Kb as the A operands of the element-by-element matrix-core product (fine_el.hip): per half h
-    // (output corners 4 h .. 4 h + 3, corners numbered x + 2 y + 4 z), piece p and lane l, 8 BF16:
-    // row l % 16 = 4 (corner - 4 h) + component, K slice 8 (l / 16) .. + 7 = input corners
-    // 2 (l / 16), 2 (l / 16) + 1 with components 0..3 (the fourth zero). The pieces split each entry
-    // v in turn, hi = bf16(v), mid = bf16(v - hi), lo = bf16(v - hi - mid), rounded to nearest.
+    // (output corners at z offset h), piece p and lane l, 8 BF16: row l % 16 = 4 component + corner
+    // (x + 2 y of the output corner; component 3 zero), K slice 8 (l / 16) .. + 7 = input corners
+    // (0, g & 1, g >> 1) and (1, g & 1, g >> 1), g = l / 16, with components 0..3 (the fourth zero).
+    // The pieces split each entry v in turn, hi = bf16(v), mid = bf16(v - hi), lo = bf16(v - hi - mid),
+    // rounded to nearest.
     static std::vector<uint32_t> elementOperands(const double* Kb) {
         auto bf = [](double v) -> uint32_t {
             const float f = float(v);
@@ -289,14 +290,14 @@ public:
             std::memcpy(&f, &u, 4);
             return double(f);
         };
-        auto corner = [](int k) { return ((k >> 1) & 1 ? ((k & 1) ? 2 : 3) : ((k & 1) ? 1 : 0)) + 4 * (k >> 2); };
+        auto corner = [](int x, int y, int z) { return (y ? (x ? 2 : 3) : (x ? 1 : 0)) + 4 * z; };
         std::vector<uint32_t> out(size_t(2) * 3 * 64 * 4);
         for (int h = 0; h < 2; h++)
             for (int l = 0; l < 64; l++) {
-                const int m = l & 15, kg = l >> 4, d = m & 3, ao = corner(4 * h + (m >> 2));
+                const int m = l & 15, kg = l >> 4, d = m >> 2, r = m & 3, ao = corner(r & 1, r >> 1, h);
                 uint32_t pc[3][8];
                 for (int t = 0; t < 8; t++) {
-                    const int c = t & 3, ai = corner(2 * kg + (t >> 2));
+                    const int c = t & 3, ai = corner(t >> 2, kg & 1, kg >> 1);
                     double v = (d < 3 && c < 3) ? Kb[(3 * ao + d) * 24 + 3 * ai + c] : 0.0;
                     for (int p = 0; p < 3; p++) {
                         pc[p][t] = bf(v);
