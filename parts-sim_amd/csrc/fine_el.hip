@@ -48,12 +48,11 @@ constexpr int PX = BX + 2, PY = R + 2, NP = PX * PY;     // node plane with halo
 constexpr int NQ = (NP + NT - 1) / NT;                   // nodes staged per thread
 constexpr int VX = BX + 1, VY = R + 1, NV = VX * VY;     // voxel layer of the block (61 x 9)
 constexpr int NVQ = (NV + NT - 1) / NT;                  // voxels staged per thread
-constexpr int EPI = 4;                                   // node rows per epilogue pass
 
 template <int XP> struct ElShared {
-    uint2 xq[3][XP][NP];  // node planes k .. k + 2 (ring): the pieces of x - c, 4 BF16 per node
-    float xf[3][R * BX * 3];  // the same planes' tile nodes (no halo) in FP32: the epilogue's own x
-    float s[2][NV];       // voxel scales of element layers k, k + 1 (0: empty or outside)
+    uint2 xq[3][XP][NP];      // node planes k .. k + 2 (ring): the pieces of x - c, 4 BF16 per node
+    float s[2][NV];           // voxel scales of element layers k, k + 1 (0: empty or outside)
+    float4 ey[NW][R / 2 * SX];  // per wavefront: half its nodes' sums of plane k (3 components) and voxel-scale sum
 };
 
 __device__ __forceinline__ uint32_t bfBits(float v) { return uint32_t(__builtin_bit_cast(uint16_t, (__bf16)v)); }
@@ -90,13 +89,11 @@ template <int MODE, bool DOT, int XP, int AP> __device__ __forceinline__ void el
     constexpr bool NEEDS_D = MODE == AM_CHEB2;
     constexpr bool NEEDS_INV = MODE == AM_CHEB1 || MODE == AM_CHEB2;
     // this wavefront's strip: nodes Xw .. Xw + 14; lane n's element is voxel Xw - 1 + n, its node
-    // (after the row shift) Xw - 1 + n, so lanes 1..15 own the strip's nodes; lane group g is the
-    // component c in the products and the epilogue (g = 3 holds none)
+    // (after the row shift) Xw - 1 + n, so lanes 1..15 hold the strip's node sums; lane group g is the
+    // component in the products (g = 3 holds none)
     const int Xw = X0 + SX * wid;
     const int rows = min(R, L.NY - Y0);
     const bool waveActive = Xw < L.NX;
-    const int Xn = Xw - 1 + n;
-    const bool owner = n >= 1 && Xn < L.NX && g < 3;
 
     // the block's reference value c: the node at the tile's centre (uniform loads)
     float cref[3];
@@ -142,11 +139,6 @@ template <int MODE, bool DOT, int XP, int AP> __device__ __forceinline__ void el
         for (int q = 0; q < NQ; q++) {
             const int e = tid + q * NT;
             if (e >= NP) continue;
-            const int lx = e % PX - 1, ly = e / PX - 1;
-            if (lx >= 0 && lx < BX && ly >= 0 && ly < R) {
-#pragma unroll
-                for (int c = 0; c < 3; c++) sh.xf[sl][(ly * BX + lx) * 3 + c] = px[q][c];
-            }
             float v[3];
 #pragma unroll
             for (int c = 0; c < 3; c++) v[c] = px[q][c] - cref[c];
@@ -189,34 +181,19 @@ template <int MODE, bool DOT, int XP, int AP> __device__ __forceinline__ void el
             if (e < NV) sh.s[(k + 2) & 1][e] = pv[q];
         }
     };
-    // the epilogue's operands of node (Xn, Y0 + j, z), component g, fetched before the layer's products
-    const int32_t eoff = owner ? int32_t(4 * (3 * (Xn + NX * Y0) + g)) : OUT_OF_RANGE;  // + 12 NX j
-    const int32_t ioff = owner ? int32_t(2 * (Xn + NX * Y0)) : OUT_OF_RANGE;           // + 2 NX j
-    const int32_t rowF = int32_t(12 * NX), rowI = int32_t(2 * NX);
-    struct Epi {
-        float x[EPI], r[EPI], d[EPI], qd[EPI];
-        uint32_t info[EPI];
-    };
-    auto fetchEpi = [&](int z, int j0, Epi& o) {
-        const bool zin = z >= z0 && z < z1;
-        const int64_t pb = 3 * NXY * (zin ? z : z0);
-        const uint32_t vbytes = zin ? planeBytes : 0u;
-        const auto rr = planeRsrc(r ? r + pb : x, r ? vbytes : 0u);
-        const auto rd = planeRsrc(zp ? zp + pb : x, zp ? vbytes : 0u);
-        const auto rq = planeRsrc(qdot ? qdot + pb : x, qdot ? vbytes : 0u);
-        const auto ri = planeRsrc(L.nodeInfo + NXY * (zin ? z : z0), zin ? uint32_t(2 * NXY) : 0u);
+    // the epilogue, one node per lane, in two passes of R / 2 rows: node q of a pass (local
+    // (q % SX, q / SX) in the pass's rows), q = lane < R / 2 * SX; its offsets within a plane
+    // (OUT_OF_RANGE past the part's edge)
+    constexpr int HR = R / 2;
+    static_assert(HR * SX <= 64, "one node per lane and pass");
+    int32_t noff[2], ioff[2];
 #pragma unroll
-        for (int jj = 0; jj < EPI; jj++) {
-            const int j = j0 + jj;
-            const bool jin = j < rows && owner;
-            const int32_t fo = jin ? eoff + j * rowF : OUT_OF_RANGE, io = jin ? ioff + j * rowI : OUT_OF_RANGE;
-            o.x[jj] = jin ? sh.xf[(z + 3) % 3][(j * BX + SX * wid + n - 1) * 3 + g] : 0.f;
-            if constexpr (NEEDS_R) o.r[jj] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rr, fo, 0, 0));
-            if constexpr (NEEDS_D) o.d[jj] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rd, fo, 0, 0));
-            if constexpr (NEEDS_INV && DOT) o.qd[jj] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rq, fo, 0, 0));
-            o.info[jj] = __builtin_amdgcn_raw_buffer_load_b16(ri, io, 0, 0);
-        }
-    };
+    for (int u = 0; u < 2; u++) {
+        const int xq = Xw + lane % SX, yq = Y0 + HR * u + lane / SX;
+        const bool in = lane < HR * SX && xq < L.NX && yq < L.NY;
+        noff[u] = in ? int32_t(12 * (xq + NX * yq)) : OUT_OF_RANGE;
+        ioff[u] = in ? int32_t(2 * (xq + NX * yq)) : OUT_OF_RANGE;
+    }
     // the lane's operand reads: node pair (local x SX wid + n, + 1) of local row i + (g & 1) in node
     // plane k + (g >> 1)
     const int bBase = (g & 1) * PX + SX * wid + n, zOff = g >> 1;
@@ -313,48 +290,69 @@ template <int MODE, bool DOT, int XP, int AP> __device__ __forceinline__ void el
             }
             gather(R, C[R & 1], sc[R % 3]);
         }
-        // node plane k is complete: the upper corners of layer k - 1 and the lower ones of layer k
-        if (k >= z0 && owner) {
+        // node plane k is complete: the upper corners of layer k - 1 and the lower ones of layer k.
+        // The (element, component) lanes hand the sums to one lane per node through the wavefront's
+        // LDS rows (lane group 3, which holds no component, writes the voxel-scale sum)
+        if (k >= z0 && waveActive) {
+            float4* __restrict__ wy = sh.ey[wid];
             const int64_t pb = 3 * NXY * k;
-            const auto ry = planeRsrc(y + pb, planeBytes);
+            const auto rx = planeRsrc(x + pb, planeBytes), ry = planeRsrc(y + pb, planeBytes);
+            const auto rr = planeRsrc(r ? r + pb : x, r ? planeBytes : 0u);
+            const auto rd = planeRsrc(zp ? zp + pb : x, zp ? planeBytes : 0u);
+            const auto rq = planeRsrc(qdot ? qdot + pb : x, qdot ? planeBytes : 0u);
+            const auto ri = planeRsrc(L.nodeInfo + NXY * k, uint32_t(2 * NXY));
 #pragma unroll
-            for (int j0 = 0; j0 < R; j0 += EPI) {
-                // one pass of EPI rows: its operands, then its results (the scheduling barrier keeps the
-                // next pass's loads after this pass's stores, which bounds the registers)
-                __builtin_amdgcn_sched_barrier(0);
-                Epi ep;
-                fetchEpi(k, j0, ep);
+            for (int u = 0; u < 2; u++) {
+                if (n >= 1) {
 #pragma unroll
-                for (int jj = 0; jj < EPI; jj++) {
-                    const int j = j0 + jj;
-                    if (j >= rows) break;
-                    float out = carry[j] + lo[j];
-                    const int32_t fo = eoff + j * rowF;
-                    // the optional diagonal shift of this DOF (fo / 4: the DOF within the plane)
-                    const float dA = L.diagAdd ? float(L.diagAdd[pb + (fo >> 2)]) : 0.f;
-                    if (L.diagAdd) out += dA * ep.x[jj];
-                    const bool held = (ep.info[jj] >> (8 + g)) & 1u;
-                    float inv = 0.f;
-                    if constexpr (NEEDS_INV) {
-                        // inverse diagonal from the voxel scales: diag = Kb_00 sum_v s_v (+ the shift)
-                        const float ssum = syPrev[j] + sy[j];
-                        inv = held ? 0.f : 1.f / (L.diagAdd ? k00 * ssum + dA : k00 * ssum);
+                    for (int jj = 0; jj < HR; jj++) {
+                        const int j = HR * u + jj;
+                        const float v = g < 3 ? carry[j] + lo[j] : (NEEDS_INV ? syPrev[j] + sy[j] : 0.f);
+                        reinterpret_cast<float*>(wy)[4 * (jj * SX + n - 1) + g] = v;
                     }
-                    float yv;
-                    if constexpr (MODE == AM_APPLY) yv = held ? 0.f : out;
-                    else if constexpr (MODE == AM_RAW) yv = out;
-                    else if constexpr (MODE == AM_RESID) yv = held ? 0.f : ep.r[jj] - out;
-                    else if constexpr (MODE == AM_CHEB1) yv = ep.x[jj] + c1 * inv * (ep.r[jj] - out);
-                    else yv = ep.x[jj] + c1 * (ep.x[jj] - ep.d[jj]) + c2 * inv * (ep.r[jj] - out);
-                    if constexpr (DOT) {
-                        if constexpr (MODE == AM_APPLY) dot0 += double(ep.x[jj]) * double(yv);
+                }
+                __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): this wavefront's LDS writes done
+                __builtin_amdgcn_wave_barrier();
+                if (noff[u] != OUT_OF_RANGE) {
+                    const float4 sum = wy[lane];
+                    float xv[3], rv[3] = {0.f, 0.f, 0.f}, dv[3] = {0.f, 0.f, 0.f}, qv[3] = {0.f, 0.f, 0.f};
+                    bload3(rx, noff[u], xv);
+                    if constexpr (NEEDS_R) bload3(rr, noff[u], rv);
+                    if constexpr (NEEDS_D) bload3(rd, noff[u], dv);
+                    if constexpr (NEEDS_INV && DOT) bload3(rq, noff[u], qv);
+                    const uint32_t info = __builtin_amdgcn_raw_buffer_load_b16(ri, ioff[u], 0, 0);
+                    const float out0[3] = {sum.x, sum.y, sum.z};
+                    float yv[3];
+#pragma unroll
+                    for (int d = 0; d < 3; d++) {
+                        float out = out0[d];
+                        // the optional diagonal shift of this DOF (noff / 4: the node's first DOF in the plane)
+                        const float dA = L.diagAdd ? float(L.diagAdd[pb + (noff[u] >> 2) + d]) : 0.f;
+                        if (L.diagAdd) out += dA * xv[d];
+                        const bool held = (info >> (8 + d)) & 1u;
+                        float inv = 0.f;
                         if constexpr (NEEDS_INV) {
-                            dot0 += double(ep.r[jj]) * double(yv);
-                            dot1 += double(yv) * double(ep.qd[jj]);
+                            // inverse diagonal from the voxel scales: diag = Kb_00 sum_v s_v (+ the shift)
+                            inv = held ? 0.f : 1.f / (L.diagAdd ? k00 * sum.w + dA : k00 * sum.w);
+                        }
+                        if constexpr (MODE == AM_APPLY) yv[d] = held ? 0.f : out;
+                        else if constexpr (MODE == AM_RAW) yv[d] = out;
+                        else if constexpr (MODE == AM_RESID) yv[d] = held ? 0.f : rv[d] - out;
+                        else if constexpr (MODE == AM_CHEB1) yv[d] = xv[d] + c1 * inv * (rv[d] - out);
+                        else yv[d] = xv[d] + c1 * (xv[d] - dv[d]) + c2 * inv * (rv[d] - out);
+                        if constexpr (DOT) {
+                            if constexpr (MODE == AM_APPLY) dot0 += double(xv[d]) * double(yv[d]);
+                            if constexpr (NEEDS_INV) {
+                                dot0 += double(rv[d]) * double(yv[d]);
+                                dot1 += double(yv[d]) * double(qv[d]);
+                            }
                         }
                     }
-                    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(yv), ry, fo, 0, 0);
+                    bstore3(ry, noff[u], yv);
                 }
+                // the next pass rewrites the rows: every lane's reads first
+                __builtin_amdgcn_s_waitcnt(0xc07f);
+                __builtin_amdgcn_wave_barrier();
             }
         }
 #pragma unroll
