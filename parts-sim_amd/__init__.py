@@ -787,7 +787,7 @@ class Server:
         self._keep = {}
 
     def submit(self, dims, density, nu, bc, f, x0=None, part_id=-1, coarsest_max_dof=1100, tol=1e-6, max_iter=500, precision="fp64",
-               cheb_degree=0, diag_add=None, device_build=True):
+               cheb_degree=0, diag_add=None, device_build=True, mfma_smoother=0):
         """Queues one part with its load cases f (nrhs x nDof or nDof); returns a ticket.
         device_build: the request's hierarchy is built by the device kernels (False: on the host)."""
         keep = []
@@ -800,7 +800,7 @@ class Server:
         x0a = None if x0 is None else np.ascontiguousarray(x0, np.float64)
         if x0a is not None and x0a.size not in (n, nrhs * n):
             raise ValueError("Force or displacement vector has the wrong size.")
-        o, _ = Model._opts(tol, max_iter, precision, 0, 0, None, cheb_degree)
+        o, _ = Model._opts(tol, max_iter, precision, 0, 0, None, cheb_degree, mfma_smoother)
         req = Request(d, nrhs, f.ctypes.data, None if x0a is None else x0a.ctypes.data, int(x0a is not None and x0a.size == nrhs * n),
                       int(part_id), o)
         t = C.c_int64()

@@ -472,10 +472,14 @@ int psx_profile(psx_model m, int reps, char* buf, size_t len) {
             const psx::Level& L = M.H.levels[l];
             const double t64 = timeIt([&] { M.op<double>(l, psx::AM_APPLY, V.a64, V.b64, nullptr, nullptr, 0, 0, nullptr); });
             const double t32 = timeIt([&] { M.op<float>(l, psx::AM_APPLY, V.a32, V.b32, nullptr, nullptr, 0, 0, nullptr); });
-            double tmf = 0, tmf3 = 0, c64 = 0, c32 = 0;
+            double tmf = 0, tmf3 = 0, s32 = 0, smf = 0, c64 = 0, c32 = 0;
             if (L.nOwn == 0) {
                 tmf = timeIt([&] { M.op<float>(l, psx::AM_APPLY, V.a32, V.b32, nullptr, nullptr, 0, 0, nullptr, psx::OF_MFMA); });
                 tmf3 = timeIt([&] { M.op<float>(l, psx::AM_APPLY, V.a32, V.b32, nullptr, nullptr, 0, 0, nullptr, psx::OF_EL3); });
+                // a later Chebyshev step y = x + c1 (x - d) + c2 D^-1 (r - K x), the V-cycle's most frequent
+                // launch, in the FP32 VALU form and as the element product on the matrix cores
+                s32 = timeIt([&] { M.op<float>(l, psx::AM_CHEB2, V.a32, V.b32, V.r32, V.d32, 0.5, 0.5, nullptr); });
+                smf = timeIt([&] { M.op<float>(l, psx::AM_CHEB2, V.a32, V.b32, V.r32, V.d32, 0.5, 0.5, nullptr, psx::OF_MFMA); });
             }
             if (V.dl.childForm) {
                 c64 = timeIt([&] { M.op<double>(l, psx::AM_APPLY, V.a64, V.b64, nullptr, nullptr, 0, 0, nullptr, psx::OF_CHILD); });
@@ -483,8 +487,8 @@ int psx_profile(psx_model m, int reps, char* buf, size_t len) {
             }
             std::snprintf(line, sizeof line,
                           "L%zu nodes=%lld elems=%zu own matrices=%lld: matvec64 %.4f ms  matvec32 %.4f ms  matvec-mfma2 %.4f ms  matvec-mfma3 %.4f ms  "
-                          "child-form64 %.4f ms  child-form32 %.4f ms\n",
-                          l, (long long)L.nNodes, size_t(L.nElem), (long long)L.nOwn, t64, t32, tmf, tmf3, c64, c32);
+                          "smooth32 %.4f ms  smooth-mfma2 %.4f ms  child-form64 %.4f ms  child-form32 %.4f ms\n",
+                          l, (long long)L.nNodes, size_t(L.nElem), (long long)L.nOwn, t64, t32, tmf, tmf3, s32, smf, c64, c32);
             out += line;
         }
         const double tv64 = timeIt([&] { M.vcycle<double>(0, M.lv[0].r64, nullptr, psx::OF_AUTO); });
