@@ -451,7 +451,8 @@ def main():
         d_f, d_u = P.DeviceArray(n_dof, w["f"]), P.DeviceArray(n_dof)
         for k in range(args.steps):
             t = time.perf_counter()
-            info = m.solve_device(d_f, d_u, tol=args.tol, max_iter=2000, precision=args.profile_solves, check_every=8)
+            info = m.solve_device(d_f, d_u, tol=args.tol, max_iter=2000, precision=args.profile_solves, check_every=8, cheb_degree=args.degree,
+                                  mfma_smoother=1 if args.profile_solves == "mixed" and args.smoother == "mfma" else 0)
             log("%s solve %d: %d iterations, %.2f ms" % (args.profile_solves, k, info["iterations"], 1e3 * (time.perf_counter() - t)))
         return
     # hierarchy build + smoother estimates: on the host CPU (first, so that it also carries the
