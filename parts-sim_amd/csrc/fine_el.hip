@@ -92,7 +92,6 @@ template <int MODE, bool DOT, int XP, int AP> __device__ __forceinline__ void el
     // (after the row shift) Xw - 1 + n, so lanes 1..15 hold the strip's node sums; lane group g is the
     // component in the products (g = 3 holds none)
     const int Xw = X0 + SX * wid;
-    const int rows = min(R, L.NY - Y0);
     const bool waveActive = Xw < L.NX;
 
     // the block's reference value c: the node at the tile's centre (uniform loads)
@@ -315,8 +314,9 @@ template <int MODE, bool DOT, int XP, int AP> __device__ __forceinline__ void el
                 __builtin_amdgcn_wave_barrier();
                 if (noff[u] != OUT_OF_RANGE) {
                     const float4 sum = wy[lane];
-                    float xv[3], rv[3] = {0.f, 0.f, 0.f}, dv[3] = {0.f, 0.f, 0.f}, qv[3] = {0.f, 0.f, 0.f};
-                    bload3(rx, noff[u], xv);
+                    float xv[3] = {0.f, 0.f, 0.f}, rv[3] = {0.f, 0.f, 0.f}, dv[3] = {0.f, 0.f, 0.f}, qv[3] = {0.f, 0.f, 0.f};
+                    // x itself only where the step uses it: the Chebyshev steps, the fused p.q, the diagonal shift
+                    if (NEEDS_INV || (MODE == AM_APPLY && DOT) || L.diagAdd) bload3(rx, noff[u], xv);
                     if constexpr (NEEDS_R) bload3(rr, noff[u], rv);
                     if constexpr (NEEDS_D) bload3(rd, noff[u], dv);
                     if constexpr (NEEDS_INV && DOT) bload3(rq, noff[u], qv);
