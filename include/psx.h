@@ -77,7 +77,10 @@ typedef struct {
     int cheb_degree;            /* Chebyshev smoother degree 1..6 (0 = 2, the reference's, voxel_fea.cpp:692-725) */
     int mfma_smoother;          /* PSX_PREC_MIXED, bit flags (0 = FP32 VALU everywhere): 1 = the V-cycle's levels
                                    without own element matrices as the element-by-element product on the
-                                   matrix cores (two BF16 pieces of x - c and of Kb, FP32 accumulation);
+                                   matrix cores (two BF16 pieces of x - c and of Kb, FP32 accumulation),
+                                   and the child-scale level likewise, its own elements' products
+                                   sum_c s_c M_c (x_e - x_e0) over the 8 child matrices in a matrix-core
+                                   pass of their own;
                                    4 = the outer FP32 operator as the same product with three pieces each;
                                    2 = levels with stored own element matrices apply them on the matrix
                                    cores (one v_mfma_f32_16x16x32_bf16 per node and voxel over 16 load
@@ -146,11 +149,13 @@ int psx_apply32(psx_model m, int level, const float* x, float* y, int raw);
  * fp32 selects float x and y. No reference counterpart (the reference always streams its matrices). */
 int psx_apply_child(psx_model m, int level, int fp32, const void* x, void* y, int raw);
 
-/* y = K x with the matrix-core operator of the mixed-precision solve (levels without own element
- * matrices, FP32 vectors): the element-by-element product sum_e s_e Kb (x_e - c), c one node's value per
- * 32 x 8 x zChunk tile, as v_mfma_f32_16x16x32_bf16 products of `pieces` BF16 pieces of x - c and of Kb
- * with FP32 accumulation. pieces = 2 (hi + lo: the V-cycle's smoothing, opts.mfma_smoother bit 0) or 3
- * (the outer operator, bit 2). No reference counterpart: the reference's operator is FP32
+/* y = K x with the matrix-core operator of the mixed-precision solve (FP32 vectors; levels without own
+ * element matrices and the child-scale level): the element-by-element product sum_e s_e Kb (x_e - c), c
+ * one node's value per 60 x 8 x zChunk tile, as v_mfma_f32_16x16x32_bf16 products of `pieces` BF16
+ * pieces of x - c and of Kb with FP32 accumulation. pieces = 2 (hi + lo: the V-cycle's smoothing,
+ * opts.mfma_smoother bit 0) or 3 (the outer operator, bit 2). On the child-scale level the own elements
+ * add sum_c s_c M_c (x_e - x_e0), two-piece products over the 8 child matrices, in either form. Levels
+ * with stored own matrices are rejected. No reference counterpart: the reference's operator is FP32
  * (voxel_fea.cpp:393, fea.wgsl:173-206). */
 int psx_apply_mfma(psx_model m, int level, int pieces, const float* x, float* y);
 

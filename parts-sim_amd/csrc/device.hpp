@@ -49,6 +49,13 @@ struct DevLevel {
     const uint16_t* ctab16;
     const float* ctabRowScale;
     int childForm;
+    // the child-scale level's own elements for the matrix-core operator: the voxel of each own index,
+    // and a work buffer of their corner rows ([instance][own][corner x + 2 y + 4 z][component], FP32;
+    // ownTStride floats per instance), written before each product (null elsewhere)
+    const int32_t* ownVox;
+    const uint4* childA;  // the 8 child products M_c as matrix-core A operands ([c][half][piece][lane])
+    float* ownT;
+    int64_t ownTStride;
     // kFine sums the 8 voxels of every node directly (no stencil, no pair lists): set on levels
     // without own elements where most nodes have differing voxels (per-voxel densities)
     int dense;
@@ -128,6 +135,8 @@ void launchApplyOwnMf(const DevLevel& L, int mode, const float* x, float* y, con
 // smoothing)
 template <class T> void launchFine(const DevLevel& L, int mode, const T* x, T* y, const T* r, const T* zp, double c1, double c2, const int* stop,
                                    hipStream_t s, int ninst = 1, int el = 0, bool half = false);
+// ownVox[own index] = the voxel of each own element (words < 0)
+void launchOwnVox(const int32_t* words, int64_t nVox, int32_t* ownVox, hipStream_t s);
 // the element-by-element matrix-core operator itself (fine_el.hip); dots: fused dot partials, one per
 // tile (the kinds of launchFineDot); returns the number of partials written (0: no fused dot)
 unsigned launchFineEl(const DevLevel& L, int mode, const float* x, float* y, const float* r, const float* zp, double c1, double c2, const int* stop,

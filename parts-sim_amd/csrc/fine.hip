@@ -33,47 +33,6 @@ constexpr int TX = 32, TY = 8, NT = TX * TY;
 constexpr int PX = TX + 2, PY = TY + 2;  // node plane with halo
 constexpr int VX = TX + 1, VY = TY + 1;  // voxel plane
 
-__device__ constexpr int cx(int b) { return (b == 1 || b == 2 || b == 5 || b == 6) ? 1 : 0; }
-__device__ constexpr int cy(int b) { return (b == 2 || b == 3 || b == 6 || b == 7) ? 1 : 0; }
-__device__ constexpr int cz(int b) { return b >= 4 ? 1 : 0; }
-__device__ constexpr int cornerOf(int x, int y, int z) { return (y ? (x ? 2 : 3) : (x ? 1 : 0)) + 4 * z; }
-// Symmetry classes (the host's elementClass, hierarchy.cpp): every element coefficient is
-// +-one of 22 values, at an index known at compile time once the loops are unrolled, so the values
-// stay in scalar registers for the whole kernel.
-__device__ constexpr int elementClass(int a, int d, int b, int c) {
-    const int ca[3] = {cx(a), cy(a), cz(a)};
-    int r[3] = {cx(a) ^ cx(b), cy(a) ^ cy(b), cz(a) ^ cz(b)};
-    const bool neg = ((ca[d] + ca[c]) & 1) != 0;
-    int t = r[0];
-    r[0] = r[d];
-    r[d] = t;
-    int c2 = c == 0 ? d : (c == d ? 0 : c);
-    if (c2 == 2) {
-        t = r[1];
-        r[1] = r[2];
-        r[2] = t;
-        c2 = 1;
-    } else if (c2 == 0 && r[1] < r[2]) {
-        t = r[1];
-        r[1] = r[2];
-        r[2] = t;
-    }
-    const int idx = c2 == 0 ? r[0] * 3 + r[1] + r[2] : 6 + r[0] + 2 * r[1] + 4 * r[2];
-    return idx | (neg ? 0x100 : 0);
-}
-template <class T> struct Sym {
-    T v[22];
-};
-// coefficient times x, class and sign fixed at compile time
-template <int CLS, class T> __device__ __forceinline__ T coefMul(const Sym<T>& C, T x) {
-    if constexpr ((CLS & 0x100) != 0) return -C.v[CLS & 0xff] * x;
-    else return C.v[CLS & 0xff] * x;
-}
-// loop with a compile-time index: f(std::integral_constant<int, i>) for i in [0, N)
-template <class F, int... I> __device__ __forceinline__ void staticForImpl(F&& f, std::integer_sequence<int, I...>) {
-    (f(std::integral_constant<int, I>{}), ...);
-}
-template <int N, class F> __device__ __forceinline__ void staticFor(F&& f) { staticForImpl(f, std::make_integer_sequence<int, N>{}); }
 
 template <class T> struct FineP;
 template <> struct FineP<double> {
@@ -745,7 +704,8 @@ unsigned launchFineDot(const DevLevel& L, int mode, const T* x, T* y, const T* r
     const T* qdot = fused && fd.kind == FD_RZ ? static_cast<const T*>(fd.q) : nullptr;
     if constexpr (sizeof(T) == 4) {
         // the element-by-element product on the matrix cores (fine_el.hip), on its own tiles
-        if (el > 0 && !L.childForm) return launchFineEl(L, mode, x, y, r, d, c1, c2, stop, dots, fd.red.stride, qdot, s, ninst, el);
+        // (the child-scale level with its own elements' corner-row pass)
+        if (el > 0 && (!L.childForm || L.ownVox)) return launchFineEl(L, mode, x, y, r, d, c1, c2, stop, dots, fd.red.stride, qdot, s, ninst, el);
     }
     Sym<T> C;
     for (int q = 0; q < 22; q++) C.v[q] = sizeof(T) == 8 ? T(L.sym64[q]) : T(L.sym32[q]);

@@ -49,9 +49,13 @@ constexpr int NQ = (NP + NT - 1) / NT;                   // nodes staged per thr
 constexpr int VX = BX + 1, VY = R + 1, NV = VX * VY;     // voxel layer of the block (61 x 9)
 constexpr int NVQ = (NV + NT - 1) / NT;                  // voxels staged per thread
 
+// node-plane slots: a ring of 3 (plane k + 2 staged while planes k, k + 1 are read: one barrier per
+// layer), or 2 for three pieces (staged after a second barrier), which keeps the block at 38 KB and 4
+// blocks per compute unit
+template <int XP> constexpr int slotsFor() { return XP >= 3 ? 2 : 3; }
 template <int XP> struct ElShared {
-    uint2 xq[3][XP][NP];      // node planes k .. k + 2 (ring): the pieces of x - c, 4 BF16 per node
-    float s[2][NV];           // voxel scales of element layers k, k + 1 (0: empty or outside)
+    uint2 xq[slotsFor<XP>()][XP][NP];  // node planes: the pieces of x - c, 4 BF16 per node
+    int32_t s[3][NV];         // voxel words of element layers k - 1 .. k + 1 (0: empty or outside)
     float4 ey[NW][R / 2 * SX];  // per wavefront: half its nodes' sums of plane k (3 components) and voxel-scale sum
 };
 
@@ -66,7 +70,9 @@ __device__ __forceinline__ float fromLeft(float v) {
     DevLevel L, const float *__restrict__ x, float *__restrict__ y, const float *__restrict__ r, const float *__restrict__ zp, float c1, float c2, \
         const int *stop, int zChunk, int gx, int gy, int nTiles, double *__restrict__ dots, int64_t dotStride, const float *__restrict__ qdot
 
-template <int MODE, bool DOT, int XP, int AP> __device__ __forceinline__ void elBody(VBlk vb, PSX_EL_PARAMS) {
+// OWNC: the child-scale level; its own elements enter with scale 0 in the products, and their corner
+// rows (the element's own product in the difference form, written by OwnRows) are added to the sums
+template <int MODE, bool DOT, int XP, int AP, bool OWNC> __device__ __forceinline__ void elBody(VBlk vb, PSX_EL_PARAMS) {
     if (stop) stop += int64_t(vb.y) * CG_INTS;
     if (stop && __builtin_nontemporal_load(stop) != 0) return;
     {
@@ -132,8 +138,9 @@ template <int MODE, bool DOT, int XP, int AP> __device__ __forceinline__ void el
 #pragma unroll
         for (int q = 0; q < NQ; q++) bload3(rs, poff[q], px[q]);
     };
+    constexpr int NS = slotsFor<XP>();
     auto storePlane = [&](int z, const float (&px)[NQ][3]) {
-        const int sl = (z + 3) % 3;
+        const int sl = (z + 2 * NS) % NS;
 #pragma unroll
         for (int q = 0; q < NQ; q++) {
             const int e = tid + q * NT;
@@ -164,20 +171,17 @@ template <int MODE, bool DOT, int XP, int AP> __device__ __forceinline__ void el
         const bool in = e < NV && vi >= 0 && vj >= 0 && vi < L.nx && vj < L.ny;
         voff[q] = in ? int32_t(4 * (vi + int64_t(L.nx) * vj)) : OUT_OF_RANGE;
     }
-    auto fetchVox = [&](int k, float (&pv)[NVQ]) {
+    auto fetchVox = [&](int k, int32_t (&pv)[NVQ]) {
         const bool kin = k >= 0 && k < L.nz;
         const auto rs = planeRsrc(L.words + nxy * (kin ? k : 0), kin ? uint32_t(4 * nxy) : 0u);
 #pragma unroll
-        for (int q = 0; q < NVQ; q++) {
-            const int32_t w = int32_t(__builtin_amdgcn_raw_buffer_load_b32(rs, voff[q], 0, 0));
-            pv[q] = w > 0 ? __int_as_float(w) : 0.f;
-        }
+        for (int q = 0; q < NVQ; q++) pv[q] = int32_t(__builtin_amdgcn_raw_buffer_load_b32(rs, voff[q], 0, 0));
     };
-    auto storeVox = [&](int k, const float (&pv)[NVQ]) {
+    auto storeVox = [&](int k, const int32_t (&pv)[NVQ]) {
 #pragma unroll
         for (int q = 0; q < NVQ; q++) {
             const int e = tid + q * NT;
-            if (e < NV) sh.s[(k + 2) & 1][e] = pv[q];
+            if (e < NV) sh.s[(k + 3) % 3][e] = pv[q];
         }
     };
     // the epilogue, one node per lane, in two passes of R / 2 rows: node q of a pass (local
@@ -199,9 +203,11 @@ template <int MODE, bool DOT, int XP, int AP> __device__ __forceinline__ void el
 
     // prologue: node planes z0 - 1, z0 and voxel layer z0 - 1 in LDS; plane z0 + 1 and layer z0 in
     // flight
-    float pn[NQ][3], vn[NVQ];
+    float pn[NQ][3];
+    int32_t vn[NVQ];
     {
-        float pa[NQ][3], pb[NQ][3], va[NVQ];
+        float pa[NQ][3], pb[NQ][3];
+        int32_t va[NVQ];
         fetchPlane(z0 - 1, pa);
         fetchPlane(z0, pb);
         fetchVox(z0 - 1, va);
@@ -224,10 +230,10 @@ template <int MODE, bool DOT, int XP, int AP> __device__ __forceinline__ void el
 #pragma unroll
         for (int j = 0; j < R; j++) lo[j] = up[j] = sy[j] = 0.f;
         if (waveActive) {
-            const uint2* __restrict__ xs = &sh.xq[(k + zOff + 3) % 3][0][0];
-            const float* __restrict__ sv = &sh.s[(k + 2) & 1][SX * wid + n];
+            const uint2* __restrict__ xs = &sh.xq[(k + zOff + 2 * NS) % NS][0][0];
+            const int32_t* __restrict__ sv = &sh.s[(k + 3) % 3][SX * wid + n];
             // element row Y0 - 1 + i: its operands (two node pairs per lane and piece) and scale
-            auto fetch = [&](int i, bf16x8 (&B)[XP], float& sc) {
+            auto fetch = [&](int i, bf16x8 (&B)[XP], int32_t& wd) {
 #pragma unroll
                 for (int p = 0; p < XP; p++) {
                     const uint2 a = xs[p * NP + bBase + i * PX], b = xs[p * NP + bBase + i * PX + 1];
@@ -235,7 +241,7 @@ template <int MODE, bool DOT, int XP, int AP> __device__ __forceinline__ void el
                     const u4 w = {a.x, a.y, b.x, b.y};
                     B[p] = __builtin_bit_cast(bf16x8, w);
                 }
-                sc = sv[i * VX];
+                wd = sv[i * VX];
             };
             // its products: C[h] = Kb (x_e - c) at the corners of z side h, the smaller piece
             // products first
@@ -255,7 +261,10 @@ template <int MODE, bool DOT, int XP, int AP> __device__ __forceinline__ void el
             };
             // its node sums: lane (n, g) holds the element's corners (x, y) = (r & 1, r >> 1) of
             // component g; the x-upper corners belong to the next lane's node
-            auto gather = [&](int i, const floatx4 (&C)[2], float s) {
+            auto gather = [&](int i, const floatx4 (&C)[2], int32_t wd) {
+                // the voxel word: > 0 the scale of s * Kb, < 0 an own element (scale 0 here: its rows
+                // join in the epilogue)
+                const float s = wd > 0 ? __int_as_float(wd) : 0.f;
 #pragma unroll
                 for (int h = 0; h < 2; h++) {
                     float(&acc)[R] = h ? up : lo;
@@ -268,7 +277,7 @@ template <int MODE, bool DOT, int XP, int AP> __device__ __forceinline__ void el
                         acc[i] += fromLeft(s * C[h][3]);
                     }
                 }
-                if constexpr (NEEDS_INV) {
+                if constexpr (NEEDS_INV && !OWNC) {
                     const float sx = s + fromLeft(s);  // the node's two voxels in x of this element row
                     if (i >= 1) sy[i - 1] += sx;
                     if (i < R) sy[i] += sx;
@@ -279,7 +288,7 @@ template <int MODE, bool DOT, int XP, int AP> __device__ __forceinline__ void el
             // of row i
             bf16x8 B[2][XP];
             floatx4 C[2][2];
-            float sc[3];
+            int32_t sc[3];
             fetch(0, B[0], sc[0]);
 #pragma unroll
             for (int i = 0; i <= R; i++) {
@@ -300,13 +309,14 @@ template <int MODE, bool DOT, int XP, int AP> __device__ __forceinline__ void el
             const auto rd = planeRsrc(zp ? zp + pb : x, zp ? planeBytes : 0u);
             const auto rq = planeRsrc(qdot ? qdot + pb : x, qdot ? planeBytes : 0u);
             const auto ri = planeRsrc(L.nodeInfo + NXY * k, uint32_t(2 * NXY));
+            const auto rinv = planeRsrc(OWNC ? L.invD32 + pb : x, OWNC ? planeBytes : 0u);
 #pragma unroll
             for (int u = 0; u < 2; u++) {
                 if (n >= 1) {
 #pragma unroll
                     for (int jj = 0; jj < HR; jj++) {
                         const int j = HR * u + jj;
-                        const float v = g < 3 ? carry[j] + lo[j] : (NEEDS_INV ? syPrev[j] + sy[j] : 0.f);
+                        const float v = g < 3 ? carry[j] + lo[j] : (NEEDS_INV && !OWNC ? syPrev[j] + sy[j] : 0.f);
                         reinterpret_cast<float*>(wy)[4 * (jj * SX + n - 1) + g] = v;
                     }
                 }
@@ -320,8 +330,27 @@ template <int MODE, bool DOT, int XP, int AP> __device__ __forceinline__ void el
                     if constexpr (NEEDS_R) bload3(rr, noff[u], rv);
                     if constexpr (NEEDS_D) bload3(rd, noff[u], dv);
                     if constexpr (NEEDS_INV && DOT) bload3(rq, noff[u], qv);
+                    float iv[3] = {0.f, 0.f, 0.f};  // the stored inverse diagonal (levels with own elements)
+                    if constexpr (NEEDS_INV && OWNC) bload3(rinv, noff[u], iv);
                     const uint32_t info = __builtin_amdgcn_raw_buffer_load_b16(ri, ioff[u], 0, 0);
-                    const float out0[3] = {sum.x, sum.y, sum.z};
+                    float out0[3] = {sum.x, sum.y, sum.z};
+                    if constexpr (OWNC) {
+                        // the rows of the node's own elements (voxels of layers k - 1, k around it, in voxel
+                        // order): the node is corner (1 - dx, 1 - dy, 1 - dz) of voxel (dx, dy, dz)
+                        const int lxn = SX * wid + lane % SX, lyn = HR * u + lane / SX;
+                        const float* __restrict__ ownT = L.ownT + int64_t(vb.y) * L.ownTStride;
+#pragma unroll
+                        for (int vq = 0; vq < 8; vq++) {
+                            const int dx = vq & 1, dy = (vq >> 1) & 1, dz = vq >> 2;
+                            const int32_t w = sh.s[(k - 1 + dz + 3) % 3][(lyn + dy) * VX + lxn + dx];
+                            if (w < 0) {
+                                const float* tr = ownT + 24 * int64_t(w & 0x7fffffff) + 3 * ((1 - dx) + 2 * (1 - dy) + 4 * (1 - dz));
+                                out0[0] += tr[0];
+                                out0[1] += tr[1];
+                                out0[2] += tr[2];
+                            }
+                        }
+                    }
                     float yv[3];
 #pragma unroll
                     for (int d = 0; d < 3; d++) {
@@ -331,7 +360,8 @@ template <int MODE, bool DOT, int XP, int AP> __device__ __forceinline__ void el
                         if (L.diagAdd) out += dA * xv[d];
                         const bool held = (info >> (8 + d)) & 1u;
                         float inv = 0.f;
-                        if constexpr (NEEDS_INV) {
+                        if constexpr (NEEDS_INV && OWNC) inv = iv[d];
+                        else if constexpr (NEEDS_INV) {
                             // inverse diagonal from the voxel scales: diag = Kb_00 sum_v s_v (+ the shift)
                             inv = held ? 0.f : 1.f / (L.diagAdd ? k00 * sum.w + dA : k00 * sum.w);
                         }
@@ -360,8 +390,10 @@ template <int MODE, bool DOT, int XP, int AP> __device__ __forceinline__ void el
             carry[j] = up[j];
             syPrev[j] = sy[j];
         }
-        // staging: node plane k + 2 into the slot of plane k - 1 and voxel layer k + 1 into that of
-        // layer k - 1 (both read before the last barrier), then the next fetches
+        // staging: node plane k + 2 into the slot of plane k - 1 (ring of 3) or of plane k (2 slots:
+        // after every wavefront's reads of this layer) and voxel layer k + 1 into that of layer k - 1
+        // (read before the last barrier), then the next fetches
+        if constexpr (NS == 2) __syncthreads();
         storePlane(k + 2, pn);
         storeVox(k + 1, vn);
         fetchPlane(k + 3, pn);
@@ -395,41 +427,182 @@ template <int MODE, bool DOT, int XP, int AP> __device__ __forceinline__ void el
 }
 
 #define PSX_EL_FWD L, x, y, r, zp, c1, c2, stop, zChunk, gx, gy, nTiles, dots, dotStride, qdot
-template <int MODE, bool DOT, int XP, int AP> struct FineEl {
+template <int MODE, bool DOT, int XP, int AP, bool OWNC> struct FineEl {
     static constexpr int NTH = NT, WPE = 3;
-    __device__ static void run(VBlk vb, PSX_EL_PARAMS) { elBody<MODE, DOT, XP, AP>(vb, PSX_EL_FWD); }
+    __device__ static void run(VBlk vb, PSX_EL_PARAMS) { elBody<MODE, DOT, XP, AP, OWNC>(vb, PSX_EL_FWD); }
 };
 // held to 168 registers: 3 wavefronts per SIMD
-template <int MODE, bool DOT, int XP, int AP> __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void kFineEl(PSX_EL_PARAMS) {
-    elBody<MODE, DOT, XP, AP>(ownBlock(), PSX_EL_FWD);
+template <int MODE, bool DOT, int XP, int AP, bool OWNC>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void kFineEl(PSX_EL_PARAMS) {
+    elBody<MODE, DOT, XP, AP, OWNC>(ownBlock(), PSX_EL_FWD);
+}
+
+// The child-scale level's own elements for the matrix-core operator: each element's product
+// K_e (x_e - x_e0) = sum_c s_c M_c (x_e - x_e0) (M_c the Galerkin product of the finer level's
+// uniform element at child position c, s_c the child's scale, x_e0 the element's corner-0 node: the
+// products annihilate translations), written to L.ownT as [own][corner x + 2 y + 4 z][component].
+// Each wavefront takes OWN_TILES tiles of 16 elements, one element per MFMA column: per child c and
+// half (output corners 0..3, 4..7) the BF16 products of A = M_c's pieces (L.childA) and B = the
+// pieces of the element's corner differences, FP32 accumulation, then y += s_c (A B) per column.
+constexpr int OWN_TILES = 2;
+struct OwnRowsMf {
+    static constexpr int NTH = 256, WPE = 0;
+    __device__ static void run(VBlk vb, DevLevel L, const float* __restrict__ x, const int* stop, int64_t nOwn) {
+        if (stop) stop += int64_t(vb.y) * CG_INTS;
+        if (stop && __builtin_nontemporal_load(stop) != 0) return;
+        x += int64_t(vb.y) * L.nDof;
+        float* __restrict__ T = L.ownT + int64_t(vb.y) * L.ownTStride;
+        const int lane = threadIdx.x & 63, n = lane & 15, g = lane >> 4;
+        // the A operands (32 KB) in LDS, read by every wavefront of the block for each child
+        __shared__ uint4 sA[8 * 2 * 2 * 64];
+        for (int i = threadIdx.x; i < 8 * 2 * 2 * 64; i += NTH) sA[i] = L.childA[i];
+        __syncthreads();
+        const int64_t tile0 = (int64_t(vb.x) * (NTH / 64) + (threadIdx.x >> 6)) * OWN_TILES;
+        if (tile0 * 16 >= nOwn) return;  // the whole wavefront past the last element
+        const int64_t NX = L.NX, NXY = int64_t(L.NX) * L.NY, nx = L.nx, nxy = int64_t(L.nx) * L.ny;
+        // B operands: lane (n, g) holds element n's corners 2 g, 2 g + 1 (x = 0, 1; y = g & 1;
+        // z = g >> 1), components 0..2 and a zero, as differences from corner 0
+        bf16x8 B[OWN_TILES][2];
+        float sc[OWN_TILES][8];
+        int64_t own[OWN_TILES];
+#pragma unroll
+        for (int t = 0; t < OWN_TILES; t++) {
+            own[t] = (tile0 + t) * 16 + n;
+            const bool live = own[t] < nOwn;
+            const int64_t V = live ? L.ownVox[own[t]] : 0;
+            const int64_t m0 = (V % nx) + NX * ((V / nx) % L.ny) + NXY * (V / nxy);
+            const int64_t mg = m0 + NX * (g & 1) + NXY * (g >> 1);
+            float x0[3] = {0.f, 0.f, 0.f}, xa[3] = {0.f, 0.f, 0.f}, xb[3] = {0.f, 0.f, 0.f};
+            if (live) {  // 12-byte node loads
+                __builtin_memcpy(x0, x + 3 * m0, 12);
+                __builtin_memcpy(xa, x + 3 * mg, 12);
+                __builtin_memcpy(xb, x + 3 * (mg + 1), 12);
+            }
+            uint32_t hi[8], lo[8];
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const float v = (q & 3) == 3 ? 0.f : ((q < 4 ? xa[q & 3] : xb[q & 3]) - x0[q & 3]);
+                hi[q] = bfBits(v);
+                lo[q] = bfBits(v - bfValue(hi[q]));
+            }
+            typedef unsigned u4 __attribute__((ext_vector_type(4)));
+            const u4 wh = {hi[0] | (hi[1] << 16), hi[2] | (hi[3] << 16), hi[4] | (hi[5] << 16), hi[6] | (hi[7] << 16)};
+            const u4 wl = {lo[0] | (lo[1] << 16), lo[2] | (lo[3] << 16), lo[4] | (lo[5] << 16), lo[6] | (lo[7] << 16)};
+            B[t][0] = __builtin_bit_cast(bf16x8, wh);
+            B[t][1] = __builtin_bit_cast(bf16x8, wl);
+            const float4* cs = reinterpret_cast<const float4*>(L.childS + 8 * (live ? own[t] : 0));
+            const float4 s0 = cs[0], s1 = cs[1];
+            const float sv[8] = {s0.x, s0.y, s0.z, s0.w, s1.x, s1.y, s1.z, s1.w};
+#pragma unroll
+            for (int c = 0; c < 8; c++) sc[t][c] = live ? sv[c] : 0.f;
+        }
+        floatx4 Y[OWN_TILES][2];
+#pragma unroll
+        for (int t = 0; t < OWN_TILES; t++) Y[t][0] = Y[t][1] = floatx4{0.f, 0.f, 0.f, 0.f};
+        const uint4* __restrict__ CA = sA;
+#pragma unroll 2
+        for (int c = 0; c < 8; c++) {
+            bf16x8 A[2][2];
+#pragma unroll
+            for (int h = 0; h < 2; h++)
+#pragma unroll
+                for (int p = 0; p < 2; p++) {
+                    const uint4 v = CA[((c * 2 + h) * 2 + p) * 64 + lane];
+                    typedef unsigned u4 __attribute__((ext_vector_type(4)));
+                    const u4 w = {v.x, v.y, v.z, v.w};
+                    A[h][p] = __builtin_bit_cast(bf16x8, w);
+                }
+#pragma unroll
+            for (int t = 0; t < OWN_TILES; t++)
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    floatx4 P = {0.f, 0.f, 0.f, 0.f};
+                    P = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[h][1], B[t][0], P, 0, 0, 0);
+                    P = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[h][0], B[t][1], P, 0, 0, 0);
+                    P = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[h][0], B[t][0], P, 0, 0, 0);
+#pragma unroll
+                    for (int r = 0; r < 4; r++) Y[t][h][r] += sc[t][c] * P[r];
+                }
+        }
+        // lane (n, g) holds rows 4 g .. 4 g + 3 of each half: corner 4 h + g, components 0..2
+#pragma unroll
+        for (int t = 0; t < OWN_TILES; t++) {
+            if (own[t] >= nOwn) continue;
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const float v[3] = {Y[t][h][0], Y[t][h][1], Y[t][h][2]};
+                __builtin_memcpy(T + 24 * own[t] + 3 * (4 * h + g), v, 12);
+            }
+        }
+    }
+};
+__global__ __launch_bounds__(256) void kOwnRowsMf(DevLevel L, const float* __restrict__ x, const int* stop, int64_t nOwn) {
+    OwnRowsMf::run(ownBlock(), L, x, stop, nOwn);
+}
+
+__global__ void kOwnVox(const int32_t* __restrict__ words, int64_t nVox, int32_t* __restrict__ ownVox) {
+    const int64_t v = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (v >= nVox) return;
+    const int32_t w = words[v];
+    if (w < 0) ownVox[w & 0x7fffffff] = int32_t(v);
 }
 
 }  // namespace
+
+void launchOwnVox(const int32_t* words, int64_t nVox, int32_t* ownVox, hipStream_t s) {
+    if (nVox > 0) kOwnVox<<<unsigned((nVox + 255) / 256), 256, 0, s>>>(words, nVox, ownVox);
+}
 
 // pieces: 2 = BF16 hi + lo of x - c and of Kb (the V-cycle's smoothing), 3 = three of each (y = K x only)
 unsigned launchFineEl(const DevLevel& L, int mode, const float* x, float* y, const float* r, const float* zp, double c1, double c2, const int* stop,
                       double* dots, int64_t dotStride, const float* qdot, hipStream_t s, int ninst, int pieces) {
     const int gx = (L.NX + BX - 1) / BX, gy = (L.NY + R - 1) / R;
-    // the longest z-chunk (16, 8 or 4 planes) that still gives >= 2048 blocks (8 per CU), 2 planes for
-    // small levels; it ignores the instance count, so that an instance's tiles (and its fused dot
-    // partials) are those of a lone launch
-    int zChunk = 4;
+    // The longest z-chunk (16 or 8 planes) that still gives >= 2048 blocks (8 per CU): large levels are
+    // bound by their traffic, which the halo planes of shorter chunks raise. A smaller level is bound
+    // by each block's chain of plane steps: its chunk (2..8 planes) minimises the rounds of resident
+    // blocks (4 per CU) times the planes a block steps through (its chunk and one halo plane).
+    // Measured on the 512x128x128 block: 8 planes on level 0 (4 and 16: 2-11 % slower), 3 on level 1
+    // (34.0 us against 40.0 at 2, 37.8 at 4, 43.5 at 5). The rule ignores the instance count, so that
+    // an instance's tiles (and its fused dot partials) are those of a lone launch.
+    static const int64_t slots = [] {
+        int dev = 0, cus = 0;
+        if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
+        return int64_t(4) * cus;
+    }();
+    int zChunk = 0;
     for (int c : {16, 8})
         if (int64_t(gx) * gy * ((L.NZ + c - 1) / c) >= 2048) {
             zChunk = c;
             break;
         }
-    if (zChunk == 4 && int64_t(gx) * gy * ((L.NZ + 3) / 4) < 1024) zChunk = 2;
+    if (zChunk == 0) {
+        int64_t best = INT64_MAX;
+        for (int c = 2; c <= 8; c++) {
+            const int64_t rounds = (int64_t(gx) * gy * ((L.NZ + c - 1) / c) + slots - 1) / slots, cost = rounds * (c + 1);
+            if (cost < best) {
+                best = cost;
+                zChunk = c;
+            }
+        }
+    }
     if (const char* e = std::getenv("PSX_EL_ZCHUNK")) zChunk = std::max(2, std::atoi(e));
     const int gz = (L.NZ + zChunk - 1) / zChunk;
     const int nTiles = gx * gy * gz;
     const dim3 g(unsigned(8 * ((nTiles + 7) / 8)), unsigned(ninst));
     const bool dot = dots && (mode == AM_APPLY || mode == AM_CHEB1 || mode == AM_CHEB2);
     const float f1 = float(c1), f2 = float(c2);
+    const bool ownc = L.childForm != 0;
+    if (ownc) {
+        // the own elements' corner rows of this product's input first
+        const int64_t nOwn = L.ownTStride / 24, perBlock = 16 * OWN_TILES * (OwnRowsMf::NTH / 64);
+        go<OwnRowsMf>(kOwnRowsMf, dim3(unsigned((nOwn + perBlock - 1) / perBlock), unsigned(ninst)), s, L, x, stop, nOwn);
+    }
     auto form = [&](auto m, auto dt, auto pc) {
         constexpr int M = decltype(m)::value, P = decltype(pc)::value;
         constexpr bool DT = decltype(dt)::value;
-        go<FineEl<M, DT, P, P>>(kFineEl<M, DT, P, P>, g, s, L, x, y, r, zp, f1, f2, stop, zChunk, gx, gy, nTiles, dots, dotStride, qdot);
+        if (ownc) go<FineEl<M, DT, P, P, true>>(kFineEl<M, DT, P, P, true>, g, s, L, x, y, r, zp, f1, f2, stop, zChunk, gx, gy, nTiles, dots, dotStride, qdot);
+        else go<FineEl<M, DT, P, P, false>>(kFineEl<M, DT, P, P, false>, g, s, L, x, y, r, zp, f1, f2, stop, zChunk, gx, gy, nTiles, dots, dotStride, qdot);
     };
     // three pieces for the outer operator (y = K x); the smoothing steps take two
     auto byPieces = [&](auto m, auto dt) {

@@ -249,6 +249,15 @@ public:
     void finishSetup() {
         stressS = alloc<double>(8 * 144);
         red.counter = alloc<unsigned>(1);
+        // the child-scale level's own-element voxels (the matrix-core operator's corner-row pass)
+        for (size_t l = 0; l < lv.size(); l++) {
+            DevLevel& D = lv[l].dl;
+            if (!D.childForm || H.levels[l].nOwn == 0) continue;
+            int32_t* ov = alloc<int32_t>(H.levels[l].nOwn);
+            launchOwnVox(D.words, int64_t(D.nx) * D.ny * D.nz, ov, stream);
+            D.ownVox = ov;
+            D.ownTStride = 24 * H.levels[l].nOwn;
+        }
         reserve(1);
     }
     // the level's uniform element, its stencil and symmetry classes (from H.levels[l].Kb)
@@ -309,6 +318,44 @@ public:
             }
         return out;
     }
+    // the 8 child products M_c as A operands of the own-element matrix-core pass (fine_el.hip), per
+    // child c, half h (output corners 4 h .. 4 h + 3, numbered x + 2 y + 4 z), piece p and lane l:
+    // row l % 16 = 4 (output corner - 4 h) + component (component 3 zero), K slice 8 (l / 16) .. + 7 =
+    // input corners 2 g, 2 g + 1 (g = l / 16) with components 0..3 (the fourth zero); two BF16 pieces
+    // hi = bf16(v), lo = bf16(v - hi), rounded to nearest
+    static std::vector<uint32_t> childOperands(const double* M) {
+        auto bf = [](double v) -> uint32_t {
+            const float f = float(v);
+            uint32_t u;
+            std::memcpy(&u, &f, 4);
+            return (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
+        };
+        auto val = [](uint32_t b) {
+            const uint32_t u = b << 16;
+            float f;
+            std::memcpy(&f, &u, 4);
+            return double(f);
+        };
+        auto corner = [](int q) { const int x = q & 1, y = (q >> 1) & 1, z = q >> 2; return (y ? (x ? 2 : 3) : (x ? 1 : 0)) + 4 * z; };
+        std::vector<uint32_t> out(size_t(8) * 2 * 2 * 64 * 4);
+        for (int c = 0; c < 8; c++)
+            for (int h = 0; h < 2; h++)
+                for (int l = 0; l < 64; l++) {
+                    const int m = l & 15, kg = l >> 4, d = m & 3, ao = corner(4 * h + (m >> 2));
+                    uint32_t pc[2][8];
+                    for (int t = 0; t < 8; t++) {
+                        const int ci = t & 3, ai = corner(2 * kg + (t >> 2));
+                        double v = (d < 3 && ci < 3) ? M[size_t(c) * 576 + size_t(3 * ao + d) * 24 + 3 * ai + ci] : 0.0;
+                        for (int p = 0; p < 2; p++) {
+                            pc[p][t] = bf(v);
+                            v -= val(pc[p][t]);
+                        }
+                    }
+                    for (int p = 0; p < 2; p++)
+                        for (int w = 0; w < 4; w++) out[(((size_t(c) * 2 + h) * 2 + p) * 64 + l) * 4 + w] = pc[p][2 * w] | (pc[p][2 * w + 1] << 16);
+                }
+        return out;
+    }
     // child-scale form: corner-0 rows of the 8 Galerkin products and the subset table (from
     // H.levels[l].childM / childTab)
     void uploadChildConstants(size_t l) {
@@ -319,6 +366,8 @@ public:
         for (int c = 0; c < 8; c++)
             for (int u = 0; u < 72; u++) rows[c * 72 + u] = L.childM[size_t(c) * 576 + u];  // rows 0..2 of M_c
         for (int u = 0; u < 8 * 72; u++) rows32[u] = float(rows[u]);
+        const auto CA = childOperands(L.childM.data());
+        D.childA = reinterpret_cast<const uint4*>(upload(CA.data(), int64_t(CA.size())));
         D.crow64 = upload(rows, 8 * 72);
         D.crow32 = upload(rows32, 8 * 72);
         std::vector<float> tab32(L.childTab.begin(), L.childTab.end());
@@ -389,6 +438,7 @@ public:
             lv[l].a32 = alloc<float>(nl, true);
             lv[l].b32 = alloc<float>(nl, true);
             lv[l].d32 = alloc<float>(nl, true);
+            if (lv[l].dl.ownVox) lv[l].dl.ownT = alloc<float>(lv[l].dl.ownTStride * k, true);
         }
         const int64_t n = nDof(), nk = n * k;
         x = alloc<double>(nk, true);
@@ -436,7 +486,8 @@ public:
             const DotFusion* fd = nullptr) {
         const DevLevel& D = lv[l].dl;
         // the element-by-element matrix-core form: FP32 levels without own elements
-        const int el = sizeof(T) == 4 && H.levels[l].nOwn == 0 ? ((form & OF_EL3) ? 3 : (form & OF_MFMA) ? 2 : 0) : 0;
+        // (the child-scale level too: its own elements' corner rows come from a pass of their own)
+        const int el = sizeof(T) == 4 && (H.levels[l].nOwn == 0 || (D.childForm && D.ownVox)) ? ((form & OF_EL3) ? 3 : (form & OF_MFMA) ? 2 : 0) : 0;
         if (!useFine) launchApply<T>(D, mode, xx, yy, rr, dd, c1, c2, stop, stream, ni);
         else if (D.symOk && (H.levels[l].nOwn == 0 || (D.childForm && ((form & OF_CHILD) || ((form & OF_AUTO) && lv[l].childCheap))))) {
             // the mixed V-cycle's smoothing (OF_AUTO, FP32) reads the child-scale subset table in FP16

@@ -88,6 +88,9 @@ __device__ __forceinline__ DevLevel globalArg(DevLevel v) {
     v.invD64 = asGlobal(v.invD64);
     v.invD32 = asGlobal(v.invD32);
     v.diagAdd = asGlobal(v.diagAdd);
+    v.ownVox = asGlobal(v.ownVox);
+    v.childA = asGlobal(v.childA);
+    v.ownT = asGlobal(v.ownT);
     return v;
 }
 

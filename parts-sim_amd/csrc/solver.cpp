@@ -208,10 +208,11 @@ int psx_apply_mfma(psx_model m, int level, int pieces, const float* x, float* y)
         if (pieces != 2 && pieces != 3) throw std::invalid_argument("pieces must be 2 or 3");
         psx::Model& M = *m->m;
         auto& V = M.lv[level];
-        if (M.H.levels[size_t(level)].nOwn > 0) throw std::invalid_argument("the level has own element matrices: no matrix-core operator");
+        if (M.H.levels[size_t(level)].nOwn > 0 && !V.dl.ownVox)
+            throw std::invalid_argument("the level has stored own element matrices: no matrix-core operator");
         const int64_t n = V.dl.nDof;
         HIPCHK(hipMemcpyAsync(V.a32, x, size_t(n) * 4, hipMemcpyHostToDevice, M.stream));
-        M.op<float>(size_t(level), psx::AM_APPLY, V.a32, V.b32, nullptr, nullptr, 0, 0, nullptr, pieces == 3 ? psx::OF_EL3 : psx::OF_MFMA);
+        M.op<float>(size_t(level), psx::AM_APPLY, V.a32, V.b32, nullptr, nullptr, 0, 0, nullptr, (pieces == 3 ? psx::OF_EL3 : psx::OF_MFMA) | psx::OF_CHILD);
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(y, V.b32, size_t(n) * 4, hipMemcpyDeviceToHost, M.stream));
         M.sync();
@@ -473,13 +474,15 @@ int psx_profile(psx_model m, int reps, char* buf, size_t len) {
             const double t64 = timeIt([&] { M.op<double>(l, psx::AM_APPLY, V.a64, V.b64, nullptr, nullptr, 0, 0, nullptr); });
             const double t32 = timeIt([&] { M.op<float>(l, psx::AM_APPLY, V.a32, V.b32, nullptr, nullptr, 0, 0, nullptr); });
             double tmf = 0, tmf3 = 0, s32 = 0, smf = 0, c64 = 0, c32 = 0;
-            if (L.nOwn == 0) {
-                tmf = timeIt([&] { M.op<float>(l, psx::AM_APPLY, V.a32, V.b32, nullptr, nullptr, 0, 0, nullptr, psx::OF_MFMA); });
-                tmf3 = timeIt([&] { M.op<float>(l, psx::AM_APPLY, V.a32, V.b32, nullptr, nullptr, 0, 0, nullptr, psx::OF_EL3); });
+            if (L.nOwn == 0 || V.dl.ownVox) {
+                // (on the child-scale level with its own elements' corner-row pass)
+                const int ch = L.nOwn ? psx::OF_CHILD : 0;
+                tmf = timeIt([&] { M.op<float>(l, psx::AM_APPLY, V.a32, V.b32, nullptr, nullptr, 0, 0, nullptr, psx::OF_MFMA | ch); });
+                tmf3 = timeIt([&] { M.op<float>(l, psx::AM_APPLY, V.a32, V.b32, nullptr, nullptr, 0, 0, nullptr, psx::OF_EL3 | ch); });
                 // a later Chebyshev step y = x + c1 (x - d) + c2 D^-1 (r - K x), the V-cycle's most frequent
                 // launch, in the FP32 VALU form and as the element product on the matrix cores
-                s32 = timeIt([&] { M.op<float>(l, psx::AM_CHEB2, V.a32, V.b32, V.r32, V.d32, 0.5, 0.5, nullptr); });
-                smf = timeIt([&] { M.op<float>(l, psx::AM_CHEB2, V.a32, V.b32, V.r32, V.d32, 0.5, 0.5, nullptr, psx::OF_MFMA); });
+                s32 = timeIt([&] { M.op<float>(l, psx::AM_CHEB2, V.a32, V.b32, V.r32, V.d32, 0.5, 0.5, nullptr, ch ? psx::OF_AUTO : 0); });
+                smf = timeIt([&] { M.op<float>(l, psx::AM_CHEB2, V.a32, V.b32, V.r32, V.d32, 0.5, 0.5, nullptr, psx::OF_MFMA | (ch ? psx::OF_AUTO : 0)); });
             }
             if (V.dl.childForm) {
                 c64 = timeIt([&] { M.op<double>(l, psx::AM_APPLY, V.a64, V.b64, nullptr, nullptr, 0, 0, nullptr, psx::OF_CHILD); });

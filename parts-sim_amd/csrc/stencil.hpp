@@ -4,8 +4,54 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
+#include <utility>
 
 namespace psx {
+
+// The reference element's corners (x, y, z offsets of corner b, numbered 0..3 counter-clockwise in
+// the lower face, 4..7 above) and its symmetry classes
+__device__ constexpr int cx(int b) { return (b == 1 || b == 2 || b == 5 || b == 6) ? 1 : 0; }
+__device__ constexpr int cy(int b) { return (b == 2 || b == 3 || b == 6 || b == 7) ? 1 : 0; }
+__device__ constexpr int cz(int b) { return b >= 4 ? 1 : 0; }
+__device__ constexpr int cornerOf(int x, int y, int z) { return (y ? (x ? 2 : 3) : (x ? 1 : 0)) + 4 * z; }
+// Symmetry classes (the host's elementClass, hierarchy.cpp): every element coefficient is
+// +-one of 22 values, at an index known at compile time once the loops are unrolled, so the values
+// stay in scalar registers for the whole kernel.
+__device__ constexpr int elementClass(int a, int d, int b, int c) {
+    const int ca[3] = {cx(a), cy(a), cz(a)};
+    int r[3] = {cx(a) ^ cx(b), cy(a) ^ cy(b), cz(a) ^ cz(b)};
+    const bool neg = ((ca[d] + ca[c]) & 1) != 0;
+    int t = r[0];
+    r[0] = r[d];
+    r[d] = t;
+    int c2 = c == 0 ? d : (c == d ? 0 : c);
+    if (c2 == 2) {
+        t = r[1];
+        r[1] = r[2];
+        r[2] = t;
+        c2 = 1;
+    } else if (c2 == 0 && r[1] < r[2]) {
+        t = r[1];
+        r[1] = r[2];
+        r[2] = t;
+    }
+    const int idx = c2 == 0 ? r[0] * 3 + r[1] + r[2] : 6 + r[0] + 2 * r[1] + 4 * r[2];
+    return idx | (neg ? 0x100 : 0);
+}
+template <class T> struct Sym {
+    T v[22];
+};
+// coefficient times x, class and sign fixed at compile time
+template <int CLS, class T> __device__ __forceinline__ T coefMul(const Sym<T>& C, T x) {
+    if constexpr ((CLS & 0x100) != 0) return -C.v[CLS & 0xff] * x;
+    else return C.v[CLS & 0xff] * x;
+}
+// loop with a compile-time index: f(std::integral_constant<int, i>) for i in [0, N)
+template <class F, int... I> __device__ __forceinline__ void staticForImpl(F&& f, std::integer_sequence<int, I...>) {
+    (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, class F> __device__ __forceinline__ void staticFor(F&& f) { staticForImpl(f, std::make_integer_sequence<int, N>{}); }
 
 // Raw buffer accesses over one plane: the descriptor's byte count is the plane's, so a lane whose
 // offset is OUT_OF_RANGE (a position outside the part) reads 0 and its store is dropped, and a plane

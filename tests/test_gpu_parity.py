@@ -176,17 +176,36 @@ def test_element_matrix_core_operator(P, W, pieces):
         # void surfaces and the part's faces included
         t = np.tile([1.0, -2.0, 0.5], n // 3)
         assert np.all(g.apply_mfma(t.astype(np.float32), 0, pieces) == 0)
+        # the child-scale level: uniform elements as above, own elements sum_c s_c M_c (x_e - x_e0) as
+        # two-piece products over the 8 child matrices in a pass of their own (both forms), against the
+        # FP64 child form; a translation is again exactly zero
+        if g.num_levels > 1 and g.level(1)["nOwn"] > 0:
+            info = g.level(1)
+            n1 = info["nDof"]
+            mx, my, mz = (d + 1 for d in info["dims"])
+            Z1, Y1, X1 = np.meshgrid(np.arange(mz), np.arange(my), np.arange(mx), indexing="ij")
+            smooth1 = np.stack([np.sin(X1 / 5.0) + 0.1 * Y1, np.cos(Y1 / 3.0) * Z1 / 4.0, 0.03 * X1 * Z1], -1).reshape(-1)
+            rough1 = np.random.default_rng(6).standard_normal(n1)
+            for x, tol in ((rough1, 2e-5), (smooth1, 2e-4)):
+                y64 = g.apply_child(x, 1)
+                ym = g.apply_mfma(x.astype(np.float32), 1, pieces).astype(np.float64)
+                err = np.linalg.norm(ym - y64) / np.linalg.norm(y64)
+                print(w["dims"], "level 1 nOwn", info["nOwn"], pieces, "pieces: rel err", err, flush=True)
+                assert err < tol
+            t1 = np.tile([1.0, -2.0, 0.5], n1 // 3)
+            assert np.all(g.apply_mfma(t1.astype(np.float32), 1, pieces) == 0)
         g.close()
 
 
 def test_element_matrix_core_operator_rejects(P, W):
-    w = W.void_block((16, 8, 8), seed=1)
+    # pieces other than 2 or 3, and the levels with stored own matrices (below the child-scale one)
+    w = W.void_block((32, 16, 16), seed=1)
     g = P.Model(w["dims"], w["density"], w["nu"], w["bc"], 60)
     with pytest.raises(Exception):
         g.apply_mfma(np.zeros(g.n_dof, np.float32), 0, 4)
-    if g.num_levels > 1 and g.level(1)["nOwn"] > 0:
-        with pytest.raises(Exception):
-            g.apply_mfma(np.zeros(g.level(1)["nDof"], np.float32), 1, 2)
+    assert g.num_levels > 2 and g.level(2)["nOwn"] > 0
+    with pytest.raises(Exception):
+        g.apply_mfma(np.zeros(g.level(2)["nDof"], np.float32), 2, 2)
     g.close()
 
 
