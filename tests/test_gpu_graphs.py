@@ -8,21 +8,23 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed", "ref"])
-def test_graph_replays_equal_eager(P, W, precision):
+# (mixed, 1): the smoothing of levels 0 and 1 as element products on the matrix cores, degree 3
+@pytest.mark.parametrize("precision,mfma", [("fp64", 0), ("mixed", 0), ("ref", 0), ("mixed", 1)])
+def test_graph_replays_equal_eager(P, W, precision, mfma):
     w = W.void_block((40, 16, 12), seed=21)
     m = P.Model(w["dims"], w["density"], w["nu"], w["bc"], 300)
     f = w["f"]
-    eager = m.solve(f, tol=1e-9, precision=precision, eager=True)
+    kw = dict(mfma_smoother=1, cheb_degree=3) if mfma else {}
+    eager = m.solve(f, tol=1e-9, precision=precision, eager=True, **kw)
     for rep in range(3):
         # the second and third solves replay the graph the first one captured, with eager solves
         # and reads in between
-        g = m.solve(f * (1 + rep), tol=1e-9, precision=precision, check_every=4)
-        e = m.solve(f * (1 + rep), tol=1e-9, precision=precision, check_every=4, eager=True)
-        print(precision, rep, g["iterations"], e["iterations"], flush=True)
+        g = m.solve(f * (1 + rep), tol=1e-9, precision=precision, check_every=4, **kw)
+        e = m.solve(f * (1 + rep), tol=1e-9, precision=precision, check_every=4, eager=True, **kw)
+        print(precision, mfma, rep, g["iterations"], e["iterations"], flush=True)
         assert g["iterations"] == e["iterations"]
         assert np.array_equal(g["u"], e["u"])
-    assert np.array_equal(m.solve(f, tol=1e-9, precision=precision)["u"], eager["u"])
+    assert np.array_equal(m.solve(f, tol=1e-9, precision=precision, **kw)["u"], eager["u"])
     m.close()
 
 

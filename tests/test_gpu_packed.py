@@ -74,6 +74,21 @@ def test_packed_equals_own_solves(P, W, precision):
         m.close()
 
 
+def test_packed_matrix_core_smoothing_equals_own_solves(P, W):
+    # the mixed V-cycle's smoothing as element products on the matrix cores (levels 0 and 1, the
+    # child-scale level's own elements in their pass of their own), degree 3 as the bench line runs
+    # it: the packed launches give every model its own multi-instance results bit for bit, replayed
+    # from the merged graph and launched eagerly
+    parts = _parts(W)
+    models = _models(P, parts)
+    kw = dict(tol=1e-9, precision="mixed", cheb_degree=3, mfma_smoother=1)
+    lone = _lone(P, models, parts, **kw)
+    _same(lone, _packed(P, models, parts, **kw))
+    _same(lone, _packed(P, models, parts, eager=True, **kw))
+    for m in models:
+        m.close()
+
+
 def test_packed_warm_start_and_degree3(P, W):
     parts = _parts(W)
     models = _models(P, parts)

@@ -18,19 +18,22 @@ def _loads(w, nrhs, seed):
     return np.stack([f * (1 + k) + 0.01 * np.abs(f).max() * rng.standard_normal(f.size) * (f != 0) for k in range(nrhs)])
 
 
-# pack_max 0: the small parts run in the shared-launch pack; 1: each on its own stream
-@pytest.mark.parametrize("precision,device_build,pack_max", [("fp64", True, 0), ("mixed", True, 0), ("mixed", False, 0), ("mixed", True, 1)])
-def test_served_results_equal_lone_solves(P, W, precision, device_build, pack_max):
+# pack_max 0: the small parts run in the shared-launch pack; 1: each on its own stream; mfma 1: the
+# smoothing of levels 0 and 1 as element products on the matrix cores at degree 3 (the bench's setting)
+@pytest.mark.parametrize("precision,device_build,pack_max,mfma", [("fp64", True, 0, 0), ("mixed", True, 0, 0), ("mixed", False, 0, 0),
+                                                                  ("mixed", True, 1, 0), ("mixed", True, 0, 1)])
+def test_served_results_equal_lone_solves(P, W, precision, device_build, pack_max, mfma):
     parts = _parts(W)
+    kw = dict(mfma_smoother=1, cheb_degree=3) if mfma else {}
     alone = []
     for q, (w, nrhs) in enumerate(parts):
         m = P.Model(w["dims"], w["density"], w["nu"], w["bc"], 300, device_build=device_build)
         F = _loads(w, nrhs, q)
-        alone.append(np.stack([m.solve(F[k], tol=1e-9, precision=precision)["u"] for k in range(nrhs)]))
+        alone.append(np.stack([m.solve(F[k], tol=1e-9, precision=precision, **kw)["u"] for k in range(nrhs)]))
         m.close()
     srv = P.Server(max_inflight=3, pack_max=pack_max)
     tickets = [srv.submit(w["dims"], w["density"], w["nu"], w["bc"], _loads(w, nrhs, q), coarsest_max_dof=300, tol=1e-9, precision=precision,
-                          device_build=device_build)
+                          device_build=device_build, **kw)
                for q, (w, nrhs) in enumerate(parts)]
     for q, t in enumerate(tickets):
         u, res = srv.wait(t)
