@@ -1,5 +1,6 @@
 // Element-by-element operator on the matrix cores, for FP32 levels whose elements are all s * Kb
-// (every finest level, uniform or per-voxel scales): y = K x with the fused epilogues of kFine.
+// (every finest level, uniform or per-voxel scales) and for the child-scale level below one: y = K x
+// with the fused epilogues of kFine.
 //
 // K x = sum over voxels e of s_e Kb x_e, and Kb annihilates rigid translations, so the product is
 // taken of x' = x - c, c one node's value per block (the tile's centre): only the spread of x over
@@ -24,6 +25,12 @@
 // are the only shared data: one barrier per layer.
 // Nothing depends on which voxels differ in scale, so void surfaces and per-voxel densities cost the
 // same as solid regions (no pair lists).
+//
+// The child-scale level: its uniform elements are s * Kb with that level's Kb and take the path above;
+// an own element (voxel word < 0) enters the products with scale 0, and its product
+// sum_c s_c M_c (x_e - x_e0) over the 8 child matrices (OwnRowsMf, one element per MFMA column) is
+// written to a work buffer just before and added to its corners' nodes in the epilogue. The inverse
+// diagonal there is the level's stored one.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
