@@ -5,7 +5,8 @@ Workload (one GPU): the 512x128x128 void block (25 610 499 DOF): seeded spherica
 U[2, 8] voxels, 20 % void fraction, void stiffness 1e-3), E = 70 GPa, nu = 0.33, x = 0 clamped,
 -z pressure of 5 MPa on the top face over x > 200 mm, 6-level geometric multigrid, Chebyshev degree-3
 smoothing (the workload as configured). Mixed precision: FP32 outer operator, FP32 V-cycle whose
-level-0 smoothing takes the element products on the matrix cores (BF16 pieces, FP32 accumulation),
+level-0 and level-1 (child-scale) smoothing takes the element products on the matrix cores (BF16
+pieces, FP32 accumulation),
 FP64 coarsest solve, FP32 CG recurrences with FP64 dot products, the FP64 solution and the true
 residual recomputed in FP64 at least every 20 iterations. The plain FP64 MG-PCG runs the same
 multigrid in FP64. Both solves run to the reference solver's default tolerance, relative residual
@@ -310,8 +311,8 @@ def batched_leg(P, W, dims, nrhs, steps, warmup, tol, degree=3, mf=1):
         return m.solve_multi_device(k, d_f, d_u, tol=tol, max_iter=3000, precision=precision, check_every=8, mfma_smoother=mfma,
                                     cheb_degree=deg)
 
-    # the configuration of the line: Chebyshev degree 3, level-0 smoothing (per-voxel densities) as the
-    # element products on the matrix cores
+    # the configuration of the line: Chebyshev degree 3, the smoothing of level 0 (per-voxel densities)
+    # and level 1 (all own elements, child-scale form) as element products on the matrix cores
     lat_b, infos = timed(lambda: multi("mixed", mf), steps, warmup)
     e_b = errs(d_u.download().reshape(k, n))
     if not all(i["converged"] for i in infos) or max(e_b) > 1e-6:
@@ -348,7 +349,7 @@ def batched_leg(P, W, dims, nrhs, steps, warmup, tol, degree=3, mf=1):
            "unit": "solves/s", "n_gpus": 1, "steps": steps, "warmup": warmup, "ms_per_step": round(1e3 * lat_b.mean(), 3),
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
            "config": {"workload": w["name"] + ", SIMP x^3 densities U[0.001,1], E0 200 GPa, nu 0.3, x=0 clamped, 1 kN point loads on x=%d, "
-                                   "Chebyshev degree %d, level-0 smoothing %s" % (dims[0], degree, "on the matrix cores" if mf else "FP32 VALU"),
+                                   "Chebyshev degree %d, level-0 and level-1 smoothing %s" % (dims[0], degree, "on the matrix cores" if mf else "FP32 VALU"),
                       "load_cases": k, "dof_per_case": n, "levels": m.num_levels, "tol": tol, "parallelism": "single GPU"},
            "p50_ms": round(1e3 * float(np.percentile(lat_b, 50)), 3), "p99_ms": round(1e3 * float(np.percentile(lat_b, 99)), 3),
            "iterations_max": int(max(its)), "iterations_mean": float(np.mean(its)), "rel_error_vs_fp64_max": max(e_b),
@@ -400,7 +401,7 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-6, help="relative residual tolerance (the reference solver's default, voxel_fea.hpp:113)")
     ap.add_argument("--degree", type=int, default=3, help="Chebyshev smoother degree (3: the workload as configured)")
     ap.add_argument("--smoother", default="mfma", choices=["mfma", "valu"],
-                    help="mixed precision: level-0 smoothing as the element products on the matrix cores, or the FP32 VALU stencil")
+                    help="mixed precision: level-0 and level-1 smoothing as the element products on the matrix cores, or the FP32 VALU stencils")
     ap.add_argument("--pre-degree", type=int, default=0, help="Chebyshev degree of the pre-smoothing (0 = --degree)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extras", action="store_true", help="skip the extra legs of the default line (C3 as specified, batched, serving)")
@@ -576,7 +577,7 @@ def main():
         "ms_per_step": round(1e3 * lat.mean(), 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32+f64", "data": "synthetic",
         "config": {"workload": "void block %dx%dx%d, seeded spherical voids 20%% at 1e-3 stiffness, E 70 GPa, nu 0.33, x=0 clamped, "
-                               "5 MPa top pressure x>200 mm, %d-level MG, Chebyshev degree %d, level-0 smoothing %s" %
+                               "5 MPa top pressure x>200 mm, %d-level MG, Chebyshev degree %d, level-0 and level-1 smoothing %s" %
                                (dims + (levels, args.degree, "as element products on the matrix cores (BF16 pieces, FP32 accumulation)" if mf
                                         else "as the FP32 VALU stencil")),
                    "dof": n_dof, "levels": levels, "tol": args.tol, "parallelism": "single GPU"},
