@@ -282,7 +282,7 @@ __device__ void ApplyOwn<T, MODE, H16>::run(VBlk vb, DevLevel L, const T* __rest
     const int64_t n = int64_t(vb.x) * (BLOCK / 8) + threadIdx.x / 8;
     const int v = threadIdx.x & 7;
     const int64_t NX = L.NX, NXY = int64_t(L.NX) * L.NY;
-    bool has = false;
+    bool has = false, inVox = false;
     int32_t w = 0;
     int64_t nb = 0;
     int a = 0;
@@ -293,11 +293,46 @@ __device__ void ApplyOwn<T, MODE, H16>::run(VBlk vb, DevLevel L, const T* __rest
         const int ei = i - 1 + di, ej = j - 1 + dj, ek = k - 1 + dk;
         if (ei >= 0 && ej >= 0 && ek >= 0 && ei < L.nx && ej < L.ny && ek < L.nz) {
             w = L.words[ei + int64_t(L.nx) * (ej + int64_t(L.ny) * ek)];
-            has = w != 0;
+            inVox = true;
             a = d_corner(1 - di, 1 - dj, 1 - dk);
             nb = ei + NX * ej + NXY * ek;
         }
     }
+    // the first instance's operands do not depend on the voxel word: the element's corner values and
+    // (one lane per node) the node's own vectors are loaded beside it, so a wavefront waits for the
+    // word, then the element rows, instead of four memory round trips in turn
+    struct alignas(sizeof(T)) V3 {
+        T c[3];
+    };
+    const T* __restrict__ invD = Prec<T>::invD(L);
+    T xe[24];
+    V3 idv{}, xv{}, rv{}, zv{};
+    auto loadCorners = [&](int64_t io) {
+        const T* __restrict__ xi = x + io;
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            const T* xm = xi + 3 * (nb + kCX(b) + kCY(b) * NX + kCZ(b) * NXY);
+            xe[3 * b] = xm[0];
+            xe[3 * b + 1] = xm[1];
+            xe[3 * b + 2] = xm[2];
+        }
+    };
+    auto loadNode = [&](int64_t io) {
+        idv = *reinterpret_cast<const V3*>(invD + 3 * n);
+        if constexpr (MODE == AM_CHEB1 || MODE == AM_CHEB2) xv = *reinterpret_cast<const V3*>(x + io + 3 * n);
+        if constexpr (MODE == AM_RESID || MODE == AM_CHEB1 || MODE == AM_CHEB2) rv = *reinterpret_cast<const V3*>(r + io + 3 * n);
+        if constexpr (MODE == AM_CHEB2)
+            if (zp) zv = *reinterpret_cast<const V3*>(zp + io + 3 * n);
+    };
+    // (measured on the 512x128x128 block: level 2 FP32 28.7 -> 27.8 us, the mixed solve 105.0 ->
+    // 104.2 ms. FP64 loads them after the rows: its prefetch registers cost the later Chebyshev step a
+    // wavefront per SIMD, and the FP64 solve did not gain)
+    constexpr bool PREFETCH = sizeof(T) == 4;
+    if constexpr (PREFETCH) {
+        if (inVox) loadCorners(0);
+        if (v == 0 && n < L.nNodes) loadNode(0);
+    }
+    has = w != 0;
     // the node's 3 rows of the voxel's element: s * Kb (scaled after the sum) or the own matrix
     T K[72];
     // s scales the summed rows: s for s * Kb, 1 for a stored own matrix, its power-of-two scale for an
@@ -333,21 +368,14 @@ __device__ void ApplyOwn<T, MODE, H16>::run(VBlk vb, DevLevel L, const T* __rest
             }
         }
     }
-    const T* __restrict__ invD = Prec<T>::invD(L);
     for (int inst = 0; inst < ninst; inst++) {
         if (stopped(stop ? stop + int64_t(inst) * CG_INTS : nullptr)) continue;  // uniform
         const int64_t io = int64_t(inst) * L.nDof;
         const T* __restrict__ xi = x + io;
+        const bool load = inst > 0 || !PREFETCH;
         T b0 = 0, b1 = 0, b2 = 0;
         if (has) {
-            T xe[24];
-#pragma unroll
-            for (int b = 0; b < 8; b++) {
-                const T* xm = xi + 3 * (nb + kCX(b) + kCY(b) * NX + kCZ(b) * NXY);
-                xe[3 * b] = xm[0];
-                xe[3 * b + 1] = xm[1];
-                xe[3 * b + 2] = xm[2];
-            }
+            if (load) loadCorners(io);
 #pragma unroll
             for (int c = 0; c < 24; c++) {
                 b0 += K[c] * xe[c];
@@ -370,16 +398,9 @@ __device__ void ApplyOwn<T, MODE, H16>::run(VBlk vb, DevLevel L, const T* __rest
 #pragma unroll
                 for (int d = 0; d < 3; d++) acc[d] += T(L.diagAdd[3 * n + d]) * xi[3 * n + d];
             }
-            // the node's operands and result as three-component loads and one store
-            struct alignas(sizeof(T)) V3 {
-                T c[3];
-            };
-            const V3 idv = *reinterpret_cast<const V3*>(invD + 3 * n);
-            V3 xv{}, rv{}, zv{}, out;
-            if constexpr (MODE == AM_CHEB1 || MODE == AM_CHEB2) xv = *reinterpret_cast<const V3*>(xi + 3 * n);
-            if constexpr (MODE == AM_RESID || MODE == AM_CHEB1 || MODE == AM_CHEB2) rv = *reinterpret_cast<const V3*>(r + io + 3 * n);
-            if constexpr (MODE == AM_CHEB2)
-                if (zp) zv = *reinterpret_cast<const V3*>(zp + io + 3 * n);
+            if (load) loadNode(io);
+            // the node's result as one three-component store
+            V3 out;
 #pragma unroll
             for (int d = 0; d < 3; d++) {
                 const T id = idv.c[d];
