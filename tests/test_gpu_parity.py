@@ -286,6 +286,35 @@ def test_mixed_with_matrix_core_operators(P, W, form):
         g.close()
 
 
+def test_matrix_core_forms_with_diagonal_shift(P, W):
+    """A shifted operator K + diag(a) (a shifted eigen model's, restricted to the coarse levels as
+    the FP64 path restricts it): the element products add a x and fold a into the inverse diagonal
+    on level 0 (from the voxel scales) and level 1 (the stored inverse), to the tolerances of the
+    unshifted forms; the mixed solve with them meets the FP64 solution to 1e-7."""
+    w = W.void_block((48, 16, 16), seed=3)
+    n0 = 3 * W.nodes(w["dims"])
+    shift = 0.05 * np.random.default_rng(8).random(n0)
+    g = P.Model(w["dims"], w["density"], w["nu"], w["bc"], 1200, diag_add=shift)
+    x = np.random.default_rng(4).standard_normal(n0)
+    y64 = g.apply(x)
+    ym = g.apply_mfma(x.astype(np.float32), 0, 2).astype(np.float64)
+    err0 = _rel(ym, y64)
+    info = g.level(1)
+    x1 = np.random.default_rng(6).standard_normal(info["nDof"])
+    err1 = _rel(g.apply_mfma(x1.astype(np.float32), 1, 2).astype(np.float64), g.apply_child(x1, 1))
+    f = w["f"] if w["f"].ndim == 1 else w["f"][0]
+    a = g.solve(f, tol=1e-10, precision="mixed", mfma_smoother=1, cheb_degree=3)
+    v = g.solve(f, tol=1e-10, precision="mixed", cheb_degree=3)
+    b = g.solve(f, tol=1e-10, precision="fp64", cheb_degree=3)
+    print("shifted: level 0 %.2e level 1 (nOwn %d) %.2e its %d valu %d fp64 %d" % (err0, info["nOwn"], err1, a["iterations"], v["iterations"],
+                                                                                 b["iterations"]), flush=True)
+    assert info["nOwn"] > 0
+    assert err0 < 2e-5 and err1 < 2e-5
+    assert a["converged"] and _rel(a["u"], b["u"]) < 1e-7
+    assert abs(a["iterations"] - v["iterations"]) <= 2
+    g.close()
+
+
 def test_stresses_match_oracle(models):
     """Nodal von Mises, principal stresses and tensors on the device against the oracle's
     (voxel_fea.cpp:968-1084): the 1e-5 relative von Mises bar, met with margin."""
