@@ -149,6 +149,10 @@ struct DotFusion {
     int kind = FD_NONE;
     const void* q = nullptr;  // FD_RZ: the previous search direction's product K p (type T)
     Reduce red{};
+    // FD_PQ on the VALU operator (type T): the direction update p = z + beta x fused into y = K p,
+    // z the preconditioned residual, the new p stored to pw (not over x); both null: x is p
+    const void* pz = nullptr;
+    void* pw = nullptr;
 };
 // launchFine with a fused dot; returns the number of block partials written (0: not fused)
 template <class T> unsigned launchFineDot(const DevLevel& L, int mode, const T* x, T* y, const T* r, const T* zp, double c1, double c2, const int* stop,

@@ -17,7 +17,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstddef>
 #include <cstdlib>
+#include <stdexcept>
 #include <type_traits>
 #include <utility>
 
@@ -60,7 +62,7 @@ template <class T, bool OWNC> struct Ring {
 #define PSX_FINE_PARAMS                                                                                                                    \
     DevLevel L, Sym<T> C, const T *__restrict__ crow, const T *__restrict__ ctab, const T *__restrict__ x, T *__restrict__ y,                 \
         const T *__restrict__ r, const T *__restrict__ zp, T c1, T c2, const int *stop, int zChunk, int gx, int gy, int nTiles, int xcdOrder, \
-        double *__restrict__ dots, int64_t dotStride, const T *__restrict__ qdot
+        double *__restrict__ dots, int64_t dotStride, const T *__restrict__ qdot, const T *__restrict__ pz, T *__restrict__ pw
 // DOT: the fused dot products are accumulated (dots != nullptr). DENSE (levels without own
 // elements): every node sums its 8 voxels, s_v E_v (x - x_node) with E_v the node's 3 rows of Kb in
 // voxel v (compile-time class coefficients), instead of the stencil and the pair lists; for per-voxel
@@ -80,7 +82,17 @@ __device__ __forceinline__ void fineBody(VBlk vb, PSX_FINE_PARAMS) {
         if (r) r += io;
         if (zp) zp += io;
         if (qdot) qdot += io;
+        if (pz) pz += io;
+        if (pw) pw += io;
         if (DOT) dots += int64_t(vb.y) * dotStride;
+    }
+    // the conjugate gradients' direction update fused into y = K p (pz: the preconditioned residual z,
+    // x: the previous direction): the staged planes are p = z + beta x, with beta from the instance's
+    // state (the record that holds its stop flag), and the owning threads store p to pw (beside x,
+    // never over it: neighbouring tiles still read the previous direction in their halos)
+    T beta = T(0);
+    if constexpr (MODE == AM_APPLY) {
+        if (pz) beta = T(reinterpret_cast<const CgState*>(reinterpret_cast<const char*>(stop) - offsetof(CgState, stop))->beta);
     }
     // XCD-aware tile order: the hardware deals consecutive blocks round-robin to the 8 XCDs, so
     // block b takes tile (b % 8) * per + b / 8 and each XCD walks one contiguous run of tiles (x
@@ -130,6 +142,18 @@ __device__ __forceinline__ void fineBody(VBlk vb, PSX_FINE_PARAMS) {
         const auto rs = planeRsrc(x + 3 * NXY * (zin ? z : 0), zin ? planeBytes : 0u);
 #pragma unroll
         for (int q = 0; q < NPL; q++) bload3(rs, poff[q], px[q]);
+        if constexpr (MODE == AM_APPLY) {
+            if (pz) {
+                const auto rz = planeRsrc(pz + 3 * NXY * (zin ? z : 0), zin ? planeBytes : 0u);
+#pragma unroll
+                for (int q = 0; q < NPL; q++) {
+                    T zv[3];
+                    bload3(rz, poff[q], zv);
+#pragma unroll
+                    for (int c = 0; c < 3; c++) px[q][c] = zv[c] + beta * px[q][c];
+                }
+            }
+        }
     };
     auto storePlane = [&](int slot, const T (&px)[NPL][3]) {
 #pragma unroll
@@ -545,6 +569,9 @@ __device__ __forceinline__ void fineBody(VBlk vb, PSX_FINE_PARAMS) {
                     }
                 }
                 bstore3(planeRsrc(y + pb3, planeBytes), noff, yv);
+                if constexpr (MODE == AM_APPLY) {
+                    if (pw) bstore3(planeRsrc(pw + pb3, planeBytes), noff, xcPrev);
+                }
             }
         }
         // DENSE reads node plane k - 2, which the next step's store replaces
@@ -586,7 +613,7 @@ __device__ __forceinline__ void fineBody(VBlk vb, PSX_FINE_PARAMS) {
     }
 }
 
-#define PSX_FINE_FWD L, C, crow, ctab, x, y, r, zp, c1, c2, stop, zChunk, gx, gy, nTiles, xcdOrder, dots, dotStride, qdot
+#define PSX_FINE_FWD L, C, crow, ctab, x, y, r, zp, c1, c2, stop, zChunk, gx, gy, nTiles, xcdOrder, dots, dotStride, qdot, pz, pw
 // the kernel bodies for packed launches: WPE_ as the named kernel's waves-per-SIMD attribute
 template <class T, int MODE, bool OWNC, bool DOT, bool DENSE, int WPE_, bool H16 = false> struct Fine {
     static constexpr int NTH = NT, WPE = WPE_;
@@ -604,13 +631,16 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void kF
                                                                                   const float* __restrict__ zp, float c1, float c2, const int* stop,
                                                                                   int zChunk, int gx, int gy, int nTiles, int xcdOrder,
                                                                                   double* __restrict__ dots, int64_t dotStride,
-                                                                                  const float* __restrict__ qdot) {
+                                                                                  const float* __restrict__ qdot, const float* __restrict__ pz,
+                                                                                  float* __restrict__ pw) {
     fineBody<float, MODE, true, DOT, false, true>(ownBlock(), PSX_FINE_FWD);
 }
 // FP32 without own elements: held to 128 VGPRs (4 waves per SIMD instead of 3), which it fits
-// without spilling
+// without spilling; the outer operator (APPLY with the fused p.q and direction update) to 102, 5 waves
+// per SIMD as its LDS allows (103 VGPRs and 4 waves otherwise)
+template <class T, int MODE, bool DOT> constexpr int fine4Waves() { return MODE == AM_APPLY && DOT ? 5 : 4; }
 template <class T, int MODE, bool DOT>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void kFine4(PSX_FINE_PARAMS) {
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(fine4Waves<T, MODE, DOT>()))) void kFine4(PSX_FINE_PARAMS) {
     fineBody<T, MODE, false, DOT>(ownBlock(), PSX_FINE_FWD);
 }
 // per-voxel form (levels without own elements)
@@ -622,12 +652,13 @@ template <class T, int MODE, bool DOT> __global__ __launch_bounds__(NT) void kFi
 // launches (or records) the operator kernel of one form, mode and dot choice
 template <class T, int MODE, bool OWNC, bool DOT, bool DENSE, bool H16 = false>
 void fineGo(dim3 g, hipStream_t s, const DevLevel& L, const Sym<T>& C, const T* crow, const T* ctab, const T* x, T* y, const T* r, const T* zp, T c1, T c2,
-            const int* stop, int zChunk, int gx, int gy, int nTiles, int xcdOrder, double* dots, int64_t dotStride, const T* qdot) {
-#define PSX_FINE_ARGS L, C, crow, ctab, x, y, r, zp, c1, c2, stop, zChunk, gx, gy, nTiles, xcdOrder, dots, dotStride, qdot
+            const int* stop, int zChunk, int gx, int gy, int nTiles, int xcdOrder, double* dots, int64_t dotStride, const T* qdot, const T* pz,
+            T* pw) {
+#define PSX_FINE_ARGS L, C, crow, ctab, x, y, r, zp, c1, c2, stop, zChunk, gx, gy, nTiles, xcdOrder, dots, dotStride, qdot, pz, pw
     if constexpr (DENSE) {
         go<Fine<T, MODE, false, DOT, true, 0>>(kFineDense<T, MODE, DOT>, g, s, PSX_FINE_ARGS);
     } else if constexpr (sizeof(T) == 4 && !OWNC) {
-        go<Fine<T, MODE, false, DOT, false, 4>>(kFine4<T, MODE, DOT>, g, s, PSX_FINE_ARGS);
+        go<Fine<T, MODE, false, DOT, false, fine4Waves<T, MODE, DOT>()>>(kFine4<T, MODE, DOT>, g, s, PSX_FINE_ARGS);
     } else if constexpr (H16) {
         static_assert(sizeof(T) == 4 && OWNC, "FP16 subset table: FP32 child-scale levels");
         go<Fine<float, MODE, true, DOT, false, 3, true>>(kFineH<MODE, DOT>, g, s, PSX_FINE_ARGS);
@@ -642,7 +673,7 @@ void fineGo(dim3 g, hipStream_t s, const DevLevel& L, const Sym<T>& C, const T* 
 template <class T, bool OWNC>
 static void launchFineT(const DevLevel& L, int mode, const Sym<T>& C, const T* crow, const T* ctab, const T* x, T* y, const T* r, const T* d, double c1, double c2,
                         const int* stop, int zChunk, double* dots, int64_t dotStride, const T* qdot, hipStream_t s, int ninst,
-                        bool half = false) {
+                        bool half = false, const T* pz = nullptr, T* pw = nullptr) {
     const int gx = (L.NX + TX - 1) / TX, gy = (L.NY + TY - 1) / TY, gz = (L.NZ + zChunk - 1) / zChunk;
     const int nTiles = gx * gy * gz;
     const dim3 g(unsigned(8 * ((nTiles + 7) / 8)), unsigned(ninst));  // whole rounds over the 8 XCDs
@@ -663,18 +694,19 @@ static void launchFineT(const DevLevel& L, int mode, const Sym<T>& C, const T* c
         const T f1 = T(c1), f2 = T(c2);
         if constexpr (!OWNC) {
             if (dense) {
-                fineGo<T, M, false, DT, true>(g, s, L, C, crow, ctab, x, y, r, d, f1, f2, stop, zChunk, gx, gy, nTiles, xcdOrder, dots, dotStride, qdot);
+                fineGo<T, M, false, DT, true>(g, s, L, C, crow, ctab, x, y, r, d, f1, f2, stop, zChunk, gx, gy, nTiles, xcdOrder, dots, dotStride, qdot, pz,
+                                              pw);
                 return;
             }
         }
         if constexpr (sizeof(T) == 4 && OWNC) {
             if (useH16) {
                 fineGo<T, M, OWNC, DT, false, true>(g, s, L, C, crow, ctab, x, y, r, d, f1, f2, stop, zChunk, gx, gy, nTiles, xcdOrder, dots, dotStride,
-                                                           qdot);
+                                                    qdot, pz, pw);
                 return;
             }
         }
-        fineGo<T, M, OWNC, DT, false>(g, s, L, C, crow, ctab, x, y, r, d, f1, f2, stop, zChunk, gx, gy, nTiles, xcdOrder, dots, dotStride, qdot);
+        fineGo<T, M, OWNC, DT, false>(g, s, L, C, crow, ctab, x, y, r, d, f1, f2, stop, zChunk, gx, gy, nTiles, xcdOrder, dots, dotStride, qdot, pz, pw);
     };
     using yes = std::true_type;
     using no = std::false_type;
@@ -702,10 +734,17 @@ unsigned launchFineDot(const DevLevel& L, int mode, const T* x, T* y, const T* r
     const bool fused = (fd.kind == FD_PQ && mode == AM_APPLY) || (fd.kind == FD_RZ && (mode == AM_CHEB1 || mode == AM_CHEB2));
     double* dots = fused ? fd.red.partials : nullptr;
     const T* qdot = fused && fd.kind == FD_RZ ? static_cast<const T*>(fd.q) : nullptr;
+    // the direction update p = z + beta p_old fused into the outer operator (FD_PQ with z and p given)
+    const T* pz = fused && fd.kind == FD_PQ ? static_cast<const T*>(fd.pz) : nullptr;
+    T* pw = fused && fd.kind == FD_PQ ? static_cast<T*>(fd.pw) : nullptr;
+    if ((pz != nullptr) != (pw != nullptr)) throw std::invalid_argument("fused direction update: z and the new direction go together");
     if constexpr (sizeof(T) == 4) {
         // the element-by-element product on the matrix cores (fine_el.hip), on its own tiles
         // (the child-scale level with its own elements' corner-row pass)
-        if (el > 0 && (!L.childForm || L.ownVox)) return launchFineEl(L, mode, x, y, r, d, c1, c2, stop, dots, fd.red.stride, qdot, s, ninst, el);
+        if (el > 0 && (!L.childForm || L.ownVox)) {
+            if (pz) throw std::invalid_argument("the fused direction update needs the VALU operator");
+            return launchFineEl(L, mode, x, y, r, d, c1, c2, stop, dots, fd.red.stride, qdot, s, ninst, el);
+        }
     }
     Sym<T> C;
     for (int q = 0; q < 22; q++) C.v[q] = sizeof(T) == 8 ? T(L.sym64[q]) : T(L.sym32[q]);
@@ -731,9 +770,9 @@ unsigned launchFineDot(const DevLevel& L, int mode, const T* x, T* y, const T* r
         const T *crow = nullptr, *ctab = nullptr;
         if constexpr (sizeof(T) == 8) crow = L.crow64, ctab = L.ctab64;
         else crow = L.crow32, ctab = L.ctab32;
-        launchFineT<T, true>(L, mode, C, crow, ctab, x, y, r, d, c1, c2, stop, zChunk, dots, fd.red.stride, qdot, s, ninst, half);
+        launchFineT<T, true>(L, mode, C, crow, ctab, x, y, r, d, c1, c2, stop, zChunk, dots, fd.red.stride, qdot, s, ninst, half, pz, pw);
     } else {
-        launchFineT<T, false>(L, mode, C, nullptr, nullptr, x, y, r, d, c1, c2, stop, zChunk, dots, fd.red.stride, qdot, s, ninst);
+        launchFineT<T, false>(L, mode, C, nullptr, nullptr, x, y, r, d, c1, c2, stop, zChunk, dots, fd.red.stride, qdot, s, ninst, false, pz, pw);
     }
     return fused ? unsigned(gx * gy * gz) : 0u;
 }

@@ -88,6 +88,10 @@ public:
     // level-0 conjugate-gradient vectors
     double *x = nullptr, *r = nullptr, *p = nullptr, *q = nullptr, *f = nullptr, *u = nullptr;
     float *p32 = nullptr, *q32 = nullptr, *x32 = nullptr;
+    // the second buffer of the search direction: an iteration whose direction update is fused into
+    // the outer operator writes the new direction beside the old one
+    double* pAlt = nullptr;
+    float* p32Alt = nullptr;
     double* stressS = nullptr;  // corner stress matrices scaled by E / h (8 x 144)
     CgState* st = nullptr;      // one per right-hand side
     CgState* stHost = nullptr;  // pinned mirror
@@ -448,6 +452,8 @@ public:
         f = alloc<double>(nk, true);
         u = alloc<double>(nk, true);
         p32 = alloc<float>(nk, true);
+        pAlt = alloc<double>(nk, true);
+        p32Alt = alloc<float>(nk, true);
         x32 = alloc<float>(nk, true);
         q32 = alloc<float>(nk, true);
         st = alloc<CgState>(k, true);
@@ -509,6 +515,18 @@ public:
             launchApplyOwn<T>(D, mode, xx, yy, rr, dd, c1, c2, stop, stream, ni);
         }
         if (fd) dotAfter<T>(*fd, D.nDof, xx, yy, rr);
+    }
+    // whether y = K p on level 0 (with this form) is the VALU z-march, which can fuse the direction update
+    bool fusesDirection(int form) const {
+        const DevLevel& D = lv[0].dl;
+        return useFine && D.symOk && H.levels[0].nOwn == 0 && !(form & OF_EL3) && !directionUnfused();
+    }
+    static bool directionUnfused() {
+        static const bool off = [] {
+            const char* e = std::getenv("PSX_FUSE_P");
+            return e && e[0] == '0';
+        }();
+        return off;
     }
     template <class T> void dotAfter(const DotFusion& fd, int64_t n, const T* xx, const T* yy, const T* rr) {
         if (fd.kind == FD_PQ) launchDotPQ<T>(xx, yy, n, st, red, stream, ni);
@@ -842,14 +860,28 @@ public:
         int* stop = &st->stop;
         float* r32 = lv[0].r32;
         const int prec = R.prec, form = R.form, degree = R.degree;
+        // The direction update p = z + beta p joins the outer operator (one pass over z and p instead
+        // of two) where that operator is the VALU z-march: the new direction goes to the other buffer,
+        // so the buffers alternate. An odd batch updates in place in its last iteration, which leaves
+        // the direction in its home buffer at every batch boundary (the batch replays as one graph).
+        const bool fuseP = fusesDirection(prec == PSX_PREC_MIXED ? form & ~(OF_MFMA | OF_MFMA_OWN) : form);
         for (int b = 0; b < R.batch; b++) {
+            const bool fused = fuseP && (b + 1 < R.batch || R.batch % 2 == 0);
             if (prec == PSX_PREC_MIXED) {
-                const DotFusion fz{FD_RZ, q32, red}, fp{FD_PQ, nullptr, red};
+                float* pOld = b % 2 && fuseP ? p32Alt : p32;
+                float* pNew = fused ? (pOld == p32 ? p32Alt : p32) : pOld;
+                const DotFusion fz{FD_RZ, q32, red};
                 const float* z = vcycle<float>(0, r32, stop, form & ~OF_EL3, degree, &fz, R.fuseFirst);
-                launchUpdateP<float>(z, p32, n, st, stream, k);
+                DotFusion fp{FD_PQ, nullptr, red};
+                if (fused) {
+                    fp.pz = z;
+                    fp.pw = pNew;
+                } else {
+                    launchUpdateP<float>(z, pOld, n, st, stream, k);
+                }
                 // the outer operator: the FP32 VALU form, or the three-piece element product (OF_EL3)
-                op<float>(0, AM_APPLY, p32, q32, nullptr, nullptr, 0, 0, stop, form & ~(OF_MFMA | OF_MFMA_OWN), &fp);
-                launchUpdateXR32Fold(x32, p32, r32, q32, n, st, red, stream, k, R.fuseFirst ? lv[0].a32 : nullptr, D0.invD32, R.c0first);
+                op<float>(0, AM_APPLY, pOld, q32, nullptr, nullptr, 0, 0, stop, form & ~(OF_MFMA | OF_MFMA_OWN), &fp);
+                launchUpdateXR32Fold(x32, pNew, r32, q32, n, st, red, stream, k, R.fuseFirst ? lv[0].a32 : nullptr, D0.invD32, R.c0first);
                 // reliable update: the correction joins the FP64 solution and the FP64 true
                 // residual replaces the recurrence's
                 launchFoldX(R.du, x32, n, st, stream, k);
@@ -858,16 +890,24 @@ public:
                 continue;
             }
             const double* z;
-            const DotFusion fz{FD_RZ, q, red}, fp{FD_PQ, nullptr, red};
+            const DotFusion fz{FD_RZ, q, red};
             if (prec == PSX_PREC_FP64) z = vcycle<double>(0, r, stop, form, degree, &fz);
             else {
                 precondition32(r, lv[0].b64, stop, degree);
                 z = lv[0].b64;
                 launchDotsRZ<double>(r, z, q, n, st, red, stream, k);
             }
-            launchUpdateP<double>(z, p, n, st, stream, k);
-            op<double>(0, AM_APPLY, p, q, nullptr, nullptr, 0, 0, stop, form, &fp);
-            launchUpdateXR<double>(R.du, p, r, q, n, st, red, stream, k);
+            double* pOld = b % 2 && fuseP ? pAlt : p;
+            double* pNew = fused ? (pOld == p ? pAlt : p) : pOld;
+            DotFusion fp{FD_PQ, nullptr, red};
+            if (fused) {
+                fp.pz = z;
+                fp.pw = pNew;
+            } else {
+                launchUpdateP<double>(z, pOld, n, st, stream, k);
+            }
+            op<double>(0, AM_APPLY, pOld, q, nullptr, nullptr, 0, 0, stop, form, &fp);
+            launchUpdateXR<double>(R.du, pNew, r, q, n, st, red, stream, k);
         }
     }
 

@@ -119,10 +119,15 @@ template <class B, class... A>
 __global__ __launch_bounds__(B::NTH) __attribute__((amdgpu_waves_per_eu(4))) void kPacked4(PackHead h, const Args<A...>* args) {
     runPacked<B, A...>(h, args, std::index_sequence_for<A...>{});
 }
+template <class B, class... A>
+__global__ __launch_bounds__(B::NTH) __attribute__((amdgpu_waves_per_eu(5))) void kPacked5(PackHead h, const Args<A...>* args) {
+    runPacked<B, A...>(h, args, std::index_sequence_for<A...>{});
+}
 
 template <class B, class... A> void launchPacked(const PackHead& h, const void* args, uint32_t blocks, hipStream_t s) {
     const auto* a = static_cast<const Args<A...>*>(args);
-    if constexpr (B::WPE == 4) kPacked4<B, A...><<<blocks, B::NTH, 0, s>>>(h, a);
+    if constexpr (B::WPE == 5) kPacked5<B, A...><<<blocks, B::NTH, 0, s>>>(h, a);
+    else if constexpr (B::WPE == 4) kPacked4<B, A...><<<blocks, B::NTH, 0, s>>>(h, a);
     else if constexpr (B::WPE == 3) kPacked3<B, A...><<<blocks, B::NTH, 0, s>>>(h, a);
     else kPacked<B, A...><<<blocks, B::NTH, 0, s>>>(h, a);
 }
