@@ -451,9 +451,13 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void kF
 // Each wavefront takes OWN_TILES tiles of 16 elements, one element per MFMA column: per child c and
 // half (output corners 0..3, 4..7) the BF16 products of A = M_c's pieces (L.childA) and B = the
 // pieces of the element's corner differences, FP32 accumulation, then y += s_c (A B) per column.
-constexpr int OWN_TILES = 2;
+// Blocks of 8 wavefronts share one LDS copy of the A operands. Measured on the 512x128x128 block's
+// level 1 (133 834 own elements; the level's Chebyshev step with this pass): one tile per wavefront in
+// 512-thread blocks 46.5 us, two tiles 48.8 us, one tile in 256-thread blocks 49.5 us, in 1024-thread
+// blocks 47.4 us.
+constexpr int OWN_TILES = 1;
 struct OwnRowsMf {
-    static constexpr int NTH = 256, WPE = 0;
+    static constexpr int NTH = 512, WPE = 0;
     __device__ static void run(VBlk vb, DevLevel L, const float* __restrict__ x, const int* stop, int64_t nOwn) {
         if (stop) stop += int64_t(vb.y) * CG_INTS;
         if (stop && __builtin_nontemporal_load(stop) != 0) return;
@@ -543,7 +547,7 @@ struct OwnRowsMf {
         }
     }
 };
-__global__ __launch_bounds__(256) void kOwnRowsMf(DevLevel L, const float* __restrict__ x, const int* stop, int64_t nOwn) {
+__global__ __launch_bounds__(OwnRowsMf::NTH) void kOwnRowsMf(DevLevel L, const float* __restrict__ x, const int* stop, int64_t nOwn) {
     OwnRowsMf::run(ownBlock(), L, x, stop, nOwn);
 }
 
