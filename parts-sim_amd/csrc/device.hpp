@@ -192,7 +192,8 @@ void launchStressTensors(const DevLevel& L, const double* S, const double* u, fl
 void launchUpdateXR32Fold(float* x, const float* p, float* r, const float* q, int64_t n, CgState* st, Reduce red, hipStream_t s, int ninst = 1,
                           float* z0 = nullptr, const float* invD32 = nullptr, double c0 = 0);
 void launchFoldX(double* u, float* x, int64_t n, CgState* st, hipStream_t s, int ninst = 1);
-// fold, part 2 (after Kx = K u in FP64): r64 = f - Kx, r32 = r64 / rmax, true residual, stop test
+// fold, part 2 (after Kx = K u in FP64): r64 = f - Kx (r64 may be null: not stored), r32 = r64 / rmax,
+// true residual, stop test; held DOFs from invD (FP64) or, when it is null, invD32
 void launchFoldR(const double* f, const double* Kx, const double* invD, double* r64, float* r32, int64_t n, CgState* st, Reduce red, hipStream_t s, int ninst = 1,
                  float* z0 = nullptr, const float* invD32 = nullptr, double c0 = 0);
 

@@ -874,7 +874,7 @@ __device__ void FoldR::run(VBlk vb, const double* __restrict__ f, const double* 
     if (st->noFold) return;
     f += instOffset(vb, n);
     Kx += instOffset(vb, n);
-    r64 += instOffset(vb, n);
+    if (r64) r64 += instOffset(vb, n);
     r32 += instOffset(vb, n);
     if (z0) z0 += instOffset(vb, n);
     const double inv = 1.0 / st->rmax;
@@ -885,8 +885,10 @@ __device__ void FoldR::run(VBlk vb, const double* __restrict__ f, const double* 
         for (int k = 0; k < RED_TILE / BLOCK; k++) {
             const int64_t i = t0 + threadIdx.x + k * BLOCK;
             if (i < n) {
-                const double ri = invD[i] != 0 ? f[i] - Kx[i] : 0.0;
-                r64[i] = ri;
+                // held DOFs: a zero inverse diagonal (the FP32 copy where the FP64 one is not given)
+                const bool free = invD ? invD[i] != 0 : invD32[i] != 0.f;
+                const double ri = free ? f[i] - Kx[i] : 0.0;
+                if (r64) r64[i] = ri;
                 const float rf = float(ri * inv);
                 r32[i] = rf;
                 if (z0) z0[i] = c0 * invD32[i] * rf;
@@ -1250,6 +1252,7 @@ void launchFoldX(double* u, float* x, int64_t n, CgState* st, hipStream_t s, int
 
 void launchFoldR(const double* f, const double* Kx, const double* invD, double* r64, float* r32, int64_t n, CgState* st, Reduce red, hipStream_t s,
                  int ninst, float* z0, const float* invD32, double c0) {
+    if (!invD && !invD32) throw std::invalid_argument("launchFoldR: an inverse diagonal marks the held DOFs");
     const unsigned g = std::min(reduceGrid(n), FOLD_GRID);
     go<FoldR>(kFoldR, dim3(g, ninst), s, f, Kx, invD, r64, r32, n, st, red, z0, invD32, float(c0));
     finish<1, FinFoldR>(ninst, s, red, g, &st->noFold, FinFoldR{st});

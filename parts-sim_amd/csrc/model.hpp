@@ -886,7 +886,9 @@ public:
                 // residual replaces the recurrence's
                 launchFoldX(R.du, x32, n, st, stream, k);
                 op<double>(0, AM_APPLY, R.du, q, nullptr, nullptr, 0, 0, &st->noFold);
-                launchFoldR(R.df, q, D0.invD64, r, r32, n, st, red, stream, k, R.fuseFirst ? lv[0].a32 : nullptr, D0.invD32, R.c0first);
+                // (the FP64 residual itself is not kept: nothing reads it before the run's final one; the
+                // held DOFs come from the FP32 inverse diagonal)
+                launchFoldR(R.df, q, nullptr, nullptr, r32, n, st, red, stream, k, R.fuseFirst ? lv[0].a32 : nullptr, D0.invD32, R.c0first);
                 continue;
             }
             const double* z;
