@@ -56,3 +56,24 @@ def test_graph_follows_new_smoother_estimates(P, W, precision):
     assert after["iterations"] == eager["iterations"] and np.array_equal(after["u"], eager["u"])
     assert not np.array_equal(after["u"], before["u"])
     m.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batch_length_does_not_change_the_solve(P, W, precision):
+    """The direction update p = z + beta p runs inside the outer operator, the new direction written
+    beside the old (two buffers alternating), except in the last iteration of an odd-length batch,
+    which updates in place so that every batch starts from the same buffer. Both forms compute the
+    same expression, so the solve is bit for bit the same for any batch length, replayed or eager."""
+    w = W.void_block((40, 16, 12), seed=21)
+    m = P.Model(w["dims"], w["density"], w["nu"], w["bc"], 300)
+    f = w["f"]
+    kw = dict(tol=1e-9, precision=precision, cheb_degree=3)
+    if precision == "mixed":
+        kw["mfma_smoother"] = 1
+    base = m.solve(f, check_every=8, **kw)
+    for ce, eager in ((1, False), (3, False), (5, True), (7, False)):
+        o = m.solve(f, check_every=ce, eager=eager, **kw)
+        print(precision, ce, eager, o["iterations"], base["iterations"], flush=True)
+        assert o["iterations"] == base["iterations"] and o["folds"] == base["folds"]
+        assert np.array_equal(o["u"], base["u"])
+    m.close()
