@@ -587,8 +587,9 @@ __device__ void Restrict<T>::run(VBlk vb, DevLevel F, DevLevel C, const T* __res
     const T* __restrict__ cinv = Prec<T>::invD(C);
     const T i0 = cinv[3 * cn], i1 = cinv[3 * cn + 1], i2 = cinv[3 * cn + 2];
     T s0 = 0, s1 = 0, s2 = 0;
-    if (i0 != T(0) || i1 != T(0) || i2 != T(0)) {
-        // the 27 fine nodes around 2I, all loads issued together: a neighbour outside the fine grid
+    {
+        // the 27 fine nodes around 2I, all loads issued together and beside the coarse inverse
+        // diagonal (a held coarse DOF discards its sum below): a neighbour outside the fine grid
         // (2I itself is outside for the last coarse node of an odd fine count) reads the nearest
         // fine node, clamped per axis, with weight 0 — an exact zero added to a sum that starts at
         // +0 leaves its bits unchanged, so the result is the sum over the in-range nodes in order
@@ -694,7 +695,8 @@ template <class T> __device__ void Prolong<T>::run(VBlk vb, DevLevel F, DevLevel
 
 // The same, one thread per coarse node writing its 2 x 2 x 2 block of fine nodes (the 8 coarse
 // nodes around the block loaded once for all 8): large levels, where it saves the per-fine-node
-// gathers (level 0 of the 512x128x128 block: 73.4 -> 58.8 us); small ones keep one thread per fine
+// gathers (level 0 of the 512x128x128 block: 73.4 -> 58.8 us, 45.3 us with the fine nodes' loads
+// issued beside the coarse ones); small ones keep one thread per fine
 // node (their few coarse nodes leave too few threads: 5.5 -> 8.8 us on a 18 785-node level)
 template <class T> struct ProlongCells {
     static constexpr int NTH = BLOCK, WPE = 0;
@@ -735,13 +737,26 @@ template <class T> __device__ void ProlongCells<T>::run(VBlk vb, DevLevel F, Dev
         const bool in = (!mx || ox) && (!my || oy) && (!mz || oz);
         q[m] = in ? cb[mx + my * int64_t(C.NX) + mz * CXY] : V3{T(0), T(0), T(0)};
     }
+    // the fine nodes' info words and values, loaded beside the coarse ones (not after each node's
+    // held test)
+    V3 fv[8];
+    uint32_t finfo[8];
+#pragma unroll
+    for (int f = 0; f < 8; f++) {
+        const int a = f & 1, b = (f >> 1) & 1, c = f >> 2;
+        finfo[f] = 0x700u;
+        if (!(a ? ox : ex) || !(b ? oy : ey) || !(c ? oz : ez)) continue;
+        const int64_t fn = (2 * I + a) + FNX * (2 * J + b) + FXY * (2 * K + c);
+        finfo[f] = F.nodeInfo[fn];
+        fv[f] = *reinterpret_cast<const V3*>(zf + 3 * fn);
+    }
 #pragma unroll
     for (int f = 0; f < 8; f++) {
         const int a = f & 1, b = (f >> 1) & 1, c = f >> 2;
         if (!(a ? ox : ex) || !(b ? oy : ey) || !(c ? oz : ez)) continue;
         const int64_t fn = (2 * I + a) + FNX * (2 * J + b) + FXY * (2 * K + c);
         // held / inactive DOFs from the node info word (bits 8 + d: inverse diagonal 0)
-        const uint32_t info = F.nodeInfo[fn];
+        const uint32_t info = finfo[f];
         const bool f0 = !(info & 0x100u), f1 = !(info & 0x200u), f2 = !(info & 0x400u);
         if (!(f0 || f1 || f2)) continue;
         const T w = T(1.0 / ((a + 1) * (b + 1) * (c + 1)));
@@ -758,12 +773,11 @@ template <class T> __device__ void ProlongCells<T>::run(VBlk vb, DevLevel F, Dev
                     s1 += w * t.b;
                     s2 += w * t.c;
                 }
-        V3* __restrict__ dst = reinterpret_cast<V3*>(zf + 3 * fn);
-        V3 v = *dst;
+        V3 v = fv[f];
         v.a = f0 ? v.a + s0 : v.a;
         v.b = f1 ? v.b + s1 : v.b;
         v.c = f2 ? v.c + s2 : v.c;
-        *dst = v;
+        *reinterpret_cast<V3*>(zf + 3 * fn) = v;
     }
 }
 
