@@ -470,7 +470,8 @@ struct OwnRowsMf {
         __syncthreads();
         const int64_t tile0 = (int64_t(vb.x) * (NTH / 64) + (threadIdx.x >> 6)) * OWN_TILES;
         if (tile0 * 16 >= nOwn) return;  // the whole wavefront past the last element
-        const int64_t NX = L.NX, NXY = int64_t(L.NX) * L.NY, nx = L.nx, nxy = int64_t(L.nx) * L.ny;
+        const int64_t NX = L.NX, NXY = int64_t(L.NX) * L.NY;
+        const uint32_t nx = uint32_t(L.nx), ny = uint32_t(L.ny);
         // B operands: lane (n, g) holds element n's corners 2 g, 2 g + 1 (x = 0, 1; y = g & 1;
         // z = g >> 1), components 0..2 and a zero, as differences from corner 0
         bf16x8 B[OWN_TILES][2];
@@ -480,8 +481,9 @@ struct OwnRowsMf {
         for (int t = 0; t < OWN_TILES; t++) {
             own[t] = (tile0 + t) * 16 + n;
             const bool live = own[t] < nOwn;
-            const int64_t V = live ? L.ownVox[own[t]] : 0;
-            const int64_t m0 = (V % nx) + NX * ((V / nx) % L.ny) + NXY * (V / nxy);
+            // (voxel indices are 31-bit: the voxel's coordinates by 32-bit divisions)
+            const uint32_t V = live ? uint32_t(L.ownVox[own[t]]) : 0u, Vy = V / nx;
+            const int64_t m0 = (V - Vy * nx) + NX * (Vy % ny) + NXY * (Vy / ny);
             const int64_t mg = m0 + NX * (g & 1) + NXY * (g >> 1);
             float x0[3] = {0.f, 0.f, 0.f}, xa[3] = {0.f, 0.f, 0.f}, xb[3] = {0.f, 0.f, 0.f};
             if (live) {  // 12-byte node loads
